@@ -1,0 +1,177 @@
+"""Benchmark of the wave-action time step on one MI355X.
+
+A step is one SSP-RK3 time step (three fused tendency+stage kernel launches)
+of the sheared-eddy configuration: 512 x 512 cells x 32 kappa x 36 phi bins,
+FP32 state, WENO5 transport with Doppler shift and spectral refraction,
+synthetic eddy current.  The action is resident in HBM before timing starts.
+
+Prints one JSON line: cell-bin updates per second per RK3 step (the value),
+the roofline of the stage kernel measured with HIP events on the library's
+stream, the FP32 error against the FP64 path, and the CPU oracle timed on a
+bounded sample of the same workload on this host.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "ripple.jl_amd"))
+
+import numpy as np  # noqa: E402
+
+import ripple_amd as ra  # noqa: E402
+from ripple_amd import workloads, _lib  # noqa: E402
+
+METRIC = "cell-bin updates/s per RK3 step (plain vs surrogate), HBM GB/s; err vs FP64"
+HBM_PEAK_GBS = 8000.0
+
+
+def build_model(wl, float_type, kernel="auto"):
+    from ripple_amd import grids
+    g, sg = wl.grid, wl.spectral_grid
+    grid = grids.RectilinearGrid(float_type=float_type, size=(g.Nx, g.Ny, g.Nz), x=g.x_faces, y=g.y_faces,
+                                 z=g.z_faces)
+    sgrid = grids.PolarWaveVectorGrid(float_type, kappa=sg.kappa, phi=sg.phi, kappa_faces=sg.kappa_faces,
+                                      phi_faces=sg.phi_faces)
+    m = ra.SpectralWaveModel(grid, sgrid, velocities=wl.velocities, kernel=kernel, **wl.model_kwargs)
+    ra.set_(m, N=wl.N0)
+    return m
+
+
+def stage_bytes_per_launch(nx, ny, nk, nphi, esize):
+    """Algorithmic HBM bytes of one stage launch, averaged over the three
+    stages: N_k read and N_{k+1} written every stage, N0 read in stages 2-3,
+    and the two Doppler fields U_x, U_y (the gradients are formed in-kernel)."""
+    cells = nx * ny * nk * nphi
+    per_step = esize * (8 * cells + 3 * 2 * nx * ny * nk)
+    return per_step / 3.0
+
+
+def time_model(model, dt, steps, warmup):
+    lib, ctx = model._ctx.lib, model._ctx.ptr
+    ra.time_step_(model, dt, warmup)
+    model.synchronize()
+    _lib.check(lib.rpl_enable_kernel_timing(ctx, 1))
+    model.synchronize()
+    t0 = time.perf_counter()
+    _lib.check(lib.rpl_time_step(ctx, float(dt), int(steps)))
+    model.synchronize()
+    t1 = time.perf_counter()
+    import ctypes
+    ms, n = ctypes.c_double(), ctypes.c_int64()
+    _lib.check(lib.rpl_get_kernel_timing(ctx, ctypes.byref(ms), ctypes.byref(n)))
+    _lib.check(lib.rpl_enable_kernel_timing(ctx, 0))
+    return (t1 - t0) / steps, ms.value / max(n.value, 1), n.value
+
+
+def err_vs_fp64():
+    """FP32 against FP64 after the 400-step vortex-refraction run (config 1)."""
+    wl = workloads.vortex_refraction()
+    m64 = build_model(wl, np.float64)
+    m32 = build_model(wl, np.float32)
+    for m in (m64, m32):
+        ra.run_(ra.Simulation(m, dt=wl.dt, stop_iteration=wl.nsteps))
+    N64, N32 = m64.action, m32.action.astype(np.float64)
+
+    def rl2(a, b):
+        return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+
+    out = {"workload": wl.name + "_400_steps", "N_rel_l2": rl2(N32, N64)}
+    out["m0_rel_l2"] = rl2(ra.m0(m32), ra.m0(m64))
+    out["Hs_rel_l2"] = rl2(ra.significant_wave_height(m32), ra.significant_wave_height(m64))
+    out["kappa_rms_rel_l2"] = rl2(ra.root_mean_square_wavenumber(m32), ra.root_mean_square_wavenumber(m64))
+    d = np.angle(np.exp(1j * (ra.mean_direction(m32) - ra.mean_direction(m64))))
+    out["mean_dir_max_abs_rad"] = float(np.max(np.abs(d)))
+    a64, a32 = ra.total_action(m64), ra.total_action(m32)
+    out["total_action_rel_diff"] = abs(a32 - a64) / abs(a64)
+    return out
+
+
+def cpu_baseline(threads):
+    """The CPU oracle (FP32, reference operation order, OpenMP) on a bounded
+    sample: the same spectral grid, current and physics on a 96 x 96 sub-box."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    oracle.set_threads(threads)
+    wl = workloads.sheared_eddies(nx=96, ny=96)
+    om = oracle.OracleModel(wl, dtype=np.float32)
+    om.step(wl.dt, 1)  # warm-up
+    steps = 2
+    t0 = time.perf_counter()
+    om.step(wl.dt, steps)
+    t = (time.perf_counter() - t0) / steps
+    return {"value": wl.cell_bins / t, "unit": "cell-bin updates/s", "cores": threads, "kind": "port",
+            "sample": f"{wl.name} (96x96 sub-box of the 512x512 workload), FP32, {steps} RK3 steps after 1 "
+                      f"warm-up, OpenMP {threads} threads, {t * 1e3:.0f} ms/step"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--no-extra", action="store_true", help="skip the FP64 and larger-grid lines")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the wave step is built and measured on a single MI355X; "
+              "multi-GPU runs are not part of this build.", file=sys.stderr)
+        sys.exit(2)
+
+    wl = workloads.sheared_eddies()
+    nx, ny, nk, nphi = wl.N0.shape
+    model = build_model(wl, np.float32)
+    t_step, t_launch, launches = time_model(model, wl.dt, args.steps, args.warmup)
+    value = wl.cell_bins / t_step
+    bpl = stage_bytes_per_launch(nx, ny, nk, nphi, 4)
+    achieved = bpl / (t_launch * 1e-3) / 1e9
+    record = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cell-bin updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t_step * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": wl.name, "nx": nx, "ny": ny, "nkappa": nk, "nphi": nphi, "nz": wl.grid.Nz,
+                   "dt": wl.dt, "transport": "fused WENO5 Doppler transport + refraction",
+                   "sources": None, "timestepper": "SSP-RK3", "kernel": "fast"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "stage_fast_kernel", "bytes_per_launch": bpl,
+                     "avg_launch_ms": t_launch, "launches_timed": launches},
+    }
+    extra = {}
+    if not args.no_extra:
+        del model
+        m64 = build_model(wl, np.float64)
+        t64, l64, _ = time_model(m64, wl.dt, max(3, args.steps // 4), 1)
+        extra["fp64_reference_kernel"] = {
+            "value": wl.cell_bins / t64, "ms_per_step": t64 * 1e3, "avg_launch_ms": l64,
+            "achieved_GBs": stage_bytes_per_launch(nx, ny, nk, nphi, 8) / (l64 * 1e-3) / 1e9}
+        del m64
+        big = workloads.sheared_eddies(nx=1024, ny=1024)
+        mb = build_model(big, np.float32)
+        tb, lb, _ = time_model(mb, big.dt, max(3, args.steps // 2), 1)
+        extra["fp32_1024x1024"] = {
+            "workload": big.name, "value": big.cell_bins / tb, "ms_per_step": tb * 1e3, "avg_launch_ms": lb,
+            "achieved_GBs": stage_bytes_per_launch(1024, 1024, nk, nphi, 4) / (lb * 1e-3) / 1e9}
+        del mb
+        extra["err_vs_fp64"] = err_vs_fp64()
+    record["extra"] = extra
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        record["cpu_baseline"] = cpu_baseline(threads)
+    print(json.dumps(record), flush=True)
+
+
+if __name__ == "__main__":
+    main()

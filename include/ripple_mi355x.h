@@ -1,0 +1,150 @@
+/*
+ * ripple_mi355x.h -- C ABI of the MI355X (gfx950) wave-action time step.
+ *
+ * The reference (Ripple.jl) has no C ABI: its device boundary is a
+ * KernelAbstractions launch inside Julia methods.  Each entry point below
+ * replaces the reference interface cited beside it (paths relative to the
+ * Ripple.jl source tree).  A host layer (the Julia module in
+ * ripple.jl_amd/julia/, the Python module ripple.jl_amd/ripple_amd/) keeps
+ * the reference's SpectralWaveModel / set! / time_step! / Simulation surface
+ * and calls these functions.
+ *
+ * Conventions
+ *   - Every function returns RPL_OK (0) or a negative RPL_ERR_* code; the
+ *     message of the calling thread's last failure is rpl_last_error().
+ *   - Host pointers are borrowed for the duration of the call only.
+ *   - All device memory is owned by the context.  A context is used from one
+ *     host thread at a time; several contexts may run concurrently.
+ *   - Array layout at the boundary is the interior of the reference's
+ *     ProductField, i fastest: N[i + nx*(j + ny*(m + nkappa*n))] for
+ *     N[i,j,m,n] (ProductFields/product_field.jl:166-186 without the halo).
+ *     (nx, ny, nkappa) arrays are U[i + nx*(j + ny*m)]; (nx, ny, nz) current
+ *     arrays are u[i + nx*(j + ny*k)].
+ *   - Work is enqueued on the context's HIP stream; functions that return
+ *     data to the host synchronize that stream first.
+ */
+#ifndef RIPPLE_MI355X_H
+#define RIPPLE_MI355X_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RPL_OK 0
+#define RPL_ERR_ARG (-1)         /* invalid argument (reference: ArgumentError) */
+#define RPL_ERR_STATE (-2)       /* call out of order (e.g. step before geometry) */
+#define RPL_ERR_HIP (-3)         /* HIP runtime failure */
+#define RPL_ERR_UNSUPPORTED (-4) /* configuration outside this build's scope */
+#define RPL_ERR_NOMEM (-5)       /* device allocation failed */
+
+enum { RPL_F32 = 0, RPL_F64 = 1 };
+
+/* Models/time_step.jl:117-166: :RK3, :LSRK3 (same arithmetic), :ForwardEuler. */
+enum { RPL_RK3 = 0, RPL_LSRK3 = 1, RPL_FORWARD_EULER = 2 };
+
+/* Models/tendencies.jl:18-109 dispatch, resolved by the host layer:
+ *   NONE       horizontal_advection = nothing, no coupling (G = 0 + sources)
+ *   INTRINSIC  fused intrinsic transport (intrinsic_transport.jl:14-68)
+ *   REFRACTION fused Doppler transport + refraction (wave_current_refraction.jl:74-151) */
+enum { RPL_TRANSPORT_NONE = 0, RPL_TRANSPORT_INTRINSIC = 1, RPL_TRANSPORT_REFRACTION = 2 };
+
+/* Intrinsic group velocity (Models/transport.jl:53-95). */
+enum { RPL_DEPTH_INFINITE = 0, RPL_DEPTH_CONSTANT = 1 };
+
+/* Stencil implementation.  REFERENCE evaluates the reference formulas in
+ * their order (the FP64 parity path); FAST is the restructured FP32 kernel. */
+enum { RPL_KERNEL_AUTO = 0, RPL_KERNEL_REFERENCE = 1, RPL_KERNEL_FAST = 2 };
+
+typedef struct rpl_config {
+    int32_t nx, ny;        /* physical interior cells (periodic x and y) */
+    int32_t nkappa, nphi;  /* spectral bins */
+    int32_t dtype;         /* RPL_F32 | RPL_F64: the action's element type */
+    int32_t timestepper;   /* RPL_RK3 | RPL_LSRK3 | RPL_FORWARD_EULER */
+    int32_t transport;     /* RPL_TRANSPORT_* */
+    int32_t kernel;        /* RPL_KERNEL_* */
+} rpl_config;
+
+typedef struct rpl_ctx rpl_ctx;
+
+/* SpectralWaveModel construction (Models/spectral_wave_model.jl:155-211):
+ * allocates the action, its RK3 stage buffers and coupling caches on the
+ * current HIP device.  Requires nx, ny, nphi >= 3 (the reference's periodic
+ * wraps, wave_current_refraction.jl:66, reach 3 cells) and nkappa >= 1. */
+int rpl_create(rpl_ctx **out, const rpl_config *cfg);
+int rpl_destroy(rpl_ctx *ctx);
+
+/* Spectral and physical geometry: PolarWaveVectorGrid / FrequencyDirectionGrid
+ * centres and faces (CoordinateGrids/polar_wavevector_grid.jl:18-56,
+ * frequency_direction_grid.jl:20-69; for frequency grids pass the derived
+ * kappa), first-cell spacings dx, dy (wave_current_refraction.jl:200-203) and
+ * depth.  Builds the cell measures and the intrinsic group-velocity tables
+ * (Models/transport.jl:111-139) and the cos/sin tables
+ * (wave_current_refraction.jl:154-175). */
+int rpl_set_geometry(rpl_ctx *ctx, double dx, double dy,
+                     const double *kappa_c, const double *kappa_f,
+                     const double *phi_c, const double *phi_f,
+                     int32_t depth_kind, double depth, double gravity);
+
+/* Prescribed Lagrangian-mean current (Coupling/current_coupling.jl:73-91,
+ * 127-138): builds the Q weights W[k,m] = CDF(z_{k+1}) - CDF(z_k)
+ * (q_kernel.jl:77-92) and runs the Doppler contraction
+ * U[i,j,m] = sum_k u[i,j,k] W[k,m] (doppler_velocity.jl:5-23,45-62) and the
+ * current gradients (wave_current_refraction.jl:7-39) on the device.
+ * u, v: (nx, ny, nz) in FP64; z_faces: nz+1 values; q_depth > 0. */
+int rpl_set_current(rpl_ctx *ctx, int32_t nz, const double *z_faces,
+                    const double *u, const double *v, double q_depth);
+
+/* Direct access to the Doppler velocity caches Ux, Uy (nx, ny, nkappa), FP64
+ * at the boundary (the reference's coupling.Ux / coupling.Uy). */
+int rpl_set_doppler(rpl_ctx *ctx, const double *Ux, const double *Uy);
+int rpl_get_doppler(rpl_ctx *ctx, double *Ux, double *Uy);
+
+/* set!(model; N=array) (ProductFields/set.jl:20-30, 44-48) and interior(N)
+ * (product_field.jl:221-230).  Element type is the context dtype. */
+int rpl_set_action(rpl_ctx *ctx, const void *N);
+int rpl_get_action(rpl_ctx *ctx, void *N);
+
+/* compute_tendencies!(model) (Models/tendencies.jl:109): one evaluation of
+ * G = L(N) on the current action, copied to the host (the parity hook). */
+int rpl_tendency(rpl_ctx *ctx, void *G);
+
+/* time_step!(model, dt) repeated nsteps times (Models/time_step.jl:119-170):
+ * dt > 0, else RPL_ERR_ARG (time_step.jl:120).  Enqueued, not waited for. */
+int rpl_time_step(rpl_ctx *ctx, double dt, int32_t nsteps);
+int rpl_synchronize(rpl_ctx *ctx);
+
+/* model.clock (Oceananigans Clock): time and iteration, advanced once per
+ * step (time_step.jl:168). */
+int rpl_get_clock(rpl_ctx *ctx, double *time, int64_t *iteration);
+int rpl_set_clock(rpl_ctx *ctx, double time, int64_t iteration);
+
+/* m0, root_mean_square_wavenumber and mean_direction
+ * (Diagnostics/moments.jl:33-123, diagnostic_operations.jl:56-165): three
+ * (nx, ny) FP64 arrays; any pointer may be NULL. */
+int rpl_diagnostics(rpl_ctx *ctx, double *m0, double *kappa_rms, double *mean_dir);
+
+/* total_action (Diagnostics/bulk_statistics.jl:87-91) and
+ * total_deep_water_energy (bulk_statistics.jl:93-99), both accumulated on
+ * the device in FP64 with compensated (double-double) summation. */
+int rpl_conservation(rpl_ctx *ctx, double *total_action, double *total_energy);
+
+/* Per-kernel timing of the fused stage kernel: when enabled, HIP events on
+ * the context stream bracket every stage launch; the accumulated elapsed
+ * milliseconds and launch count are read (and reset) with
+ * rpl_get_kernel_timing, which synchronizes. */
+int rpl_enable_kernel_timing(rpl_ctx *ctx, int32_t enable);
+int rpl_get_kernel_timing(rpl_ctx *ctx, double *total_ms, int64_t *launches);
+
+/* The context's hipStream_t (for callers that order their own work after it). */
+void *rpl_stream(rpl_ctx *ctx);
+
+/* Message of this thread's last failed call ("" if none). */
+const char *rpl_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RIPPLE_MI355X_H */

@@ -1,0 +1,220 @@
+"""ctypes front end of the CPU oracle (liboracle.so).  TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+import this module; the product package never does.  `OracleModel` drives the
+oracle from the same Workload inputs the device model takes.
+"""
+import ctypes
+import math
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+_D = ctypes.c_double
+_F = ctypes.c_float
+_I = ctypes.c_int
+_P = ctypes.c_void_p
+_lib = None
+
+MODE_NONE, MODE_INTRINSIC, MODE_REFRACTION = 0, 1, 2
+STEPPERS = {"RK3": 0, "LSRK3": 1, "LowStorageRK3": 1, "ForwardEuler": 2}
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(LIB_PATH)
+        _lib.orc_q_cdf.restype = _D
+        _lib.orc_q_cdf.argtypes = [_D, _D, _D, _D]
+        _lib.orc_total_weighted_f64.restype = _D
+        _lib.orc_total_weighted_f32.restype = _D
+        _lib.orc_weno5_f64.restype = _D
+        _lib.orc_weno5_f64.argtypes = [_D] * 5
+        _lib.orc_weno5_f32.restype = _F
+        _lib.orc_weno5_f32.argtypes = [_F] * 5
+    return _lib
+
+
+def _p(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def _c(a, dtype=np.float64):
+    """Contiguous i-fastest copy of an array given in Julia index order."""
+    return np.ascontiguousarray(np.asarray(a, dtype=dtype).ravel(order="F"))
+
+
+def set_threads(n):
+    return lib().orc_set_threads(_I(n))
+
+
+def weno5(f, dtype=np.float64):
+    L = lib()
+    if dtype == np.float64:
+        return L.orc_weno5_f64(*[float(x) for x in f])
+    return L.orc_weno5_f32(*[float(x) for x in f])
+
+
+def centers_to_faces(c, lower_bounded=False, lower=0.0):
+    c = _c(c)
+    f = np.zeros(len(c) + 1)
+    lib().orc_centers_to_faces(_I(len(c)), _p(c), _p(f), _I(int(lower_bounded)), _D(lower))
+    return f
+
+
+def phi_faces(phi):
+    phi = _c(phi)
+    f = np.zeros(len(phi) + 1)
+    lib().orc_phi_faces(_I(len(phi)), _p(phi), _p(f))
+    return f
+
+
+def geometry(kc, kf, pc, pf, finite_depth=False, depth=0.0, gravity=9.81):
+    """Tables (Nk, Nphi): w, mu_x, mu_y, trace, cgx, cgy."""
+    kc, kf, pc, pf = _c(kc), _c(kf), _c(pc), _c(pf)
+    nk, npf = len(kc), len(pc)
+    out = {k: np.zeros(nk * npf) for k in ("w", "mu_x", "mu_y", "trace", "cgx", "cgy")}
+    L = lib()
+    L.orc_cell_measures(_I(nk), _I(npf), _p(kf), _p(pf), _p(out["w"]))
+    L.orc_moment_measures(_I(nk), _I(npf), _p(kf), _p(pf), _p(out["mu_x"]), _p(out["mu_y"]), _p(out["trace"]))
+    L.orc_group_velocity_tables(_I(nk), _I(npf), _p(kc), _p(kf), _p(pc), _p(pf), _p(out["w"]),
+                                _I(int(finite_depth)), _D(depth), _D(gravity), _p(out["cgx"]), _p(out["cgy"]))
+    return {k: v.reshape((nk, npf), order="F") for k, v in out.items()}
+
+
+def q_cdf(kappa, z, depth, small_mu=1e-6):
+    return lib().orc_q_cdf(kappa, z, depth, small_mu)
+
+
+def q_weights(z_faces, kappa, depth):
+    zf, kc = _c(z_faces), _c(kappa)
+    W = np.zeros((len(zf) - 1) * len(kc))
+    lib().orc_q_weights(_I(len(zf) - 1), _p(zf), _I(len(kc)), _p(kc), _D(depth), _p(W))
+    return W.reshape((len(zf) - 1, len(kc)), order="F")
+
+
+def doppler(u, v, W):
+    nx, ny, nz = u.shape
+    nk = W.shape[1]
+    uf, vf, Wf = _c(u), _c(v), _c(W)
+    Ux = np.zeros(nx * ny * nk)
+    Uy = np.zeros(nx * ny * nk)
+    lib().orc_doppler(_I(nx), _I(ny), _I(nz), _p(uf), _p(vf), _I(nk), _p(Wf), _p(Ux), _p(Uy))
+    return Ux.reshape((nx, ny, nk), order="F"), Uy.reshape((nx, ny, nk), order="F")
+
+
+class OracleModel:
+    """The oracle's version of SpectralWaveModel over a Workload's inputs."""
+
+    def __init__(self, workload, dtype=np.float64, depth=None, gravity=9.81, doppler_override=None,
+                 transport=None, timestepper=None):
+        wl = workload
+        self.dtype = np.dtype(dtype)
+        g, sg = wl.grid, wl.spectral_grid
+        self.size = (g.Nx, g.Ny, len(sg.kappa), len(sg.phi))
+        kw = wl.model_kwargs
+        self.stepper = STEPPERS[timestepper or kw.get("timestepper", "ForwardEuler")]
+        if transport is not None:
+            self.mode = transport
+        elif wl.velocities is not None:
+            self.mode = MODE_REFRACTION
+        elif kw.get("horizontal_advection", "WENO") is None:
+            self.mode = MODE_NONE
+        else:
+            self.mode = MODE_INTRINSIC
+        finite = depth is not None
+        tabs = geometry(sg.kappa, sg.kappa_faces, sg.phi, sg.phi_faces, finite, depth or 0.0, gravity)
+        self.tables = tabs
+        self.dx = float(g.x_faces[1] - g.x_faces[0])
+        self.dy = float(g.y_faces[1] - g.y_faces[0])
+        self.dk = float(sg.kappa_faces[1] - sg.kappa_faces[0])
+        self.dp = float(sg.phi_faces[1] - sg.phi_faces[0])
+        t = self.dtype
+        self.kappa = _c(sg.kappa, t)
+        self.cgx, self.cgy = _c(tabs["cgx"], t), _c(tabs["cgy"], t)
+        # libm cos/sin, as the device library forms its tables on the host.
+        self.cos = _c([math.cos(p) for p in sg.phi], t)
+        self.sin = _c([math.sin(p) for p in sg.phi], t)
+        nx, ny, nk, _ = self.size
+        if doppler_override is not None:
+            Ux, Uy = doppler_override
+        elif wl.velocities is not None:
+            q_depth = g.Lz if depth is None else depth
+            W = q_weights(g.z_faces, sg.kappa, q_depth)
+            if t == np.float32:
+                W = W.astype(np.float32).astype(np.float64)
+                u = wl.velocities[0].astype(np.float32).astype(np.float64)
+                v = wl.velocities[1].astype(np.float32).astype(np.float64)
+            else:
+                u, v = wl.velocities
+            Ux, Uy = doppler(u, v, W)
+        else:
+            Ux = Uy = np.zeros((nx, ny, nk))
+        self.Ux, self.Uy = _c(Ux, t), _c(Uy, t)
+        self.N = _c(wl.N0, t).copy()  # never alias the workload's array
+        self.time, self.iteration = 0.0, 0
+
+    def _inv(self):
+        t = np.float64 if self.dtype == np.float64 else np.float32
+        one = t(1)
+        return [one / t(self.dx), one / t(self.dy), one / t(self.dk), one / t(self.dp)]
+
+    def _scalar(self, x):
+        return _D(x) if self.dtype == np.float64 else _F(x)
+
+    def step(self, dt, nsteps=1, hook=None, user=None):
+        L = lib()
+        nx, ny, nk, npf = self.size
+        sfx = "f64" if self.dtype == np.float64 else "f32"
+        fn = getattr(L, f"orc_run_{sfx}")
+        inv = [self._scalar(v) for v in self._inv()]
+        rc = fn(_I(nx), _I(ny), _I(nk), _I(npf), _I(self.mode), _I(self.stepper), _p(self.N),
+                _p(self.Ux), _p(self.Uy), _p(self.kappa), _p(self.cgx), _p(self.cgy), _p(self.cos), _p(self.sin),
+                *inv, _D(dt), _I(nsteps), hook, user)
+        if rc != 0:
+            raise MemoryError("oracle run failed to allocate")
+        self.time += dt * nsteps
+        self.iteration += nsteps
+
+    def tendency(self):
+        L = lib()
+        nx, ny, nk, npf = self.size
+        sfx = "f64" if self.dtype == np.float64 else "f32"
+        G = np.zeros_like(self.N)
+        inv = [self._scalar(v) for v in self._inv()]
+        getattr(L, f"orc_tendency_{sfx}")(_I(nx), _I(ny), _I(nk), _I(npf), _I(self.mode), _p(self.N), _p(G),
+                                          _p(self.Ux), _p(self.Uy), _p(self.kappa), _p(self.cgx), _p(self.cgy),
+                                          _p(self.cos), _p(self.sin), *inv)
+        return G.reshape(self.size, order="F")
+
+    @property
+    def action(self):
+        return self.N.reshape(self.size, order="F")
+
+    def diagnostics(self):
+        nx, ny, nk, npf = self.size
+        t = self.tables
+        w, mx, my, tr = (_c(t[k]) for k in ("w", "mu_x", "mu_y", "trace"))
+        m0, kr, md = np.zeros(nx * ny), np.zeros(nx * ny), np.zeros(nx * ny)
+        sfx = "f64" if self.dtype == np.float64 else "f32"
+        getattr(lib(), f"orc_column_sums_{sfx}")(_I(nx), _I(ny), _I(nk), _I(npf), _p(self.N), _p(w), _p(mx), _p(my),
+                                                 _p(tr), _p(m0), _p(kr), _p(md))
+        shp = (nx, ny)
+        return m0.reshape(shp, order="F"), kr.reshape(shp, order="F"), md.reshape(shp, order="F")
+
+    def total(self, measure):
+        nx, ny, nk, npf = self.size
+        sfx = "f64" if self.dtype == np.float64 else "f32"
+        w = _c(measure)
+        return getattr(lib(), f"orc_total_weighted_{sfx}")(_I(nx), _I(ny), _I(nk), _I(npf), _p(self.N), _p(w),
+                                                           _D(self.dx), _D(self.dy))

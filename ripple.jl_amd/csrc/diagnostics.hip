@@ -1,0 +1,142 @@
+// Spectral diagnostics: per-column weighted sums (m0, kappa_rms, mean
+// direction) and the compensated global totals used for conservation checks.
+#include "rpl_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace rpl {
+namespace {
+
+// Diagnostics/diagnostic_operations.jl:71-79 (sum), :110-122 (ratio with
+// fallback 0 when the denominator is zero), :153-165 (atan(sy, sx)).  One
+// thread per column, spectral loop n outer / m inner as the reference, FP64
+// accumulation.  Consecutive threads take consecutive i, so each (m, n) plane
+// is read with unit stride.
+template <typename T>
+__global__ __launch_bounds__(256) void column_diagnostics_kernel(int nx, int ny, int nk, int nphi,
+                                                                 const T *__restrict__ N,
+                                                                 const double *__restrict__ w,
+                                                                 const double *__restrict__ trace,
+                                                                 const double *__restrict__ mux,
+                                                                 const double *__restrict__ muy, double *m0,
+                                                                 double *krms, double *mdir)
+{
+    const size_t plane = (size_t)nx * ny;
+    const size_t ij = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (ij >= plane)
+        return;
+    double acc = 0, num = 0, sx = 0, sy = 0;
+    for (int n = 0; n < nphi; ++n)
+        for (int m = 0; m < nk; ++m) {
+            const int t = m + nk * n;
+            const double v = (double)N[ij + plane * t];
+            acc += v * w[t];
+            num += v * trace[t];
+            sx += v * mux[t];
+            sy += v * muy[t];
+        }
+    if (m0)
+        m0[ij] = acc;
+    if (krms)
+        krms[ij] = sqrt(acc == 0.0 ? 0.0 : num / acc);
+    if (mdir)
+        mdir[ij] = atan2(sy, sx);
+}
+
+__device__ __forceinline__ void two_sum(double a, double b, double &s, double &e)
+{
+    double x = a + b;
+    double bv = x - a;
+    double av = x - bv;
+    s = x;
+    e = (a - av) + (b - bv);
+}
+
+// Adds (bh, bl) into (ah, al) keeping the rounding error of the high parts.
+__device__ __forceinline__ void dd_add(double &ah, double &al, double bh, double bl)
+{
+    double s, e;
+    two_sum(ah, bh, s, e);
+    ah = s;
+    al = al + bl + e;
+}
+
+// bulk_statistics.jl:87-99: sum over columns of (sum_mn N w) * dx * dy, with
+// a double-double accumulator per thread and a double-double tree in LDS.
+template <typename T>
+__global__ __launch_bounds__(256) void weighted_total_kernel(int nx, int ny, int nk, int nphi,
+                                                             const T *__restrict__ N,
+                                                             const double *__restrict__ w, double area,
+                                                             double *partials)
+{
+    __shared__ double sh[256], sl[256];
+    const size_t plane = (size_t)nx * ny;
+    double hi = 0, lo = 0;
+    for (size_t ij = (size_t)blockIdx.x * 256 + threadIdx.x; ij < plane; ij += (size_t)gridDim.x * 256) {
+        double acc = 0;
+        for (int n = 0; n < nphi; ++n)
+            for (int m = 0; m < nk; ++m) {
+                const int t = m + nk * n;
+                acc += (double)N[ij + plane * t] * w[t];
+            }
+        dd_add(hi, lo, acc * area, 0.0);
+    }
+    sh[threadIdx.x] = hi;
+    sl[threadIdx.x] = lo;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            double h = sh[threadIdx.x], l = sl[threadIdx.x];
+            dd_add(h, l, sh[threadIdx.x + s], sl[threadIdx.x + s]);
+            sh[threadIdx.x] = h;
+            sl[threadIdx.x] = l;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partials[2 * blockIdx.x] = sh[0];
+        partials[2 * blockIdx.x + 1] = sl[0];
+    }
+}
+
+}  // namespace
+
+int weighted_total_blocks(int nx, int ny)
+{
+    size_t plane = (size_t)nx * ny;
+    size_t b = (plane + 255) / 256;
+    return (int)(b < 1024 ? b : 1024);
+}
+
+template <typename T>
+hipError_t launch_column_diagnostics(int nx, int ny, int nk, int nphi, const T *N, const double *w,
+                                     const double *trace, const double *mux, const double *muy, double *m0,
+                                     double *krms, double *mdir, hipStream_t s)
+{
+    size_t plane = (size_t)nx * ny;
+    hipLaunchKernelGGL(column_diagnostics_kernel<T>, dim3((unsigned)((plane + 255) / 256)), dim3(256), 0, s, nx, ny,
+                       nk, nphi, N, w, trace, mux, muy, m0, krms, mdir);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_weighted_total(int nx, int ny, int nk, int nphi, const T *N, const double *w, double area,
+                                 double *partials, int nblocks, hipStream_t s)
+{
+    hipLaunchKernelGGL(weighted_total_kernel<T>, dim3(nblocks), dim3(256), 0, s, nx, ny, nk, nphi, N, w, area,
+                       partials);
+    return hipGetLastError();
+}
+
+template hipError_t launch_column_diagnostics<float>(int, int, int, int, const float *, const double *,
+                                                     const double *, const double *, const double *, double *,
+                                                     double *, double *, hipStream_t);
+template hipError_t launch_column_diagnostics<double>(int, int, int, int, const double *, const double *,
+                                                      const double *, const double *, const double *, double *,
+                                                      double *, double *, hipStream_t);
+template hipError_t launch_weighted_total<float>(int, int, int, int, const float *, const double *, double, double *,
+                                                 int, hipStream_t);
+template hipError_t launch_weighted_total<double>(int, int, int, int, const double *, const double *, double,
+                                                  double *, int, hipStream_t);
+
+}  // namespace rpl

@@ -1,0 +1,724 @@
+// Host side of the C ABI declared in include/ripple_mi355x.h: the context,
+// geometry tables, stage-buffer rotation and dispatch onto the HIP kernels.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "rpl_internal.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg)
+{
+    g_last_error = msg;
+    return code;
+}
+
+#define RPL_HIP(expr)                                                                                     \
+    do {                                                                                                  \
+        hipError_t e_ = (expr);                                                                           \
+        if (e_ != hipSuccess)                                                                             \
+            return fail(RPL_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));                  \
+    } while (0)
+
+}  // namespace
+
+struct rpl_ctx {
+    rpl_config cfg{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    size_t cnt = 0;    // interior cell-bins
+    size_t plane = 0;  // nx * ny
+    size_t esize = 0;  // sizeof element
+
+    void *A = nullptr, *B = nullptr, *C = nullptr;  // action (A) and stage buffers
+    void *Ux = nullptr, *Uy = nullptr, *Uxx = nullptr, *Uxy = nullptr, *Uyx = nullptr, *Uyy = nullptr;
+    void *kappa = nullptr, *cgx = nullptr, *cgy = nullptr, *cosp = nullptr, *sinp = nullptr;
+    double *w = nullptr, *trace = nullptr, *mux = nullptr, *muy = nullptr, *energy = nullptr;
+    double *diag = nullptr;      // 3 * plane
+    double *partials = nullptr;  // 2 * blocks
+    void *scratch_in = nullptr;  // FP64 staging for host uploads
+
+    double dx = 0, dy = 0, dk = 0, dp = 0;
+    bool has_geometry = false, has_current = false;
+    double time = 0;
+    int64_t iteration = 0;
+
+    bool timing = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+    size_t events_used = 0;
+    double timing_ms = 0;
+    int64_t timing_launches = 0;
+};
+
+namespace {
+
+void free_ptr(void *p)
+{
+    if (p)
+        (void)hipFree(p);
+}
+
+int alloc(void **p, size_t bytes)
+{
+    hipError_t e = hipMalloc(p, bytes == 0 ? 1 : bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(RPL_ERR_NOMEM, std::string("hipMalloc(") + std::to_string(bytes) + "): " + hipGetErrorString(e));
+    }
+    return RPL_OK;
+}
+
+bool is_f64(const rpl_ctx *c) { return c->cfg.dtype == RPL_F64; }
+
+// Uploads n doubles to a device array of the context's element type.
+int upload_real(rpl_ctx *c, const double *host, void *dst, size_t n)
+{
+    if (is_f64(c)) {
+        RPL_HIP(hipMemcpyAsync(dst, host, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    } else {
+        std::vector<float> tmp(n);
+        for (size_t k = 0; k < n; ++k)
+            tmp[k] = (float)host[k];
+        RPL_HIP(hipMemcpyAsync(dst, tmp.data(), n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+        RPL_HIP(hipStreamSynchronize(c->stream));
+    }
+    return RPL_OK;
+}
+
+int upload_f64(rpl_ctx *c, const std::vector<double> &host, double *dst)
+{
+    RPL_HIP(hipMemcpyAsync(dst, host.data(), host.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    RPL_HIP(hipStreamSynchronize(c->stream));
+    return RPL_OK;
+}
+
+template <typename T>
+rpl::StageParams<T> stage_params(const rpl_ctx *c, double dt)
+{
+    rpl::StageParams<T> p{};
+    p.nx = c->cfg.nx;
+    p.ny = c->cfg.ny;
+    p.nk = c->cfg.nkappa;
+    p.nphi = c->cfg.nphi;
+    p.Ux = (const T *)c->Ux;
+    p.Uy = (const T *)c->Uy;
+    p.Uxx = (const T *)c->Uxx;
+    p.Uxy = (const T *)c->Uxy;
+    p.Uyx = (const T *)c->Uyx;
+    p.Uyy = (const T *)c->Uyy;
+    p.kappa = (const T *)c->kappa;
+    p.cgx = (const T *)c->cgx;
+    p.cgy = (const T *)c->cgy;
+    p.cosp = (const T *)c->cosp;
+    p.sinp = (const T *)c->sinp;
+    // 1 / first spacing, formed in the element type (wave_current_refraction.jl:213).
+    p.dx_inv = T(1) / (T)c->dx;
+    p.dy_inv = T(1) / (T)c->dy;
+    p.dk_inv = T(1) / (T)c->dk;
+    p.dp_inv = T(1) / (T)c->dp;
+    p.dt = (T)dt;
+    p.src_add = nullptr;
+    return p;
+}
+
+bool use_fast(const rpl_ctx *c)
+{
+    if (is_f64(c))
+        return false;
+    if (c->cfg.kernel == RPL_KERNEL_REFERENCE)
+        return false;
+    return true;
+}
+
+int record_begin(rpl_ctx *c)
+{
+    if (!c->timing)
+        return RPL_OK;
+    if (c->events_used == c->events.size()) {
+        hipEvent_t a, b;
+        RPL_HIP(hipEventCreate(&a));
+        RPL_HIP(hipEventCreate(&b));
+        c->events.emplace_back(a, b);
+    }
+    RPL_HIP(hipEventRecord(c->events[c->events_used].first, c->stream));
+    return RPL_OK;
+}
+
+int fold_timing(rpl_ctx *c)
+{
+    if (c->events_used == 0)
+        return RPL_OK;
+    RPL_HIP(hipStreamSynchronize(c->stream));
+    for (size_t k = 0; k < c->events_used; ++k) {
+        float ms = 0;
+        RPL_HIP(hipEventElapsedTime(&ms, c->events[k].first, c->events[k].second));
+        c->timing_ms += ms;
+    }
+    c->timing_launches += (int64_t)c->events_used;
+    c->events_used = 0;
+    return RPL_OK;
+}
+
+int record_end(rpl_ctx *c)
+{
+    if (!c->timing)
+        return RPL_OK;
+    RPL_HIP(hipEventRecord(c->events[c->events_used].second, c->stream));
+    c->events_used++;
+    if (c->events_used >= 3072)
+        return fold_timing(c);
+    return RPL_OK;
+}
+
+int launch_stage(rpl_ctx *c, double dt, int stage, const void *Nk, const void *N0, void *out)
+{
+    int rc = record_begin(c);
+    if (rc != RPL_OK)
+        return rc;
+    hipError_t e;
+    const int tr = c->cfg.transport;
+    if (is_f64(c)) {
+        auto p = stage_params<double>(c, dt);
+        e = rpl::launch_stage_reference(p, tr, stage, (const double *)Nk, (const double *)N0, (double *)out,
+                                        c->stream);
+    } else {
+        auto p = stage_params<float>(c, dt);
+        if (use_fast(c))
+            e = rpl::launch_stage_fast(p, tr, stage, (const float *)Nk, (const float *)N0, (float *)out, c->stream);
+        else
+            e = rpl::launch_stage_reference(p, tr, stage, (const float *)Nk, (const float *)N0, (float *)out,
+                                            c->stream);
+    }
+    if (e != hipSuccess)
+        return fail(RPL_ERR_HIP, std::string("stage kernel launch: ") + hipGetErrorString(e));
+    return record_end(c);
+}
+
+int check_ready(rpl_ctx *c)
+{
+    if (!c)
+        return fail(RPL_ERR_ARG, "null context");
+    if (!c->has_geometry)
+        return fail(RPL_ERR_STATE, "rpl_set_geometry has not been called");
+    if (c->cfg.transport == RPL_TRANSPORT_REFRACTION && !c->has_current)
+        return fail(RPL_ERR_STATE, "refraction transport needs rpl_set_current or rpl_set_doppler first");
+    RPL_HIP(hipSetDevice(c->device));
+    return RPL_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *rpl_last_error(void) { return g_last_error.c_str(); }
+
+int rpl_create(rpl_ctx **out, const rpl_config *cfg)
+{
+    if (!out || !cfg)
+        return fail(RPL_ERR_ARG, "null argument");
+    *out = nullptr;
+    const rpl_config &k = *cfg;
+    if (k.nx < 3 || k.ny < 3)
+        return fail(RPL_ERR_ARG, "nx and ny must be >= 3 (periodic WENO5 stencils reach 3 cells)");
+    if (k.nphi < 3)
+        return fail(RPL_ERR_ARG, "nphi must be >= 3 (periodic WENO5 stencil in phi)");
+    if (k.nkappa < 1)
+        return fail(RPL_ERR_ARG, "nkappa must be >= 1");
+    if (k.dtype != RPL_F32 && k.dtype != RPL_F64)
+        return fail(RPL_ERR_ARG, "dtype must be RPL_F32 or RPL_F64");
+    if (k.timestepper != RPL_RK3 && k.timestepper != RPL_LSRK3 && k.timestepper != RPL_FORWARD_EULER)
+        return fail(RPL_ERR_ARG, "unsupported timestepper");
+    if (k.transport < RPL_TRANSPORT_NONE || k.transport > RPL_TRANSPORT_REFRACTION)
+        return fail(RPL_ERR_ARG, "unsupported transport");
+    if (k.kernel < RPL_KERNEL_AUTO || k.kernel > RPL_KERNEL_FAST)
+        return fail(RPL_ERR_ARG, "unsupported kernel selector");
+    if (k.kernel == RPL_KERNEL_FAST && k.dtype != RPL_F32)
+        return fail(RPL_ERR_UNSUPPORTED, "the FAST stencil is FP32 only");
+    if ((long long)k.nkappa * k.nphi > 65535)
+        return fail(RPL_ERR_UNSUPPORTED, "nkappa * nphi must be <= 65535");
+
+    rpl_ctx *c = new rpl_ctx();
+    c->cfg = k;
+    c->plane = (size_t)k.nx * k.ny;
+    c->cnt = c->plane * k.nkappa * k.nphi;
+    c->esize = k.dtype == RPL_F64 ? sizeof(double) : sizeof(float);
+    int rc = RPL_OK;
+    hipError_t e = hipGetDevice(&c->device);
+    if (e == hipSuccess)
+        e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(RPL_ERR_HIP, std::string("stream creation: ") + hipGetErrorString(e));
+    }
+    const size_t p3 = c->plane * k.nkappa, tab = (size_t)k.nkappa * k.nphi;
+    void **bufs3[] = {&c->Ux, &c->Uy, &c->Uxx, &c->Uxy, &c->Uyx, &c->Uyy};
+    if ((rc = alloc(&c->A, c->cnt * c->esize)) || (rc = alloc(&c->B, c->cnt * c->esize)) ||
+        (rc = alloc(&c->C, c->cnt * c->esize)))
+        goto error;
+    for (void **b : bufs3)
+        if ((rc = alloc(b, p3 * c->esize)))
+            goto error;
+    if ((rc = alloc(&c->kappa, k.nkappa * c->esize)) || (rc = alloc(&c->cgx, tab * c->esize)) ||
+        (rc = alloc(&c->cgy, tab * c->esize)) || (rc = alloc(&c->cosp, k.nphi * c->esize)) ||
+        (rc = alloc(&c->sinp, k.nphi * c->esize)))
+        goto error;
+    if ((rc = alloc((void **)&c->w, tab * 8)) || (rc = alloc((void **)&c->trace, tab * 8)) ||
+        (rc = alloc((void **)&c->mux, tab * 8)) || (rc = alloc((void **)&c->muy, tab * 8)) ||
+        (rc = alloc((void **)&c->energy, tab * 8)))
+        goto error;
+    if ((rc = alloc((void **)&c->partials, 2 * 1024 * 8)))
+        goto error;
+    e = hipMemsetAsync(c->A, 0, c->cnt * c->esize, c->stream);
+    if (e == hipSuccess)
+        e = hipMemsetAsync(c->Ux, 0, p3 * c->esize, c->stream);
+    if (e == hipSuccess)
+        e = hipMemsetAsync(c->Uy, 0, p3 * c->esize, c->stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        rc = fail(RPL_ERR_HIP, std::string("initial clear: ") + hipGetErrorString(e));
+        goto error;
+    }
+    *out = c;
+    return RPL_OK;
+error:
+    rpl_destroy(c);
+    return rc;
+}
+
+int rpl_destroy(rpl_ctx *c)
+{
+    if (!c)
+        return RPL_OK;
+    (void)hipSetDevice(c->device);
+    if (c->stream)
+        (void)hipStreamSynchronize(c->stream);
+    for (void *p : {c->A, c->B, c->C, c->Ux, c->Uy, c->Uxx, c->Uxy, c->Uyx, c->Uyy, c->kappa, c->cgx, c->cgy,
+                    c->cosp, c->sinp, (void *)c->w, (void *)c->trace, (void *)c->mux, (void *)c->muy,
+                    (void *)c->energy, (void *)c->diag, (void *)c->partials, c->scratch_in})
+        free_ptr(p);
+    for (auto &ev : c->events) {
+        (void)hipEventDestroy(ev.first);
+        (void)hipEventDestroy(ev.second);
+    }
+    if (c->stream)
+        (void)hipStreamDestroy(c->stream);
+    delete c;
+    return RPL_OK;
+}
+
+int rpl_set_geometry(rpl_ctx *c, double dx, double dy, const double *kappa_c, const double *kappa_f,
+                     const double *phi_c, const double *phi_f, int32_t depth_kind, double depth, double gravity)
+{
+    if (!c || !kappa_c || !kappa_f || !phi_c || !phi_f)
+        return fail(RPL_ERR_ARG, "null argument");
+    if (!(dx > 0) || !(dy > 0))
+        return fail(RPL_ERR_ARG, "dx and dy must be positive");
+    if (!(gravity > 0))
+        return fail(RPL_ERR_ARG, "gravity must be positive");
+    if (depth_kind != RPL_DEPTH_INFINITE && depth_kind != RPL_DEPTH_CONSTANT)
+        return fail(RPL_ERR_ARG, "depth_kind must be RPL_DEPTH_INFINITE or RPL_DEPTH_CONSTANT");
+    if (depth_kind == RPL_DEPTH_CONSTANT && !(depth > 0))
+        return fail(RPL_ERR_ARG, "finite-depth dispersion requires positive depth");  // transport.jl:77
+    const int nk = c->cfg.nkappa, np = c->cfg.nphi;
+    for (int m = 0; m < nk; ++m)
+        if (!(kappa_f[m + 1] > kappa_f[m]) || !(kappa_f[m] <= kappa_c[m] && kappa_c[m] <= kappa_f[m + 1]))
+            return fail(RPL_ERR_ARG, "kappa faces must increase and bracket the centres");
+    for (int n = 0; n < np; ++n)
+        if (!(phi_f[n + 1] > phi_f[n]))
+            return fail(RPL_ERR_ARG, "phi faces must increase");
+    RPL_HIP(hipSetDevice(c->device));
+    const size_t tab = (size_t)nk * np;
+    std::vector<double> w(tab), tr(tab), mx(tab), my(tab), en(tab), cgx(tab), cgy(tab), cs(np), sn(np);
+    const double sqrt_g = std::sqrt(gravity);
+    for (int n = 0; n < np; ++n) {
+        cs[n] = std::cos(phi_c[n]);
+        sn[n] = std::sin(phi_c[n]);
+        for (int m = 0; m < nk; ++m) {
+            const size_t t = (size_t)m + (size_t)nk * n;
+            const double k1 = kappa_f[m], k2 = kappa_f[m + 1], p1 = phi_f[n], p2 = phi_f[n + 1];
+            // polar_wavevector_grid.jl:44-47: exact polar cell measure.
+            w[t] = ((k2 * k2 - k1 * k1) / 2) * (p2 - p1);
+            // polar_wavevector_grid.jl:73-115: first moments and trace of the second.
+            const double r1 = (k2 * k2 * k2 - k1 * k1 * k1) / 3;
+            mx[t] = r1 * (std::sin(p2) - std::sin(p1));
+            my[t] = r1 * (std::cos(p1) - std::cos(p2));
+            const double r2 = (std::pow(k2, 4) - std::pow(k1, 4)) / 4;
+            const double s2 = std::sin(2 * p2) - std::sin(2 * p1);
+            tr[t] = r2 * ((p2 - p1) / 2 + s2 / 4) + r2 * ((p2 - p1) / 2 - s2 / 4);
+            // bulk_statistics.jl:24-26 + finite_volume_integration.jl:24-27 (power 1/2).
+            en[t] = sqrt_g * ((p2 - p1) * (std::pow(k2, 2.5) - std::pow(k1, 2.5)) / 2.5);
+            if (depth_kind == RPL_DEPTH_INFINITE) {
+                // transport.jl:61-70: cell-averaged deep-water group velocity.
+                const double radial = (std::pow(k2, 1.5) - std::pow(k1, 1.5)) / 1.5;
+                const double scale = sqrt_g / (2 * w[t]);
+                cgx[t] = scale * radial * (std::sin(p2) - std::sin(p1));
+                cgy[t] = scale * radial * (std::cos(p1) - std::cos(p2));
+            } else {
+                // transport.jl:72-83: finite depth, point-sampled.
+                const double kk = kappa_c[m];
+                const double kx = kk * std::cos(phi_c[n]), ky = kk * std::sin(phi_c[n]);
+                if (kk == 0) {
+                    cgx[t] = cgy[t] = 0;
+                } else {
+                    const double mu = kk * depth;
+                    const double phase = std::sqrt(gravity * std::tanh(mu) / kk);
+                    const double factor = (1 + 2 * mu / std::sinh(2 * mu)) / 2;
+                    const double cg = factor * phase;
+                    cgx[t] = cg * kx / kk;
+                    cgy[t] = cg * ky / kk;
+                }
+            }
+        }
+    }
+    std::vector<double> kc(kappa_c, kappa_c + nk);
+    int rc;
+    if ((rc = upload_real(c, kc.data(), c->kappa, nk)) || (rc = upload_real(c, cgx.data(), c->cgx, tab)) ||
+        (rc = upload_real(c, cgy.data(), c->cgy, tab)) || (rc = upload_real(c, cs.data(), c->cosp, np)) ||
+        (rc = upload_real(c, sn.data(), c->sinp, np)))
+        return rc;
+    if ((rc = upload_f64(c, w, c->w)) || (rc = upload_f64(c, tr, c->trace)) || (rc = upload_f64(c, mx, c->mux)) ||
+        (rc = upload_f64(c, my, c->muy)) || (rc = upload_f64(c, en, c->energy)))
+        return rc;
+    RPL_HIP(hipStreamSynchronize(c->stream));
+    c->dx = dx;
+    c->dy = dy;
+    c->dk = kappa_f[1] - kappa_f[0];  // first(coordinate_spacings(cgrid, 1))
+    c->dp = phi_f[1] - phi_f[0];
+    c->has_geometry = true;
+    return RPL_OK;
+}
+
+static int compute_gradients(rpl_ctx *c)
+{
+    const int nx = c->cfg.nx, ny = c->cfg.ny, nk = c->cfg.nkappa;
+    hipError_t e;
+    if (is_f64(c))
+        e = rpl::launch_gradients<double>(nx, ny, nk, (const double *)c->Ux, (const double *)c->Uy, 1.0 / c->dx,
+                                          1.0 / c->dy, (double *)c->Uxx, (double *)c->Uxy, (double *)c->Uyx,
+                                          (double *)c->Uyy, c->stream);
+    else
+        e = rpl::launch_gradients<float>(nx, ny, nk, (const float *)c->Ux, (const float *)c->Uy,
+                                         1.0f / (float)c->dx, 1.0f / (float)c->dy, (float *)c->Uxx,
+                                         (float *)c->Uxy, (float *)c->Uyx, (float *)c->Uyy, c->stream);
+    if (e != hipSuccess)
+        return fail(RPL_ERR_HIP, std::string("gradients: ") + hipGetErrorString(e));
+    RPL_HIP(hipStreamSynchronize(c->stream));
+    c->has_current = true;
+    return RPL_OK;
+}
+
+// q_kernel.jl:77-85 (q_cdf_kernel).
+static double q_cdf(double kappa, double z, double depth)
+{
+    const double small_mu = 1e-6;
+    const double mu = kappa * depth;
+    const double mus = std::fabs(mu) < small_mu ? (mu < 0 ? -small_mu : small_mu) : mu;
+    double s = (z + depth) / depth;
+    s = s < 0 ? 0 : (s > 1 ? 1 : s);
+    const double a = 2 * mus * s, b = 2 * mus;
+    const double fin = (std::exp(a - b) - std::exp(-a - b)) / (1 - std::exp(-2 * b));
+    return std::fabs(mu) < small_mu ? s : fin;
+}
+
+int rpl_set_current(rpl_ctx *c, int32_t nz, const double *z_faces, const double *u, const double *v, double q_depth)
+{
+    if (!c || !z_faces || !u || !v)
+        return fail(RPL_ERR_ARG, "null argument");
+    if (!c->has_geometry)
+        return fail(RPL_ERR_STATE, "rpl_set_geometry has not been called");
+    if (nz < 1)
+        return fail(RPL_ERR_ARG, "nz must be >= 1");
+    if (!(q_depth > 0))
+        return fail(RPL_ERR_ARG, "depth must be positive");  // q_kernel.jl:19
+    const int nx = c->cfg.nx, ny = c->cfg.ny, nk = c->cfg.nkappa;
+    RPL_HIP(hipSetDevice(c->device));
+    std::vector<double> kc(nk);
+    if (is_f64(c)) {
+        RPL_HIP(hipMemcpy(kc.data(), c->kappa, nk * 8, hipMemcpyDeviceToHost));
+    } else {
+        std::vector<float> kf(nk);
+        RPL_HIP(hipMemcpy(kf.data(), c->kappa, nk * 4, hipMemcpyDeviceToHost));
+        for (int m = 0; m < nk; ++m)
+            kc[m] = kf[m];
+    }
+    // q_kernel.jl:87-92: W[k,m] = CDF(zhigh) - CDF(zlow), k fastest.
+    std::vector<double> W((size_t)nz * nk);
+    for (int m = 0; m < nk; ++m)
+        for (int k = 0; k < nz; ++k) {
+            const double lo = std::min(z_faces[k], z_faces[k + 1]), hi = std::max(z_faces[k], z_faces[k + 1]);
+            W[k + (size_t)nz * m] = q_cdf(kc[m], hi, q_depth) - q_cdf(kc[m], lo, q_depth);
+        }
+    const size_t nu = c->plane * nz;
+    void *du = nullptr, *dv = nullptr, *dW = nullptr;
+    int rc;
+    if ((rc = alloc(&du, nu * c->esize)) || (rc = alloc(&dv, nu * c->esize)) ||
+        (rc = alloc(&dW, W.size() * c->esize))) {
+        free_ptr(du);
+        free_ptr(dv);
+        return rc;
+    }
+    rc = upload_real(c, u, du, nu);
+    if (!rc)
+        rc = upload_real(c, v, dv, nu);
+    if (!rc)
+        rc = upload_real(c, W.data(), dW, W.size());
+    if (!rc) {
+        hipError_t e;
+        if (is_f64(c))
+            e = rpl::launch_doppler(nx, ny, nz, nk, (const double *)du, (const double *)dv, (const double *)dW,
+                                    (double *)c->Ux, (double *)c->Uy, c->stream);
+        else
+            e = rpl::launch_doppler(nx, ny, nz, nk, (const float *)du, (const float *)dv, (const float *)dW,
+                                    (float *)c->Ux, (float *)c->Uy, c->stream);
+        if (e != hipSuccess)
+            rc = fail(RPL_ERR_HIP, std::string("doppler: ") + hipGetErrorString(e));
+        else if ((e = hipStreamSynchronize(c->stream)) != hipSuccess)
+            rc = fail(RPL_ERR_HIP, std::string("doppler: ") + hipGetErrorString(e));
+    }
+    free_ptr(du);
+    free_ptr(dv);
+    free_ptr(dW);
+    if (rc)
+        return rc;
+    return compute_gradients(c);
+}
+
+int rpl_set_doppler(rpl_ctx *c, const double *Ux, const double *Uy)
+{
+    if (!c || !Ux || !Uy)
+        return fail(RPL_ERR_ARG, "null argument");
+    if (!c->has_geometry)
+        return fail(RPL_ERR_STATE, "rpl_set_geometry has not been called");
+    RPL_HIP(hipSetDevice(c->device));
+    const size_t p3 = c->plane * c->cfg.nkappa;
+    int rc;
+    if ((rc = upload_real(c, Ux, c->Ux, p3)) || (rc = upload_real(c, Uy, c->Uy, p3)))
+        return rc;
+    return compute_gradients(c);
+}
+
+int rpl_get_doppler(rpl_ctx *c, double *Ux, double *Uy)
+{
+    if (!c || !Ux || !Uy)
+        return fail(RPL_ERR_ARG, "null argument");
+    RPL_HIP(hipSetDevice(c->device));
+    RPL_HIP(hipStreamSynchronize(c->stream));
+    const size_t p3 = c->plane * c->cfg.nkappa;
+    if (is_f64(c)) {
+        RPL_HIP(hipMemcpy(Ux, c->Ux, p3 * 8, hipMemcpyDeviceToHost));
+        RPL_HIP(hipMemcpy(Uy, c->Uy, p3 * 8, hipMemcpyDeviceToHost));
+    } else {
+        std::vector<float> t(p3);
+        RPL_HIP(hipMemcpy(t.data(), c->Ux, p3 * 4, hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < p3; ++k)
+            Ux[k] = t[k];
+        RPL_HIP(hipMemcpy(t.data(), c->Uy, p3 * 4, hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < p3; ++k)
+            Uy[k] = t[k];
+    }
+    return RPL_OK;
+}
+
+int rpl_set_action(rpl_ctx *c, const void *N)
+{
+    if (!c || !N)
+        return fail(RPL_ERR_ARG, "null argument");
+    RPL_HIP(hipSetDevice(c->device));
+    RPL_HIP(hipMemcpyAsync(c->A, N, c->cnt * c->esize, hipMemcpyHostToDevice, c->stream));
+    RPL_HIP(hipStreamSynchronize(c->stream));
+    return RPL_OK;
+}
+
+int rpl_get_action(rpl_ctx *c, void *N)
+{
+    if (!c || !N)
+        return fail(RPL_ERR_ARG, "null argument");
+    RPL_HIP(hipSetDevice(c->device));
+    RPL_HIP(hipStreamSynchronize(c->stream));
+    RPL_HIP(hipMemcpy(N, c->A, c->cnt * c->esize, hipMemcpyDeviceToHost));
+    return RPL_OK;
+}
+
+int rpl_tendency(rpl_ctx *c, void *G)
+{
+    int rc = check_ready(c);
+    if (rc)
+        return rc;
+    if (!G)
+        return fail(RPL_ERR_ARG, "null argument");
+    if ((rc = launch_stage(c, 0.0, rpl::STAGE_TENDENCY, c->A, c->A, c->B)))
+        return rc;
+    RPL_HIP(hipStreamSynchronize(c->stream));
+    RPL_HIP(hipMemcpy(G, c->B, c->cnt * c->esize, hipMemcpyDeviceToHost));
+    return RPL_OK;
+}
+
+int rpl_time_step(rpl_ctx *c, double dt, int32_t nsteps)
+{
+    int rc = check_ready(c);
+    if (rc)
+        return rc;
+    if (!(dt > 0))
+        return fail(RPL_ERR_ARG, "time step must be positive");  // time_step.jl:120
+    if (nsteps < 0)
+        return fail(RPL_ERR_ARG, "nsteps must be >= 0");
+    for (int s = 0; s < nsteps; ++s) {
+        if (c->cfg.timestepper == RPL_FORWARD_EULER) {
+            if ((rc = launch_stage(c, dt, rpl::STAGE_1, c->A, c->A, c->B)))
+                return rc;
+            std::swap(c->A, c->B);
+        } else {
+            // SSP-RK3 over three buffers; A holds N0 and receives N3 in place.
+            if ((rc = launch_stage(c, dt, rpl::STAGE_1, c->A, c->A, c->B)) ||
+                (rc = launch_stage(c, dt, rpl::STAGE_2, c->B, c->A, c->C)) ||
+                (rc = launch_stage(c, dt, rpl::STAGE_3, c->C, c->A, c->A)))
+                return rc;
+        }
+        c->time += dt;  // tick!(clock, dt), once per step
+        c->iteration += 1;
+    }
+    return RPL_OK;
+}
+
+int rpl_synchronize(rpl_ctx *c)
+{
+    if (!c)
+        return fail(RPL_ERR_ARG, "null context");
+    RPL_HIP(hipSetDevice(c->device));
+    RPL_HIP(hipStreamSynchronize(c->stream));
+    return RPL_OK;
+}
+
+int rpl_get_clock(rpl_ctx *c, double *time, int64_t *iteration)
+{
+    if (!c)
+        return fail(RPL_ERR_ARG, "null context");
+    if (time)
+        *time = c->time;
+    if (iteration)
+        *iteration = c->iteration;
+    return RPL_OK;
+}
+
+int rpl_set_clock(rpl_ctx *c, double time, int64_t iteration)
+{
+    if (!c)
+        return fail(RPL_ERR_ARG, "null context");
+    c->time = time;
+    c->iteration = iteration;
+    return RPL_OK;
+}
+
+int rpl_diagnostics(rpl_ctx *c, double *m0, double *kappa_rms, double *mean_dir)
+{
+    if (!c)
+        return fail(RPL_ERR_ARG, "null context");
+    if (!c->has_geometry)
+        return fail(RPL_ERR_STATE, "rpl_set_geometry has not been called");
+    RPL_HIP(hipSetDevice(c->device));
+    int rc;
+    if (!c->diag && (rc = alloc((void **)&c->diag, 3 * c->plane * 8)))
+        return rc;
+    const int nx = c->cfg.nx, ny = c->cfg.ny, nk = c->cfg.nkappa, np = c->cfg.nphi;
+    double *d0 = c->diag, *d1 = c->diag + c->plane, *d2 = c->diag + 2 * c->plane;
+    hipError_t e;
+    if (is_f64(c))
+        e = rpl::launch_column_diagnostics<double>(nx, ny, nk, np, (const double *)c->A, c->w, c->trace, c->mux,
+                                                   c->muy, d0, d1, d2, c->stream);
+    else
+        e = rpl::launch_column_diagnostics<float>(nx, ny, nk, np, (const float *)c->A, c->w, c->trace, c->mux,
+                                                  c->muy, d0, d1, d2, c->stream);
+    if (e != hipSuccess)
+        return fail(RPL_ERR_HIP, std::string("diagnostics: ") + hipGetErrorString(e));
+    RPL_HIP(hipStreamSynchronize(c->stream));
+    if (m0)
+        RPL_HIP(hipMemcpy(m0, d0, c->plane * 8, hipMemcpyDeviceToHost));
+    if (kappa_rms)
+        RPL_HIP(hipMemcpy(kappa_rms, d1, c->plane * 8, hipMemcpyDeviceToHost));
+    if (mean_dir)
+        RPL_HIP(hipMemcpy(mean_dir, d2, c->plane * 8, hipMemcpyDeviceToHost));
+    return RPL_OK;
+}
+
+static int weighted_total(rpl_ctx *c, const double *measure, double *result)
+{
+    const int nx = c->cfg.nx, ny = c->cfg.ny, nk = c->cfg.nkappa, np = c->cfg.nphi;
+    const int blocks = rpl::weighted_total_blocks(nx, ny);
+    const double area = c->dx * c->dy;
+    hipError_t e;
+    if (is_f64(c))
+        e = rpl::launch_weighted_total<double>(nx, ny, nk, np, (const double *)c->A, measure, area, c->partials,
+                                               blocks, c->stream);
+    else
+        e = rpl::launch_weighted_total<float>(nx, ny, nk, np, (const float *)c->A, measure, area, c->partials,
+                                              blocks, c->stream);
+    if (e != hipSuccess)
+        return fail(RPL_ERR_HIP, std::string("weighted total: ") + hipGetErrorString(e));
+    std::vector<double> part(2 * (size_t)blocks);
+    RPL_HIP(hipStreamSynchronize(c->stream));
+    RPL_HIP(hipMemcpy(part.data(), c->partials, part.size() * 8, hipMemcpyDeviceToHost));
+    double hi = 0, lo = 0;
+    for (int b = 0; b < blocks; ++b) {
+        const double x = part[2 * b];
+        const double s = hi + x, bv = s - hi, av = s - bv;
+        lo += (hi - av) + (x - bv) + part[2 * b + 1];
+        hi = s;
+    }
+    *result = hi + lo;
+    return RPL_OK;
+}
+
+int rpl_conservation(rpl_ctx *c, double *total_action, double *total_energy)
+{
+    if (!c)
+        return fail(RPL_ERR_ARG, "null context");
+    if (!c->has_geometry)
+        return fail(RPL_ERR_STATE, "rpl_set_geometry has not been called");
+    RPL_HIP(hipSetDevice(c->device));
+    int rc;
+    if (total_action && (rc = weighted_total(c, c->w, total_action)))
+        return rc;
+    if (total_energy && (rc = weighted_total(c, c->energy, total_energy)))
+        return rc;
+    return RPL_OK;
+}
+
+int rpl_enable_kernel_timing(rpl_ctx *c, int32_t enable)
+{
+    if (!c)
+        return fail(RPL_ERR_ARG, "null context");
+    int rc = fold_timing(c);
+    if (rc)
+        return rc;
+    c->timing = enable != 0;
+    return RPL_OK;
+}
+
+int rpl_get_kernel_timing(rpl_ctx *c, double *total_ms, int64_t *launches)
+{
+    if (!c)
+        return fail(RPL_ERR_ARG, "null context");
+    int rc = fold_timing(c);
+    if (rc)
+        return rc;
+    if (total_ms)
+        *total_ms = c->timing_ms;
+    if (launches)
+        *launches = c->timing_launches;
+    c->timing_ms = 0;
+    c->timing_launches = 0;
+    return RPL_OK;
+}
+
+void *rpl_stream(rpl_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
+}  // extern "C"

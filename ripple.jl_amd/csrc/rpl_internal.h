@@ -1,0 +1,70 @@
+// Internal declarations shared by the host API (rpl_capi.cpp) and the HIP
+// kernel translation units.  Not part of the ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+#include "../../include/ripple_mi355x.h"
+
+namespace rpl {
+
+// Everything the fused tendency + RK3 stage kernels read besides the action.
+// T is the action's element type.  Tables are (nkappa, nphi), m fastest.
+template <typename T>
+struct StageParams {
+    int nx, ny, nk, nphi;
+    const T *Ux, *Uy;                 // Doppler velocity caches (nx, ny, nk)
+    const T *Uxx, *Uxy, *Uyx, *Uyy;   // current gradients (nx, ny, nk)
+    const T *kappa;                   // kappa centres (nk)
+    const T *cgx, *cgy;               // intrinsic group velocity (nk, nphi)
+    const T *cosp, *sinp;             // cos/sin of phi centres (nphi)
+    T dx_inv, dy_inv, dk_inv, dp_inv; // 1 / first spacing
+    T dt;                             // time step in T (time_step.jl:16-20)
+    const T *src_add;                 // optional additive tendency (nx,ny,nk,nphi) or null
+};
+
+// Stage selector of the fused stage kernels (time_step.jl:138-152):
+//   0  out = L(Nk)                                   (tendency only)
+//   1  out = max(0, Nk + dt L(Nk))                    (RK3 stage 1, Euler)
+//   2  out = max(0, 3/4 N0 + 1/4 max(0, Nk + dt L))   (RK3 stage 2)
+//   3  out = max(0, 1/3 N0 + 2/3 max(0, Nk + dt L))   (RK3 stage 3)
+// N0 is read only at the output cell, so stage 3 may write in place over N0.
+enum { STAGE_TENDENCY = 0, STAGE_1 = 1, STAGE_2 = 2, STAGE_3 = 3 };
+
+hipError_t launch_stage_reference(const StageParams<double> &p, int transport, int stage,
+                                  const double *Nk, const double *N0, double *out, hipStream_t s);
+hipError_t launch_stage_reference(const StageParams<float> &p, int transport, int stage,
+                                  const float *Nk, const float *N0, float *out, hipStream_t s);
+hipError_t launch_stage_fast(const StageParams<float> &p, int transport, int stage,
+                             const float *Nk, const float *N0, float *out, hipStream_t s);
+
+// Coupling: U[i,j,m] = sum_k u[i,j,k] W[k,m] (W is (nz, nk), k fastest) and the
+// centred periodic gradients of U.
+hipError_t launch_doppler(int nx, int ny, int nz, int nk, const double *u, const double *v,
+                          const double *W, double *Ux, double *Uy, hipStream_t s);
+hipError_t launch_doppler(int nx, int ny, int nz, int nk, const float *u, const float *v,
+                          const float *W, float *Ux, float *Uy, hipStream_t s);
+template <typename T>
+hipError_t launch_gradients(int nx, int ny, int nk, const T *Ux, const T *Uy, T dx_inv, T dy_inv,
+                            T *Uxx, T *Uxy, T *Uyx, T *Uyy, hipStream_t s);
+
+// Diagnostics: per-column sums with FP64 accumulation; compensated totals.
+template <typename T>
+hipError_t launch_column_diagnostics(int nx, int ny, int nk, int nphi, const T *N,
+                                     const double *w, const double *trace, const double *mux,
+                                     const double *muy, double *m0, double *krms, double *mdir,
+                                     hipStream_t s);
+// Writes 2*nblocks doubles (hi, lo pairs) of the weighted total sum_ij (sum_mn N w) * area.
+template <typename T>
+hipError_t launch_weighted_total(int nx, int ny, int nk, int nphi, const T *N, const double *w,
+                                 double area, double *partials, int nblocks, hipStream_t s);
+int weighted_total_blocks(int nx, int ny);
+
+template <typename T>
+hipError_t launch_cast_from_f64(const double *src, T *dst, size_t n, hipStream_t s);
+template <typename T>
+hipError_t launch_cast_to_f64(const T *src, double *dst, size_t n, hipStream_t s);
+
+}  // namespace rpl

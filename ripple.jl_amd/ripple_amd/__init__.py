@@ -1,0 +1,26 @@
+"""MI355X wave-action time step with the Ripple.jl host surface.
+
+The reference API names map as: SpectralWaveModel, set! -> set_,
+time_step! -> time_step_, Simulation, run! -> run_, interior, m0,
+root_mean_square_wavenumber, mean_direction, significant_wave_height,
+total_action, total_deep_water_energy.  Compute runs in the gfx950 library
+libripple_mi355x.so through its C ABI.
+"""
+from ._lib import RippleError, RippleArgumentError, load as load_library, LIB_PATH
+from .grids import (RectilinearGrid, PolarWaveVectorGrid, FrequencyDirectionGrid, centers_to_faces,
+                    spectral_cell_measures, xnodes, ynodes, znodes, xfaces, yfaces, zfaces,
+                    xspacings, yspacings, zspacings, PERIODIC, BOUNDED, FLAT)
+from .model import (SpectralWaveModel, InfiniteDepth, Clock, Simulation, WENO, set_, time_step_, run_,
+                    interior)
+from .diagnostics import (m0, root_mean_square_wavenumber, mean_direction, significant_wave_height,
+                          total_action, total_deep_water_energy)
+
+__all__ = [
+    "RippleError", "RippleArgumentError", "load_library", "LIB_PATH",
+    "RectilinearGrid", "PolarWaveVectorGrid", "FrequencyDirectionGrid", "centers_to_faces",
+    "spectral_cell_measures", "xnodes", "ynodes", "znodes", "xfaces", "yfaces", "zfaces",
+    "xspacings", "yspacings", "zspacings", "PERIODIC", "BOUNDED", "FLAT",
+    "SpectralWaveModel", "InfiniteDepth", "Clock", "Simulation", "WENO", "set_", "time_step_", "run_",
+    "interior", "m0", "root_mean_square_wavenumber", "mean_direction", "significant_wave_height",
+    "total_action", "total_deep_water_energy",
+]

@@ -1,0 +1,116 @@
+"""ctypes binding of libripple_mi355x.so (the C ABI in include/ripple_mi355x.h).
+
+The library is the only compute path: if it is missing, or no HIP device is
+visible when a context is created, the calls fail loudly.  There is no CPU
+fallback.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libripple_mi355x.so")
+
+RPL_OK = 0
+RPL_F32, RPL_F64 = 0, 1
+RPL_RK3, RPL_LSRK3, RPL_FORWARD_EULER = 0, 1, 2
+RPL_TRANSPORT_NONE, RPL_TRANSPORT_INTRINSIC, RPL_TRANSPORT_REFRACTION = 0, 1, 2
+RPL_DEPTH_INFINITE, RPL_DEPTH_CONSTANT = 0, 1
+RPL_KERNEL_AUTO, RPL_KERNEL_REFERENCE, RPL_KERNEL_FAST = 0, 1, 2
+
+ERRORS = {
+    -1: "RPL_ERR_ARG",
+    -2: "RPL_ERR_STATE",
+    -3: "RPL_ERR_HIP",
+    -4: "RPL_ERR_UNSUPPORTED",
+    -5: "RPL_ERR_NOMEM",
+}
+
+
+class RippleError(RuntimeError):
+    """A negative status returned by the C ABI."""
+
+    def __init__(self, code, message):
+        super().__init__(f"{ERRORS.get(code, code)}: {message}")
+        self.code = code
+
+
+class RippleArgumentError(RippleError, ValueError):
+    """RPL_ERR_ARG: the reference raises ArgumentError in the same places."""
+
+
+class rpl_config(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int32), ("ny", ctypes.c_int32),
+        ("nkappa", ctypes.c_int32), ("nphi", ctypes.c_int32),
+        ("dtype", ctypes.c_int32), ("timestepper", ctypes.c_int32),
+        ("transport", ctypes.c_int32), ("kernel", ctypes.c_int32),
+    ]
+
+
+_P = ctypes.c_void_p
+_D = ctypes.c_double
+_DP = ctypes.POINTER(ctypes.c_double)
+_I32 = ctypes.c_int32
+_I64P = ctypes.POINTER(ctypes.c_int64)
+
+# name -> (argtypes, restype); every entry point declared in include/ripple_mi355x.h.
+SIGNATURES = {
+    "rpl_create": ([ctypes.POINTER(_P), ctypes.POINTER(rpl_config)], ctypes.c_int),
+    "rpl_destroy": ([_P], ctypes.c_int),
+    "rpl_set_geometry": ([_P, _D, _D, _DP, _DP, _DP, _DP, _I32, _D, _D], ctypes.c_int),
+    "rpl_set_current": ([_P, _I32, _DP, _DP, _DP, _D], ctypes.c_int),
+    "rpl_set_doppler": ([_P, _DP, _DP], ctypes.c_int),
+    "rpl_get_doppler": ([_P, _DP, _DP], ctypes.c_int),
+    "rpl_set_action": ([_P, _P], ctypes.c_int),
+    "rpl_get_action": ([_P, _P], ctypes.c_int),
+    "rpl_tendency": ([_P, _P], ctypes.c_int),
+    "rpl_time_step": ([_P, _D, _I32], ctypes.c_int),
+    "rpl_synchronize": ([_P], ctypes.c_int),
+    "rpl_get_clock": ([_P, _DP, _I64P], ctypes.c_int),
+    "rpl_set_clock": ([_P, _D, ctypes.c_int64], ctypes.c_int),
+    "rpl_diagnostics": ([_P, _DP, _DP, _DP], ctypes.c_int),
+    "rpl_conservation": ([_P, _DP, _DP], ctypes.c_int),
+    "rpl_enable_kernel_timing": ([_P, _I32], ctypes.c_int),
+    "rpl_get_kernel_timing": ([_P, _DP, _I64P], ctypes.c_int),
+    "rpl_stream": ([_P], ctypes.c_void_p),
+    "rpl_last_error": ([], ctypes.c_char_p),
+}
+
+_lib = None
+
+
+def load():
+    """Load the shared library once; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `python ripple.jl_amd/build.py` "
+            "(the HIP library is the only compute path; there is no CPU fallback)")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (args, res) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = lib
+    return lib
+
+
+def check(code):
+    """Raise the matching exception for a negative status."""
+    if code == RPL_OK:
+        return
+    msg = load().rpl_last_error().decode(errors="replace")
+    if code == -1:
+        raise RippleArgumentError(code, msg)
+    raise RippleError(code, msg)
+
+
+def dptr(a):
+    """Pointer to the data of a C-contiguous float64 numpy array."""
+    return a.ctypes.data_as(_DP)
+
+
+def vptr(a):
+    return ctypes.c_void_p(a.ctypes.data)

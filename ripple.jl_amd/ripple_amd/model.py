@@ -1,0 +1,270 @@
+"""SpectralWaveModel, set_, time_step_, Simulation and run_ on the MI355X.
+
+Mirrors Models/spectral_wave_model.jl:155-211 (construction and validation),
+Models/tendencies.jl:18-109 (which fused tendency runs), ProductFields/set.jl
+(set!), Models/time_step.jl:119-170 (time_step!) and Oceananigans'
+Simulation/run!.  Julia's `f!` names are spelt `f_` here.  All state lives in
+device memory owned by one C-ABI context; this module only marshals.
+"""
+import ctypes
+import math
+
+import numpy as np
+
+from . import _lib
+from .grids import PERIODIC, RectilinearGrid, PolarWaveVectorGrid, FrequencyDirectionGrid, xnodes, ynodes
+
+
+class InfiniteDepth:
+    """Deep-water marker (the reference's default `depth=InfiniteDepth()`)."""
+
+    def __repr__(self):
+        return "InfiniteDepth()"
+
+
+WENO = "WENO"
+_TIMESTEPPERS = {"RK3": _lib.RPL_RK3, "LSRK3": _lib.RPL_LSRK3, "LowStorageRK3": _lib.RPL_LSRK3,
+                 "ForwardEuler": _lib.RPL_FORWARD_EULER}
+_KERNELS = {"auto": _lib.RPL_KERNEL_AUTO, "reference": _lib.RPL_KERNEL_REFERENCE, "fast": _lib.RPL_KERNEL_FAST}
+
+
+class Clock:
+    def __init__(self, time=0.0, iteration=0):
+        self.time = float(time)
+        self.iteration = int(iteration)
+
+
+class _Context:
+    """Owns one rpl_ctx; destroyed with the model."""
+
+    def __init__(self, cfg):
+        self.lib = _lib.load()
+        self.ptr = ctypes.c_void_p()
+        _lib.check(self.lib.rpl_create(ctypes.byref(self.ptr), ctypes.byref(cfg)))
+
+    def __del__(self):
+        ptr = getattr(self, "ptr", None)
+        if ptr is not None and ptr.value:
+            self.lib.rpl_destroy(ptr)
+            self.ptr = ctypes.c_void_p()
+
+
+def _evaluate_on_centres(fun, grid, nz_nodes=None):
+    x = xnodes(grid)[:, None, None]
+    y = ynodes(grid)[None, :, None]
+    z = nz_nodes[None, None, :]
+    try:
+        v = np.asarray(fun(x, y, z), dtype=np.float64)
+        return np.broadcast_to(v, (len(x.ravel()), len(y.ravel()), len(z.ravel()))).copy()
+    except Exception:  # scalar-only user function
+        return np.vectorize(fun, otypes=[np.float64])(x, y, z)
+
+
+class SpectralWaveModel:
+    """`SpectralWaveModel(grid, spectral_grid; velocities, horizontal_advection,
+    spectral_advection, sources, depth, timestepper, clock)`.
+
+    velocities: dict/tuple with `u`, `v` as (Nx, Ny, Nz) arrays (or callables of
+    (x, y, z) sampled at cell centres), the prescribed Lagrangian-mean current.
+    `kernel` picks the stencil implementation ("auto", "reference", "fast").
+    """
+
+    def __init__(self, grid, spectral_grid, *, velocities=None, horizontal_advection=WENO,
+                 spectral_advection=WENO, sources=None, depth=None, timestepper="ForwardEuler",
+                 clock=None, kernel="auto", gravity=9.81):
+        if not isinstance(grid, RectilinearGrid):
+            raise TypeError(f"grid must be a RectilinearGrid; got {type(grid).__name__}")
+        if not isinstance(spectral_grid, (PolarWaveVectorGrid, FrequencyDirectionGrid)):
+            raise TypeError("spectral_grid must be a PolarWaveVectorGrid or FrequencyDirectionGrid")
+        if timestepper not in _TIMESTEPPERS:
+            raise ValueError(f"unsupported timestepper {timestepper}")
+        if grid.topology[0] != PERIODIC or grid.topology[1] != PERIODIC:
+            raise NotImplementedError("the fused kernels cover Periodic x Periodic grids only")
+        self.grid, self.spectral_grid = grid, spectral_grid
+        self.depth = InfiniteDepth() if depth is None else depth
+        if not isinstance(self.depth, InfiniteDepth) and not float(self.depth) > 0:
+            raise ValueError("depth must be positive")
+        self.horizontal_advection = horizontal_advection
+        self.spectral_advection = spectral_advection
+        self.sources = sources
+        self.timestepper = timestepper
+        self.clock = clock if clock is not None else Clock()
+        self.dtype = np.dtype(np.float64 if grid.float_type == np.float64 or
+                              spectral_grid.float_type == np.float64 else np.float32)
+        Nk, Np = spectral_grid.shape
+        self.size = (grid.Nx, grid.Ny, Nk, Np)
+
+        has_coupling = velocities is not None
+        if has_coupling and not isinstance(spectral_grid, PolarWaveVectorGrid):
+            raise ValueError("CWCM current coupling requires a PolarWaveVectorGrid")
+        # Models/tendencies.jl:18-109.
+        if has_coupling and spectral_advection is not None:
+            transport = _lib.RPL_TRANSPORT_REFRACTION
+        elif horizontal_advection is None:
+            transport = _lib.RPL_TRANSPORT_NONE
+        elif horizontal_advection == WENO and not has_coupling and Nk * Np > 1:
+            transport = _lib.RPL_TRANSPORT_INTRINSIC
+        else:
+            raise NotImplementedError("only the fused transport kernels are built (WENO, periodic, >1 bin)")
+        self.transport = transport
+
+        cfg = _lib.rpl_config(grid.Nx, grid.Ny, Nk, Np,
+                              _lib.RPL_F64 if self.dtype == np.float64 else _lib.RPL_F32,
+                              _TIMESTEPPERS[timestepper], transport, _KERNELS[kernel])
+        self._ctx = _Context(cfg)
+        lib, ctx = self._ctx.lib, self._ctx.ptr
+        sg = spectral_grid
+        depth_kind = _lib.RPL_DEPTH_INFINITE if isinstance(self.depth, InfiniteDepth) else _lib.RPL_DEPTH_CONSTANT
+        arrays = [np.ascontiguousarray(a, dtype=np.float64) for a in
+                  (sg.kappa, sg.kappa_faces, sg.phi, sg.phi_faces)]
+        _lib.check(lib.rpl_set_geometry(ctx, float(grid.x_faces[1] - grid.x_faces[0]),
+                                        float(grid.y_faces[1] - grid.y_faces[0]),
+                                        *[_lib.dptr(a) for a in arrays], depth_kind,
+                                        0.0 if depth_kind == _lib.RPL_DEPTH_INFINITE else float(self.depth),
+                                        float(gravity)))
+        self.velocities = None
+        if has_coupling:
+            self._set_velocities(velocities)
+        _lib.check(lib.rpl_set_clock(ctx, self.clock.time, self.clock.iteration))
+        if sources is not None:
+            sources.attach(self)
+
+    # --- coupling -----------------------------------------------------------
+    def _set_velocities(self, velocities):
+        g = self.grid
+        if g.Nz < 1:
+            raise ValueError("QTransform requires a grid with a resolved vertical coordinate")
+        u, v = (velocities["u"], velocities["v"]) if isinstance(velocities, dict) else velocities
+        zc = 0.5 * (g.z_faces[1:] + g.z_faces[:-1])
+        fields = []
+        for f in (u, v):
+            a = _evaluate_on_centres(f, g, zc) if callable(f) else np.asarray(f, dtype=np.float64)
+            if a.shape != (g.Nx, g.Ny, g.Nz):
+                raise ValueError(f"velocity arrays must have size {(g.Nx, g.Ny, g.Nz)}")
+            fields.append(np.asfortranarray(a))
+        # Coupling/lagrangian_velocities.jl:113-115: Q depth is the column depth
+        # for InfiniteDepth, else the model depth.
+        q_depth = g.Lz if isinstance(self.depth, InfiniteDepth) else float(self.depth)
+        zf = np.ascontiguousarray(g.z_faces, dtype=np.float64)
+        uf = np.ascontiguousarray(fields[0].ravel(order="F"))
+        vf = np.ascontiguousarray(fields[1].ravel(order="F"))
+        _lib.check(self._ctx.lib.rpl_set_current(self._ctx.ptr, g.Nz, _lib.dptr(zf), _lib.dptr(uf),
+                                                 _lib.dptr(vf), q_depth))
+        self.velocities = (fields[0], fields[1])
+
+    def doppler_velocity(self):
+        """(Ux, Uy) caches, each (Nx, Ny, Nkappa)."""
+        Nx, Ny, Nk, _ = self.size
+        Ux = np.zeros(Nx * Ny * Nk)
+        Uy = np.zeros(Nx * Ny * Nk)
+        _lib.check(self._ctx.lib.rpl_get_doppler(self._ctx.ptr, _lib.dptr(Ux), _lib.dptr(Uy)))
+        return Ux.reshape((Nx, Ny, Nk), order="F"), Uy.reshape((Nx, Ny, Nk), order="F")
+
+    def set_doppler_velocity(self, Ux, Uy):
+        a = [np.ascontiguousarray(np.asarray(x, dtype=np.float64).ravel(order="F")) for x in (Ux, Uy)]
+        _lib.check(self._ctx.lib.rpl_set_doppler(self._ctx.ptr, _lib.dptr(a[0]), _lib.dptr(a[1])))
+
+    # --- state ----------------------------------------------------------------
+    @property
+    def action(self):
+        """Interior of the wave-action ProductField as an (Nx, Ny, Nk, Nphi) array."""
+        return interior(self)
+
+    def _upload(self, values):
+        a = np.asarray(values, dtype=self.dtype)
+        if a.shape != self.size:
+            raise ValueError(f"array size {a.shape} does not match field size {self.size}")
+        flat = np.ascontiguousarray(a.ravel(order="F"))
+        _lib.check(self._ctx.lib.rpl_set_action(self._ctx.ptr, _lib.vptr(flat)))
+
+    def tendency(self):
+        """compute_tendencies!(model) followed by interior(model.tendencies)."""
+        G = np.empty(int(np.prod(self.size)), dtype=self.dtype)
+        _lib.check(self._ctx.lib.rpl_tendency(self._ctx.ptr, _lib.vptr(G)))
+        return G.reshape(self.size, order="F")
+
+    def synchronize(self):
+        _lib.check(self._ctx.lib.rpl_synchronize(self._ctx.ptr))
+
+    @property
+    def stream(self):
+        return self._ctx.lib.rpl_stream(self._ctx.ptr)
+
+    @property
+    def context(self):
+        return self._ctx.ptr
+
+
+def interior(model):
+    """interior(model.action): host copy, (Nx, Ny, Nk, Nphi), Fortran order."""
+    N = np.empty(int(np.prod(model.size)), dtype=model.dtype)
+    _lib.check(model._ctx.lib.rpl_get_action(model._ctx.ptr, _lib.vptr(N)))
+    return N.reshape(model.size, order="F")
+
+
+def set_(model, N=None):
+    """`set!(model; N)`: N is a number, an (Nx, Ny, Nk, Nphi) array, or a
+    function of (x, y, kx, ky) sampled at cell centres (ProductFields/set.jl:5-48)."""
+    if N is None:
+        return model
+    Nx, Ny, Nk, Np = model.size
+    if np.isscalar(N):
+        model._upload(np.full(model.size, N, dtype=model.dtype))
+    elif callable(N):
+        sg = model.spectral_grid
+        kx = sg.kappa[:, None] * np.cos(sg.phi)[None, :]
+        ky = sg.kappa[:, None] * np.sin(sg.phi)[None, :]
+        x = xnodes(model.grid)[:, None, None, None]
+        y = ynodes(model.grid)[None, :, None, None]
+        try:
+            vals = np.asarray(N(x, y, kx[None, None], ky[None, None]), dtype=np.float64)
+            vals = np.broadcast_to(vals, model.size)
+        except Exception:
+            vals = np.vectorize(N, otypes=[np.float64])(x, y, kx[None, None], ky[None, None])
+        model._upload(vals)
+    else:
+        model._upload(N)
+    return model
+
+
+def time_step_(model, dt, nsteps=1):
+    """`time_step!(model, dt)`, nsteps times, on the device."""
+    if not dt > 0:
+        raise _lib.RippleArgumentError(-1, "time step must be positive")
+    if model.sources is not None:
+        model.sources.before_step(model, dt)
+    _lib.check(model._ctx.lib.rpl_time_step(model._ctx.ptr, float(dt), int(nsteps)))
+    t = ctypes.c_double()
+    it = ctypes.c_int64()
+    _lib.check(model._ctx.lib.rpl_get_clock(model._ctx.ptr, ctypes.byref(t), ctypes.byref(it)))
+    model.clock.time, model.clock.iteration = t.value, it.value
+    return model
+
+
+class Simulation:
+    """`Simulation(model; Δt, stop_iteration, stop_time)` (Oceananigans)."""
+
+    def __init__(self, model, *, dt, stop_iteration=None, stop_time=None):
+        if not dt > 0:
+            raise ValueError("Δt must be positive")
+        self.model, self.dt = model, float(dt)
+        self.stop_iteration, self.stop_time = stop_iteration, stop_time
+
+
+def run_(sim):
+    """`run!(simulation)`: advance until stop_iteration / stop_time."""
+    m = sim.model
+    if sim.stop_iteration is not None:
+        n = int(sim.stop_iteration) - m.clock.iteration
+    elif sim.stop_time is not None:
+        n = int(math.ceil((sim.stop_time - m.clock.time) / sim.dt - 1e-9))
+    else:
+        raise ValueError("Simulation needs stop_iteration or stop_time")
+    if n > 0:
+        if m.sources is not None and getattr(m.sources, "per_step_host_work", False):
+            for _ in range(n):
+                time_step_(m, sim.dt)
+        else:
+            time_step_(m, sim.dt, n)
+    m.synchronize()
+    return sim

@@ -1,0 +1,110 @@
+"""Synthetic workloads: grids, currents and initial spectra of the benchmark
+configurations, built deterministically from a seed (Philox, a counter-based
+generator, so every consumer draws the same numbers).
+
+Each builder returns a Workload: the constructor arguments of
+SpectralWaveModel plus the initial action array and the run length.
+"""
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .grids import RectilinearGrid, PolarWaveVectorGrid, FrequencyDirectionGrid, xnodes, ynodes, PERIODIC, BOUNDED
+
+DEFAULT_SEED = 20261203
+
+
+@dataclass
+class Workload:
+    name: str
+    grid: RectilinearGrid
+    spectral_grid: object
+    velocities: object  # (u, v) arrays (Nx, Ny, Nz) or None
+    N0: np.ndarray      # (Nx, Ny, Nk, Nphi), Fortran order, float64
+    dt: float
+    nsteps: int
+    model_kwargs: dict = field(default_factory=dict)
+
+    @property
+    def cell_bins(self):
+        return int(np.prod(self.N0.shape))
+
+
+def narrow_band_action(grid, sg, kappa0=0.4, sigma_kappa=0.05, sigma_phi=0.30):
+    """exp(-((kappa - kappa0)/sigma_kappa)^2 - sin^2(phi/2)/sigma_phi^2), uniform
+    in (x, y), sampled at the cell centres from (kx, ky) as the reference's
+    set! passes them (examples/vortex_refraction.jl:107-111)."""
+    kx = sg.kappa[:, None] * np.cos(sg.phi)[None, :]
+    ky = sg.kappa[:, None] * np.sin(sg.phi)[None, :]
+    kap = np.hypot(kx, ky)
+    phi = np.arctan2(ky, kx)
+    col = np.exp(-((kap - kappa0) / sigma_kappa) ** 2 - np.sin(phi / 2) ** 2 / sigma_phi ** 2)
+    N = np.broadcast_to(col[None, None], (grid.Nx, grid.Ny) + col.shape)
+    return np.asfortranarray(N, dtype=np.float64)
+
+
+def vortex_refraction(float_type=np.float64, nx=64, ny=64, nz=16, nsteps=400):
+    """README.md:57-108 quick start: 64x64x16 periodic box of 80 x 80, z in
+    [-1, 0]; kappa = 4 bins 0.30-0.50, phi = 16 bins from 0; barotropic
+    Gaussian vortex u_theta = 0.4 (r/a) exp(0.5 - 0.5 (r/a)^2), a = 10; fused
+    refraction kernel, no sources, SSP-RK3, dt = 0.05."""
+    Lx = Ly = 80.0
+    grid = RectilinearGrid(float_type=float_type, size=(nx, ny, nz), x=(0, Lx), y=(0, Ly), z=(-1, 0),
+                           halo=(3, 3, 3), topology=(PERIODIC, PERIODIC, BOUNDED))
+    sg = PolarWaveVectorGrid(float_type, kappa=np.linspace(0.30, 0.50, 4),
+                             phi=np.linspace(0, 2 * math.pi, 17)[:16])
+    xc, yc, a, U0 = Lx / 2, Ly / 2, Lx / 8, 0.4
+    x = xnodes(grid)[:, None]
+    y = ynodes(grid)[None, :]
+    r = np.hypot(x - xc, y - yc)
+    ut = np.where(r == 0, 0.0, U0 * (r / a) * np.exp(0.5 - 0.5 * (r / a) ** 2))
+    rr = np.maximum(r, np.finfo(np.float64).eps)
+    u2 = -ut * (y - yc) / rr
+    v2 = ut * (x - xc) / rr
+    u = np.repeat(u2[:, :, None], nz, axis=2)
+    v = np.repeat(v2[:, :, None], nz, axis=2)
+    return Workload("vortex_refraction_64x64x16_k4_p16", grid, sg, (u, v), narrow_band_action(grid, sg),
+                    0.05, nsteps, dict(horizontal_advection=None, timestepper="RK3"))
+
+
+def eddy_streamfunction(nx, ny, Lx, Ly, peak_wavelength, rms_speed, seed):
+    """Random-phase Gaussian-spectrum streamfunction on the periodic box; returns
+    (u, v) = (-dpsi/dy, dpsi/dx) at the cell centres scaled to the rms speed."""
+    rng = np.random.Generator(np.random.Philox(seed))
+    kx = 2 * math.pi * np.fft.fftfreq(nx, d=Lx / nx)
+    ky = 2 * math.pi * np.fft.fftfreq(ny, d=Ly / ny)
+    KX, KY = np.meshgrid(kx, ky, indexing="ij")
+    K = np.hypot(KX, KY)
+    k0 = 2 * math.pi / peak_wavelength
+    amp = np.exp(-0.5 * ((K - k0) / (0.25 * k0)) ** 2)
+    amp[0, 0] = 0.0
+    phase = rng.uniform(0.0, 2 * math.pi, size=K.shape)
+    spec = amp * np.exp(1j * phase)
+    # Cell centres sit half a cell off the FFT nodes: shift by exp(i k dx/2).
+    shift = np.exp(1j * (KX * (Lx / nx) / 2 + KY * (Ly / ny) / 2))
+    u = np.real(np.fft.ifft2(-1j * KY * spec * shift))
+    v = np.real(np.fft.ifft2(1j * KX * spec * shift))
+    rms = math.sqrt(float(np.mean(u * u + v * v)))
+    return u * (rms_speed / rms), v * (rms_speed / rms)
+
+
+def sheared_eddies(float_type=np.float32, nx=512, ny=512, nz=32, nkappa=32, nphi=36, nsteps=200,
+                   seed=DEFAULT_SEED):
+    """512^2 x 32 periodic box of 80 x 80, depth 1 (z in [-1, 0]); kappa 32 bins
+    log-spaced 0.05-2.0, phi 36 bins; baroclinic eddy current (peak wavelength
+    20, rms 0.3) times exp(z/0.2) at the z centres; narrow-band spectrum at
+    kappa0 = 0.4; WENO5 transport through the fused refraction kernel; 200
+    steps of dt = 0.02, no sources.  Deep-water intrinsic c_g, Q depth 1."""
+    Lx = Ly = 80.0
+    grid = RectilinearGrid(float_type=float_type, size=(nx, ny, nz), x=(0, Lx), y=(0, Ly), z=(-1, 0),
+                           halo=(3, 3, 3), topology=(PERIODIC, PERIODIC, BOUNDED))
+    sg = PolarWaveVectorGrid(float_type, kappa=np.geomspace(0.05, 2.0, nkappa),
+                             phi=np.linspace(0, 2 * math.pi, nphi + 1)[:nphi])
+    u2, v2 = eddy_streamfunction(nx, ny, Lx, Ly, 20.0, 0.3, seed)
+    zc = 0.5 * (grid.z_faces[1:] + grid.z_faces[:-1])
+    decay = np.exp(zc / 0.2)
+    u = np.asfortranarray(u2[:, :, None] * decay[None, None, :])
+    v = np.asfortranarray(v2[:, :, None] * decay[None, None, :])
+    return Workload(f"sheared_eddies_{nx}x{ny}x{nz}_k{nkappa}_p{nphi}", grid, sg, (u, v),
+                    narrow_band_action(grid, sg), 0.02, nsteps, dict(timestepper="RK3"))

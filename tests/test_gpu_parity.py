@@ -1,0 +1,212 @@
+"""Device parity: the HIP path through the C ABI against the CPU oracle on the
+same inputs.  FP64 must match to <= 1e-12 relative L2 (the reference-order
+kernel is expected to be bit-identical); FP32 is held to the tolerances in
+each test and reported against FP64."""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+import ripple_amd as ra
+from ripple_amd import workloads, grids
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(f"[gpu-test] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
+
+
+def device_model(wl, float_type, kernel="auto", **extra):
+    g = wl.grid
+    grid = grids.RectilinearGrid(float_type=float_type, size=(g.Nx, g.Ny, g.Nz), x=g.x_faces, y=g.y_faces,
+                                 z=g.z_faces)
+    sg = wl.spectral_grid
+    sgrid = grids.PolarWaveVectorGrid(float_type, kappa=sg.kappa, phi=sg.phi, kappa_faces=sg.kappa_faces,
+                                      phi_faces=sg.phi_faces)
+    kw = dict(wl.model_kwargs)
+    kw.update(extra)
+    m = ra.SpectralWaveModel(grid, sgrid, velocities=wl.velocities, kernel=kernel, **kw)
+    ra.set_(m, N=wl.N0)
+    return m
+
+
+def random_workload(nx, ny, nk, nphi, seed=0, current=True, nz=6, timestepper="RK3"):
+    rng = np.random.default_rng(seed)
+    g = grids.RectilinearGrid(size=(nx, ny, nz), x=(0, 0.5 * nx), y=(0, 0.4 * ny), z=(-1, 0))
+    sg = grids.PolarWaveVectorGrid(kappa=np.geomspace(0.2, 1.5, nk), phi=np.linspace(0, 2 * math.pi, nphi + 1)[:nphi])
+    vel = None
+    if current:
+        x = grids.xnodes(g)[:, None, None]
+        y = grids.ynodes(g)[None, :, None]
+        z = grids.znodes(g)[None, None, :]
+        u = 0.3 * np.sin(2 * math.pi * y / (0.4 * ny) + 0.3) * np.exp(z / 0.5) + 0.05 * rng.normal(size=(nx, ny, nz))
+        v = 0.2 * np.cos(2 * math.pi * x / (0.5 * nx)) * np.exp(z / 0.5) + 0.05 * rng.normal(size=(nx, ny, nz))
+        vel = (np.asfortranarray(u), np.asfortranarray(v))
+    N0 = np.asfortranarray(0.2 + rng.random((nx, ny, nk, nphi)))
+    return workloads.Workload("random", g, sg, vel, N0, 0.01, 1, dict(timestepper=timestepper))
+
+
+def test_doppler_velocity_fp64_matches_oracle(oracle):
+    say("doppler fp64 vortex 64x64x16")
+    wl = workloads.vortex_refraction()
+    m = device_model(wl, np.float64)
+    om = oracle.OracleModel(wl)
+    Ux, Uy = m.doppler_velocity()
+    np.testing.assert_array_equal(Ux.ravel(order="F"), om.Ux)
+    np.testing.assert_array_equal(Uy.ravel(order="F"), om.Uy)
+
+
+def test_tendency_fp64_reference_kernel_bitwise(oracle):
+    say("tendency fp64 random 19x13x7x6")
+    wl = random_workload(19, 13, 7, 6, seed=1)
+    m = device_model(wl, np.float64)
+    om = oracle.OracleModel(wl)
+    G_dev, G_orc = m.tendency(), om.tendency()
+    assert rel_l2(G_dev, G_orc) <= 1e-12
+    np.testing.assert_array_equal(G_dev, G_orc)
+
+
+def test_vortex_refraction_fp64_400_steps(oracle):
+    """Config 1: the whole 400-step run in FP64, relative L2 of N <= 1e-12."""
+    say("vortex fp64 400 steps")
+    wl = workloads.vortex_refraction()
+    m = device_model(wl, np.float64)
+    ra.run_(ra.Simulation(m, dt=wl.dt, stop_iteration=wl.nsteps))
+    om = oracle.OracleModel(wl)
+    om.step(wl.dt, wl.nsteps)
+    err = rel_l2(m.action, om.action)
+    say(f"vortex fp64 rel-L2 = {err:.3e}, max abs = {np.max(np.abs(m.action - om.action)):.3e}")
+    assert err <= 1e-12
+    assert m.clock.iteration == 400 and m.clock.time == pytest.approx(20.0)
+    m0_d, kr_d, md_d = ra.m0(m), ra.root_mean_square_wavenumber(m), ra.mean_direction(m)
+    m0_o, kr_o, md_o = om.diagnostics()
+    assert rel_l2(m0_d, m0_o) <= 1e-13 and rel_l2(kr_d, kr_o) <= 1e-13
+    assert np.max(np.abs(md_d - md_o)) <= 1e-12
+
+
+@pytest.mark.parametrize("kernel", ["fast", "reference"])
+def test_vortex_refraction_fp32(oracle, kernel):
+    """Config 1 in FP32: one tendency and one step against the FP32 oracle, then
+    the full run's error against the FP64 oracle (reported, bounded)."""
+    say(f"vortex fp32 kernel={kernel}")
+    wl = workloads.vortex_refraction()
+    m = device_model(wl, np.float32, kernel=kernel)
+    o32 = oracle.OracleModel(wl, dtype=np.float32)
+    G_dev, G_orc = m.tendency(), o32.tendency()
+    scale = np.max(np.abs(G_orc))
+    assert np.max(np.abs(G_dev - G_orc)) <= 2e-4 * max(scale, 1.0)
+    ra.time_step_(m, wl.dt)
+    o32.step(wl.dt)
+    assert np.max(np.abs(m.action - o32.action)) <= 2e-5
+    ra.run_(ra.Simulation(m, dt=wl.dt, stop_iteration=wl.nsteps))
+    o64 = oracle.OracleModel(wl)
+    o64.step(wl.dt, wl.nsteps)
+    err = rel_l2(m.action, o64.action)
+    m0_err = rel_l2(ra.m0(m), o64.diagnostics()[0])
+    say(f"vortex fp32 ({kernel}) vs fp64 oracle: N rel-L2 {err:.3e}, m0 rel-L2 {m0_err:.3e}")
+    assert err < 2e-3 and m0_err < 1e-3
+
+
+@pytest.mark.parametrize("nk", [1, 2, 3, 4, 5, 6, 7, 9])
+def test_kappa_boundary_cases_fp64(oracle, nk):
+    """First-order kappa fallback and zero outer flux for every small Nk."""
+    say(f"kappa edge nk={nk}")
+    wl = random_workload(11, 9, nk, 5, seed=nk)
+    m = device_model(wl, np.float64)
+    om = oracle.OracleModel(wl)
+    np.testing.assert_array_equal(m.tendency(), om.tendency())
+    ra.time_step_(m, 0.02, 3)
+    om.step(0.02, 3)
+    assert rel_l2(m.action, om.action) <= 1e-12
+
+
+@pytest.mark.parametrize("shape", [(3, 3, 4, 3), (67, 5, 6, 7), (130, 66, 5, 36)])
+def test_ragged_and_minimal_shapes(oracle, shape):
+    say(f"shape {shape}")
+    wl = random_workload(*shape, seed=7)
+    for dt, kernel, tol in ((np.float64, "reference", 1e-12), (np.float32, "fast", 5e-5)):
+        m = device_model(wl, dt, kernel=kernel)
+        om = oracle.OracleModel(wl, dtype=dt)
+        ra.time_step_(m, 0.01, 2)
+        om.step(0.01, 2)
+        assert rel_l2(m.action, om.action) <= tol
+
+
+@pytest.mark.parametrize("stepper", ["RK3", "LSRK3", "ForwardEuler"])
+def test_intrinsic_transport_parity(oracle, stepper):
+    say(f"intrinsic {stepper}")
+    wl = random_workload(40, 24, 5, 8, seed=3, current=False, timestepper=stepper)
+    m = device_model(wl, np.float64)
+    om = oracle.OracleModel(wl)
+    assert om.mode == oracle.MODE_INTRINSIC
+    ra.time_step_(m, 0.01, 5)
+    om.step(0.01, 5)
+    np.testing.assert_array_equal(m.action, om.action)
+    m32 = device_model(wl, np.float32)
+    o32 = oracle.OracleModel(wl, dtype=np.float32)
+    ra.time_step_(m32, 0.01, 5)
+    o32.step(0.01, 5)
+    assert rel_l2(m32.action, o32.action) <= 5e-6
+
+
+def test_conservation_sums(oracle):
+    say("conservation sums")
+    wl = random_workload(50, 30, 6, 8, seed=5, current=False)
+    m = device_model(wl, np.float64)
+    om = oracle.OracleModel(wl)
+    ta = ra.total_action(m)
+    assert ta == pytest.approx(om.total(om.tables["w"]), rel=1e-13)
+    # Intrinsic periodic transport telescopes: drift <= 1e-10 over 20 RK3 steps.
+    ra.time_step_(m, 0.01, 20)
+    assert abs(ra.total_action(m) - ta) <= 1e-10 * abs(ta)
+    energy = ra.total_deep_water_energy(m)
+    assert energy > 0
+
+
+def test_error_paths():
+    say("error paths")
+    wl = random_workload(8, 8, 3, 4, seed=1)
+    m = device_model(wl, np.float64)
+    with pytest.raises(ra.RippleArgumentError):
+        ra.time_step_(m, 0.0)
+    with pytest.raises(ValueError):
+        ra.set_(m, N=np.zeros((8, 8, 3, 5)))
+    # NaN passes through max(0, .) as in Julia.
+    N = wl.N0.copy()
+    N[2, 3, 1, 0] = np.nan
+    ra.set_(m, N=N)
+    ra.time_step_(m, 0.01)
+    assert np.isnan(m.action).any()
+
+
+def test_full_size_sheared_eddies_properties():
+    """Config 2 at its full 512^2 x 32 x 36 FP32 size: a uniform spectrum under
+    a uniform current has zero tendency everywhere; the eddy run stays finite
+    and non-negative and its Doppler caches match the contraction sampled on
+    the host."""
+    say("sheared eddies full size fp32")
+    wl = workloads.sheared_eddies()
+    m = device_model(wl, np.float32)
+    Ux, Uy = m.doppler_velocity()
+    rng = np.random.default_rng(0)
+    from oracle import q_weights
+    W = q_weights(wl.grid.z_faces, wl.spectral_grid.kappa, 1.0)
+    for _ in range(20):
+        i, j, k = (int(rng.integers(s)) for s in Ux.shape)
+        exact = float(np.dot(wl.velocities[0][i, j, :], W[:, k]))
+        assert Ux[i, j, k] == pytest.approx(exact, rel=2e-6, abs=2e-7)
+    ra.time_step_(m, wl.dt, 3)
+    N = m.action
+    assert np.all(np.isfinite(N)) and np.all(N >= 0)
+    m.set_doppler_velocity(np.full(Ux.shape, 0.1), np.full(Ux.shape, -0.2))
+    ra.set_(m, N=0.5)
+    assert np.all(m.tendency() == 0.0)

@@ -1,0 +1,292 @@
+"""Pins the CPU oracle against the reference's own known answers and against
+closed forms that a transcription slip would break (CPU only)."""
+import ctypes
+import math
+
+import numpy as np
+import pytest
+from scipy import integrate
+
+from ripple_amd import grids, workloads
+
+
+# --- spectral geometry: test/coordinate_grids/finite_volume_integration.jl:112-141 ---
+
+def test_polar_measures_known_answers(oracle):
+    kc = np.linspace(0.25, 1.0, 4)
+    pc = np.linspace(0, 2 * math.pi, 9)[:8]
+    kf = oracle.centers_to_faces(kc, lower_bounded=True, lower=0.0)
+    pf = oracle.phi_faces(pc)
+    t = oracle.geometry(kc, kf, pc, pf)
+    area = math.pi * (kf[-1] ** 2 - kf[0] ** 2)
+    assert t["w"].sum() == pytest.approx(area, rel=1e-14)
+    assert abs(t["mu_x"].sum()) < 1e-14 and abs(t["mu_y"].sum()) < 1e-14
+    assert t["trace"].sum() == pytest.approx(math.pi / 2 * (kf[-1] ** 4 - kf[0] ** 4), abs=1e-14)
+    # centers_to_faces: midpoints with mirrored ends (spectral_grids.jl:4-21).
+    np.testing.assert_allclose(kf, [0.125, 0.375, 0.625, 0.875, 1.125], rtol=0, atol=1e-15)
+    np.testing.assert_allclose(np.diff(pf), 2 * math.pi / 8, rtol=1e-14)
+    assert pf[0] == pytest.approx(-math.pi / 8)
+
+
+def test_lower_face_clamped(oracle):
+    kf = oracle.centers_to_faces([1e-4, 0.8], lower_bounded=True, lower=0.0)
+    assert kf[0] == 0.0 and np.all(np.diff(kf) > 0)
+    g = grids.PolarWaveVectorGrid(kappa=[1e-4, 0.8], phi=[0.0])
+    assert g.kappa_faces[0] == 0.0 and np.all(g.weights > 0)
+    np.testing.assert_array_equal(g.kappa_faces, kf)
+
+
+def test_measures_against_quadrature(oracle):
+    """Exact cell integrals of 1, kx, ky and kx^2+ky^2 over annular sectors."""
+    kc, pc = np.array([0.3, 0.5, 0.9]), np.array([0.2, 1.4, 2.9])
+    kf, pf = np.array([0.2, 0.4, 0.7, 1.0]), np.array([-0.4, 0.8, 2.0, 3.5])
+    t = oracle.geometry(kc, kf, pc, pf)
+    for m in range(3):
+        for n in range(3):
+            def q(f):
+                return integrate.dblquad(lambda k, p: f(k, p) * k, pf[n], pf[n + 1], kf[m], kf[m + 1],
+                                         epsabs=1e-13, epsrel=1e-13)[0]
+            assert t["w"][m, n] == pytest.approx(q(lambda k, p: 1.0), rel=1e-12)
+            assert t["mu_x"][m, n] == pytest.approx(q(lambda k, p: k * math.cos(p)), rel=1e-10, abs=1e-13)
+            assert t["mu_y"][m, n] == pytest.approx(q(lambda k, p: k * math.sin(p)), rel=1e-10, abs=1e-13)
+            assert t["trace"][m, n] == pytest.approx(q(lambda k, p: k * k), rel=1e-12)
+            # Deep-water c_g averaged over the cell: sqrt(g)/2 k^{-1/2} (cos, sin) (transport.jl:61-70).
+            cgx = q(lambda k, p: 0.5 * math.sqrt(9.81 / k) * math.cos(p)) / t["w"][m, n]
+            cgy = q(lambda k, p: 0.5 * math.sqrt(9.81 / k) * math.sin(p)) / t["w"][m, n]
+            assert t["cgx"][m, n] == pytest.approx(cgx, rel=1e-10, abs=1e-12)
+            assert t["cgy"][m, n] == pytest.approx(cgy, rel=1e-10, abs=1e-12)
+
+
+def test_finite_depth_group_velocity_limits(oracle):
+    """Finite-depth c_g -> deep-water sqrt(g/k)/2 for kd >> 1 and sqrt(g d) for kd << 1."""
+    kc = np.array([1e-3, 5.0])
+    kf = np.array([0.5e-3, 2e-3, 10.0])
+    pc, pf = np.array([0.0]), np.array([-0.1, 0.1])
+    t = oracle.geometry(kc, kf, pc, pf, finite_depth=True, depth=10.0)
+    assert t["cgx"][1, 0] == pytest.approx(0.5 * math.sqrt(9.81 / 5.0), rel=1e-12)
+    assert t["cgx"][0, 0] == pytest.approx(math.sqrt(9.81 * 10.0), rel=1e-4)
+    assert t["cgy"][0, 0] == 0.0
+
+
+# --- Q-transform: test/coupling/q_transform.jl:21-80 ---
+
+Z_FACES = [-1.0, -0.73, -0.31, -0.08, 0.0]
+
+
+def test_q_weights_known_answers(oracle):
+    for kappa in (1e-8, 0.5, 5.0):
+        W = oracle.q_weights(Z_FACES, [kappa], 1.0)
+        assert W.sum() == pytest.approx(1.0, abs=1e-14)
+    cell = lambda z1, z2: oracle.q_cdf(0.7, max(z1, z2), 1.0) - oracle.q_cdf(0.7, min(z1, z2), 1.0)
+    assert cell(-10.0, 10.0) == pytest.approx(1.0, abs=1e-14)
+    assert cell(10.0, -10.0) == pytest.approx(1.0, abs=1e-14)
+    assert cell(-2.0, -1.0) == pytest.approx(0.0, abs=1e-14)
+    assert cell(0.0, 2.0) == pytest.approx(0.0, abs=1e-14)
+
+
+def test_q_weights_against_quadrature(oracle):
+    """W[k,m] is the integral of Q(z) = 2 kappa cosh(2 kappa (z+d)) / sinh(2 kappa d) over the cell."""
+    for kappa, d in ((0.2, 1.0), (3.0, 1.0), (0.05, 200.0), (40.0, 1.0)):
+        zf = np.linspace(-d, 0, 9)
+        W = oracle.q_weights(zf, [kappa], d)[:, 0]
+        for k in range(8):
+            dens = lambda z: 2 * kappa * math.exp(2 * kappa * (z + d) - 2 * kappa * d) * \
+                (1 + math.exp(-4 * kappa * (z + d))) / (1 - math.exp(-4 * kappa * d))
+            ref = integrate.quad(dens, zf[k], zf[k + 1], epsabs=1e-15, epsrel=1e-13)[0]
+            assert W[k] == pytest.approx(ref, rel=1e-10, abs=1e-15)
+
+
+def test_doppler_contraction_known_answer(oracle):
+    u = np.array([0.1, -0.4, 0.8, 1.6]).reshape(1, 1, 4)
+    v = np.array([1.2, 0.7, -0.5, 0.3]).reshape(1, 1, 4)
+    W = oracle.q_weights(Z_FACES, [0.2, 1.0], 1.0)
+    Ux, Uy = oracle.doppler(u, v, W)
+    expected_Ux = sum(u[0, 0, k] * W[k, 0] for k in range(4))
+    expected_Uy = sum(v[0, 0, k] * W[k, 1] for k in range(4))
+    assert Ux[0, 0, 0] == pytest.approx(expected_Ux, abs=1e-14)
+    assert Uy[0, 0, 1] == pytest.approx(expected_Uy, abs=1e-14)
+    # Barotropic current: U = u for every kappa (q_transform.jl:126-143).
+    zf = np.linspace(-1, 0, 33)
+    W = oracle.q_weights(zf, [0.5, 1.0], 1.0)
+    Ux, Uy = oracle.doppler(np.ones((2, 2, 32)), 2 * np.ones((2, 2, 32)), W)
+    np.testing.assert_allclose(Ux, 1.0, atol=1e-12)
+    np.testing.assert_allclose(Uy, 2.0, atol=1e-12)
+
+
+# --- WENO5 reconstruction ---
+
+def _cell_avg_poly(coef, i):
+    """Average of sum c_p x^p over [i-1/2, i+1/2]."""
+    a, b = i - 0.5, i + 0.5
+    return sum(c * (b ** (p + 1) - a ** (p + 1)) / (p + 1) for p, c in enumerate(coef))
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-12), (np.float32, 2e-5)])
+def test_weno5_exact_on_quadratic_cell_averages(oracle, dtype, tol):
+    """Each 3-point candidate is exact for quadratics, so any convex weighting is."""
+    rng = np.random.default_rng(1)
+    for _ in range(200):
+        coef = rng.normal(size=3)
+        f = [_cell_avg_poly(coef, i) for i in (-2, -1, 0, 1, 2)]
+        exact = sum(c * 0.5 ** p for p, c in enumerate(coef))
+        got = oracle.weno5(f, dtype)
+        assert got == pytest.approx(exact, abs=tol * (1 + abs(exact) + np.abs(coef).sum()))
+
+
+def test_weno5_fifth_order_on_smooth_data(oracle):
+    """Halving the spacing of smooth data divides the face error by ~2^5."""
+    errs = []
+    for h in (0.1, 0.05, 0.025):
+        x = np.array([-2, -1, 0, 1, 2]) * h
+        f = [(math.cos(xi - h / 2) - math.cos(xi + h / 2)) / h for xi in x]  # averages of sin
+        errs.append(abs(oracle.weno5(f) - math.sin(h / 2)))
+    assert errs[0] / errs[1] > 20 and errs[1] / errs[2] > 20
+
+
+def test_weno5_constant_and_mirror(oracle):
+    assert oracle.weno5([3.0] * 5) == pytest.approx(3.0, rel=4e-16)
+    f = [0.3, 1.7, 0.2, 0.9, 2.5]
+    assert oracle.weno5(f) != oracle.weno5(f[::-1])
+    # Non-oscillatory: a step is reconstructed from the smooth side.
+    step = oracle.weno5([0.0, 0.0, 0.0, 1.0, 1.0])
+    assert 0.0 <= step < 0.01
+
+
+# --- fused tendencies and RK3 ---
+
+def _tiny_workload(nx=8, ny=7, nk=6, nphi=5, seed=3, current=True):
+    rng = np.random.default_rng(seed)
+    g = grids.RectilinearGrid(size=(nx, ny, 4), x=(0, 8.0), y=(0, 7.0), z=(-1, 0))
+    sg = grids.PolarWaveVectorGrid(kappa=np.geomspace(0.2, 1.0, nk), phi=np.linspace(0, 2 * math.pi, nphi + 1)[:nphi])
+    vel = (rng.normal(size=(nx, ny, 4)) * 0.3, rng.normal(size=(nx, ny, 4)) * 0.3) if current else None
+    N0 = np.asfortranarray(rng.random((nx, ny, nk, nphi)))
+    return workloads.Workload("tiny", g, sg, vel, N0, 0.01, 1, dict(timestepper="RK3"))
+
+
+def test_uniform_action_has_zero_tendency(oracle):
+    wl = _tiny_workload(current=False)
+    wl.N0[...] = 0.7
+    om = oracle.OracleModel(wl, doppler_override=(np.full((8, 7, 6), 0.2), np.full((8, 7, 6), -0.1)),
+                            transport=oracle.MODE_REFRACTION)
+    assert np.all(om.tendency() == 0.0)
+
+
+def test_refraction_tendency_matches_direct_transcription(oracle):
+    """The fused tendency equals -sum of flux differences assembled here from
+    the oracle's own reconstruction, with the reference's stencil indexing
+    (wave_current_refraction.jl:94-149) written independently in Python."""
+    wl = _tiny_workload()
+    om = oracle.OracleModel(wl)
+    G = om.tendency()
+    N = wl.N0
+    nx, ny, nk, nphi = N.shape
+    Ux, Uy = (a.reshape((nx, ny, nk), order="F") for a in (om.Ux, om.Uy))
+    cgx, cgy = om.tables["cgx"], om.tables["cgy"]
+    dxi, dyi, dki, dpi = (1 / om.dx, 1 / om.dy, 1 / om.dk, 1 / om.dp)
+
+    def face(vals, vel, has=True):  # vals = f[-2..3] around the face
+        if not has:
+            return vals[2] if vel >= 0 else vals[3]
+        return oracle.weno5(vals[0:5]) if vel >= 0 else oracle.weno5(vals[5:0:-1])
+
+    rng = np.random.default_rng(0)
+    for _ in range(40):
+        i, j, m, n = (int(rng.integers(s)) for s in N.shape)
+        gx = lambda a, di, dj: a[(i + di) % nx, (j + dj) % ny, m]
+        Uxx = (gx(Ux, 1, 0) - gx(Ux, -1, 0)) * dxi / 2
+        Uxy = (gx(Ux, 0, 1) - gx(Ux, 0, -1)) * dyi / 2
+        Uyx = (gx(Uy, 1, 0) - gx(Uy, -1, 0)) * dxi / 2
+        Uyy = (gx(Uy, 0, 1) - gx(Uy, 0, -1)) * dyi / 2
+        c, s, kap = math.cos(wl.spectral_grid.phi[n]), math.sin(wl.spectral_grid.phi[n]), wl.spectral_grid.kappa[m]
+        ck = -kap * (c * c * Uxx + c * s * (Uyx + Uxy) + s * s * Uyy)
+        cp = c * s * (Uxx - Uyy) + s * s * Uyx - c * c * Uxy
+        ux, uy = cgx[m, n] + Ux[i, j, m], cgy[m, n] + Uy[i, j, m]
+        xs = [N[(i + d) % nx, j, m, n] for d in range(-3, 4)]
+        ys = [N[i, (j + d) % ny, m, n] for d in range(-3, 4)]
+        ps = [N[i, j, m, (n + d) % nphi] for d in range(-3, 4)]
+        ks = [N[i, j, min(max(m + d, 0), nk - 1), n] for d in range(-3, 4)]
+        fx = ux * (face(xs[1:], ux) - face(xs[:-1], ux)) * dxi
+        fy = uy * (face(ys[1:], uy) - face(ys[:-1], uy)) * dyi
+        fkp = 0.0 if m == nk - 1 else ck * face(ks[1:], ck, m >= 2 and m + 3 <= nk - 1)
+        fkm = 0.0 if m == 0 else ck * face(ks[:-1], ck, m >= 3 and m + 2 <= nk - 1)
+        fp = cp * (face(ps[1:], cp) - face(ps[:-1], cp)) * dpi
+        expected = -(fx + fy + (fkp - fkm) * dki + fp)
+        assert G[i, j, m, n] == pytest.approx(expected, rel=1e-13, abs=1e-13)
+
+
+def test_intrinsic_sinusoid_matches_analytic_derivative(oracle):
+    """test/integration/transport.jl:53-64: N = 1 + 0.1 sin(2 pi x / L) on 64
+    cells of a 64 m box; G = -u dN/dx within 1e-7."""
+    g = grids.RectilinearGrid(size=(64, 4, 1), x=(0, 64.0), y=(0, 4.0), z=(-1, 0))
+    sg = grids.PolarWaveVectorGrid(kappa=[1.0, 1.2], phi=[0.0, 2.0, 4.0], phi_faces=[-0.1, 0.1, 2.1, 4.1])
+    x = grids.xnodes(g)
+    N0 = np.zeros((64, 4, 2, 3))
+    N0[...] = (1 + 0.1 * np.sin(2 * math.pi * x / 64.0))[:, None, None, None]
+    wl = workloads.Workload("sin", g, sg, None, np.asfortranarray(N0), 0.1, 1,
+                            dict(timestepper="ForwardEuler"))
+    om = oracle.OracleModel(wl)
+    G = om.tendency()
+    cgx = om.tables["cgx"]
+    for m in range(2):
+        u = cgx[m, 0]
+        expected = -u * 0.1 * 2 * math.pi / 64.0 * np.cos(2 * math.pi * x / 64.0)
+        assert np.max(np.abs(G[:, 0, m, 0] - expected)) < 1e-7
+
+
+def test_intrinsic_transport_conserves_action(oracle):
+    """Periodic transport at a per-bin constant velocity telescopes: the total
+    action is conserved to rounding (transport.jl:94-97 asks 1e-10)."""
+    wl = _tiny_workload(nx=16, ny=12, current=False)
+    wl.N0[...] = 1.0 + 0.3 * wl.N0
+    wl.model_kwargs = dict(timestepper="RK3")
+    om = oracle.OracleModel(wl)
+    w = om.tables["w"]
+    before = om.total(w)
+    om.step(0.05, 10)
+    assert om.total(w) == pytest.approx(before, rel=1e-13)
+
+
+HOOK64 = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p)
+
+
+def test_rk3_linear_decay_amplification(oracle):
+    """dN/dt = -lambda N: one SSP-RK3 step multiplies by 1 - z + z^2/2 - z^3/6
+    (test/integration/model_api.jl:325-331 uses BottomFriction(rate=1), dt=0.1)."""
+    wl = _tiny_workload(current=False)
+    wl.N0[...] = 1.0
+    cnt = wl.N0.size
+
+    def decay(Np, Gp, user):
+        N = np.ctypeslib.as_array(ctypes.cast(Np, ctypes.POINTER(ctypes.c_double)), shape=(cnt,))
+        G = np.ctypeslib.as_array(ctypes.cast(Gp, ctypes.POINTER(ctypes.c_double)), shape=(cnt,))
+        G += -1.0 * N
+
+    hook = HOOK64(decay)
+    for stepper, stages in (("RK3", 3), ("LSRK3", 3), ("ForwardEuler", 1)):
+        om = oracle.OracleModel(wl, transport=oracle.MODE_NONE, timestepper=stepper)
+        om.step(0.1, 1, hook, None)
+        z = 0.1
+        amp = 1 - z + z * z / 2 - z ** 3 / 6 if stages == 3 else 1 - z
+        np.testing.assert_allclose(om.action, amp, rtol=1e-15)
+
+
+def test_diagnostics_uniform_action(oracle):
+    """test/diagnostics/moments.jl:1-24: uniform N on kappa = [1], 8 directions."""
+    g = grids.RectilinearGrid(size=(3, 3, 1), x=(0, 1), y=(0, 1), z=(0, 1))
+    sg = grids.PolarWaveVectorGrid(kappa=[1.0], phi=np.linspace(0, 2 * math.pi, 9)[:8])
+    wl = workloads.Workload("u", g, sg, None, np.ones((3, 3, 1, 8), order="F"), 0.1, 1, {})
+    om = oracle.OracleModel(wl)
+    m0, krms, mdir = om.diagnostics()
+    kf = sg.kappa_faces
+    ms = (math.pi / 2 * (kf[-1] ** 4 - kf[0] ** 4)) / (math.pi * (kf[-1] ** 2 - kf[0] ** 2))
+    np.testing.assert_allclose(krms, math.sqrt(ms), rtol=1e-14)
+    np.testing.assert_allclose(m0, math.pi * (kf[-1] ** 2 - kf[0] ** 2), rtol=1e-14)
+    # Single-direction action points along that direction.
+    N = np.zeros((3, 3, 1, 8), order="F")
+    N[:, :, 0, 2] = 1.0
+    wl.N0 = N
+    om = oracle.OracleModel(wl)
+    np.testing.assert_allclose(om.diagnostics()[2], sg.phi[2], atol=1e-14)
+    # Zero action: kappa_rms falls back to 0 (diagnostic_operations.jl:121).
+    wl.N0 = np.zeros_like(N)
+    assert np.all(oracle.OracleModel(wl).diagnostics()[1] == 0.0)
