@@ -244,6 +244,8 @@ int rpl_create(rpl_ctx **out, const rpl_config *cfg)
         return fail(RPL_ERR_UNSUPPORTED, "the FAST stencil is FP32 only");
     if ((long long)k.nkappa * k.nphi > 65535)
         return fail(RPL_ERR_UNSUPPORTED, "nkappa * nphi must be <= 65535");
+    if ((long long)k.nx * k.ny >= (1LL << 30))
+        return fail(RPL_ERR_UNSUPPORTED, "nx * ny must be < 2^30 (32-bit in-slab byte offsets)");
 
     rpl_ctx *c = new rpl_ctx();
     c->cfg = k;

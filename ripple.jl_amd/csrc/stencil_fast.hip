@@ -4,13 +4,13 @@
 // Models/intrinsic_transport.jl:14-68 (WENO5-JS with the reference's FP32
 // epsilon 1e4*eps(Float32), cell-centre velocity on both faces, first-order
 // kappa fallback, zero outer kappa flux), with the arithmetic reorganised:
-//   * only the upwind reconstruction of each face is evaluated;
-//   * the reconstruction is written as v1 + w0 (v0 - v1) + w2 (v2 - v1), whose
-//     two corrections are third differences of the data;
-//   * the nonlinear weights are formed from the ratios q1/q0 and q1/q2
-//     (q_k = eps + beta_k), so there is one reciprocal per ratio and one for
-//     the final normalisation instead of four IEEE divides, and nothing
-//     overflows for any finite action;
+//   * only the upwind reconstruction of each face is evaluated, and the
+//     upwind order is chosen per lane by selecting the stencil values, so the
+//     wavefront never diverges on the velocity sign;
+//   * the two faces of a cell share their second differences and smoothness
+//     indicators (4 triplets instead of 6);
+//   * each reconstruction is v1 + weighted third differences with one
+//     reciprocal instead of four IEEE divides;
 //   * the current gradients are centred differences of U formed in-kernel
 //     (wave_current_refraction.jl:7-19) instead of four extra arrays.
 // Results differ from the reference-order kernel by rounding only.
@@ -21,145 +21,277 @@ namespace {
 
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
-// Upwind-left WENO5 value at face i+1/2 from (f[i-2] .. f[i+2]).
-__device__ __forceinline__ float weno5_left(float fm2, float fm1, float f0, float fp1, float fp2)
+// The two upwind-left WENO5 face values of one cell along one direction.
+//
+// The six values w0..w5 are the cell's neighbours ordered upwind-first (for a
+// non-negative velocity w_k = f[c-3+k], otherwise w_k = f[c+3-k]), so that
+// A = WENO5(w0..w4) is the upwind value at the cell's upstream face and
+// B = WENO5(w1..w5) the value at its downstream face, and the reference's
+// vel * (N_plus - N_minus) equals |vel| * (B - A) for either sign
+// (wave_current_refraction.jl:57-64, 111-113).  The two reconstructions share
+// their second differences and smoothness indicators.
+//
+// Each reconstruction is v1 + (A0 (v0 - v1) + A2 (v2 - v1)) / (A0 + A1 + A2),
+// A_k = d_k / (eps + beta_k)^2 multiplied through by the three squared
+// denominators, so one reciprocal replaces the reference's four divides.  The
+// smoothness indicators are scaled by 4 together with epsilon, which leaves
+// the weights unchanged, and every value is carried times 6 (the 1/6 is
+// applied once to the flux).  If the product form overflows (differences of
+// the action above ~1e4) the lane falls back to the ratio form.
+//
+// weno_face6 returns 6 * (value - v1) for the face with smoothness q0, q1, q2
+// and third differences U0 = D_l - D_c, U1 = D_c - D_r.
+__device__ __forceinline__ float weno_face6(float q0, float q1, float q2, float U0, float U1)
 {
-    // Smoothness indicators scaled by 4 (and epsilon with them): the weights
-    // depend only on ratios, so the common factor cancels.
-    const float eps4 = 4.0f * (10000.0f * 1.1920928955078125e-7f);
-    const float c13_3 = 13.0f / 3.0f;
-    const float d0 = (fm2 + f0) - 2.0f * fm1;
-    const float d1 = (fm1 + fp1) - 2.0f * f0;
-    const float d2 = (f0 + fp2) - 2.0f * fp1;
-    const float e0 = d0 + 2.0f * (f0 - fm1);
-    const float e1 = fm1 - fp1;
-    const float e2 = d2 + 2.0f * (f0 - fp1);
-    const float q0 = fmaf(c13_3 * d0, d0, fmaf(e0, e0, eps4));
-    const float q1 = fmaf(c13_3 * d1, d1, fmaf(e1, e1, eps4));
-    const float q2 = fmaf(c13_3 * d2, d2, fmaf(e2, e2, eps4));
-    const float r0 = q1 * frcp(q0), r2 = q1 * frcp(q2);
-    const float s0 = r0 * r0, s2 = r2 * r2;
-    // alpha0/alpha1 = s0/6, alpha2/alpha1 = s2/2;  v0-v1 = T0/3, v2-v1 = T1/6.
-    const float T0 = d0 - d1;  // fm2 - 3 fm1 + 3 f0 - fp1
-    const float T1 = d1 - d2;  // fm1 - 3 f0 + 3 fp1 - fp2
-    const float v1 = (fmaf(5.0f, f0, 2.0f * fp1) - fm1) * (1.0f / 6.0f);
-    const float num = fmaf(2.0f * s0, T0, 3.0f * s2 * T1);
-    const float den = fmaf(6.0f, s0, fmaf(18.0f, s2, 36.0f));
-    return fmaf(num, frcp(den), v1);
+    const float s0 = q0 * q0, s1 = q1 * q1, s2 = q2 * q2;
+    // 6 (v0 - v1) = 2 U0, 6 (v2 - v1) = U1; weights 1 : 6 : 3 over s0, s1, s2.
+    float num = s1 * fmaf(s2 * 2.0f, U0, 3.0f * s0 * U1);
+    float den = fmaf(s2, fmaf(6.0f, s0, s1), 3.0f * s0 * s1);
+    if (!(den <= 3.0e38f)) {
+        const float r0 = q1 * frcp(q0), r2 = q1 * frcp(q2);
+        const float t0 = r0 * r0, t2 = r2 * r2;
+        num = fmaf(2.0f * t0, U0, 3.0f * t2 * U1);
+        den = fmaf(6.0f, t0, fmaf(18.0f, t2, 36.0f)) * (1.0f / 6.0f);
+    }
+    return num * frcp(den);
 }
 
-__device__ __forceinline__ float face(float im2, float im1, float i0, float ip1, float ip2, float ip3, float vel)
+// 6 * (B - A) for the window w0..w5, and optionally A and B themselves.
+template <bool VALUES>
+__device__ __forceinline__ float weno_pair6(float w0, float w1, float w2, float w3, float w4, float w5, float *A,
+                                           float *B)
 {
-    return vel >= 0.0f ? weno5_left(im2, im1, i0, ip1, ip2) : weno5_left(ip3, ip2, ip1, i0, im1);
+    const float eps4 = 4.0f * (10000.0f * 1.1920928955078125e-7f);  // 4 * 1e4 eps(Float32)
+    const float c13_3 = 13.0f / 3.0f;
+    const float g0 = w1 - w0, g1 = w2 - w1, g2 = w3 - w2, g3 = w4 - w3, g4 = w5 - w4;
+    const float D1 = g1 - g0, D2 = g2 - g1, D3 = g3 - g2, D4 = g4 - g3;
+    const float t1 = fmaf(c13_3 * D1, D1, eps4), t2 = fmaf(c13_3 * D2, D2, eps4);
+    const float t3 = fmaf(c13_3 * D3, D3, eps4), t4 = fmaf(c13_3 * D4, D4, eps4);
+    // beta0 of a face uses the right-edge form on its first triplet, beta1 the
+    // centred form, beta2 the left-edge form on its last triplet.
+    const float eA1 = fmaf(2.0f, g1, D1), eA2 = fmaf(2.0f, g2, D2);
+    const float eC2 = g1 + g2, eC3 = g2 + g3;
+    const float eB3 = fmaf(-2.0f, g2, D3), eB4 = fmaf(-2.0f, g3, D4);
+    const float qA1 = fmaf(eA1, eA1, t1), qA2 = fmaf(eA2, eA2, t2);
+    const float qC2 = fmaf(eC2, eC2, t2), qC3 = fmaf(eC3, eC3, t3);
+    const float qB3 = fmaf(eB3, eB3, t3), qB4 = fmaf(eB4, eB4, t4);
+    const float U1 = D1 - D2, U2 = D2 - D3, U3 = D3 - D4;
+    const float fa = weno_face6(qA1, qC2, qB3, U1, U2);
+    const float fb = weno_face6(qA2, qC3, qB4, U2, U3);
+    // 6 v1 of each face: 5 w_c + 2 w_r - w_l.
+    const float va = fmaf(5.0f, w2, fmaf(2.0f, w3, -w1));
+    const float vb = fmaf(5.0f, w3, fmaf(2.0f, w4, -w2));
+    if (VALUES) {
+        *A = (va + fa) * (1.0f / 6.0f);
+        *B = (vb + fb) * (1.0f / 6.0f);
+    }
+    return (vb - va) + (fb - fa);
+}
+
+__device__ __forceinline__ void weno_pair(float w0, float w1, float w2, float w3, float w4, float w5, float &A,
+                                          float &B)
+{
+    (void)weno_pair6<true>(w0, w1, w2, w3, w4, w5, &A, &B);
+}
+
+// |vel| * (N_plus - N_minus) for a direction with a full stencil on both faces.
+// The 1/6 folds into the product with |vel|.
+__device__ __forceinline__ float upwind_difference(float fm3, float fm2, float fm1, float f0, float fp1, float fp2,
+                                                   float fp3, float vel)
+{
+    const bool pos = vel >= 0.0f;
+    const float d6 = weno_pair6<false>(pos ? fm3 : fp3, pos ? fm2 : fp2, pos ? fm1 : fp1, f0, pos ? fp1 : fm1,
+                                       pos ? fp2 : fm2, nullptr, nullptr);
+    return fabsf(vel) * (1.0f / 6.0f) * d6;
+}
+
+// Load at a 32-bit byte offset from a block-uniform base, so the address is a
+// scalar base plus a 32-bit lane offset (global_load ... saddr form) and costs
+// no 64-bit vector arithmetic.  Offsets stay below 4 GiB: a slab is nx*ny
+// elements and the host rejects nx*ny >= 2^30.
+__device__ __forceinline__ float ldo(const float *base, unsigned byte_off)
+{
+    return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(base) + byte_off);
 }
 
 __device__ __forceinline__ int wrap(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 __device__ __forceinline__ int clampi(int i, int n) { return i < 0 ? 0 : (i >= n ? n - 1 : i); }
 __device__ __forceinline__ float clip0(float x) { return x < 0.0f ? 0.0f : x; }
 
+// Work order.  One 64 x 4 (i, j) tile of one (kappa, phi) bin per workgroup.
+// Workgroups are dealt round-robin to the eight XCDs, so the linear block id
+// is remapped to give every XCD one contiguous range of the logical order
+// phi (fastest), kappa, tile row, tile column.  The kappa and phi neighbours
+// a tile reads are then processed on the same XCD at about the same time and
+// are served by that XCD's L2, and a tile's y-halo rows were just read by the
+// tile before it.
+struct Tile {
+    int ib, jb, m, n;
+};
+
+__device__ __forceinline__ Tile tile_of(int b, int total, int nbx, int nby, int nk, int nphi)
+{
+    const int xcd = b & 7, r = b >> 3;
+    const int per = total >> 3, rem = total & 7;
+    const int L = xcd < rem ? xcd * (per + 1) + r : rem * (per + 1) + (xcd - rem) * per + r;
+    Tile t;
+    t.n = L % nphi;
+    int q = L / nphi;
+    t.m = q % nk;
+    q /= nk;
+    t.jb = q % nby;
+    t.ib = q / nby;
+    (void)nbx;
+    return t;
+}
+
 template <int TRANSPORT, int STAGE>
 __global__ __launch_bounds__(256) void stage_fast_kernel(const float *__restrict__ Nk, const float *__restrict__ N0,
-                                                         float *out, StageParams<float> p)
+                                                         float *out, StageParams<float> p, int nbx, int nby)
 {
-    const int i = blockIdx.x * 64 + threadIdx.x;
-    const int j = blockIdx.y * 4 + threadIdx.y;
-    const int mn = blockIdx.z;
     const int nx = p.nx, ny = p.ny, nk = p.nk, nphi = p.nphi;
+    const Tile t = tile_of(blockIdx.x, nbx * nby * nk * nphi, nbx, nby, nk, nphi);
+    const int m = t.m, n = t.n;
+    const int i0 = t.ib * 64, j0 = t.jb * 4;
+    const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
     if (i >= nx || j >= ny)
         return;
-    const int m = mn % nk, n = mn / nk;
     const size_t plane = (size_t)nx * ny;
-    const size_t sk = plane, sp = plane * nk;
-    const size_t c = (size_t)i + (size_t)nx * j + sk * m + sp * n;
-    const float x0 = Nk[c];
+    // Block-uniform slab bases (scalar registers); per-lane byte offsets are 32-bit.
+    const size_t bin = (size_t)m + (size_t)nk * n;
+    const float *slab = Nk + plane * bin;
+    const unsigned nxu = (unsigned)nx;
+    const unsigned off = (unsigned)i + nxu * (unsigned)j;
+    const unsigned bo = off << 2;
+    const float x0 = ldo(slab, bo);
+    const bool x_inner = i0 >= 3 && i0 + 64 + 3 <= nx;  // block-uniform
+    const bool y_inner = j0 >= 3 && j0 + 4 + 3 <= ny;
     float L = 0.0f;
     if (TRANSPORT != RPL_TRANSPORT_NONE) {
-        const int ip1 = wrap(i + 1, nx), im1 = wrap(i - 1, nx);
-        const int jp1 = wrap(j + 1, ny), jm1 = wrap(j - 1, ny);
+        const unsigned rowb = (nxu * (unsigned)j) << 2, colb = (unsigned)i << 2;
+        const unsigned sx = 4u, sy = nxu << 2;
         float ux = p.cgx[m + nk * n], uy = p.cgy[m + nk * n];
         float ck = 0.0f, cp = 0.0f;
         if (TRANSPORT == RPL_TRANSPORT_REFRACTION) {
-            const size_t b3 = plane * m;
-            const float *Ux = p.Ux + b3, *Uy = p.Uy + b3;
+            const float *Ux = p.Ux + plane * m, *Uy = p.Uy + plane * m;
+            const unsigned xp = x_inner ? bo + sx : rowb + ((unsigned)wrap(i + 1, nx) << 2);
+            const unsigned xm = x_inner ? bo - sx : rowb + ((unsigned)wrap(i - 1, nx) << 2);
+            const unsigned yp = y_inner ? bo + sy : colb + sy * (unsigned)wrap(j + 1, ny);
+            const unsigned ym = y_inner ? bo - sy : colb + sy * (unsigned)wrap(j - 1, ny);
             const float hx = 0.5f * p.dx_inv, hy = 0.5f * p.dy_inv;
-            const float Uxx = (Ux[ip1 + (size_t)nx * j] - Ux[im1 + (size_t)nx * j]) * hx;
-            const float Uxy = (Ux[i + (size_t)nx * jp1] - Ux[i + (size_t)nx * jm1]) * hy;
-            const float Uyx = (Uy[ip1 + (size_t)nx * j] - Uy[im1 + (size_t)nx * j]) * hx;
-            const float Uyy = (Uy[i + (size_t)nx * jp1] - Uy[i + (size_t)nx * jm1]) * hy;
+            const float Uxx = (ldo(Ux, xp) - ldo(Ux, xm)) * hx;
+            const float Uxy = (ldo(Ux, yp) - ldo(Ux, ym)) * hy;
+            const float Uyx = (ldo(Uy, xp) - ldo(Uy, xm)) * hx;
+            const float Uyy = (ldo(Uy, yp) - ldo(Uy, ym)) * hy;
             const float kap = p.kappa[m], cs = p.cosp[n], sn = p.sinp[n];
             const float cc = cs * cs, ss = sn * sn, csn = cs * sn;
             ck = -kap * (cc * Uxx + csn * (Uyx + Uxy) + ss * Uyy);
             cp = csn * (Uxx - Uyy) + ss * Uyx - cc * Uxy;
-            ux += Ux[i + (size_t)nx * j];
-            uy += Uy[i + (size_t)nx * j];
+            ux += ldo(Ux, bo);
+            uy += ldo(Uy, bo);
         }
-        const size_t row = c - i;
-        const float xm3 = Nk[row + wrap(i - 3, nx)], xm2 = Nk[row + wrap(i - 2, nx)], xm1 = Nk[row + im1];
-        const float xp1 = Nk[row + ip1], xp2 = Nk[row + wrap(i + 2, nx)], xp3 = Nk[row + wrap(i + 3, nx)];
-        const float fx = ux * (face(xm2, xm1, x0, xp1, xp2, xp3, ux) - face(xm3, xm2, xm1, x0, xp1, xp2, ux));
-        const size_t col = c - (size_t)nx * j;
-        const float ym3 = Nk[col + (size_t)nx * wrap(j - 3, ny)], ym2 = Nk[col + (size_t)nx * wrap(j - 2, ny)];
-        const float ym1 = Nk[col + (size_t)nx * jm1], yp1 = Nk[col + (size_t)nx * jp1];
-        const float yp2 = Nk[col + (size_t)nx * wrap(j + 2, ny)], yp3 = Nk[col + (size_t)nx * wrap(j + 3, ny)];
-        const float fy = uy * (face(ym2, ym1, x0, yp1, yp2, yp3, uy) - face(ym3, ym2, ym1, x0, yp1, yp2, uy));
-        L = -fmaf(fx, p.dx_inv, fy * p.dy_inv);
+        float fx[7], fy[7];
+        if (x_inner) {
+#pragma unroll
+            for (int d = 0; d < 7; ++d)
+                fx[d] = ldo(slab, bo + (unsigned)(4 * (d - 3)));
+        } else {
+#pragma unroll
+            for (int d = 0; d < 7; ++d)
+                fx[d] = ldo(slab, rowb + ((unsigned)wrap(i + d - 3, nx) << 2));
+        }
+        if (y_inner) {
+#pragma unroll
+            for (int d = 0; d < 7; ++d)
+                fy[d] = ldo(slab, bo + sy * (unsigned)(d - 3));
+        } else {
+#pragma unroll
+            for (int d = 0; d < 7; ++d)
+                fy[d] = ldo(slab, colb + sy * (unsigned)wrap(j + d - 3, ny));
+        }
+        const float dx = upwind_difference(fx[0], fx[1], fx[2], x0, fx[4], fx[5], fx[6], ux);
+        const float dy = upwind_difference(fy[0], fy[1], fy[2], x0, fy[4], fy[5], fy[6], uy);
+        L = -fmaf(dx, p.dx_inv, dy * p.dy_inv);
         if (TRANSPORT == RPL_TRANSPORT_REFRACTION) {
-            const size_t kb = c - sk * m;
-            const float km3 = Nk[kb + sk * clampi(m - 3, nk)], km2 = Nk[kb + sk * clampi(m - 2, nk)];
-            const float km1 = Nk[kb + sk * clampi(m - 1, nk)], kp1 = Nk[kb + sk * clampi(m + 1, nk)];
-            const float kp2 = Nk[kb + sk * clampi(m + 2, nk)], kp3 = Nk[kb + sk * clampi(m + 3, nk)];
-            const bool pos = ck >= 0.0f;
-            float Nkp, Nkm;
-            if ((m - 2 >= 0) && (m + 3 <= nk - 1))
-                Nkp = face(km2, km1, x0, kp1, kp2, kp3, ck);
-            else
-                Nkp = pos ? x0 : kp1;
-            if ((m - 3 >= 0) && (m + 2 <= nk - 1))
-                Nkm = face(km3, km2, km1, x0, kp1, kp2, ck);
-            else
-                Nkm = pos ? km1 : x0;
-            const float fkp = (m == nk - 1) ? 0.0f : ck * Nkp;
-            const float fkm = (m == 0) ? 0.0f : ck * Nkm;
-            const size_t pb = c - sp * n;
-            const float pm3 = Nk[pb + sp * wrap(n - 3, nphi)], pm2 = Nk[pb + sp * wrap(n - 2, nphi)];
-            const float pm1 = Nk[pb + sp * wrap(n - 1, nphi)], pp1 = Nk[pb + sp * wrap(n + 1, nphi)];
-            const float pp2 = Nk[pb + sp * wrap(n + 2, nphi)], pp3 = Nk[pb + sp * wrap(n + 3, nphi)];
-            const float fp = cp * (face(pm2, pm1, x0, pp1, pp2, pp3, cp) - face(pm3, pm2, pm1, x0, pp1, pp2, cp));
-            L = L - fmaf(fkp - fkm, p.dk_inv, fp * p.dp_inv);
+            float fp[7], fk[7];
+#pragma unroll
+            for (int d = 0; d < 7; ++d) {
+                if (d == 3)
+                    continue;
+                const size_t pbin = (size_t)m + (size_t)nk * wrap(n + d - 3, nphi);
+                const size_t kbin = (size_t)clampi(m + d - 3, nk) + (size_t)nk * n;
+                fp[d] = ldo(Nk + plane * pbin, bo);
+                fk[d] = ldo(Nk + plane * kbin, bo);
+            }
+            const float dp = upwind_difference(fp[0], fp[1], fp[2], x0, fp[4], fp[5], fp[6], cp);
+            // kappa: bounded; m is uniform across the block, so the edge path
+            // (first-order faces, zero outer flux) does not diverge.
+            float dk;
+            if (m >= 3 && m + 3 <= nk - 1) {
+                dk = upwind_difference(fk[0], fk[1], fk[2], x0, fk[4], fk[5], fk[6], ck);
+            } else {
+                const float km3 = fk[0], km2 = fk[1], km1 = fk[2], kp1 = fk[4], kp2 = fk[5], kp3 = fk[6];
+                const bool pos = ck >= 0.0f;
+                const float w2 = pos ? km1 : kp1, w4 = pos ? kp1 : km1;
+                float A = w2, B = x0;  // first-order upstream / downstream faces
+                const bool has_p = (m - 2 >= 0) && (m + 3 <= nk - 1);
+                const bool has_m = (m - 3 >= 0) && (m + 2 <= nk - 1);
+                if (has_p || has_m) {
+                    float Aw, Bw;
+                    weno_pair(pos ? km3 : kp3, pos ? km2 : kp2, w2, x0, w4, pos ? kp2 : km2, Aw, Bw);
+                    // For ck >= 0 the downstream face is the plus face, else the minus face.
+                    if (pos ? has_m : has_p)
+                        A = Aw;
+                    if (pos ? has_p : has_m)
+                        B = Bw;
+                }
+                const float plus = pos ? B : A, minus = pos ? A : B;
+                const float fkp = (m == nk - 1) ? 0.0f : ck * plus;
+                const float fkm = (m == 0) ? 0.0f : ck * minus;
+                dk = fkp - fkm;
+            }
+            L = L - fmaf(dk, p.dk_inv, dp * p.dp_inv);
         }
     }
     if (p.src_add)
-        L += p.src_add[c];
+        L += ldo(p.src_add + plane * bin, bo);
+    float *dst = out + plane * bin;
+    float r;
     if (STAGE == STAGE_TENDENCY) {
-        out[c] = L;
+        r = L;
     } else if (STAGE == STAGE_1) {
-        out[c] = clip0(fmaf(p.dt, L, x0));
+        r = clip0(fmaf(p.dt, L, x0));
     } else {
         const float a = STAGE == STAGE_2 ? 0.75f : (float)(1.0 / 3.0);
         const float b = STAGE == STAGE_2 ? 0.25f : (float)(2.0 / 3.0);
         const float st = clip0(fmaf(p.dt, L, x0));
-        out[c] = clip0(fmaf(a, N0[c], b * st));
+        r = clip0(fmaf(a, ldo(N0 + plane * bin, bo), b * st));
     }
+    *reinterpret_cast<float *>(reinterpret_cast<char *>(dst) + bo) = r;
 }
 
 template <int TRANSPORT>
 hipError_t launch_t(const StageParams<float> &p, int stage, const float *Nk, const float *N0, float *out,
                     hipStream_t s)
 {
+    const int nbx = (p.nx + 63) / 64, nby = (p.ny + 3) / 4;
+    const long long total = (long long)nbx * nby * p.nk * p.nphi;
+    if (total > 0x7fffffffLL)
+        return hipErrorInvalidValue;
     dim3 block(64, 4, 1);
-    dim3 grid((p.nx + 63) / 64, (p.ny + 3) / 4, p.nk * p.nphi);
+    dim3 grid((unsigned)total, 1, 1);
     switch (stage) {
     case STAGE_TENDENCY:
-        hipLaunchKernelGGL((stage_fast_kernel<TRANSPORT, STAGE_TENDENCY>), grid, block, 0, s, Nk, N0, out, p);
+        hipLaunchKernelGGL((stage_fast_kernel<TRANSPORT, STAGE_TENDENCY>), grid, block, 0, s, Nk, N0, out, p, nbx, nby);
         break;
     case STAGE_1:
-        hipLaunchKernelGGL((stage_fast_kernel<TRANSPORT, STAGE_1>), grid, block, 0, s, Nk, N0, out, p);
+        hipLaunchKernelGGL((stage_fast_kernel<TRANSPORT, STAGE_1>), grid, block, 0, s, Nk, N0, out, p, nbx, nby);
         break;
     case STAGE_2:
-        hipLaunchKernelGGL((stage_fast_kernel<TRANSPORT, STAGE_2>), grid, block, 0, s, Nk, N0, out, p);
+        hipLaunchKernelGGL((stage_fast_kernel<TRANSPORT, STAGE_2>), grid, block, 0, s, Nk, N0, out, p, nbx, nby);
         break;
     default:
-        hipLaunchKernelGGL((stage_fast_kernel<TRANSPORT, STAGE_3>), grid, block, 0, s, Nk, N0, out, p);
+        hipLaunchKernelGGL((stage_fast_kernel<TRANSPORT, STAGE_3>), grid, block, 0, s, Nk, N0, out, p, nbx, nby);
         break;
     }
     return hipGetLastError();
