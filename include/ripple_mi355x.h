@@ -130,6 +130,43 @@ int rpl_diagnostics(rpl_ctx *ctx, double *m0, double *kappa_rms, double *mean_di
  * the device in FP64 with compensated (double-double) summation. */
 int rpl_conservation(rpl_ctx *ctx, double *total_action, double *total_energy);
 
+/* Source terms of the PrecomputedSources bundle
+ * (Physics/Packages/precomputed_sources.jl:23-283), evaluated on the device on
+ * every stage input and added to the transport tendency
+ * (Models/tendencies.jl:88-94):
+ *   wind input  PressureCorrelationInput (Physics/WindInput/pressure_correlation.jl:24-156)
+ *               with BulkWindDrag(:linear | :capped) (Physics/shared/drag.jl:21-34)
+ *               and, when stress_cap = 1, the bundle's wave-supported-stress
+ *               cap (precomputed_sources.jl:126-170);
+ *   dissipation MeanSpectrumWhitecapping (Physics/Dissipation/mean_spectrum.jl:20-140)
+ *               or LocalSaturationDissipation (Physics/Dissipation/local_saturation.jl:18-52).
+ * Winds: a uniform speed and direction (the reference's `wind::Number` with
+ * `direction`), or HollandHurricaneWind on a two-point LinearStormTrack
+ * (Forcing/winds.jl:4-41, 89-198), evaluated per column at the clock time of
+ * each step. */
+enum { RPL_WIND_INPUT_NONE = 0, RPL_WIND_INPUT_PRESSURE_CORRELATION = 1 };
+enum { RPL_DRAG_LINEAR = 0, RPL_DRAG_CAPPED = 1 };
+enum { RPL_WIND_UNIFORM = 0, RPL_WIND_HOLLAND = 1 };
+enum { RPL_DISSIPATION_NONE = 0, RPL_DISSIPATION_MEAN_SPECTRUM = 1, RPL_DISSIPATION_LOCAL_SATURATION = 2 };
+
+typedef struct rpl_sources {
+    int32_t wind_input, drag, wind_field, stress_cap, dissipation, rotation; /* rotation: +1 ccw, -1 cw */
+    double wind_speed, wind_direction;                                       /* RPL_WIND_UNIFORM */
+    double beta_max, z_alpha, p_in, alpha0, rho_air, rho_water, von_karman, gravity, cap_epsilon;
+    double vmax, rmax, radius, shape, inflow_angle, background_x, background_y; /* RPL_WIND_HOLLAND */
+    double track_t0, track_t1, track_x0, track_y0, track_x1, track_y1;
+    double cds, delta1, delta2, p_mean;          /* MeanSpectrumWhitecapping */
+    double ls_cds, ls_br, ls_p, ls_sigma_power;  /* LocalSaturationDissipation */
+} rpl_sources;
+
+/* Installs (or, with NULL, removes) the source terms.  x_nodes / y_nodes are
+ * the nx / ny cell-centre coordinates the winds are evaluated at. */
+int rpl_set_sources(rpl_ctx *ctx, const rpl_sources *src, const double *x_nodes, const double *y_nodes);
+
+/* Per-column wind state of the last step: U10, wind angle, friction velocity,
+ * (nx, ny) FP64 each (the reference's _compute_wind_input_state host loop). */
+int rpl_get_wind(rpl_ctx *ctx, double *U10, double *theta_u, double *u_star);
+
 /* Per-kernel timing of the fused stage kernel: when enabled, HIP events on
  * the context stream bracket every stage launch; the accumulated elapsed
  * milliseconds and launch count are read (and reset) with

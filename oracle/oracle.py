@@ -218,3 +218,73 @@ class OracleModel:
         w = _c(measure)
         return getattr(lib(), f"orc_total_weighted_{sfx}")(_I(nx), _I(ny), _I(nk), _I(npf), _p(self.N), _p(w),
                                                            _D(self.dx), _D(self.dy))
+
+
+# --- source terms (FP64 oracle only) -----------------------------------------
+
+class OrcSources(ctypes.Structure):
+    """Mirror of orc_sources (oracle/sources_impl.h); same layout as rpl_sources."""
+    _fields_ = [(n, ctypes.c_int) for n in ("wind_input", "drag", "wind_field", "stress_cap", "dissipation",
+                                             "rotation")] + \
+               [(n, ctypes.c_double) for n in (
+                   "wind_speed", "wind_direction",
+                   "beta_max", "z_alpha", "p_in", "alpha0", "rho_air", "rho_water", "von_karman", "gravity",
+                   "cap_epsilon",
+                   "vmax", "rmax", "radius", "shape", "inflow_angle", "background_x", "background_y",
+                   "track_t0", "track_t1", "track_x0", "track_y0", "track_x1", "track_y1",
+                   "cds", "delta1", "delta2", "p_mean",
+                   "ls_cds", "ls_br", "ls_p", "ls_sigma_power")]
+
+
+class OrcSourceCtx(ctypes.Structure):
+    _fields_ = [("src", ctypes.POINTER(OrcSources)), ("nx", ctypes.c_int), ("ny", ctypes.c_int),
+                ("nk", ctypes.c_int), ("nphi", ctypes.c_int), ("x_nodes", ctypes.c_void_p),
+                ("y_nodes", ctypes.c_void_p), ("kappa", ctypes.c_void_p), ("phi", ctypes.c_void_p),
+                ("w", ctypes.c_void_p), ("time", ctypes.c_double)]
+
+
+def sources_from(struct):
+    """Copy any ctypes structure with the rpl_sources field names into OrcSources."""
+    s = OrcSources()
+    for name, _ in OrcSources._fields_:
+        setattr(s, name, getattr(struct, name))
+    return s
+
+
+def wind_columns(src, x_nodes, y_nodes, t):
+    xn, yn = _c(x_nodes), _c(y_nodes)
+    nx, ny = len(xn), len(yn)
+    out = [np.zeros(nx * ny) for _ in range(3)]
+    lib().orc_wind_columns(ctypes.byref(src), _I(nx), _I(ny), _p(xn), _p(yn), _D(t), *[_p(a) for a in out])
+    return [a.reshape((nx, ny), order="F") for a in out]
+
+
+class OracleSourceModel(OracleModel):
+    """OracleModel plus the FP64 source terms of the PrecomputedSources bundle."""
+
+    def __init__(self, workload, sources, **kw):
+        super().__init__(workload, dtype=np.float64, **kw)
+        g = workload.grid
+        self._src = sources
+        self._xn = _c(0.5 * (g.x_faces[1:] + g.x_faces[:-1]))
+        self._yn = _c(0.5 * (g.y_faces[1:] + g.y_faces[:-1]))
+        sg = workload.spectral_grid
+        self._kc, self._pc = _c(sg.kappa), _c(sg.phi)
+        self._w = _c(self.tables["w"])
+        nx, ny, nk, npf = self.size
+        self._ctx = OrcSourceCtx(ctypes.pointer(self._src), nx, ny, nk, npf, self._xn.ctypes.data,
+                                 self._yn.ctypes.data, self._kc.ctypes.data, self._pc.ctypes.data,
+                                 self._w.ctypes.data, 0.0)
+        self._hook = ctypes.cast(lib().orc_sources_hook_f64, ctypes.c_void_p)
+
+    def step(self, dt, nsteps=1, hook=None, user=None):
+        for _ in range(nsteps):
+            self._ctx.time = self.time
+            super().step(dt, 1, self._hook, ctypes.cast(ctypes.pointer(self._ctx), ctypes.c_void_p))
+
+    def tendency(self):
+        G = super().tendency()
+        Gf = _c(G)
+        self._ctx.time = self.time
+        lib().orc_sources_hook_f64(_p(self.N), _p(Gf), ctypes.pointer(self._ctx))
+        return Gf.reshape(self.size, order="F")

@@ -48,6 +48,8 @@ static inline int orc_clamp(int i, int n) { return i < 0 ? 0 : (i >= n ? n - 1 :
 #undef REAL_EPS
 #undef SFX
 
+#include "sources_impl.h"
+
 int orc_set_threads(int n)
 {
 #ifdef _OPENMP

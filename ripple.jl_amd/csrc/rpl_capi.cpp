@@ -51,6 +51,18 @@ struct rpl_ctx {
     double time = 0;
     int64_t iteration = 0;
 
+    // source terms
+    bool has_sources = false;
+    rpl_sources src{};
+    rpl::WindParams wind{};
+    double *xn = nullptr, *yn = nullptr;                       // cell centres (FP64)
+    void *U10 = nullptr, *theta_u = nullptr, *ustar = nullptr;  // per column
+    void *factor = nullptr, *cm0 = nullptr, *kbar = nullptr, *sbar = nullptr;
+    void *k_cell = nullptr, *theta_cell = nullptr, *sigma_cell = nullptr, *C_cell = nullptr;  // per bin
+    void *t_phi = nullptr, *t_w = nullptr, *t_sigma_k = nullptr, *t_cos = nullptr, *t_sin = nullptr;
+    void *t_kpow = nullptr, *t_spow = nullptr;
+    std::vector<double> h_kc, h_pc, h_kf, h_pf;  // geometry kept for source tables
+
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
     size_t events_used = 0;
@@ -126,7 +138,103 @@ rpl::StageParams<T> stage_params(const rpl_ctx *c, double dt)
     p.dp_inv = T(1) / (T)c->dp;
     p.dt = (T)dt;
     p.src_add = nullptr;
+    p.src = rpl::SourceParams<T>{};
+    if (c->has_sources) {
+        const rpl_sources &s = c->src;
+        rpl::SourceParams<T> &q = p.src;
+        q.on = 1;
+        q.wind_input = s.wind_input;
+        q.stress_cap = s.stress_cap;
+        q.dissipation = s.dissipation;
+        q.U10 = (const T *)c->U10;
+        q.theta_u = (const T *)c->theta_u;
+        q.ustar = (const T *)c->ustar;
+        q.factor = (const T *)c->factor;
+        q.m0 = (const T *)c->cm0;
+        q.kbar = (const T *)c->kbar;
+        q.sbar = (const T *)c->sbar;
+        q.k_cell = (const T *)c->k_cell;
+        q.theta_cell = (const T *)c->theta_cell;
+        q.sigma_cell = (const T *)c->sigma_cell;
+        q.C_cell = (const T *)c->C_cell;
+        q.c0 = (T)((s.rho_air / s.rho_water) * (s.beta_max / (s.von_karman * s.von_karman)));
+        q.z_alpha = (T)s.z_alpha;
+        q.p_in = (T)s.p_in;
+        q.alpha0 = (T)s.alpha0;
+        q.kvk = (T)s.von_karman;
+        q.g = (T)s.gravity;
+        q.cds = (T)s.cds;
+        q.d1 = (T)s.delta1;
+        q.d2 = (T)s.delta2;
+        q.ls_cds = (T)s.ls_cds;
+        q.ls_br = (T)s.ls_br;
+        q.ls_p = (T)s.ls_p;
+        q.ls_q = (T)s.ls_sigma_power;
+    }
     return p;
+}
+
+template <typename T>
+rpl::StateTables<T> state_tables(const rpl_ctx *c)
+{
+    rpl::StateTables<T> t{};
+    t.kappa = (const T *)c->kappa;
+    t.phi = (const T *)c->t_phi;
+    t.w = (const T *)c->t_w;
+    t.sigma_k = (const T *)c->t_sigma_k;
+    t.cos_phi = (const T *)c->t_cos;
+    t.sin_phi = (const T *)c->t_sin;
+    t.k_pow = (const T *)c->t_kpow;
+    t.sigma_pow = (const T *)c->t_spow;
+    t.rho_air = (T)c->src.rho_air;
+    t.rho_w_g = (T)(c->src.rho_water * c->src.gravity);
+    t.cap_eps = (T)c->src.cap_epsilon;
+    t.inv_p = (T)(1 / c->src.p_mean);
+    return t;
+}
+
+// Wind of the step at the clock time (constant over the three stages).
+int launch_wind(rpl_ctx *c)
+{
+    if (!c->has_sources || !c->src.wind_input)
+        return RPL_OK;
+    hipError_t e;
+    const int nx = c->cfg.nx, ny = c->cfg.ny;
+    if (is_f64(c))
+        e = rpl::launch_wind_columns<double>(nx, ny, c->xn, c->yn, c->time, c->wind, (double *)c->U10,
+                                             (double *)c->theta_u, (double *)c->ustar, c->stream);
+    else
+        e = rpl::launch_wind_columns<float>(nx, ny, c->xn, c->yn, c->time, c->wind, (float *)c->U10,
+                                            (float *)c->theta_u, (float *)c->ustar, c->stream);
+    if (e != hipSuccess)
+        return fail(RPL_ERR_HIP, std::string("wind kernel: ") + hipGetErrorString(e));
+    return RPL_OK;
+}
+
+// Bundle state of one stage input (prepare_sources, precomputed_sources.jl:230-238).
+int launch_state(rpl_ctx *c, const void *Nk)
+{
+    if (!c->has_sources)
+        return RPL_OK;
+    const bool needs = (c->src.wind_input && c->src.stress_cap) || c->src.dissipation == RPL_DISSIPATION_MEAN_SPECTRUM;
+    if (!needs)
+        return RPL_OK;
+    const int nx = c->cfg.nx, ny = c->cfg.ny, nk = c->cfg.nkappa, np = c->cfg.nphi;
+    hipError_t e;
+    if (is_f64(c)) {
+        auto p = stage_params<double>(c, 0.0);
+        e = rpl::launch_source_state<double>(nx, ny, nk, np, (const double *)Nk, p.src, state_tables<double>(c),
+                                             (double *)c->factor, (double *)c->cm0, (double *)c->kbar,
+                                             (double *)c->sbar, c->stream);
+    } else {
+        auto p = stage_params<float>(c, 0.0);
+        e = rpl::launch_source_state<float>(nx, ny, nk, np, (const float *)Nk, p.src, state_tables<float>(c),
+                                            (float *)c->factor, (float *)c->cm0, (float *)c->kbar, (float *)c->sbar,
+                                            c->stream);
+    }
+    if (e != hipSuccess)
+        return fail(RPL_ERR_HIP, std::string("source state kernel: ") + hipGetErrorString(e));
+    return RPL_OK;
 }
 
 bool use_fast(const rpl_ctx *c)
@@ -305,7 +413,10 @@ int rpl_destroy(rpl_ctx *c)
         (void)hipStreamSynchronize(c->stream);
     for (void *p : {c->A, c->B, c->C, c->Ux, c->Uy, c->Uxx, c->Uxy, c->Uyx, c->Uyy, c->kappa, c->cgx, c->cgy,
                     c->cosp, c->sinp, (void *)c->w, (void *)c->trace, (void *)c->mux, (void *)c->muy,
-                    (void *)c->energy, (void *)c->diag, (void *)c->partials, c->scratch_in})
+                    (void *)c->energy, (void *)c->diag, (void *)c->partials, c->scratch_in, (void *)c->xn,
+                    (void *)c->yn, c->U10, c->theta_u, c->ustar, c->factor, c->cm0, c->kbar, c->sbar, c->k_cell,
+                    c->theta_cell, c->sigma_cell, c->C_cell, c->t_phi, c->t_w, c->t_sigma_k, c->t_cos, c->t_sin,
+                    c->t_kpow, c->t_spow})
         free_ptr(p);
     for (auto &ev : c->events) {
         (void)hipEventDestroy(ev.first);
@@ -382,6 +493,11 @@ int rpl_set_geometry(rpl_ctx *c, double dx, double dy, const double *kappa_c, co
         }
     }
     std::vector<double> kc(kappa_c, kappa_c + nk);
+    c->h_kc = kc;
+    c->h_kf.assign(kappa_f, kappa_f + nk + 1);
+    c->h_pc.assign(phi_c, phi_c + np);
+    c->h_pf.assign(phi_f, phi_f + np + 1);
+    c->has_sources = false;  // source tables depend on the geometry
     int rc;
     if ((rc = upload_real(c, kc.data(), c->kappa, nk)) || (rc = upload_real(c, cgx.data(), c->cgx, tab)) ||
         (rc = upload_real(c, cgy.data(), c->cgy, tab)) || (rc = upload_real(c, cs.data(), c->cosp, np)) ||
@@ -557,7 +673,8 @@ int rpl_tendency(rpl_ctx *c, void *G)
         return rc;
     if (!G)
         return fail(RPL_ERR_ARG, "null argument");
-    if ((rc = launch_stage(c, 0.0, rpl::STAGE_TENDENCY, c->A, c->A, c->B)))
+    if ((rc = launch_wind(c)) || (rc = launch_state(c, c->A)) ||
+        (rc = launch_stage(c, 0.0, rpl::STAGE_TENDENCY, c->A, c->A, c->B)))
         return rc;
     RPL_HIP(hipStreamSynchronize(c->stream));
     RPL_HIP(hipMemcpy(G, c->B, c->cnt * c->esize, hipMemcpyDeviceToHost));
@@ -574,15 +691,17 @@ int rpl_time_step(rpl_ctx *c, double dt, int32_t nsteps)
     if (nsteps < 0)
         return fail(RPL_ERR_ARG, "nsteps must be >= 0");
     for (int s = 0; s < nsteps; ++s) {
+        if ((rc = launch_wind(c)))
+            return rc;
         if (c->cfg.timestepper == RPL_FORWARD_EULER) {
-            if ((rc = launch_stage(c, dt, rpl::STAGE_1, c->A, c->A, c->B)))
+            if ((rc = launch_state(c, c->A)) || (rc = launch_stage(c, dt, rpl::STAGE_1, c->A, c->A, c->B)))
                 return rc;
             std::swap(c->A, c->B);
         } else {
             // SSP-RK3 over three buffers; A holds N0 and receives N3 in place.
-            if ((rc = launch_stage(c, dt, rpl::STAGE_1, c->A, c->A, c->B)) ||
-                (rc = launch_stage(c, dt, rpl::STAGE_2, c->B, c->A, c->C)) ||
-                (rc = launch_stage(c, dt, rpl::STAGE_3, c->C, c->A, c->A)))
+            if ((rc = launch_state(c, c->A)) || (rc = launch_stage(c, dt, rpl::STAGE_1, c->A, c->A, c->B)) ||
+                (rc = launch_state(c, c->B)) || (rc = launch_stage(c, dt, rpl::STAGE_2, c->B, c->A, c->C)) ||
+                (rc = launch_state(c, c->C)) || (rc = launch_stage(c, dt, rpl::STAGE_3, c->C, c->A, c->A)))
                 return rc;
         }
         c->time += dt;  // tick!(clock, dt), once per step
@@ -691,6 +810,168 @@ int rpl_conservation(rpl_ctx *c, double *total_action, double *total_energy)
         return rc;
     if (total_energy && (rc = weighted_total(c, c->energy, total_energy)))
         return rc;
+    return RPL_OK;
+}
+
+int rpl_set_sources(rpl_ctx *c, const rpl_sources *src, const double *x_nodes, const double *y_nodes)
+{
+    if (!c)
+        return fail(RPL_ERR_ARG, "null context");
+    if (!src) {
+        c->has_sources = false;
+        return RPL_OK;
+    }
+    if (!c->has_geometry)
+        return fail(RPL_ERR_STATE, "rpl_set_geometry has not been called");
+    const rpl_sources &s = *src;
+    if (s.wind_input != RPL_WIND_INPUT_NONE && s.wind_input != RPL_WIND_INPUT_PRESSURE_CORRELATION)
+        return fail(RPL_ERR_ARG, "unsupported wind input");
+    if (s.dissipation < RPL_DISSIPATION_NONE || s.dissipation > RPL_DISSIPATION_LOCAL_SATURATION)
+        return fail(RPL_ERR_ARG, "unsupported dissipation");
+    if (s.wind_input) {
+        if (!x_nodes || !y_nodes)
+            return fail(RPL_ERR_ARG, "wind input needs the cell-centre coordinates");
+        if (s.drag != RPL_DRAG_LINEAR && s.drag != RPL_DRAG_CAPPED)
+            return fail(RPL_ERR_ARG, "unsupported drag");
+        if (s.wind_field != RPL_WIND_UNIFORM && s.wind_field != RPL_WIND_HOLLAND)
+            return fail(RPL_ERR_ARG, "unsupported wind field");
+        if (!(s.gravity > 0) || !(s.rho_water > 0) || !(s.von_karman > 0))
+            return fail(RPL_ERR_ARG, "wind input constants must be positive");
+        if (s.wind_field == RPL_WIND_HOLLAND) {
+            // winds.jl:108-111
+            if (!(s.vmax >= 0))
+                return fail(RPL_ERR_ARG, "Holland hurricane vmax must be nonnegative");
+            if (!(s.rmax > 0))
+                return fail(RPL_ERR_ARG, "Holland hurricane rmax must be positive");
+            if (!(s.radius > s.rmax))
+                return fail(RPL_ERR_ARG, "Holland hurricane radius must exceed rmax");
+            if (!(s.shape > 0))
+                return fail(RPL_ERR_ARG, "Holland hurricane shape_parameter must be positive");
+            if (!(s.track_t1 > s.track_t0))
+                return fail(RPL_ERR_ARG, "storm-track times must be strictly increasing");
+        }
+    }
+    if (s.dissipation == RPL_DISSIPATION_MEAN_SPECTRUM && !(s.p_mean != 0))
+        return fail(RPL_ERR_ARG, "mean-spectrum exponent p must be nonzero");
+    RPL_HIP(hipSetDevice(c->device));
+    const int nx = c->cfg.nx, ny = c->cfg.ny, nk = c->cfg.nkappa, np = c->cfg.nphi;
+    const size_t tab = (size_t)nk * np, plane = c->plane;
+    int rc;
+    void **col_bufs[] = {&c->U10, &c->theta_u, &c->ustar, &c->factor, &c->cm0, &c->kbar, &c->sbar};
+    for (void **b : col_bufs)
+        if (!*b && (rc = alloc(b, plane * c->esize)))
+            return rc;
+    void **tab_bufs[] = {&c->k_cell, &c->theta_cell, &c->sigma_cell, &c->C_cell, &c->t_w};
+    for (void **b : tab_bufs)
+        if (!*b && (rc = alloc(b, tab * c->esize)))
+            return rc;
+    void **k_bufs[] = {&c->t_sigma_k, &c->t_kpow, &c->t_spow};
+    for (void **b : k_bufs)
+        if (!*b && (rc = alloc(b, nk * c->esize)))
+            return rc;
+    void **p_bufs[] = {&c->t_phi, &c->t_cos, &c->t_sin};
+    for (void **b : p_bufs)
+        if (!*b && (rc = alloc(b, np * c->esize)))
+            return rc;
+    if (!c->xn && (rc = alloc((void **)&c->xn, nx * 8)))
+        return rc;
+    if (!c->yn && (rc = alloc((void **)&c->yn, ny * 8)))
+        return rc;
+    // Per-bin values of the per-cell path: k = hypot(kx, ky), theta = atan(ky, kx)
+    // from (kx, ky) = kappa (cos phi, sin phi) (pressure_correlation.jl:118-123).
+    const double g = s.gravity;
+    std::vector<double> kcell(tab), thcell(tab), scell(tab), Ccell(tab), w(tab);
+    std::vector<double> sk(nk), kpow(nk), spow(nk), cphi(np), sphi(np);
+    for (int n = 0; n < np; ++n) {
+        cphi[n] = std::cos(c->h_pc[n]);
+        sphi[n] = std::sin(c->h_pc[n]);
+        for (int m = 0; m < nk; ++m) {
+            const size_t t = (size_t)m + (size_t)nk * n;
+            const double kx = c->h_kc[m] * std::cos(c->h_pc[n]), ky = c->h_kc[m] * std::sin(c->h_pc[n]);
+            const double k = std::hypot(kx, ky);
+            kcell[t] = k;
+            thcell[t] = std::atan2(ky, kx);
+            scell[t] = std::sqrt(g * k);
+            Ccell[t] = k > 0 ? scell[t] / k : 0.0;
+            const double k1 = c->h_kf[m], k2 = c->h_kf[m + 1];
+            w[t] = ((k2 * k2 - k1 * k1) / 2) * (c->h_pf[n + 1] - c->h_pf[n]);
+        }
+    }
+    for (int m = 0; m < nk; ++m) {
+        sk[m] = std::sqrt(g * c->h_kc[m]);
+        kpow[m] = s.dissipation == RPL_DISSIPATION_MEAN_SPECTRUM ? std::pow(c->h_kc[m], s.p_mean) : 0.0;
+        spow[m] = s.dissipation == RPL_DISSIPATION_MEAN_SPECTRUM ? std::pow(sk[m], s.p_mean) : 0.0;
+    }
+    if ((rc = upload_real(c, kcell.data(), c->k_cell, tab)) || (rc = upload_real(c, thcell.data(), c->theta_cell, tab)) ||
+        (rc = upload_real(c, scell.data(), c->sigma_cell, tab)) || (rc = upload_real(c, Ccell.data(), c->C_cell, tab)) ||
+        (rc = upload_real(c, w.data(), c->t_w, tab)) || (rc = upload_real(c, sk.data(), c->t_sigma_k, nk)) ||
+        (rc = upload_real(c, kpow.data(), c->t_kpow, nk)) || (rc = upload_real(c, spow.data(), c->t_spow, nk)) ||
+        (rc = upload_real(c, c->h_pc.data(), c->t_phi, np)) || (rc = upload_real(c, cphi.data(), c->t_cos, np)) ||
+        (rc = upload_real(c, sphi.data(), c->t_sin, np)))
+        return rc;
+    if (s.wind_input) {
+        RPL_HIP(hipMemcpy(c->xn, x_nodes, nx * 8, hipMemcpyHostToDevice));
+        RPL_HIP(hipMemcpy(c->yn, y_nodes, ny * 8, hipMemcpyHostToDevice));
+    }
+    // Stress factor defaults to 1 where no cap is computed.
+    std::vector<double> ones(plane, 1.0);
+    if ((rc = upload_real(c, ones.data(), c->factor, plane)))
+        return rc;
+    rpl::WindParams &wp = c->wind;
+    wp = rpl::WindParams{};
+    wp.field = s.wind_field;
+    wp.drag = s.drag;
+    wp.rotation = s.rotation >= 0 ? 1 : -1;
+    wp.speed = s.wind_speed;
+    wp.direction = s.wind_direction;
+    wp.vmax = s.vmax;
+    wp.rmax = s.rmax;
+    wp.radius = s.radius;
+    wp.shape = s.shape;
+    // winds.jl:150: max(radius - rmax, eps(radius)).
+    const double eps_r = std::nextafter(s.radius, INFINITY) - s.radius;
+    wp.decay = std::max(s.radius - s.rmax, eps_r);
+    wp.bx = s.background_x;
+    wp.by = s.background_y;
+    wp.cos_inflow = std::cos(s.inflow_angle);
+    wp.sin_inflow = std::sin(s.inflow_angle);
+    wp.t0 = s.track_t0;
+    wp.t1 = s.track_t1;
+    wp.x0 = s.track_x0;
+    wp.y0 = s.track_y0;
+    wp.x1 = s.track_x1;
+    wp.y1 = s.track_y1;
+    c->src = s;
+    c->has_sources = s.wind_input != RPL_WIND_INPUT_NONE || s.dissipation != RPL_DISSIPATION_NONE;
+    RPL_HIP(hipStreamSynchronize(c->stream));
+    return RPL_OK;
+}
+
+int rpl_get_wind(rpl_ctx *c, double *U10, double *theta_u, double *u_star)
+{
+    if (!c)
+        return fail(RPL_ERR_ARG, "null context");
+    if (!c->has_sources || !c->src.wind_input)
+        return fail(RPL_ERR_STATE, "no wind input installed");
+    RPL_HIP(hipSetDevice(c->device));
+    int rc = launch_wind(c);
+    if (rc)
+        return rc;
+    RPL_HIP(hipStreamSynchronize(c->stream));
+    double *outs[] = {U10, theta_u, u_star};
+    void *ins[] = {c->U10, c->theta_u, c->ustar};
+    for (int q = 0; q < 3; ++q) {
+        if (!outs[q])
+            continue;
+        if (is_f64(c)) {
+            RPL_HIP(hipMemcpy(outs[q], ins[q], c->plane * 8, hipMemcpyDeviceToHost));
+        } else {
+            std::vector<float> t(c->plane);
+            RPL_HIP(hipMemcpy(t.data(), ins[q], c->plane * 4, hipMemcpyDeviceToHost));
+            for (size_t k = 0; k < c->plane; ++k)
+                outs[q][k] = t[k];
+        }
+    }
     return RPL_OK;
 }
 

@@ -7,6 +7,7 @@
 #include <cstdint>
 
 #include "../../include/ripple_mi355x.h"
+#include "sources_device.h"
 
 namespace rpl {
 
@@ -23,7 +24,33 @@ struct StageParams {
     T dx_inv, dy_inv, dk_inv, dp_inv; // 1 / first spacing
     T dt;                             // time step in T (time_step.jl:16-20)
     const T *src_add;                 // optional additive tendency (nx,ny,nk,nphi) or null
+    SourceParams<T> src;              // source terms (src.on == 0: none)
 };
+
+// Wind of one step (Forcing/winds.jl), FP64.
+struct WindParams {
+    int field, drag, rotation;
+    double speed, direction;                               // uniform
+    double vmax, rmax, radius, shape, decay, bx, by;       // Holland
+    double cos_inflow, sin_inflow;
+    double t0, t1, x0, y0, x1, y1;                         // two-point storm track
+};
+
+// Tables of the per-column state kernel (precomputed_sources.jl:41-74, 126-170).
+template <typename T>
+struct StateTables {
+    const T *kappa, *phi, *w;           // centres, cell measures (nk, nphi)
+    const T *sigma_k, *cos_phi, *sin_phi;
+    const T *k_pow, *sigma_pow;         // kappa^p, sigma^p (mean-spectrum moments)
+    T rho_air, rho_w_g, cap_eps, inv_p;
+};
+
+template <typename T>
+hipError_t launch_wind_columns(int nx, int ny, const double *xn, const double *yn, double t, const WindParams &w,
+                               T *U10, T *theta_u, T *ustar, hipStream_t s);
+template <typename T>
+hipError_t launch_source_state(int nx, int ny, int nk, int nphi, const T *N, const SourceParams<T> &sp,
+                               const StateTables<T> &tb, T *factor, T *m0, T *kbar, T *sbar, hipStream_t s);
 
 // Stage selector of the fused stage kernels (time_step.jl:138-152):
 //   0  out = L(Nk)                                   (tendency only)

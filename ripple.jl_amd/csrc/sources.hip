@@ -1,0 +1,174 @@
+// Per-column source state: the wind of the step and the bundle state of each
+// stage input (stress cap, mean-spectrum moments).
+#include "rpl_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace rpl {
+namespace {
+
+// Forcing/winds.jl:28-41 (two-point LinearStormTrack), 144-154
+// (hurricane_speed for HollandHurricaneWind), 184-204 (wind_velocity,
+// wind_speed, wind_angle); drag.jl:25-34 (friction_velocity).  FP64 per
+// column, as the reference's host loop (precomputed_sources.jl:182-191).
+template <typename T>
+__global__ __launch_bounds__(256) void wind_columns_kernel(int nx, int ny, const double *__restrict__ xn,
+                                                           const double *__restrict__ yn, double t, WindParams w,
+                                                           T *U10, T *theta_u, T *ustar)
+{
+    const size_t ij = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (ij >= (size_t)nx * ny)
+        return;
+    const int i = (int)(ij % nx), j = (int)(ij / nx);
+    double speed, angle;
+    if (w.field == RPL_WIND_UNIFORM) {
+        speed = w.speed;
+        angle = w.direction;
+    } else {
+        double cx, cy;
+        if (t <= w.t0) {
+            cx = w.x0;
+            cy = w.y0;
+        } else if (t >= w.t1) {
+            cx = w.x1;
+            cy = w.y1;
+        } else {
+            const double a = (t - w.t0) / (w.t1 - w.t0);
+            cx = (1 - a) * w.x0 + a * w.x1;
+            cy = (1 - a) * w.y0 + a * w.y1;
+        }
+        const double dx = xn[i] - cx, dy = yn[j] - cy;
+        const double r = hypot(dx, dy);
+        double vx, vy;
+        if (r == 0) {
+            vx = w.bx;
+            vy = w.by;
+        } else {
+            const double shape_term = pow(w.rmax / r, w.shape);
+            const double v = shape_term * exp(1 - shape_term);
+            double sp = w.vmax * sqrt(v > 0 ? v : 0.0);
+            if (r > w.radius)
+                sp *= exp(-(r - w.radius) / w.decay);
+            const double sg = (double)w.rotation;
+            const double tx = -sg * dy / r, ty = sg * dx / r;
+            const double rx = dx / r, ry = dy / r;
+            vx = sp * (w.cos_inflow * tx - w.sin_inflow * rx) + w.bx;
+            vy = sp * (w.cos_inflow * ty - w.sin_inflow * ry) + w.by;
+        }
+        speed = hypot(vx, vy);
+        angle = atan2(vy, vx);
+    }
+    double cd;
+    if (w.drag == RPL_DRAG_LINEAR) {
+        cd = (0.8 + 0.065 * speed) * 1e-3;
+    } else {
+        cd = (8.058 + 0.967 * speed - 0.016 * (speed * speed)) * 1e-4;
+        cd = cd > 4e-4 ? cd : 4e-4;
+    }
+    U10[ij] = (T)speed;
+    theta_u[ij] = (T)angle;
+    ustar[ij] = (T)(speed * sqrt(cd > 0 ? cd : 0.0));
+}
+
+// precomputed_sources.jl:126-170 (stress factor) and :41-74 (mean-spectrum
+// moments), thread per column, spectral loop n outer / m inner.  Consecutive
+// threads read consecutive i of each (m, n) plane.
+template <typename T>
+__global__ __launch_bounds__(256) void source_state_kernel(int nx, int ny, int nk, int nphi, const T *__restrict__ N,
+                                                           SourceParams<T> s, StateTables<T> tb, T *factor, T *m0o,
+                                                           T *kbaro, T *sbaro)
+{
+    const size_t plane = (size_t)nx * ny;
+    const size_t ij = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (ij >= plane)
+        return;
+    if (s.wind_input && s.stress_cap) {
+        const T U10 = s.U10[ij], ustar = s.ustar[ij];
+        T f = T(1);
+        if (U10 > T(0) && ustar > T(0)) {
+            const T theta_u = s.theta_u[ij];
+            const T tau_max = tb.rho_air * (ustar * ustar);
+            T twx = T(0), twy = T(0);
+            for (int n = 0; n < nphi; ++n)
+                for (int m = 0; m < nk; ++m) {
+                    const T a = N[ij + plane * ((size_t)m + (size_t)nk * n)];
+                    if (a > T(0)) {
+                        const T k = tb.kappa[m];
+                        if (k > T(0)) {
+                            const T sigma = tb.sigma_k[m], C = sigma / k;
+                            const T th = tb.phi[n];
+                            const T rate = jp_rate<T>(s, k, sigma, C, th, theta_u, ustar);
+                            if (rate > T(0)) {
+                                const T mflux = rate * a / C * tb.w[m + nk * n] * tb.rho_w_g;
+                                twx += mflux * tb.cos_phi[n];
+                                twy += mflux * tb.sin_phi[n];
+                            }
+                        }
+                    }
+                }
+            const T par = twx * dcos(theta_u) + twy * dsin(theta_u);
+            const T eff = par > T(0) ? par : T(0);
+            f = (eff > tb.cap_eps * tau_max && tau_max > T(0)) ? tb.cap_eps * tau_max / eff : T(1);
+        }
+        factor[ij] = f;
+    }
+    if (s.dissipation == 1) {
+        T m0 = T(0), kp = T(0), sp = T(0);
+        for (int n = 0; n < nphi; ++n)
+            for (int m = 0; m < nk; ++m) {
+                const T a = N[ij + plane * ((size_t)m + (size_t)nk * n)];
+                if (a > T(0)) {
+                    const T k = tb.kappa[m];
+                    if (k > T(0)) {
+                        const T wt = tb.w[m + nk * n];
+                        m0 += a * wt;
+                        kp += a * wt * tb.k_pow[m];
+                        sp += a * wt * tb.sigma_pow[m];
+                    }
+                }
+            }
+        m0o[ij] = m0;
+        if (m0 > T(0)) {
+            kbaro[ij] = dpow(kp / m0, tb.inv_p);
+            sbaro[ij] = dpow(sp / m0, tb.inv_p);
+        } else {
+            kbaro[ij] = T(0);
+            sbaro[ij] = T(0);
+        }
+    }
+}
+
+}  // namespace
+
+template <typename T>
+hipError_t launch_wind_columns(int nx, int ny, const double *xn, const double *yn, double t, const WindParams &w,
+                               T *U10, T *theta_u, T *ustar, hipStream_t s)
+{
+    const size_t plane = (size_t)nx * ny;
+    hipLaunchKernelGGL(wind_columns_kernel<T>, dim3((unsigned)((plane + 255) / 256)), dim3(256), 0, s, nx, ny, xn, yn,
+                       t, w, U10, theta_u, ustar);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_source_state(int nx, int ny, int nk, int nphi, const T *N, const SourceParams<T> &sp,
+                               const StateTables<T> &tb, T *factor, T *m0, T *kbar, T *sbar, hipStream_t s)
+{
+    const size_t plane = (size_t)nx * ny;
+    hipLaunchKernelGGL(source_state_kernel<T>, dim3((unsigned)((plane + 255) / 256)), dim3(256), 0, s, nx, ny, nk,
+                       nphi, N, sp, tb, factor, m0, kbar, sbar);
+    return hipGetLastError();
+}
+
+template hipError_t launch_wind_columns<float>(int, int, const double *, const double *, double, const WindParams &,
+                                              float *, float *, float *, hipStream_t);
+template hipError_t launch_wind_columns<double>(int, int, const double *, const double *, double, const WindParams &,
+                                               double *, double *, double *, hipStream_t);
+template hipError_t launch_source_state<float>(int, int, int, int, const float *, const SourceParams<float> &,
+                                               const StateTables<float> &, float *, float *, float *, float *,
+                                               hipStream_t);
+template hipError_t launch_source_state<double>(int, int, int, int, const double *, const SourceParams<double> &,
+                                                const StateTables<double> &, double *, double *, double *, double *,
+                                                hipStream_t);
+
+}  // namespace rpl

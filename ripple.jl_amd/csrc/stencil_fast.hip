@@ -145,7 +145,7 @@ __device__ __forceinline__ Tile tile_of(int b, int total, int nbx, int nby, int 
     return t;
 }
 
-template <int TRANSPORT, int STAGE>
+template <int TRANSPORT, int STAGE, bool SRC>
 __global__ __launch_bounds__(256) void stage_fast_kernel(const float *__restrict__ Nk, const float *__restrict__ N0,
                                                          float *out, StageParams<float> p, int nbx, int nby)
 {
@@ -253,6 +253,8 @@ __global__ __launch_bounds__(256) void stage_fast_kernel(const float *__restrict
             L = L - fmaf(dk, p.dk_inv, dp * p.dp_inv);
         }
     }
+    if (SRC)  // tendencies.jl:88-94, G += source_tendency
+        L += source_term<float>(p.src, off, m + nk * n, x0);
     if (p.src_add)
         L += ldo(p.src_add + plane * bin, bo);
     float *dst = out + plane * bin;
@@ -270,7 +272,7 @@ __global__ __launch_bounds__(256) void stage_fast_kernel(const float *__restrict
     *reinterpret_cast<float *>(reinterpret_cast<char *>(dst) + bo) = r;
 }
 
-template <int TRANSPORT>
+template <int TRANSPORT, bool SRC>
 hipError_t launch_t(const StageParams<float> &p, int stage, const float *Nk, const float *N0, float *out,
                     hipStream_t s)
 {
@@ -282,16 +284,16 @@ hipError_t launch_t(const StageParams<float> &p, int stage, const float *Nk, con
     dim3 grid((unsigned)total, 1, 1);
     switch (stage) {
     case STAGE_TENDENCY:
-        hipLaunchKernelGGL((stage_fast_kernel<TRANSPORT, STAGE_TENDENCY>), grid, block, 0, s, Nk, N0, out, p, nbx, nby);
+        hipLaunchKernelGGL((stage_fast_kernel<TRANSPORT, STAGE_TENDENCY, SRC>), grid, block, 0, s, Nk, N0, out, p, nbx, nby);
         break;
     case STAGE_1:
-        hipLaunchKernelGGL((stage_fast_kernel<TRANSPORT, STAGE_1>), grid, block, 0, s, Nk, N0, out, p, nbx, nby);
+        hipLaunchKernelGGL((stage_fast_kernel<TRANSPORT, STAGE_1, SRC>), grid, block, 0, s, Nk, N0, out, p, nbx, nby);
         break;
     case STAGE_2:
-        hipLaunchKernelGGL((stage_fast_kernel<TRANSPORT, STAGE_2>), grid, block, 0, s, Nk, N0, out, p, nbx, nby);
+        hipLaunchKernelGGL((stage_fast_kernel<TRANSPORT, STAGE_2, SRC>), grid, block, 0, s, Nk, N0, out, p, nbx, nby);
         break;
     default:
-        hipLaunchKernelGGL((stage_fast_kernel<TRANSPORT, STAGE_3>), grid, block, 0, s, Nk, N0, out, p, nbx, nby);
+        hipLaunchKernelGGL((stage_fast_kernel<TRANSPORT, STAGE_3, SRC>), grid, block, 0, s, Nk, N0, out, p, nbx, nby);
         break;
     }
     return hipGetLastError();
@@ -302,11 +304,18 @@ hipError_t launch_t(const StageParams<float> &p, int stage, const float *Nk, con
 hipError_t launch_stage_fast(const StageParams<float> &p, int transport, int stage, const float *Nk, const float *N0,
                              float *out, hipStream_t s)
 {
+    if (p.src.on) {
+        if (transport == RPL_TRANSPORT_REFRACTION)
+            return launch_t<RPL_TRANSPORT_REFRACTION, true>(p, stage, Nk, N0, out, s);
+        if (transport == RPL_TRANSPORT_INTRINSIC)
+            return launch_t<RPL_TRANSPORT_INTRINSIC, true>(p, stage, Nk, N0, out, s);
+        return launch_t<RPL_TRANSPORT_NONE, true>(p, stage, Nk, N0, out, s);
+    }
     if (transport == RPL_TRANSPORT_REFRACTION)
-        return launch_t<RPL_TRANSPORT_REFRACTION>(p, stage, Nk, N0, out, s);
+        return launch_t<RPL_TRANSPORT_REFRACTION, false>(p, stage, Nk, N0, out, s);
     if (transport == RPL_TRANSPORT_INTRINSIC)
-        return launch_t<RPL_TRANSPORT_INTRINSIC>(p, stage, Nk, N0, out, s);
-    return launch_t<RPL_TRANSPORT_NONE>(p, stage, Nk, N0, out, s);
+        return launch_t<RPL_TRANSPORT_INTRINSIC, false>(p, stage, Nk, N0, out, s);
+    return launch_t<RPL_TRANSPORT_NONE, false>(p, stage, Nk, N0, out, s);
 }
 
 }  // namespace rpl

@@ -60,7 +60,7 @@ __device__ __forceinline__ int clampi(int i, int n) { return i < 0 ? 0 : (i >= n
 template <typename T>
 __device__ __forceinline__ T clip0(T x) { return x < T(0) ? T(0) : x; }
 
-template <typename T, int TRANSPORT, int STAGE>
+template <typename T, int TRANSPORT, int STAGE, bool SRC>
 __global__ __launch_bounds__(256) void stage_reference_kernel(const T *__restrict__ Nk, const T *__restrict__ N0,
                                                               T *out, StageParams<T> p)
 {
@@ -131,8 +131,10 @@ __global__ __launch_bounds__(256) void stage_reference_kernel(const T *__restric
             L = -(flux_x + flux_y);
         }
     }
+    if (SRC)  // tendencies.jl:88-94, G += source_tendency
+        L = L + source_term<T>(p.src, (unsigned)(i + nx * j), m + nk * n, x0);
     if (p.src_add)
-        L = L + p.src_add[c];  // tendencies.jl:92, G += source_tendency
+        L = L + p.src_add[c];
 
     if (STAGE == STAGE_TENDENCY) {
         out[c] = L;
@@ -146,23 +148,23 @@ __global__ __launch_bounds__(256) void stage_reference_kernel(const T *__restric
     }
 }
 
-template <typename T, int TRANSPORT>
+template <typename T, int TRANSPORT, bool SRC>
 hipError_t launch_t(const StageParams<T> &p, int stage, const T *Nk, const T *N0, T *out, hipStream_t s)
 {
     dim3 block(64, 4, 1);
     dim3 grid((p.nx + 63) / 64, (p.ny + 3) / 4, p.nk * p.nphi);
     switch (stage) {
     case STAGE_TENDENCY:
-        hipLaunchKernelGGL((stage_reference_kernel<T, TRANSPORT, STAGE_TENDENCY>), grid, block, 0, s, Nk, N0, out, p);
+        hipLaunchKernelGGL((stage_reference_kernel<T, TRANSPORT, STAGE_TENDENCY, SRC>), grid, block, 0, s, Nk, N0, out, p);
         break;
     case STAGE_1:
-        hipLaunchKernelGGL((stage_reference_kernel<T, TRANSPORT, STAGE_1>), grid, block, 0, s, Nk, N0, out, p);
+        hipLaunchKernelGGL((stage_reference_kernel<T, TRANSPORT, STAGE_1, SRC>), grid, block, 0, s, Nk, N0, out, p);
         break;
     case STAGE_2:
-        hipLaunchKernelGGL((stage_reference_kernel<T, TRANSPORT, STAGE_2>), grid, block, 0, s, Nk, N0, out, p);
+        hipLaunchKernelGGL((stage_reference_kernel<T, TRANSPORT, STAGE_2, SRC>), grid, block, 0, s, Nk, N0, out, p);
         break;
     default:
-        hipLaunchKernelGGL((stage_reference_kernel<T, TRANSPORT, STAGE_3>), grid, block, 0, s, Nk, N0, out, p);
+        hipLaunchKernelGGL((stage_reference_kernel<T, TRANSPORT, STAGE_3, SRC>), grid, block, 0, s, Nk, N0, out, p);
         break;
     }
     return hipGetLastError();
@@ -172,11 +174,18 @@ template <typename T>
 hipError_t launch_any(const StageParams<T> &p, int transport, int stage, const T *Nk, const T *N0, T *out,
                       hipStream_t s)
 {
+    if (p.src.on) {
+        if (transport == RPL_TRANSPORT_REFRACTION)
+            return launch_t<T, RPL_TRANSPORT_REFRACTION, true>(p, stage, Nk, N0, out, s);
+        if (transport == RPL_TRANSPORT_INTRINSIC)
+            return launch_t<T, RPL_TRANSPORT_INTRINSIC, true>(p, stage, Nk, N0, out, s);
+        return launch_t<T, RPL_TRANSPORT_NONE, true>(p, stage, Nk, N0, out, s);
+    }
     if (transport == RPL_TRANSPORT_REFRACTION)
-        return launch_t<T, RPL_TRANSPORT_REFRACTION>(p, stage, Nk, N0, out, s);
+        return launch_t<T, RPL_TRANSPORT_REFRACTION, false>(p, stage, Nk, N0, out, s);
     if (transport == RPL_TRANSPORT_INTRINSIC)
-        return launch_t<T, RPL_TRANSPORT_INTRINSIC>(p, stage, Nk, N0, out, s);
-    return launch_t<T, RPL_TRANSPORT_NONE>(p, stage, Nk, N0, out, s);
+        return launch_t<T, RPL_TRANSPORT_INTRINSIC, false>(p, stage, Nk, N0, out, s);
+    return launch_t<T, RPL_TRANSPORT_NONE, false>(p, stage, Nk, N0, out, s);
 }
 
 }  // namespace

@@ -12,6 +12,8 @@ from .grids import (RectilinearGrid, PolarWaveVectorGrid, FrequencyDirectionGrid
                     xspacings, yspacings, zspacings, PERIODIC, BOUNDED, FLAT)
 from .model import (SpectralWaveModel, InfiniteDepth, Clock, Simulation, WENO, set_, time_step_, run_,
                     interior)
+from .sources import (BulkWindDrag, LinearStormTrack, HollandHurricaneWind, PressureCorrelationInput,
+                      MeanSpectrumWhitecapping, LocalSaturationDissipation, PrecomputedSources, wind_state)
 from .diagnostics import (m0, root_mean_square_wavenumber, mean_direction, significant_wave_height,
                           total_action, total_deep_water_energy)
 
@@ -23,4 +25,6 @@ __all__ = [
     "SpectralWaveModel", "InfiniteDepth", "Clock", "Simulation", "WENO", "set_", "time_step_", "run_",
     "interior", "m0", "root_mean_square_wavenumber", "mean_direction", "significant_wave_height",
     "total_action", "total_deep_water_energy",
+    "BulkWindDrag", "LinearStormTrack", "HollandHurricaneWind", "PressureCorrelationInput",
+    "MeanSpectrumWhitecapping", "LocalSaturationDissipation", "PrecomputedSources", "wind_state",
 ]

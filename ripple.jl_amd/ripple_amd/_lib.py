@@ -47,6 +47,19 @@ class rpl_config(ctypes.Structure):
     ]
 
 
+class rpl_sources(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int32) for n in ("wind_input", "drag", "wind_field", "stress_cap", "dissipation",
+                                               "rotation")] + \
+               [(n, ctypes.c_double) for n in (
+                   "wind_speed", "wind_direction",
+                   "beta_max", "z_alpha", "p_in", "alpha0", "rho_air", "rho_water", "von_karman", "gravity",
+                   "cap_epsilon",
+                   "vmax", "rmax", "radius", "shape", "inflow_angle", "background_x", "background_y",
+                   "track_t0", "track_t1", "track_x0", "track_y0", "track_x1", "track_y1",
+                   "cds", "delta1", "delta2", "p_mean",
+                   "ls_cds", "ls_br", "ls_p", "ls_sigma_power")]
+
+
 _P = ctypes.c_void_p
 _D = ctypes.c_double
 _DP = ctypes.POINTER(ctypes.c_double)
@@ -70,6 +83,8 @@ SIGNATURES = {
     "rpl_set_clock": ([_P, _D, ctypes.c_int64], ctypes.c_int),
     "rpl_diagnostics": ([_P, _DP, _DP, _DP], ctypes.c_int),
     "rpl_conservation": ([_P, _DP, _DP], ctypes.c_int),
+    "rpl_set_sources": ([_P, ctypes.POINTER(rpl_sources), _DP, _DP], ctypes.c_int),
+    "rpl_get_wind": ([_P, _DP, _DP, _DP], ctypes.c_int),
     "rpl_enable_kernel_timing": ([_P, _I32], ctypes.c_int),
     "rpl_get_kernel_timing": ([_P, _DP, _I64P], ctypes.c_int),
     "rpl_stream": ([_P], ctypes.c_void_p),

@@ -127,7 +127,8 @@ class SpectralWaveModel:
             self._set_velocities(velocities)
         _lib.check(lib.rpl_set_clock(ctx, self.clock.time, self.clock.iteration))
         if sources is not None:
-            sources.attach(self)
+            from .sources import attach
+            attach(self, sources)
 
     # --- coupling -----------------------------------------------------------
     def _set_velocities(self, velocities):
@@ -231,8 +232,6 @@ def time_step_(model, dt, nsteps=1):
     """`time_step!(model, dt)`, nsteps times, on the device."""
     if not dt > 0:
         raise _lib.RippleArgumentError(-1, "time step must be positive")
-    if model.sources is not None:
-        model.sources.before_step(model, dt)
     _lib.check(model._ctx.lib.rpl_time_step(model._ctx.ptr, float(dt), int(nsteps)))
     t = ctypes.c_double()
     it = ctypes.c_int64()
@@ -261,10 +260,6 @@ def run_(sim):
     else:
         raise ValueError("Simulation needs stop_iteration or stop_time")
     if n > 0:
-        if m.sources is not None and getattr(m.sources, "per_step_host_work", False):
-            for _ in range(n):
-                time_step_(m, sim.dt)
-        else:
-            time_step_(m, sim.dt, n)
+        time_step_(m, sim.dt, n)
     m.synchronize()
     return sim

@@ -1,0 +1,171 @@
+"""Source terms and winds with the reference's constructors and defaults.
+
+PressureCorrelationInput, BulkWindDrag, MeanSpectrumWhitecapping,
+LocalSaturationDissipation and PrecomputedSources follow
+Physics/WindInput/pressure_correlation.jl:24-52, Physics/shared/drag.jl:21-34,
+Physics/Dissipation/mean_spectrum.jl:20-35, local_saturation.jl:18-28 and
+Physics/Packages/precomputed_sources.jl:23-35; HollandHurricaneWind and
+LinearStormTrack follow Forcing/winds.jl:4-41, 89-118.  Everything is
+evaluated on the device (rpl_set_sources); these classes only carry
+parameters.
+"""
+import ctypes
+import math
+
+import numpy as np
+
+from . import _lib
+from .grids import xnodes, ynodes
+
+
+rpl_sources = _lib.rpl_sources
+
+
+class BulkWindDrag:
+    """`BulkWindDrag(:linear)` (Wu 1982) or `BulkWindDrag(:capped)` (Hwang 2001)."""
+
+    def __init__(self, form="linear"):
+        if form not in ("linear", "capped"):
+            raise ValueError(f"unsupported drag form {form}")
+        self.form = form
+
+    def friction_velocity(self, U10):
+        if self.form == "linear":
+            cd = (0.8 + 0.065 * U10) * 1e-3
+        else:
+            cd = max((8.058 + 0.967 * U10 - 0.016 * U10 ** 2) * 1e-4, 4e-4)
+        return U10 * math.sqrt(max(cd, 0.0))
+
+
+class LinearStormTrack:
+    """`LinearStormTrack(times, centers)`; two points (a constant translation)."""
+
+    def __init__(self, times, centers):
+        t = [float(x) for x in times]
+        c = [(float(a), float(b)) for a, b in centers]
+        if len(t) != len(c):
+            raise ValueError("storm-track times and centers must have the same length")
+        if len(t) != 2:
+            raise NotImplementedError("the device wind kernel takes a two-point storm track")
+        if not t[1] > t[0]:
+            raise ValueError("storm-track times must be strictly increasing")
+        self.times, self.centers = t, c
+
+
+class HollandHurricaneWind:
+    """`HollandHurricaneWind(; center, vmax, rmax, radius, shape_parameter,
+    inflow_angle, background, rotation)` with center a (x, y) tuple or a
+    LinearStormTrack."""
+
+    def __init__(self, center=(0.0, 0.0), vmax=1.0, rmax=1.0, radius=4.0, shape_parameter=1.5, inflow_angle=0.0,
+                 background=(0.0, 0.0), rotation="counterclockwise"):
+        if not vmax >= 0:
+            raise ValueError("Holland hurricane vmax must be nonnegative")
+        if not rmax > 0:
+            raise ValueError("Holland hurricane rmax must be positive")
+        if not radius > rmax:
+            raise ValueError("Holland hurricane radius must exceed rmax")
+        if not shape_parameter > 0:
+            raise ValueError("Holland hurricane shape_parameter must be positive")
+        if isinstance(center, LinearStormTrack):
+            self.track = center
+        else:
+            self.track = LinearStormTrack([0.0, 1.0], [center, center])
+        self.vmax, self.rmax, self.radius = float(vmax), float(rmax), float(radius)
+        self.shape, self.inflow_angle = float(shape_parameter), float(inflow_angle)
+        self.background = (float(background[0]), float(background[1]))
+        self.rotation = 1 if rotation == "counterclockwise" else -1
+
+
+class PressureCorrelationInput:
+    def __init__(self, *, drag, wind, direction=0.0, beta_max=1.2, z_alpha=0.011, p_in=2.0, alpha0=0.0095,
+                 rho_air=1.225, rho_water=1025.0, von_karman=0.4, gravity=9.81):
+        self.drag, self.wind, self.direction = drag, wind, float(direction)
+        self.beta_max, self.z_alpha, self.p_in, self.alpha0 = float(beta_max), float(z_alpha), float(p_in), float(alpha0)
+        self.rho_air, self.rho_water = float(rho_air), float(rho_water)
+        self.von_karman, self.gravity = float(von_karman), float(gravity)
+
+
+class MeanSpectrumWhitecapping:
+    def __init__(self, C_ds=-2.1, delta1=0.4, delta2=0.6, p=0.5, gravity=9.81):
+        self.C_ds, self.delta1, self.delta2, self.p, self.gravity = (float(v) for v in (C_ds, delta1, delta2, p,
+                                                                                        gravity))
+
+
+class LocalSaturationDissipation:
+    def __init__(self, C_ds=-2.2e-5, B_r=9.0e-4, p=2.0, sigma_power=1.0, gravity=9.81):
+        self.C_ds, self.B_r, self.p, self.sigma_power, self.gravity = (float(v) for v in (C_ds, B_r, p, sigma_power,
+                                                                                           gravity))
+
+
+class PrecomputedSources:
+    """`PrecomputedSources(; wind_input, dissipation)`: the bundle whose wind
+    input carries the wave-supported-stress cap."""
+
+    def __init__(self, wind_input=None, dissipation=None):
+        self.wind_input, self.dissipation = wind_input, dissipation
+
+
+def to_struct(sources):
+    """rpl_sources for a bundle, or for a single term (no stress cap)."""
+    s = rpl_sources()
+    s.rotation = 1
+    s.cap_epsilon = 0.4
+    s.gravity = 9.81
+    s.rho_water = 1025.0
+    s.von_karman = 0.4
+    s.p_mean = 0.5
+    if isinstance(sources, PrecomputedSources):
+        wi, diss, cap = sources.wind_input, sources.dissipation, 1
+    elif isinstance(sources, PressureCorrelationInput):
+        wi, diss, cap = sources, None, 0
+    elif isinstance(sources, (MeanSpectrumWhitecapping, LocalSaturationDissipation)):
+        wi, diss, cap = None, sources, 0
+    else:
+        raise TypeError(f"unsupported sources {type(sources).__name__}")
+    if isinstance(diss, MeanSpectrumWhitecapping) and not isinstance(sources, PrecomputedSources):
+        raise NotImplementedError("MeanSpectrumWhitecapping runs through the PrecomputedSources bundle state")
+    if wi is not None:
+        s.wind_input = 1
+        s.drag = 0 if wi.drag.form == "linear" else 1
+        s.stress_cap = cap
+        for n in ("beta_max", "z_alpha", "p_in", "alpha0", "rho_air", "rho_water", "von_karman", "gravity"):
+            setattr(s, n, getattr(wi, n))
+        if isinstance(wi.wind, HollandHurricaneWind):
+            w = wi.wind
+            s.wind_field = 1
+            s.vmax, s.rmax, s.radius, s.shape, s.inflow_angle = w.vmax, w.rmax, w.radius, w.shape, w.inflow_angle
+            s.background_x, s.background_y = w.background
+            s.rotation = w.rotation
+            s.track_t0, s.track_t1 = w.track.times
+            (s.track_x0, s.track_y0), (s.track_x1, s.track_y1) = w.track.centers
+        else:
+            s.wind_field = 0
+            s.wind_speed = float(wi.wind)
+            s.wind_direction = wi.direction
+    if isinstance(diss, MeanSpectrumWhitecapping):
+        s.dissipation = 1
+        s.cds, s.delta1, s.delta2, s.p_mean = diss.C_ds, diss.delta1, diss.delta2, diss.p
+    elif isinstance(diss, LocalSaturationDissipation):
+        s.dissipation = 2
+        s.ls_cds, s.ls_br, s.ls_p, s.ls_sigma_power = diss.C_ds, diss.B_r, diss.p, diss.sigma_power
+    return s
+
+
+def attach(model, sources):
+    """Install `sources` on the model's device context."""
+    lib = _lib.load()
+    s = to_struct(sources)
+    xn = np.ascontiguousarray(xnodes(model.grid), dtype=np.float64)
+    yn = np.ascontiguousarray(ynodes(model.grid), dtype=np.float64)
+    _lib.check(lib.rpl_set_sources(model._ctx.ptr, ctypes.byref(s), _lib.dptr(xn), _lib.dptr(yn)))
+    return s
+
+
+def wind_state(model):
+    """(U10, theta_u, u_star) per column at the model clock, each (Nx, Ny)."""
+    Nx, Ny = model.size[0], model.size[1]
+    out = [np.zeros(Nx * Ny) for _ in range(3)]
+    lib = _lib.load()
+    _lib.check(lib.rpl_get_wind(model._ctx.ptr, *[_lib.dptr(a) for a in out]))
+    return [a.reshape((Nx, Ny), order="F") for a in out]

@@ -108,3 +108,73 @@ def sheared_eddies(float_type=np.float32, nx=512, ny=512, nz=32, nkappa=32, nphi
     v = np.asfortranarray(v2[:, :, None] * decay[None, None, :])
     return Workload(f"sheared_eddies_{nx}x{ny}x{nz}_k{nkappa}_p{nphi}", grid, sg, (u, v),
                     narrow_band_action(grid, sg), 0.02, nsteps, dict(timestepper="RK3"))
+
+
+def jonswap_action(grid, sg, Hs, fp, gravity=9.81, gamma=3.3):
+    """A JONSWAP-shaped seed with cos^2 directional spreading about phi = 0,
+    expressed as action density on the spectral cells and scaled so that the
+    diagnostic 4 sqrt(m0) of the action equals Hs (the reference's
+    significant_wave_height, Diagnostics/bulk_statistics.jl:9)."""
+    f = np.sqrt(gravity * sg.kappa) / (2 * math.pi)
+    sig = np.where(f <= fp, 0.07, 0.09)
+    r = np.exp(-((f - fp) ** 2) / (2 * sig ** 2 * fp ** 2))
+    S = f ** -5.0 * np.exp(-1.25 * (fp / f) ** 4) * gamma ** r
+    # E(f) df = E(kappa) kappa dkappa, df/dkappa = g / (8 pi^2 f) for deep water.
+    Ek = S * gravity / (8 * math.pi ** 2 * f) / sg.kappa
+    D = np.where(np.cos(sg.phi) > 0, np.cos(sg.phi) ** 2, 0.0)
+    col = Ek[:, None] * D[None, :]
+    m0 = float(np.sum(col * sg.weights))
+    col *= (Hs / 4) ** 2 / m0
+    N = np.broadcast_to(col[None, None], (grid.Nx, grid.Ny) + col.shape)
+    return np.asfortranarray(N, dtype=np.float64)
+
+
+def fetch_limited_growth(float_type=np.float32, nx=256, ny=256, nz=16, nfreq=40, nphi=36, dt=10.0,
+                         hours=6.0):
+    """256^2 x 16 periodic box of 100 km x 100 km, depth 50 m, no current;
+    FrequencyDirectionGrid of 40 log-spaced frequencies 0.05-2 Hz x 36
+    directions; uniform U10 = 10 m/s along x; PrecomputedSources of
+    PressureCorrelationInput (linear bulk drag, stress cap) and
+    MeanSpectrumWhitecapping; JONSWAP seed Hs = 0.05 m, fp = 0.5 Hz, cos^2
+    spreading; SSP-RK3 with dt = 10 s for 6 h (2160 steps); WENO5 intrinsic
+    transport at the finite-depth c_g."""
+    from .sources import PrecomputedSources, PressureCorrelationInput, MeanSpectrumWhitecapping, BulkWindDrag
+    L = 100e3
+    grid = RectilinearGrid(float_type=float_type, size=(nx, ny, nz), x=(0, L), y=(0, L), z=(-50.0, 0.0),
+                           topology=(PERIODIC, PERIODIC, BOUNDED))
+    sg = FrequencyDirectionGrid(float_type, frequency=np.geomspace(0.05, 2.0, nfreq),
+                                phi=np.linspace(0, 2 * math.pi, nphi + 1)[:nphi])
+    sources = PrecomputedSources(wind_input=PressureCorrelationInput(drag=BulkWindDrag("linear"), wind=10.0,
+                                                                     direction=0.0),
+                                 dissipation=MeanSpectrumWhitecapping())
+    N0 = jonswap_action(grid, sg, 0.05, 0.5)
+    return Workload(f"fetch_limited_growth_{nx}x{ny}x{nz}_f{nfreq}_p{nphi}", grid, sg, None, N0, dt,
+                    int(round(hours * 3600 / dt)),
+                    dict(timestepper="RK3", sources=sources, depth=50.0))
+
+
+def translating_hurricane(float_type=np.float32, nx=1024, ny=1024, nz=32, nkappa=32, nphi=36, dt=60.0,
+                          nsteps=60):
+    """1024^2 x 32 periodic box of 800 km x 800 km, depth 200 m; polar kappa
+    32 bins log-spaced 0.05-2 rad/m x 36 directions; Holland hurricane (vmax
+    50 m/s, rmax 30 km, B = 1.5, radius 300 km, counter-clockwise, 20 deg
+    inflow) translating along x at 5 m/s from x = 200 km on the centre line;
+    PrecomputedSources of PressureCorrelationInput (linear bulk drag) and
+    LocalSaturationDissipation (B_r = 1.05e-2) as in the reference's
+    translating-hurricane example; dt = 60 s; WENO5 intrinsic transport."""
+    from .sources import (PrecomputedSources, PressureCorrelationInput, LocalSaturationDissipation, BulkWindDrag,
+                          HollandHurricaneWind, LinearStormTrack)
+    L = 800e3
+    grid = RectilinearGrid(float_type=float_type, size=(nx, ny, nz), x=(0, L), y=(-L / 2, L / 2),
+                           z=(-200.0, 0.0), topology=(PERIODIC, PERIODIC, BOUNDED))
+    sg = PolarWaveVectorGrid(float_type, kappa=np.geomspace(0.05, 2.0, nkappa),
+                             phi=np.linspace(0, 2 * math.pi, nphi + 1)[:nphi])
+    T = 10 * 86400.0
+    track = LinearStormTrack([0.0, T], [(200e3, 0.0), (200e3 + 5.0 * T, 0.0)])
+    hurricane = HollandHurricaneWind(center=track, vmax=50.0, rmax=30e3, radius=300e3, shape_parameter=1.5,
+                                     inflow_angle=math.radians(20.0))
+    sources = PrecomputedSources(wind_input=PressureCorrelationInput(drag=BulkWindDrag("linear"), wind=hurricane),
+                                 dissipation=LocalSaturationDissipation(B_r=1.05e-2))
+    N0 = jonswap_action(grid, sg, 0.5, 0.15)
+    return Workload(f"translating_hurricane_{nx}x{ny}x{nz}_k{nkappa}_p{nphi}", grid, sg, None, N0, dt, nsteps,
+                    dict(timestepper="RK3", sources=sources, depth=200.0))
