@@ -31,6 +31,64 @@ __global__ __launch_bounds__(256) void doppler_sequential_kernel(int nx, int ny,
     Uy[ij + plane * m] = ay;
 }
 
+// The same contraction on the matrix cores: U[ij, m] = sum_k u[ij, k] W[k, m]
+// is an [nx*ny x nz] x [nz x nk] product (M = nx*ny rows, K = nz, N = nk).
+// One wavefront owns a 16 x 16 tile of U (16 columns ij, 16 bins m) for both
+// u and v, and walks K in steps of 4 with v_mfma_*_16x16x4 (FP32 inputs with
+// FP32 accumulation, or FP64).  Operand layout of the 16x16x4 instructions
+// (measured on gfx950): lane l holds A[l % 16][l / 16] and B[l / 16][l % 16];
+// accumulator register r of lane l is D[4 (l / 16) + r][l % 16] for the FP32
+// instruction and D[(l / 16) + 4 r][l % 16] for the FP64 one.  Out-of-range
+// rows, columns and k contribute zeros; stores are guarded.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ f32x4 mfma16x16x4(float a, float b, f32x4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f64x4 mfma16x16x4(double a, double b, f64x4 c)
+{
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+template <typename T, typename V>
+__global__ __launch_bounds__(256) void doppler_mfma_kernel(int nx, int ny, int nz, int nk, const T *__restrict__ u,
+                                                           const T *__restrict__ v, const T *__restrict__ W, T *Ux,
+                                                           T *Uy)
+{
+    const size_t M = (size_t)nx * ny;
+    const int lane = threadIdx.x & 63;
+    const size_t tile_row = ((size_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 16;
+    const int tile_col = blockIdx.y * 16;
+    if (tile_row >= M)
+        return;  // whole wavefront leaves together
+    const int r16 = lane & 15, q = lane >> 4;
+    const size_t arow = tile_row + r16;
+    const int bcol = tile_col + r16;
+    V accu = {0, 0, 0, 0}, accv = {0, 0, 0, 0};
+    for (int k0 = 0; k0 < nz; k0 += 4) {
+        const int k = k0 + q;
+        const bool ka = k < nz;
+        const T au = (ka && arow < M) ? u[arow + M * k] : T(0);
+        const T av = (ka && arow < M) ? v[arow + M * k] : T(0);
+        const T b = (ka && bcol < nk) ? W[k + (size_t)nz * bcol] : T(0);
+        accu = mfma16x16x4(au, b, accu);
+        accv = mfma16x16x4(av, b, accv);
+    }
+    const int col = tile_col + r16;
+    if (col < nk) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const size_t row = sizeof(T) == 4 ? tile_row + 4 * q + r : tile_row + q + 4 * r;
+            if (row < M) {
+                Ux[row + M * col] = accu[r];
+                Uy[row + M * col] = accv[r];
+            }
+        }
+    }
+}
+
 // wave_current_refraction.jl:7-19.
 template <typename T>
 __global__ __launch_bounds__(256) void gradients_kernel(int nx, int ny, int nk, const T *__restrict__ Ux,
@@ -83,11 +141,29 @@ hipError_t launch_doppler(int nx, int ny, int nz, int nk, const double *u, const
 hipError_t launch_doppler(int nx, int ny, int nz, int nk, const float *u, const float *v, const float *W,
                           float *Ux, float *Uy, hipStream_t s)
 {
-    size_t plane = (size_t)nx * ny;
-    dim3 grid((unsigned)((plane + 255) / 256), nk);
-    hipLaunchKernelGGL(doppler_sequential_kernel<float>, grid, dim3(256), 0, s, nx, ny, nz, nk, u, v, W, Ux, Uy);
+    return launch_doppler_mfma(nx, ny, nz, nk, u, v, W, Ux, Uy, s);
+}
+
+template <typename T>
+hipError_t launch_doppler_mfma(int nx, int ny, int nz, int nk, const T *u, const T *v, const T *W, T *Ux, T *Uy,
+                               hipStream_t s)
+{
+    const size_t M = (size_t)nx * ny;
+    const size_t tiles = (M + 15) / 16;
+    dim3 grid((unsigned)((tiles + 3) / 4), (unsigned)((nk + 15) / 16));
+    if (sizeof(T) == 4)
+        hipLaunchKernelGGL((doppler_mfma_kernel<float, f32x4>), grid, dim3(256), 0, s, nx, ny, nz, nk,
+                           (const float *)u, (const float *)v, (const float *)W, (float *)Ux, (float *)Uy);
+    else
+        hipLaunchKernelGGL((doppler_mfma_kernel<double, f64x4>), grid, dim3(256), 0, s, nx, ny, nz, nk,
+                           (const double *)u, (const double *)v, (const double *)W, (double *)Ux, (double *)Uy);
     return hipGetLastError();
 }
+
+template hipError_t launch_doppler_mfma<float>(int, int, int, int, const float *, const float *, const float *, float *,
+                                               float *, hipStream_t);
+template hipError_t launch_doppler_mfma<double>(int, int, int, int, const double *, const double *, const double *,
+                                                double *, double *, hipStream_t);
 
 template <typename T>
 hipError_t launch_gradients(int nx, int ny, int nk, const T *Ux, const T *Uy, T dx_inv, T dy_inv, T *Uxx, T *Uxy,

@@ -348,8 +348,6 @@ int rpl_create(rpl_ctx **out, const rpl_config *cfg)
         return fail(RPL_ERR_ARG, "unsupported transport");
     if (k.kernel < RPL_KERNEL_AUTO || k.kernel > RPL_KERNEL_FAST)
         return fail(RPL_ERR_ARG, "unsupported kernel selector");
-    if (k.kernel == RPL_KERNEL_FAST && k.dtype != RPL_F32)
-        return fail(RPL_ERR_UNSUPPORTED, "the FAST stencil is FP32 only");
     if ((long long)k.nkappa * k.nphi > 65535)
         return fail(RPL_ERR_UNSUPPORTED, "nkappa * nphi must be <= 65535");
     if ((long long)k.nx * k.ny >= (1LL << 30))
@@ -591,7 +589,10 @@ int rpl_set_current(rpl_ctx *c, int32_t nz, const double *z_faces, const double 
         rc = upload_real(c, W.data(), dW, W.size());
     if (!rc) {
         hipError_t e;
-        if (is_f64(c))
+        if (is_f64(c) && c->cfg.kernel == RPL_KERNEL_FAST)
+            e = rpl::launch_doppler_mfma<double>(nx, ny, nz, nk, (const double *)du, (const double *)dv,
+                                                 (const double *)dW, (double *)c->Ux, (double *)c->Uy, c->stream);
+        else if (is_f64(c))
             e = rpl::launch_doppler(nx, ny, nz, nk, (const double *)du, (const double *)dv, (const double *)dW,
                                     (double *)c->Ux, (double *)c->Uy, c->stream);
         else

@@ -74,6 +74,9 @@ hipError_t launch_doppler(int nx, int ny, int nz, int nk, const double *u, const
 hipError_t launch_doppler(int nx, int ny, int nz, int nk, const float *u, const float *v,
                           const float *W, float *Ux, float *Uy, hipStream_t s);
 template <typename T>
+hipError_t launch_doppler_mfma(int nx, int ny, int nz, int nk, const T *u, const T *v, const T *W, T *Ux, T *Uy,
+                               hipStream_t s);
+template <typename T>
 hipError_t launch_gradients(int nx, int ny, int nk, const T *Ux, const T *Uy, T dx_inv, T dy_inv,
                             T *Uxx, T *Uxy, T *Uyx, T *Uyy, hipStream_t s);
 

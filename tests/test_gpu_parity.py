@@ -65,6 +65,38 @@ def test_doppler_velocity_fp64_matches_oracle(oracle):
     np.testing.assert_array_equal(Uy.ravel(order="F"), om.Uy)
 
 
+def test_doppler_velocity_mfma(oracle):
+    """The matrix-core contraction (FP32 always, FP64 under kernel="fast")
+    against the oracle's sequential sum: FP64 to 1e-14 relative, FP32 to its
+    accumulation rounding."""
+    say("doppler mfma fp64/fp32 sheared eddies 96x80x20 k21")
+    wl = workloads.sheared_eddies(float_type=np.float64, nx=96, ny=80, nz=20, nkappa=21, nphi=6)
+    om = oracle.OracleModel(wl)
+    Uxo = om.Ux.reshape((96, 80, 21), order="F")
+    Uyo = om.Uy.reshape((96, 80, 21), order="F")
+    scale = max(np.abs(Uxo).max(), np.abs(Uyo).max())
+    m64 = device_model(wl, np.float64, kernel="fast")
+    Ux, Uy = m64.doppler_velocity()
+    assert np.max(np.abs(Ux - Uxo)) <= 1e-14 * scale and np.max(np.abs(Uy - Uyo)) <= 1e-14 * scale
+    m32 = device_model(wl, np.float32)
+    Ux, Uy = m32.doppler_velocity()
+    assert np.max(np.abs(Ux - Uxo)) <= 2e-6 * scale and np.max(np.abs(Uy - Uyo)) <= 2e-6 * scale
+
+
+def test_vortex_refraction_fp64_mfma_contraction(oracle):
+    """Config 1 in FP64 with the matrix-core Doppler contraction: the 400-step
+    run still matches the oracle to 1e-12 relative L2."""
+    say("vortex fp64 400 steps, mfma contraction")
+    wl = workloads.vortex_refraction()
+    m = device_model(wl, np.float64, kernel="fast")
+    ra.run_(ra.Simulation(m, dt=wl.dt, stop_iteration=wl.nsteps))
+    om = oracle.OracleModel(wl)
+    om.step(wl.dt, wl.nsteps)
+    err = rel_l2(m.action, om.action)
+    say(f"vortex fp64 (mfma contraction) rel-L2 = {err:.3e}")
+    assert err <= 1e-12
+
+
 def test_tendency_fp64_reference_kernel_bitwise(oracle):
     say("tendency fp64 random 19x13x7x6")
     wl = random_workload(19, 13, 7, 6, seed=1)

@@ -38,7 +38,7 @@ def test_create_rejects_bad_configs_without_touching_the_device():
         ptr = ctypes.c_void_p()
         rc = lib.rpl_create(ctypes.byref(ptr), ctypes.byref(c))
         assert rc == -1 and frag in lib.rpl_last_error().decode()
-    c = _lib.rpl_config(8, 8, 4, 8, _lib.RPL_F64, _lib.RPL_RK3, _lib.RPL_TRANSPORT_REFRACTION, _lib.RPL_KERNEL_FAST)
+    c = _lib.rpl_config(8, 8, 300, 300, _lib.RPL_F64, _lib.RPL_RK3, _lib.RPL_TRANSPORT_REFRACTION, 0)
     assert lib.rpl_create(ctypes.byref(ctypes.c_void_p()), ctypes.byref(c)) == -4
     assert lib.rpl_time_step(None, 0.1, 1) == -1
 
