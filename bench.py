@@ -67,6 +67,51 @@ def time_model(model, dt, steps, warmup):
     return (t1 - t0) / steps, ms.value / max(n.value, 1), n.value
 
 
+MLP_FLOP_PER_COLUMN = 2 * (1155 * 512 + 512 * 512 * 2 + 512 * 1152)  # 1155 -> 512 x3 -> 1152
+
+
+def storm_plain_vs_surrogate(steps, warmup):
+    """The translating-hurricane configuration at full size (1024 x 1024 cells
+    x 32 kappa x 36 phi, FP32, intrinsic WENO5 transport, Holland wind input
+    with stress cap and local-saturation whitecapping): RK3 step time with the
+    plain source bundle and with the 1155 -> 512 x 3 -> 1152 network supplying
+    the sources on every column.  The network weights are random (a timing
+    run: the work per step does not depend on them); the envelope is open so
+    that no column falls back."""
+    import ctypes
+    from ripple_amd import surrogate as sg
+    wl = workloads.translating_hurricane()
+    nx, ny, nk, nphi = wl.N0.shape
+    nb = nk * nphi
+    m = build_model(wl, np.float32)
+    del wl.N0
+    t_plain, l_plain, _ = time_model(m, wl.dt, steps, warmup)
+    net = sg.SourceNetwork(nb + 3, nb, hidden=512, hidden_layers=3, batch_capacity=64, seed=1)
+    cfg = sg.SurrogateConfig(rate_unit=1e-7)
+    sg.attach_surrogate(m, net, cfg)
+    lib, ctx = m._ctx.lib, m._ctx.ptr
+    ra.time_step_(m, wl.dt, 1)
+    m.synchronize()
+    _lib.check(lib.rpl_get_surrogate_timing(ctx, None, None))
+    t_surr, l_surr, _ = time_model(m, wl.dt, steps, 0)
+    ms, n = ctypes.c_double(), ctypes.c_int64()
+    _lib.check(lib.rpl_get_surrogate_timing(ctx, ctypes.byref(ms), ctypes.byref(n)))
+    st = sg.surrogate_stats(m)
+    t_inf = ms.value / max(n.value, 1)
+    flops = MLP_FLOP_PER_COLUMN * nx * ny
+    cells = nx * ny * nk * nphi
+    sg.attach_surrogate(m, None, None)
+    net.close()
+    return {"workload": f"translating_hurricane_{nx}x{ny}_k{nk}_p{nphi}", "dtype": "f32 state, bf16 network",
+            "plain": {"value": cells / t_plain, "ms_per_step": t_plain * 1e3, "stage_kernel_ms": l_plain},
+            "surrogate": {"value": cells / t_surr, "ms_per_step": t_surr * 1e3, "stage_kernel_ms": l_surr,
+                          "inference_ms_per_stage": t_inf,
+                          "mlp_TFLOPs": flops / (t_inf * 1e-3) / 1e12,
+                          "mlp_frac_of_bf16_dense_peak": flops / (t_inf * 1e-3) / 2.5e15,
+                          "fallback_columns": st["plain"], "columns": st["columns"]},
+            "step_speedup_surrogate_over_plain": t_plain / t_surr}
+
+
 def err_vs_fp64():
     """FP32 against FP64 after the 400-step vortex-refraction run (config 1)."""
     wl = workloads.vortex_refraction()
@@ -115,11 +160,24 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-extra", action="store_true", help="skip the FP64 and larger-grid lines")
+    ap.add_argument("--no-storm", action="store_true", help="skip the plain vs surrogate hurricane line")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device-to-device copy (reference for HBM byte counters) and exit")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the wave step is built and measured on a single MI355X; "
               "multi-GPU runs are not part of this build.", file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_calibration:
+        import torch
+        a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": a.numel() * 4, "copy_bytes_written": b.numel() * 4, "launches": 3}))
+        return
 
     wl = workloads.sheared_eddies()
     nx, ny, nk, nphi = wl.N0.shape
@@ -166,6 +224,8 @@ def main():
             "achieved_GBs": stage_bytes_per_launch(1024, 1024, nk, nphi, 4) / (lb * 1e-3) / 1e9}
         del mb
         extra["err_vs_fp64"] = err_vs_fp64()
+    if not args.no_storm:
+        extra["storm_plain_vs_surrogate"] = storm_plain_vs_surrogate(max(2, args.steps // 4), 1)
     record["extra"] = extra
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)

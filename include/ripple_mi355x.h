@@ -171,12 +171,67 @@ int rpl_set_sources(rpl_ctx *ctx, const rpl_sources *src, const double *x_nodes,
  * (nx, ny) FP64 each (the reference's _compute_wind_input_state host loop). */
 int rpl_get_wind(rpl_ctx *ctx, double *U10, double *theta_u, double *u_star);
 
+/* Learned source surrogate.  No counterpart in the reference: a small MLP
+ * (in_dim -> hidden x hidden_layers -> out_dim, GELU) stands in for the
+ * per-column source evaluation of the PrecomputedSources bundle above and is
+ * trained on that bundle's own tendencies.  BF16 weights and activations on
+ * the matrix cores with FP32 accumulation; FP32 master weights and Adam.
+ * Parameters (get/set, gradients) are the concatenation over layers of the
+ * padded W_l [out_pad_l x in_pad_l] (row-major) and b_l [out_pad_l]; padded
+ * dims are multiples of 64 (rpl_mlp_dims).  X rows are in_dim features, Y / T
+ * rows out_dim values, all FP32 on the host. */
+typedef struct rpl_mlp rpl_mlp;
+int rpl_mlp_create(rpl_mlp **out, int32_t in_dim, int32_t hidden, int32_t hidden_layers, int32_t out_dim,
+                   int32_t batch_capacity, uint64_t seed);
+int rpl_mlp_destroy(rpl_mlp *net);
+int rpl_mlp_dims(rpl_mlp *net, int32_t *in_pad, int32_t *out_pad, int64_t *nparams);
+int rpl_mlp_set_params(rpl_mlp *net, const float *params);
+int rpl_mlp_get_params(rpl_mlp *net, float *params);
+int rpl_mlp_forward(rpl_mlp *net, int32_t batch, const float *X, float *Y);
+/* Loss = mean over the batch and out_dim of (Y - T)^2; grads of the loss. */
+int rpl_mlp_gradients(rpl_mlp *net, int32_t batch, const float *X, const float *T, float *loss, float *grads);
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) on the batch. */
+int rpl_mlp_train_step(rpl_mlp *net, int32_t batch, const float *X, const float *T, float lr, float *loss);
+
+/* Surrogate use in the time step: on every stage the network maps each
+ * column's features to per-bin rates r (source = r * rate_unit * sigma_bin * N).
+ * Columns outside the training envelope (U10 outside [wind_lo, wind_hi], an
+ * action above action_hi) or with a non-finite output use the plain bundle,
+ * which must be installed with rpl_set_sources (with a wind input).  Every
+ * audit_stride-th column (0: none) is audited: its network source is compared
+ * with the plain bundle's, and where the relative L2 error exceeds
+ * audit_bound the column takes the plain source.  Features: (log(N + floor)
+ * - shift) * scale per bin, U10 (cos, sin) * wind_scale, depth_feature.
+ * net = NULL detaches.  The network must outlive its attachment. */
+typedef struct rpl_surrogate {
+    float floor, shift, scale, wind_scale, depth_feature;
+    float rate_unit;
+    float wind_lo, wind_hi, action_hi;
+    int32_t audit_stride;
+    float audit_bound;
+} rpl_surrogate;
+int rpl_attach_surrogate(rpl_ctx *ctx, rpl_mlp *net, const rpl_surrogate *cfg);
+/* Columns evaluated, of which by the plain bundle, audited, audit failures,
+ * and the largest audited error since the last call (then reset). */
+int rpl_surrogate_stats(rpl_ctx *ctx, int64_t *columns, int64_t *plain, int64_t *audited, int64_t *failed,
+                        double *err_max);
+/* The surrogate features of the current action and clock, (nx*ny) rows of
+ * nbins + 3 FP32 values (bf16-representable: what the network sees), and the
+ * plain bundle's per-cell rates r = source / (N sigma_bin rate_unit) as
+ * network targets (0 where N <= 0), (nx*ny) rows of nbins.  Either output
+ * may be NULL.  Column order is i fastest; nbins = nkappa * nphi, m fastest. */
+int rpl_surrogate_training_pairs(rpl_ctx *ctx, const rpl_surrogate *cfg, float *features, float *targets);
+
 /* Per-kernel timing of the fused stage kernel: when enabled, HIP events on
  * the context stream bracket every stage launch; the accumulated elapsed
  * milliseconds and launch count are read (and reset) with
  * rpl_get_kernel_timing, which synchronizes. */
 int rpl_enable_kernel_timing(rpl_ctx *ctx, int32_t enable);
 int rpl_get_kernel_timing(rpl_ctx *ctx, double *total_ms, int64_t *launches);
+/* The same for the surrogate inference of each stage input (features, the
+ * network forward pass over all chunks, and the envelope flags), under the
+ * same enable switch.  No counterpart in the reference. */
+int rpl_get_surrogate_timing(rpl_ctx *ctx, double *total_ms, int64_t *evaluations);
 
 /* The context's hipStream_t (for callers that order their own work after it). */
 void *rpl_stream(rpl_ctx *ctx);

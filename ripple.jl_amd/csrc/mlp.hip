@@ -1,0 +1,446 @@
+// Learned source surrogate: BF16 matrix-core GEMMs with FP32 accumulation and
+// fused epilogues, GELU, bias reductions, Adam on FP32 master weights, and the
+// column feature / fallback kernels that put the network into the RK3 stage.
+//
+// The reference has no learned component; this network stands in for the
+// per-column source evaluation of the PrecomputedSources bundle
+// (Physics/Packages/precomputed_sources.jl:240-261) and is trained on that
+// bundle's own tendencies.
+#include "mlp_internal.h"
+
+#include <cstdint>
+
+namespace rpl {
+namespace mlp {
+namespace {
+
+typedef short s16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float bf2f(bf16 v) { return __uint_as_float(((unsigned)v) << 16); }
+
+// Round to nearest even; NaN stays NaN.
+__device__ __forceinline__ bf16 f2bf(float f)
+{
+    unsigned u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u)
+        return (bf16)((u >> 16) | 0x40);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (bf16)(u >> 16);
+}
+
+__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+__device__ __forceinline__ float gelu_grad(float z)
+{
+    const float cdf = 0.5f * (1.0f + erff(z * 0.70710678118654752f));
+    const float pdf = 0.39894228040143268f * expf(-0.5f * z * z);
+    return cdf + z * pdf;
+}
+
+// C[m, n] = sum_k A(m, k) B(k, n), A(m, k) = A[m*sam + k*sak], B(k, n) = B[k*sbk + n*sbn],
+// bf16 operands, FP32 accumulation on v_mfma_f32_16x16x16bf16_1k.  Block tile
+// 64 x 64, four wavefronts of 32 x 32 (2 x 2 MFMA tiles), K in chunks of 32
+// staged through LDS as As[m][k] and Bs[n][k] (k contiguous, so each lane's
+// four-k operand is one 8-byte LDS read).  M, N multiples of 64, K of 32.
+// Operand layout (measured on gfx950): lane l, element e holds
+// A[l % 16][4 (l / 16) + e] and B[4 (l / 16) + e][l % 16]; accumulator
+// register r holds D[4 (l / 16) + r][l % 16].
+constexpr int BM = 64, BN = 64, BK = 32, LDK = BK + 16;  // rows padded to 96 bytes (16-byte aligned)
+
+template <bool K_CONTIG>
+__device__ __forceinline__ void stage_tile(bf16 (*S)[LDK], const bf16 *__restrict__ G, size_t s_row, size_t s_k,
+                                           int tid)
+{
+    if (K_CONTIG) {
+        // 64 rows x 32 k: each thread moves 8 consecutive k (16 bytes).
+        const int r = tid >> 2, k = (tid & 3) * 8;
+        const uint4 v = *reinterpret_cast<const uint4 *>(G + (size_t)r * s_row + k);
+        *reinterpret_cast<uint4 *>(&S[r][k]) = v;
+    } else {
+        // row index contiguous in memory: each thread moves 8 consecutive rows at one k.
+        const int k = tid >> 3, r = (tid & 7) * 8;
+        const uint4 v = *reinterpret_cast<const uint4 *>(G + (size_t)k * s_k + r);
+        const bf16 *e = reinterpret_cast<const bf16 *>(&v);
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            S[r + q][k] = e[q];
+    }
+}
+
+template <bool A_KC, bool B_KC, int EPI>
+__global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g)
+{
+    __shared__ bf16 As[BM][LDK];
+    __shared__ bf16 Bs[BN][LDK];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+    const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+    const bf16 *Ab = g.A + (size_t)m0 * g.sam;
+    const bf16 *Bb = g.B + (size_t)n0 * g.sbn;
+    f32x4 acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+            acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const int r16 = lane & 15, q = lane >> 4;
+    for (int k0 = 0; k0 < g.K; k0 += BK) {
+        __syncthreads();
+        stage_tile<A_KC>(As, Ab + (size_t)k0 * g.sak, g.sam, g.sak, tid);
+        stage_tile<B_KC>(Bs, Bb + (size_t)k0 * g.sbk, g.sbn, g.sbk, tid);
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 16) {
+            s16x4 af[2], bfr[2];
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+                af[a] = *reinterpret_cast<const s16x4 *>(&As[wm + a * 16 + r16][kk + 4 * q]);
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+                bfr[b] = *reinterpret_cast<const s16x4 *>(&Bs[wn + b * 16 + r16][kk + 4 * q]);
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b)
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af[a], bfr[b], acc[a][b], 0, 0, 0);
+        }
+    }
+    // Epilogue.
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int m = m0 + wm + a * 16 + 4 * q + r;
+                const int n = n0 + wn + b * 16 + r16;
+                float c = acc[a][b][r];
+                const size_t o = (size_t)m * g.scm + (size_t)n * g.scn;
+                if (EPI == EPI_STORE_F32) {
+                    g.Cf[o] = c;
+                } else if (EPI == EPI_BIAS_GELU) {
+                    const float z = c + g.bias[n];
+                    g.Cf[o] = z;
+                    g.Cb[o] = f2bf(gelu(z));
+                } else if (EPI == EPI_BIAS_GELU_INFER) {
+                    g.Cb[o] = f2bf(gelu(c + g.bias[n]));
+                } else if (EPI == EPI_BIAS_F32) {
+                    g.Cf[o] = c + g.bias[n];
+                } else if (EPI == EPI_BIAS_ROW_F32) {
+                    g.Cf[o] = c + g.bias[m];
+                } else {  // EPI_GELU_BACKWARD: dZ = dA * gelu'(Z)
+                    g.Cb[o] = f2bf(c * gelu_grad(g.Zin[o]));
+                }
+            }
+}
+
+__global__ __launch_bounds__(256) void column_sum_kernel(const bf16 *__restrict__ D, int M, int N, float *out)
+{
+    // out[n] = sum over m of D[m*N + n] (bias gradient).  Thread per n, rows in
+    // four interleaved partial sums for accuracy and ILP.
+    const int n = blockIdx.x * 256 + threadIdx.x;
+    if (n >= N)
+        return;
+    float s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+    int m = 0;
+    for (; m + 3 < M; m += 4) {
+        s0 += bf2f(D[(size_t)m * N + n]);
+        s1 += bf2f(D[(size_t)(m + 1) * N + n]);
+        s2 += bf2f(D[(size_t)(m + 2) * N + n]);
+        s3 += bf2f(D[(size_t)(m + 3) * N + n]);
+    }
+    for (; m < M; ++m)
+        s0 += bf2f(D[(size_t)m * N + n]);
+    out[n] = (s0 + s1) + (s2 + s3);
+}
+
+// Mean-squared error over the valid rows and output features, and its
+// gradient dY = 2 (Y - T) / count in bf16 (zero on padded rows / columns).
+__global__ __launch_bounds__(256) void mse_kernel(const float *__restrict__ Y, const float *__restrict__ T, int rows,
+                                                  int rows_valid, int cols, int cols_valid, float inv_count,
+                                                  bf16 *dY, float *loss_partial)
+{
+    __shared__ float red[256];
+    const size_t total = (size_t)rows * cols;
+    float s = 0;
+    for (size_t k = (size_t)blockIdx.x * 256 + threadIdx.x; k < total; k += (size_t)gridDim.x * 256) {
+        const int m = (int)(k / cols), n = (int)(k % cols);
+        float d = 0;
+        if (m < rows_valid && n < cols_valid) {
+            d = Y[k] - T[k];
+            s += d * d;
+        }
+        dY[k] = f2bf(2.0f * d * inv_count);
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h)
+            red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        loss_partial[blockIdx.x] = red[0];
+}
+
+// Adam (Kingma & Ba 2015) on FP32 master parameters, bias-corrected, writing
+// the bf16 copies the GEMMs read.
+__global__ __launch_bounds__(256) void adam_kernel(float *w, const float *__restrict__ g, float *mom, float *vel,
+                                                   bf16 *wb, size_t n, float lr, float b1, float b2, float eps,
+                                                   float c1, float c2)
+{
+    const size_t k = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (k >= n)
+        return;
+    const float gk = g[k];
+    const float mk = b1 * mom[k] + (1 - b1) * gk;
+    const float vk = b2 * vel[k] + (1 - b2) * gk * gk;
+    mom[k] = mk;
+    vel[k] = vk;
+    const float upd = lr * (mk / c1) / (sqrtf(vk / c2) + eps);
+    const float nw = w[k] - upd;
+    w[k] = nw;
+    if (wb)
+        wb[k] = f2bf(nw);
+}
+
+__global__ __launch_bounds__(256) void to_bf16_kernel(const float *__restrict__ src, bf16 *dst, size_t n)
+{
+    const size_t k = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (k < n)
+        dst[k] = f2bf(src[k]);
+}
+
+// Surrogate features of one block of 64 columns: x = (log(N + floor) - shift)
+// * scale for every bin, then U10x/ref, U10y/ref, depth/ref, zero padding.
+// The action planes are read along i (coalesced) and transposed in LDS.
+template <typename T>
+__global__ __launch_bounds__(256) void features_kernel(const T *__restrict__ N, size_t plane, size_t col0, int nbins,
+                                                       int in_pad, const T *__restrict__ U10,
+                                                       const T *__restrict__ theta, FeatureScales fs, bf16 *X)
+{
+    __shared__ float tile[64][65];
+    const size_t c0 = col0 + (size_t)blockIdx.x * 64;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    for (int b0 = 0; b0 < in_pad; b0 += 64) {
+        __syncthreads();
+        for (int bb = ty; bb < 64; bb += 4) {
+            const int bin = b0 + bb;
+            const size_t col = c0 + tx;
+            float v = 0.0f;
+            if (col < plane) {
+                if (bin < nbins) {
+                    const float a = (float)N[col + plane * bin];
+                    v = (logf(fmaxf(a, 0.0f) + fs.floor) - fs.shift) * fs.scale;
+                } else if (bin == nbins) {
+                    v = (float)U10[col] * cosf((float)theta[col]) * fs.wind_scale;
+                } else if (bin == nbins + 1) {
+                    v = (float)U10[col] * sinf((float)theta[col]) * fs.wind_scale;
+                } else if (bin == nbins + 2) {
+                    v = fs.depth_feature;
+                }
+            }
+            tile[bb][tx] = v;
+        }
+        __syncthreads();
+        for (int cc = ty; cc < 64; cc += 4) {
+            const size_t col = c0 + cc;
+            X[(col - col0) * (size_t)in_pad + b0 + tx] = f2bf(col < plane ? tile[tx][cc] : 0.0f);
+        }
+    }
+}
+
+// Envelope and audit decision per column: the plain bundle is used where the
+// wind is outside the training range, where the column's action is outside
+// it, where the network produced a non-finite rate, and on every audit column.
+template <typename T>
+__global__ __launch_bounds__(256) void envelope_kernel(const T *__restrict__ N, size_t plane, int nbins,
+                                                       const T *__restrict__ U10, const float *__restrict__ rate,
+                                                       size_t ld, Envelope env, unsigned char *use_plain,
+                                                       unsigned *counts)
+{
+    const size_t col = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (col >= plane)
+        return;
+    const float u = (float)U10[col];
+    bool plain = !(u >= env.wind_lo && u <= env.wind_hi);
+    float amax = 0.0f;
+    bool finite = true;
+    for (int b = 0; b < nbins; ++b) {
+        const float a = (float)N[col + plane * b];
+        amax = fmaxf(amax, a);
+        const float r = rate[col + ld * b];
+        finite = finite && (r == r) && fabsf(r) < 3.0e38f;
+    }
+    plain = plain || !(amax <= env.action_hi) || !finite;
+    const bool audit = env.audit_stride > 0 && (col % (size_t)env.audit_stride) == 0;
+    use_plain[col] = (unsigned char)((plain ? 1 : 0) | (audit ? 2 : 0));
+    if (plain)
+        atomicAdd(&counts[0], 1u);
+    if (audit)
+        atomicAdd(&counts[1], 1u);
+}
+
+// Audit: on audit columns compare the network's source with the plain
+// bundle's (both on the same state, sp with the surrogate switched off);
+// columns whose relative L2 error exceeds the bound are counted and marked
+// (bit 2) so that the stage takes the plain source there.
+template <typename T>
+__global__ __launch_bounds__(64) void audit_kernel(const T *__restrict__ N, size_t plane, int nbins,
+                                                   const float *__restrict__ rate, size_t ld,
+                                                   const float *__restrict__ out_scale,
+                                                   unsigned char *use_plain, int stride, SourceParams<T> sp,
+                                                   float bound, unsigned *counts, float *err_max)
+{
+    const size_t col = (size_t)blockIdx.x * (size_t)stride;
+    if (col >= plane || (use_plain[col] & 3) != 2)
+        return;
+    float num = 0, den = 0;
+    for (int b = threadIdx.x; b < nbins; b += 64) {
+        const T a = N[col + plane * b];
+        const float sp_plain = (float)source_term<T>(sp, (unsigned)col, b, a);
+        const float sp_net = rate[col + ld * b] * out_scale[b] * (float)a;
+        num += (sp_net - sp_plain) * (sp_net - sp_plain);
+        den += sp_plain * sp_plain;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        num += __shfl_down(num, off);
+        den += __shfl_down(den, off);
+    }
+    if (threadIdx.x == 0) {
+        const float e = den > 0 ? sqrtf(num / den) : (num > 0 ? 1.0f : 0.0f);
+        if (!(e <= bound)) {
+            atomicAdd(&counts[2], 1u);
+            use_plain[col] |= 4;
+        }
+        atomicMax(reinterpret_cast<unsigned *>(err_max), __float_as_uint(e));  // e >= 0: bit order = value order
+    }
+}
+
+}  // namespace
+
+template <bool A_KC, bool B_KC>
+static hipError_t gemm_dispatch(const GemmArgs &g, int epi, hipStream_t s)
+{
+    dim3 grid(g.M / BM, g.N / BN);
+    switch (epi) {
+    case EPI_STORE_F32:
+        hipLaunchKernelGGL((gemm_kernel<A_KC, B_KC, EPI_STORE_F32>), grid, dim3(256), 0, s, g);
+        break;
+    case EPI_BIAS_GELU:
+        hipLaunchKernelGGL((gemm_kernel<A_KC, B_KC, EPI_BIAS_GELU>), grid, dim3(256), 0, s, g);
+        break;
+    case EPI_BIAS_F32:
+        hipLaunchKernelGGL((gemm_kernel<A_KC, B_KC, EPI_BIAS_F32>), grid, dim3(256), 0, s, g);
+        break;
+    case EPI_BIAS_GELU_INFER:
+        hipLaunchKernelGGL((gemm_kernel<A_KC, B_KC, EPI_BIAS_GELU_INFER>), grid, dim3(256), 0, s, g);
+        break;
+    case EPI_BIAS_ROW_F32:
+        hipLaunchKernelGGL((gemm_kernel<A_KC, B_KC, EPI_BIAS_ROW_F32>), grid, dim3(256), 0, s, g);
+        break;
+    default:
+        hipLaunchKernelGGL((gemm_kernel<A_KC, B_KC, EPI_GELU_BACKWARD>), grid, dim3(256), 0, s, g);
+        break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t gemm(const GemmArgs &g, int epi, hipStream_t s)
+{
+    if (g.M % BM || g.N % BN || g.K % BK)
+        return hipErrorInvalidValue;
+    const bool akc = g.sak == 1, bkc = g.sbk == 1;
+    // The non-k-contiguous operand must be contiguous along its row index.
+    if ((!akc && g.sam != 1) || (!bkc && g.sbn != 1))
+        return hipErrorInvalidValue;
+    if (akc && bkc)
+        return gemm_dispatch<true, true>(g, epi, s);
+    if (akc)
+        return gemm_dispatch<true, false>(g, epi, s);
+    if (bkc)
+        return gemm_dispatch<false, true>(g, epi, s);
+    return gemm_dispatch<false, false>(g, epi, s);
+}
+
+hipError_t column_sum(const bf16 *D, int M, int N, float *out, hipStream_t s)
+{
+    hipLaunchKernelGGL(column_sum_kernel, dim3((N + 255) / 256), dim3(256), 0, s, D, M, N, out);
+    return hipGetLastError();
+}
+
+hipError_t mse(const float *Y, const float *T, int rows, int rows_valid, int cols, int cols_valid, bf16 *dY,
+               float *partials, int blocks, hipStream_t s)
+{
+    const float inv = 1.0f / ((float)rows_valid * (float)cols_valid);
+    hipLaunchKernelGGL(mse_kernel, dim3(blocks), dim3(256), 0, s, Y, T, rows, rows_valid, cols, cols_valid, inv, dY,
+                       partials);
+    return hipGetLastError();
+}
+
+hipError_t adam(float *w, const float *g, float *m, float *v, bf16 *wb, size_t n, float lr, float b1, float b2,
+                float eps, int step, hipStream_t s)
+{
+    const float c1 = 1.0f - powf(b1, (float)step), c2 = 1.0f - powf(b2, (float)step);
+    hipLaunchKernelGGL(adam_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, w, g, m, v, wb, n, lr, b1, b2,
+                       eps, c1, c2);
+    return hipGetLastError();
+}
+
+hipError_t to_bf16(const float *src, bf16 *dst, size_t n, hipStream_t s)
+{
+    if (n == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(to_bf16_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src, dst, n);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t features(const T *N, size_t plane, size_t col0, size_t ncols, int nbins, int in_pad, const T *U10,
+                    const T *theta, const FeatureScales &fs, bf16 *X, hipStream_t s)
+{
+    if (ncols % 64 || in_pad % 64)
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL(features_kernel<T>, dim3((unsigned)(ncols / 64)), dim3(256), 0, s, N, plane, col0, nbins,
+                       in_pad, U10, theta, fs, X);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t envelope(const T *N, size_t plane, int nbins, const T *U10, const float *rate, size_t ld,
+                    const Envelope &env, unsigned char *use_plain, unsigned *counts, hipStream_t s)
+{
+    hipLaunchKernelGGL(envelope_kernel<T>, dim3((unsigned)((plane + 255) / 256)), dim3(256), 0, s, N, plane, nbins, U10,
+                       rate, ld, env, use_plain, counts);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t audit(const T *N, size_t plane, int nbins, const float *rate, size_t ld, const float *out_scale,
+                 unsigned char *use_plain, int stride, const SourceParams<T> &sp, float bound, unsigned *counts,
+                 float *err_max, hipStream_t s)
+{
+    if (stride < 1)
+        return hipSuccess;
+    hipLaunchKernelGGL(audit_kernel<T>, dim3((unsigned)((plane + stride - 1) / stride)), dim3(64), 0, s, N, plane,
+                       nbins, rate, ld, out_scale, use_plain, stride, sp, bound, counts, err_max);
+    return hipGetLastError();
+}
+
+template hipError_t features<float>(const float *, size_t, size_t, size_t, int, int, const float *, const float *,
+                                    const FeatureScales &, bf16 *, hipStream_t);
+template hipError_t features<double>(const double *, size_t, size_t, size_t, int, int, const double *,
+                                     const double *, const FeatureScales &, bf16 *, hipStream_t);
+template hipError_t envelope<float>(const float *, size_t, int, const float *, const float *, size_t,
+                                    const Envelope &, unsigned char *, unsigned *, hipStream_t);
+template hipError_t envelope<double>(const double *, size_t, int, const double *, const float *, size_t,
+                                     const Envelope &, unsigned char *, unsigned *, hipStream_t);
+template hipError_t audit<float>(const float *, size_t, int, const float *, size_t, const float *,
+                                 unsigned char *, int, const SourceParams<float> &, float, unsigned *, float *,
+                                 hipStream_t);
+template hipError_t audit<double>(const double *, size_t, int, const float *, size_t, const float *,
+                                  unsigned char *, int, const SourceParams<double> &, float, unsigned *, float *,
+                                  hipStream_t);
+
+}  // namespace mlp
+}  // namespace rpl

@@ -9,6 +9,7 @@
 #include <string>
 #include <vector>
 
+#include "mlp_internal.h"
 #include "rpl_internal.h"
 
 namespace {
@@ -29,6 +30,8 @@ int fail(int code, const std::string &msg)
     } while (0)
 
 }  // namespace
+
+int rpl::set_error(int code, const char *msg) { return fail(code, msg); }
 
 struct rpl_ctx {
     rpl_config cfg{};
@@ -62,12 +65,27 @@ struct rpl_ctx {
     void *t_phi = nullptr, *t_w = nullptr, *t_sigma_k = nullptr, *t_cos = nullptr, *t_sin = nullptr;
     void *t_kpow = nullptr, *t_spow = nullptr;
     std::vector<double> h_kc, h_pc, h_kf, h_pf;  // geometry kept for source tables
+    std::vector<double> h_sigma;                 // per-bin intrinsic frequency of the source tables
 
+    // learned surrogate (rpl_attach_surrogate)
+    rpl_mlp *net = nullptr;
+    rpl_surrogate scfg{};
+    bool s_active = false;  // the stage takes the network's source where allowed
+    rpl::mlp::bf16 *sX = nullptr, *sA = nullptr, *sB = nullptr;
+    float *rate = nullptr, *out_scale = nullptr, *s_err = nullptr;
+    unsigned char *use_plain = nullptr;
+    unsigned *s_counts = nullptr;
+    size_t s_ld = 0, s_chunk = 0;
+    int64_t s_columns = 0;
+
+    // event timing: phase 0 = stage kernel launches, 1 = surrogate inference
+    // (features and network forward of one stage input)
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+    std::vector<int> event_phase;
     size_t events_used = 0;
-    double timing_ms = 0;
-    int64_t timing_launches = 0;
+    double timing_ms[2] = {0, 0};
+    int64_t timing_launches[2] = {0, 0};
 };
 
 namespace {
@@ -170,6 +188,13 @@ rpl::StageParams<T> stage_params(const rpl_ctx *c, double dt)
         q.ls_br = (T)s.ls_br;
         q.ls_p = (T)s.ls_p;
         q.ls_q = (T)s.ls_sigma_power;
+        if (c->s_active) {
+            q.surrogate = 1;
+            q.rate = c->rate;
+            q.out_scale = c->out_scale;
+            q.rate_ld = c->s_ld;
+            q.use_plain = c->use_plain;
+        }
     }
     return p;
 }
@@ -237,6 +262,68 @@ int launch_state(rpl_ctx *c, const void *Nk)
     return RPL_OK;
 }
 
+// Network inference of one stage input: features and forward pass chunk by
+// chunk into the rate planes, then the per-column envelope / audit flags.
+template <typename T>
+int surrogate_rates(rpl_ctx *c, const T *Nk)
+{
+    const int nbins = c->cfg.nkappa * c->cfg.nphi;
+    const rpl_surrogate &g = c->scfg;
+    rpl::mlp::FeatureScales fs{g.floor, g.shift, g.scale, g.wind_scale, g.depth_feature};
+    hipError_t e = hipSuccess;
+    for (size_t col0 = 0; col0 < c->plane && e == hipSuccess; col0 += c->s_chunk) {
+        const size_t rows = std::min(c->s_chunk, (c->plane - col0 + 63) / 64 * 64);
+        e = rpl::mlp::features<T>(Nk, c->plane, col0, rows, nbins, c->net->dims[0], (const T *)c->U10,
+                                  (const T *)c->theta_u, fs, c->sX, c->stream);
+        if (e == hipSuccess)
+            e = rpl::mlp::forward_rows(c->net, c->sX, (int)rows, c->sA, c->sB, c->rate + col0, c->s_ld, c->stream);
+    }
+    if (e == hipSuccess) {
+        rpl::mlp::Envelope env{g.wind_lo, g.wind_hi, g.action_hi, g.audit_stride};
+        e = rpl::mlp::envelope<T>(Nk, c->plane, nbins, (const T *)c->U10, c->rate, c->s_ld, env, c->use_plain,
+                                  c->s_counts, c->stream);
+    }
+    if (e != hipSuccess)
+        return fail(RPL_ERR_HIP, std::string("surrogate inference: ") + hipGetErrorString(e));
+    c->s_columns += (int64_t)c->plane;
+    return RPL_OK;
+}
+
+template <typename T>
+int surrogate_audit(rpl_ctx *c, const T *Nk)
+{
+    auto p = stage_params<T>(c, 0.0);
+    p.src.surrogate = 0;  // the audit compares against the plain bundle
+    hipError_t e = rpl::mlp::audit<T>(Nk, c->plane, c->cfg.nkappa * c->cfg.nphi, c->rate, c->s_ld, c->out_scale,
+                                      c->use_plain, c->scfg.audit_stride, p.src, c->scfg.audit_bound,
+                                      c->s_counts, c->s_err, c->stream);
+    if (e != hipSuccess)
+        return fail(RPL_ERR_HIP, std::string("surrogate audit: ") + hipGetErrorString(e));
+    return RPL_OK;
+}
+
+int record_begin(rpl_ctx *c, int phase);
+int record_end(rpl_ctx *c);
+
+// Source preparation of one stage input: surrogate inference (if attached),
+// the bundle's column state (only on plain and audit columns when the
+// surrogate runs), then the audit.
+int launch_sources(rpl_ctx *c, const void *Nk)
+{
+    if (!c->s_active)
+        return launch_state(c, Nk);
+    int rc = record_begin(c, 1);
+    if (rc == RPL_OK)
+        rc = is_f64(c) ? surrogate_rates<double>(c, (const double *)Nk) : surrogate_rates<float>(c, (const float *)Nk);
+    if (rc == RPL_OK)
+        rc = record_end(c);
+    if (rc || (rc = launch_state(c, Nk)))
+        return rc;
+    if (c->scfg.audit_stride > 0)
+        rc = is_f64(c) ? surrogate_audit<double>(c, (const double *)Nk) : surrogate_audit<float>(c, (const float *)Nk);
+    return rc;
+}
+
 bool use_fast(const rpl_ctx *c)
 {
     if (is_f64(c))
@@ -246,7 +333,7 @@ bool use_fast(const rpl_ctx *c)
     return true;
 }
 
-int record_begin(rpl_ctx *c)
+int record_begin(rpl_ctx *c, int phase)
 {
     if (!c->timing)
         return RPL_OK;
@@ -255,7 +342,9 @@ int record_begin(rpl_ctx *c)
         RPL_HIP(hipEventCreate(&a));
         RPL_HIP(hipEventCreate(&b));
         c->events.emplace_back(a, b);
+        c->event_phase.push_back(0);
     }
+    c->event_phase[c->events_used] = phase;
     RPL_HIP(hipEventRecord(c->events[c->events_used].first, c->stream));
     return RPL_OK;
 }
@@ -268,9 +357,9 @@ int fold_timing(rpl_ctx *c)
     for (size_t k = 0; k < c->events_used; ++k) {
         float ms = 0;
         RPL_HIP(hipEventElapsedTime(&ms, c->events[k].first, c->events[k].second));
-        c->timing_ms += ms;
+        c->timing_ms[c->event_phase[k]] += ms;
+        c->timing_launches[c->event_phase[k]] += 1;
     }
-    c->timing_launches += (int64_t)c->events_used;
     c->events_used = 0;
     return RPL_OK;
 }
@@ -288,7 +377,7 @@ int record_end(rpl_ctx *c)
 
 int launch_stage(rpl_ctx *c, double dt, int stage, const void *Nk, const void *N0, void *out)
 {
-    int rc = record_begin(c);
+    int rc = record_begin(c, 0);
     if (rc != RPL_OK)
         return rc;
     hipError_t e;
@@ -322,6 +411,82 @@ int check_ready(rpl_ctx *c)
     return RPL_OK;
 }
 
+}  // namespace
+
+namespace {
+void free_surrogate(rpl_ctx *c)
+{
+    for (void *p : {(void *)c->sX, (void *)c->sA, (void *)c->sB, (void *)c->rate, (void *)c->out_scale,
+                    (void *)c->s_err, (void *)c->use_plain, (void *)c->s_counts})
+        free_ptr(p);
+    c->sX = c->sA = c->sB = nullptr;
+    c->rate = c->out_scale = c->s_err = nullptr;
+    c->use_plain = nullptr;
+    c->s_counts = nullptr;
+    c->net = nullptr;
+    c->s_active = false;
+}
+
+template <typename T>
+int training_pairs(rpl_ctx *c, const rpl_surrogate *cfg, float *features, float *targets)
+{
+    const int nbins = c->cfg.nkappa * c->cfg.nphi, in_dim = nbins + 3, in_pad = (in_dim + 63) / 64 * 64;
+    const size_t chunk = std::min<size_t>((c->plane + 63) / 64 * 64, 16384);
+    rpl::mlp::bf16 *X = nullptr;
+    float *r = nullptr, *unit = nullptr;
+    int rc = RPL_OK;
+    if ((rc = alloc((void **)&X, chunk * in_pad * 2)) || (rc = alloc((void **)&r, c->plane * nbins * 4)) ||
+        (rc = alloc((void **)&unit, nbins * 4))) {
+        free_ptr(X);
+        free_ptr(r);
+        return rc;
+    }
+    hipError_t e = hipSuccess;
+    if (features) {
+        rpl::mlp::FeatureScales fs{cfg->floor, cfg->shift, cfg->scale, cfg->wind_scale, cfg->depth_feature};
+        std::vector<rpl::mlp::bf16> h(chunk * in_pad);
+        for (size_t col0 = 0; col0 < c->plane && e == hipSuccess; col0 += chunk) {
+            const size_t rows = std::min(chunk, (c->plane - col0 + 63) / 64 * 64);
+            e = rpl::mlp::features<T>((const T *)c->A, c->plane, col0, rows, nbins, in_pad, (const T *)c->U10,
+                                      (const T *)c->theta_u, fs, X, c->stream);
+            if (e == hipSuccess)
+                e = hipStreamSynchronize(c->stream);
+            if (e == hipSuccess)
+                e = hipMemcpy(h.data(), X, rows * in_pad * 2, hipMemcpyDeviceToHost);
+            const size_t valid = std::min(rows, c->plane - col0);
+            for (size_t q = 0; e == hipSuccess && q < valid; ++q)
+                for (int f = 0; f < in_dim; ++f) {
+                    const uint32_t bits = (uint32_t)h[q * in_pad + f] << 16;
+                    float v;
+                    std::memcpy(&v, &bits, 4);
+                    features[(col0 + q) * in_dim + f] = v;
+                }
+        }
+    }
+    if (e == hipSuccess && targets) {
+        std::vector<float> u(nbins);
+        for (int t = 0; t < nbins; ++t)
+            u[t] = (float)(c->h_sigma[t] * cfg->rate_unit);
+        e = hipMemcpy(unit, u.data(), nbins * 4, hipMemcpyHostToDevice);
+        auto p = stage_params<T>(c, 0.0);
+        if (e == hipSuccess)
+            e = rpl::launch_source_rates<T>(c->cfg.nx, c->cfg.ny, nbins, (const T *)c->A, p.src, unit, r, c->stream);
+        if (e == hipSuccess)
+            e = hipStreamSynchronize(c->stream);
+        std::vector<float> h(c->plane * nbins);
+        if (e == hipSuccess)
+            e = hipMemcpy(h.data(), r, h.size() * 4, hipMemcpyDeviceToHost);
+        for (int t = 0; e == hipSuccess && t < nbins; ++t)
+            for (size_t col = 0; col < c->plane; ++col)
+                targets[col * nbins + t] = h[col + c->plane * t];
+    }
+    free_ptr(X);
+    free_ptr(r);
+    free_ptr(unit);
+    if (e != hipSuccess)
+        return fail(RPL_ERR_HIP, std::string("surrogate training pairs: ") + hipGetErrorString(e));
+    return RPL_OK;
+}
 }  // namespace
 
 extern "C" {
@@ -414,7 +579,8 @@ int rpl_destroy(rpl_ctx *c)
                     (void *)c->energy, (void *)c->diag, (void *)c->partials, c->scratch_in, (void *)c->xn,
                     (void *)c->yn, c->U10, c->theta_u, c->ustar, c->factor, c->cm0, c->kbar, c->sbar, c->k_cell,
                     c->theta_cell, c->sigma_cell, c->C_cell, c->t_phi, c->t_w, c->t_sigma_k, c->t_cos, c->t_sin,
-                    c->t_kpow, c->t_spow})
+                    c->t_kpow, c->t_spow, (void *)c->sX, (void *)c->sA, (void *)c->sB, (void *)c->rate,
+                    (void *)c->out_scale, (void *)c->s_err, (void *)c->use_plain, (void *)c->s_counts})
         free_ptr(p);
     for (auto &ev : c->events) {
         (void)hipEventDestroy(ev.first);
@@ -674,7 +840,7 @@ int rpl_tendency(rpl_ctx *c, void *G)
         return rc;
     if (!G)
         return fail(RPL_ERR_ARG, "null argument");
-    if ((rc = launch_wind(c)) || (rc = launch_state(c, c->A)) ||
+    if ((rc = launch_wind(c)) || (rc = launch_sources(c, c->A)) ||
         (rc = launch_stage(c, 0.0, rpl::STAGE_TENDENCY, c->A, c->A, c->B)))
         return rc;
     RPL_HIP(hipStreamSynchronize(c->stream));
@@ -695,14 +861,14 @@ int rpl_time_step(rpl_ctx *c, double dt, int32_t nsteps)
         if ((rc = launch_wind(c)))
             return rc;
         if (c->cfg.timestepper == RPL_FORWARD_EULER) {
-            if ((rc = launch_state(c, c->A)) || (rc = launch_stage(c, dt, rpl::STAGE_1, c->A, c->A, c->B)))
+            if ((rc = launch_sources(c, c->A)) || (rc = launch_stage(c, dt, rpl::STAGE_1, c->A, c->A, c->B)))
                 return rc;
             std::swap(c->A, c->B);
         } else {
             // SSP-RK3 over three buffers; A holds N0 and receives N3 in place.
-            if ((rc = launch_state(c, c->A)) || (rc = launch_stage(c, dt, rpl::STAGE_1, c->A, c->A, c->B)) ||
-                (rc = launch_state(c, c->B)) || (rc = launch_stage(c, dt, rpl::STAGE_2, c->B, c->A, c->C)) ||
-                (rc = launch_state(c, c->C)) || (rc = launch_stage(c, dt, rpl::STAGE_3, c->C, c->A, c->A)))
+            if ((rc = launch_sources(c, c->A)) || (rc = launch_stage(c, dt, rpl::STAGE_1, c->A, c->A, c->B)) ||
+                (rc = launch_sources(c, c->B)) || (rc = launch_stage(c, dt, rpl::STAGE_2, c->B, c->A, c->C)) ||
+                (rc = launch_sources(c, c->C)) || (rc = launch_stage(c, dt, rpl::STAGE_3, c->C, c->A, c->A)))
                 return rc;
         }
         c->time += dt;  // tick!(clock, dt), once per step
@@ -903,6 +1069,7 @@ int rpl_set_sources(rpl_ctx *c, const rpl_sources *src, const double *x_nodes, c
         kpow[m] = s.dissipation == RPL_DISSIPATION_MEAN_SPECTRUM ? std::pow(c->h_kc[m], s.p_mean) : 0.0;
         spow[m] = s.dissipation == RPL_DISSIPATION_MEAN_SPECTRUM ? std::pow(sk[m], s.p_mean) : 0.0;
     }
+    c->h_sigma = scell;
     if ((rc = upload_real(c, kcell.data(), c->k_cell, tab)) || (rc = upload_real(c, thcell.data(), c->theta_cell, tab)) ||
         (rc = upload_real(c, scell.data(), c->sigma_cell, tab)) || (rc = upload_real(c, Ccell.data(), c->C_cell, tab)) ||
         (rc = upload_real(c, w.data(), c->t_w, tab)) || (rc = upload_real(c, sk.data(), c->t_sigma_k, nk)) ||
@@ -995,12 +1162,126 @@ int rpl_get_kernel_timing(rpl_ctx *c, double *total_ms, int64_t *launches)
     if (rc)
         return rc;
     if (total_ms)
-        *total_ms = c->timing_ms;
+        *total_ms = c->timing_ms[0];
     if (launches)
-        *launches = c->timing_launches;
-    c->timing_ms = 0;
-    c->timing_launches = 0;
+        *launches = c->timing_launches[0];
+    c->timing_ms[0] = 0;
+    c->timing_launches[0] = 0;
     return RPL_OK;
+}
+
+int rpl_get_surrogate_timing(rpl_ctx *c, double *total_ms, int64_t *evaluations)
+{
+    if (!c)
+        return fail(RPL_ERR_ARG, "null context");
+    int rc = fold_timing(c);
+    if (rc)
+        return rc;
+    if (total_ms)
+        *total_ms = c->timing_ms[1];
+    if (evaluations)
+        *evaluations = c->timing_launches[1];
+    c->timing_ms[1] = 0;
+    c->timing_launches[1] = 0;
+    return RPL_OK;
+}
+
+int rpl_attach_surrogate(rpl_ctx *c, rpl_mlp *net, const rpl_surrogate *cfg)
+{
+    if (!c)
+        return fail(RPL_ERR_ARG, "null context");
+    RPL_HIP(hipSetDevice(c->device));
+    RPL_HIP(hipStreamSynchronize(c->stream));
+    if (!net) {
+        free_surrogate(c);
+        return RPL_OK;
+    }
+    if (!cfg)
+        return fail(RPL_ERR_ARG, "null surrogate configuration");
+    if (!c->has_sources || !c->src.wind_input)
+        return fail(RPL_ERR_STATE, "the surrogate needs rpl_set_sources with a wind input (its plain fallback)");
+    const int nbins = c->cfg.nkappa * c->cfg.nphi;
+    if (net->in_dim != nbins + 3 || net->out_dim != nbins)
+        return fail(RPL_ERR_ARG, "network must map nkappa*nphi + 3 features to nkappa*nphi rates");
+    if (net->device != c->device)
+        return fail(RPL_ERR_ARG, "network and context live on different devices");
+    if (!(cfg->rate_unit > 0) || cfg->audit_stride < 0 || !(cfg->floor > 0))
+        return fail(RPL_ERR_ARG, "need rate_unit > 0, floor > 0, audit_stride >= 0");
+    free_surrogate(c);
+    const int in_pad = net->dims[0], out_pad = net->dims.back(), hid = net->hidden;
+    c->s_ld = (c->plane + 63) / 64 * 64;
+    c->s_chunk = std::min<size_t>(c->s_ld, 65536);
+    int rc;
+    if ((rc = alloc((void **)&c->sX, c->s_chunk * in_pad * 2)) || (rc = alloc((void **)&c->sA, c->s_chunk * hid * 2)) ||
+        (rc = alloc((void **)&c->sB, c->s_chunk * hid * 2)) ||
+        (rc = alloc((void **)&c->rate, c->s_ld * out_pad * sizeof(float))) ||
+        (rc = alloc((void **)&c->out_scale, out_pad * sizeof(float))) || (rc = alloc((void **)&c->s_err, 16)) ||
+        (rc = alloc((void **)&c->use_plain, c->s_ld)) || (rc = alloc((void **)&c->s_counts, 16))) {
+        free_surrogate(c);
+        return rc;
+    }
+    std::vector<float> scale(out_pad, 1.0f);
+    for (int t = 0; t < nbins; ++t)
+        scale[t] = (float)(c->h_sigma[t] * cfg->rate_unit);
+    RPL_HIP(hipMemcpy(c->out_scale, scale.data(), out_pad * sizeof(float), hipMemcpyHostToDevice));
+    RPL_HIP(hipMemset(c->s_counts, 0, 16));
+    RPL_HIP(hipMemset(c->s_err, 0, 16));
+    RPL_HIP(hipMemset(c->use_plain, 1, c->s_ld));
+    c->net = net;
+    c->scfg = *cfg;
+    c->s_active = true;
+    c->s_columns = 0;
+    return RPL_OK;
+}
+
+int rpl_surrogate_stats(rpl_ctx *c, int64_t *columns, int64_t *plain, int64_t *audited, int64_t *failed,
+                        double *err_max)
+{
+    if (!c)
+        return fail(RPL_ERR_ARG, "null context");
+    if (!c->s_active)
+        return fail(RPL_ERR_STATE, "no surrogate attached");
+    RPL_HIP(hipSetDevice(c->device));
+    RPL_HIP(hipStreamSynchronize(c->stream));
+    unsigned h[4] = {0, 0, 0, 0};
+    float e = 0;
+    RPL_HIP(hipMemcpy(h, c->s_counts, 16, hipMemcpyDeviceToHost));
+    RPL_HIP(hipMemcpy(&e, c->s_err, 4, hipMemcpyDeviceToHost));
+    if (columns)
+        *columns = c->s_columns;
+    if (plain)
+        *plain = h[0];
+    if (audited)
+        *audited = h[1];
+    if (failed)
+        *failed = h[2];
+    if (err_max)
+        *err_max = e;
+    RPL_HIP(hipMemset(c->s_counts, 0, 16));
+    RPL_HIP(hipMemset(c->s_err, 0, 16));
+    c->s_columns = 0;
+    return RPL_OK;
+}
+
+
+int rpl_surrogate_training_pairs(rpl_ctx *c, const rpl_surrogate *cfg, float *features, float *targets)
+{
+    int rc = check_ready(c);
+    if (rc)
+        return rc;
+    if (!cfg)
+        return fail(RPL_ERR_ARG, "null surrogate configuration");
+    if (!c->has_sources || !c->src.wind_input)
+        return fail(RPL_ERR_STATE, "training pairs need rpl_set_sources with a wind input");
+    if (!(cfg->rate_unit > 0) || !(cfg->floor > 0))
+        return fail(RPL_ERR_ARG, "need rate_unit > 0 and floor > 0");
+    const bool was = c->s_active;
+    c->s_active = false;  // targets come from the plain bundle on every column
+    if ((rc = launch_wind(c)) == RPL_OK && (rc = launch_state(c, c->A)) == RPL_OK)
+        rc = is_f64(c) ? training_pairs<double>(c, cfg, features, targets)
+                       : training_pairs<float>(c, cfg, features, targets);
+    c->s_active = was;
+    return rc;
 }
 
 void *rpl_stream(rpl_ctx *c) { return c ? (void *)c->stream : nullptr; }

@@ -11,6 +11,9 @@
 
 namespace rpl {
 
+// Records msg as the thread's last error (rpl_last_error) and returns code.
+int set_error(int code, const char *msg);
+
 // Everything the fused tendency + RK3 stage kernels read besides the action.
 // T is the action's element type.  Tables are (nkappa, nphi), m fastest.
 template <typename T>
@@ -48,6 +51,9 @@ struct StateTables {
 template <typename T>
 hipError_t launch_wind_columns(int nx, int ny, const double *xn, const double *yn, double t, const WindParams &w,
                                T *U10, T *theta_u, T *ustar, hipStream_t s);
+template <typename T>
+hipError_t launch_source_rates(int nx, int ny, int nbins, const T *N, const SourceParams<T> &sp, const float *unit,
+                               float *r, hipStream_t s);
 template <typename T>
 hipError_t launch_source_state(int nx, int ny, int nk, int nphi, const T *N, const SourceParams<T> &sp,
                                const StateTables<T> &tb, T *factor, T *m0, T *kbar, T *sbar, hipStream_t s);

@@ -82,6 +82,8 @@ __global__ __launch_bounds__(256) void source_state_kernel(int nx, int ny, int n
     const size_t ij = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (ij >= plane)
         return;
+    if (s.surrogate && !(s.use_plain[ij] & 3))
+        return;  // the network supplies this column's source
     if (s.wind_input && s.stress_cap) {
         const T U10 = s.U10[ij], ustar = s.ustar[ij];
         T f = T(1);
@@ -149,6 +151,43 @@ hipError_t launch_wind_columns(int nx, int ny, const double *xn, const double *y
                        t, w, U10, theta_u, ustar);
     return hipGetLastError();
 }
+
+namespace {
+// Per-cell plain-bundle rates normalised per bin, r = S / (N * unit[t]), and 0
+// where N <= 0: the learned surrogate's training targets.  Bin-major like the
+// action (r[ij + plane * t]).
+template <typename T>
+__global__ __launch_bounds__(256) void source_rates_kernel(int nx, int ny, const T *__restrict__ N,
+                                                           SourceParams<T> s, const float *__restrict__ unit,
+                                                           float *r)
+{
+    const size_t plane = (size_t)nx * ny;
+    const size_t ij = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const int t = blockIdx.y;
+    if (ij >= plane)
+        return;
+    const T a = N[ij + plane * t];
+    float v = 0.0f;
+    if (a > T(0))
+        v = (float)(source_term<T>(s, (unsigned)ij, t, a) / a) / unit[t];
+    r[ij + plane * t] = v;
+}
+}  // namespace
+
+template <typename T>
+hipError_t launch_source_rates(int nx, int ny, int nbins, const T *N, const SourceParams<T> &sp, const float *unit,
+                               float *r, hipStream_t s)
+{
+    const size_t plane = (size_t)nx * ny;
+    hipLaunchKernelGGL(source_rates_kernel<T>, dim3((unsigned)((plane + 255) / 256), nbins), dim3(256), 0, s, nx, ny,
+                       N, sp, unit, r);
+    return hipGetLastError();
+}
+
+template hipError_t launch_source_rates<float>(int, int, int, const float *, const SourceParams<float> &,
+                                               const float *, float *, hipStream_t);
+template hipError_t launch_source_rates<double>(int, int, int, const double *, const SourceParams<double> &,
+                                                const float *, float *, hipStream_t);
 
 template <typename T>
 hipError_t launch_source_state(int nx, int ny, int nk, int nphi, const T *N, const SourceParams<T> &sp,

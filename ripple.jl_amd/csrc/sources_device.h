@@ -34,6 +34,13 @@ struct SourceParams {
     T z_alpha, p_in, alpha0, kvk, g;
     T cds, d1, d2;              // mean-spectrum whitecapping
     T ls_cds, ls_br, ls_p, ls_q; // local saturation
+    // learned surrogate: where surrogate != 0 and use_plain[col] has neither
+    // bit 0 (outside the training envelope) nor bit 2 (failed audit), the
+    // source is rate[col + rate_ld * t] * out_scale[t] * N.
+    int surrogate;
+    const float *rate, *out_scale;
+    size_t rate_ld;
+    const unsigned char *use_plain;
 };
 
 __device__ __forceinline__ float dcos(float x) { return cosf(x); }
@@ -75,6 +82,8 @@ template <typename T>
 __device__ __forceinline__ T source_term(const SourceParams<T> &s, unsigned col, int t, T a)
 {
 #pragma clang fp contract(off)
+    if (s.surrogate && !(s.use_plain[col] & 5))
+        return (T)(s.rate[col + s.rate_ld * (size_t)t] * s.out_scale[t]) * a;
     T positive = T(0), damping = T(0);
     const T k = s.k_cell[t];
     if (s.wind_input) {

@@ -14,6 +14,8 @@ from .model import (SpectralWaveModel, InfiniteDepth, Clock, Simulation, WENO, s
                     interior)
 from .sources import (BulkWindDrag, LinearStormTrack, HollandHurricaneWind, PressureCorrelationInput,
                       MeanSpectrumWhitecapping, LocalSaturationDissipation, PrecomputedSources, wind_state)
+from .surrogate import (SourceNetwork, SurrogateConfig, training_pairs, attach_surrogate, surrogate_stats,
+                        train as train_surrogate)
 from .diagnostics import (m0, root_mean_square_wavenumber, mean_direction, significant_wave_height,
                           total_action, total_deep_water_energy)
 
@@ -27,4 +29,6 @@ __all__ = [
     "total_action", "total_deep_water_energy",
     "BulkWindDrag", "LinearStormTrack", "HollandHurricaneWind", "PressureCorrelationInput",
     "MeanSpectrumWhitecapping", "LocalSaturationDissipation", "PrecomputedSources", "wind_state",
+    "SourceNetwork", "SurrogateConfig", "training_pairs", "attach_surrogate", "surrogate_stats",
+    "train_surrogate",
 ]

@@ -60,11 +60,18 @@ class rpl_sources(ctypes.Structure):
                    "ls_cds", "ls_br", "ls_p", "ls_sigma_power")]
 
 
+class rpl_surrogate(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_float) for n in ("floor", "shift", "scale", "wind_scale", "depth_feature", "rate_unit",
+                                               "wind_lo", "wind_hi", "action_hi")] + \
+               [("audit_stride", ctypes.c_int32), ("audit_bound", ctypes.c_float)]
+
+
 _P = ctypes.c_void_p
 _D = ctypes.c_double
 _DP = ctypes.POINTER(ctypes.c_double)
 _I32 = ctypes.c_int32
 _I64P = ctypes.POINTER(ctypes.c_int64)
+_FP = ctypes.POINTER(ctypes.c_float)
 
 # name -> (argtypes, restype); every entry point declared in include/ripple_mi355x.h.
 SIGNATURES = {
@@ -87,7 +94,19 @@ SIGNATURES = {
     "rpl_get_wind": ([_P, _DP, _DP, _DP], ctypes.c_int),
     "rpl_enable_kernel_timing": ([_P, _I32], ctypes.c_int),
     "rpl_get_kernel_timing": ([_P, _DP, _I64P], ctypes.c_int),
+    "rpl_get_surrogate_timing": ([_P, _DP, _I64P], ctypes.c_int),
     "rpl_stream": ([_P], ctypes.c_void_p),
+    "rpl_mlp_create": ([ctypes.POINTER(_P), _I32, _I32, _I32, _I32, _I32, ctypes.c_uint64], ctypes.c_int),
+    "rpl_mlp_destroy": ([_P], ctypes.c_int),
+    "rpl_mlp_dims": ([_P, ctypes.POINTER(_I32), ctypes.POINTER(_I32), _I64P], ctypes.c_int),
+    "rpl_mlp_set_params": ([_P, _FP], ctypes.c_int),
+    "rpl_mlp_get_params": ([_P, _FP], ctypes.c_int),
+    "rpl_mlp_forward": ([_P, _I32, _FP, _FP], ctypes.c_int),
+    "rpl_mlp_gradients": ([_P, _I32, _FP, _FP, _FP, _FP], ctypes.c_int),
+    "rpl_mlp_train_step": ([_P, _I32, _FP, _FP, ctypes.c_float, _FP], ctypes.c_int),
+    "rpl_attach_surrogate": ([_P, _P, ctypes.POINTER(rpl_surrogate)], ctypes.c_int),
+    "rpl_surrogate_stats": ([_P, _I64P, _I64P, _I64P, _I64P, _DP], ctypes.c_int),
+    "rpl_surrogate_training_pairs": ([_P, ctypes.POINTER(rpl_surrogate), _FP, _FP], ctypes.c_int),
     "rpl_last_error": ([], ctypes.c_char_p),
 }
 
@@ -129,3 +148,10 @@ def dptr(a):
 
 def vptr(a):
     return ctypes.c_void_p(a.ctypes.data)
+
+
+def fptr(a):
+    """Pointer to the data of a C-contiguous float32 numpy array (None -> NULL)."""
+    if a is None:
+        return None
+    return a.ctypes.data_as(_FP)

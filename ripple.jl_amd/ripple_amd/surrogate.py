@@ -1,0 +1,212 @@
+"""Learned per-column source surrogate (no counterpart in the reference).
+
+A small MLP stands in for the per-column evaluation of the PrecomputedSources
+bundle (Physics/Packages/precomputed_sources.jl:240-261): it maps a column's
+log-scaled action spectrum, the 10 m wind vector and the depth to per-bin
+growth rates r, with source = r * rate_unit * sigma_bin * N.  It is trained
+on the GPU by the library's own BF16 matrix-core forward/backward kernels
+(FP32 master weights, Adam) on (features, rate) pairs that the plain bundle
+produces along the model's own runs; the plain bundle stays installed as the
+fallback outside the training envelope and on failed audit columns.
+"""
+import ctypes
+from dataclasses import dataclass, fields
+
+import numpy as np
+
+from . import _lib
+
+
+class SourceNetwork:
+    """Device-resident MLP in_dim -> hidden x hidden_layers -> out_dim (GELU).
+
+    Parameters are the concatenation over layers of W_l (out_pad_l x in_pad_l,
+    row-major, zero padding) and b_l (out_pad_l), FP32.
+    """
+
+    def __init__(self, in_dim, out_dim, hidden=512, hidden_layers=3, batch_capacity=4096, seed=0):
+        lib = _lib.load()
+        p = ctypes.c_void_p()
+        _lib.check(lib.rpl_mlp_create(ctypes.byref(p), in_dim, hidden, hidden_layers, out_dim, batch_capacity,
+                                      ctypes.c_uint64(seed)))
+        self._lib, self.ptr = lib, p
+        self.in_dim, self.out_dim, self.hidden, self.hidden_layers = in_dim, out_dim, hidden, hidden_layers
+        self.batch_capacity = batch_capacity
+        ip, op, n = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int64()
+        _lib.check(lib.rpl_mlp_dims(p, ctypes.byref(ip), ctypes.byref(op), ctypes.byref(n)))
+        self.in_pad, self.out_pad, self.nparams = ip.value, op.value, n.value
+        self.dims = [self.in_pad] + [hidden] * hidden_layers + [self.out_pad]
+
+    def close(self):
+        if getattr(self, "ptr", None) is not None and self.ptr.value:
+            self._lib.rpl_mlp_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self):
+        out = np.empty(self.nparams, dtype=np.float32)
+        _lib.check(self._lib.rpl_mlp_get_params(self.ptr, _lib.fptr(out)))
+        return out
+
+    def set_params(self, flat):
+        flat = np.ascontiguousarray(flat, dtype=np.float32)
+        if flat.size != self.nparams:
+            raise ValueError(f"expected {self.nparams} parameters, got {flat.size}")
+        _lib.check(self._lib.rpl_mlp_set_params(self.ptr, _lib.fptr(flat)))
+
+    def layers(self, flat=None):
+        """[(W, b)] views of a flat parameter vector (padded shapes)."""
+        flat = self.params() if flat is None else flat
+        out, off = [], 0
+        for din, dout in zip(self.dims[:-1], self.dims[1:]):
+            W = flat[off:off + din * dout].reshape(dout, din)
+            off += din * dout
+            b = flat[off:off + dout]
+            off += dout
+            out.append((W, b))
+        return out
+
+    def _rows(self, X, width):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        if X.ndim != 2 or X.shape[1] != width:
+            raise ValueError(f"expected rows of {width} values, got shape {X.shape}")
+        if not 1 <= X.shape[0] <= self.batch_capacity:
+            raise ValueError(f"batch must be in [1, {self.batch_capacity}]")
+        return X
+
+    def forward(self, X):
+        X = self._rows(X, self.in_dim)
+        Y = np.empty((X.shape[0], self.out_dim), dtype=np.float32)
+        _lib.check(self._lib.rpl_mlp_forward(self.ptr, X.shape[0], _lib.fptr(X), _lib.fptr(Y)))
+        return Y
+
+    def gradients(self, X, T):
+        """(loss, flat gradient) of mean((Y - T)^2) over the batch."""
+        X = self._rows(X, self.in_dim)
+        T = self._rows(T, self.out_dim)
+        loss = np.zeros(1, dtype=np.float32)
+        g = np.empty(self.nparams, dtype=np.float32)
+        _lib.check(self._lib.rpl_mlp_gradients(self.ptr, X.shape[0], _lib.fptr(X), _lib.fptr(T), _lib.fptr(loss),
+                                               _lib.fptr(g)))
+        return float(loss[0]), g
+
+    def train_step(self, X, T, lr):
+        X = self._rows(X, self.in_dim)
+        T = self._rows(T, self.out_dim)
+        loss = np.zeros(1, dtype=np.float32)
+        _lib.check(self._lib.rpl_mlp_train_step(self.ptr, X.shape[0], _lib.fptr(X), _lib.fptr(T), float(lr),
+                                                _lib.fptr(loss)))
+        return float(loss[0])
+
+
+@dataclass
+class SurrogateConfig:
+    """Feature scaling, output unit, training envelope and audit policy."""
+    floor: float = 1e-12          # x = (log(N + floor) - shift) * scale
+    shift: float = 0.0
+    scale: float = 0.1
+    wind_scale: float = 1.0 / 20.0
+    depth_feature: float = 1.0
+    rate_unit: float = 1e-5       # source = r * rate_unit * sigma * N
+    wind_lo: float = 0.0          # envelope: U10 in [wind_lo, wind_hi]
+    wind_hi: float = 1e9
+    action_hi: float = 3.0e38     # envelope: every bin's action <= action_hi
+    audit_stride: int = 0         # every audit_stride-th column is audited (0: none)
+    audit_bound: float = 0.1      # relative L2 source error above which an audit column falls back
+
+    def to_struct(self):
+        s = _lib.rpl_surrogate()
+        for f in fields(self):
+            setattr(s, f.name, getattr(self, f.name))
+        return s
+
+
+def training_pairs(model, cfg):
+    """Features (Nx*Ny, nbins + 3) and plain-bundle rate targets (Nx*Ny, nbins)
+    of the model's current action and clock; rows are columns, i fastest."""
+    Nx, Ny, Nk, Np = model.size
+    nb = Nk * Np
+    X = np.empty((Nx * Ny, nb + 3), dtype=np.float32)
+    T = np.empty((Nx * Ny, nb), dtype=np.float32)
+    s = cfg.to_struct()
+    _lib.check(_lib.load().rpl_surrogate_training_pairs(model._ctx.ptr, ctypes.byref(s), _lib.fptr(X),
+                                                        _lib.fptr(T)))
+    return X, T
+
+
+def attach_surrogate(model, net, cfg):
+    """Let the model's time step take its sources from `net` (None detaches)."""
+    lib = _lib.load()
+    if net is None:
+        _lib.check(lib.rpl_attach_surrogate(model._ctx.ptr, None, None))
+        model._surrogate = None
+        return
+    s = cfg.to_struct()
+    _lib.check(lib.rpl_attach_surrogate(model._ctx.ptr, net.ptr, ctypes.byref(s)))
+    model._surrogate = (net, cfg)  # keeps the network alive while attached
+
+
+def surrogate_stats(model):
+    """Counters since the last call: columns evaluated, plain-fallback columns,
+    audited columns, failed audits, largest audited relative error."""
+    v = [ctypes.c_int64() for _ in range(4)]
+    e = ctypes.c_double()
+    _lib.check(_lib.load().rpl_surrogate_stats(model._ctx.ptr, *[ctypes.byref(x) for x in v], ctypes.byref(e)))
+    return {"columns": v[0].value, "plain": v[1].value, "audited": v[2].value, "failed": v[3].value,
+            "err_max": e.value}
+
+
+def train(net, X, T, *, steps, batch=2048, lr=1e-3, seed=0, lr_final=None, log_every=0):
+    """Adam on random mini-batches of (X, T) rows; returns the loss history.
+
+    The learning rate decays geometrically from lr to lr_final (default lr/10).
+    """
+    rng = np.random.default_rng(seed)
+    batch = min(batch, net.batch_capacity, X.shape[0])
+    lr_final = lr / 10 if lr_final is None else lr_final
+    losses = []
+    for k in range(steps):
+        idx = rng.integers(0, X.shape[0], size=batch)
+        rate = lr * (lr_final / lr) ** (k / max(steps - 1, 1))
+        losses.append(net.train_step(X[idx], T[idx], rate))
+        if log_every and (k % log_every == 0 or k == steps - 1):
+            print(f"step {k}: loss {losses[-1]:.4e}", flush=True)
+    return np.array(losses)
+
+
+def collect_pairs(model, dt, steps, every, cfg, columns=None, seed=0):
+    """Advance `model` with its plain sources for `steps` steps, taking
+    training pairs every `every` steps (and at the start); `columns` caps the
+    number of columns kept per snapshot (random subset).  Returns (X, T)."""
+    from .model import time_step_
+    rng = np.random.default_rng(seed)
+    Xs, Ts = [], []
+    for k in range(steps + 1):
+        if k % every == 0:
+            X, T = training_pairs(model, cfg)
+            if columns is not None and columns < X.shape[0]:
+                idx = rng.choice(X.shape[0], size=columns, replace=False)
+                X, T = X[idx], T[idx]
+            Xs.append(X)
+            Ts.append(T)
+        if k < steps:
+            time_step_(model, dt)
+    return np.concatenate(Xs), np.concatenate(Ts)
+
+
+def envelope_from_pairs(X, cfg, margin=0.02):
+    """Set cfg's training envelope (wind range, largest action) from the
+    feature rows the network was trained on; returns cfg."""
+    nb = X.shape[1] - 3
+    x = X[:, :nb].astype(np.float64)
+    amax = float(np.max(np.exp(np.max(x) / cfg.scale + cfg.shift) - cfg.floor))
+    u = np.hypot(X[:, nb].astype(np.float64), X[:, nb + 1].astype(np.float64)) / cfg.wind_scale
+    cfg.wind_lo = float(max(0.0, u.min() * (1 - margin)))
+    cfg.wind_hi = float(u.max() * (1 + margin))
+    cfg.action_hi = amax * (1 + margin)
+    return cfg

@@ -1,0 +1,249 @@
+"""Learned source surrogate on the device: the BF16 matrix-core forward and
+backward passes against the CPU oracle (oracle/mlp_oracle.py, itself pinned
+by finite differences), the Adam step, training, the plain-bundle fallback
+paths, and what a trained network gains over running without sources."""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+import mlp_oracle as mo
+import ripple_amd as ra
+from ripple_amd import grids, workloads
+from ripple_amd import surrogate as sg
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(f"[gpu-test] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def rel_l2(a, b):
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
+
+
+def _net_and_batch(seed=0, in_dim=100, out_dim=70, hidden=64, layers=2, batch=100):
+    net = sg.SourceNetwork(in_dim, out_dim, hidden=hidden, hidden_layers=layers, batch_capacity=256, seed=seed)
+    rng = np.random.default_rng(seed)
+    X = rng.normal(size=(batch, in_dim)).astype(np.float32)
+    T = (0.3 * rng.normal(size=(batch, out_dim))).astype(np.float32)
+    return net, X, T
+
+
+def _oracle_inputs(net, X, T):
+    layers = mo.unpack(net.params(), net.dims)
+    rows = (X.shape[0] + 63) // 64 * 64
+    return layers, mo.pad_rows(X, net.in_pad, rows), mo.pad_rows(T, net.out_pad, rows)
+
+
+def test_forward_matches_oracle():
+    say("mlp forward vs oracle")
+    net, X, T = _net_and_batch()
+    layers, Xp, _ = _oracle_inputs(net, X, T)
+    Y = net.forward(X)
+    Ye, _ = mo.forward(layers, Xp, emulate=True)
+    Yx, _ = mo.forward(layers, Xp)
+    e_emu = rel_l2(Y, Ye[:X.shape[0], :net.out_dim])
+    e_exact = rel_l2(Y, Yx[:X.shape[0], :net.out_dim])
+    say(f"forward rel-L2 vs bf16-emulating oracle {e_emu:.2e}, vs FP64 oracle {e_exact:.2e}")
+    assert e_emu < 2e-3
+    assert e_exact < 3e-2
+    net.close()
+
+
+def test_gradients_match_oracle():
+    say("mlp gradients vs oracle")
+    net, X, T = _net_and_batch(1)
+    layers, Xp, Tp = _oracle_inputs(net, X, T)
+    loss, g = net.gradients(X, T)
+    Le, ge = mo.backward(layers, Xp, Tp, X.shape[0], net.out_dim, emulate=True)
+    Lx, gx = mo.backward(layers, Xp, Tp, X.shape[0], net.out_dim)
+    ge_flat = mo.pack(ge)
+    gx_flat = mo.pack(gx)
+    say(f"loss {loss:.6e} oracle {Le:.6e}; grad rel-L2 emu {rel_l2(g, ge_flat):.2e} exact {rel_l2(g, gx_flat):.2e}")
+    assert abs(loss - Le) / Le < 1e-4
+    assert rel_l2(g, ge_flat) < 5e-3
+    assert rel_l2(g, gx_flat) < 5e-2
+    # per-layer, so that a wrong layer cannot hide behind a large one
+    off = 0
+    for (W, b), (gW, gb) in zip(layers, ge):
+        nW = W.size
+        assert rel_l2(g[off:off + nW], gW.ravel()) < 1e-2
+        off += nW
+        assert rel_l2(g[off:off + b.size], gb) < 1e-2
+        off += b.size
+    net.close()
+
+
+def test_adam_step_matches_oracle():
+    say("mlp adam step")
+    net, X, T = _net_and_batch(2)
+    p0 = net.params()
+    _, g = net.gradients(X, T)
+    lr = 1e-3
+    net.train_step(X, T, lr)
+    p1 = net.params()
+    ref, _, _ = mo.adam(p0.astype(np.float64), g.astype(np.float64), 0.0, 0.0, 1, lr)
+    assert np.max(np.abs(p1 - ref)) < 1e-3 * lr
+    # padded weights stay zero
+    layers = net.layers(p1)
+    W0, _ = layers[0]
+    assert not np.any(W0[:, net.in_dim:])
+    assert not np.any(layers[-1][0][net.out_dim:, :])
+    net.close()
+
+
+def test_training_reduces_loss():
+    say("mlp training")
+    rng = np.random.default_rng(5)
+    net = sg.SourceNetwork(48, 40, hidden=128, hidden_layers=2, batch_capacity=512, seed=3)
+    X = rng.uniform(-1, 1, size=(4096, 48)).astype(np.float32)
+    A = rng.normal(size=(48, 40)) / 7
+    T = np.tanh(X @ A).astype(np.float32) + 0.5 * np.sin(X[:, :40] * 2).astype(np.float32)
+    l0, _ = net.gradients(X[:512], T[:512])
+    losses = sg.train(net, X, T, steps=400, batch=512, lr=2e-3)
+    l1, _ = net.gradients(X[:512], T[:512])
+    say(f"training loss {l0:.3e} -> {l1:.3e}")
+    assert l1 < l0 / 10
+    assert np.all(np.isfinite(losses))
+    net.close()
+
+
+def _storm(float_type, vmax=50.0, n=32, nkappa=12, nphi=12):
+    wl = workloads.translating_hurricane(float_type=float_type, nx=n, ny=n, nz=4, nkappa=nkappa, nphi=nphi,
+                                         dt=60.0)
+    s = wl.model_kwargs["sources"].wind_input.wind
+    s.vmax = vmax
+    s.rmax, s.radius = 20e3, 150e3
+    # a smaller box (200 km) so that the storm covers much of it
+    L = 200e3
+    g = wl.grid
+    grid = grids.RectilinearGrid(float_type=float_type, size=(n, n, g.Nz), x=(0, L), y=(-L / 2, L / 2),
+                                 z=g.z_faces, topology=(grids.PERIODIC, grids.PERIODIC, grids.BOUNDED))
+    s.track.centers = [(60e3, 0.0), (60e3 + 5.0 * s.track.times[1], 0.0)]
+    wl.grid = grid
+    wl.N0 = workloads.jonswap_action(grid, wl.spectral_grid, 0.5, 0.15)
+    return wl
+
+
+def _model(wl, float_type, sources=True):
+    kw = dict(wl.model_kwargs)
+    if not sources:
+        kw.pop("sources")
+    sgr = wl.spectral_grid
+    spec = grids.PolarWaveVectorGrid(float_type, kappa=sgr.kappa, phi=sgr.phi, kappa_faces=sgr.kappa_faces,
+                                     phi_faces=sgr.phi_faces)
+    m = ra.SpectralWaveModel(wl.grid, spec, velocities=None, **kw)
+    ra.set_(m, N=wl.N0)
+    return m
+
+
+@pytest.mark.parametrize("float_type", [np.float32, np.float64])
+def test_fallbacks_reproduce_the_plain_bundle(float_type):
+    """Envelope fallback on every column, and failed audits on every column,
+    both give the plain-source run exactly; the counters say so."""
+    say(f"surrogate fallbacks {np.dtype(float_type).name}")
+    wl = _storm(float_type)
+    nb = wl.spectral_grid.kappa.size * wl.spectral_grid.phi.size
+    plane = wl.grid.Nx * wl.grid.Ny
+    ref = _model(wl, float_type)
+    ra.time_step_(ref, wl.dt, 3)
+    net = sg.SourceNetwork(nb + 3, nb, hidden=64, hidden_layers=2, batch_capacity=256, seed=7)
+    for name, cfg in (("envelope", sg.SurrogateConfig(wind_lo=1e6, wind_hi=2e6)),
+                      ("audit", sg.SurrogateConfig(audit_stride=1, audit_bound=0.0))):
+        m = _model(wl, float_type)
+        sg.attach_surrogate(m, net, cfg)
+        ra.time_step_(m, wl.dt, 3)
+        st = sg.surrogate_stats(m)
+        say(f"{name}: {st}")
+        assert st["columns"] == 9 * plane
+        if name == "envelope":
+            assert st["plain"] == 9 * plane and st["audited"] == 0
+        else:
+            assert st["audited"] == 9 * plane and st["plain"] == 0
+            assert st["failed"] >= 9 * plane - 9 * np.count_nonzero(np.all(wl.N0 == 0, axis=(2, 3)))
+        assert np.array_equal(m.action, ref.action), rel_l2(m.action, ref.action)
+        sg.attach_surrogate(m, None, None)
+    net.close()
+
+
+def test_trained_surrogate_gains_over_no_sources():
+    """Train on plain-bundle pairs from storms of vmax 35, 40 and 50 m/s, run a
+    45 m/s storm with the network supplying the sources, and compare the
+    significant wave height and total action against the plain-source run and
+    against the run without sources."""
+    say("surrogate train + run")
+    ft = np.float32
+    nk, npp = 12, 12
+    nb = nk * npp
+    cfg = sg.SurrogateConfig(rate_unit=1.0, scale=0.1, floor=1e-12)
+    Xs, Ts = [], []
+    for vmax in (35.0, 40.0, 50.0):
+        m = _model(_storm(ft, vmax, nkappa=nk, nphi=npp), ft)
+        X, T = sg.collect_pairs(m, 60.0, 60, 4, cfg)
+        Xs.append(X)
+        Ts.append(T)
+    X, T = np.concatenate(Xs), np.concatenate(Ts)
+    unit = float(np.sqrt(np.mean(T[T != 0] ** 2)))
+    T = T / unit
+    cfg.rate_unit = unit
+    sg.envelope_from_pairs(X, cfg)
+    say(f"pairs {X.shape}, rate unit {unit:.3e}, envelope U10 [{cfg.wind_lo:.2f}, {cfg.wind_hi:.2f}], "
+        f"action <= {cfg.action_hi:.3e}")
+    net = sg.SourceNetwork(nb + 3, nb, hidden=256, hidden_layers=3, batch_capacity=1024, seed=11)
+    l0, _ = net.gradients(X[:1024], T[:1024])
+    sg.train(net, X, T, steps=4000, batch=1024, lr=1e-3, lr_final=2e-5)
+    l1, _ = net.gradients(X[:1024], T[:1024])
+    say(f"loss {l0:.3e} -> {l1:.3e}")
+    assert l1 < l0 / 10
+
+    wl = _storm(ft, 45.0, nkappa=nk, nphi=npp)
+    cfg.audit_stride, cfg.audit_bound = 16, 0.5
+    plain, surr, none = _model(wl, ft), _model(wl, ft), _model(wl, ft, sources=False)
+    sg.attach_surrogate(surr, net, cfg)
+    nsteps = 30
+    for m in (plain, surr, none):
+        ra.time_step_(m, wl.dt, nsteps)
+    st = sg.surrogate_stats(surr)
+    hs = [ra.significant_wave_height(m) for m in (plain, surr, none)]
+    ta = [ra.total_action(m) for m in (plain, surr, none)]
+    e_surr = rel_l2(hs[1], hs[0])
+    e_none = rel_l2(hs[2], hs[0])
+    a_surr = abs(ta[1] - ta[0]) / ta[0]
+    a_none = abs(ta[2] - ta[0]) / ta[0]
+    say(f"stats {st}")
+    say(f"Hs rel-L2 vs plain: surrogate {e_surr:.3e}, no sources {e_none:.3e}; "
+        f"total action rel: surrogate {a_surr:.3e}, no sources {a_none:.3e}")
+    assert st["plain"] < st["columns"] // 2
+    assert np.all(np.isfinite(surr.action))
+    assert e_surr < 0.25 * e_none
+    assert a_surr < 0.25 * a_none
+    net.close()
+
+
+def test_surrogate_error_paths():
+    say("surrogate error paths")
+    wl = _storm(np.float32)
+    m = _model(wl, np.float32, sources=False)
+    nb = wl.spectral_grid.kappa.size * wl.spectral_grid.phi.size
+    net = sg.SourceNetwork(nb + 3, nb, hidden=64, hidden_layers=1, batch_capacity=64)
+    with pytest.raises(ra.RippleError):
+        sg.attach_surrogate(m, net, sg.SurrogateConfig())  # no plain bundle installed
+    with pytest.raises(ra.RippleError):
+        sg.surrogate_stats(m)
+    m2 = _model(wl, np.float32)
+    bad = sg.SourceNetwork(nb + 2, nb, hidden=64, hidden_layers=1, batch_capacity=64)
+    with pytest.raises(ra.RippleArgumentError):
+        sg.attach_surrogate(m2, bad, sg.SurrogateConfig())
+    with pytest.raises(ra.RippleArgumentError):
+        sg.attach_surrogate(m2, net, sg.SurrogateConfig(rate_unit=0.0))
+    with pytest.raises(ValueError):
+        net.forward(np.zeros((65, nb + 3), np.float32))
+    with pytest.raises(ra.RippleArgumentError):
+        sg.SourceNetwork(10, 10, hidden=100)
+    net.close()
+    bad.close()
