@@ -196,7 +196,8 @@ int rpl_mlp_train_step(rpl_mlp *net, int32_t batch, const float *X, const float 
 /* Surrogate use in the time step: on every stage the network maps each
  * column's features to per-bin rates r (source = r * rate_unit * sigma_bin * N).
  * Columns outside the training envelope (U10 outside [wind_lo, wind_hi], an
- * action above action_hi) or with a non-finite output use the plain bundle,
+ * action above action_hi or above the bin's bin_action_hi) or with a
+ * non-finite output use the plain bundle,
  * which must be installed with rpl_set_sources (with a wind input).  Every
  * audit_stride-th column (0: none) is audited: its network source is compared
  * with the plain bundle's, and where the relative L2 error exceeds
@@ -210,7 +211,10 @@ typedef struct rpl_surrogate {
     int32_t audit_stride;
     float audit_bound;
 } rpl_surrogate;
-int rpl_attach_surrogate(rpl_ctx *ctx, rpl_mlp *net, const rpl_surrogate *cfg);
+/* bin_action_hi (nkappa*nphi values, may be NULL): the per-bin largest
+ * action of the training set; a column with any bin above it is outside the
+ * envelope. */
+int rpl_attach_surrogate(rpl_ctx *ctx, rpl_mlp *net, const rpl_surrogate *cfg, const float *bin_action_hi);
 /* Columns evaluated, of which by the plain bundle, audited, audit failures,
  * and the largest audited error since the last call (then reset). */
 int rpl_surrogate_stats(rpl_ctx *ctx, int64_t *columns, int64_t *plain, int64_t *audited, int64_t *failed,
@@ -218,7 +222,8 @@ int rpl_surrogate_stats(rpl_ctx *ctx, int64_t *columns, int64_t *plain, int64_t 
 /* The surrogate features of the current action and clock, (nx*ny) rows of
  * nbins + 3 FP32 values (bf16-representable: what the network sees), and the
  * plain bundle's per-cell rates r = source / (N sigma_bin rate_unit) as
- * network targets (0 where N <= 0), (nx*ny) rows of nbins.  Either output
+ * network targets (for N <= 1e-30, the rate at N = 1e-30), (nx*ny) rows of
+ * nbins.  Either output
  * may be NULL.  Column order is i fastest; nbins = nkappa * nphi, m fastest. */
 int rpl_surrogate_training_pairs(rpl_ctx *ctx, const rpl_surrogate *cfg, float *features, float *targets);
 

@@ -251,8 +251,9 @@ __global__ __launch_bounds__(256) void features_kernel(const T *__restrict__ N, 
 }
 
 // Envelope and audit decision per column: the plain bundle is used where the
-// wind is outside the training range, where the column's action is outside
-// it, where the network produced a non-finite rate, and on every audit column.
+// wind is outside the training range, where the column's action (overall, or
+// in any bin against that bin's training maximum) is outside it, and where
+// the network produced a non-finite rate; audit columns are flagged too.
 template <typename T>
 __global__ __launch_bounds__(256) void envelope_kernel(const T *__restrict__ N, size_t plane, int nbins,
                                                        const T *__restrict__ U10, const float *__restrict__ rate,
@@ -266,13 +267,16 @@ __global__ __launch_bounds__(256) void envelope_kernel(const T *__restrict__ N, 
     bool plain = !(u >= env.wind_lo && u <= env.wind_hi);
     float amax = 0.0f;
     bool finite = true;
+    bool over = false;
     for (int b = 0; b < nbins; ++b) {
         const float a = (float)N[col + plane * b];
         amax = fmaxf(amax, a);
+        if (env.bin_hi)
+            over = over || !(a <= env.bin_hi[b]);
         const float r = rate[col + ld * b];
         finite = finite && (r == r) && fabsf(r) < 3.0e38f;
     }
-    plain = plain || !(amax <= env.action_hi) || !finite;
+    plain = plain || !(amax <= env.action_hi) || over || !finite;
     const bool audit = env.audit_stride > 0 && (col % (size_t)env.audit_stride) == 0;
     use_plain[col] = (unsigned char)((plain ? 1 : 0) | (audit ? 2 : 0));
     if (plain)

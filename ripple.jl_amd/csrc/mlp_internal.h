@@ -43,6 +43,7 @@ struct FeatureScales {
 struct Envelope {
     float wind_lo, wind_hi;  // training wind range
     float action_hi;         // largest action seen in training
+    const float *bin_hi;     // per-bin largest action seen in training (nbins), or null
     int audit_stride;        // every audit_stride-th column is audited (0: none)
 };
 

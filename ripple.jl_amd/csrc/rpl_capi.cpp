@@ -72,7 +72,7 @@ struct rpl_ctx {
     rpl_surrogate scfg{};
     bool s_active = false;  // the stage takes the network's source where allowed
     rpl::mlp::bf16 *sX = nullptr, *sA = nullptr, *sB = nullptr;
-    float *rate = nullptr, *out_scale = nullptr, *s_err = nullptr;
+    float *rate = nullptr, *out_scale = nullptr, *s_err = nullptr, *bin_hi = nullptr;
     unsigned char *use_plain = nullptr;
     unsigned *s_counts = nullptr;
     size_t s_ld = 0, s_chunk = 0;
@@ -279,7 +279,7 @@ int surrogate_rates(rpl_ctx *c, const T *Nk)
             e = rpl::mlp::forward_rows(c->net, c->sX, (int)rows, c->sA, c->sB, c->rate + col0, c->s_ld, c->stream);
     }
     if (e == hipSuccess) {
-        rpl::mlp::Envelope env{g.wind_lo, g.wind_hi, g.action_hi, g.audit_stride};
+        rpl::mlp::Envelope env{g.wind_lo, g.wind_hi, g.action_hi, c->bin_hi, g.audit_stride};
         e = rpl::mlp::envelope<T>(Nk, c->plane, nbins, (const T *)c->U10, c->rate, c->s_ld, env, c->use_plain,
                                   c->s_counts, c->stream);
     }
@@ -417,10 +417,10 @@ namespace {
 void free_surrogate(rpl_ctx *c)
 {
     for (void *p : {(void *)c->sX, (void *)c->sA, (void *)c->sB, (void *)c->rate, (void *)c->out_scale,
-                    (void *)c->s_err, (void *)c->use_plain, (void *)c->s_counts})
+                    (void *)c->s_err, (void *)c->use_plain, (void *)c->s_counts, (void *)c->bin_hi})
         free_ptr(p);
     c->sX = c->sA = c->sB = nullptr;
-    c->rate = c->out_scale = c->s_err = nullptr;
+    c->rate = c->out_scale = c->s_err = c->bin_hi = nullptr;
     c->use_plain = nullptr;
     c->s_counts = nullptr;
     c->net = nullptr;
@@ -580,7 +580,8 @@ int rpl_destroy(rpl_ctx *c)
                     (void *)c->yn, c->U10, c->theta_u, c->ustar, c->factor, c->cm0, c->kbar, c->sbar, c->k_cell,
                     c->theta_cell, c->sigma_cell, c->C_cell, c->t_phi, c->t_w, c->t_sigma_k, c->t_cos, c->t_sin,
                     c->t_kpow, c->t_spow, (void *)c->sX, (void *)c->sA, (void *)c->sB, (void *)c->rate,
-                    (void *)c->out_scale, (void *)c->s_err, (void *)c->use_plain, (void *)c->s_counts})
+                    (void *)c->out_scale, (void *)c->s_err, (void *)c->use_plain, (void *)c->s_counts,
+                    (void *)c->bin_hi})
         free_ptr(p);
     for (auto &ev : c->events) {
         (void)hipEventDestroy(ev.first);
@@ -1186,7 +1187,7 @@ int rpl_get_surrogate_timing(rpl_ctx *c, double *total_ms, int64_t *evaluations)
     return RPL_OK;
 }
 
-int rpl_attach_surrogate(rpl_ctx *c, rpl_mlp *net, const rpl_surrogate *cfg)
+int rpl_attach_surrogate(rpl_ctx *c, rpl_mlp *net, const rpl_surrogate *cfg, const float *bin_action_hi)
 {
     if (!c)
         return fail(RPL_ERR_ARG, "null context");
@@ -1227,6 +1228,13 @@ int rpl_attach_surrogate(rpl_ctx *c, rpl_mlp *net, const rpl_surrogate *cfg)
     RPL_HIP(hipMemset(c->s_counts, 0, 16));
     RPL_HIP(hipMemset(c->s_err, 0, 16));
     RPL_HIP(hipMemset(c->use_plain, 1, c->s_ld));
+    if (bin_action_hi) {
+        if ((rc = alloc((void **)&c->bin_hi, nbins * sizeof(float)))) {
+            free_surrogate(c);
+            return rc;
+        }
+        RPL_HIP(hipMemcpy(c->bin_hi, bin_action_hi, nbins * sizeof(float), hipMemcpyHostToDevice));
+    }
     c->net = net;
     c->scfg = *cfg;
     c->s_active = true;

@@ -153,9 +153,11 @@ hipError_t launch_wind_columns(int nx, int ny, const double *xn, const double *y
 }
 
 namespace {
-// Per-cell plain-bundle rates normalised per bin, r = S / (N * unit[t]), and 0
-// where N <= 0: the learned surrogate's training targets.  Bin-major like the
-// action (r[ij + plane * t]).
+// Per-cell plain-bundle rates normalised per bin, r = S / (N * unit[t]): the
+// learned surrogate's training targets.  Where N <= 0 the target is the rate
+// of a vanishing action (evaluated at N = 1e-30), which is what the features
+// of an empty bin cannot tell apart from a nearly empty one.  Bin-major like
+// the action (r[ij + plane * t]).
 template <typename T>
 __global__ __launch_bounds__(256) void source_rates_kernel(int nx, int ny, const T *__restrict__ N,
                                                            SourceParams<T> s, const float *__restrict__ unit,
@@ -166,10 +168,9 @@ __global__ __launch_bounds__(256) void source_rates_kernel(int nx, int ny, const
     const int t = blockIdx.y;
     if (ij >= plane)
         return;
-    const T a = N[ij + plane * t];
-    float v = 0.0f;
-    if (a > T(0))
-        v = (float)(source_term<T>(s, (unsigned)ij, t, a) / a) / unit[t];
+    const T a0 = N[ij + plane * t];
+    const T a = a0 > T(1e-30) ? a0 : T(1e-30);
+    const float v = (float)(source_term<T>(s, (unsigned)ij, t, a) / a) / unit[t];
     r[ij + plane * t] = v;
 }
 }  // namespace

@@ -104,7 +104,7 @@ SIGNATURES = {
     "rpl_mlp_forward": ([_P, _I32, _FP, _FP], ctypes.c_int),
     "rpl_mlp_gradients": ([_P, _I32, _FP, _FP, _FP, _FP], ctypes.c_int),
     "rpl_mlp_train_step": ([_P, _I32, _FP, _FP, ctypes.c_float, _FP], ctypes.c_int),
-    "rpl_attach_surrogate": ([_P, _P, ctypes.POINTER(rpl_surrogate)], ctypes.c_int),
+    "rpl_attach_surrogate": ([_P, _P, ctypes.POINTER(rpl_surrogate), _FP], ctypes.c_int),
     "rpl_surrogate_stats": ([_P, _I64P, _I64P, _I64P, _I64P, _DP], ctypes.c_int),
     "rpl_surrogate_training_pairs": ([_P, ctypes.POINTER(rpl_surrogate), _FP, _FP], ctypes.c_int),
     "rpl_last_error": ([], ctypes.c_char_p),

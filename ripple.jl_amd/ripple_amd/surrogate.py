@@ -139,15 +139,24 @@ def training_pairs(model, cfg):
     return X, T
 
 
-def attach_surrogate(model, net, cfg):
-    """Let the model's time step take its sources from `net` (None detaches)."""
+def attach_surrogate(model, net, cfg, bin_action_hi=None):
+    """Let the model's time step take its sources from `net` (None detaches).
+
+    bin_action_hi: optional per-bin largest training action (nkappa*nphi,
+    m fastest); a column with any bin above it uses the plain bundle."""
     lib = _lib.load()
     if net is None:
-        _lib.check(lib.rpl_attach_surrogate(model._ctx.ptr, None, None))
+        _lib.check(lib.rpl_attach_surrogate(model._ctx.ptr, None, None, None))
         model._surrogate = None
         return
     s = cfg.to_struct()
-    _lib.check(lib.rpl_attach_surrogate(model._ctx.ptr, net.ptr, ctypes.byref(s)))
+    hi = None
+    if bin_action_hi is not None:
+        hi = np.ascontiguousarray(bin_action_hi, dtype=np.float32).ravel()
+        Nx, Ny, Nk, Np = model.size
+        if hi.size != Nk * Np:
+            raise ValueError(f"bin_action_hi needs {Nk * Np} values, got {hi.size}")
+    _lib.check(lib.rpl_attach_surrogate(model._ctx.ptr, net.ptr, ctypes.byref(s), _lib.fptr(hi)))
     model._surrogate = (net, cfg)  # keeps the network alive while attached
 
 
@@ -179,16 +188,23 @@ def train(net, X, T, *, steps, batch=2048, lr=1e-3, seed=0, lr_final=None, log_e
     return np.array(losses)
 
 
-def collect_pairs(model, dt, steps, every, cfg, columns=None, seed=0):
+def collect_pairs(model, dt, steps, every, cfg, columns=None, seed=0, return_bin_max=False):
     """Advance `model` with its plain sources for `steps` steps, taking
     training pairs every `every` steps (and at the start); `columns` caps the
-    number of columns kept per snapshot (random subset).  Returns (X, T)."""
+    number of columns kept per snapshot (random subset).  Returns (X, T), and
+    with return_bin_max also the per-bin largest action over the snapshots
+    (nkappa*nphi, m fastest), for attach_surrogate's bin_action_hi."""
     from .model import time_step_
     rng = np.random.default_rng(seed)
     Xs, Ts = [], []
+    Nx, Ny, Nk, Np = model.size
+    bin_max = np.zeros(Nk * Np)
     for k in range(steps + 1):
         if k % every == 0:
             X, T = training_pairs(model, cfg)
+            if return_bin_max:
+                a = np.asarray(model.action, dtype=np.float64).reshape(Nx * Ny, Nk * Np, order="F")
+                bin_max = np.maximum(bin_max, a.max(axis=0))
             if columns is not None and columns < X.shape[0]:
                 idx = rng.choice(X.shape[0], size=columns, replace=False)
                 X, T = X[idx], T[idx]
@@ -196,12 +212,16 @@ def collect_pairs(model, dt, steps, every, cfg, columns=None, seed=0):
             Ts.append(T)
         if k < steps:
             time_step_(model, dt)
+    if return_bin_max:
+        return np.concatenate(Xs), np.concatenate(Ts), bin_max
     return np.concatenate(Xs), np.concatenate(Ts)
 
 
 def envelope_from_pairs(X, cfg, margin=0.02):
     """Set cfg's training envelope (wind range, largest action) from the
-    feature rows the network was trained on; returns cfg."""
+    feature rows the network was trained on; returns cfg.  The features are
+    bf16, so the action bound is recovered to within bf16 rounding of the
+    log (see collect_pairs' return_bin_max for exact per-bin maxima)."""
     nb = X.shape[1] - 3
     x = X[:, :nb].astype(np.float64)
     amax = float(np.max(np.exp(np.max(x) / cfg.scale + cfg.shift) - cfg.floor))
@@ -210,3 +230,28 @@ def envelope_from_pairs(X, cfg, margin=0.02):
     cfg.wind_hi = float(u.max() * (1 + margin))
     cfg.action_hi = amax * (1 + margin)
     return cfg
+
+
+def fit(net, X, T, *, steps, batch=2048, lr=1e-3, lr_final=None, seed=0, log_every=0):
+    """Train `net` to map feature rows X to target rows T (both as returned by
+    `training_pairs`), with every output bin standardised over the training
+    set, then fold that affine map into the last layer (FP32 bias, the scale
+    multiplies the weights) so that the network emits rates in the
+    configuration's rate_unit.  Inputs are not standardised per feature:
+    folding 1/std into bf16 first-layer weights would cancel catastrophically
+    against the folded mean; the feature scaling of the configuration keeps
+    them O(1).  Returns the loss history (in standardised units)."""
+    T = np.asarray(T, dtype=np.float64)
+    mt, st = T.mean(axis=0), T.std(axis=0)
+    st = np.maximum(st, 1e-3 * max(float(np.sqrt(np.mean(T * T))), 1e-30))
+    Tn = ((T - mt) / st).astype(np.float32)
+    losses = train(net, np.asarray(X, dtype=np.float32), Tn, steps=steps, batch=batch, lr=lr, lr_final=lr_final,
+                   seed=seed, log_every=log_every)
+    flat = net.params().astype(np.float64)
+    layers = net.layers(flat)
+    WL, bL = layers[-1]
+    n_out = net.out_dim
+    WL[:n_out, :] *= st[:, None]
+    bL[:n_out] = st * bL[:n_out] + mt
+    net.set_params(flat.astype(np.float32))
+    return losses
