@@ -50,6 +50,21 @@ def stage_bytes_per_launch(nx, ny, nk, nphi, esize):
     return per_step / 3.0
 
 
+def measured_traffic(workload):
+    """HBM bytes per stage launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary of this workload (scaled by the copy-kernel factors
+    recorded there), or None if there is none for this workload."""
+    path = os.path.join(ROOT, "profiles", "stage_fast_kernel_hbm_traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if rec.get("workload") != workload:
+        return None
+    return rec["traffic_bytes_per_launch"]
+
+
 def time_model(model, dt, steps, warmup):
     lib, ctx = model._ctx.lib, model._ctx.ptr
     ra.time_step_(model, dt, warmup)
@@ -203,7 +218,7 @@ def main():
                    "dt": wl.dt, "transport": "fused WENO5 Doppler transport + refraction",
                    "sources": None, "timestepper": "SSP-RK3", "kernel": "fast"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(wl.name),
                      "kernel": "stage_fast_kernel", "bytes_per_launch": bpl,
                      "avg_launch_ms": t_launch, "launches_timed": launches},
     }
