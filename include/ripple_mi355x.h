@@ -238,6 +238,37 @@ int rpl_get_kernel_timing(rpl_ctx *ctx, double *total_ms, int64_t *launches);
  * same enable switch.  No counterpart in the reference. */
 int rpl_get_surrogate_timing(rpl_ctx *ctx, double *total_ms, int64_t *evaluations);
 
+/* Ensemble batching.  No counterpart in the reference, which runs one model
+ * per process.  Members are ordinary contexts -- FP32 on the fast kernel,
+ * SSP-RK3, one transport kind (intrinsic or refraction) and either all with
+ * or all without source terms -- whose sizes and spectral grids may differ.
+ * rpl_ensemble_step advances every member together: per RK3 stage, the
+ * members' source preparation (column state, surrogate inference) and then
+ * ONE persistent-kernel launch over a work list of all members' tiles (64 x
+ * tile_rows cells of one bin).  While in an ensemble, a member's work runs on
+ * the ensemble's stream; members must outlive the ensemble, and are used
+ * through the ensemble only (their own calls still work and are ordered on
+ * that stream).  Each member's clock advances as with rpl_time_step. */
+typedef struct rpl_ensemble rpl_ensemble;
+int rpl_ensemble_create(rpl_ensemble **out, rpl_ctx *const *members, int32_t count);
+int rpl_ensemble_destroy(rpl_ensemble *ens);
+int rpl_ensemble_step(rpl_ensemble *ens, double dt, int32_t nsteps);
+int rpl_ensemble_synchronize(rpl_ensemble *ens);
+/* Launch shape: tile_rows 2, 4 or 8 (0 keeps the current one) and the
+ * persistent grid size in workgroups (a multiple of 8; 0: 8 per CU). */
+int rpl_ensemble_set_launch(rpl_ensemble *ens, int32_t tile_rows, int32_t blocks);
+int rpl_ensemble_get_launch(rpl_ensemble *ens, int32_t *tile_rows, int32_t *blocks, int64_t *tiles);
+/* Times the stage-1 launch (which leaves the members' state unchanged) for
+ * tile_rows in {2, 4, 8} and grids of {2, 4, 8, 16} workgroups per CU or one
+ * workgroup per tile, reps launches each; keeps and returns the fastest. */
+int rpl_ensemble_autotune(rpl_ensemble *ens, double dt, int32_t reps, int32_t *tile_rows, int32_t *blocks,
+                          double *best_ms);
+/* Capture the three stages of a step (source preparation + persistent
+ * launches) in a hipGraph and replay it per step; the wind, which depends on
+ * the clock, is launched before each replay.  Recaptured when dt or the launch
+ * shape changes; call again after changing a member's sources or surrogate. */
+int rpl_ensemble_use_graph(rpl_ensemble *ens, int32_t enable);
+
 /* The context's hipStream_t (for callers that order their own work after it). */
 void *rpl_stream(rpl_ctx *ctx);
 

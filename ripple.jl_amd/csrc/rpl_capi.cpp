@@ -489,6 +489,107 @@ int training_pairs(rpl_ctx *c, const rpl_surrogate *cfg, float *features, float 
 }
 }  // namespace
 
+struct rpl_ensemble {
+    std::vector<rpl_ctx *> members;      // borrowed
+    std::vector<hipStream_t> own_stream; // each member's stream, restored on destroy
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int transport = 0;
+    bool src = false;
+    int ty = 4, blocks = 0, cus = 0;
+    rpl::EnsembleMember *desc = nullptr;  // 3 stages x members, device
+    long long total = 0;
+    double desc_dt = -1;
+    int desc_ty = 0;
+    bool use_graph = false;
+    hipGraphExec_t exec = nullptr;
+    double graph_dt = -1;
+    int graph_ty = 0, graph_blocks = 0;
+};
+
+namespace {
+
+// Stage buffers of an RK3 step: stage 1 A -> B, stage 2 (B, A) -> C, stage 3 (C, A) -> A.
+void stage_buffers(rpl_ctx *c, int stage, const void **Nk, const void **N0, void **out)
+{
+    *N0 = c->A;
+    if (stage == rpl::STAGE_1) {
+        *Nk = c->A;
+        *out = c->B;
+    } else if (stage == rpl::STAGE_2) {
+        *Nk = c->B;
+        *out = c->C;
+    } else {
+        *Nk = c->C;
+        *out = c->A;
+    }
+}
+
+int ensemble_descriptors(rpl_ensemble *e, double dt)
+{
+    const int M = (int)e->members.size();
+    std::vector<rpl::EnsembleMember> h(3 * (size_t)M);
+    long long t0 = 0;
+    for (int k = 0; k < M; ++k) {
+        rpl_ctx *c = e->members[k];
+        const int nbx = (c->cfg.nx + 63) / 64, nby = (c->cfg.ny + e->ty - 1) / e->ty;
+        for (int st = 0; st < 3; ++st) {
+            rpl::EnsembleMember &d = h[(size_t)st * M + k];
+            d.p = stage_params<float>(c, dt);
+            const void *Nk, *N0;
+            void *out;
+            stage_buffers(c, st + 1, &Nk, &N0, &out);
+            d.Nk = (const float *)Nk;
+            d.N0 = (const float *)N0;
+            d.out = (float *)out;
+            d.nbx = nbx;
+            d.nby = nby;
+            d.tile0 = t0;
+        }
+        t0 += (long long)nbx * nby * c->cfg.nkappa * c->cfg.nphi;
+    }
+    e->total = t0;
+    RPL_HIP(hipMemcpy(e->desc, h.data(), h.size() * sizeof(rpl::EnsembleMember), hipMemcpyHostToDevice));
+    e->desc_dt = dt;
+    e->desc_ty = e->ty;
+    return RPL_OK;
+}
+
+// Sources and persistent stage launches of one RK3 step (no wind: that is
+// evaluated at the clock time before each step).
+int ensemble_stages(rpl_ensemble *e)
+{
+    const int M = (int)e->members.size();
+    for (int st = rpl::STAGE_1; st <= rpl::STAGE_3; ++st) {
+        if (e->src)
+            for (rpl_ctx *c : e->members) {
+                const void *Nk, *N0;
+                void *out;
+                stage_buffers(c, st, &Nk, &N0, &out);
+                int rc = launch_sources(c, Nk);
+                if (rc)
+                    return rc;
+            }
+        hipError_t err = rpl::launch_ensemble_stage(e->desc + (size_t)(st - 1) * M, M, e->total, e->transport, e->src,
+                                                    st, e->ty, e->blocks, e->stream);
+        if (err != hipSuccess)
+            return fail(RPL_ERR_HIP, std::string("ensemble stage launch: ") + hipGetErrorString(err));
+    }
+    return RPL_OK;
+}
+
+void drop_graph(rpl_ensemble *e)
+{
+    if (e->exec)
+        (void)hipGraphExecDestroy(e->exec);
+    e->exec = nullptr;
+    e->graph_dt = -1;
+}
+
+int default_blocks(const rpl_ensemble *e) { return std::max(8, e->cus * 8); }
+
+}  // namespace
+
 extern "C" {
 
 const char *rpl_last_error(void) { return g_last_error.c_str(); }
@@ -1290,6 +1391,262 @@ int rpl_surrogate_training_pairs(rpl_ctx *c, const rpl_surrogate *cfg, float *fe
                        : training_pairs<float>(c, cfg, features, targets);
     c->s_active = was;
     return rc;
+}
+
+int rpl_ensemble_create(rpl_ensemble **out, rpl_ctx *const *members, int32_t count)
+{
+    if (!out || !members || count < 1)
+        return fail(RPL_ERR_ARG, "need an output pointer and at least one member");
+    *out = nullptr;
+    rpl_ctx *first = members[0];
+    for (int k = 0; k < count; ++k) {
+        rpl_ctx *c = members[k];
+        if (!c)
+            return fail(RPL_ERR_ARG, "null member");
+        int rc = check_ready(c);
+        if (rc)
+            return rc;
+        for (int q = 0; q < k; ++q)
+            if (members[q] == c)
+                return fail(RPL_ERR_ARG, "a context appears twice in the ensemble");
+        if (c->cfg.dtype != RPL_F32 || !use_fast(c))
+            return fail(RPL_ERR_UNSUPPORTED, "ensemble members must be FP32 contexts on the fast kernel");
+        if (c->cfg.timestepper == RPL_FORWARD_EULER)
+            return fail(RPL_ERR_UNSUPPORTED, "ensemble members must use SSP-RK3");
+        if (c->cfg.transport != first->cfg.transport || c->cfg.transport == RPL_TRANSPORT_NONE)
+            return fail(RPL_ERR_UNSUPPORTED, "ensemble members need the same transport (intrinsic or refraction)");
+        if (c->has_sources != first->has_sources)
+            return fail(RPL_ERR_UNSUPPORTED, "either every member or no member has source terms");
+        if (c->device != first->device)
+            return fail(RPL_ERR_ARG, "ensemble members must live on one device");
+    }
+    rpl_ensemble *e = new rpl_ensemble();
+    e->device = first->device;
+    e->transport = first->cfg.transport;
+    e->src = first->has_sources;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, e->device) != hipSuccess ||
+        hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete e;
+        return fail(RPL_ERR_HIP, "ensemble stream creation");
+    }
+    e->cus = prop.multiProcessorCount;
+    e->blocks = default_blocks(e);
+    if (hipMalloc((void **)&e->desc, 3 * (size_t)count * sizeof(rpl::EnsembleMember)) != hipSuccess) {
+        (void)hipStreamDestroy(e->stream);
+        delete e;
+        return fail(RPL_ERR_NOMEM, "ensemble descriptors");
+    }
+    for (int k = 0; k < count; ++k) {
+        rpl_ctx *c = members[k];
+        (void)hipStreamSynchronize(c->stream);
+        (void)fold_timing(c);
+        c->timing = false;
+        e->members.push_back(c);
+        e->own_stream.push_back(c->stream);
+        c->stream = e->stream;  // every member's work is ordered on the ensemble stream
+    }
+    *out = e;
+    return RPL_OK;
+}
+
+int rpl_ensemble_destroy(rpl_ensemble *e)
+{
+    if (!e)
+        return RPL_OK;
+    (void)hipSetDevice(e->device);
+    (void)hipStreamSynchronize(e->stream);
+    drop_graph(e);
+    for (size_t k = 0; k < e->members.size(); ++k)
+        e->members[k]->stream = e->own_stream[k];
+    if (e->desc)
+        (void)hipFree(e->desc);
+    (void)hipStreamDestroy(e->stream);
+    delete e;
+    return RPL_OK;
+}
+
+int rpl_ensemble_set_launch(rpl_ensemble *e, int32_t tile_rows, int32_t blocks)
+{
+    if (!e)
+        return fail(RPL_ERR_ARG, "null ensemble");
+    if (tile_rows != 0 && tile_rows != 2 && tile_rows != 4 && tile_rows != 8)
+        return fail(RPL_ERR_ARG, "tile_rows must be 2, 4 or 8 (0: keep)");
+    if (blocks < 0 || blocks % 8)
+        return fail(RPL_ERR_ARG, "blocks must be a multiple of 8 (0: default)");
+    if (tile_rows)
+        e->ty = tile_rows;
+    e->blocks = blocks ? blocks : default_blocks(e);
+    return RPL_OK;
+}
+
+int rpl_ensemble_get_launch(rpl_ensemble *e, int32_t *tile_rows, int32_t *blocks, int64_t *tiles)
+{
+    if (!e)
+        return fail(RPL_ERR_ARG, "null ensemble");
+    if (tile_rows)
+        *tile_rows = e->ty;
+    if (blocks)
+        *blocks = e->blocks;
+    if (tiles) {
+        long long t = 0;
+        for (rpl_ctx *c : e->members)
+            t += (long long)((c->cfg.nx + 63) / 64) * ((c->cfg.ny + e->ty - 1) / e->ty) * c->cfg.nkappa *
+                 c->cfg.nphi;
+        *tiles = t;
+    }
+    return RPL_OK;
+}
+
+int rpl_ensemble_use_graph(rpl_ensemble *e, int32_t enable)
+{
+    if (!e)
+        return fail(RPL_ERR_ARG, "null ensemble");
+    RPL_HIP(hipSetDevice(e->device));
+    RPL_HIP(hipStreamSynchronize(e->stream));
+    drop_graph(e);
+    e->use_graph = enable != 0;
+    return RPL_OK;
+}
+
+int rpl_ensemble_step(rpl_ensemble *e, double dt, int32_t nsteps)
+{
+    if (!e)
+        return fail(RPL_ERR_ARG, "null ensemble");
+    if (!(dt > 0))
+        return fail(RPL_ERR_ARG, "time step must be positive");
+    if (nsteps < 0)
+        return fail(RPL_ERR_ARG, "nsteps must be >= 0");
+    RPL_HIP(hipSetDevice(e->device));
+    int rc;
+    for (rpl_ctx *c : e->members)
+        if ((rc = check_ready(c)))
+            return rc;
+    // Parameters may change between calls (sources, surrogate attachment);
+    // the descriptors are small, so they are rebuilt on every call.
+    RPL_HIP(hipStreamSynchronize(e->stream));
+    if ((rc = ensemble_descriptors(e, dt)))
+        return rc;
+    if (e->use_graph && (e->exec == nullptr || e->graph_dt != dt || e->graph_ty != e->ty ||
+                         e->graph_blocks != e->blocks)) {
+        drop_graph(e);
+        hipGraph_t g = nullptr;
+        RPL_HIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+        rc = ensemble_stages(e);
+        hipError_t err = hipStreamEndCapture(e->stream, &g);
+        if (rc)
+            return rc;
+        if (err != hipSuccess)
+            return fail(RPL_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(err));
+        err = hipGraphInstantiate(&e->exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (err != hipSuccess) {
+            e->exec = nullptr;
+            return fail(RPL_ERR_HIP, std::string("graph instantiation: ") + hipGetErrorString(err));
+        }
+        e->graph_dt = dt;
+        e->graph_ty = e->ty;
+        e->graph_blocks = e->blocks;
+    }
+    for (int s = 0; s < nsteps; ++s) {
+        for (rpl_ctx *c : e->members)
+            if ((rc = launch_wind(c)))
+                return rc;
+        if (e->use_graph) {
+            RPL_HIP(hipGraphLaunch(e->exec, e->stream));
+        } else if ((rc = ensemble_stages(e))) {
+            return rc;
+        }
+        for (rpl_ctx *c : e->members) {
+            c->time += dt;
+            c->iteration += 1;
+        }
+    }
+    return RPL_OK;
+}
+
+int rpl_ensemble_synchronize(rpl_ensemble *e)
+{
+    if (!e)
+        return fail(RPL_ERR_ARG, "null ensemble");
+    RPL_HIP(hipSetDevice(e->device));
+    RPL_HIP(hipStreamSynchronize(e->stream));
+    return RPL_OK;
+}
+
+int rpl_ensemble_autotune(rpl_ensemble *e, double dt, int32_t reps, int32_t *tile_rows, int32_t *blocks,
+                          double *best_ms)
+{
+    if (!e)
+        return fail(RPL_ERR_ARG, "null ensemble");
+    if (!(dt > 0) || reps < 1)
+        return fail(RPL_ERR_ARG, "need dt > 0 and reps >= 1");
+    RPL_HIP(hipSetDevice(e->device));
+    RPL_HIP(hipStreamSynchronize(e->stream));
+    drop_graph(e);
+    int rc;
+    // Stage 1 reads A and writes the scratch buffer B, so timing it leaves
+    // every member's state unchanged.  Sources are prepared once on A.
+    for (rpl_ctx *c : e->members)
+        if ((rc = launch_wind(c)) || (e->src && (rc = launch_sources(c, c->A))))
+            return rc;
+    hipEvent_t a, b;
+    RPL_HIP(hipEventCreate(&a));
+    RPL_HIP(hipEventCreate(&b));
+    const int M = (int)e->members.size();
+    double best = 1e300;
+    int best_ty = e->ty, best_blocks = e->blocks;
+    const int tys[3] = {2, 4, 8};
+    const int mult[5] = {2, 4, 8, 16, 0};  // workgroups per CU; 0: one workgroup per tile
+    hipError_t err = hipSuccess;
+    for (int ty : tys) {
+        e->ty = ty;
+        if ((rc = ensemble_descriptors(e, dt)))
+            break;
+        for (int mu : mult) {
+            long long g = mu ? (long long)e->cus * mu : (e->total + 7) / 8 * 8;
+            g = std::max<long long>(8, std::min<long long>(g, 0x7ffffff8LL) / 8 * 8);
+            err = rpl::launch_ensemble_stage(e->desc, M, e->total, e->transport, e->src, rpl::STAGE_1, ty, (int)g,
+                                             e->stream);  // warm-up
+            if (err == hipSuccess)
+                err = hipEventRecord(a, e->stream);
+            for (int r = 0; r < reps && err == hipSuccess; ++r)
+                err = rpl::launch_ensemble_stage(e->desc, M, e->total, e->transport, e->src, rpl::STAGE_1, ty, (int)g,
+                                                 e->stream);
+            if (err == hipSuccess)
+                err = hipEventRecord(b, e->stream);
+            if (err == hipSuccess)
+                err = hipEventSynchronize(b);
+            float ms = 0;
+            if (err == hipSuccess)
+                err = hipEventElapsedTime(&ms, a, b);
+            if (err != hipSuccess)
+                break;
+            if (ms / reps < best) {
+                best = ms / reps;
+                best_ty = ty;
+                best_blocks = (int)g;
+            }
+        }
+        if (err != hipSuccess)
+            break;
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (rc)
+        return rc;
+    if (err != hipSuccess)
+        return fail(RPL_ERR_HIP, std::string("autotune: ") + hipGetErrorString(err));
+    e->ty = best_ty;
+    e->blocks = best_blocks;
+    e->desc_dt = -1;
+    if (tile_rows)
+        *tile_rows = best_ty;
+    if (blocks)
+        *blocks = best_blocks;
+    if (best_ms)
+        *best_ms = best;
+    return RPL_OK;
 }
 
 void *rpl_stream(rpl_ctx *c) { return c ? (void *)c->stream : nullptr; }

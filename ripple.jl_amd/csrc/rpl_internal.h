@@ -30,6 +30,17 @@ struct StageParams {
     SourceParams<T> src;              // source terms (src.on == 0: none)
 };
 
+// One ensemble member's stage launch in the work-list kernel: its stage
+// parameters, buffers and tile numbering (64 x TY tiles per bin; nby =
+// ceil(ny / TY); tile0 = tiles of all earlier members).
+struct EnsembleMember {
+    StageParams<float> p;
+    const float *Nk, *N0;
+    float *out;
+    int nbx, nby;
+    long long tile0;
+};
+
 // Wind of one step (Forcing/winds.jl), FP64.
 struct WindParams {
     int field, drag, rotation;
@@ -70,6 +81,10 @@ hipError_t launch_stage_reference(const StageParams<double> &p, int transport, i
                                   const double *Nk, const double *N0, double *out, hipStream_t s);
 hipError_t launch_stage_reference(const StageParams<float> &p, int transport, int stage,
                                   const float *Nk, const float *N0, float *out, hipStream_t s);
+// One persistent launch of stage `stage` (STAGE_1..3) over every member's
+// tiles; `blocks` (a multiple of 8) workgroups of 64 x ty threads.
+hipError_t launch_ensemble_stage(const EnsembleMember *mem, int members, long long total, int transport, bool src,
+                                 int stage, int ty, int blocks, hipStream_t s);
 hipError_t launch_stage_fast(const StageParams<float> &p, int transport, int stage,
                              const float *Nk, const float *N0, float *out, hipStream_t s);
 

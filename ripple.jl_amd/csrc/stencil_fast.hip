@@ -145,14 +145,16 @@ __device__ __forceinline__ Tile tile_of(int b, int total, int nbx, int nby, int 
     return t;
 }
 
-template <int TRANSPORT, int STAGE, bool SRC>
-__global__ __launch_bounds__(256) void stage_fast_kernel(const float *__restrict__ Nk, const float *__restrict__ N0,
-                                                         float *out, StageParams<float> p, int nbx, int nby)
+// One 64 x TY (i, j) tile of bin (m, n): tendency, sources and the SSP-RK3
+// stage update for each cell.  Shared by the per-model kernel below and the
+// ensemble work-list kernel; p may live in kernel arguments or in global
+// memory (block-uniform either way).
+template <int TRANSPORT, int STAGE, bool SRC, int TY>
+__device__ __forceinline__ void stage_tile(const float *__restrict__ Nk, const float *__restrict__ N0, float *out,
+                                           const StageParams<float> &p, int ib, int jb, int m, int n)
 {
     const int nx = p.nx, ny = p.ny, nk = p.nk, nphi = p.nphi;
-    const Tile t = tile_of(blockIdx.x, nbx * nby * nk * nphi, nbx, nby, nk, nphi);
-    const int m = t.m, n = t.n;
-    const int i0 = t.ib * 64, j0 = t.jb * 4;
+    const int i0 = ib * 64, j0 = jb * TY;
     const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
     if (i >= nx || j >= ny)
         return;
@@ -165,7 +167,7 @@ __global__ __launch_bounds__(256) void stage_fast_kernel(const float *__restrict
     const unsigned bo = off << 2;
     const float x0 = ldo(slab, bo);
     const bool x_inner = i0 >= 3 && i0 + 64 + 3 <= nx;  // block-uniform
-    const bool y_inner = j0 >= 3 && j0 + 4 + 3 <= ny;
+    const bool y_inner = j0 >= 3 && j0 + TY + 3 <= ny;
     float L = 0.0f;
     if (TRANSPORT != RPL_TRANSPORT_NONE) {
         const unsigned rowb = (nxu * (unsigned)j) << 2, colb = (unsigned)i << 2;
@@ -272,6 +274,47 @@ __global__ __launch_bounds__(256) void stage_fast_kernel(const float *__restrict
     *reinterpret_cast<float *>(reinterpret_cast<char *>(dst) + bo) = r;
 }
 
+template <int TRANSPORT, int STAGE, bool SRC>
+__global__ __launch_bounds__(256) void stage_fast_kernel(const float *__restrict__ Nk, const float *__restrict__ N0,
+                                                         float *out, StageParams<float> p, int nbx, int nby)
+{
+    const Tile t = tile_of(blockIdx.x, nbx * nby * p.nk * p.nphi, nbx, nby, p.nk, p.nphi);
+    stage_tile<TRANSPORT, STAGE, SRC, 4>(Nk, N0, out, p, t.ib, t.jb, t.m, t.n);
+}
+
+// Ensemble work list: every member's tiles (64 x TY cells of one bin) are
+// numbered consecutively, member after member (EnsembleMember::tile0 is the
+// prefix sum).  The grid is persistent: workgroups are dealt round-robin to
+// the eight XCDs, so XCD x takes the contiguous range [x T / 8, (x+1) T / 8)
+// of the T tiles and its workgroups stride through it; within a member the
+// order is phi fastest, then kappa, tile row, tile column, as in tile_of.
+template <int TRANSPORT, int STAGE, bool SRC, int TY>
+__global__ __launch_bounds__(64 * TY) void ensemble_stage_kernel(const EnsembleMember *__restrict__ mem, int members,
+                                                                 long long total)
+{
+    const int xcd = blockIdx.x & 7;
+    const long long per_block = (long long)(gridDim.x >> 3);
+    const long long lo = total * xcd / 8, hi = total * (xcd + 1) / 8;
+    for (long long L = lo + (blockIdx.x >> 3); L < hi; L += per_block) {
+        int a = 0, b = members - 1;  // last member with tile0 <= L
+        while (a < b) {
+            const int c = (a + b + 1) >> 1;
+            if (mem[c].tile0 <= L)
+                a = c;
+            else
+                b = c - 1;
+        }
+        const EnsembleMember &e = mem[a];
+        int q = (int)(L - e.tile0);
+        const int n = q % e.p.nphi;
+        q /= e.p.nphi;
+        const int m = q % e.p.nk;
+        q /= e.p.nk;
+        const int jb = q % e.nby, ib = q / e.nby;
+        stage_tile<TRANSPORT, STAGE, SRC, TY>(e.Nk, e.N0, e.out, e.p, ib, jb, m, n);
+    }
+}
+
 template <int TRANSPORT, bool SRC>
 hipError_t launch_t(const StageParams<float> &p, int stage, const float *Nk, const float *N0, float *out,
                     hipStream_t s)
@@ -299,7 +342,56 @@ hipError_t launch_t(const StageParams<float> &p, int stage, const float *Nk, con
     return hipGetLastError();
 }
 
+template <int TRANSPORT, int STAGE, bool SRC>
+hipError_t launch_ens_ty(const EnsembleMember *mem, int members, long long total, int ty, int blocks, hipStream_t s)
+{
+    switch (ty) {
+    case 2:
+        hipLaunchKernelGGL((ensemble_stage_kernel<TRANSPORT, STAGE, SRC, 2>), dim3(blocks), dim3(64, 2), 0, s, mem,
+                           members, total);
+        break;
+    case 8:
+        hipLaunchKernelGGL((ensemble_stage_kernel<TRANSPORT, STAGE, SRC, 8>), dim3(blocks), dim3(64, 8), 0, s, mem,
+                           members, total);
+        break;
+    default:
+        hipLaunchKernelGGL((ensemble_stage_kernel<TRANSPORT, STAGE, SRC, 4>), dim3(blocks), dim3(64, 4), 0, s, mem,
+                           members, total);
+        break;
+    }
+    return hipGetLastError();
+}
+
+template <int TRANSPORT, bool SRC>
+hipError_t launch_ens_stage(const EnsembleMember *mem, int members, long long total, int stage, int ty, int blocks,
+                            hipStream_t s)
+{
+    switch (stage) {
+    case STAGE_1:
+        return launch_ens_ty<TRANSPORT, STAGE_1, SRC>(mem, members, total, ty, blocks, s);
+    case STAGE_2:
+        return launch_ens_ty<TRANSPORT, STAGE_2, SRC>(mem, members, total, ty, blocks, s);
+    default:
+        return launch_ens_ty<TRANSPORT, STAGE_3, SRC>(mem, members, total, ty, blocks, s);
+    }
+}
+
 }  // namespace
+
+hipError_t launch_ensemble_stage(const EnsembleMember *mem, int members, long long total, int transport, bool src,
+                                 int stage, int ty, int blocks, hipStream_t s)
+{
+    if (members < 1 || total < 1 || blocks < 8 || blocks % 8 || (ty != 2 && ty != 4 && ty != 8) ||
+        stage < STAGE_1 || stage > STAGE_3)
+        return hipErrorInvalidValue;
+    if (transport == RPL_TRANSPORT_REFRACTION)
+        return src ? launch_ens_stage<RPL_TRANSPORT_REFRACTION, true>(mem, members, total, stage, ty, blocks, s)
+                   : launch_ens_stage<RPL_TRANSPORT_REFRACTION, false>(mem, members, total, stage, ty, blocks, s);
+    if (transport == RPL_TRANSPORT_INTRINSIC)
+        return src ? launch_ens_stage<RPL_TRANSPORT_INTRINSIC, true>(mem, members, total, stage, ty, blocks, s)
+                   : launch_ens_stage<RPL_TRANSPORT_INTRINSIC, false>(mem, members, total, stage, ty, blocks, s);
+    return hipErrorInvalidValue;
+}
 
 hipError_t launch_stage_fast(const StageParams<float> &p, int transport, int stage, const float *Nk, const float *N0,
                              float *out, hipStream_t s)

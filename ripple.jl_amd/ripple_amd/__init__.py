@@ -16,6 +16,7 @@ from .sources import (BulkWindDrag, LinearStormTrack, HollandHurricaneWind, Pres
                       MeanSpectrumWhitecapping, LocalSaturationDissipation, PrecomputedSources, wind_state)
 from .surrogate import (SourceNetwork, SurrogateConfig, training_pairs, attach_surrogate, surrogate_stats,
                         train as train_surrogate)
+from .ensemble import Ensemble
 from .diagnostics import (m0, root_mean_square_wavenumber, mean_direction, significant_wave_height,
                           total_action, total_deep_water_energy)
 
@@ -30,5 +31,5 @@ __all__ = [
     "BulkWindDrag", "LinearStormTrack", "HollandHurricaneWind", "PressureCorrelationInput",
     "MeanSpectrumWhitecapping", "LocalSaturationDissipation", "PrecomputedSources", "wind_state",
     "SourceNetwork", "SurrogateConfig", "training_pairs", "attach_surrogate", "surrogate_stats",
-    "train_surrogate",
+    "train_surrogate", "Ensemble",
 ]

@@ -96,6 +96,14 @@ SIGNATURES = {
     "rpl_get_kernel_timing": ([_P, _DP, _I64P], ctypes.c_int),
     "rpl_get_surrogate_timing": ([_P, _DP, _I64P], ctypes.c_int),
     "rpl_stream": ([_P], ctypes.c_void_p),
+    "rpl_ensemble_create": ([ctypes.POINTER(_P), ctypes.POINTER(_P), _I32], ctypes.c_int),
+    "rpl_ensemble_destroy": ([_P], ctypes.c_int),
+    "rpl_ensemble_step": ([_P, _D, _I32], ctypes.c_int),
+    "rpl_ensemble_synchronize": ([_P], ctypes.c_int),
+    "rpl_ensemble_set_launch": ([_P, _I32, _I32], ctypes.c_int),
+    "rpl_ensemble_get_launch": ([_P, ctypes.POINTER(_I32), ctypes.POINTER(_I32), _I64P], ctypes.c_int),
+    "rpl_ensemble_autotune": ([_P, _D, _I32, ctypes.POINTER(_I32), ctypes.POINTER(_I32), _DP], ctypes.c_int),
+    "rpl_ensemble_use_graph": ([_P, _I32], ctypes.c_int),
     "rpl_mlp_create": ([ctypes.POINTER(_P), _I32, _I32, _I32, _I32, _I32, ctypes.c_uint64], ctypes.c_int),
     "rpl_mlp_destroy": ([_P], ctypes.c_int),
     "rpl_mlp_dims": ([_P, ctypes.POINTER(_I32), ctypes.POINTER(_I32), _I64P], ctypes.c_int),

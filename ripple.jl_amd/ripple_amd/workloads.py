@@ -178,3 +178,40 @@ def translating_hurricane(float_type=np.float32, nx=1024, ny=1024, nz=32, nkappa
     N0 = jonswap_action(grid, sg, 0.5, 0.15)
     return Workload(f"translating_hurricane_{nx}x{ny}x{nz}_k{nkappa}_p{nphi}", grid, sg, None, N0, dt, nsteps,
                     dict(timestepper="RK3", sources=sources, depth=200.0))
+
+
+def ensemble_member(index, float_type=np.float32, nz=16, sources=True, size_range=(64, 512),
+                    kappa_range=(16, 40), phi_range=(24, 48), seed_base=0):
+    """One member of the batched ensemble, drawn from seed seed_base + index:
+    an n x n periodic box of 80 x 80 (n uniform in size_range), depth 1 with
+    nz levels; kappa bins (count uniform in kappa_range) log-spaced 0.05-2,
+    phi bins (count uniform in phi_range); its own eddy current (peak
+    wavelength 20, rms speed uniform in 0.1-0.5) times exp(z/0.2); uniform
+    wind of speed uniform in 5-25 m/s and random direction, driving
+    PrecomputedSources(PressureCorrelationInput, LocalSaturationDissipation)
+    when `sources`; the narrow-band spectrum at kappa0 = 0.4; dt = 0.02."""
+    from .sources import PrecomputedSources, PressureCorrelationInput, LocalSaturationDissipation, BulkWindDrag
+    rng = np.random.Generator(np.random.Philox(seed_base + index))
+    n = int(rng.integers(size_range[0], size_range[1] + 1))
+    nk = int(rng.integers(kappa_range[0], kappa_range[1] + 1))
+    nphi = int(rng.integers(phi_range[0], phi_range[1] + 1))
+    rms = float(rng.uniform(0.1, 0.5))
+    wind = float(rng.uniform(5.0, 25.0))
+    direction = float(rng.uniform(0.0, 2 * math.pi))
+    Lx = Ly = 80.0
+    grid = RectilinearGrid(float_type=float_type, size=(n, n, nz), x=(0, Lx), y=(0, Ly), z=(-1, 0),
+                           halo=(3, 3, 3), topology=(PERIODIC, PERIODIC, BOUNDED))
+    sg = PolarWaveVectorGrid(float_type, kappa=np.geomspace(0.05, 2.0, nk),
+                             phi=np.linspace(0, 2 * math.pi, nphi + 1)[:nphi])
+    u2, v2 = eddy_streamfunction(n, n, Lx, Ly, 20.0, rms, DEFAULT_SEED + index)
+    zc = 0.5 * (grid.z_faces[1:] + grid.z_faces[:-1])
+    decay = np.exp(zc / 0.2)
+    u = np.asfortranarray(u2[:, :, None] * decay[None, None, :])
+    v = np.asfortranarray(v2[:, :, None] * decay[None, None, :])
+    kw = dict(timestepper="RK3")
+    if sources:
+        kw["sources"] = PrecomputedSources(
+            wind_input=PressureCorrelationInput(drag=BulkWindDrag("linear"), wind=wind, direction=direction),
+            dissipation=LocalSaturationDissipation())
+    name = f"ensemble_member_{index}_{n}x{n}x{nz}_k{nk}_p{nphi}_U{wind:.1f}"
+    return Workload(name, grid, sg, (u, v), narrow_band_action(grid, sg), 0.02, 200, kw)
