@@ -127,6 +127,72 @@ def storm_plain_vs_surrogate(steps, warmup):
             "step_speedup_surrogate_over_plain": t_plain / t_surr}
 
 
+def ensemble_line(count, steps, with_surrogate=True, **member_kw):
+    """The batched ensemble: `count` members drawn as ensemble_member(0..count-1)
+    (64^2-512^2 cells x 16 levels, 16-40 kappa x 24-48 phi bins, own eddy
+    current and uniform wind, wind input + local-saturation sources), FP32.
+    Times the members stepped one after another (rpl_time_step each) and the
+    ensemble (persistent launch per stage, autotuned shape, hipGraph replay),
+    first with the plain sources, then with every member's sources from a
+    512 x 3 network of its own bin count (random weights, open envelope: a
+    timing run)."""
+    from ripple_amd import surrogate as sg
+    models, bins, cells = [], [], 0
+    for k in range(count):
+        wl = workloads.ensemble_member(k, **member_kw)
+        models.append(build_model(wl, np.float32))
+        cells += wl.cell_bins
+        bins.append(wl.N0.shape[2] * wl.N0.shape[3])
+        del wl
+    dt = 0.02
+
+    def run(label):
+        for m in models:
+            ra.time_step_(m, dt, 1)
+        for m in models:
+            m.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            for m in models:
+                ra.time_step_(m, dt, 1)
+        for m in models:
+            m.synchronize()
+        t_seq = (time.perf_counter() - t0) / steps
+        ens = ra.Ensemble(models)
+        tuned = ens.autotune(dt, reps=2)
+        ens.use_graph(True)
+        ens.step(dt, 1)
+        ens.synchronize()
+        t0 = time.perf_counter()
+        ens.step(dt, steps)
+        ens.synchronize()
+        t_ens = (time.perf_counter() - t0) / steps
+        launch = ens.launch()
+        ens.close()
+        return {"sources": label,
+                "sequential": {"value": cells / t_seq, "ms_per_step": t_seq * 1e3},
+                "ensemble": {"value": cells / t_ens, "ms_per_step": t_ens * 1e3, "graph": True,
+                             "autotuned": tuned, "launch": launch},
+                "speedup_ensemble_over_sequential": t_seq / t_ens}
+
+    out = {"members": count, "cell_bins": cells, "dtype": "f32", "steps": steps, "plain": run("plain bundle")}
+    if member_kw:
+        out["member_ranges"] = {k: list(v) for k, v in member_kw.items()}
+    if not with_surrogate:
+        return out
+    nets = []
+    for k, (m, nb) in enumerate(zip(models, bins)):
+        net = sg.SourceNetwork(nb + 3, nb, hidden=512, hidden_layers=3, batch_capacity=64, seed=k)
+        sg.attach_surrogate(m, net, sg.SurrogateConfig(rate_unit=1e-7))
+        nets.append(net)
+    out["surrogate"] = run("network, 512 x 3 per member")
+    for m in models:
+        sg.attach_surrogate(m, None, None)
+    for n in nets:
+        n.close()
+    return out
+
+
 def err_vs_fp64():
     """FP32 against FP64 after the 400-step vortex-refraction run (config 1)."""
     wl = workloads.vortex_refraction()
@@ -176,6 +242,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-extra", action="store_true", help="skip the FP64 and larger-grid lines")
     ap.add_argument("--no-storm", action="store_true", help="skip the plain vs surrogate hurricane line")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble line")
+    ap.add_argument("--ensemble-members", type=int, default=64)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy (reference for HBM byte counters) and exit")
     args = ap.parse_args()
@@ -241,6 +309,10 @@ def main():
         extra["err_vs_fp64"] = err_vs_fp64()
     if not args.no_storm:
         extra["storm_plain_vs_surrogate"] = storm_plain_vs_surrogate(max(2, args.steps // 4), 1)
+    if not args.no_ensemble:
+        extra["ensemble"] = ensemble_line(args.ensemble_members, max(2, args.steps // 5))
+        extra["ensemble_small_members"] = ensemble_line(256, 10, with_surrogate=False, size_range=(16, 64),
+                                                        kappa_range=(8, 16), phi_range=(12, 24))
     record["extra"] = extra
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)

@@ -254,13 +254,15 @@ int rpl_ensemble_create(rpl_ensemble **out, rpl_ctx *const *members, int32_t cou
 int rpl_ensemble_destroy(rpl_ensemble *ens);
 int rpl_ensemble_step(rpl_ensemble *ens, double dt, int32_t nsteps);
 int rpl_ensemble_synchronize(rpl_ensemble *ens);
-/* Launch shape: tile_rows 2, 4 or 8 (0 keeps the current one) and the
- * persistent grid size in workgroups (a multiple of 8; 0: 8 per CU). */
+/* Launch shape: tile_rows 2, 4 or 8 (0 keeps the current one) and the grid
+ * size in workgroups (a multiple of 8: a persistent grid sweeping the work
+ * list; 0: one workgroup per tile, the default). */
 int rpl_ensemble_set_launch(rpl_ensemble *ens, int32_t tile_rows, int32_t blocks);
 int rpl_ensemble_get_launch(rpl_ensemble *ens, int32_t *tile_rows, int32_t *blocks, int64_t *tiles);
 /* Times the stage-1 launch (which leaves the members' state unchanged) for
  * tile_rows in {2, 4, 8} and grids of {2, 4, 8, 16} workgroups per CU or one
- * workgroup per tile, reps launches each; keeps and returns the fastest. */
+ * workgroup per tile, reps launches each; keeps and returns the fastest
+ * (blocks 0: one workgroup per tile). */
 int rpl_ensemble_autotune(rpl_ensemble *ens, double dt, int32_t reps, int32_t *tile_rows, int32_t *blocks,
                           double *best_ms);
 /* Capture the three stages of a step (source preparation + persistent

@@ -499,6 +499,9 @@ struct rpl_ensemble {
     int ty = 4, blocks = 0, cus = 0;
     rpl::EnsembleMember *desc = nullptr;  // 3 stages x members, device
     long long total = 0;
+    rpl::EnsembleState *sdesc = nullptr;  // 3 stages x members with a column state, device
+    int smembers = 0;
+    long long sblocks = 0;
     double desc_dt = -1;
     int desc_ty = 0;
     bool use_graph = false;
@@ -525,6 +528,19 @@ void stage_buffers(rpl_ctx *c, int stage, const void **Nk, const void **N0, void
     }
 }
 
+// Default grid: one workgroup per tile (0 = resolved against the tile count
+// of the current launch shape when launching).
+int default_blocks(const rpl_ensemble *) { return 0; }
+
+int launch_blocks(const rpl_ensemble *e)
+{
+    if (e->blocks > 0)
+        return e->blocks;
+    const long long g = (e->total + 7) / 8 * 8;
+    return (int)std::max<long long>(8, std::min<long long>(g, 0x7ffffff8LL));
+}
+
+
 int ensemble_descriptors(rpl_ensemble *e, double dt)
 {
     const int M = (int)e->members.size();
@@ -550,6 +566,41 @@ int ensemble_descriptors(rpl_ensemble *e, double dt)
     }
     e->total = t0;
     RPL_HIP(hipMemcpy(e->desc, h.data(), h.size() * sizeof(rpl::EnsembleMember), hipMemcpyHostToDevice));
+    // Column states (stress cap, mean-spectrum moments) of the members that need one.
+    std::vector<rpl_ctx *> sm;
+    for (rpl_ctx *c : e->members)
+        if (c->has_sources && ((c->src.wind_input && c->src.stress_cap) ||
+                               c->src.dissipation == RPL_DISSIPATION_MEAN_SPECTRUM))
+            sm.push_back(c);
+    const int S = (int)sm.size();
+    std::vector<rpl::EnsembleState> hs(3 * (size_t)S);
+    long long b0 = 0;
+    for (int k = 0; k < S; ++k) {
+        rpl_ctx *c = sm[k];
+        for (int st = 0; st < 3; ++st) {
+            rpl::EnsembleState &d = hs[(size_t)st * S + k];
+            d.nx = c->cfg.nx;
+            d.ny = c->cfg.ny;
+            d.nk = c->cfg.nkappa;
+            d.nphi = c->cfg.nphi;
+            const void *Nk, *N0;
+            void *out;
+            stage_buffers(c, st + 1, &Nk, &N0, &out);
+            d.N = (const float *)Nk;
+            d.s = stage_params<float>(c, 0.0).src;
+            d.tb = state_tables<float>(c);
+            d.factor = (float *)c->factor;
+            d.m0 = (float *)c->cm0;
+            d.kbar = (float *)c->kbar;
+            d.sbar = (float *)c->sbar;
+            d.blk0 = b0;
+        }
+        b0 += (long long)((c->plane + 63) / 64);
+    }
+    e->smembers = S;
+    e->sblocks = b0;
+    if (S)
+        RPL_HIP(hipMemcpy(e->sdesc, hs.data(), hs.size() * sizeof(rpl::EnsembleState), hipMemcpyHostToDevice));
     e->desc_dt = dt;
     e->desc_ty = e->ty;
     return RPL_OK;
@@ -561,17 +612,38 @@ int ensemble_stages(rpl_ensemble *e)
 {
     const int M = (int)e->members.size();
     for (int st = rpl::STAGE_1; st <= rpl::STAGE_3; ++st) {
-        if (e->src)
+        if (e->src) {
+            // Network inference of surrogate members, then every member's
+            // column state in one launch, then the surrogate audits.
             for (rpl_ctx *c : e->members) {
+                if (!c->s_active)
+                    continue;
                 const void *Nk, *N0;
                 void *out;
                 stage_buffers(c, st, &Nk, &N0, &out);
-                int rc = launch_sources(c, Nk);
+                int rc = surrogate_rates<float>(c, (const float *)Nk);
                 if (rc)
                     return rc;
             }
+            if (e->smembers) {
+                hipError_t err = rpl::launch_ensemble_state(e->sdesc + (size_t)(st - 1) * e->smembers, e->smembers,
+                                                            e->sblocks, e->stream);
+                if (err != hipSuccess)
+                    return fail(RPL_ERR_HIP, std::string("ensemble state launch: ") + hipGetErrorString(err));
+            }
+            for (rpl_ctx *c : e->members) {
+                if (!c->s_active || c->scfg.audit_stride <= 0)
+                    continue;
+                const void *Nk, *N0;
+                void *out;
+                stage_buffers(c, st, &Nk, &N0, &out);
+                int rc = surrogate_audit<float>(c, (const float *)Nk);
+                if (rc)
+                    return rc;
+            }
+        }
         hipError_t err = rpl::launch_ensemble_stage(e->desc + (size_t)(st - 1) * M, M, e->total, e->transport, e->src,
-                                                    st, e->ty, e->blocks, e->stream);
+                                                    st, e->ty, launch_blocks(e), e->stream);
         if (err != hipSuccess)
             return fail(RPL_ERR_HIP, std::string("ensemble stage launch: ") + hipGetErrorString(err));
     }
@@ -585,8 +657,6 @@ void drop_graph(rpl_ensemble *e)
     e->exec = nullptr;
     e->graph_dt = -1;
 }
-
-int default_blocks(const rpl_ensemble *e) { return std::max(8, e->cus * 8); }
 
 }  // namespace
 
@@ -1432,7 +1502,10 @@ int rpl_ensemble_create(rpl_ensemble **out, rpl_ctx *const *members, int32_t cou
     }
     e->cus = prop.multiProcessorCount;
     e->blocks = default_blocks(e);
-    if (hipMalloc((void **)&e->desc, 3 * (size_t)count * sizeof(rpl::EnsembleMember)) != hipSuccess) {
+    if (hipMalloc((void **)&e->desc, 3 * (size_t)count * sizeof(rpl::EnsembleMember)) != hipSuccess ||
+        hipMalloc((void **)&e->sdesc, 3 * (size_t)count * sizeof(rpl::EnsembleState)) != hipSuccess) {
+        if (e->desc)
+            (void)hipFree(e->desc);
         (void)hipStreamDestroy(e->stream);
         delete e;
         return fail(RPL_ERR_NOMEM, "ensemble descriptors");
@@ -1461,6 +1534,8 @@ int rpl_ensemble_destroy(rpl_ensemble *e)
         e->members[k]->stream = e->own_stream[k];
     if (e->desc)
         (void)hipFree(e->desc);
+    if (e->sdesc)
+        (void)hipFree(e->sdesc);
     (void)hipStreamDestroy(e->stream);
     delete e;
     return RPL_OK;
@@ -1487,7 +1562,7 @@ int rpl_ensemble_get_launch(rpl_ensemble *e, int32_t *tile_rows, int32_t *blocks
     if (tile_rows)
         *tile_rows = e->ty;
     if (blocks)
-        *blocks = e->blocks;
+        *blocks = e->blocks;  // 0: one workgroup per tile
     if (tiles) {
         long long t = 0;
         for (rpl_ctx *c : e->members)
@@ -1625,7 +1700,7 @@ int rpl_ensemble_autotune(rpl_ensemble *e, double dt, int32_t reps, int32_t *til
             if (ms / reps < best) {
                 best = ms / reps;
                 best_ty = ty;
-                best_blocks = (int)g;
+                best_blocks = mu ? (int)g : 0;
             }
         }
         if (err != hipSuccess)

@@ -59,6 +59,18 @@ struct StateTables {
     T rho_air, rho_w_g, cap_eps, inv_p;
 };
 
+// One member's column-state launch in an ensemble (FP32): inputs, bundle
+// parameters, outputs, and its first 64-column block in the work list.
+struct EnsembleState {
+    int nx, ny, nk, nphi;
+    const float *N;
+    SourceParams<float> s;
+    StateTables<float> tb;
+    float *factor, *m0, *kbar, *sbar;
+    long long blk0;
+};
+
+
 template <typename T>
 hipError_t launch_wind_columns(int nx, int ny, const double *xn, const double *yn, double t, const WindParams &w,
                                T *U10, T *theta_u, T *ustar, hipStream_t s);
@@ -81,6 +93,8 @@ hipError_t launch_stage_reference(const StageParams<double> &p, int transport, i
                                   const double *Nk, const double *N0, double *out, hipStream_t s);
 hipError_t launch_stage_reference(const StageParams<float> &p, int transport, int stage,
                                   const float *Nk, const float *N0, float *out, hipStream_t s);
+// Column states of every member in one launch (`blocks` 64-column blocks in all).
+hipError_t launch_ensemble_state(const EnsembleState *mem, int members, long long blocks, hipStream_t s);
 // One persistent launch of stage `stage` (STAGE_1..3) over every member's
 // tiles; `blocks` (a multiple of 8) workgroups of 64 x ty threads.
 hipError_t launch_ensemble_stage(const EnsembleMember *mem, int members, long long total, int transport, bool src,

@@ -140,6 +140,130 @@ __global__ __launch_bounds__(256) void source_state_kernel(int nx, int ny, int n
     }
 }
 
+// FP32 form of the same state: one 256-thread workgroup per 64 consecutive
+// columns, four bin groups per column (group g takes every fourth bin of the
+// n-outer / m-inner order), partial sums combined in LDS in group order.
+// Four times the parallelism of thread-per-column at the same coalesced
+// reads, which matters for small planes.  `active` lets an ensemble launch
+// skip members without a column state.
+__device__ __forceinline__ void state_block_f32(int nx, int ny, int nk, int nphi, const float *__restrict__ N,
+                                                const SourceParams<float> &s, const StateTables<float> &tb,
+                                                float *factor, float *m0o, float *kbaro, float *sbaro, size_t col0)
+{
+    constexpr int G = 4;
+    __shared__ float red[5][G][64];
+    const size_t plane = (size_t)nx * ny;
+    const int tx = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const size_t ij = col0 + tx;
+    bool active = ij < plane;
+    if (active && s.surrogate && !(gld<true>(s.use_plain, ij) & 3))
+        active = false;  // the network supplies this column's source
+    const bool cap = s.wind_input && s.stress_cap, ms = s.dissipation == 1;
+    float twx = 0.0f, twy = 0.0f, m0 = 0.0f, kp = 0.0f, sp = 0.0f;
+    float U10 = 0.0f, ustar = 0.0f, theta_u = 0.0f;
+    if (active && cap) {
+        U10 = gld<true>(s.U10, ij);
+        ustar = gld<true>(s.ustar, ij);
+        theta_u = gld<true>(s.theta_u, ij);
+    }
+    const bool wind = cap && U10 > 0.0f && ustar > 0.0f;
+    if (active && (wind || ms)) {
+        int m = g, n = 0;
+        while (m >= nk) {
+            m -= nk;
+            ++n;
+        }
+        const int nbins = nk * nphi;
+        for (int t = g; t < nbins; t += G) {
+            const float a = N[ij + plane * (size_t)t];
+            const float k = gld<true>(tb.kappa, m);
+            if (a > 0.0f && k > 0.0f) {
+                if (wind) {
+                    const float sigma = gld<true>(tb.sigma_k, m), C = sigma / k;
+                    const float rate = jp_rate<float>(s, k, sigma, C, gld<true>(tb.phi, n), theta_u, ustar);
+                    if (rate > 0.0f) {
+                        const float mflux = rate * a / C * gld<true>(tb.w, t) * tb.rho_w_g;
+                        twx += mflux * gld<true>(tb.cos_phi, n);
+                        twy += mflux * gld<true>(tb.sin_phi, n);
+                    }
+                }
+                if (ms) {
+                    const float wt = gld<true>(tb.w, t);
+                    m0 += a * wt;
+                    kp += a * wt * gld<true>(tb.k_pow, m);
+                    sp += a * wt * gld<true>(tb.sigma_pow, m);
+                }
+            }
+            m += G;
+            while (m >= nk) {
+                m -= nk;
+                ++n;
+            }
+        }
+    }
+    red[0][g][tx] = twx;
+    red[1][g][tx] = twy;
+    red[2][g][tx] = m0;
+    red[3][g][tx] = kp;
+    red[4][g][tx] = sp;
+    __syncthreads();
+    if (g != 0 || !active)
+        return;
+    twx = red[0][0][tx];
+    twy = red[1][0][tx];
+    m0 = red[2][0][tx];
+    kp = red[3][0][tx];
+    sp = red[4][0][tx];
+    for (int h = 1; h < G; ++h) {
+        twx += red[0][h][tx];
+        twy += red[1][h][tx];
+        m0 += red[2][h][tx];
+        kp += red[3][h][tx];
+        sp += red[4][h][tx];
+    }
+    if (cap) {
+        float f = 1.0f;
+        if (wind) {
+            const float tau_max = tb.rho_air * (ustar * ustar);
+            const float par = twx * cosf(theta_u) + twy * sinf(theta_u);
+            const float eff = par > 0.0f ? par : 0.0f;
+            f = (eff > tb.cap_eps * tau_max && tau_max > 0.0f) ? tb.cap_eps * tau_max / eff : 1.0f;
+        }
+        gp_store(factor, ij, f);
+    }
+    if (ms) {
+        gp_store(m0o, ij, m0);
+        gp_store(kbaro, ij, m0 > 0.0f ? powf(kp / m0, tb.inv_p) : 0.0f);
+        gp_store(sbaro, ij, m0 > 0.0f ? powf(sp / m0, tb.inv_p) : 0.0f);
+    }
+}
+
+__global__ __launch_bounds__(256) void source_state_f32_kernel(int nx, int ny, int nk, int nphi,
+                                                               const float *__restrict__ N, SourceParams<float> s,
+                                                               StateTables<float> tb, float *factor, float *m0o,
+                                                               float *kbaro, float *sbaro)
+{
+    state_block_f32(nx, ny, nk, nphi, N, s, tb, factor, m0o, kbaro, sbaro, (size_t)blockIdx.x * 64);
+}
+
+// All members' column states of one RK3 stage in one launch: the members'
+// 64-column blocks are numbered member after member (EnsembleState::blk0).
+__global__ __launch_bounds__(256) void ensemble_state_kernel(const EnsembleState *__restrict__ mem, int members)
+{
+    const long long b = blockIdx.x;
+    int u = 0, v = members - 1;
+    while (u < v) {
+        const int c = (u + v + 1) >> 1;
+        if (mem[c].blk0 <= b)
+            u = c;
+        else
+            v = c - 1;
+    }
+    const EnsembleState &e = mem[__builtin_amdgcn_readfirstlane(u)];
+    const size_t col0 = (size_t)__builtin_amdgcn_readfirstlane((int)(b - e.blk0)) * 64;
+    state_block_f32(e.nx, e.ny, e.nk, e.nphi, e.N, e.s, e.tb, e.factor, e.m0, e.kbar, e.sbar, col0);
+}
+
 }  // namespace
 
 template <typename T>
@@ -195,6 +319,13 @@ hipError_t launch_source_state(int nx, int ny, int nk, int nphi, const T *N, con
                                const StateTables<T> &tb, T *factor, T *m0, T *kbar, T *sbar, hipStream_t s)
 {
     const size_t plane = (size_t)nx * ny;
+    if (sizeof(T) == 4) {
+        hipLaunchKernelGGL(source_state_f32_kernel, dim3((unsigned)((plane + 63) / 64)), dim3(256), 0, s, nx, ny, nk,
+                           nphi, (const float *)N, reinterpret_cast<const SourceParams<float> &>(sp),
+                           reinterpret_cast<const StateTables<float> &>(tb), (float *)factor, (float *)m0,
+                           (float *)kbar, (float *)sbar);
+        return hipGetLastError();
+    }
     hipLaunchKernelGGL(source_state_kernel<T>, dim3((unsigned)((plane + 255) / 256)), dim3(256), 0, s, nx, ny, nk,
                        nphi, N, sp, tb, factor, m0, kbar, sbar);
     return hipGetLastError();
@@ -204,6 +335,14 @@ template hipError_t launch_wind_columns<float>(int, int, const double *, const d
                                               float *, float *, float *, hipStream_t);
 template hipError_t launch_wind_columns<double>(int, int, const double *, const double *, double, const WindParams &,
                                                double *, double *, double *, hipStream_t);
+hipError_t launch_ensemble_state(const EnsembleState *mem, int members, long long blocks, hipStream_t s)
+{
+    if (members < 1 || blocks < 1 || blocks > 0x7fffffffLL)
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL(ensemble_state_kernel, dim3((unsigned)blocks), dim3(256), 0, s, mem, members);
+    return hipGetLastError();
+}
+
 template hipError_t launch_source_state<float>(int, int, int, int, const float *, const SourceParams<float> &,
                                                const StateTables<float> &, float *, float *, float *, float *,
                                                hipStream_t);

@@ -43,6 +43,26 @@ struct SourceParams {
     const unsigned char *use_plain;
 };
 
+// Loads of a pointer parameter.  With AS1 the load goes through an
+// address-space-1 (global) pointer: pointers read from memory (the ensemble's
+// member descriptors) are generic to the compiler, which would otherwise emit
+// flat loads with 64-bit address arithmetic.  Pointers that arrive as kernel
+// arguments are loaded as they are (AS1 = false); the explicit cast costs the
+// per-model kernels a few percent.
+template <bool AS1, typename T>
+__device__ __forceinline__ T gld(const T *p, size_t i)
+{
+    if (AS1)
+        return ((const __attribute__((address_space(1))) T *)p)[i];
+    return p[i];
+}
+
+template <typename T>
+__device__ __forceinline__ void gp_store(T *p, size_t i, T v)
+{
+    ((__attribute__((address_space(1))) T *)p)[i] = v;
+}
+
 __device__ __forceinline__ float dcos(float x) { return cosf(x); }
 __device__ __forceinline__ double dcos(double x) { return cos(x); }
 __device__ __forceinline__ float dsin(float x) { return sinf(x); }
@@ -78,20 +98,20 @@ __device__ __forceinline__ T jp_rate(const SourceParams<T> &s, T k, T sigma, T C
 
 // Source tendency of one cell: positive - damping * a.  col is the column
 // index i + nx*j, t the bin index m + nk*n.
-template <typename T>
+template <typename T, bool AS1 = false>
 __device__ __forceinline__ T source_term(const SourceParams<T> &s, unsigned col, int t, T a)
 {
 #pragma clang fp contract(off)
-    if (s.surrogate && !(s.use_plain[col] & 5))
-        return (T)(s.rate[col + s.rate_ld * (size_t)t] * s.out_scale[t]) * a;
+    if (s.surrogate && !(gld<AS1>(s.use_plain, col) & 5))
+        return (T)(gld<AS1>(s.rate, col + s.rate_ld * (size_t)t) * gld<AS1>(s.out_scale, t)) * a;
     T positive = T(0), damping = T(0);
-    const T k = s.k_cell[t];
+    const T k = gld<AS1>(s.k_cell, t);
     if (s.wind_input) {
-        const T U10 = s.U10[col], ustar = s.ustar[col];
+        const T U10 = gld<AS1>(s.U10, col), ustar = gld<AS1>(s.ustar, col);
         if (U10 > T(0) && ustar > T(0) && k > T(0)) {
-            const T raw = jp_rate<T>(s, k, s.sigma_cell[t], s.C_cell[t], s.theta_cell[t], s.theta_u[col], ustar);
+            const T raw = jp_rate<T>(s, k, gld<AS1>(s.sigma_cell, t), gld<AS1>(s.C_cell, t), gld<AS1>(s.theta_cell, t), gld<AS1>(s.theta_u, col), ustar);
             if (raw > T(0)) {
-                const T rate = s.stress_cap ? raw * s.factor[col] : raw;
+                const T rate = s.stress_cap ? raw * gld<AS1>(s.factor, col) : raw;
                 if (rate >= T(0))
                     positive += rate * a;
                 else
@@ -100,12 +120,12 @@ __device__ __forceinline__ T source_term(const SourceParams<T> &s, unsigned col,
         }
     }
     if (s.dissipation == 1) {
-        const T m0 = s.m0[col], kb = s.kbar[col];
+        const T m0 = gld<AS1>(s.m0, col), kb = gld<AS1>(s.kbar, col);
         if (m0 > T(0) && kb > T(0) && k > T(0)) {
             const T ab = m0 * (kb * kb);
             const T ratio = k / kb;
             const T bracket = s.d1 * ratio + s.d2 * (ratio * ratio);
-            const T rate = -s.cds * (ab * ab) * s.sbar[col] * bracket;
+            const T rate = -s.cds * (ab * ab) * gld<AS1>(s.sbar, col) * bracket;
             if (rate >= T(0))
                 damping += rate;
             else
@@ -113,7 +133,7 @@ __device__ __forceinline__ T source_term(const SourceParams<T> &s, unsigned col,
         }
     } else if (s.dissipation == 2) {
         if (a > T(0) && k > T(0)) {
-            const T sigma = s.sigma_cell[t];
+            const T sigma = gld<AS1>(s.sigma_cell, t);
             const T B = a * sigma * (k * k * k);
             T over = B / s.ls_br - T(1);
             over = over > T(0) ? over : T(0);

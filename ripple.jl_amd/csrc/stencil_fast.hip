@@ -109,9 +109,29 @@ __device__ __forceinline__ float upwind_difference(float fm3, float fm2, float f
 // scalar base plus a 32-bit lane offset (global_load ... saddr form) and costs
 // no 64-bit vector arithmetic.  Offsets stay below 4 GiB: a slab is nx*ny
 // elements and the host rejects nx*ny >= 2^30.
+// AS1: the base came from memory (ensemble descriptors) and is marked global
+// so that the load stays a global (not flat) load.
+template <bool AS1 = false>
 __device__ __forceinline__ float ldo(const float *base, unsigned byte_off)
 {
+    if (AS1) {
+        typedef const __attribute__((address_space(1))) char gchar;
+        typedef const __attribute__((address_space(1))) float gfloat;
+        return *(gfloat *)((gchar *)base + byte_off);
+    }
     return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(base) + byte_off);
+}
+
+template <bool AS1 = false>
+__device__ __forceinline__ void sto(float *base, unsigned byte_off, float v)
+{
+    if (AS1) {
+        typedef __attribute__((address_space(1))) char gchar;
+        typedef __attribute__((address_space(1))) float gfloat;
+        *(gfloat *)((gchar *)base + byte_off) = v;
+        return;
+    }
+    *reinterpret_cast<float *>(reinterpret_cast<char *>(base) + byte_off) = v;
 }
 
 __device__ __forceinline__ int wrap(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
@@ -149,10 +169,13 @@ __device__ __forceinline__ Tile tile_of(int b, int total, int nbx, int nby, int 
 // stage update for each cell.  Shared by the per-model kernel below and the
 // ensemble work-list kernel; p may live in kernel arguments or in global
 // memory (block-uniform either way).
-template <int TRANSPORT, int STAGE, bool SRC, int TY>
-__device__ __forceinline__ void stage_tile(const float *__restrict__ Nk, const float *__restrict__ N0, float *out,
+template <int TRANSPORT, int STAGE, bool SRC, int TY, bool AS1>
+__device__ __forceinline__ void stage_tile(const float *__restrict__ Nk_, const float *__restrict__ N0_, float *out_,
                                            const StageParams<float> &p, int ib, int jb, int m, int n)
 {
+    const float *__restrict__ Nk = Nk_;
+    const float *__restrict__ N0 = N0_;
+    float *out = out_;
     const int nx = p.nx, ny = p.ny, nk = p.nk, nphi = p.nphi;
     const int i0 = ib * 64, j0 = jb * TY;
     const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
@@ -165,14 +188,14 @@ __device__ __forceinline__ void stage_tile(const float *__restrict__ Nk, const f
     const unsigned nxu = (unsigned)nx;
     const unsigned off = (unsigned)i + nxu * (unsigned)j;
     const unsigned bo = off << 2;
-    const float x0 = ldo(slab, bo);
+    const float x0 = ldo<AS1>(slab, bo);
     const bool x_inner = i0 >= 3 && i0 + 64 + 3 <= nx;  // block-uniform
     const bool y_inner = j0 >= 3 && j0 + TY + 3 <= ny;
     float L = 0.0f;
     if (TRANSPORT != RPL_TRANSPORT_NONE) {
         const unsigned rowb = (nxu * (unsigned)j) << 2, colb = (unsigned)i << 2;
         const unsigned sx = 4u, sy = nxu << 2;
-        float ux = p.cgx[m + nk * n], uy = p.cgy[m + nk * n];
+        float ux = gld<AS1>(p.cgx, m + nk * n), uy = gld<AS1>(p.cgy, m + nk * n);
         float ck = 0.0f, cp = 0.0f;
         if (TRANSPORT == RPL_TRANSPORT_REFRACTION) {
             const float *Ux = p.Ux + plane * m, *Uy = p.Uy + plane * m;
@@ -181,35 +204,35 @@ __device__ __forceinline__ void stage_tile(const float *__restrict__ Nk, const f
             const unsigned yp = y_inner ? bo + sy : colb + sy * (unsigned)wrap(j + 1, ny);
             const unsigned ym = y_inner ? bo - sy : colb + sy * (unsigned)wrap(j - 1, ny);
             const float hx = 0.5f * p.dx_inv, hy = 0.5f * p.dy_inv;
-            const float Uxx = (ldo(Ux, xp) - ldo(Ux, xm)) * hx;
-            const float Uxy = (ldo(Ux, yp) - ldo(Ux, ym)) * hy;
-            const float Uyx = (ldo(Uy, xp) - ldo(Uy, xm)) * hx;
-            const float Uyy = (ldo(Uy, yp) - ldo(Uy, ym)) * hy;
-            const float kap = p.kappa[m], cs = p.cosp[n], sn = p.sinp[n];
+            const float Uxx = (ldo<AS1>(Ux, xp) - ldo<AS1>(Ux, xm)) * hx;
+            const float Uxy = (ldo<AS1>(Ux, yp) - ldo<AS1>(Ux, ym)) * hy;
+            const float Uyx = (ldo<AS1>(Uy, xp) - ldo<AS1>(Uy, xm)) * hx;
+            const float Uyy = (ldo<AS1>(Uy, yp) - ldo<AS1>(Uy, ym)) * hy;
+            const float kap = gld<AS1>(p.kappa, m), cs = gld<AS1>(p.cosp, n), sn = gld<AS1>(p.sinp, n);
             const float cc = cs * cs, ss = sn * sn, csn = cs * sn;
             ck = -kap * (cc * Uxx + csn * (Uyx + Uxy) + ss * Uyy);
             cp = csn * (Uxx - Uyy) + ss * Uyx - cc * Uxy;
-            ux += ldo(Ux, bo);
-            uy += ldo(Uy, bo);
+            ux += ldo<AS1>(Ux, bo);
+            uy += ldo<AS1>(Uy, bo);
         }
         float fx[7], fy[7];
         if (x_inner) {
 #pragma unroll
             for (int d = 0; d < 7; ++d)
-                fx[d] = ldo(slab, bo + (unsigned)(4 * (d - 3)));
+                fx[d] = ldo<AS1>(slab, bo + (unsigned)(4 * (d - 3)));
         } else {
 #pragma unroll
             for (int d = 0; d < 7; ++d)
-                fx[d] = ldo(slab, rowb + ((unsigned)wrap(i + d - 3, nx) << 2));
+                fx[d] = ldo<AS1>(slab, rowb + ((unsigned)wrap(i + d - 3, nx) << 2));
         }
         if (y_inner) {
 #pragma unroll
             for (int d = 0; d < 7; ++d)
-                fy[d] = ldo(slab, bo + sy * (unsigned)(d - 3));
+                fy[d] = ldo<AS1>(slab, bo + sy * (unsigned)(d - 3));
         } else {
 #pragma unroll
             for (int d = 0; d < 7; ++d)
-                fy[d] = ldo(slab, colb + sy * (unsigned)wrap(j + d - 3, ny));
+                fy[d] = ldo<AS1>(slab, colb + sy * (unsigned)wrap(j + d - 3, ny));
         }
         const float dx = upwind_difference(fx[0], fx[1], fx[2], x0, fx[4], fx[5], fx[6], ux);
         const float dy = upwind_difference(fy[0], fy[1], fy[2], x0, fy[4], fy[5], fy[6], uy);
@@ -222,8 +245,8 @@ __device__ __forceinline__ void stage_tile(const float *__restrict__ Nk, const f
                     continue;
                 const size_t pbin = (size_t)m + (size_t)nk * wrap(n + d - 3, nphi);
                 const size_t kbin = (size_t)clampi(m + d - 3, nk) + (size_t)nk * n;
-                fp[d] = ldo(Nk + plane * pbin, bo);
-                fk[d] = ldo(Nk + plane * kbin, bo);
+                fp[d] = ldo<AS1>(Nk + plane * pbin, bo);
+                fk[d] = ldo<AS1>(Nk + plane * kbin, bo);
             }
             const float dp = upwind_difference(fp[0], fp[1], fp[2], x0, fp[4], fp[5], fp[6], cp);
             // kappa: bounded; m is uniform across the block, so the edge path
@@ -256,9 +279,9 @@ __device__ __forceinline__ void stage_tile(const float *__restrict__ Nk, const f
         }
     }
     if (SRC)  // tendencies.jl:88-94, G += source_tendency
-        L += source_term<float>(p.src, off, m + nk * n, x0);
+        L += source_term<float, AS1>(p.src, off, m + nk * n, x0);
     if (p.src_add)
-        L += ldo(p.src_add + plane * bin, bo);
+        L += ldo<AS1>(p.src_add + plane * bin, bo);
     float *dst = out + plane * bin;
     float r;
     if (STAGE == STAGE_TENDENCY) {
@@ -269,9 +292,9 @@ __device__ __forceinline__ void stage_tile(const float *__restrict__ Nk, const f
         const float a = STAGE == STAGE_2 ? 0.75f : (float)(1.0 / 3.0);
         const float b = STAGE == STAGE_2 ? 0.25f : (float)(2.0 / 3.0);
         const float st = clip0(fmaf(p.dt, L, x0));
-        r = clip0(fmaf(a, ldo(N0 + plane * bin, bo), b * st));
+        r = clip0(fmaf(a, ldo<AS1>(N0 + plane * bin, bo), b * st));
     }
-    *reinterpret_cast<float *>(reinterpret_cast<char *>(dst) + bo) = r;
+    sto<AS1>(dst, bo, r);
 }
 
 template <int TRANSPORT, int STAGE, bool SRC>
@@ -279,39 +302,53 @@ __global__ __launch_bounds__(256) void stage_fast_kernel(const float *__restrict
                                                          float *out, StageParams<float> p, int nbx, int nby)
 {
     const Tile t = tile_of(blockIdx.x, nbx * nby * p.nk * p.nphi, nbx, nby, p.nk, p.nphi);
-    stage_tile<TRANSPORT, STAGE, SRC, 4>(Nk, N0, out, p, t.ib, t.jb, t.m, t.n);
+    stage_tile<TRANSPORT, STAGE, SRC, 4, false>(Nk, N0, out, p, t.ib, t.jb, t.m, t.n);
 }
 
 // Ensemble work list: every member's tiles (64 x TY cells of one bin) are
 // numbered consecutively, member after member (EnsembleMember::tile0 is the
-// prefix sum).  The grid is persistent: workgroups are dealt round-robin to
+// prefix sum).  The grid is persistent.  Workgroups are dealt round-robin to
 // the eight XCDs, so XCD x takes the contiguous range [x T / 8, (x+1) T / 8)
-// of the T tiles and its workgroups stride through it; within a member the
-// order is phi fastest, then kappa, tile row, tile column, as in tile_of.
+// of the T tiles and its workgroups sweep it together, each taking every
+// (workgroups per XCD)-th tile.  Within a member the order is phi fastest,
+// then kappa, tile row, tile column (as in tile_of), so the workgroups of one
+// XCD work on neighbouring bins of the same (x, y) tile at the same time and
+// share the stencil's phi and kappa neighbours in that XCD's L2.  The member
+// is looked up (binary search over tile0) only when a workgroup's next tile
+// lies past the member it last worked on.
 template <int TRANSPORT, int STAGE, bool SRC, int TY>
 __global__ __launch_bounds__(64 * TY) void ensemble_stage_kernel(const EnsembleMember *__restrict__ mem, int members,
                                                                  long long total)
 {
     const int xcd = blockIdx.x & 7;
-    const long long per_block = (long long)(gridDim.x >> 3);
+    const long long stride = (long long)(gridDim.x >> 3);
     const long long lo = total * xcd / 8, hi = total * (xcd + 1) / 8;
-    for (long long L = lo + (blockIdx.x >> 3); L < hi; L += per_block) {
-        int a = 0, b = members - 1;  // last member with tile0 <= L
-        while (a < b) {
-            const int c = (a + b + 1) >> 1;
-            if (mem[c].tile0 <= L)
-                a = c;
-            else
-                b = c - 1;
+    int a = 0;
+    long long a_begin = 0, a_end = 0;
+    for (long long L = lo + (blockIdx.x >> 3); L < hi; L += stride) {
+        if (L >= a_end) {
+            int u = a, v = members - 1;  // last member with tile0 <= L (members are in tile order)
+            while (u < v) {
+                const int c = (u + v + 1) >> 1;
+                if (mem[c].tile0 <= L)
+                    u = c;
+                else
+                    v = c - 1;
+            }
+            a = __builtin_amdgcn_readfirstlane(u);
+            a_begin = mem[a].tile0;
+            a_end = a + 1 < members ? mem[a + 1].tile0 : total;
         }
         const EnsembleMember &e = mem[a];
-        int q = (int)(L - e.tile0);
+        // The tile coordinates are uniform across the workgroup; saying so keeps
+        // the member's parameters and the slab bases in scalar registers.
+        int q = __builtin_amdgcn_readfirstlane((int)(L - a_begin));
         const int n = q % e.p.nphi;
         q /= e.p.nphi;
         const int m = q % e.p.nk;
         q /= e.p.nk;
         const int jb = q % e.nby, ib = q / e.nby;
-        stage_tile<TRANSPORT, STAGE, SRC, TY>(e.Nk, e.N0, e.out, e.p, ib, jb, m, n);
+        stage_tile<TRANSPORT, STAGE, SRC, TY, true>(e.Nk, e.N0, e.out, e.p, ib, jb, m, n);
     }
 }
 

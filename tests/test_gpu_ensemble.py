@@ -74,7 +74,8 @@ def test_autotune_keeps_results_and_state():
     ens = ra.Ensemble(ms)
     best = ens.autotune(0.02, reps=2)
     say(f"autotune {best}")
-    assert best["tile_rows"] in (2, 4, 8) and best["blocks"] % 8 == 0 and best["stage_ms"] > 0
+    assert best["tile_rows"] in (2, 4, 8) and best["blocks"] % 8 == 0 and best["blocks"] >= 0
+    assert best["stage_ms"] > 0
     for m, b in zip(ms, before):
         assert np.array_equal(m.action, b)  # tuning leaves the state alone
     ens.use_graph(True)
