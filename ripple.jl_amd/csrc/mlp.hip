@@ -67,6 +67,27 @@ __device__ __forceinline__ void stage_tile(bf16 (*S)[LDK], const bf16 *__restric
     }
 }
 
+template <int EPI>
+__device__ __forceinline__ void epilogue(const GemmArgs &g, int m, int n, float c)
+{
+    const size_t o = (size_t)m * g.scm + (size_t)n * g.scn;
+    if (EPI == EPI_STORE_F32) {
+        g.Cf[o] = c;
+    } else if (EPI == EPI_BIAS_GELU) {
+        const float z = c + g.bias[n];
+        g.Cf[o] = z;
+        g.Cb[o] = f2bf(gelu(z));
+    } else if (EPI == EPI_BIAS_GELU_INFER) {
+        g.Cb[o] = f2bf(gelu(c + g.bias[n]));
+    } else if (EPI == EPI_BIAS_F32) {
+        g.Cf[o] = c + g.bias[n];
+    } else if (EPI == EPI_BIAS_ROW_F32) {
+        g.Cf[o] = c + g.bias[m];
+    } else {  // EPI_GELU_BACKWARD: dZ = dA * gelu'(Z)
+        g.Cb[o] = f2bf(c * gelu_grad(g.Zin[o]));
+    }
+}
+
 template <bool A_KC, bool B_KC, int EPI>
 __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g)
 {
@@ -114,23 +135,122 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g)
             for (int r = 0; r < 4; ++r) {
                 const int m = m0 + wm + a * 16 + 4 * q + r;
                 const int n = n0 + wn + b * 16 + r16;
-                float c = acc[a][b][r];
-                const size_t o = (size_t)m * g.scm + (size_t)n * g.scn;
-                if (EPI == EPI_STORE_F32) {
-                    g.Cf[o] = c;
-                } else if (EPI == EPI_BIAS_GELU) {
-                    const float z = c + g.bias[n];
-                    g.Cf[o] = z;
-                    g.Cb[o] = f2bf(gelu(z));
-                } else if (EPI == EPI_BIAS_GELU_INFER) {
-                    g.Cb[o] = f2bf(gelu(c + g.bias[n]));
-                } else if (EPI == EPI_BIAS_F32) {
-                    g.Cf[o] = c + g.bias[n];
-                } else if (EPI == EPI_BIAS_ROW_F32) {
-                    g.Cf[o] = c + g.bias[m];
-                } else {  // EPI_GELU_BACKWARD: dZ = dA * gelu'(Z)
-                    g.Cb[o] = f2bf(c * gelu_grad(g.Zin[o]));
-                }
+                epilogue<EPI>(g, m, n, acc[a][b][r]);
+            }
+}
+
+// Large-tile GEMM for MFMA-bound shapes: block tile 128 x 128, K in steps of
+// 64, v_mfma_f32_16x16x32_bf16 (the gfx950 double-rate bf16 instruction).
+// Four wavefronts, each a 64 x 64 quadrant = 4 x 4 MFMA tiles.  Both tiles
+// are staged through LDS as [row][k] (k contiguous, rows padded to 144 bytes
+// so the 16-byte fragment reads of one wavefront spread over the banks) in
+// two buffers: the next K step is fetched into registers while the current
+// one is multiplied.  Operand layout (measured on gfx950): lane l supplies
+// A[l % 16][8 (l / 16) + e] and B[8 (l / 16) + e][l % 16], e = 0..7, and
+// accumulator register r holds D[4 (l / 16) + r][l % 16].  Needs M and N
+// multiples of 128 and K of 64.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+constexpr int WM = 128, WN = 128, WK = 64, WLD = WK + 8;
+
+template <bool K_CONTIG>
+__device__ __forceinline__ void fetch_tile(u32x4 (&r)[4], const bf16 *__restrict__ G, size_t s_row, size_t s_k,
+                                           int tid)
+{
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int idx = tid + 256 * q;
+        if (K_CONTIG) {
+            const int row = idx >> 3, k = (idx & 7) * 8;
+            r[q] = *reinterpret_cast<const u32x4 *>(G + (size_t)row * s_row + k);
+        } else {
+            const int k = idx >> 4, row = (idx & 15) * 8;
+            r[q] = *reinterpret_cast<const u32x4 *>(G + (size_t)k * s_k + row);
+        }
+    }
+}
+
+template <bool K_CONTIG>
+__device__ __forceinline__ void store_tile(bf16 (*S)[WLD], const u32x4 (&r)[4], int tid)
+{
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int idx = tid + 256 * q;
+        if (K_CONTIG) {
+            const int row = idx >> 3, k = (idx & 7) * 8;
+            *reinterpret_cast<u32x4 *>(&S[row][k]) = r[q];
+        } else {
+            const int k = idx >> 4, row = (idx & 15) * 8;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                S[row + 2 * u][k] = (bf16)(r[q][u] & 0xffffu);
+                S[row + 2 * u + 1][k] = (bf16)(r[q][u] >> 16);
+            }
+        }
+    }
+}
+
+template <bool A_KC, bool B_KC, int EPI>
+__global__ __launch_bounds__(256) void gemm_wide_kernel(GemmArgs g)
+{
+    __shared__ bf16 As[2][WM][WLD];
+    __shared__ bf16 Bs[2][WN][WLD];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int m0 = blockIdx.x * WM, n0 = blockIdx.y * WN;
+    const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
+    const bf16 *Ab = g.A + (size_t)m0 * g.sam;
+    const bf16 *Bb = g.B + (size_t)n0 * g.sbn;
+    f32x4 acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+            acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const int r16 = lane & 15, q8 = (lane >> 4) * 8;
+    u32x4 ra[4], rb[4];
+    fetch_tile<A_KC>(ra, Ab, g.sam, g.sak, tid);
+    fetch_tile<B_KC>(rb, Bb, g.sbn, g.sbk, tid);
+    store_tile<A_KC>(As[0], ra, tid);
+    store_tile<B_KC>(Bs[0], rb, tid);
+    __syncthreads();
+    const int nk = g.K / WK;
+    for (int kt = 0; kt < nk; ++kt) {
+        const int cur = kt & 1;
+        const bool more = kt + 1 < nk;
+        if (more) {
+            fetch_tile<A_KC>(ra, Ab + (size_t)(kt + 1) * WK * g.sak, g.sam, g.sak, tid);
+            fetch_tile<B_KC>(rb, Bb + (size_t)(kt + 1) * WK * g.sbk, g.sbn, g.sbk, tid);
+        }
+#pragma unroll
+        for (int ks = 0; ks < WK; ks += 32) {
+            bf16x8 af[4], bfr[4];
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+                af[a] = *reinterpret_cast<const bf16x8 *>(&As[cur][wm + a * 16 + r16][ks + q8]);
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+                bfr[b] = *reinterpret_cast<const bf16x8 *>(&Bs[cur][wn + b * 16 + r16][ks + q8]);
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b)
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a], bfr[b], acc[a][b], 0, 0, 0);
+        }
+        if (more) {
+            store_tile<A_KC>(As[cur ^ 1], ra, tid);
+            store_tile<B_KC>(Bs[cur ^ 1], rb, tid);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int m = m0 + wm + a * 16 + 4 * (lane >> 4) + r;
+                const int n = n0 + wn + b * 16 + r16;
+                epilogue<EPI>(g, m, n, acc[a][b][r]);
             }
 }
 
@@ -323,28 +443,36 @@ __global__ __launch_bounds__(64) void audit_kernel(const T *__restrict__ N, size
 
 }  // namespace
 
+template <bool A_KC, bool B_KC, int EPI>
+static void gemm_launch(const GemmArgs &g, hipStream_t s)
+{
+    if (g.M % WM == 0 && g.N % WN == 0 && g.K % WK == 0)
+        hipLaunchKernelGGL((gemm_wide_kernel<A_KC, B_KC, EPI>), dim3(g.M / WM, g.N / WN), dim3(256), 0, s, g);
+    else
+        hipLaunchKernelGGL((gemm_kernel<A_KC, B_KC, EPI>), dim3(g.M / BM, g.N / BN), dim3(256), 0, s, g);
+}
+
 template <bool A_KC, bool B_KC>
 static hipError_t gemm_dispatch(const GemmArgs &g, int epi, hipStream_t s)
 {
-    dim3 grid(g.M / BM, g.N / BN);
     switch (epi) {
     case EPI_STORE_F32:
-        hipLaunchKernelGGL((gemm_kernel<A_KC, B_KC, EPI_STORE_F32>), grid, dim3(256), 0, s, g);
+        gemm_launch<A_KC, B_KC, EPI_STORE_F32>(g, s);
         break;
     case EPI_BIAS_GELU:
-        hipLaunchKernelGGL((gemm_kernel<A_KC, B_KC, EPI_BIAS_GELU>), grid, dim3(256), 0, s, g);
+        gemm_launch<A_KC, B_KC, EPI_BIAS_GELU>(g, s);
         break;
     case EPI_BIAS_F32:
-        hipLaunchKernelGGL((gemm_kernel<A_KC, B_KC, EPI_BIAS_F32>), grid, dim3(256), 0, s, g);
+        gemm_launch<A_KC, B_KC, EPI_BIAS_F32>(g, s);
         break;
     case EPI_BIAS_GELU_INFER:
-        hipLaunchKernelGGL((gemm_kernel<A_KC, B_KC, EPI_BIAS_GELU_INFER>), grid, dim3(256), 0, s, g);
+        gemm_launch<A_KC, B_KC, EPI_BIAS_GELU_INFER>(g, s);
         break;
     case EPI_BIAS_ROW_F32:
-        hipLaunchKernelGGL((gemm_kernel<A_KC, B_KC, EPI_BIAS_ROW_F32>), grid, dim3(256), 0, s, g);
+        gemm_launch<A_KC, B_KC, EPI_BIAS_ROW_F32>(g, s);
         break;
     default:
-        hipLaunchKernelGGL((gemm_kernel<A_KC, B_KC, EPI_GELU_BACKWARD>), grid, dim3(256), 0, s, g);
+        gemm_launch<A_KC, B_KC, EPI_GELU_BACKWARD>(g, s);
         break;
     }
     return hipGetLastError();

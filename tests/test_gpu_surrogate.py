@@ -27,7 +27,8 @@ def rel_l2(a, b):
 
 
 def _net_and_batch(seed=0, in_dim=100, out_dim=70, hidden=64, layers=2, batch=100):
-    net = sg.SourceNetwork(in_dim, out_dim, hidden=hidden, hidden_layers=layers, batch_capacity=256, seed=seed)
+    net = sg.SourceNetwork(in_dim, out_dim, hidden=hidden, hidden_layers=layers, batch_capacity=max(256, batch),
+                           seed=seed)
     rng = np.random.default_rng(seed)
     X = rng.normal(size=(batch, in_dim)).astype(np.float32)
     T = (0.3 * rng.normal(size=(batch, out_dim))).astype(np.float32)
@@ -76,6 +77,28 @@ def test_gradients_match_oracle():
         off += nW
         assert rel_l2(g[off:off + b.size], gb) < 1e-2
         off += b.size
+    net.close()
+
+
+def test_large_tile_gemm_forward_and_gradients_match_oracle():
+    """Dims that are multiples of 128 take the 128 x 128 MFMA 16x16x32 GEMM
+    (forward, weight and input gradients, all operand orientations)."""
+    say("mlp large-tile gemm vs oracle")
+    net, X, T = _net_and_batch(3, in_dim=200, out_dim=250, hidden=256, layers=2, batch=256)
+    assert net.dims == [256, 256, 256, 256]
+    layers, Xp, Tp = _oracle_inputs(net, X, T)
+    Y = net.forward(X)
+    Ye, _ = mo.forward(layers, Xp, emulate=True)
+    e_f = rel_l2(Y, Ye[:X.shape[0], :net.out_dim])
+    loss, g = net.gradients(X, T)
+    Le, ge = mo.backward(layers, Xp, Tp, X.shape[0], net.out_dim, emulate=True)
+    e_g = rel_l2(g, mo.pack(ge))
+    say(f"large-tile: forward rel-L2 {e_f:.2e}, gradient rel-L2 {e_g:.2e}")
+    assert e_f < 2e-3 and e_g < 5e-3 and abs(loss - Le) / Le < 1e-4
+    off = 0
+    for (W, b), (gW, gb) in zip(layers, ge):
+        assert rel_l2(g[off:off + W.size], gW.ravel()) < 1e-2
+        off += W.size + b.size
     net.close()
 
 
