@@ -142,8 +142,9 @@ int rpl_conservation(rpl_ctx *ctx, double *total_action, double *total_energy);
  *               with BulkWindDrag(:linear | :capped) (Physics/shared/drag.jl:21-34)
  *               and, when stress_cap = 1, the bundle's wave-supported-stress
  *               cap (precomputed_sources.jl:126-170);
- *   dissipation MeanSpectrumWhitecapping (Physics/Dissipation/mean_spectrum.jl:20-140)
- *               or LocalSaturationDissipation (Physics/Dissipation/local_saturation.jl:18-52).
+ *   dissipation MeanSpectrumWhitecapping (Physics/Dissipation/mean_spectrum.jl:20-140),
+ *               LocalSaturationDissipation (Physics/Dissipation/local_saturation.jl:18-52)
+ *               or WhitecappingDissipation (Sources/Sources.jl:105-122, 583-598).
  * Winds: a uniform speed and direction (the reference's `wind::Number` with
  * `direction`), or HollandHurricaneWind on a two-point LinearStormTrack
  * (Forcing/winds.jl:4-41, 89-198), evaluated per column at the clock time of
@@ -151,7 +152,12 @@ int rpl_conservation(rpl_ctx *ctx, double *total_action, double *total_energy);
 enum { RPL_WIND_INPUT_NONE = 0, RPL_WIND_INPUT_PRESSURE_CORRELATION = 1 };
 enum { RPL_DRAG_LINEAR = 0, RPL_DRAG_CAPPED = 1 };
 enum { RPL_WIND_UNIFORM = 0, RPL_WIND_HOLLAND = 1 };
-enum { RPL_DISSIPATION_NONE = 0, RPL_DISSIPATION_MEAN_SPECTRUM = 1, RPL_DISSIPATION_LOCAL_SATURATION = 2 };
+enum {
+    RPL_DISSIPATION_NONE = 0,
+    RPL_DISSIPATION_MEAN_SPECTRUM = 1,
+    RPL_DISSIPATION_LOCAL_SATURATION = 2,
+    RPL_DISSIPATION_WHITECAPPING = 3
+};
 
 typedef struct rpl_sources {
     int32_t wind_input, drag, wind_field, stress_cap, dissipation, rotation; /* rotation: +1 ccw, -1 cw */
@@ -161,6 +167,10 @@ typedef struct rpl_sources {
     double track_t0, track_t1, track_x0, track_y0, track_x1, track_y1;
     double cds, delta1, delta2, p_mean;          /* MeanSpectrumWhitecapping */
     double ls_cds, ls_br, ls_p, ls_sigma_power;  /* LocalSaturationDissipation */
+    /* WhitecappingDissipation (Sources/Sources.jl:105-122, 583-598) with a
+     * constant rate: lambda = rate * max(m0 / threshold - 1, 0)^saturation_power
+     * * <kappa^wavenumber_power>_cell / reference_wavenumber^wavenumber_power. */
+    double wc_rate, wc_threshold, wc_saturation_power, wc_wavenumber_power, wc_reference_wavenumber;
 } rpl_sources;
 
 /* Installs (or, with NULL, removes) the source terms.  x_nodes / y_nodes are

@@ -233,14 +233,17 @@ class OrcSources(ctypes.Structure):
                    "vmax", "rmax", "radius", "shape", "inflow_angle", "background_x", "background_y",
                    "track_t0", "track_t1", "track_x0", "track_y0", "track_x1", "track_y1",
                    "cds", "delta1", "delta2", "p_mean",
-                   "ls_cds", "ls_br", "ls_p", "ls_sigma_power")]
+                   "ls_cds", "ls_br", "ls_p", "ls_sigma_power",
+                   "wc_rate", "wc_threshold", "wc_saturation_power", "wc_wavenumber_power",
+                   "wc_reference_wavenumber")]
 
 
 class OrcSourceCtx(ctypes.Structure):
     _fields_ = [("src", ctypes.POINTER(OrcSources)), ("nx", ctypes.c_int), ("ny", ctypes.c_int),
                 ("nk", ctypes.c_int), ("nphi", ctypes.c_int), ("x_nodes", ctypes.c_void_p),
                 ("y_nodes", ctypes.c_void_p), ("kappa", ctypes.c_void_p), ("phi", ctypes.c_void_p),
-                ("w", ctypes.c_void_p), ("time", ctypes.c_double)]
+                ("w", ctypes.c_void_p), ("time", ctypes.c_double),
+                ("kappa_faces", ctypes.c_void_p), ("phi_faces", ctypes.c_void_p)]
 
 
 def sources_from(struct):
@@ -271,10 +274,11 @@ class OracleSourceModel(OracleModel):
         sg = workload.spectral_grid
         self._kc, self._pc = _c(sg.kappa), _c(sg.phi)
         self._w = _c(self.tables["w"])
+        self._kf, self._pf = _c(sg.kappa_faces), _c(sg.phi_faces)
         nx, ny, nk, npf = self.size
         self._ctx = OrcSourceCtx(ctypes.pointer(self._src), nx, ny, nk, npf, self._xn.ctypes.data,
                                  self._yn.ctypes.data, self._kc.ctypes.data, self._pc.ctypes.data,
-                                 self._w.ctypes.data, 0.0)
+                                 self._w.ctypes.data, 0.0, self._kf.ctypes.data, self._pf.ctypes.data)
         self._hook = ctypes.cast(lib().orc_sources_hook_f64, ctypes.c_void_p)
 
     def step(self, dt, nsteps=1, hook=None, user=None):

@@ -7,6 +7,7 @@
  *               with BulkWindDrag          (Physics/shared/drag.jl:21-34)
  *   dissipation MeanSpectrumWhitecapping   (Physics/Dissipation/mean_spectrum.jl:20-140)
  *               LocalSaturationDissipation (Physics/Dissipation/local_saturation.jl:18-52)
+ *               WhitecappingDissipation    (Sources/Sources.jl:105-122, 583-598; constant rate)
  * and the winds UniformWind (a number + direction) and HollandHurricaneWind
  * on a two-point LinearStormTrack (Forcing/winds.jl:4-41, 89-198).
  *
@@ -20,7 +21,7 @@ typedef struct orc_sources {
     int drag;         /* 0 BulkWindDrag(:linear), 1 BulkWindDrag(:capped) */
     int wind_field;   /* 0 uniform (speed, direction), 1 Holland hurricane */
     int stress_cap;   /* 1: bundle stress cap (precomputed_sources.jl:126-170) */
-    int dissipation;  /* 0 none, 1 MeanSpectrumWhitecapping, 2 LocalSaturationDissipation */
+    int dissipation;  /* 0 none, 1 MeanSpectrumWhitecapping, 2 LocalSaturationDissipation, 3 WhitecappingDissipation */
     int rotation;     /* +1 counter-clockwise, -1 clockwise */
     double wind_speed, wind_direction;
     double beta_max, z_alpha, p_in, alpha0, rho_air, rho_water, von_karman, gravity, cap_epsilon;
@@ -28,6 +29,7 @@ typedef struct orc_sources {
     double track_t0, track_t1, track_x0, track_y0, track_x1, track_y1;
     double cds, delta1, delta2, p_mean;
     double ls_cds, ls_br, ls_p, ls_sigma_power;
+    double wc_rate, wc_threshold, wc_saturation_power, wc_wavenumber_power, wc_reference_wavenumber;
 } orc_sources;
 
 /* drag.jl:25-34. */
@@ -125,7 +127,20 @@ typedef struct orc_source_ctx {
     const double *kappa, *phi;        /* nk, nphi centres */
     const double *w;                  /* (nk, nphi) cell measures */
     double time;                      /* clock time of the step */
+    const double *kappa_faces, *phi_faces;  /* nk + 1, nphi + 1 */
 } orc_source_ctx;
+
+/* radial_power_source_factor (Sources.jl:569-574): the cell average of
+ * kappa^p, radial_power_annular_measure / cell measure
+ * (finite_volume_integration.jl:24-33, 71-73), over kappa_ref^p. */
+static double orc_radial_power_factor(const orc_source_ctx *c, int m, int n, double p, double kref)
+{
+    if (p == 0)
+        return 1.0;
+    const double k1 = c->kappa_faces[m], k2 = c->kappa_faces[m + 1];
+    const double measure = (c->phi_faces[n + 1] - c->phi_faces[n]) * (pow(k2, p + 2) - pow(k1, p + 2)) / (p + 2);
+    return measure / c->w[m + (size_t)c->nk * n] / pow(kref, p);
+}
 
 /* Adds positive - damping * N for every cell (tendencies.jl:88-94), with the
  * bundle state of precomputed_sources.jl:41-101, 126-218 formed first from
@@ -193,6 +208,12 @@ void orc_sources_hook_f64(const double *N, double *G, void *user)
                 sbar = pow(sp / m0, 1 / s->p_mean);
             }
         }
+        /* local_zeroth_moment (Sources.jl:1471-1480) for WhitecappingDissipation. */
+        double m0w = 0;
+        if (s->dissipation == 3)
+            for (int n = 0; n < nphi; ++n)
+                for (int m = 0; m < nk; ++m)
+                    m0w += N[ij + (size_t)nx * ny * ((size_t)m + (size_t)nk * n)] * c->w[m + (size_t)nk * n];
         /* Per cell (source_split of each member, bundle sum). */
         for (int n = 0; n < nphi; ++n)
             for (int m = 0; m < nk; ++m) {
@@ -236,6 +257,16 @@ void orc_sources_hook_f64(const double *N, double *G, void *user)
                         else
                             positive += -rate * a;
                     }
+                } else if (s->dissipation == 3) {
+                    /* Sources.jl:588-597. */
+                    double excess = m0w / s->wc_threshold - 1;
+                    excess = excess > 0 ? excess : 0.0;
+                    double rate = s->wc_rate * pow(excess, s->wc_saturation_power) *
+                                  orc_radial_power_factor(c, m, n, s->wc_wavenumber_power, s->wc_reference_wavenumber);
+                    if (rate >= 0)
+                        damping += rate;
+                    else
+                        positive += -rate * a;
                 }
                 G[cidx] += positive - damping * a;
             }

@@ -63,7 +63,7 @@ struct rpl_ctx {
     void *factor = nullptr, *cm0 = nullptr, *kbar = nullptr, *sbar = nullptr;
     void *k_cell = nullptr, *theta_cell = nullptr, *sigma_cell = nullptr, *C_cell = nullptr;  // per bin
     void *t_phi = nullptr, *t_w = nullptr, *t_sigma_k = nullptr, *t_cos = nullptr, *t_sin = nullptr;
-    void *t_kpow = nullptr, *t_spow = nullptr;
+    void *t_kpow = nullptr, *t_spow = nullptr, *t_wc = nullptr;
     std::vector<double> h_kc, h_pc, h_kf, h_pf;  // geometry kept for source tables
     std::vector<double> h_sigma;                 // per-bin intrinsic frequency of the source tables
 
@@ -188,6 +188,10 @@ rpl::StageParams<T> stage_params(const rpl_ctx *c, double dt)
         q.ls_br = (T)s.ls_br;
         q.ls_p = (T)s.ls_p;
         q.ls_q = (T)s.ls_sigma_power;
+        q.wc_rate = (T)s.wc_rate;
+        q.wc_thr = (T)s.wc_threshold;
+        q.wc_sp = (T)s.wc_saturation_power;
+        q.wc_factor = (const T *)c->t_wc;
         if (c->s_active) {
             q.surrogate = 1;
             q.rate = c->rate;
@@ -241,7 +245,8 @@ int launch_state(rpl_ctx *c, const void *Nk)
 {
     if (!c->has_sources)
         return RPL_OK;
-    const bool needs = (c->src.wind_input && c->src.stress_cap) || c->src.dissipation == RPL_DISSIPATION_MEAN_SPECTRUM;
+    const bool needs = (c->src.wind_input && c->src.stress_cap) || c->src.dissipation == RPL_DISSIPATION_MEAN_SPECTRUM ||
+                       c->src.dissipation == RPL_DISSIPATION_WHITECAPPING;
     if (!needs)
         return RPL_OK;
     const int nx = c->cfg.nx, ny = c->cfg.ny, nk = c->cfg.nkappa, np = c->cfg.nphi;
@@ -570,7 +575,8 @@ int ensemble_descriptors(rpl_ensemble *e, double dt)
     std::vector<rpl_ctx *> sm;
     for (rpl_ctx *c : e->members)
         if (c->has_sources && ((c->src.wind_input && c->src.stress_cap) ||
-                               c->src.dissipation == RPL_DISSIPATION_MEAN_SPECTRUM))
+                               c->src.dissipation == RPL_DISSIPATION_MEAN_SPECTRUM ||
+                               c->src.dissipation == RPL_DISSIPATION_WHITECAPPING))
             sm.push_back(c);
     const int S = (int)sm.size();
     std::vector<rpl::EnsembleState> hs(3 * (size_t)S);
@@ -750,7 +756,7 @@ int rpl_destroy(rpl_ctx *c)
                     (void *)c->energy, (void *)c->diag, (void *)c->partials, c->scratch_in, (void *)c->xn,
                     (void *)c->yn, c->U10, c->theta_u, c->ustar, c->factor, c->cm0, c->kbar, c->sbar, c->k_cell,
                     c->theta_cell, c->sigma_cell, c->C_cell, c->t_phi, c->t_w, c->t_sigma_k, c->t_cos, c->t_sin,
-                    c->t_kpow, c->t_spow, (void *)c->sX, (void *)c->sA, (void *)c->sB, (void *)c->rate,
+                    c->t_kpow, c->t_spow, c->t_wc, (void *)c->sX, (void *)c->sA, (void *)c->sB, (void *)c->rate,
                     (void *)c->out_scale, (void *)c->s_err, (void *)c->use_plain, (void *)c->s_counts,
                     (void *)c->bin_hi})
         free_ptr(p);
@@ -1165,7 +1171,7 @@ int rpl_set_sources(rpl_ctx *c, const rpl_sources *src, const double *x_nodes, c
     const rpl_sources &s = *src;
     if (s.wind_input != RPL_WIND_INPUT_NONE && s.wind_input != RPL_WIND_INPUT_PRESSURE_CORRELATION)
         return fail(RPL_ERR_ARG, "unsupported wind input");
-    if (s.dissipation < RPL_DISSIPATION_NONE || s.dissipation > RPL_DISSIPATION_LOCAL_SATURATION)
+    if (s.dissipation < RPL_DISSIPATION_NONE || s.dissipation > RPL_DISSIPATION_WHITECAPPING)
         return fail(RPL_ERR_ARG, "unsupported dissipation");
     if (s.wind_input) {
         if (!x_nodes || !y_nodes)
@@ -1192,6 +1198,15 @@ int rpl_set_sources(rpl_ctx *c, const rpl_sources *src, const double *x_nodes, c
     }
     if (s.dissipation == RPL_DISSIPATION_MEAN_SPECTRUM && !(s.p_mean != 0))
         return fail(RPL_ERR_ARG, "mean-spectrum exponent p must be nonzero");
+    if (s.dissipation == RPL_DISSIPATION_WHITECAPPING) {
+        // Sources.jl:589-590, 570-571
+        if (!(s.wc_threshold > 0))
+            return fail(RPL_ERR_ARG, "whitecapping saturation threshold must be positive");
+        if (!(s.wc_reference_wavenumber > 0))
+            return fail(RPL_ERR_ARG, "whitecapping reference wavenumber must be positive");
+        if (!(s.wc_wavenumber_power >= 0))
+            return fail(RPL_ERR_ARG, "wavenumber power must be nonnegative");
+    }
     RPL_HIP(hipSetDevice(c->device));
     const int nx = c->cfg.nx, ny = c->cfg.ny, nk = c->cfg.nkappa, np = c->cfg.nphi;
     const size_t tab = (size_t)nk * np, plane = c->plane;
@@ -1200,7 +1215,7 @@ int rpl_set_sources(rpl_ctx *c, const rpl_sources *src, const double *x_nodes, c
     for (void **b : col_bufs)
         if (!*b && (rc = alloc(b, plane * c->esize)))
             return rc;
-    void **tab_bufs[] = {&c->k_cell, &c->theta_cell, &c->sigma_cell, &c->C_cell, &c->t_w};
+    void **tab_bufs[] = {&c->k_cell, &c->theta_cell, &c->sigma_cell, &c->C_cell, &c->t_w, &c->t_wc};
     for (void **b : tab_bufs)
         if (!*b && (rc = alloc(b, tab * c->esize)))
             return rc;
@@ -1219,7 +1234,7 @@ int rpl_set_sources(rpl_ctx *c, const rpl_sources *src, const double *x_nodes, c
     // Per-bin values of the per-cell path: k = hypot(kx, ky), theta = atan(ky, kx)
     // from (kx, ky) = kappa (cos phi, sin phi) (pressure_correlation.jl:118-123).
     const double g = s.gravity;
-    std::vector<double> kcell(tab), thcell(tab), scell(tab), Ccell(tab), w(tab);
+    std::vector<double> kcell(tab), thcell(tab), scell(tab), Ccell(tab), w(tab), wcf(tab, 0.0);
     std::vector<double> sk(nk), kpow(nk), spow(nk), cphi(np), sphi(np);
     for (int n = 0; n < np; ++n) {
         cphi[n] = std::cos(c->h_pc[n]);
@@ -1234,6 +1249,18 @@ int rpl_set_sources(rpl_ctx *c, const rpl_sources *src, const double *x_nodes, c
             Ccell[t] = k > 0 ? scell[t] / k : 0.0;
             const double k1 = c->h_kf[m], k2 = c->h_kf[m + 1];
             w[t] = ((k2 * k2 - k1 * k1) / 2) * (c->h_pf[n + 1] - c->h_pf[n]);
+            if (s.dissipation == RPL_DISSIPATION_WHITECAPPING) {
+                // radial_power_source_factor (Sources.jl:569-574): the cell average
+                // of kappa^p (finite_volume_integration.jl:24-33, 71-73) over kappa_ref^p.
+                const double pw = s.wc_wavenumber_power;
+                if (pw == 0) {
+                    wcf[t] = 1.0;
+                } else {
+                    const double measure = (c->h_pf[n + 1] - c->h_pf[n]) * (std::pow(k2, pw + 2) - std::pow(k1, pw + 2)) /
+                                           (pw + 2);
+                    wcf[t] = measure / w[t] / std::pow(s.wc_reference_wavenumber, pw);
+                }
+            }
         }
     }
     for (int m = 0; m < nk; ++m) {
@@ -1244,7 +1271,8 @@ int rpl_set_sources(rpl_ctx *c, const rpl_sources *src, const double *x_nodes, c
     c->h_sigma = scell;
     if ((rc = upload_real(c, kcell.data(), c->k_cell, tab)) || (rc = upload_real(c, thcell.data(), c->theta_cell, tab)) ||
         (rc = upload_real(c, scell.data(), c->sigma_cell, tab)) || (rc = upload_real(c, Ccell.data(), c->C_cell, tab)) ||
-        (rc = upload_real(c, w.data(), c->t_w, tab)) || (rc = upload_real(c, sk.data(), c->t_sigma_k, nk)) ||
+        (rc = upload_real(c, w.data(), c->t_w, tab)) || (rc = upload_real(c, wcf.data(), c->t_wc, tab)) ||
+        (rc = upload_real(c, sk.data(), c->t_sigma_k, nk)) ||
         (rc = upload_real(c, kpow.data(), c->t_kpow, nk)) || (rc = upload_real(c, spow.data(), c->t_spow, nk)) ||
         (rc = upload_real(c, c->h_pc.data(), c->t_phi, np)) || (rc = upload_real(c, cphi.data(), c->t_cos, np)) ||
         (rc = upload_real(c, sphi.data(), c->t_sin, np)))

@@ -114,6 +114,14 @@ __global__ __launch_bounds__(256) void source_state_kernel(int nx, int ny, int n
         }
         factor[ij] = f;
     }
+    if (s.dissipation == 3) {
+        // local_zeroth_moment (Sources.jl:1471-1480): every cell, n outer / m inner.
+        T m0 = T(0);
+        for (int n = 0; n < nphi; ++n)
+            for (int m = 0; m < nk; ++m)
+                m0 += N[ij + plane * ((size_t)m + (size_t)nk * n)] * tb.w[m + nk * n];
+        m0o[ij] = m0;
+    }
     if (s.dissipation == 1) {
         T m0 = T(0), kp = T(0), sp = T(0);
         for (int n = 0; n < nphi; ++n)
@@ -158,7 +166,7 @@ __device__ __forceinline__ void state_block_f32(int nx, int ny, int nk, int nphi
     bool active = ij < plane;
     if (active && s.surrogate && !(gld<true>(s.use_plain, ij) & 3))
         active = false;  // the network supplies this column's source
-    const bool cap = s.wind_input && s.stress_cap, ms = s.dissipation == 1;
+    const bool cap = s.wind_input && s.stress_cap, ms = s.dissipation == 1, wc = s.dissipation == 3;
     float twx = 0.0f, twy = 0.0f, m0 = 0.0f, kp = 0.0f, sp = 0.0f;
     float U10 = 0.0f, ustar = 0.0f, theta_u = 0.0f;
     if (active && cap) {
@@ -167,7 +175,7 @@ __device__ __forceinline__ void state_block_f32(int nx, int ny, int nk, int nphi
         theta_u = gld<true>(s.theta_u, ij);
     }
     const bool wind = cap && U10 > 0.0f && ustar > 0.0f;
-    if (active && (wind || ms)) {
+    if (active && (wind || ms || wc)) {
         int m = g, n = 0;
         while (m >= nk) {
             m -= nk;
@@ -177,6 +185,8 @@ __device__ __forceinline__ void state_block_f32(int nx, int ny, int nk, int nphi
         for (int t = g; t < nbins; t += G) {
             const float a = N[ij + plane * (size_t)t];
             const float k = gld<true>(tb.kappa, m);
+            if (wc)
+                m0 += a * gld<true>(tb.w, t);  // zeroth moment over every cell
             if (a > 0.0f && k > 0.0f) {
                 if (wind) {
                     const float sigma = gld<true>(tb.sigma_k, m), C = sigma / k;
@@ -231,6 +241,8 @@ __device__ __forceinline__ void state_block_f32(int nx, int ny, int nk, int nphi
         }
         gp_store(factor, ij, f);
     }
+    if (wc)
+        gp_store(m0o, ij, m0);
     if (ms) {
         gp_store(m0o, ij, m0);
         gp_store(kbaro, ij, m0 > 0.0f ? powf(kp / m0, tb.inv_p) : 0.0f);

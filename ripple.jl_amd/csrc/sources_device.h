@@ -34,6 +34,8 @@ struct SourceParams {
     T z_alpha, p_in, alpha0, kvk, g;
     T cds, d1, d2;              // mean-spectrum whitecapping
     T ls_cds, ls_br, ls_p, ls_q; // local saturation
+    T wc_rate, wc_thr, wc_sp;    // whitecapping: rate, saturation threshold and power
+    const T *wc_factor;          // whitecapping: <kappa^p>_cell / kappa_ref^p per bin
     // learned surrogate: where surrogate != 0 and use_plain[col] has neither
     // bit 0 (outside the training envelope) nor bit 2 (failed audit), the
     // source is rate[col + rate_ld * t] * out_scale[t] * N.
@@ -146,6 +148,16 @@ __device__ __forceinline__ T source_term(const SourceParams<T> &s, unsigned col,
                     positive += -rate * a;
             }
         }
+    } else if (s.dissipation == 3) {
+        // Sources.jl:588-598 with the column's zeroth moment from the state kernel.
+        const T m0 = gld<AS1>(s.m0, col);
+        T excess = m0 / s.wc_thr - T(1);
+        excess = excess > T(0) ? excess : T(0);
+        const T rate = s.wc_rate * dpow(excess, s.wc_sp) * gld<AS1>(s.wc_factor, t);
+        if (rate >= T(0))
+            damping += rate;
+        else
+            positive += -rate * a;
     }
     return positive - damping * a;
 }

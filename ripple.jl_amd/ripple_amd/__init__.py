@@ -13,7 +13,8 @@ from .grids import (RectilinearGrid, PolarWaveVectorGrid, FrequencyDirectionGrid
 from .model import (SpectralWaveModel, InfiniteDepth, Clock, Simulation, WENO, set_, time_step_, run_,
                     interior)
 from .sources import (BulkWindDrag, LinearStormTrack, HollandHurricaneWind, PressureCorrelationInput,
-                      MeanSpectrumWhitecapping, LocalSaturationDissipation, PrecomputedSources, wind_state)
+                      MeanSpectrumWhitecapping, LocalSaturationDissipation, WhitecappingDissipation,
+                      PrecomputedSources, wind_state)
 from .surrogate import (SourceNetwork, SurrogateConfig, training_pairs, attach_surrogate, surrogate_stats,
                         train as train_surrogate)
 from .ensemble import Ensemble
@@ -29,7 +30,8 @@ __all__ = [
     "interior", "m0", "root_mean_square_wavenumber", "mean_direction", "significant_wave_height",
     "total_action", "total_deep_water_energy",
     "BulkWindDrag", "LinearStormTrack", "HollandHurricaneWind", "PressureCorrelationInput",
-    "MeanSpectrumWhitecapping", "LocalSaturationDissipation", "PrecomputedSources", "wind_state",
+    "MeanSpectrumWhitecapping", "LocalSaturationDissipation", "WhitecappingDissipation", "PrecomputedSources",
+    "wind_state",
     "SourceNetwork", "SurrogateConfig", "training_pairs", "attach_surrogate", "surrogate_stats",
     "train_surrogate", "Ensemble",
 ]

@@ -57,7 +57,9 @@ class rpl_sources(ctypes.Structure):
                    "vmax", "rmax", "radius", "shape", "inflow_angle", "background_x", "background_y",
                    "track_t0", "track_t1", "track_x0", "track_y0", "track_x1", "track_y1",
                    "cds", "delta1", "delta2", "p_mean",
-                   "ls_cds", "ls_br", "ls_p", "ls_sigma_power")]
+                   "ls_cds", "ls_br", "ls_p", "ls_sigma_power",
+                   "wc_rate", "wc_threshold", "wc_saturation_power", "wc_wavenumber_power",
+                   "wc_reference_wavenumber")]
 
 
 class rpl_surrogate(ctypes.Structure):

@@ -98,6 +98,24 @@ class LocalSaturationDissipation:
                                                                                            gravity))
 
 
+class WhitecappingDissipation:
+    """`WhitecappingDissipation(; rate, saturation_threshold, saturation_power,
+    wavenumber_power, reference_wavenumber)` (Sources/Sources.jl:105-122,
+    583-598): S = -rate * max(m0 / threshold - 1, 0)^saturation_power *
+    <kappa^wavenumber_power>_cell / reference_wavenumber^wavenumber_power * N,
+    with m0 the column's zeroth moment.  The rate is a constant here."""
+
+    def __init__(self, rate=0.0, saturation_threshold=1.0, saturation_power=1.0, wavenumber_power=1.0,
+                 reference_wavenumber=1.0):
+        if callable(rate) or np.ndim(rate) != 0:
+            raise NotImplementedError("WhitecappingDissipation takes a constant rate on the device")
+        self.rate = float(rate)
+        self.saturation_threshold = float(saturation_threshold)
+        self.saturation_power = float(saturation_power)
+        self.wavenumber_power = float(wavenumber_power)
+        self.reference_wavenumber = float(reference_wavenumber)
+
+
 class PrecomputedSources:
     """`PrecomputedSources(; wind_input, dissipation)`: the bundle whose wind
     input carries the wave-supported-stress cap."""
@@ -119,7 +137,7 @@ def to_struct(sources):
         wi, diss, cap = sources.wind_input, sources.dissipation, 1
     elif isinstance(sources, PressureCorrelationInput):
         wi, diss, cap = sources, None, 0
-    elif isinstance(sources, (MeanSpectrumWhitecapping, LocalSaturationDissipation)):
+    elif isinstance(sources, (MeanSpectrumWhitecapping, LocalSaturationDissipation, WhitecappingDissipation)):
         wi, diss, cap = None, sources, 0
     else:
         raise TypeError(f"unsupported sources {type(sources).__name__}")
@@ -149,6 +167,11 @@ def to_struct(sources):
     elif isinstance(diss, LocalSaturationDissipation):
         s.dissipation = 2
         s.ls_cds, s.ls_br, s.ls_p, s.ls_sigma_power = diss.C_ds, diss.B_r, diss.p, diss.sigma_power
+    elif isinstance(diss, WhitecappingDissipation):
+        s.dissipation = 3
+        s.wc_rate, s.wc_threshold = diss.rate, diss.saturation_threshold
+        s.wc_saturation_power, s.wc_wavenumber_power = diss.saturation_power, diss.wavenumber_power
+        s.wc_reference_wavenumber = diss.reference_wavenumber
     return s
 
 

@@ -99,6 +99,34 @@ def test_source_steps_fp32_fast(oracle, case):
     assert err <= 2e-3
 
 
+@pytest.mark.parametrize("bundle", [True, False])
+def test_whitecapping_fp64_parity(oracle, bundle):
+    """WhitecappingDissipation (Sources.jl:583-598) on the device against the
+    oracle, inside the bundle with the wind input and on its own."""
+    say(f"whitecapping fp64 bundle={bundle}")
+    wl = workloads.fetch_limited_growth(float_type=np.float64, nx=20, ny=18, nz=4, nfreq=10, nphi=12, dt=10.0)
+    wc = ra.WhitecappingDissipation(rate=2e-4, saturation_threshold=1e-5, saturation_power=1.5,
+                                    wavenumber_power=2.0, reference_wavenumber=0.5)
+    if bundle:
+        wl.model_kwargs["sources"] = ra.PrecomputedSources(
+            wind_input=wl.model_kwargs["sources"].wind_input, dissipation=wc)
+    else:
+        wl.model_kwargs["sources"] = wc
+    m = device(wl, np.float64)
+    om = oracle_model(oracle, wl)
+    G, Go = m.tendency(), om.tendency()
+    e0 = rel_l2(G, Go)
+    for _ in range(3):
+        ra.time_step_(m, wl.dt)
+        om.step(wl.dt)
+    e = rel_l2(m.action, om.action)
+    say(f"whitecapping bundle={bundle}: tendency rel-L2 {e0:.2e}, after 3 steps {e:.2e}")
+    assert e0 <= 1e-12 and e <= 1e-12
+    m32 = device(wl, np.float32)
+    ra.time_step_(m32, wl.dt, 3)
+    assert rel_l2(m32.action, om.action) <= 2e-3
+
+
 def test_fetch_limited_growth_full_size_properties():
     """Config 3 at its full size in FP32: waves grow under a 10 m/s wind, stay
     finite and non-negative, and the spectrum's energy moves toward the wind

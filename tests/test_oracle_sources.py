@@ -141,3 +141,29 @@ def test_stress_cap_limits_wave_supported_stress(oracle):
     # tau_w of the capped rates (rate N / C * w * rho_w g, projected on the wind) = 0.4 rho_a u*^2.
     tw = np.sum(capped[0, 0] / C[:, None] * sg.weights * 1025.0 * 9.81 * np.cos(sg.phi)[None, :])
     assert tw == pytest.approx(0.4 * 1.225 * us * us, rel=1e-10)
+
+
+def test_whitecapping_known_answers(oracle):
+    """test/sources/source_terms.jl:236-261: uniform N = 1 on kappa = [0.5, 1, 2],
+    phi = [0]: every tendency negative, growing in magnitude with kappa, the
+    ratio of two bins' tendencies equal to the ratio of their cell averages of
+    kappa^2; a saturation threshold above m0 gives no tendency."""
+    g = grids.RectilinearGrid(size=(1, 1, 1), x=(0, 1), y=(0, 1), z=(0, 1))
+    sg = grids.PolarWaveVectorGrid(kappa=[0.5, 1.0, 2.0], phi=[0.0])
+    wl = workloads.Workload("wc", g, sg, None, np.ones((1, 1, 3, 1), order="F"), 1.0, 1,
+                            dict(horizontal_advection=None, timestepper="ForwardEuler"))
+    wc = ra.WhitecappingDissipation(rate=0.2, saturation_threshold=0.1, saturation_power=1.0, wavenumber_power=2.0,
+                                    reference_wavenumber=1.0)
+    G = oracle.OracleSourceModel(wl, src_struct(oracle, wc), transport=oracle.MODE_NONE).tendency()[0, 0, :, 0]
+    kf, pf = sg.kappa_faces, sg.phi_faces
+    w = (kf[1:] ** 2 - kf[:-1] ** 2) / 2 * (pf[1] - pf[0])
+    avg_k2 = (pf[1] - pf[0]) * (kf[1:] ** 4 - kf[:-1] ** 4) / 4 / w
+    assert np.all(G < 0)
+    assert G[1] < G[0] and G[2] < G[1]
+    assert G[2] / G[1] == pytest.approx(avg_k2[2] / avg_k2[1], rel=1e-12)
+    # closed form of the first bin: rate * (m0 / threshold - 1) * <k^2> * N
+    m0 = float(np.sum(w))
+    assert G[0] == pytest.approx(-0.2 * (m0 / 0.1 - 1) * avg_k2[0], rel=1e-12)
+    weak = ra.WhitecappingDissipation(rate=0.2, saturation_threshold=100.0)
+    G = oracle.OracleSourceModel(wl, src_struct(oracle, weak), transport=oracle.MODE_NONE).tendency()
+    assert np.max(np.abs(G)) < 1e-14
