@@ -177,6 +177,13 @@ typedef struct rpl_sources {
  * the nx / ny cell-centre coordinates the winds are evaluated at. */
 int rpl_set_sources(rpl_ctx *ctx, const rpl_sources *src, const double *x_nodes, const double *y_nodes);
 
+/* LinearStormTrack of n >= 1 points (Forcing/winds.jl:4-41): the Holland
+ * centre follows the piecewise-linear track (clamped to the end points)
+ * instead of the two track points of rpl_sources.  n = 0 removes it.  Times
+ * must increase strictly (RPL_ERR_ARG otherwise, as the reference's
+ * ArgumentError). */
+int rpl_set_storm_track(rpl_ctx *ctx, int32_t n, const double *times, const double *x, const double *y);
+
 /* Per-column wind state of the last step: U10, wind angle, friction velocity,
  * (nx, ny) FP64 each (the reference's _compute_wind_input_state host loop). */
 int rpl_get_wind(rpl_ctx *ctx, double *U10, double *theta_u, double *u_star);

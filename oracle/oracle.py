@@ -235,7 +235,24 @@ class OrcSources(ctypes.Structure):
                    "cds", "delta1", "delta2", "p_mean",
                    "ls_cds", "ls_br", "ls_p", "ls_sigma_power",
                    "wc_rate", "wc_threshold", "wc_saturation_power", "wc_wavenumber_power",
-                   "wc_reference_wavenumber")]
+                   "wc_reference_wavenumber")] + \
+               [("track_n", ctypes.c_int), ("track_t", ctypes.c_void_p), ("track_x", ctypes.c_void_p),
+                ("track_y", ctypes.c_void_p)]
+
+
+def set_track(src, times, centers):
+    """Give OrcSources an N-point LinearStormTrack (keeps the arrays alive on src)."""
+    src._track = [_c(times), _c([p[0] for p in centers]), _c([p[1] for p in centers])]
+    src.track_n = len(times)
+    src.track_t, src.track_x, src.track_y = [a.ctypes.data for a in src._track]
+    return src
+
+
+def track_at(times, centers, t):
+    T, X, Y = _c(times), _c([p[0] for p in centers]), _c([p[1] for p in centers])
+    cx, cy = ctypes.c_double(), ctypes.c_double()
+    lib().orc_track_at(_I(len(T)), _p(T), _p(X), _p(Y), _D(t), ctypes.byref(cx), ctypes.byref(cy))
+    return cx.value, cy.value
 
 
 class OrcSourceCtx(ctypes.Structure):
@@ -250,7 +267,8 @@ def sources_from(struct):
     """Copy any ctypes structure with the rpl_sources field names into OrcSources."""
     s = OrcSources()
     for name, _ in OrcSources._fields_:
-        setattr(s, name, getattr(struct, name))
+        if hasattr(struct, name):
+            setattr(s, name, getattr(struct, name))
     return s
 
 

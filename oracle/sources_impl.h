@@ -30,6 +30,8 @@ typedef struct orc_sources {
     double cds, delta1, delta2, p_mean;
     double ls_cds, ls_br, ls_p, ls_sigma_power;
     double wc_rate, wc_threshold, wc_saturation_power, wc_wavenumber_power, wc_reference_wavenumber;
+    int track_n;  /* > 0: LinearStormTrack of track_n points, replacing the two-point track */
+    const double *track_t, *track_x, *track_y;
 } orc_sources;
 
 /* drag.jl:25-34. */
@@ -45,9 +47,34 @@ static double orc_friction_velocity(const orc_sources *s, double U10)
     return U10 * sqrt(cd > 0 ? cd : 0.0);
 }
 
-/* winds.jl:28-41: two-point linear storm track. */
+/* winds.jl:28-41 (searchsortedfirst for the upper point). */
+void orc_track_at(int n, const double *T, const double *X, const double *Y, double t, double *cx, double *cy)
+{
+    if (t <= T[0]) {
+        *cx = X[0];
+        *cy = Y[0];
+        return;
+    }
+    if (t >= T[n - 1]) {
+        *cx = X[n - 1];
+        *cy = Y[n - 1];
+        return;
+    }
+    int upper = 0;
+    while (T[upper] < t)
+        ++upper;
+    const int lower = upper - 1;
+    const double w = (t - T[lower]) / (T[upper] - T[lower]);
+    *cx = (1 - w) * X[lower] + w * X[upper];
+    *cy = (1 - w) * Y[lower] + w * Y[upper];
+}
+
 static void orc_track(const orc_sources *s, double t, double *cx, double *cy)
 {
+    if (s->track_n > 0) {
+        orc_track_at(s->track_n, s->track_t, s->track_x, s->track_y, t, cx, cy);
+        return;
+    }
     if (t <= s->track_t0) {
         *cx = s->track_x0;
         *cy = s->track_y0;

@@ -66,6 +66,7 @@ struct rpl_ctx {
     void *t_kpow = nullptr, *t_spow = nullptr, *t_wc = nullptr;
     std::vector<double> h_kc, h_pc, h_kf, h_pf;  // geometry kept for source tables
     std::vector<double> h_sigma;                 // per-bin intrinsic frequency of the source tables
+    std::vector<double> track_t, track_x, track_y;  // storm track (rpl_set_storm_track), or empty
 
     // learned surrogate (rpl_attach_surrogate)
     rpl_mlp *net = nullptr;
@@ -229,6 +230,24 @@ int launch_wind(rpl_ctx *c)
         return RPL_OK;
     hipError_t e;
     const int nx = c->cfg.nx, ny = c->cfg.ny;
+    if (!c->track_t.empty()) {
+        // LinearStormTrack (winds.jl:28-41) of any length: the segment holding
+        // the clock time; the wind kernel interpolates (and clamps) on it.
+        const std::vector<double> &T = c->track_t;
+        const size_t n = T.size();
+        size_t lo = 0, hi = 0;
+        if (n > 1) {
+            const size_t up = std::upper_bound(T.begin(), T.end(), c->time) - T.begin();  // first T > time
+            hi = std::min(std::max<size_t>(up, 1), n - 1);
+            lo = hi - 1;
+        }
+        c->wind.t0 = T[lo];
+        c->wind.t1 = T[hi];
+        c->wind.x0 = c->track_x[lo];
+        c->wind.y0 = c->track_y[lo];
+        c->wind.x1 = c->track_x[hi];
+        c->wind.y1 = c->track_y[hi];
+    }
     if (is_f64(c))
         e = rpl::launch_wind_columns<double>(nx, ny, c->xn, c->yn, c->time, c->wind, (double *)c->U10,
                                              (double *)c->theta_u, (double *)c->ustar, c->stream);
@@ -1312,6 +1331,27 @@ int rpl_set_sources(rpl_ctx *c, const rpl_sources *src, const double *x_nodes, c
     c->src = s;
     c->has_sources = s.wind_input != RPL_WIND_INPUT_NONE || s.dissipation != RPL_DISSIPATION_NONE;
     RPL_HIP(hipStreamSynchronize(c->stream));
+    return RPL_OK;
+}
+
+int rpl_set_storm_track(rpl_ctx *c, int32_t n, const double *times, const double *x, const double *y)
+{
+    if (!c)
+        return fail(RPL_ERR_ARG, "null context");
+    if (n == 0) {
+        c->track_t.clear();
+        c->track_x.clear();
+        c->track_y.clear();
+        return RPL_OK;
+    }
+    if (n < 0 || !times || !x || !y)
+        return fail(RPL_ERR_ARG, "storm-track times and centers must be non-empty");  // winds.jl:14-15
+    for (int k = 1; k < n; ++k)
+        if (!(times[k] > times[k - 1]))
+            return fail(RPL_ERR_ARG, "storm-track times must be strictly increasing");  // winds.jl:16-19
+    c->track_t.assign(times, times + n);
+    c->track_x.assign(x, x + n);
+    c->track_y.assign(y, y + n);
     return RPL_OK;
 }
 

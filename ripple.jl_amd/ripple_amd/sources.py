@@ -38,18 +38,33 @@ class BulkWindDrag:
 
 
 class LinearStormTrack:
-    """`LinearStormTrack(times, centers)`; two points (a constant translation)."""
+    """`LinearStormTrack(times, centers)` (Forcing/winds.jl:4-41): a
+    piecewise-linear storm centre through the points, clamped to the ends."""
 
     def __init__(self, times, centers):
         t = [float(x) for x in times]
-        c = [(float(a), float(b)) for a, b in centers]
+        c = [tuple(float(v) for v in p) for p in centers]
         if len(t) != len(c):
             raise ValueError("storm-track times and centers must have the same length")
-        if len(t) != 2:
-            raise NotImplementedError("the device wind kernel takes a two-point storm track")
-        if not t[1] > t[0]:
+        if len(t) == 0:
+            raise ValueError("storm-track times and centers must be non-empty")
+        if any(not t[k] > t[k - 1] for k in range(1, len(t))):
             raise ValueError("storm-track times must be strictly increasing")
+        if any(len(p) != 2 for p in c):
+            raise ValueError("storm-track centers must be two-component coordinates")
         self.times, self.centers = t, c
+
+    def __call__(self, time):
+        """The centre at `time` (winds.jl:28-41)."""
+        t, c = self.times, self.centers
+        if time <= t[0]:
+            return c[0]
+        if time >= t[-1]:
+            return c[-1]
+        upper = next(k for k in range(len(t)) if t[k] >= time)
+        lower = upper - 1
+        w = (time - t[lower]) / (t[upper] - t[lower])
+        return ((1 - w) * c[lower][0] + w * c[upper][0], (1 - w) * c[lower][1] + w * c[upper][1])
 
 
 class HollandHurricaneWind:
@@ -155,8 +170,10 @@ def to_struct(sources):
             s.vmax, s.rmax, s.radius, s.shape, s.inflow_angle = w.vmax, w.rmax, w.radius, w.shape, w.inflow_angle
             s.background_x, s.background_y = w.background
             s.rotation = w.rotation
-            s.track_t0, s.track_t1 = w.track.times
-            (s.track_x0, s.track_y0), (s.track_x1, s.track_y1) = w.track.centers
+            s.track_t0, s.track_t1 = w.track.times[0], w.track.times[-1]
+            if len(w.track.times) == 1:
+                s.track_t1 = s.track_t0 + 1.0  # a fixed centre
+            (s.track_x0, s.track_y0), (s.track_x1, s.track_y1) = w.track.centers[0], w.track.centers[-1]
         else:
             s.wind_field = 0
             s.wind_speed = float(wi.wind)
@@ -182,6 +199,15 @@ def attach(model, sources):
     xn = np.ascontiguousarray(xnodes(model.grid), dtype=np.float64)
     yn = np.ascontiguousarray(ynodes(model.grid), dtype=np.float64)
     _lib.check(lib.rpl_set_sources(model._ctx.ptr, ctypes.byref(s), _lib.dptr(xn), _lib.dptr(yn)))
+    wi = sources.wind_input if isinstance(sources, PrecomputedSources) else sources
+    track = getattr(getattr(wi, "wind", None), "track", None)
+    if track is not None and len(track.times) != 2:
+        t = np.ascontiguousarray(track.times, dtype=np.float64)
+        cx = np.ascontiguousarray([p[0] for p in track.centers], dtype=np.float64)
+        cy = np.ascontiguousarray([p[1] for p in track.centers], dtype=np.float64)
+        _lib.check(lib.rpl_set_storm_track(model._ctx.ptr, len(t), _lib.dptr(t), _lib.dptr(cx), _lib.dptr(cy)))
+    else:
+        _lib.check(lib.rpl_set_storm_track(model._ctx.ptr, 0, None, None, None))
     return s
 
 

@@ -58,6 +58,25 @@ def test_holland_wind_columns(oracle):
         assert np.max(np.abs(np.angle(np.exp(1j * (th - tho))))) < 1e-12
 
 
+def test_holland_wind_on_a_multi_point_track(oracle):
+    """LinearStormTrack of four points (winds.jl:4-41) on the device against
+    the oracle's track, at knots, inside segments and past both ends."""
+    say("holland wind, 4-point track")
+    wl = workloads.translating_hurricane(float_type=np.float64, nx=48, ny=40, nz=4, nkappa=6, nphi=8)
+    w = wl.model_kwargs["sources"].wind_input.wind
+    times = [3600.0, 7200.0, 14400.0, 30000.0]
+    centers = [(150e3, -50e3), (250e3, 20e3), (300e3, 120e3), (500e3, 0.0)]
+    w.track = ra.LinearStormTrack(times, centers)
+    m = device(wl, np.float64)
+    src = oracle.set_track(oracle.sources_from(to_struct(wl.model_kwargs["sources"])), times, centers)
+    for t in (0.0, 3600.0, 5000.0, 7200.0, 10000.0, 29999.0, 40000.0):
+        ra._lib.check(m._ctx.lib.rpl_set_clock(m._ctx.ptr, t, 0))
+        U, th, us = ra.wind_state(m)
+        Uo, tho, uso = oracle.wind_columns(src, grids.xnodes(wl.grid), grids.ynodes(wl.grid), t)
+        np.testing.assert_allclose(U, Uo, rtol=1e-13, atol=1e-13)
+        assert np.max(np.abs(np.angle(np.exp(1j * (th - tho))))) < 1e-12
+
+
 @pytest.mark.parametrize("case", ["fetch", "hurricane"])
 def test_source_tendency_and_steps_fp64(oracle, case):
     say(f"sources fp64 {case}")

@@ -167,3 +167,21 @@ def test_whitecapping_known_answers(oracle):
     weak = ra.WhitecappingDissipation(rate=0.2, saturation_threshold=100.0)
     G = oracle.OracleSourceModel(wl, src_struct(oracle, weak), transport=oracle.MODE_NONE).tendency()
     assert np.max(np.abs(G)) < 1e-14
+
+
+def test_storm_track_known_answers(oracle):
+    """test/forcing/winds.jl:16-21: a three-point track, clamped at both ends."""
+    times, centers = [0.0, 1.0, 3.0], [(0.0, 0.0), (2.0, 1.0), (4.0, -1.0)]
+    assert oracle.track_at(times, centers, -1.0) == (0.0, 0.0)
+    assert oracle.track_at(times, centers, 0.5) == (1.0, 0.5)
+    assert oracle.track_at(times, centers, 2.0) == (3.0, 0.0)
+    assert oracle.track_at(times, centers, 4.0) == (4.0, -1.0)
+    tr = ra.LinearStormTrack(times, centers)
+    for t in (-1.0, 0.0, 0.5, 1.0, 2.0, 3.0, 4.0):
+        assert tr(t) == oracle.track_at(times, centers, t)
+    with pytest.raises(ValueError):
+        ra.LinearStormTrack([0.0, 0.0], [(0.0, 0.0), (1.0, 1.0)])
+    with pytest.raises(ValueError):
+        ra.LinearStormTrack([0.0], [(0.0, 0.0), (1.0, 1.0)])
+    with pytest.raises(ValueError):
+        ra.LinearStormTrack([0.0], [(0.0, 0.0, 0.0)])
