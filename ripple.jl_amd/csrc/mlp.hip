@@ -82,7 +82,10 @@ __device__ __forceinline__ void epilogue(const GemmArgs &g, int m, int n, float 
     } else if (EPI == EPI_BIAS_F32) {
         g.Cf[o] = c + g.bias[n];
     } else if (EPI == EPI_BIAS_ROW_F32) {
-        g.Cf[o] = c + g.bias[m];
+        const float v = c + g.bias[m];
+        g.Cf[o] = v;
+        if (g.nonfinite && !(fabsf(v) < 3.0e38f))
+            g.nonfinite[n] = 2;
     } else {  // EPI_GELU_BACKWARD: dZ = dA * gelu'(Z)
         g.Cb[o] = f2bf(c * gelu_grad(g.Zin[o]));
     }
@@ -331,72 +334,93 @@ __global__ __launch_bounds__(256) void to_bf16_kernel(const float *__restrict__ 
         dst[k] = f2bf(src[k]);
 }
 
-// Surrogate features of one block of 64 columns: x = (log(N + floor) - shift)
-// * scale for every bin, then U10x/ref, U10y/ref, depth/ref, zero padding.
-// The action planes are read along i (coalesced) and transposed in LDS.
+// Surrogate features of one 64-column x 64-feature tile per workgroup:
+// x = (log(N + floor) - shift) * scale for every bin, then U10x/ref,
+// U10y/ref, depth/ref, zero padding.  The action planes are read along i
+// (coalesced) and transposed through LDS.  With `flags` set, the tile also
+// folds its part of the envelope: the column's largest action (atomic max of
+// the non-negative bit patterns into amax) and bit 0 of flags where a bin
+// exceeds its training maximum bin_hi (if given) or the overall action_hi.
 template <typename T>
 __global__ __launch_bounds__(256) void features_kernel(const T *__restrict__ N, size_t plane, size_t col0, int nbins,
                                                        int in_pad, const T *__restrict__ U10,
-                                                       const T *__restrict__ theta, FeatureScales fs, bf16 *X)
+                                                       const T *__restrict__ theta, FeatureScales fs, bf16 *X,
+                                                       const float *__restrict__ bin_hi, float action_hi,
+                                                       float *amax, unsigned char *flags)
 {
     __shared__ float tile[64][65];
+    __shared__ float red_max[4][64];
+    __shared__ int red_over[4][64];
     const size_t c0 = col0 + (size_t)blockIdx.x * 64;
+    const int b0 = blockIdx.y * 64;
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-    for (int b0 = 0; b0 < in_pad; b0 += 64) {
-        __syncthreads();
-        for (int bb = ty; bb < 64; bb += 4) {
-            const int bin = b0 + bb;
-            const size_t col = c0 + tx;
-            float v = 0.0f;
-            if (col < plane) {
-                if (bin < nbins) {
-                    const float a = (float)N[col + plane * bin];
-                    v = (logf(fmaxf(a, 0.0f) + fs.floor) - fs.shift) * fs.scale;
-                } else if (bin == nbins) {
-                    v = (float)U10[col] * cosf((float)theta[col]) * fs.wind_scale;
-                } else if (bin == nbins + 1) {
-                    v = (float)U10[col] * sinf((float)theta[col]) * fs.wind_scale;
-                } else if (bin == nbins + 2) {
-                    v = fs.depth_feature;
-                }
+    const size_t col = c0 + tx;
+    float mx = 0.0f;
+    bool over = false;
+    for (int bb = ty; bb < 64; bb += 4) {
+        const int bin = b0 + bb;
+        float v = 0.0f;
+        if (col < plane) {
+            if (bin < nbins) {
+                const float a = (float)N[col + plane * bin];
+                v = (logf(fmaxf(a, 0.0f) + fs.floor) - fs.shift) * fs.scale;
+                mx = fmaxf(mx, a);
+                if (bin_hi)
+                    over = over || !(a <= bin_hi[bin]);
+            } else if (bin == nbins) {
+                v = (float)U10[col] * cosf((float)theta[col]) * fs.wind_scale;
+            } else if (bin == nbins + 1) {
+                v = (float)U10[col] * sinf((float)theta[col]) * fs.wind_scale;
+            } else if (bin == nbins + 2) {
+                v = fs.depth_feature;
             }
-            tile[bb][tx] = v;
         }
-        __syncthreads();
-        for (int cc = ty; cc < 64; cc += 4) {
-            const size_t col = c0 + cc;
-            X[(col - col0) * (size_t)in_pad + b0 + tx] = f2bf(col < plane ? tile[tx][cc] : 0.0f);
-        }
+        tile[bb][tx] = v;
+    }
+    if (flags) {
+        red_max[ty][tx] = mx;
+        red_over[ty][tx] = over ? 1 : 0;
+    }
+    __syncthreads();
+    for (int cc = ty; cc < 64; cc += 4) {
+        const size_t cl = c0 + cc;
+        X[(cl - col0) * (size_t)in_pad + b0 + tx] = f2bf(cl < plane ? tile[tx][cc] : 0.0f);
+    }
+    if (flags && ty == 0 && col < plane) {
+        const float m = fmaxf(fmaxf(red_max[0][tx], red_max[1][tx]), fmaxf(red_max[2][tx], red_max[3][tx]));
+        const bool o = red_over[0][tx] | red_over[1][tx] | red_over[2][tx] | red_over[3][tx];
+        if (m > 0.0f)
+            atomicMax(reinterpret_cast<unsigned *>(amax) + col, __float_as_uint(m));  // m >= 0: bit order = value order
+        if (o || !(m <= action_hi))
+            flags[col] = 1;
+    }
+}
+
+// Clears the per-column envelope accumulators before a stage's features.
+__global__ __launch_bounds__(256) void envelope_clear_kernel(size_t plane, float *amax, unsigned char *flags)
+{
+    const size_t col = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (col < plane) {
+        amax[col] = 0.0f;
+        flags[col] = 0;
     }
 }
 
 // Envelope and audit decision per column: the plain bundle is used where the
-// wind is outside the training range, where the column's action (overall, or
-// in any bin against that bin's training maximum) is outside it, and where
-// the network produced a non-finite rate; audit columns are flagged too.
+// wind is outside the training range, where the features found an action
+// outside it (flags bit 0), and where the network produced a non-finite rate
+// (flags bit 1, set by the output layer's epilogue); every audit_stride-th
+// column is flagged for the audit.
 template <typename T>
-__global__ __launch_bounds__(256) void envelope_kernel(const T *__restrict__ N, size_t plane, int nbins,
-                                                       const T *__restrict__ U10, const float *__restrict__ rate,
-                                                       size_t ld, Envelope env, unsigned char *use_plain,
-                                                       unsigned *counts)
+__global__ __launch_bounds__(256) void envelope_kernel(size_t plane, const T *__restrict__ U10,
+                                                       const unsigned char *__restrict__ flags, Envelope env,
+                                                       unsigned char *use_plain, unsigned *counts)
 {
     const size_t col = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (col >= plane)
         return;
     const float u = (float)U10[col];
-    bool plain = !(u >= env.wind_lo && u <= env.wind_hi);
-    float amax = 0.0f;
-    bool finite = true;
-    bool over = false;
-    for (int b = 0; b < nbins; ++b) {
-        const float a = (float)N[col + plane * b];
-        amax = fmaxf(amax, a);
-        if (env.bin_hi)
-            over = over || !(a <= env.bin_hi[b]);
-        const float r = rate[col + ld * b];
-        finite = finite && (r == r) && fabsf(r) < 3.0e38f;
-    }
-    plain = plain || !(amax <= env.action_hi) || over || !finite;
+    const bool plain = !(u >= env.wind_lo && u <= env.wind_hi) || flags[col] != 0;
     const bool audit = env.audit_stride > 0 && (col % (size_t)env.audit_stride) == 0;
     use_plain[col] = (unsigned char)((plain ? 1 : 0) | (audit ? 2 : 0));
     if (plain)
@@ -529,21 +553,29 @@ hipError_t to_bf16(const float *src, bf16 *dst, size_t n, hipStream_t s)
 
 template <typename T>
 hipError_t features(const T *N, size_t plane, size_t col0, size_t ncols, int nbins, int in_pad, const T *U10,
-                    const T *theta, const FeatureScales &fs, bf16 *X, hipStream_t s)
+                    const T *theta, const FeatureScales &fs, bf16 *X, const float *bin_hi, float action_hi,
+                    float *amax, unsigned char *flags, hipStream_t s)
 {
     if (ncols % 64 || in_pad % 64)
         return hipErrorInvalidValue;
-    hipLaunchKernelGGL(features_kernel<T>, dim3((unsigned)(ncols / 64)), dim3(256), 0, s, N, plane, col0, nbins,
-                       in_pad, U10, theta, fs, X);
+    hipLaunchKernelGGL(features_kernel<T>, dim3((unsigned)(ncols / 64), (unsigned)(in_pad / 64)), dim3(256), 0, s, N,
+                       plane, col0, nbins, in_pad, U10, theta, fs, X, bin_hi, action_hi, amax, flags);
+    return hipGetLastError();
+}
+
+hipError_t envelope_clear(size_t plane, float *amax, unsigned char *flags, hipStream_t s)
+{
+    hipLaunchKernelGGL(envelope_clear_kernel, dim3((unsigned)((plane + 255) / 256)), dim3(256), 0, s, plane, amax,
+                       flags);
     return hipGetLastError();
 }
 
 template <typename T>
-hipError_t envelope(const T *N, size_t plane, int nbins, const T *U10, const float *rate, size_t ld,
-                    const Envelope &env, unsigned char *use_plain, unsigned *counts, hipStream_t s)
+hipError_t envelope(size_t plane, const T *U10, const unsigned char *flags, const Envelope &env,
+                    unsigned char *use_plain, unsigned *counts, hipStream_t s)
 {
-    hipLaunchKernelGGL(envelope_kernel<T>, dim3((unsigned)((plane + 255) / 256)), dim3(256), 0, s, N, plane, nbins, U10,
-                       rate, ld, env, use_plain, counts);
+    hipLaunchKernelGGL(envelope_kernel<T>, dim3((unsigned)((plane + 255) / 256)), dim3(256), 0, s, plane, U10, flags,
+                       env, use_plain, counts);
     return hipGetLastError();
 }
 
@@ -560,13 +592,15 @@ hipError_t audit(const T *N, size_t plane, int nbins, const float *rate, size_t 
 }
 
 template hipError_t features<float>(const float *, size_t, size_t, size_t, int, int, const float *, const float *,
-                                    const FeatureScales &, bf16 *, hipStream_t);
+                                    const FeatureScales &, bf16 *, const float *, float, float *, unsigned char *,
+                                    hipStream_t);
 template hipError_t features<double>(const double *, size_t, size_t, size_t, int, int, const double *,
-                                     const double *, const FeatureScales &, bf16 *, hipStream_t);
-template hipError_t envelope<float>(const float *, size_t, int, const float *, const float *, size_t,
-                                    const Envelope &, unsigned char *, unsigned *, hipStream_t);
-template hipError_t envelope<double>(const double *, size_t, int, const double *, const float *, size_t,
-                                     const Envelope &, unsigned char *, unsigned *, hipStream_t);
+                                     const double *, const FeatureScales &, bf16 *, const float *, float, float *,
+                                     unsigned char *, hipStream_t);
+template hipError_t envelope<float>(size_t, const float *, const unsigned char *, const Envelope &, unsigned char *,
+                                    unsigned *, hipStream_t);
+template hipError_t envelope<double>(size_t, const double *, const unsigned char *, const Envelope &,
+                                     unsigned char *, unsigned *, hipStream_t);
 template hipError_t audit<float>(const float *, size_t, int, const float *, size_t, const float *,
                                  unsigned char *, int, const SourceParams<float> &, float, unsigned *, float *,
                                  hipStream_t);

@@ -206,7 +206,7 @@ namespace rpl {
 namespace mlp {
 
 hipError_t forward_rows(rpl_mlp *n, const bf16 *X, int rows, bf16 *act_a, bf16 *act_b, float *out, size_t ld,
-                        hipStream_t s)
+                        unsigned char *nonfinite, hipStream_t s)
 {
     const int L = nlayers(n);
     const bf16 *in = X;
@@ -236,6 +236,7 @@ hipError_t forward_rows(rpl_mlp *n, const bf16 *X, int rows, bf16 *act_a, bf16 *
     g.scm = ld;
     g.scn = 1;
     g.bias = n->b[L - 1];
+    g.nonfinite = nonfinite;
     return gemm(g, EPI_BIAS_ROW_F32, s);
 }
 

@@ -32,6 +32,7 @@ struct GemmArgs {
     size_t scm, scn;   // output strides
     const float *bias; // per n (per m for EPI_BIAS_ROW_F32)
     const float *Zin;  // pre-activation read by the GELU backward epilogue (strides scm, scn)
+    unsigned char *nonfinite;  // EPI_BIAS_ROW_F32: set to 2 at column n where an output is not finite (or null)
 };
 
 struct FeatureScales {
@@ -43,7 +44,6 @@ struct FeatureScales {
 struct Envelope {
     float wind_lo, wind_hi;  // training wind range
     float action_hi;         // largest action seen in training
-    const float *bin_hi;     // per-bin largest action seen in training (nbins), or null
     int audit_stride;        // every audit_stride-th column is audited (0: none)
 };
 
@@ -55,14 +55,20 @@ hipError_t adam(float *w, const float *g, float *m, float *v, bf16 *wb, size_t n
                 float eps, int step, hipStream_t s);
 hipError_t to_bf16(const float *src, bf16 *dst, size_t n, hipStream_t s);
 // Features of columns [col0, col0 + ncols) (ncols a multiple of 64) into X
-// rows 0..ncols-1; columns at or past `plane` get zero features.
+// rows 0..ncols-1; columns at or past `plane` get zero features.  With amax
+// and flags (both cleared by envelope_clear first), also the column's largest
+// action and flag bit 0 where an action leaves the envelope (bin_hi per bin,
+// may be null; action_hi overall).
 template <typename T>
 hipError_t features(const T *N, size_t plane, size_t col0, size_t ncols, int nbins, int in_pad, const T *U10,
-                    const T *theta, const FeatureScales &fs, bf16 *X, hipStream_t s);
-// rate: network output, bin-major with leading dimension ld (rate[col + ld * bin]).
+                    const T *theta, const FeatureScales &fs, bf16 *X, const float *bin_hi, float action_hi,
+                    float *amax, unsigned char *flags, hipStream_t s);
+hipError_t envelope_clear(size_t plane, float *amax, unsigned char *flags, hipStream_t s);
+// use_plain per column from the wind range and the flags (bit 0: action
+// outside the envelope; bit 1: non-finite network output), plus the audit grid.
 template <typename T>
-hipError_t envelope(const T *N, size_t plane, int nbins, const T *U10, const float *rate, size_t ld,
-                    const Envelope &env, unsigned char *use_plain, unsigned *counts, hipStream_t s);
+hipError_t envelope(size_t plane, const T *U10, const unsigned char *flags, const Envelope &env,
+                    unsigned char *use_plain, unsigned *counts, hipStream_t s);
 template <typename T>
 hipError_t audit(const T *N, size_t plane, int nbins, const float *rate, size_t ld, const float *out_scale,
                  unsigned char *use_plain, int stride, const SourceParams<T> &sp, float bound, unsigned *counts,
@@ -98,6 +104,6 @@ namespace mlp {
 // layers with the bf16 inference epilogue into act_a / act_b, output layer
 // computed transposed (W X^T) so that out[row + ld * o] is stored along rows.
 hipError_t forward_rows(rpl_mlp *net, const bf16 *X, int rows, bf16 *act_a, bf16 *act_b, float *out, size_t ld,
-                        hipStream_t s);
+                        unsigned char *nonfinite, hipStream_t s);
 }  // namespace mlp
 }  // namespace rpl

@@ -73,7 +73,8 @@ struct rpl_ctx {
     rpl_surrogate scfg{};
     bool s_active = false;  // the stage takes the network's source where allowed
     rpl::mlp::bf16 *sX = nullptr, *sA = nullptr, *sB = nullptr;
-    float *rate = nullptr, *out_scale = nullptr, *s_err = nullptr, *bin_hi = nullptr;
+    float *rate = nullptr, *out_scale = nullptr, *s_err = nullptr, *bin_hi = nullptr, *s_amax = nullptr;
+    unsigned char *s_flags = nullptr;  // per column: envelope flags of the current stage
     unsigned char *use_plain = nullptr;
     unsigned *s_counts = nullptr;
     size_t s_ld = 0, s_chunk = 0;
@@ -294,18 +295,20 @@ int surrogate_rates(rpl_ctx *c, const T *Nk)
     const int nbins = c->cfg.nkappa * c->cfg.nphi;
     const rpl_surrogate &g = c->scfg;
     rpl::mlp::FeatureScales fs{g.floor, g.shift, g.scale, g.wind_scale, g.depth_feature};
-    hipError_t e = hipSuccess;
+    hipError_t e = rpl::mlp::envelope_clear(c->plane, c->s_amax, c->s_flags, c->stream);
     for (size_t col0 = 0; col0 < c->plane && e == hipSuccess; col0 += c->s_chunk) {
         const size_t rows = std::min(c->s_chunk, (c->plane - col0 + 63) / 64 * 64);
         e = rpl::mlp::features<T>(Nk, c->plane, col0, rows, nbins, c->net->dims[0], (const T *)c->U10,
-                                  (const T *)c->theta_u, fs, c->sX, c->stream);
+                                  (const T *)c->theta_u, fs, c->sX, c->bin_hi, g.action_hi, c->s_amax, c->s_flags,
+                                  c->stream);
         if (e == hipSuccess)
-            e = rpl::mlp::forward_rows(c->net, c->sX, (int)rows, c->sA, c->sB, c->rate + col0, c->s_ld, c->stream);
+            e = rpl::mlp::forward_rows(c->net, c->sX, (int)rows, c->sA, c->sB, c->rate + col0, c->s_ld,
+                                       c->s_flags + col0, c->stream);
     }
     if (e == hipSuccess) {
-        rpl::mlp::Envelope env{g.wind_lo, g.wind_hi, g.action_hi, c->bin_hi, g.audit_stride};
-        e = rpl::mlp::envelope<T>(Nk, c->plane, nbins, (const T *)c->U10, c->rate, c->s_ld, env, c->use_plain,
-                                  c->s_counts, c->stream);
+        rpl::mlp::Envelope env{g.wind_lo, g.wind_hi, g.action_hi, g.audit_stride};
+        e = rpl::mlp::envelope<T>(c->plane, (const T *)c->U10, c->s_flags, env, c->use_plain, c->s_counts,
+                                  c->stream);
     }
     if (e != hipSuccess)
         return fail(RPL_ERR_HIP, std::string("surrogate inference: ") + hipGetErrorString(e));
@@ -441,10 +444,12 @@ namespace {
 void free_surrogate(rpl_ctx *c)
 {
     for (void *p : {(void *)c->sX, (void *)c->sA, (void *)c->sB, (void *)c->rate, (void *)c->out_scale,
-                    (void *)c->s_err, (void *)c->use_plain, (void *)c->s_counts, (void *)c->bin_hi})
+                    (void *)c->s_err, (void *)c->use_plain, (void *)c->s_counts, (void *)c->bin_hi,
+                    (void *)c->s_amax, (void *)c->s_flags})
         free_ptr(p);
     c->sX = c->sA = c->sB = nullptr;
-    c->rate = c->out_scale = c->s_err = c->bin_hi = nullptr;
+    c->rate = c->out_scale = c->s_err = c->bin_hi = c->s_amax = nullptr;
+    c->s_flags = nullptr;
     c->use_plain = nullptr;
     c->s_counts = nullptr;
     c->net = nullptr;
@@ -472,7 +477,7 @@ int training_pairs(rpl_ctx *c, const rpl_surrogate *cfg, float *features, float 
         for (size_t col0 = 0; col0 < c->plane && e == hipSuccess; col0 += chunk) {
             const size_t rows = std::min(chunk, (c->plane - col0 + 63) / 64 * 64);
             e = rpl::mlp::features<T>((const T *)c->A, c->plane, col0, rows, nbins, in_pad, (const T *)c->U10,
-                                      (const T *)c->theta_u, fs, X, c->stream);
+                                      (const T *)c->theta_u, fs, X, nullptr, 0.0f, nullptr, nullptr, c->stream);
             if (e == hipSuccess)
                 e = hipStreamSynchronize(c->stream);
             if (e == hipSuccess)
@@ -777,7 +782,7 @@ int rpl_destroy(rpl_ctx *c)
                     c->theta_cell, c->sigma_cell, c->C_cell, c->t_phi, c->t_w, c->t_sigma_k, c->t_cos, c->t_sin,
                     c->t_kpow, c->t_spow, c->t_wc, (void *)c->sX, (void *)c->sA, (void *)c->sB, (void *)c->rate,
                     (void *)c->out_scale, (void *)c->s_err, (void *)c->use_plain, (void *)c->s_counts,
-                    (void *)c->bin_hi})
+                    (void *)c->bin_hi, (void *)c->s_amax, (void *)c->s_flags})
         free_ptr(p);
     for (auto &ev : c->events) {
         (void)hipEventDestroy(ev.first);
@@ -1456,7 +1461,8 @@ int rpl_attach_surrogate(rpl_ctx *c, rpl_mlp *net, const rpl_surrogate *cfg, con
         (rc = alloc((void **)&c->sB, c->s_chunk * hid * 2)) ||
         (rc = alloc((void **)&c->rate, c->s_ld * out_pad * sizeof(float))) ||
         (rc = alloc((void **)&c->out_scale, out_pad * sizeof(float))) || (rc = alloc((void **)&c->s_err, 16)) ||
-        (rc = alloc((void **)&c->use_plain, c->s_ld)) || (rc = alloc((void **)&c->s_counts, 16))) {
+        (rc = alloc((void **)&c->use_plain, c->s_ld)) || (rc = alloc((void **)&c->s_counts, 16)) ||
+        (rc = alloc((void **)&c->s_amax, c->s_ld * sizeof(float))) || (rc = alloc((void **)&c->s_flags, c->s_ld))) {
         free_surrogate(c);
         return rc;
     }
