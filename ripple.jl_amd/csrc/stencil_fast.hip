@@ -138,7 +138,7 @@ __device__ __forceinline__ int wrap(int i, int n) { return i < 0 ? i + n : (i >=
 __device__ __forceinline__ int clampi(int i, int n) { return i < 0 ? 0 : (i >= n ? n - 1 : i); }
 __device__ __forceinline__ float clip0(float x) { return x < 0.0f ? 0.0f : x; }
 
-// Work order.  One 64 x 4 (i, j) tile of one (kappa, phi) bin per workgroup.
+// Work order.  One 64 x STAGE_ROWS (i, j) tile of one (kappa, phi) bin per workgroup.
 // Workgroups are dealt round-robin to the eight XCDs, so the linear block id
 // is remapped to give every XCD one contiguous range of the logical order
 // phi (fastest), kappa, tile row, tile column.  The kappa and phi neighbours
@@ -297,12 +297,21 @@ __device__ __forceinline__ void stage_tile(const float *__restrict__ Nk_, const 
     sto<AS1>(dst, bo, r);
 }
 
+// Rows per workgroup tile of the per-model kernel (64 x STAGE_ROWS cells of
+// one bin): taller tiles re-read fewer y-halo rows (a tile reads STAGE_ROWS + 6
+// rows of its bin).
+#ifndef RPL_STAGE_ROWS
+#define RPL_STAGE_ROWS 4
+#endif
+constexpr int STAGE_ROWS = RPL_STAGE_ROWS;
+
 template <int TRANSPORT, int STAGE, bool SRC>
-__global__ __launch_bounds__(256) void stage_fast_kernel(const float *__restrict__ Nk, const float *__restrict__ N0,
-                                                         float *out, StageParams<float> p, int nbx, int nby)
+__global__ __launch_bounds__(64 * STAGE_ROWS) void stage_fast_kernel(const float *__restrict__ Nk,
+                                                                     const float *__restrict__ N0, float *out,
+                                                                     StageParams<float> p, int nbx, int nby)
 {
     const Tile t = tile_of(blockIdx.x, nbx * nby * p.nk * p.nphi, nbx, nby, p.nk, p.nphi);
-    stage_tile<TRANSPORT, STAGE, SRC, 4, false>(Nk, N0, out, p, t.ib, t.jb, t.m, t.n);
+    stage_tile<TRANSPORT, STAGE, SRC, STAGE_ROWS, false>(Nk, N0, out, p, t.ib, t.jb, t.m, t.n);
 }
 
 // Ensemble work list: every member's tiles (64 x TY cells of one bin) are
@@ -356,11 +365,11 @@ template <int TRANSPORT, bool SRC>
 hipError_t launch_t(const StageParams<float> &p, int stage, const float *Nk, const float *N0, float *out,
                     hipStream_t s)
 {
-    const int nbx = (p.nx + 63) / 64, nby = (p.ny + 3) / 4;
+    const int nbx = (p.nx + 63) / 64, nby = (p.ny + STAGE_ROWS - 1) / STAGE_ROWS;
     const long long total = (long long)nbx * nby * p.nk * p.nphi;
     if (total > 0x7fffffffLL)
         return hipErrorInvalidValue;
-    dim3 block(64, 4, 1);
+    dim3 block(64, STAGE_ROWS, 1);
     dim3 grid((unsigned)total, 1, 1);
     switch (stage) {
     case STAGE_TENDENCY:

@@ -69,3 +69,17 @@ def test_workloads_are_deterministic():
     wl = workloads.vortex_refraction()
     assert wl.N0.shape == (64, 64, 4, 16) and wl.cell_bins == 262144
     assert np.max(wl.N0) <= 1.0 and np.min(wl.N0) > 0
+
+
+def test_rectilinear_grid_accessors_known_answers():
+    """test/product_fields/indexing.jl:81-98: a stretched grid's faces, nodes,
+    spacings and topology."""
+    g = grids.RectilinearGrid(size=(3, 2, 1), x=[0, 0.25, 1.0, 2.0], y=[-1, 0, 3], z=(0, 1),
+                              topology=(grids.BOUNDED, grids.PERIODIC, grids.BOUNDED))
+    assert (g.Nx, g.Ny, g.Nz) == (3, 2, 1)
+    assert list(grids.xfaces(g)) == [0.0, 0.25, 1.0, 2.0]
+    assert list(grids.yfaces(g)) == [-1.0, 0.0, 3.0]
+    assert list(grids.zfaces(g)) == [0.0, 1.0]
+    assert list(grids.xnodes(g)) == [0.125, 0.625, 1.5]
+    assert list(grids.xspacings(g)) == [0.25, 0.75, 1.0]
+    assert tuple(g.topology) == (grids.BOUNDED, grids.PERIODIC, grids.BOUNDED)
