@@ -227,6 +227,9 @@ typedef struct rpl_surrogate {
     float wind_lo, wind_hi, action_hi;
     int32_t audit_stride;
     float audit_bound;
+    /* 0: the network supplies the whole source; 1: the wind input only (with
+     * its stress cap), the dissipation staying the plain bundle's. */
+    int32_t terms;
 } rpl_surrogate;
 /* bin_action_hi (nkappa*nphi values, may be NULL): the per-bin largest
  * action of the training set; a column with any bin above it is outside the
@@ -239,8 +242,8 @@ int rpl_surrogate_stats(rpl_ctx *ctx, int64_t *columns, int64_t *plain, int64_t 
 /* The surrogate features of the current action and clock, (nx*ny) rows of
  * nbins + 3 FP32 values (bf16-representable: what the network sees), and the
  * plain bundle's per-cell rates r = source / (N sigma_bin rate_unit) as
- * network targets (for N <= 1e-30, the rate at N = 1e-30), (nx*ny) rows of
- * nbins.  Either output
+ * network targets (for N <= 1e-30, the rate at N = 1e-30; with cfg->terms 1,
+ * of the wind input alone), (nx*ny) rows of nbins.  Either output
  * may be NULL.  Column order is i fastest; nbins = nkappa * nphi, m fastest. */
 int rpl_surrogate_training_pairs(rpl_ctx *ctx, const rpl_surrogate *cfg, float *features, float *targets);
 

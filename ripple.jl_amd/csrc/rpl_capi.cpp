@@ -196,6 +196,7 @@ rpl::StageParams<T> stage_params(const rpl_ctx *c, double dt)
         q.wc_factor = (const T *)c->t_wc;
         if (c->s_active) {
             q.surrogate = 1;
+            q.surrogate_terms = c->scfg.terms;
             q.rate = c->rate;
             q.out_scale = c->out_scale;
             q.rate_ld = c->s_ld;
@@ -321,6 +322,8 @@ int surrogate_audit(rpl_ctx *c, const T *Nk)
 {
     auto p = stage_params<T>(c, 0.0);
     p.src.surrogate = 0;  // the audit compares against the plain bundle
+    if (c->scfg.terms == 1)
+        p.src.dissipation = RPL_DISSIPATION_NONE;  // ... the part the network stands in for
     hipError_t e = rpl::mlp::audit<T>(Nk, c->plane, c->cfg.nkappa * c->cfg.nphi, c->rate, c->s_ld, c->out_scale,
                                       c->use_plain, c->scfg.audit_stride, p.src, c->scfg.audit_bound,
                                       c->s_counts, c->s_err, c->stream);
@@ -498,6 +501,8 @@ int training_pairs(rpl_ctx *c, const rpl_surrogate *cfg, float *features, float 
             u[t] = (float)(c->h_sigma[t] * cfg->rate_unit);
         e = hipMemcpy(unit, u.data(), nbins * 4, hipMemcpyHostToDevice);
         auto p = stage_params<T>(c, 0.0);
+        if (cfg->terms == 1)
+            p.src.dissipation = RPL_DISSIPATION_NONE;  // targets of the wind input alone
         if (e == hipSuccess)
             e = rpl::launch_source_rates<T>(c->cfg.nx, c->cfg.ny, nbins, (const T *)c->A, p.src, unit, r, c->stream);
         if (e == hipSuccess)
@@ -1452,6 +1457,8 @@ int rpl_attach_surrogate(rpl_ctx *c, rpl_mlp *net, const rpl_surrogate *cfg, con
         return fail(RPL_ERR_ARG, "network and context live on different devices");
     if (!(cfg->rate_unit > 0) || cfg->audit_stride < 0 || !(cfg->floor > 0))
         return fail(RPL_ERR_ARG, "need rate_unit > 0, floor > 0, audit_stride >= 0");
+    if (cfg->terms != 0 && cfg->terms != 1)
+        return fail(RPL_ERR_ARG, "terms must be 0 (all sources) or 1 (wind input)");
     free_surrogate(c);
     const int in_pad = net->dims[0], out_pad = net->dims.back(), hid = net->hidden;
     c->s_ld = (c->plane + 63) / 64 * 64;

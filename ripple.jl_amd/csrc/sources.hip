@@ -82,9 +82,12 @@ __global__ __launch_bounds__(256) void source_state_kernel(int nx, int ny, int n
     const size_t ij = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (ij >= plane)
         return;
-    if (s.surrogate && !(s.use_plain[ij] & 3))
-        return;  // the network supplies this column's source
-    if (s.wind_input && s.stress_cap) {
+    // Where the network supplies the column's source, nothing (all terms) or
+    // only the dissipation state (wind input from the network) is needed.
+    const bool net = s.surrogate && !(s.use_plain[ij] & 3);
+    if (net && s.surrogate_terms == 0)
+        return;
+    if (s.wind_input && s.stress_cap && !net) {
         const T U10 = s.U10[ij], ustar = s.ustar[ij];
         T f = T(1);
         if (U10 > T(0) && ustar > T(0)) {
@@ -164,9 +167,11 @@ __device__ __forceinline__ void state_block_f32(int nx, int ny, int nk, int nphi
     const int tx = threadIdx.x & 63, g = threadIdx.x >> 6;
     const size_t ij = col0 + tx;
     bool active = ij < plane;
-    if (active && s.surrogate && !(gld<true>(s.use_plain, ij) & 3))
-        active = false;  // the network supplies this column's source
-    const bool cap = s.wind_input && s.stress_cap, ms = s.dissipation == 1, wc = s.dissipation == 3;
+    const bool net = active && s.surrogate && !(gld<true>(s.use_plain, ij) & 3);
+    if (net && s.surrogate_terms == 0)
+        active = false;  // the network supplies this column's whole source
+    // with the network supplying the wind input, only the dissipation state is needed
+    const bool cap = s.wind_input && s.stress_cap && !net, ms = s.dissipation == 1, wc = s.dissipation == 3;
     float twx = 0.0f, twy = 0.0f, m0 = 0.0f, kp = 0.0f, sp = 0.0f;
     float U10 = 0.0f, ustar = 0.0f, theta_u = 0.0f;
     if (active && cap) {

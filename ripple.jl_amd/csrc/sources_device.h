@@ -38,8 +38,10 @@ struct SourceParams {
     const T *wc_factor;          // whitecapping: <kappa^p>_cell / kappa_ref^p per bin
     // learned surrogate: where surrogate != 0 and use_plain[col] has neither
     // bit 0 (outside the training envelope) nor bit 2 (failed audit), the
-    // source is rate[col + rate_ld * t] * out_scale[t] * N.
-    int surrogate;
+    // network term is rate[col + rate_ld * t] * out_scale[t] * N; it is the
+    // whole source (surrogate_terms 0) or the wind input only, the
+    // dissipation staying the plain bundle's (surrogate_terms 1).
+    int surrogate, surrogate_terms;
     const float *rate, *out_scale;
     size_t rate_ld;
     const unsigned char *use_plain;
@@ -104,11 +106,16 @@ template <typename T, bool AS1 = false>
 __device__ __forceinline__ T source_term(const SourceParams<T> &s, unsigned col, int t, T a)
 {
 #pragma clang fp contract(off)
-    if (s.surrogate && !(gld<AS1>(s.use_plain, col) & 5))
-        return (T)(gld<AS1>(s.rate, col + s.rate_ld * (size_t)t) * gld<AS1>(s.out_scale, t)) * a;
+    const bool net = s.surrogate && !(gld<AS1>(s.use_plain, col) & 5);
+    T learned = T(0);
+    if (net) {
+        learned = (T)(gld<AS1>(s.rate, col + s.rate_ld * (size_t)t) * gld<AS1>(s.out_scale, t)) * a;
+        if (s.surrogate_terms == 0)
+            return learned;
+    }
     T positive = T(0), damping = T(0);
     const T k = gld<AS1>(s.k_cell, t);
-    if (s.wind_input) {
+    if (s.wind_input && !net) {
         const T U10 = gld<AS1>(s.U10, col), ustar = gld<AS1>(s.ustar, col);
         if (U10 > T(0) && ustar > T(0) && k > T(0)) {
             const T raw = jp_rate<T>(s, k, gld<AS1>(s.sigma_cell, t), gld<AS1>(s.C_cell, t), gld<AS1>(s.theta_cell, t), gld<AS1>(s.theta_u, col), ustar);
@@ -159,6 +166,8 @@ __device__ __forceinline__ T source_term(const SourceParams<T> &s, unsigned col,
         else
             positive += -rate * a;
     }
+    if (net)
+        return (positive - damping * a) + learned;
     return positive - damping * a;
 }
 

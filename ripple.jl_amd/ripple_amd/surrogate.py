@@ -118,6 +118,7 @@ class SurrogateConfig:
     action_hi: float = 3.0e38     # envelope: every bin's action <= action_hi
     audit_stride: int = 0         # every audit_stride-th column is audited (0: none)
     audit_bound: float = 0.1      # relative L2 source error above which an audit column falls back
+    terms: int = 0                # 0: the network supplies every source term; 1: the wind input only
 
     def to_struct(self):
         s = _lib.rpl_surrogate()
