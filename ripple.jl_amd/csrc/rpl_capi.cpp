@@ -534,6 +534,8 @@ struct rpl_ensemble {
     rpl::EnsembleMember *desc = nullptr;  // 3 stages x members, device
     long long total = 0;
     rpl::EnsembleState *sdesc = nullptr;  // 3 stages x members with a column state, device
+    int *chunk_member = nullptr;          // member of every 2^ENSEMBLE_CHUNK_SHIFT-th tile, device
+    size_t chunk_cap = 0;
     int smembers = 0;
     long long sblocks = 0;
     double desc_dt = -1;
@@ -600,6 +602,23 @@ int ensemble_descriptors(rpl_ensemble *e, double dt)
     }
     e->total = t0;
     RPL_HIP(hipMemcpy(e->desc, h.data(), h.size() * sizeof(rpl::EnsembleMember), hipMemcpyHostToDevice));
+    const size_t nchunks = (size_t)((t0 + (1LL << rpl::ENSEMBLE_CHUNK_SHIFT) - 1) >> rpl::ENSEMBLE_CHUNK_SHIFT);
+    std::vector<int> cm(nchunks);
+    for (size_t q = 0, k = 0; q < nchunks; ++q) {
+        const long long tile = (long long)q << rpl::ENSEMBLE_CHUNK_SHIFT;
+        while (k + 1 < (size_t)M && h[k + 1].tile0 <= tile)
+            ++k;
+        cm[q] = (int)k;
+    }
+    if (nchunks > e->chunk_cap) {
+        if (e->chunk_member)
+            (void)hipFree(e->chunk_member);
+        e->chunk_member = nullptr;
+        e->chunk_cap = 0;
+        RPL_HIP(hipMalloc((void **)&e->chunk_member, nchunks * sizeof(int)));
+        e->chunk_cap = nchunks;
+    }
+    RPL_HIP(hipMemcpy(e->chunk_member, cm.data(), nchunks * sizeof(int), hipMemcpyHostToDevice));
     // Column states (stress cap, mean-spectrum moments) of the members that need one.
     std::vector<rpl_ctx *> sm;
     for (rpl_ctx *c : e->members)
@@ -677,8 +696,8 @@ int ensemble_stages(rpl_ensemble *e)
                     return rc;
             }
         }
-        hipError_t err = rpl::launch_ensemble_stage(e->desc + (size_t)(st - 1) * M, M, e->total, e->transport, e->src,
-                                                    st, e->ty, launch_blocks(e), e->stream);
+        hipError_t err = rpl::launch_ensemble_stage(e->desc + (size_t)(st - 1) * M, M, e->total, e->chunk_member,
+                                                    e->transport, e->src, st, e->ty, launch_blocks(e), e->stream);
         if (err != hipSuccess)
             return fail(RPL_ERR_HIP, std::string("ensemble stage launch: ") + hipGetErrorString(err));
     }
@@ -1617,6 +1636,8 @@ int rpl_ensemble_destroy(rpl_ensemble *e)
         (void)hipFree(e->desc);
     if (e->sdesc)
         (void)hipFree(e->sdesc);
+    if (e->chunk_member)
+        (void)hipFree(e->chunk_member);
     (void)hipStreamDestroy(e->stream);
     delete e;
     return RPL_OK;
@@ -1762,13 +1783,13 @@ int rpl_ensemble_autotune(rpl_ensemble *e, double dt, int32_t reps, int32_t *til
         for (int mu : mult) {
             long long g = mu ? (long long)e->cus * mu : (e->total + 7) / 8 * 8;
             g = std::max<long long>(8, std::min<long long>(g, 0x7ffffff8LL) / 8 * 8);
-            err = rpl::launch_ensemble_stage(e->desc, M, e->total, e->transport, e->src, rpl::STAGE_1, ty, (int)g,
-                                             e->stream);  // warm-up
+            err = rpl::launch_ensemble_stage(e->desc, M, e->total, e->chunk_member, e->transport, e->src,
+                                             rpl::STAGE_1, ty, (int)g, e->stream);  // warm-up
             if (err == hipSuccess)
                 err = hipEventRecord(a, e->stream);
             for (int r = 0; r < reps && err == hipSuccess; ++r)
-                err = rpl::launch_ensemble_stage(e->desc, M, e->total, e->transport, e->src, rpl::STAGE_1, ty, (int)g,
-                                                 e->stream);
+                err = rpl::launch_ensemble_stage(e->desc, M, e->total, e->chunk_member, e->transport, e->src,
+                                                 rpl::STAGE_1, ty, (int)g, e->stream);
             if (err == hipSuccess)
                 err = hipEventRecord(b, e->stream);
             if (err == hipSuccess)

@@ -324,10 +324,12 @@ __global__ __launch_bounds__(64 * STAGE_ROWS) void stage_fast_kernel(const float
 // XCD work on neighbouring bins of the same (x, y) tile at the same time and
 // share the stencil's phi and kappa neighbours in that XCD's L2.  The member
 // is looked up (binary search over tile0) only when a workgroup's next tile
-// lies past the member it last worked on.
+// lies past the member it last worked on, starting from a table that gives
+// the member of every 2^ENSEMBLE_CHUNK_SHIFT-th tile.
 template <int TRANSPORT, int STAGE, bool SRC, int TY>
 __global__ __launch_bounds__(64 * TY) void ensemble_stage_kernel(const EnsembleMember *__restrict__ mem, int members,
-                                                                 long long total)
+                                                                 long long total,
+                                                                 const int *__restrict__ chunk_member)
 {
     const int xcd = blockIdx.x & 7;
     const long long stride = (long long)(gridDim.x >> 3);
@@ -336,14 +338,10 @@ __global__ __launch_bounds__(64 * TY) void ensemble_stage_kernel(const EnsembleM
     long long a_begin = 0, a_end = 0;
     for (long long L = lo + (blockIdx.x >> 3); L < hi; L += stride) {
         if (L >= a_end) {
-            int u = a, v = members - 1;  // last member with tile0 <= L (members are in tile order)
-            while (u < v) {
-                const int c = (u + v + 1) >> 1;
-                if (mem[c].tile0 <= L)
-                    u = c;
-                else
-                    v = c - 1;
-            }
+            // the member holding the first tile of L's chunk, then forward to L's member
+            int u = chunk_member[L >> ENSEMBLE_CHUNK_SHIFT];
+            while (u + 1 < members && mem[u + 1].tile0 <= L)
+                ++u;
             a = __builtin_amdgcn_readfirstlane(u);
             a_begin = mem[a].tile0;
             a_end = a + 1 < members ? mem[a + 1].tile0 : total;
@@ -389,53 +387,55 @@ hipError_t launch_t(const StageParams<float> &p, int stage, const float *Nk, con
 }
 
 template <int TRANSPORT, int STAGE, bool SRC>
-hipError_t launch_ens_ty(const EnsembleMember *mem, int members, long long total, int ty, int blocks, hipStream_t s)
+hipError_t launch_ens_ty(const EnsembleMember *mem, int members, long long total, const int *chunk_member, int ty,
+                         int blocks, hipStream_t s)
 {
     switch (ty) {
     case 2:
         hipLaunchKernelGGL((ensemble_stage_kernel<TRANSPORT, STAGE, SRC, 2>), dim3(blocks), dim3(64, 2), 0, s, mem,
-                           members, total);
+                           members, total, chunk_member);
         break;
     case 8:
         hipLaunchKernelGGL((ensemble_stage_kernel<TRANSPORT, STAGE, SRC, 8>), dim3(blocks), dim3(64, 8), 0, s, mem,
-                           members, total);
+                           members, total, chunk_member);
         break;
     default:
         hipLaunchKernelGGL((ensemble_stage_kernel<TRANSPORT, STAGE, SRC, 4>), dim3(blocks), dim3(64, 4), 0, s, mem,
-                           members, total);
+                           members, total, chunk_member);
         break;
     }
     return hipGetLastError();
 }
 
 template <int TRANSPORT, bool SRC>
-hipError_t launch_ens_stage(const EnsembleMember *mem, int members, long long total, int stage, int ty, int blocks,
-                            hipStream_t s)
+hipError_t launch_ens_stage(const EnsembleMember *mem, int members, long long total, const int *chunk_member,
+                            int stage, int ty, int blocks, hipStream_t s)
 {
     switch (stage) {
     case STAGE_1:
-        return launch_ens_ty<TRANSPORT, STAGE_1, SRC>(mem, members, total, ty, blocks, s);
+        return launch_ens_ty<TRANSPORT, STAGE_1, SRC>(mem, members, total, chunk_member, ty, blocks, s);
     case STAGE_2:
-        return launch_ens_ty<TRANSPORT, STAGE_2, SRC>(mem, members, total, ty, blocks, s);
+        return launch_ens_ty<TRANSPORT, STAGE_2, SRC>(mem, members, total, chunk_member, ty, blocks, s);
     default:
-        return launch_ens_ty<TRANSPORT, STAGE_3, SRC>(mem, members, total, ty, blocks, s);
+        return launch_ens_ty<TRANSPORT, STAGE_3, SRC>(mem, members, total, chunk_member, ty, blocks, s);
     }
 }
 
 }  // namespace
 
-hipError_t launch_ensemble_stage(const EnsembleMember *mem, int members, long long total, int transport, bool src,
-                                 int stage, int ty, int blocks, hipStream_t s)
+hipError_t launch_ensemble_stage(const EnsembleMember *mem, int members, long long total, const int *chunk_member,
+                                 int transport, bool src, int stage, int ty, int blocks, hipStream_t s)
 {
-    if (members < 1 || total < 1 || blocks < 8 || blocks % 8 || (ty != 2 && ty != 4 && ty != 8) ||
+    if (members < 1 || total < 1 || !chunk_member || blocks < 8 || blocks % 8 || (ty != 2 && ty != 4 && ty != 8) ||
         stage < STAGE_1 || stage > STAGE_3)
         return hipErrorInvalidValue;
+    const int *cm = chunk_member;
     if (transport == RPL_TRANSPORT_REFRACTION)
-        return src ? launch_ens_stage<RPL_TRANSPORT_REFRACTION, true>(mem, members, total, stage, ty, blocks, s)
-                   : launch_ens_stage<RPL_TRANSPORT_REFRACTION, false>(mem, members, total, stage, ty, blocks, s);
+        return src ? launch_ens_stage<RPL_TRANSPORT_REFRACTION, true>(mem, members, total, cm, stage, ty, blocks, s)
+                   : launch_ens_stage<RPL_TRANSPORT_REFRACTION, false>(mem, members, total, cm, stage, ty, blocks, s);
     if (transport == RPL_TRANSPORT_INTRINSIC)
-        return src ? launch_ens_stage<RPL_TRANSPORT_INTRINSIC, true>(mem, members, total, stage, ty, blocks, s)
-                   : launch_ens_stage<RPL_TRANSPORT_INTRINSIC, false>(mem, members, total, stage, ty, blocks, s);
+        return src ? launch_ens_stage<RPL_TRANSPORT_INTRINSIC, true>(mem, members, total, cm, stage, ty, blocks, s)
+                   : launch_ens_stage<RPL_TRANSPORT_INTRINSIC, false>(mem, members, total, cm, stage, ty, blocks, s);
     return hipErrorInvalidValue;
 }
 
