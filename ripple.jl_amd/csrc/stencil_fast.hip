@@ -16,6 +16,11 @@
 // Results differ from the reference-order kernel by rounding only.
 #include "rpl_internal.h"
 
+// Every fused multiply-add here is written out (fmaf); nothing else is
+// contracted, so the per-model, phi-loop and ensemble instantiations of the
+// same cell arithmetic give the same bits.
+#pragma clang fp contract(off)
+
 namespace rpl {
 namespace {
 
@@ -138,6 +143,35 @@ __device__ __forceinline__ int wrap(int i, int n) { return i < 0 ? i + n : (i >=
 __device__ __forceinline__ int clampi(int i, int n) { return i < 0 ? 0 : (i >= n ? n - 1 : i); }
 __device__ __forceinline__ float clip0(float x) { return x < 0.0f ? 0.0f : x; }
 
+// Kappa flux difference of one cell (wave_current_refraction.jl:125-137):
+// WENO5 where the six-point stencil fits, first-order upwind faces where it
+// does not, zero flux through the outer faces.  m is uniform across the
+// workgroup, so the edge path does not diverge.  fk[3] is unused (x0).
+__device__ __forceinline__ float kappa_difference(const float (&fk)[7], float x0, float ck, int m, int nk)
+{
+    if (m >= 3 && m + 3 <= nk - 1)
+        return upwind_difference(fk[0], fk[1], fk[2], x0, fk[4], fk[5], fk[6], ck);
+    const float km3 = fk[0], km2 = fk[1], km1 = fk[2], kp1 = fk[4], kp2 = fk[5], kp3 = fk[6];
+    const bool pos = ck >= 0.0f;
+    const float w2 = pos ? km1 : kp1, w4 = pos ? kp1 : km1;
+    float A = w2, B = x0;  // first-order upstream / downstream faces
+    const bool has_p = (m - 2 >= 0) && (m + 3 <= nk - 1);
+    const bool has_m = (m - 3 >= 0) && (m + 2 <= nk - 1);
+    if (has_p || has_m) {
+        float Aw, Bw;
+        weno_pair(pos ? km3 : kp3, pos ? km2 : kp2, w2, x0, w4, pos ? kp2 : km2, Aw, Bw);
+        // For ck >= 0 the downstream face is the plus face, else the minus face.
+        if (pos ? has_m : has_p)
+            A = Aw;
+        if (pos ? has_p : has_m)
+            B = Bw;
+    }
+    const float plus = pos ? B : A, minus = pos ? A : B;
+    const float fkp = (m == nk - 1) ? 0.0f : ck * plus;
+    const float fkm = (m == 0) ? 0.0f : ck * minus;
+    return fkp - fkm;
+}
+
 // Work order.  One 64 x STAGE_ROWS (i, j) tile of one (kappa, phi) bin per workgroup.
 // Workgroups are dealt round-robin to the eight XCDs, so the linear block id
 // is remapped to give every XCD one contiguous range of the logical order
@@ -249,32 +283,7 @@ __device__ __forceinline__ void stage_tile(const float *__restrict__ Nk_, const 
                 fk[d] = ldo<AS1>(Nk + plane * kbin, bo);
             }
             const float dp = upwind_difference(fp[0], fp[1], fp[2], x0, fp[4], fp[5], fp[6], cp);
-            // kappa: bounded; m is uniform across the block, so the edge path
-            // (first-order faces, zero outer flux) does not diverge.
-            float dk;
-            if (m >= 3 && m + 3 <= nk - 1) {
-                dk = upwind_difference(fk[0], fk[1], fk[2], x0, fk[4], fk[5], fk[6], ck);
-            } else {
-                const float km3 = fk[0], km2 = fk[1], km1 = fk[2], kp1 = fk[4], kp2 = fk[5], kp3 = fk[6];
-                const bool pos = ck >= 0.0f;
-                const float w2 = pos ? km1 : kp1, w4 = pos ? kp1 : km1;
-                float A = w2, B = x0;  // first-order upstream / downstream faces
-                const bool has_p = (m - 2 >= 0) && (m + 3 <= nk - 1);
-                const bool has_m = (m - 3 >= 0) && (m + 2 <= nk - 1);
-                if (has_p || has_m) {
-                    float Aw, Bw;
-                    weno_pair(pos ? km3 : kp3, pos ? km2 : kp2, w2, x0, w4, pos ? kp2 : km2, Aw, Bw);
-                    // For ck >= 0 the downstream face is the plus face, else the minus face.
-                    if (pos ? has_m : has_p)
-                        A = Aw;
-                    if (pos ? has_p : has_m)
-                        B = Bw;
-                }
-                const float plus = pos ? B : A, minus = pos ? A : B;
-                const float fkp = (m == nk - 1) ? 0.0f : ck * plus;
-                const float fkm = (m == 0) ? 0.0f : ck * minus;
-                dk = fkp - fkm;
-            }
+            const float dk = kappa_difference(fk, x0, ck, m, nk);
             L = L - fmaf(dk, p.dk_inv, dp * p.dp_inv);
         }
     }
@@ -312,6 +321,145 @@ __global__ __launch_bounds__(64 * STAGE_ROWS) void stage_fast_kernel(const float
 {
     const Tile t = tile_of(blockIdx.x, nbx * nby * p.nk * p.nphi, nbx, nby, p.nk, p.nphi);
     stage_tile<TRANSPORT, STAGE, SRC, STAGE_ROWS, false>(Nk, N0, out, p, t.ib, t.jb, t.m, t.n);
+}
+
+// Refraction stage with the phi bins looped inside the workgroup: one
+// workgroup per 64 x 4 (i, j) tile of one kappa bin, walking n = 0..nphi-1.
+// What depends on (i, j, m) only -- the Doppler velocity and its four
+// centred gradients -- is loaded and formed once instead of once per bin,
+// and the phi stencil is a register window that takes one new load per bin.
+// The arithmetic of every cell is that of stage_tile, so the results are the
+// same bit for bit.  Work order: XCD-aware as tile_of, kappa fastest, so the
+// kappa neighbours a tile reads are processed together on one XCD.
+template <int STAGE, bool SRC>
+__global__ __launch_bounds__(256) void stage_phi_kernel(const float *__restrict__ Nk, const float *__restrict__ N0,
+                                                        float *out, StageParams<float> p, int nbx, int nby)
+{
+    const int nx = p.nx, ny = p.ny, nk = p.nk, nphi = p.nphi;
+    const Tile t = tile_of(blockIdx.x, nbx * nby * nk, nbx, nby, nk, 1);
+    const int m = t.m;
+    const int i0 = t.ib * 64, j0 = t.jb * 4;
+    const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
+    if (i >= nx || j >= ny)
+        return;
+    const size_t plane = (size_t)nx * ny;
+    const unsigned nxu = (unsigned)nx;
+    const unsigned off = (unsigned)i + nxu * (unsigned)j;
+    const unsigned bo = off << 2;
+    const bool x_inner = i0 >= 3 && i0 + 64 + 3 <= nx;  // block-uniform
+    const bool y_inner = j0 >= 3 && j0 + 4 + 3 <= ny;
+    const unsigned rowb = (nxu * (unsigned)j) << 2, colb = (unsigned)i << 2;
+    const unsigned sx = 4u, sy = nxu << 2;
+    // Doppler velocity and gradients at (i, j, m) (wave_current_refraction.jl:7-19).
+    const float *Ux = p.Ux + plane * m, *Uy = p.Uy + plane * m;
+    const unsigned xp = x_inner ? bo + sx : rowb + ((unsigned)wrap(i + 1, nx) << 2);
+    const unsigned xm = x_inner ? bo - sx : rowb + ((unsigned)wrap(i - 1, nx) << 2);
+    const unsigned yp = y_inner ? bo + sy : colb + sy * (unsigned)wrap(j + 1, ny);
+    const unsigned ym = y_inner ? bo - sy : colb + sy * (unsigned)wrap(j - 1, ny);
+    const float hx = 0.5f * p.dx_inv, hy = 0.5f * p.dy_inv;
+    const float Uxx = (ldo(Ux, xp) - ldo(Ux, xm)) * hx;
+    const float Uxy = (ldo(Ux, yp) - ldo(Ux, ym)) * hy;
+    const float Uyx = (ldo(Uy, xp) - ldo(Uy, xm)) * hx;
+    const float Uyy = (ldo(Uy, yp) - ldo(Uy, ym)) * hy;
+    const float Uxc = ldo(Ux, bo), Uyc = ldo(Uy, bo);
+    const float kap = p.kappa[m];
+    // phi window f(n - 3 .. n + 3) at this cell, periodic in phi
+    float w[7];
+#pragma unroll
+    for (int d = 0; d < 7; ++d)
+        w[d] = ldo(Nk + plane * ((size_t)m + (size_t)nk * wrap(d - 3, nphi)), bo);
+    for (int n = 0; n < nphi; ++n) {
+        const size_t bin = (size_t)m + (size_t)nk * n;
+        const float *slab = Nk + plane * bin;
+        const float x0 = w[3];
+        float ux = p.cgx[m + nk * n], uy = p.cgy[m + nk * n];
+        const float cs = p.cosp[n], sn = p.sinp[n];
+        const float cc = cs * cs, ss = sn * sn, csn = cs * sn;
+        const float ck = -kap * (cc * Uxx + csn * (Uyx + Uxy) + ss * Uyy);
+        const float cp = csn * (Uxx - Uyy) + ss * Uyx - cc * Uxy;
+        ux += Uxc;
+        uy += Uyc;
+        float fx[7], fy[7], fk[7];
+        if (x_inner) {
+#pragma unroll
+            for (int d = 0; d < 7; ++d)
+                fx[d] = ldo(slab, bo + (unsigned)(4 * (d - 3)));
+        } else {
+#pragma unroll
+            for (int d = 0; d < 7; ++d)
+                fx[d] = ldo(slab, rowb + ((unsigned)wrap(i + d - 3, nx) << 2));
+        }
+        if (y_inner) {
+#pragma unroll
+            for (int d = 0; d < 7; ++d)
+                fy[d] = ldo(slab, bo + sy * (unsigned)(d - 3));
+        } else {
+#pragma unroll
+            for (int d = 0; d < 7; ++d)
+                fy[d] = ldo(slab, colb + sy * (unsigned)wrap(j + d - 3, ny));
+        }
+#pragma unroll
+        for (int d = 0; d < 7; ++d) {
+            if (d == 3)
+                continue;
+            const size_t kbin = (size_t)clampi(m + d - 3, nk) + (size_t)nk * n;
+            fk[d] = ldo(Nk + plane * kbin, bo);
+        }
+        const float dx = upwind_difference(fx[0], fx[1], fx[2], x0, fx[4], fx[5], fx[6], ux);
+        const float dy = upwind_difference(fy[0], fy[1], fy[2], x0, fy[4], fy[5], fy[6], uy);
+        float L = -fmaf(dx, p.dx_inv, dy * p.dy_inv);
+        const float dp = upwind_difference(w[0], w[1], w[2], x0, w[4], w[5], w[6], cp);
+        const float dk = kappa_difference(fk, x0, ck, m, nk);
+        L = L - fmaf(dk, p.dk_inv, dp * p.dp_inv);
+        if (SRC)  // tendencies.jl:88-94, G += source_tendency
+            L += source_term<float>(p.src, off, m + nk * n, x0);
+        if (p.src_add)
+            L += ldo(p.src_add + plane * bin, bo);
+        float r;
+        if (STAGE == STAGE_TENDENCY) {
+            r = L;
+        } else if (STAGE == STAGE_1) {
+            r = clip0(fmaf(p.dt, L, x0));
+        } else {
+            const float a = STAGE == STAGE_2 ? 0.75f : (float)(1.0 / 3.0);
+            const float b = STAGE == STAGE_2 ? 0.25f : (float)(2.0 / 3.0);
+            const float st = clip0(fmaf(p.dt, L, x0));
+            r = clip0(fmaf(a, ldo(N0 + plane * bin, bo), b * st));
+        }
+        sto(out + plane * bin, bo, r);
+        if (n + 1 < nphi) {
+#pragma unroll
+            for (int d = 0; d < 6; ++d)
+                w[d] = w[d + 1];
+            w[6] = ldo(Nk + plane * ((size_t)m + (size_t)nk * wrap(n + 4, nphi)), bo);
+        }
+    }
+}
+
+template <bool SRC>
+hipError_t launch_phi(const StageParams<float> &p, int stage, const float *Nk, const float *N0, float *out,
+                      hipStream_t s)
+{
+    const int nbx = (p.nx + 63) / 64, nby = (p.ny + 3) / 4;
+    const long long total = (long long)nbx * nby * p.nk;
+    if (total > 0x7fffffffLL)
+        return hipErrorInvalidValue;
+    dim3 block(64, 4, 1), grid((unsigned)total, 1, 1);
+    switch (stage) {
+    case STAGE_TENDENCY:
+        hipLaunchKernelGGL((stage_phi_kernel<STAGE_TENDENCY, SRC>), grid, block, 0, s, Nk, N0, out, p, nbx, nby);
+        break;
+    case STAGE_1:
+        hipLaunchKernelGGL((stage_phi_kernel<STAGE_1, SRC>), grid, block, 0, s, Nk, N0, out, p, nbx, nby);
+        break;
+    case STAGE_2:
+        hipLaunchKernelGGL((stage_phi_kernel<STAGE_2, SRC>), grid, block, 0, s, Nk, N0, out, p, nbx, nby);
+        break;
+    default:
+        hipLaunchKernelGGL((stage_phi_kernel<STAGE_3, SRC>), grid, block, 0, s, Nk, N0, out, p, nbx, nby);
+        break;
+    }
+    return hipGetLastError();
 }
 
 // Ensemble work list: every member's tiles (64 x TY cells of one bin) are
@@ -444,13 +592,13 @@ hipError_t launch_stage_fast(const StageParams<float> &p, int transport, int sta
 {
     if (p.src.on) {
         if (transport == RPL_TRANSPORT_REFRACTION)
-            return launch_t<RPL_TRANSPORT_REFRACTION, true>(p, stage, Nk, N0, out, s);
+            return launch_phi<true>(p, stage, Nk, N0, out, s);
         if (transport == RPL_TRANSPORT_INTRINSIC)
             return launch_t<RPL_TRANSPORT_INTRINSIC, true>(p, stage, Nk, N0, out, s);
         return launch_t<RPL_TRANSPORT_NONE, true>(p, stage, Nk, N0, out, s);
     }
     if (transport == RPL_TRANSPORT_REFRACTION)
-        return launch_t<RPL_TRANSPORT_REFRACTION, false>(p, stage, Nk, N0, out, s);
+        return launch_phi<false>(p, stage, Nk, N0, out, s);
     if (transport == RPL_TRANSPORT_INTRINSIC)
         return launch_t<RPL_TRANSPORT_INTRINSIC, false>(p, stage, Nk, N0, out, s);
     return launch_t<RPL_TRANSPORT_NONE, false>(p, stage, Nk, N0, out, s);
