@@ -99,6 +99,55 @@ __device__ __forceinline__ void weno_pair(float w0, float w1, float w2, float w3
     (void)weno_pair6<true>(w0, w1, w2, w3, w4, w5, &A, &B);
 }
 
+// One face of weno_pair6 on its own: for the window u0..u4 (upwind first) the
+// face value is (v + f) / 6.  The operations are those weno_pair6 performs for
+// its A face on w0..w4 (and for its B face on w1..w5), in the same order, so a
+// face carried from one cell to its neighbour has the same bits as the pair
+// evaluated at the neighbour.
+struct Face6 {
+    float v, f;
+};
+
+__device__ __forceinline__ Face6 weno_face_of5(float u0, float u1, float u2, float u3, float u4)
+{
+    const float eps4 = 4.0f * (10000.0f * 1.1920928955078125e-7f);
+    const float c13_3 = 13.0f / 3.0f;
+    const float h0 = u1 - u0, h1 = u2 - u1, h2 = u3 - u2, h3 = u4 - u3;
+    const float E1 = h1 - h0, E2 = h2 - h1, E3 = h3 - h2;
+    const float s1 = fmaf(c13_3 * E1, E1, eps4), s2 = fmaf(c13_3 * E2, E2, eps4);
+    const float s3 = fmaf(c13_3 * E3, E3, eps4);
+    const float eA = fmaf(2.0f, h1, E1), eC = h1 + h2, eB = fmaf(-2.0f, h2, E3);
+    const float qA = fmaf(eA, eA, s1), qC = fmaf(eC, eC, s2), qB = fmaf(eB, eB, s3);
+    Face6 r;
+    r.f = weno_face6(qA, qC, qB, E1 - E2, E2 - E3);
+    r.v = fmaf(5.0f, u2, fmaf(2.0f, u3, -u1));
+    return r;
+}
+
+// weno_pair6's two faces as (v, f) parts; (b.v - a.v) + (b.f - a.f) is its
+// return value.
+__device__ __forceinline__ void weno_pair_parts(float w0, float w1, float w2, float w3, float w4, float w5, Face6 &a,
+                                                Face6 &b)
+{
+    const float eps4 = 4.0f * (10000.0f * 1.1920928955078125e-7f);
+    const float c13_3 = 13.0f / 3.0f;
+    const float g0 = w1 - w0, g1 = w2 - w1, g2 = w3 - w2, g3 = w4 - w3, g4 = w5 - w4;
+    const float D1 = g1 - g0, D2 = g2 - g1, D3 = g3 - g2, D4 = g4 - g3;
+    const float t1 = fmaf(c13_3 * D1, D1, eps4), t2 = fmaf(c13_3 * D2, D2, eps4);
+    const float t3 = fmaf(c13_3 * D3, D3, eps4), t4 = fmaf(c13_3 * D4, D4, eps4);
+    const float eA1 = fmaf(2.0f, g1, D1), eA2 = fmaf(2.0f, g2, D2);
+    const float eC2 = g1 + g2, eC3 = g2 + g3;
+    const float eB3 = fmaf(-2.0f, g2, D3), eB4 = fmaf(-2.0f, g3, D4);
+    const float qA1 = fmaf(eA1, eA1, t1), qA2 = fmaf(eA2, eA2, t2);
+    const float qC2 = fmaf(eC2, eC2, t2), qC3 = fmaf(eC3, eC3, t3);
+    const float qB3 = fmaf(eB3, eB3, t3), qB4 = fmaf(eB4, eB4, t4);
+    const float U1 = D1 - D2, U2 = D2 - D3, U3 = D3 - D4;
+    a.f = weno_face6(qA1, qC2, qB3, U1, U2);
+    b.f = weno_face6(qA2, qC3, qB4, U2, U3);
+    a.v = fmaf(5.0f, w2, fmaf(2.0f, w3, -w1));
+    b.v = fmaf(5.0f, w3, fmaf(2.0f, w4, -w2));
+}
+
 // |vel| * (N_plus - N_minus) for a direction with a full stencil on both faces.
 // The 1/6 folds into the product with |vel|.
 __device__ __forceinline__ float upwind_difference(float fm3, float fm2, float fm1, float f0, float fp1, float fp2,
@@ -363,14 +412,67 @@ __global__ __launch_bounds__(256) void stage_phi_kernel(const float *__restrict_
     const float Uyy = (ldo(Uy, yp) - ldo(Uy, ym)) * hy;
     const float Uxc = ldo(Ux, bo), Uyc = ldo(Uy, bo);
     const float kap = p.kappa[m];
+    // What bin n reads besides the phi window: its x, y and kappa stencils,
+    // the window's next value f(n + 4), and N0 and the added source at the
+    // cell.  The latency of these loads is hidden by the other waves of the
+    // SIMD, so occupancy (registers per lane) decides the speed: issuing
+    // bin n + 1's loads ahead of bin n's arithmetic costs 20 more registers
+    // and runs slower.
+    struct BinLoads {
+        float fx[7], fy[7], fk[7], wnext, n0, add;
+    };
+    const unsigned cell_off = bo;
+    auto load_bin = [&](int n, BinLoads &b) {
+        // Every load below is a scalar slab base plus this 32-bit lane offset.
+        // The empty asm hides that the offset is the same in every bin;
+        // otherwise the compiler hoists one 64-bit lane pointer per stencil
+        // point out of the phi loop (26 more registers, one wave less per
+        // SIMD).
+        unsigned bo = cell_off;
+        asm volatile("" : "+v"(bo));
+        const float *slab = Nk + plane * ((size_t)m + (size_t)nk * n);
+        if (x_inner) {
+#pragma unroll
+            for (int d = 0; d < 7; ++d)
+                if (d != 3)
+                    b.fx[d] = ldo(slab, bo + (unsigned)(4 * (d - 3)));
+        } else {
+#pragma unroll
+            for (int d = 0; d < 7; ++d)
+                if (d != 3)
+                    b.fx[d] = ldo(slab, rowb + ((unsigned)wrap(i + d - 3, nx) << 2));
+        }
+        if (y_inner) {
+#pragma unroll
+            for (int d = 0; d < 7; ++d)
+                if (d != 3)
+                    b.fy[d] = ldo(slab, bo + sy * (unsigned)(d - 3));
+        } else {
+#pragma unroll
+            for (int d = 0; d < 7; ++d)
+                if (d != 3)
+                    b.fy[d] = ldo(slab, colb + sy * (unsigned)wrap(j + d - 3, ny));
+        }
+#pragma unroll
+        for (int d = 0; d < 7; ++d)
+            if (d != 3)
+                b.fk[d] = ldo(Nk + plane * ((size_t)clampi(m + d - 3, nk) + (size_t)nk * n), bo);
+        b.wnext = ldo(Nk + plane * ((size_t)m + (size_t)nk * wrap(n + 4, nphi)), bo);
+        const size_t bin = (size_t)m + (size_t)nk * n;
+        b.n0 = (STAGE == STAGE_2 || STAGE == STAGE_3) ? ldo(N0 + plane * bin, bo) : 0.0f;
+        b.add = p.src_add ? ldo(p.src_add + plane * bin, bo) : 0.0f;
+    };
     // phi window f(n - 3 .. n + 3) at this cell, periodic in phi
     float w[7];
 #pragma unroll
     for (int d = 0; d < 7; ++d)
         w[d] = ldo(Nk + plane * ((size_t)m + (size_t)nk * wrap(d - 3, nphi)), bo);
+    Face6 carry = {0.0f, 0.0f};  // phi face n - 1/2 (as (v, f) of weno_face_of5)
+    bool cpos = true;            // its upwind side
     for (int n = 0; n < nphi; ++n) {
+        BinLoads cur;
+        load_bin(n, cur);
         const size_t bin = (size_t)m + (size_t)nk * n;
-        const float *slab = Nk + plane * bin;
         const float x0 = w[3];
         float ux = p.cgx[m + nk * n], uy = p.cgy[m + nk * n];
         const float cs = p.cosp[n], sn = p.sinp[n];
@@ -379,42 +481,35 @@ __global__ __launch_bounds__(256) void stage_phi_kernel(const float *__restrict_
         const float cp = csn * (Uxx - Uyy) + ss * Uyx - cc * Uxy;
         ux += Uxc;
         uy += Uyc;
-        float fx[7], fy[7], fk[7];
-        if (x_inner) {
-#pragma unroll
-            for (int d = 0; d < 7; ++d)
-                fx[d] = ldo(slab, bo + (unsigned)(4 * (d - 3)));
-        } else {
-#pragma unroll
-            for (int d = 0; d < 7; ++d)
-                fx[d] = ldo(slab, rowb + ((unsigned)wrap(i + d - 3, nx) << 2));
-        }
-        if (y_inner) {
-#pragma unroll
-            for (int d = 0; d < 7; ++d)
-                fy[d] = ldo(slab, bo + sy * (unsigned)(d - 3));
-        } else {
-#pragma unroll
-            for (int d = 0; d < 7; ++d)
-                fy[d] = ldo(slab, colb + sy * (unsigned)wrap(j + d - 3, ny));
-        }
-#pragma unroll
-        for (int d = 0; d < 7; ++d) {
-            if (d == 3)
-                continue;
-            const size_t kbin = (size_t)clampi(m + d - 3, nk) + (size_t)nk * n;
-            fk[d] = ldo(Nk + plane * kbin, bo);
-        }
+        const float *fx = cur.fx, *fy = cur.fy;
         const float dx = upwind_difference(fx[0], fx[1], fx[2], x0, fx[4], fx[5], fx[6], ux);
         const float dy = upwind_difference(fy[0], fy[1], fy[2], x0, fy[4], fy[5], fy[6], uy);
         float L = -fmaf(dx, p.dx_inv, dy * p.dy_inv);
-        const float dp = upwind_difference(w[0], w[1], w[2], x0, w[4], w[5], w[6], cp);
-        const float dk = kappa_difference(fk, x0, ck, m, nk);
+        // phi faces: the face n + 1/2 of this bin is the face n - 1/2 of the
+        // next one.  When every lane keeps its upwind side from the previous
+        // bin, only the new face is evaluated.
+        const bool ppos = cp >= 0.0f;
+        float d6;
+        if (n > 0 && __all(ppos == cpos)) {
+            const Face6 nf = weno_face_of5(ppos ? w[1] : w[6], ppos ? w[2] : w[5], ppos ? x0 : w[4],
+                                           ppos ? w[4] : x0, ppos ? w[5] : w[2]);
+            d6 = ppos ? (nf.v - carry.v) + (nf.f - carry.f) : (carry.v - nf.v) + (carry.f - nf.f);
+            carry = nf;
+        } else {
+            Face6 a, b;
+            weno_pair_parts(ppos ? w[0] : w[6], ppos ? w[1] : w[5], ppos ? w[2] : w[4], x0, ppos ? w[4] : w[2],
+                            ppos ? w[5] : w[1], a, b);
+            d6 = (b.v - a.v) + (b.f - a.f);
+            carry = ppos ? b : a;
+        }
+        cpos = ppos;
+        const float dp = fabsf(cp) * (1.0f / 6.0f) * d6;
+        const float dk = kappa_difference(cur.fk, x0, ck, m, nk);
         L = L - fmaf(dk, p.dk_inv, dp * p.dp_inv);
         if (SRC)  // tendencies.jl:88-94, G += source_tendency
             L += source_term<float>(p.src, off, m + nk * n, x0);
         if (p.src_add)
-            L += ldo(p.src_add + plane * bin, bo);
+            L += cur.add;
         float r;
         if (STAGE == STAGE_TENDENCY) {
             r = L;
@@ -424,15 +519,13 @@ __global__ __launch_bounds__(256) void stage_phi_kernel(const float *__restrict_
             const float a = STAGE == STAGE_2 ? 0.75f : (float)(1.0 / 3.0);
             const float b = STAGE == STAGE_2 ? 0.25f : (float)(2.0 / 3.0);
             const float st = clip0(fmaf(p.dt, L, x0));
-            r = clip0(fmaf(a, ldo(N0 + plane * bin, bo), b * st));
+            r = clip0(fmaf(a, cur.n0, b * st));
         }
         sto(out + plane * bin, bo, r);
-        if (n + 1 < nphi) {
 #pragma unroll
-            for (int d = 0; d < 6; ++d)
-                w[d] = w[d + 1];
-            w[6] = ldo(Nk + plane * ((size_t)m + (size_t)nk * wrap(n + 4, nphi)), bo);
-        }
+        for (int d = 0; d < 6; ++d)
+            w[d] = w[d + 1];
+        w[6] = cur.wnext;
     }
 }
 
