@@ -54,7 +54,7 @@ def measured_traffic(workload):
     """HBM bytes per stage launch from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE summary of this workload (scaled by the copy-kernel factors
     recorded there), or None if there is none for this workload."""
-    path = os.path.join(ROOT, "profiles", "stage_fast_kernel_hbm_traffic.json")
+    path = os.path.join(ROOT, "profiles", "stage_phi_kernel_hbm_traffic.json")
     try:
         with open(path) as f:
             rec = json.load(f)
@@ -287,7 +287,7 @@ def main():
                    "sources": None, "timestepper": "SSP-RK3", "kernel": "fast"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(wl.name),
-                     "kernel": "stage_fast_kernel", "bytes_per_launch": bpl,
+                     "kernel": "stage_phi_kernel", "bytes_per_launch": bpl,
                      "avg_launch_ms": t_launch, "launches_timed": launches},
     }
     extra = {}

@@ -598,7 +598,9 @@ int ensemble_descriptors(rpl_ensemble *e, double dt)
             d.nby = nby;
             d.tile0 = t0;
         }
-        t0 += (long long)nbx * nby * c->cfg.nkappa * c->cfg.nphi;
+        // Refraction entries walk the phi bins inside the workgroup: one per
+        // (tile, kappa bin); the other transports have one per (tile, bin).
+        t0 += (long long)nbx * nby * c->cfg.nkappa * (e->transport == RPL_TRANSPORT_REFRACTION ? 1 : c->cfg.nphi);
     }
     e->total = t0;
     RPL_HIP(hipMemcpy(e->desc, h.data(), h.size() * sizeof(rpl::EnsembleMember), hipMemcpyHostToDevice));

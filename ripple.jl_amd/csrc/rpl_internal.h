@@ -31,8 +31,10 @@ struct StageParams {
 };
 
 // One ensemble member's stage launch in the work-list kernel: its stage
-// parameters, buffers and tile numbering (64 x TY tiles per bin; nby =
-// ceil(ny / TY); tile0 = tiles of all earlier members).
+// parameters, buffers and work-list numbering (64 x TY tiles, nby =
+// ceil(ny / TY); one entry per tile and kappa bin for refraction transport,
+// which walks the phi bins inside the workgroup, one per tile and (kappa,
+// phi) bin otherwise; tile0 = entries of all earlier members).
 struct EnsembleMember {
     StageParams<float> p;
     const float *Nk, *N0;

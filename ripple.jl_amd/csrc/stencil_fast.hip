@@ -373,21 +373,19 @@ __global__ __launch_bounds__(64 * STAGE_ROWS) void stage_fast_kernel(const float
 }
 
 // Refraction stage with the phi bins looped inside the workgroup: one
-// workgroup per 64 x 4 (i, j) tile of one kappa bin, walking n = 0..nphi-1.
+// workgroup per 64 x TY (i, j) tile of one kappa bin, walking n = 0..nphi-1.
 // What depends on (i, j, m) only -- the Doppler velocity and its four
 // centred gradients -- is loaded and formed once instead of once per bin,
 // and the phi stencil is a register window that takes one new load per bin.
 // The arithmetic of every cell is that of stage_tile, so the results are the
-// same bit for bit.  Work order: XCD-aware as tile_of, kappa fastest, so the
-// kappa neighbours a tile reads are processed together on one XCD.
-template <int STAGE, bool SRC>
-__global__ __launch_bounds__(256) void stage_phi_kernel(const float *__restrict__ Nk, const float *__restrict__ N0,
-                                                        float *out, StageParams<float> p, int nbx, int nby)
+// same bit for bit.  Shared by the per-model kernel and the ensemble
+// work-list kernel (AS1: p and the slab pointers came from memory).
+template <int STAGE, bool SRC, int TY, bool AS1>
+__device__ __forceinline__ void phi_tile(const float *__restrict__ Nk, const float *__restrict__ N0, float *out,
+                                         const StageParams<float> &p, int ib, int jb, int m)
 {
     const int nx = p.nx, ny = p.ny, nk = p.nk, nphi = p.nphi;
-    const Tile t = tile_of(blockIdx.x, nbx * nby * nk, nbx, nby, nk, 1);
-    const int m = t.m;
-    const int i0 = t.ib * 64, j0 = t.jb * 4;
+    const int i0 = ib * 64, j0 = jb * TY;
     const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
     if (i >= nx || j >= ny)
         return;
@@ -396,7 +394,7 @@ __global__ __launch_bounds__(256) void stage_phi_kernel(const float *__restrict_
     const unsigned off = (unsigned)i + nxu * (unsigned)j;
     const unsigned bo = off << 2;
     const bool x_inner = i0 >= 3 && i0 + 64 + 3 <= nx;  // block-uniform
-    const bool y_inner = j0 >= 3 && j0 + 4 + 3 <= ny;
+    const bool y_inner = j0 >= 3 && j0 + TY + 3 <= ny;
     const unsigned rowb = (nxu * (unsigned)j) << 2, colb = (unsigned)i << 2;
     const unsigned sx = 4u, sy = nxu << 2;
     // Doppler velocity and gradients at (i, j, m) (wave_current_refraction.jl:7-19).
@@ -406,12 +404,12 @@ __global__ __launch_bounds__(256) void stage_phi_kernel(const float *__restrict_
     const unsigned yp = y_inner ? bo + sy : colb + sy * (unsigned)wrap(j + 1, ny);
     const unsigned ym = y_inner ? bo - sy : colb + sy * (unsigned)wrap(j - 1, ny);
     const float hx = 0.5f * p.dx_inv, hy = 0.5f * p.dy_inv;
-    const float Uxx = (ldo(Ux, xp) - ldo(Ux, xm)) * hx;
-    const float Uxy = (ldo(Ux, yp) - ldo(Ux, ym)) * hy;
-    const float Uyx = (ldo(Uy, xp) - ldo(Uy, xm)) * hx;
-    const float Uyy = (ldo(Uy, yp) - ldo(Uy, ym)) * hy;
-    const float Uxc = ldo(Ux, bo), Uyc = ldo(Uy, bo);
-    const float kap = p.kappa[m];
+    const float Uxx = (ldo<AS1>(Ux, xp) - ldo<AS1>(Ux, xm)) * hx;
+    const float Uxy = (ldo<AS1>(Ux, yp) - ldo<AS1>(Ux, ym)) * hy;
+    const float Uyx = (ldo<AS1>(Uy, xp) - ldo<AS1>(Uy, xm)) * hx;
+    const float Uyy = (ldo<AS1>(Uy, yp) - ldo<AS1>(Uy, ym)) * hy;
+    const float Uxc = ldo<AS1>(Ux, bo), Uyc = ldo<AS1>(Uy, bo);
+    const float kap = gld<AS1>(p.kappa, m);
     // What bin n reads besides the phi window: its x, y and kappa stencils,
     // the window's next value f(n + 4), and N0 and the added source at the
     // cell.  The latency of these loads is hidden by the other waves of the
@@ -435,38 +433,38 @@ __global__ __launch_bounds__(256) void stage_phi_kernel(const float *__restrict_
 #pragma unroll
             for (int d = 0; d < 7; ++d)
                 if (d != 3)
-                    b.fx[d] = ldo(slab, bo + (unsigned)(4 * (d - 3)));
+                    b.fx[d] = ldo<AS1>(slab, bo + (unsigned)(4 * (d - 3)));
         } else {
 #pragma unroll
             for (int d = 0; d < 7; ++d)
                 if (d != 3)
-                    b.fx[d] = ldo(slab, rowb + ((unsigned)wrap(i + d - 3, nx) << 2));
+                    b.fx[d] = ldo<AS1>(slab, rowb + ((unsigned)wrap(i + d - 3, nx) << 2));
         }
         if (y_inner) {
 #pragma unroll
             for (int d = 0; d < 7; ++d)
                 if (d != 3)
-                    b.fy[d] = ldo(slab, bo + sy * (unsigned)(d - 3));
+                    b.fy[d] = ldo<AS1>(slab, bo + sy * (unsigned)(d - 3));
         } else {
 #pragma unroll
             for (int d = 0; d < 7; ++d)
                 if (d != 3)
-                    b.fy[d] = ldo(slab, colb + sy * (unsigned)wrap(j + d - 3, ny));
+                    b.fy[d] = ldo<AS1>(slab, colb + sy * (unsigned)wrap(j + d - 3, ny));
         }
 #pragma unroll
         for (int d = 0; d < 7; ++d)
             if (d != 3)
-                b.fk[d] = ldo(Nk + plane * ((size_t)clampi(m + d - 3, nk) + (size_t)nk * n), bo);
-        b.wnext = ldo(Nk + plane * ((size_t)m + (size_t)nk * wrap(n + 4, nphi)), bo);
+                b.fk[d] = ldo<AS1>(Nk + plane * ((size_t)clampi(m + d - 3, nk) + (size_t)nk * n), bo);
+        b.wnext = ldo<AS1>(Nk + plane * ((size_t)m + (size_t)nk * wrap(n + 4, nphi)), bo);
         const size_t bin = (size_t)m + (size_t)nk * n;
-        b.n0 = (STAGE == STAGE_2 || STAGE == STAGE_3) ? ldo(N0 + plane * bin, bo) : 0.0f;
-        b.add = p.src_add ? ldo(p.src_add + plane * bin, bo) : 0.0f;
+        b.n0 = (STAGE == STAGE_2 || STAGE == STAGE_3) ? ldo<AS1>(N0 + plane * bin, bo) : 0.0f;
+        b.add = p.src_add ? ldo<AS1>(p.src_add + plane * bin, bo) : 0.0f;
     };
     // phi window f(n - 3 .. n + 3) at this cell, periodic in phi
     float w[7];
 #pragma unroll
     for (int d = 0; d < 7; ++d)
-        w[d] = ldo(Nk + plane * ((size_t)m + (size_t)nk * wrap(d - 3, nphi)), bo);
+        w[d] = ldo<AS1>(Nk + plane * ((size_t)m + (size_t)nk * wrap(d - 3, nphi)), bo);
     Face6 carry = {0.0f, 0.0f};  // phi face n - 1/2 (as (v, f) of weno_face_of5)
     bool cpos = true;            // its upwind side
     for (int n = 0; n < nphi; ++n) {
@@ -474,8 +472,8 @@ __global__ __launch_bounds__(256) void stage_phi_kernel(const float *__restrict_
         load_bin(n, cur);
         const size_t bin = (size_t)m + (size_t)nk * n;
         const float x0 = w[3];
-        float ux = p.cgx[m + nk * n], uy = p.cgy[m + nk * n];
-        const float cs = p.cosp[n], sn = p.sinp[n];
+        float ux = gld<AS1>(p.cgx, m + nk * n), uy = gld<AS1>(p.cgy, m + nk * n);
+        const float cs = gld<AS1>(p.cosp, n), sn = gld<AS1>(p.sinp, n);
         const float cc = cs * cs, ss = sn * sn, csn = cs * sn;
         const float ck = -kap * (cc * Uxx + csn * (Uyx + Uxy) + ss * Uyy);
         const float cp = csn * (Uxx - Uyy) + ss * Uyx - cc * Uxy;
@@ -507,7 +505,7 @@ __global__ __launch_bounds__(256) void stage_phi_kernel(const float *__restrict_
         const float dk = kappa_difference(cur.fk, x0, ck, m, nk);
         L = L - fmaf(dk, p.dk_inv, dp * p.dp_inv);
         if (SRC)  // tendencies.jl:88-94, G += source_tendency
-            L += source_term<float>(p.src, off, m + nk * n, x0);
+            L += source_term<float, AS1>(p.src, off, m + nk * n, x0);
         if (p.src_add)
             L += cur.add;
         float r;
@@ -521,12 +519,24 @@ __global__ __launch_bounds__(256) void stage_phi_kernel(const float *__restrict_
             const float st = clip0(fmaf(p.dt, L, x0));
             r = clip0(fmaf(a, cur.n0, b * st));
         }
-        sto(out + plane * bin, bo, r);
+        sto<AS1>(out + plane * bin, bo, r);
 #pragma unroll
         for (int d = 0; d < 6; ++d)
             w[d] = w[d + 1];
         w[6] = cur.wnext;
     }
+}
+
+
+// Per-model refraction stage: work order XCD-aware as tile_of, kappa
+// fastest, so the kappa neighbours a tile reads are processed together on one
+// XCD.
+template <int STAGE, bool SRC>
+__global__ __launch_bounds__(256) void stage_phi_kernel(const float *__restrict__ Nk, const float *__restrict__ N0,
+                                                        float *out, StageParams<float> p, int nbx, int nby)
+{
+    const Tile t = tile_of(blockIdx.x, nbx * nby * p.nk, nbx, nby, p.nk, 1);
+    phi_tile<STAGE, SRC, 4, false>(Nk, N0, out, p, t.ib, t.jb, t.m);
 }
 
 template <bool SRC>
@@ -600,6 +610,38 @@ __global__ __launch_bounds__(64 * TY) void ensemble_stage_kernel(const EnsembleM
     }
 }
 
+// Ensemble refraction stages: the same work list with one entry per
+// (64 x TY tile, kappa bin) of a member, ordered kappa fastest, then tile row
+// and tile column; each workgroup walks the phi bins of its entry as the
+// per-model kernel does (phi_tile), so a member's results are those of the
+// member stepped alone, bit for bit.
+template <int STAGE, bool SRC, int TY>
+__global__ __launch_bounds__(64 * TY) void ensemble_phi_kernel(const EnsembleMember *__restrict__ mem, int members,
+                                                               long long total, const int *__restrict__ chunk_member)
+{
+    const int xcd = blockIdx.x & 7;
+    const long long stride = (long long)(gridDim.x >> 3);
+    const long long lo = total * xcd / 8, hi = total * (xcd + 1) / 8;
+    int a = 0;
+    long long a_begin = 0, a_end = 0;
+    for (long long L = lo + (blockIdx.x >> 3); L < hi; L += stride) {
+        if (L >= a_end) {
+            int u = chunk_member[L >> ENSEMBLE_CHUNK_SHIFT];
+            while (u + 1 < members && mem[u + 1].tile0 <= L)
+                ++u;
+            a = __builtin_amdgcn_readfirstlane(u);
+            a_begin = mem[a].tile0;
+            a_end = a + 1 < members ? mem[a + 1].tile0 : total;
+        }
+        const EnsembleMember &e = mem[a];
+        int q = __builtin_amdgcn_readfirstlane((int)(L - a_begin));
+        const int m = q % e.p.nk;
+        q /= e.p.nk;
+        const int jb = q % e.nby, ib = q / e.nby;
+        phi_tile<STAGE, SRC, TY, true>(e.Nk, e.N0, e.out, e.p, ib, jb, m);
+    }
+}
+
 template <int TRANSPORT, bool SRC>
 hipError_t launch_t(const StageParams<float> &p, int stage, const float *Nk, const float *N0, float *out,
                     hipStream_t s)
@@ -631,6 +673,23 @@ template <int TRANSPORT, int STAGE, bool SRC>
 hipError_t launch_ens_ty(const EnsembleMember *mem, int members, long long total, const int *chunk_member, int ty,
                          int blocks, hipStream_t s)
 {
+    if constexpr (TRANSPORT == RPL_TRANSPORT_REFRACTION) {
+        switch (ty) {
+        case 2:
+            hipLaunchKernelGGL((ensemble_phi_kernel<STAGE, SRC, 2>), dim3(blocks), dim3(64, 2), 0, s, mem, members,
+                               total, chunk_member);
+            break;
+        case 8:
+            hipLaunchKernelGGL((ensemble_phi_kernel<STAGE, SRC, 8>), dim3(blocks), dim3(64, 8), 0, s, mem, members,
+                               total, chunk_member);
+            break;
+        default:
+            hipLaunchKernelGGL((ensemble_phi_kernel<STAGE, SRC, 4>), dim3(blocks), dim3(64, 4), 0, s, mem, members,
+                               total, chunk_member);
+            break;
+        }
+        return hipGetLastError();
+    }
     switch (ty) {
     case 2:
         hipLaunchKernelGGL((ensemble_stage_kernel<TRANSPORT, STAGE, SRC, 2>), dim3(blocks), dim3(64, 2), 0, s, mem,
