@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "mlp_internal.h"
@@ -64,6 +65,11 @@ struct rpl_ctx {
     void *k_cell = nullptr, *theta_cell = nullptr, *sigma_cell = nullptr, *C_cell = nullptr;  // per bin
     void *t_phi = nullptr, *t_w = nullptr, *t_sigma_k = nullptr, *t_cos = nullptr, *t_sin = nullptr;
     void *t_kpow = nullptr, *t_spow = nullptr, *t_wc = nullptr;
+    // FP32 wind input and saturation factors (FP32 contexts only)
+    void *f_cth = nullptr, *f_sth = nullptr, *f_logk = nullptr, *f_invc = nullptr, *f_sk3 = nullptr,
+         *f_sq = nullptr;                                              // per bin
+    void *f_logkap = nullptr, *f_invck = nullptr;                      // per kappa bin
+    void *f_ctu = nullptr, *f_stu = nullptr, *f_lz1 = nullptr;         // per column
     std::vector<double> h_kc, h_pc, h_kf, h_pf;  // geometry kept for source tables
     std::vector<double> h_sigma;                 // per-bin intrinsic frequency of the source tables
     std::vector<double> track_t, track_x, track_y;  // storm track (rpl_set_storm_track), or empty
@@ -194,6 +200,17 @@ rpl::StageParams<T> stage_params(const rpl_ctx *c, double dt)
         q.wc_thr = (T)s.wc_threshold;
         q.wc_sp = (T)s.wc_saturation_power;
         q.wc_factor = (const T *)c->t_wc;
+        if constexpr (std::is_same<T, float>::value) {
+            q.cth_cell = (const T *)c->f_cth;
+            q.sth_cell = (const T *)c->f_sth;
+            q.logk_cell = (const T *)c->f_logk;
+            q.invc_cell = (const T *)c->f_invc;
+            q.sk3_cell = (const T *)c->f_sk3;
+            q.sq_cell = (const T *)c->f_sq;
+            q.cos_tu = (const T *)c->f_ctu;
+            q.sin_tu = (const T *)c->f_stu;
+            q.log_z1 = (const T *)c->f_lz1;
+        }
         if (c->s_active) {
             q.surrogate = 1;
             q.surrogate_terms = c->scfg.terms;
@@ -218,6 +235,8 @@ rpl::StateTables<T> state_tables(const rpl_ctx *c)
     t.sin_phi = (const T *)c->t_sin;
     t.k_pow = (const T *)c->t_kpow;
     t.sigma_pow = (const T *)c->t_spow;
+    t.log_kappa = (const T *)c->f_logkap;
+    t.inv_c = (const T *)c->f_invck;
     t.rho_air = (T)c->src.rho_air;
     t.rho_w_g = (T)(c->src.rho_water * c->src.gravity);
     t.cap_eps = (T)c->src.cap_epsilon;
@@ -252,10 +271,12 @@ int launch_wind(rpl_ctx *c)
     }
     if (is_f64(c))
         e = rpl::launch_wind_columns<double>(nx, ny, c->xn, c->yn, c->time, c->wind, (double *)c->U10,
-                                             (double *)c->theta_u, (double *)c->ustar, c->stream);
+                                             (double *)c->theta_u, (double *)c->ustar, nullptr, nullptr, nullptr,
+                                             c->stream);
     else
         e = rpl::launch_wind_columns<float>(nx, ny, c->xn, c->yn, c->time, c->wind, (float *)c->U10,
-                                            (float *)c->theta_u, (float *)c->ustar, c->stream);
+                                            (float *)c->theta_u, (float *)c->ustar, (float *)c->f_ctu,
+                                            (float *)c->f_stu, (float *)c->f_lz1, c->stream);
     if (e != hipSuccess)
         return fail(RPL_ERR_HIP, std::string("wind kernel: ") + hipGetErrorString(e));
     return RPL_OK;
@@ -598,9 +619,11 @@ int ensemble_descriptors(rpl_ensemble *e, double dt)
             d.nby = nby;
             d.tile0 = t0;
         }
-        // Refraction entries walk the phi bins inside the workgroup: one per
-        // (tile, kappa bin); the other transports have one per (tile, bin).
-        t0 += (long long)nbx * nby * c->cfg.nkappa * (e->transport == RPL_TRANSPORT_REFRACTION ? 1 : c->cfg.nphi);
+        // Refraction and intrinsic entries walk the phi bins inside the
+        // workgroup: one per (tile, kappa bin); without transport, one per
+        // (tile, bin).
+        const bool phi_loop = e->transport == RPL_TRANSPORT_REFRACTION || e->transport == RPL_TRANSPORT_INTRINSIC;
+        t0 += (long long)nbx * nby * c->cfg.nkappa * (phi_loop ? 1 : c->cfg.nphi);
     }
     e->total = t0;
     RPL_HIP(hipMemcpy(e->desc, h.data(), h.size() * sizeof(rpl::EnsembleMember), hipMemcpyHostToDevice));
@@ -806,7 +829,8 @@ int rpl_destroy(rpl_ctx *c)
                     (void *)c->energy, (void *)c->diag, (void *)c->partials, c->scratch_in, (void *)c->xn,
                     (void *)c->yn, c->U10, c->theta_u, c->ustar, c->factor, c->cm0, c->kbar, c->sbar, c->k_cell,
                     c->theta_cell, c->sigma_cell, c->C_cell, c->t_phi, c->t_w, c->t_sigma_k, c->t_cos, c->t_sin,
-                    c->t_kpow, c->t_spow, c->t_wc, (void *)c->sX, (void *)c->sA, (void *)c->sB, (void *)c->rate,
+                    c->t_kpow, c->t_spow, c->t_wc, c->f_cth, c->f_sth, c->f_logk, c->f_invc, c->f_sk3, c->f_sq,
+                    c->f_logkap, c->f_invck, c->f_ctu, c->f_stu, c->f_lz1, (void *)c->sX, (void *)c->sA, (void *)c->sB, (void *)c->rate,
                     (void *)c->out_scale, (void *)c->s_err, (void *)c->use_plain, (void *)c->s_counts,
                     (void *)c->bin_hi, (void *)c->s_amax, (void *)c->s_flags})
         free_ptr(p);
@@ -1277,6 +1301,20 @@ int rpl_set_sources(rpl_ctx *c, const rpl_sources *src, const double *x_nodes, c
     for (void **b : p_bufs)
         if (!*b && (rc = alloc(b, np * c->esize)))
             return rc;
+    if (!is_f64(c)) {
+        void **f_col[] = {&c->f_ctu, &c->f_stu, &c->f_lz1};
+        for (void **b : f_col)
+            if (!*b && (rc = alloc(b, plane * sizeof(float))))
+                return rc;
+        void **f_tab[] = {&c->f_cth, &c->f_sth, &c->f_logk, &c->f_invc, &c->f_sk3, &c->f_sq};
+        for (void **b : f_tab)
+            if (!*b && (rc = alloc(b, tab * sizeof(float))))
+                return rc;
+        void **f_k[] = {&c->f_logkap, &c->f_invck};
+        for (void **b : f_k)
+            if (!*b && (rc = alloc(b, nk * sizeof(float))))
+                return rc;
+    }
     if (!c->xn && (rc = alloc((void **)&c->xn, nx * 8)))
         return rc;
     if (!c->yn && (rc = alloc((void **)&c->yn, ny * 8)))
@@ -1319,6 +1357,29 @@ int rpl_set_sources(rpl_ctx *c, const rpl_sources *src, const double *x_nodes, c
         spow[m] = s.dissipation == RPL_DISSIPATION_MEAN_SPECTRUM ? std::pow(sk[m], s.p_mean) : 0.0;
     }
     c->h_sigma = scell;
+    if (!is_f64(c)) {
+        // FP32 factors: cos/sin theta, log k, k / sigma, sigma k^3, sigma^q per
+        // bin; log kappa and kappa / sigma per kappa bin (state kernel).
+        std::vector<double> fc(tab), fs(tab), flk(tab), fic(tab), fsk3(tab), fsq(tab), flkap(nk), fick(nk);
+        for (size_t t = 0; t < tab; ++t) {
+            const double k = kcell[t], sg = scell[t];
+            fc[t] = std::cos(thcell[t]);
+            fs[t] = std::sin(thcell[t]);
+            flk[t] = k > 0 ? std::log(k) : 0.0;
+            fic[t] = sg > 0 ? k / sg : 0.0;
+            fsk3[t] = sg * (k * k * k);
+            fsq[t] = std::pow(sg, s.ls_sigma_power);
+        }
+        for (int m = 0; m < nk; ++m) {
+            flkap[m] = c->h_kc[m] > 0 ? std::log(c->h_kc[m]) : 0.0;
+            fick[m] = sk[m] > 0 ? c->h_kc[m] / sk[m] : 0.0;
+        }
+        if ((rc = upload_real(c, fc.data(), c->f_cth, tab)) || (rc = upload_real(c, fs.data(), c->f_sth, tab)) ||
+            (rc = upload_real(c, flk.data(), c->f_logk, tab)) || (rc = upload_real(c, fic.data(), c->f_invc, tab)) ||
+            (rc = upload_real(c, fsk3.data(), c->f_sk3, tab)) || (rc = upload_real(c, fsq.data(), c->f_sq, tab)) ||
+            (rc = upload_real(c, flkap.data(), c->f_logkap, nk)) || (rc = upload_real(c, fick.data(), c->f_invck, nk)))
+            return rc;
+    }
     if ((rc = upload_real(c, kcell.data(), c->k_cell, tab)) || (rc = upload_real(c, thcell.data(), c->theta_cell, tab)) ||
         (rc = upload_real(c, scell.data(), c->sigma_cell, tab)) || (rc = upload_real(c, Ccell.data(), c->C_cell, tab)) ||
         (rc = upload_real(c, w.data(), c->t_w, tab)) || (rc = upload_real(c, wcf.data(), c->t_wc, tab)) ||
@@ -1338,6 +1399,8 @@ int rpl_set_sources(rpl_ctx *c, const rpl_sources *src, const double *x_nodes, c
     rpl::WindParams &wp = c->wind;
     wp = rpl::WindParams{};
     wp.field = s.wind_field;
+    wp.alpha0 = s.alpha0;
+    wp.gravity = s.gravity;
     wp.drag = s.drag;
     wp.rotation = s.rotation >= 0 ? 1 : -1;
     wp.speed = s.wind_speed;

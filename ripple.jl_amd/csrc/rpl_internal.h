@@ -50,6 +50,7 @@ struct WindParams {
     double vmax, rmax, radius, shape, decay, bx, by;       // Holland
     double cos_inflow, sin_inflow;
     double t0, t1, x0, y0, x1, y1;                         // two-point storm track
+    double alpha0, gravity;                                // z1 = alpha0 u*^2 / g of the wind input
 };
 
 // Tables of the per-column state kernel (precomputed_sources.jl:41-74, 126-170).
@@ -58,6 +59,7 @@ struct StateTables {
     const T *kappa, *phi, *w;           // centres, cell measures (nk, nphi)
     const T *sigma_k, *cos_phi, *sin_phi;
     const T *k_pow, *sigma_pow;         // kappa^p, sigma^p (mean-spectrum moments)
+    const T *log_kappa, *inv_c;         // FP32 wind input: log kappa, kappa / sigma (per kappa bin)
     T rho_air, rho_w_g, cap_eps, inv_p;
 };
 
@@ -75,7 +77,7 @@ struct EnsembleState {
 
 template <typename T>
 hipError_t launch_wind_columns(int nx, int ny, const double *xn, const double *yn, double t, const WindParams &w,
-                               T *U10, T *theta_u, T *ustar, hipStream_t s);
+                               T *U10, T *theta_u, T *ustar, T *cos_tu, T *sin_tu, T *log_z1, hipStream_t s);
 template <typename T>
 hipError_t launch_source_rates(int nx, int ny, int nbins, const T *N, const SourceParams<T> &sp, const float *unit,
                                float *r, hipStream_t s);

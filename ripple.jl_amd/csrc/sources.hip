@@ -14,7 +14,8 @@ namespace {
 template <typename T>
 __global__ __launch_bounds__(256) void wind_columns_kernel(int nx, int ny, const double *__restrict__ xn,
                                                            const double *__restrict__ yn, double t, WindParams w,
-                                                           T *U10, T *theta_u, T *ustar)
+                                                           T *U10, T *theta_u, T *ustar, T *cos_tu, T *sin_tu,
+                                                           T *log_z1)
 {
     const size_t ij = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (ij >= (size_t)nx * ny)
@@ -65,9 +66,18 @@ __global__ __launch_bounds__(256) void wind_columns_kernel(int nx, int ny, const
         cd = (8.058 + 0.967 * speed - 0.016 * (speed * speed)) * 1e-4;
         cd = cd > 4e-4 ? cd : 4e-4;
     }
+    const double us = speed * sqrt(cd > 0 ? cd : 0.0);
     U10[ij] = (T)speed;
     theta_u[ij] = (T)angle;
-    ustar[ij] = (T)(speed * sqrt(cd > 0 ? cd : 0.0));
+    ustar[ij] = (T)us;
+    if (cos_tu) {
+        // Column factors of the FP32 wind input (jp_rate_f32); z1 from the
+        // stored u*, as the per-cell form computes it.
+        const double u = (double)(T)us, z1 = w.alpha0 * (u * u) / w.gravity;
+        cos_tu[ij] = (T)cos(angle);
+        sin_tu[ij] = (T)sin(angle);
+        log_z1[ij] = z1 > 0 ? (T)log(z1) : (T)INFINITY;
+    }
 }
 
 // precomputed_sources.jl:126-170 (stress factor) and :41-74 (mean-spectrum
@@ -173,11 +183,13 @@ __device__ __forceinline__ void state_block_f32(int nx, int ny, int nk, int nphi
     // with the network supplying the wind input, only the dissipation state is needed
     const bool cap = s.wind_input && s.stress_cap && !net, ms = s.dissipation == 1, wc = s.dissipation == 3;
     float twx = 0.0f, twy = 0.0f, m0 = 0.0f, kp = 0.0f, sp = 0.0f;
-    float U10 = 0.0f, ustar = 0.0f, theta_u = 0.0f;
+    float U10 = 0.0f, ustar = 0.0f, ctu = 0.0f, stu = 0.0f, lz1 = 0.0f;
     if (active && cap) {
         U10 = gld<true>(s.U10, ij);
         ustar = gld<true>(s.ustar, ij);
-        theta_u = gld<true>(s.theta_u, ij);
+        ctu = gld<true>(s.cos_tu, ij);
+        stu = gld<true>(s.sin_tu, ij);
+        lz1 = gld<true>(s.log_z1, ij);
     }
     const bool wind = cap && U10 > 0.0f && ustar > 0.0f;
     if (active && (wind || ms || wc)) {
@@ -194,10 +206,12 @@ __device__ __forceinline__ void state_block_f32(int nx, int ny, int nk, int nphi
                 m0 += a * gld<true>(tb.w, t);  // zeroth moment over every cell
             if (a > 0.0f && k > 0.0f) {
                 if (wind) {
-                    const float sigma = gld<true>(tb.sigma_k, m), C = sigma / k;
-                    const float rate = jp_rate<float>(s, k, sigma, C, gld<true>(tb.phi, n), theta_u, ustar);
+                    const float invc = gld<true>(tb.inv_c, m);
+                    const float rate = jp_rate_f32(s, gld<true>(tb.cos_phi, n), gld<true>(tb.sin_phi, n),
+                                                   gld<true>(tb.log_kappa, m), invc, gld<true>(tb.sigma_k, m), ctu,
+                                                   stu, lz1, ustar);
                     if (rate > 0.0f) {
-                        const float mflux = rate * a / C * gld<true>(tb.w, t) * tb.rho_w_g;
+                        const float mflux = rate * a * invc * gld<true>(tb.w, t) * tb.rho_w_g;
                         twx += mflux * gld<true>(tb.cos_phi, n);
                         twy += mflux * gld<true>(tb.sin_phi, n);
                     }
@@ -240,7 +254,7 @@ __device__ __forceinline__ void state_block_f32(int nx, int ny, int nk, int nphi
         float f = 1.0f;
         if (wind) {
             const float tau_max = tb.rho_air * (ustar * ustar);
-            const float par = twx * cosf(theta_u) + twy * sinf(theta_u);
+            const float par = twx * ctu + twy * stu;
             const float eff = par > 0.0f ? par : 0.0f;
             f = (eff > tb.cap_eps * tau_max && tau_max > 0.0f) ? tb.cap_eps * tau_max / eff : 1.0f;
         }
@@ -285,11 +299,11 @@ __global__ __launch_bounds__(256) void ensemble_state_kernel(const EnsembleState
 
 template <typename T>
 hipError_t launch_wind_columns(int nx, int ny, const double *xn, const double *yn, double t, const WindParams &w,
-                               T *U10, T *theta_u, T *ustar, hipStream_t s)
+                               T *U10, T *theta_u, T *ustar, T *cos_tu, T *sin_tu, T *log_z1, hipStream_t s)
 {
     const size_t plane = (size_t)nx * ny;
     hipLaunchKernelGGL(wind_columns_kernel<T>, dim3((unsigned)((plane + 255) / 256)), dim3(256), 0, s, nx, ny, xn, yn,
-                       t, w, U10, theta_u, ustar);
+                       t, w, U10, theta_u, ustar, cos_tu, sin_tu, log_z1);
     return hipGetLastError();
 }
 
@@ -349,9 +363,10 @@ hipError_t launch_source_state(int nx, int ny, int nk, int nphi, const T *N, con
 }
 
 template hipError_t launch_wind_columns<float>(int, int, const double *, const double *, double, const WindParams &,
-                                              float *, float *, float *, hipStream_t);
+                                              float *, float *, float *, float *, float *, float *, hipStream_t);
 template hipError_t launch_wind_columns<double>(int, int, const double *, const double *, double, const WindParams &,
-                                               double *, double *, double *, hipStream_t);
+                                               double *, double *, double *, double *, double *, double *,
+                                               hipStream_t);
 hipError_t launch_ensemble_state(const EnsembleState *mem, int members, long long blocks, hipStream_t s)
 {
     if (members < 1 || blocks < 1 || blocks > 0x7fffffffLL)

@@ -16,6 +16,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 namespace rpl {
 
 template <typename T>
@@ -36,6 +38,11 @@ struct SourceParams {
     T ls_cds, ls_br, ls_p, ls_q; // local saturation
     T wc_rate, wc_thr, wc_sp;    // whitecapping: rate, saturation threshold and power
     const T *wc_factor;          // whitecapping: <kappa^p>_cell / kappa_ref^p per bin
+    // FP32 forms (null in FP64, which keeps the reference's operation order):
+    // per bin cos and sin of theta, log k, k / sigma (= 1 / C), sigma k^3 and
+    // sigma^q; per column cos and sin of theta_u and log z1 (wind kernel).
+    const T *cth_cell, *sth_cell, *logk_cell, *invc_cell, *sk3_cell, *sq_cell;
+    const T *cos_tu, *sin_tu, *log_z1;
     // learned surrogate: where surrogate != 0 and use_plain[col] has neither
     // bit 0 (outside the training envelope) nor bit 2 (failed audit), the
     // network term is rate[col + rate_ld * t] * out_scale[t] * N; it is the
@@ -100,27 +107,138 @@ __device__ __forceinline__ T jp_rate(const SourceParams<T> &s, T k, T sigma, T C
     return s.c0 * dexp(Z) * (Z2 * Z2) * (inv_age * inv_age) * cp * sigma;
 }
 
-// Source tendency of one cell: positive - damping * a.  col is the column
-// index i + nx*j, t the bin index m + nk*n.
-template <typename T, bool AS1 = false>
-__device__ __forceinline__ T source_term(const SourceParams<T> &s, unsigned col, int t, T a)
+// FP32 _jp_rate from tabulated factors: cos(theta - theta_u) by the
+// angle-difference identity, log(k z1) = log k + log z1, u*/C = u* (k /
+// sigma), and one reciprocal for kvk / (inv_age cos).  The same formula as
+// jp_rate with FP32 rounding in different places (a few ulp); no range
+// reduction, log or divide per cell.  log_z1 is +inf where z1 <= 0, which
+// gives Z = +inf and the zero rate of the growth window.
+__device__ __forceinline__ float jp_rate_f32(const SourceParams<float> &s, float cth, float sth, float logk,
+                                             float invc, float sigma, float ctu, float stu, float lz1, float ustar)
 {
 #pragma clang fp contract(off)
-    const bool net = s.surrogate && !(gld<AS1>(s.use_plain, col) & 5);
+    const float cos_tu = fmaf(cth, ctu, sth * stu);
+    if (!(cos_tu > 0.0f))
+        return 0.0f;
+    const float inv_age = fmaf(ustar, invc, s.z_alpha);
+    const float Z = (logk + lz1) + s.kvk * __builtin_amdgcn_rcpf(inv_age * cos_tu);
+    if (!(Z < 0.0f))
+        return 0.0f;
+    const float cp = s.p_in == 2.0f ? cos_tu * cos_tu : powf(cos_tu, s.p_in);
+    const float Z2 = Z * Z;
+    return s.c0 * expf(Z) * (Z2 * Z2) * (inv_age * inv_age) * cp * sigma;
+}
+
+// The per-column inputs of the per-cell source (what the wind kernel and the
+// state kernel wrote for column col), loaded once so that a kernel walking
+// several bins of one column reads them once.
+template <typename T>
+struct SourceColumn {
+    T U10, ustar, theta_u, factor, m0, kbar, sbar;
+    T cos_tu, sin_tu, log_z1;  // FP32 wind-input factors
+    bool net;                  // the network supplies this column's source
+};
+
+template <typename T, bool AS1 = false>
+__device__ __forceinline__ SourceColumn<T> source_column(const SourceParams<T> &s, unsigned col)
+{
+    SourceColumn<T> c{};
+    c.net = s.surrogate && !(gld<AS1>(s.use_plain, col) & 5);
+    if (c.net && s.surrogate_terms == 0)
+        return c;
+    if (s.wind_input && !c.net) {
+        c.U10 = gld<AS1>(s.U10, col);
+        c.ustar = gld<AS1>(s.ustar, col);
+        if constexpr (std::is_same<T, float>::value) {
+            c.cos_tu = gld<AS1>(s.cos_tu, col);
+            c.sin_tu = gld<AS1>(s.sin_tu, col);
+            c.log_z1 = gld<AS1>(s.log_z1, col);
+        } else {
+            c.theta_u = gld<AS1>(s.theta_u, col);
+        }
+        if (s.stress_cap)
+            c.factor = gld<AS1>(s.factor, col);
+    }
+    if (s.dissipation == 1) {
+        c.m0 = gld<AS1>(s.m0, col);
+        c.kbar = gld<AS1>(s.kbar, col);
+        c.sbar = gld<AS1>(s.sbar, col);
+    } else if (s.dissipation == 3) {
+        c.m0 = gld<AS1>(s.m0, col);
+    }
+    return c;
+}
+
+// The per-bin inputs of the per-cell source (tables indexed by t = m + nk*n),
+// loaded together so that a kernel can issue them with its stencil loads.
+template <typename T>
+struct SourceBin {
+    T k, sigma, C, theta;                // reference-order inputs (FP64)
+    T cth, sth, logk, invc, sk3, sq;     // FP32 factors
+    T wcf, out_scale;
+};
+
+template <typename T, bool AS1 = false>
+__device__ __forceinline__ SourceBin<T> source_bin(const SourceParams<T> &s, int t)
+{
+    // Every table the plain bundle uses: with a network attached, columns
+    // outside its envelope still take the plain source.
+    SourceBin<T> b{};
+    if (s.surrogate)
+        b.out_scale = gld<AS1>(s.out_scale, t);
+    b.k = gld<AS1>(s.k_cell, t);
+    if (s.wind_input) {
+        if constexpr (std::is_same<T, float>::value) {
+            b.cth = gld<AS1>(s.cth_cell, t);
+            b.sth = gld<AS1>(s.sth_cell, t);
+            b.logk = gld<AS1>(s.logk_cell, t);
+            b.invc = gld<AS1>(s.invc_cell, t);
+            b.sigma = gld<AS1>(s.sigma_cell, t);
+        } else {
+            b.sigma = gld<AS1>(s.sigma_cell, t);
+            b.C = gld<AS1>(s.C_cell, t);
+            b.theta = gld<AS1>(s.theta_cell, t);
+        }
+    }
+    if (s.dissipation == 2) {
+        if constexpr (std::is_same<T, float>::value) {
+            b.sk3 = gld<AS1>(s.sk3_cell, t);
+            b.sq = gld<AS1>(s.sq_cell, t);
+        } else {
+            b.sigma = gld<AS1>(s.sigma_cell, t);
+        }
+    } else if (s.dissipation == 3) {
+        b.wcf = gld<AS1>(s.wc_factor, t);
+    }
+    return b;
+}
+
+// Source tendency of one cell of column col and bin t (inputs c and b):
+// positive - damping * a.
+template <typename T, bool AS1 = false>
+__device__ __forceinline__ T source_term_cb(const SourceParams<T> &s, const SourceColumn<T> &c, const SourceBin<T> &b,
+                                            unsigned col, int t, T a)
+{
+#pragma clang fp contract(off)
+    const bool net = c.net;
     T learned = T(0);
     if (net) {
-        learned = (T)(gld<AS1>(s.rate, col + s.rate_ld * (size_t)t) * gld<AS1>(s.out_scale, t)) * a;
+        learned = (T)(gld<AS1>(s.rate, col + s.rate_ld * (size_t)t) * b.out_scale) * a;
         if (s.surrogate_terms == 0)
             return learned;
     }
     T positive = T(0), damping = T(0);
-    const T k = gld<AS1>(s.k_cell, t);
+    const T k = b.k;
     if (s.wind_input && !net) {
-        const T U10 = gld<AS1>(s.U10, col), ustar = gld<AS1>(s.ustar, col);
+        const T U10 = c.U10, ustar = c.ustar;
         if (U10 > T(0) && ustar > T(0) && k > T(0)) {
-            const T raw = jp_rate<T>(s, k, gld<AS1>(s.sigma_cell, t), gld<AS1>(s.C_cell, t), gld<AS1>(s.theta_cell, t), gld<AS1>(s.theta_u, col), ustar);
+            T raw;
+            if constexpr (std::is_same<T, float>::value)
+                raw = jp_rate_f32(s, b.cth, b.sth, b.logk, b.invc, b.sigma, c.cos_tu, c.sin_tu, c.log_z1, ustar);
+            else
+                raw = jp_rate<T>(s, k, b.sigma, b.C, b.theta, c.theta_u, ustar);
             if (raw > T(0)) {
-                const T rate = s.stress_cap ? raw * gld<AS1>(s.factor, col) : raw;
+                const T rate = s.stress_cap ? raw * c.factor : raw;
                 if (rate >= T(0))
                     positive += rate * a;
                 else
@@ -129,12 +247,12 @@ __device__ __forceinline__ T source_term(const SourceParams<T> &s, unsigned col,
         }
     }
     if (s.dissipation == 1) {
-        const T m0 = gld<AS1>(s.m0, col), kb = gld<AS1>(s.kbar, col);
+        const T m0 = c.m0, kb = c.kbar;
         if (m0 > T(0) && kb > T(0) && k > T(0)) {
             const T ab = m0 * (kb * kb);
             const T ratio = k / kb;
             const T bracket = s.d1 * ratio + s.d2 * (ratio * ratio);
-            const T rate = -s.cds * (ab * ab) * gld<AS1>(s.sbar, col) * bracket;
+            const T rate = -s.cds * (ab * ab) * c.sbar * bracket;
             if (rate >= T(0))
                 damping += rate;
             else
@@ -142,13 +260,20 @@ __device__ __forceinline__ T source_term(const SourceParams<T> &s, unsigned col,
         }
     } else if (s.dissipation == 2) {
         if (a > T(0) && k > T(0)) {
-            const T sigma = gld<AS1>(s.sigma_cell, t);
-            const T B = a * sigma * (k * k * k);
+            T B, sq;
+            if constexpr (std::is_same<T, float>::value) {
+                B = a * b.sk3;  // sigma k^3 and sigma^q tabulated
+                sq = b.sq;
+            } else {
+                B = a * b.sigma * (k * k * k);
+                sq = dpow(b.sigma, s.ls_q);
+            }
             T over = B / s.ls_br - T(1);
             over = over > T(0) ? over : T(0);
             if (over > T(0)) {
                 over = over < T(1e6) ? over : T(1e6);
-                const T rate = -s.ls_cds * dpow(sigma, s.ls_q) * dpow(over, s.ls_p);
+                const T op = std::is_same<T, float>::value && s.ls_p == T(2) ? over * over : dpow(over, s.ls_p);
+                const T rate = -s.ls_cds * sq * op;
                 if (rate >= T(0))
                     damping += rate;
                 else
@@ -157,10 +282,9 @@ __device__ __forceinline__ T source_term(const SourceParams<T> &s, unsigned col,
         }
     } else if (s.dissipation == 3) {
         // Sources.jl:588-598 with the column's zeroth moment from the state kernel.
-        const T m0 = gld<AS1>(s.m0, col);
-        T excess = m0 / s.wc_thr - T(1);
+        T excess = c.m0 / s.wc_thr - T(1);
         excess = excess > T(0) ? excess : T(0);
-        const T rate = s.wc_rate * dpow(excess, s.wc_sp) * gld<AS1>(s.wc_factor, t);
+        const T rate = s.wc_rate * dpow(excess, s.wc_sp) * b.wcf;
         if (rate >= T(0))
             damping += rate;
         else
@@ -169,6 +293,14 @@ __device__ __forceinline__ T source_term(const SourceParams<T> &s, unsigned col,
     if (net)
         return (positive - damping * a) + learned;
     return positive - damping * a;
+}
+
+// Source tendency of one cell: positive - damping * a.  col is the column
+// index i + nx*j, t the bin index m + nk*n.
+template <typename T, bool AS1 = false>
+__device__ __forceinline__ T source_term(const SourceParams<T> &s, unsigned col, int t, T a)
+{
+    return source_term_cb<T, AS1>(s, source_column<T, AS1>(s, col), source_bin<T, AS1>(s, t), col, t, a);
 }
 
 }  // namespace rpl

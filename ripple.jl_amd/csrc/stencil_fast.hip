@@ -232,11 +232,18 @@ struct Tile {
     int ib, jb, m, n;
 };
 
-__device__ __forceinline__ Tile tile_of(int b, int total, int nbx, int nby, int nk, int nphi)
+// Linear work index of block b: XCD b % 8 takes the b / 8-th entry of its
+// contiguous range of the total.
+__device__ __forceinline__ int xcd_linear(int b, int total)
 {
     const int xcd = b & 7, r = b >> 3;
     const int per = total >> 3, rem = total & 7;
-    const int L = xcd < rem ? xcd * (per + 1) + r : rem * (per + 1) + (xcd - rem) * per + r;
+    return xcd < rem ? xcd * (per + 1) + r : rem * (per + 1) + (xcd - rem) * per + r;
+}
+
+__device__ __forceinline__ Tile tile_of(int b, int total, int nbx, int nby, int nk, int nphi)
+{
+    const int L = xcd_linear(b, total);
     Tile t;
     t.n = L % nphi;
     int q = L / nphi;
@@ -363,27 +370,51 @@ __device__ __forceinline__ void stage_tile(const float *__restrict__ Nk_, const 
 #endif
 constexpr int STAGE_ROWS = RPL_STAGE_ROWS;
 
+// Work order of the transports without spectral coupling (intrinsic, none):
+// every bin is an independent 2-D problem, so each XCD's contiguous range
+// walks tile columns fastest, then tile rows, then bins.  The tiles in flight
+// on one XCD then cover whole rows of one bin and share their x and y halos
+// in that XCD's L2 (the refraction order, bins fastest, would fetch every
+// tile's halo again).
+__device__ __forceinline__ Tile tile_of_spatial(int b, int total, int nbx, int nby, int nk)
+{
+    const int L = xcd_linear(b, total);
+    Tile t;
+    t.ib = L % nbx;
+    int q = L / nbx;
+    t.jb = q % nby;
+    q /= nby;
+    t.m = q % nk;
+    t.n = q / nk;
+    return t;
+}
+
+// Per-bin stage of the intrinsic and no-transport models.
 template <int TRANSPORT, int STAGE, bool SRC>
 __global__ __launch_bounds__(64 * STAGE_ROWS) void stage_fast_kernel(const float *__restrict__ Nk,
                                                                      const float *__restrict__ N0, float *out,
                                                                      StageParams<float> p, int nbx, int nby)
 {
-    const Tile t = tile_of(blockIdx.x, nbx * nby * p.nk * p.nphi, nbx, nby, p.nk, p.nphi);
+    const Tile t = tile_of_spatial(blockIdx.x, nbx * nby * p.nk * p.nphi, nbx, nby, p.nk);
     stage_tile<TRANSPORT, STAGE, SRC, STAGE_ROWS, false>(Nk, N0, out, p, t.ib, t.jb, t.m, t.n);
 }
 
-// Refraction stage with the phi bins looped inside the workgroup: one
-// workgroup per 64 x TY (i, j) tile of one kappa bin, walking n = 0..nphi-1.
-// What depends on (i, j, m) only -- the Doppler velocity and its four
-// centred gradients -- is loaded and formed once instead of once per bin,
-// and the phi stencil is a register window that takes one new load per bin.
-// The arithmetic of every cell is that of stage_tile, so the results are the
-// same bit for bit.  Shared by the per-model kernel and the ensemble
-// work-list kernel (AS1: p and the slab pointers came from memory).
-template <int STAGE, bool SRC, int TY, bool AS1>
+// Stage with the phi bins looped inside the workgroup: one workgroup per
+// 64 x TY (i, j) tile of one kappa bin, walking n = 0..nphi-1.
+// Refraction: what depends on (i, j, m) only -- the Doppler velocity and its
+// four centred gradients -- is loaded and formed once instead of once per
+// bin, and the phi stencil is a register window that takes one new load per
+// bin.  Intrinsic transport: x/y WENO only, with the column inputs of the
+// source terms loaded once for the 36 bins.  The arithmetic of every cell is
+// that of stage_tile, so the results are the same bit for bit.  Shared by the
+// per-model kernel and the ensemble work-list kernel (AS1: p and the slab
+// pointers came from memory).
+template <int TRANSPORT, int STAGE, bool SRC, int TY, bool AS1>
 __device__ __forceinline__ void phi_tile(const float *__restrict__ Nk, const float *__restrict__ N0, float *out,
                                          const StageParams<float> &p, int ib, int jb, int m)
 {
+    constexpr bool REF = TRANSPORT == RPL_TRANSPORT_REFRACTION;
+    static_assert(REF || TRANSPORT == RPL_TRANSPORT_INTRINSIC, "phi_tile: refraction or intrinsic transport");
     const int nx = p.nx, ny = p.ny, nk = p.nk, nphi = p.nphi;
     const int i0 = ib * 64, j0 = jb * TY;
     const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
@@ -397,27 +428,36 @@ __device__ __forceinline__ void phi_tile(const float *__restrict__ Nk, const flo
     const bool y_inner = j0 >= 3 && j0 + TY + 3 <= ny;
     const unsigned rowb = (nxu * (unsigned)j) << 2, colb = (unsigned)i << 2;
     const unsigned sx = 4u, sy = nxu << 2;
-    // Doppler velocity and gradients at (i, j, m) (wave_current_refraction.jl:7-19).
-    const float *Ux = p.Ux + plane * m, *Uy = p.Uy + plane * m;
-    const unsigned xp = x_inner ? bo + sx : rowb + ((unsigned)wrap(i + 1, nx) << 2);
-    const unsigned xm = x_inner ? bo - sx : rowb + ((unsigned)wrap(i - 1, nx) << 2);
-    const unsigned yp = y_inner ? bo + sy : colb + sy * (unsigned)wrap(j + 1, ny);
-    const unsigned ym = y_inner ? bo - sy : colb + sy * (unsigned)wrap(j - 1, ny);
-    const float hx = 0.5f * p.dx_inv, hy = 0.5f * p.dy_inv;
-    const float Uxx = (ldo<AS1>(Ux, xp) - ldo<AS1>(Ux, xm)) * hx;
-    const float Uxy = (ldo<AS1>(Ux, yp) - ldo<AS1>(Ux, ym)) * hy;
-    const float Uyx = (ldo<AS1>(Uy, xp) - ldo<AS1>(Uy, xm)) * hx;
-    const float Uyy = (ldo<AS1>(Uy, yp) - ldo<AS1>(Uy, ym)) * hy;
-    const float Uxc = ldo<AS1>(Ux, bo), Uyc = ldo<AS1>(Uy, bo);
-    const float kap = gld<AS1>(p.kappa, m);
-    // What bin n reads besides the phi window: its x, y and kappa stencils,
-    // the window's next value f(n + 4), and N0 and the added source at the
-    // cell.  The latency of these loads is hidden by the other waves of the
-    // SIMD, so occupancy (registers per lane) decides the speed: issuing
-    // bin n + 1's loads ahead of bin n's arithmetic costs 20 more registers
-    // and runs slower.
+    float Uxx = 0.0f, Uxy = 0.0f, Uyx = 0.0f, Uyy = 0.0f, Uxc = 0.0f, Uyc = 0.0f, kap = 0.0f;
+    if constexpr (REF) {
+        // Doppler velocity and gradients at (i, j, m) (wave_current_refraction.jl:7-19).
+        const float *Ux = p.Ux + plane * m, *Uy = p.Uy + plane * m;
+        const unsigned xp = x_inner ? bo + sx : rowb + ((unsigned)wrap(i + 1, nx) << 2);
+        const unsigned xm = x_inner ? bo - sx : rowb + ((unsigned)wrap(i - 1, nx) << 2);
+        const unsigned yp = y_inner ? bo + sy : colb + sy * (unsigned)wrap(j + 1, ny);
+        const unsigned ym = y_inner ? bo - sy : colb + sy * (unsigned)wrap(j - 1, ny);
+        const float hx = 0.5f * p.dx_inv, hy = 0.5f * p.dy_inv;
+        Uxx = (ldo<AS1>(Ux, xp) - ldo<AS1>(Ux, xm)) * hx;
+        Uxy = (ldo<AS1>(Ux, yp) - ldo<AS1>(Ux, ym)) * hy;
+        Uyx = (ldo<AS1>(Uy, xp) - ldo<AS1>(Uy, xm)) * hx;
+        Uyy = (ldo<AS1>(Uy, yp) - ldo<AS1>(Uy, ym)) * hy;
+        Uxc = ldo<AS1>(Ux, bo);
+        Uyc = ldo<AS1>(Uy, bo);
+        kap = gld<AS1>(p.kappa, m);
+    }
+    SourceColumn<float> scol{};
+    if (SRC)
+        scol = source_column<float, AS1>(p.src, off);
+    // What bin n reads: its x and y stencils, for refraction its kappa
+    // stencil and the phi window's next value f(n + 4), otherwise the cell
+    // itself, N0 and the added source at the cell, and the bin's source
+    // tables.  The latency of these
+    // loads is hidden by the other waves of the SIMD, so occupancy (registers
+    // per lane) decides the speed: issuing bin n + 1's loads ahead of bin n's
+    // arithmetic costs 20 more registers and runs slower.
     struct BinLoads {
-        float fx[7], fy[7], fk[7], wnext, n0, add;
+        float fx[7], fy[7], fk[7], wnext, x0, n0, add;
+        SourceBin<float> sb;
     };
     const unsigned cell_off = bo;
     auto load_bin = [&](int n, BinLoads &b) {
@@ -451,61 +491,74 @@ __device__ __forceinline__ void phi_tile(const float *__restrict__ Nk, const flo
                 if (d != 3)
                     b.fy[d] = ldo<AS1>(slab, colb + sy * (unsigned)wrap(j + d - 3, ny));
         }
+        if constexpr (REF) {
 #pragma unroll
-        for (int d = 0; d < 7; ++d)
-            if (d != 3)
-                b.fk[d] = ldo<AS1>(Nk + plane * ((size_t)clampi(m + d - 3, nk) + (size_t)nk * n), bo);
-        b.wnext = ldo<AS1>(Nk + plane * ((size_t)m + (size_t)nk * wrap(n + 4, nphi)), bo);
+            for (int d = 0; d < 7; ++d)
+                if (d != 3)
+                    b.fk[d] = ldo<AS1>(Nk + plane * ((size_t)clampi(m + d - 3, nk) + (size_t)nk * n), bo);
+            b.wnext = ldo<AS1>(Nk + plane * ((size_t)m + (size_t)nk * wrap(n + 4, nphi)), bo);
+        } else {
+            b.x0 = ldo<AS1>(slab, bo);
+        }
         const size_t bin = (size_t)m + (size_t)nk * n;
         b.n0 = (STAGE == STAGE_2 || STAGE == STAGE_3) ? ldo<AS1>(N0 + plane * bin, bo) : 0.0f;
         b.add = p.src_add ? ldo<AS1>(p.src_add + plane * bin, bo) : 0.0f;
+        if (SRC)
+            b.sb = source_bin<float, AS1>(p.src, (int)bin);
     };
-    // phi window f(n - 3 .. n + 3) at this cell, periodic in phi
-    float w[7];
+    // phi window f(n - 3 .. n + 3) at this cell, periodic in phi (refraction)
+    float w[7] = {};
+    if constexpr (REF) {
 #pragma unroll
-    for (int d = 0; d < 7; ++d)
-        w[d] = ldo<AS1>(Nk + plane * ((size_t)m + (size_t)nk * wrap(d - 3, nphi)), bo);
+        for (int d = 0; d < 7; ++d)
+            w[d] = ldo<AS1>(Nk + plane * ((size_t)m + (size_t)nk * wrap(d - 3, nphi)), bo);
+    }
     Face6 carry = {0.0f, 0.0f};  // phi face n - 1/2 (as (v, f) of weno_face_of5)
     bool cpos = true;            // its upwind side
     for (int n = 0; n < nphi; ++n) {
         BinLoads cur;
         load_bin(n, cur);
         const size_t bin = (size_t)m + (size_t)nk * n;
-        const float x0 = w[3];
+        const float x0 = REF ? w[3] : cur.x0;
         float ux = gld<AS1>(p.cgx, m + nk * n), uy = gld<AS1>(p.cgy, m + nk * n);
-        const float cs = gld<AS1>(p.cosp, n), sn = gld<AS1>(p.sinp, n);
-        const float cc = cs * cs, ss = sn * sn, csn = cs * sn;
-        const float ck = -kap * (cc * Uxx + csn * (Uyx + Uxy) + ss * Uyy);
-        const float cp = csn * (Uxx - Uyy) + ss * Uyx - cc * Uxy;
-        ux += Uxc;
-        uy += Uyc;
+        float ck = 0.0f, cp = 0.0f;
+        if constexpr (REF) {
+            const float cs = gld<AS1>(p.cosp, n), sn = gld<AS1>(p.sinp, n);
+            const float cc = cs * cs, ss = sn * sn, csn = cs * sn;
+            ck = -kap * (cc * Uxx + csn * (Uyx + Uxy) + ss * Uyy);
+            cp = csn * (Uxx - Uyy) + ss * Uyx - cc * Uxy;
+            ux += Uxc;
+            uy += Uyc;
+        }
         const float *fx = cur.fx, *fy = cur.fy;
         const float dx = upwind_difference(fx[0], fx[1], fx[2], x0, fx[4], fx[5], fx[6], ux);
         const float dy = upwind_difference(fy[0], fy[1], fy[2], x0, fy[4], fy[5], fy[6], uy);
         float L = -fmaf(dx, p.dx_inv, dy * p.dy_inv);
-        // phi faces: the face n + 1/2 of this bin is the face n - 1/2 of the
-        // next one.  When every lane keeps its upwind side from the previous
-        // bin, only the new face is evaluated.
-        const bool ppos = cp >= 0.0f;
-        float d6;
-        if (n > 0 && __all(ppos == cpos)) {
-            const Face6 nf = weno_face_of5(ppos ? w[1] : w[6], ppos ? w[2] : w[5], ppos ? x0 : w[4],
-                                           ppos ? w[4] : x0, ppos ? w[5] : w[2]);
-            d6 = ppos ? (nf.v - carry.v) + (nf.f - carry.f) : (carry.v - nf.v) + (carry.f - nf.f);
-            carry = nf;
-        } else {
-            Face6 a, b;
-            weno_pair_parts(ppos ? w[0] : w[6], ppos ? w[1] : w[5], ppos ? w[2] : w[4], x0, ppos ? w[4] : w[2],
-                            ppos ? w[5] : w[1], a, b);
-            d6 = (b.v - a.v) + (b.f - a.f);
-            carry = ppos ? b : a;
+        if constexpr (REF) {
+            // phi faces: the face n + 1/2 of this bin is the face n - 1/2 of
+            // the next one.  When every lane keeps its upwind side from the
+            // previous bin, only the new face is evaluated.
+            const bool ppos = cp >= 0.0f;
+            float d6;
+            if (n > 0 && __all(ppos == cpos)) {
+                const Face6 nf = weno_face_of5(ppos ? w[1] : w[6], ppos ? w[2] : w[5], ppos ? x0 : w[4],
+                                               ppos ? w[4] : x0, ppos ? w[5] : w[2]);
+                d6 = ppos ? (nf.v - carry.v) + (nf.f - carry.f) : (carry.v - nf.v) + (carry.f - nf.f);
+                carry = nf;
+            } else {
+                Face6 a, b;
+                weno_pair_parts(ppos ? w[0] : w[6], ppos ? w[1] : w[5], ppos ? w[2] : w[4], x0,
+                                ppos ? w[4] : w[2], ppos ? w[5] : w[1], a, b);
+                d6 = (b.v - a.v) + (b.f - a.f);
+                carry = ppos ? b : a;
+            }
+            cpos = ppos;
+            const float dp = fabsf(cp) * (1.0f / 6.0f) * d6;
+            const float dk = kappa_difference(cur.fk, x0, ck, m, nk);
+            L = L - fmaf(dk, p.dk_inv, dp * p.dp_inv);
         }
-        cpos = ppos;
-        const float dp = fabsf(cp) * (1.0f / 6.0f) * d6;
-        const float dk = kappa_difference(cur.fk, x0, ck, m, nk);
-        L = L - fmaf(dk, p.dk_inv, dp * p.dp_inv);
         if (SRC)  // tendencies.jl:88-94, G += source_tendency
-            L += source_term<float, AS1>(p.src, off, m + nk * n, x0);
+            L += source_term_cb<float, AS1>(p.src, scol, cur.sb, off, m + nk * n, x0);
         if (p.src_add)
             L += cur.add;
         float r;
@@ -520,26 +573,33 @@ __device__ __forceinline__ void phi_tile(const float *__restrict__ Nk, const flo
             r = clip0(fmaf(a, cur.n0, b * st));
         }
         sto<AS1>(out + plane * bin, bo, r);
+        if constexpr (REF) {
 #pragma unroll
-        for (int d = 0; d < 6; ++d)
-            w[d] = w[d + 1];
-        w[6] = cur.wnext;
+            for (int d = 0; d < 6; ++d)
+                w[d] = w[d + 1];
+            w[6] = cur.wnext;
+        }
     }
 }
 
-
-// Per-model refraction stage: work order XCD-aware as tile_of, kappa
-// fastest, so the kappa neighbours a tile reads are processed together on one
-// XCD.
-template <int STAGE, bool SRC>
+// Per-model phi-loop stage.  Work order (XCD-aware, each XCD one contiguous
+// range): refraction kappa fastest, then tile row and column, so the kappa
+// neighbours a tile reads are processed together on one XCD; intrinsic
+// transport has no spectral coupling and walks tile columns fastest, then
+// tile rows, then kappa, so the tiles in flight share their x and y halos.
+template <int TRANSPORT, int STAGE, bool SRC>
 __global__ __launch_bounds__(256) void stage_phi_kernel(const float *__restrict__ Nk, const float *__restrict__ N0,
                                                         float *out, StageParams<float> p, int nbx, int nby)
 {
-    const Tile t = tile_of(blockIdx.x, nbx * nby * p.nk, nbx, nby, p.nk, 1);
-    phi_tile<STAGE, SRC, 4, false>(Nk, N0, out, p, t.ib, t.jb, t.m);
+    Tile t;
+    if constexpr (TRANSPORT == RPL_TRANSPORT_REFRACTION)
+        t = tile_of(blockIdx.x, nbx * nby * p.nk, nbx, nby, p.nk, 1);
+    else
+        t = tile_of_spatial(blockIdx.x, nbx * nby * p.nk, nbx, nby, p.nk);
+    phi_tile<TRANSPORT, STAGE, SRC, 4, false>(Nk, N0, out, p, t.ib, t.jb, t.m);
 }
 
-template <bool SRC>
+template <int TRANSPORT, bool SRC>
 hipError_t launch_phi(const StageParams<float> &p, int stage, const float *Nk, const float *N0, float *out,
                       hipStream_t s)
 {
@@ -550,16 +610,17 @@ hipError_t launch_phi(const StageParams<float> &p, int stage, const float *Nk, c
     dim3 block(64, 4, 1), grid((unsigned)total, 1, 1);
     switch (stage) {
     case STAGE_TENDENCY:
-        hipLaunchKernelGGL((stage_phi_kernel<STAGE_TENDENCY, SRC>), grid, block, 0, s, Nk, N0, out, p, nbx, nby);
+        hipLaunchKernelGGL((stage_phi_kernel<TRANSPORT, STAGE_TENDENCY, SRC>), grid, block, 0, s, Nk, N0, out, p,
+                           nbx, nby);
         break;
     case STAGE_1:
-        hipLaunchKernelGGL((stage_phi_kernel<STAGE_1, SRC>), grid, block, 0, s, Nk, N0, out, p, nbx, nby);
+        hipLaunchKernelGGL((stage_phi_kernel<TRANSPORT, STAGE_1, SRC>), grid, block, 0, s, Nk, N0, out, p, nbx, nby);
         break;
     case STAGE_2:
-        hipLaunchKernelGGL((stage_phi_kernel<STAGE_2, SRC>), grid, block, 0, s, Nk, N0, out, p, nbx, nby);
+        hipLaunchKernelGGL((stage_phi_kernel<TRANSPORT, STAGE_2, SRC>), grid, block, 0, s, Nk, N0, out, p, nbx, nby);
         break;
     default:
-        hipLaunchKernelGGL((stage_phi_kernel<STAGE_3, SRC>), grid, block, 0, s, Nk, N0, out, p, nbx, nby);
+        hipLaunchKernelGGL((stage_phi_kernel<TRANSPORT, STAGE_3, SRC>), grid, block, 0, s, Nk, N0, out, p, nbx, nby);
         break;
     }
     return hipGetLastError();
@@ -570,13 +631,14 @@ hipError_t launch_phi(const StageParams<float> &p, int stage, const float *Nk, c
 // prefix sum).  The grid is persistent.  Workgroups are dealt round-robin to
 // the eight XCDs, so XCD x takes the contiguous range [x T / 8, (x+1) T / 8)
 // of the T tiles and its workgroups sweep it together, each taking every
-// (workgroups per XCD)-th tile.  Within a member the order is phi fastest,
-// then kappa, tile row, tile column (as in tile_of), so the workgroups of one
-// XCD work on neighbouring bins of the same (x, y) tile at the same time and
-// share the stencil's phi and kappa neighbours in that XCD's L2.  The member
-// is looked up (binary search over tile0) only when a workgroup's next tile
-// lies past the member it last worked on, starting from a table that gives
-// the member of every 2^ENSEMBLE_CHUNK_SHIFT-th tile.
+// (workgroups per XCD)-th tile.  This kernel serves the transports without
+// spectral coupling (refraction members go to ensemble_phi_kernel); within a
+// member the order is tile column fastest, then tile row, kappa, phi (as in
+// tile_of_spatial), so the workgroups of one XCD work on neighbouring tiles
+// of one bin at the same time and share their x and y halos in that XCD's
+// L2.  The member is looked up (binary search over tile0) only when a
+// workgroup's next tile lies past the member it last worked on, starting from
+// a table that gives the member of every 2^ENSEMBLE_CHUNK_SHIFT-th tile.
 template <int TRANSPORT, int STAGE, bool SRC, int TY>
 __global__ __launch_bounds__(64 * TY) void ensemble_stage_kernel(const EnsembleMember *__restrict__ mem, int members,
                                                                  long long total,
@@ -601,21 +663,22 @@ __global__ __launch_bounds__(64 * TY) void ensemble_stage_kernel(const EnsembleM
         // The tile coordinates are uniform across the workgroup; saying so keeps
         // the member's parameters and the slab bases in scalar registers.
         int q = __builtin_amdgcn_readfirstlane((int)(L - a_begin));
-        const int n = q % e.p.nphi;
-        q /= e.p.nphi;
-        const int m = q % e.p.nk;
-        q /= e.p.nk;
-        const int jb = q % e.nby, ib = q / e.nby;
+        const int ib = q % e.nbx;
+        q /= e.nbx;
+        const int jb = q % e.nby;
+        q /= e.nby;
+        const int m = q % e.p.nk, n = q / e.p.nk;
         stage_tile<TRANSPORT, STAGE, SRC, TY, true>(e.Nk, e.N0, e.out, e.p, ib, jb, m, n);
     }
 }
 
-// Ensemble refraction stages: the same work list with one entry per
-// (64 x TY tile, kappa bin) of a member, ordered kappa fastest, then tile row
-// and tile column; each workgroup walks the phi bins of its entry as the
-// per-model kernel does (phi_tile), so a member's results are those of the
-// member stepped alone, bit for bit.
-template <int STAGE, bool SRC, int TY>
+// Ensemble refraction and intrinsic stages: the same work list with one
+// entry per (64 x TY tile, kappa bin) of a member, in the per-model kernel's
+// order (refraction: kappa fastest, then tile row and tile column;
+// intrinsic: tile column fastest, then tile row and kappa); each workgroup
+// walks the phi bins of its entry as the per-model kernel does (phi_tile),
+// so a member's results are those of the member stepped alone, bit for bit.
+template <int TRANSPORT, int STAGE, bool SRC, int TY>
 __global__ __launch_bounds__(64 * TY) void ensemble_phi_kernel(const EnsembleMember *__restrict__ mem, int members,
                                                                long long total, const int *__restrict__ chunk_member)
 {
@@ -635,10 +698,19 @@ __global__ __launch_bounds__(64 * TY) void ensemble_phi_kernel(const EnsembleMem
         }
         const EnsembleMember &e = mem[a];
         int q = __builtin_amdgcn_readfirstlane((int)(L - a_begin));
-        const int m = q % e.p.nk;
-        q /= e.p.nk;
-        const int jb = q % e.nby, ib = q / e.nby;
-        phi_tile<STAGE, SRC, TY, true>(e.Nk, e.N0, e.out, e.p, ib, jb, m);
+        int m, jb, ib;
+        if constexpr (TRANSPORT == RPL_TRANSPORT_REFRACTION) {
+            m = q % e.p.nk;
+            q /= e.p.nk;
+            jb = q % e.nby;
+            ib = q / e.nby;
+        } else {
+            ib = q % e.nbx;
+            q /= e.nbx;
+            jb = q % e.nby;
+            m = q / e.nby;
+        }
+        phi_tile<TRANSPORT, STAGE, SRC, TY, true>(e.Nk, e.N0, e.out, e.p, ib, jb, m);
     }
 }
 
@@ -673,19 +745,19 @@ template <int TRANSPORT, int STAGE, bool SRC>
 hipError_t launch_ens_ty(const EnsembleMember *mem, int members, long long total, const int *chunk_member, int ty,
                          int blocks, hipStream_t s)
 {
-    if constexpr (TRANSPORT == RPL_TRANSPORT_REFRACTION) {
+    if constexpr (TRANSPORT == RPL_TRANSPORT_REFRACTION || TRANSPORT == RPL_TRANSPORT_INTRINSIC) {
         switch (ty) {
         case 2:
-            hipLaunchKernelGGL((ensemble_phi_kernel<STAGE, SRC, 2>), dim3(blocks), dim3(64, 2), 0, s, mem, members,
-                               total, chunk_member);
+            hipLaunchKernelGGL((ensemble_phi_kernel<TRANSPORT, STAGE, SRC, 2>), dim3(blocks), dim3(64, 2), 0, s, mem,
+                               members, total, chunk_member);
             break;
         case 8:
-            hipLaunchKernelGGL((ensemble_phi_kernel<STAGE, SRC, 8>), dim3(blocks), dim3(64, 8), 0, s, mem, members,
-                               total, chunk_member);
+            hipLaunchKernelGGL((ensemble_phi_kernel<TRANSPORT, STAGE, SRC, 8>), dim3(blocks), dim3(64, 8), 0, s, mem,
+                               members, total, chunk_member);
             break;
         default:
-            hipLaunchKernelGGL((ensemble_phi_kernel<STAGE, SRC, 4>), dim3(blocks), dim3(64, 4), 0, s, mem, members,
-                               total, chunk_member);
+            hipLaunchKernelGGL((ensemble_phi_kernel<TRANSPORT, STAGE, SRC, 4>), dim3(blocks), dim3(64, 4), 0, s, mem,
+                               members, total, chunk_member);
             break;
         }
         return hipGetLastError();
@@ -744,15 +816,15 @@ hipError_t launch_stage_fast(const StageParams<float> &p, int transport, int sta
 {
     if (p.src.on) {
         if (transport == RPL_TRANSPORT_REFRACTION)
-            return launch_phi<true>(p, stage, Nk, N0, out, s);
+            return launch_phi<RPL_TRANSPORT_REFRACTION, true>(p, stage, Nk, N0, out, s);
         if (transport == RPL_TRANSPORT_INTRINSIC)
-            return launch_t<RPL_TRANSPORT_INTRINSIC, true>(p, stage, Nk, N0, out, s);
+            return launch_phi<RPL_TRANSPORT_INTRINSIC, true>(p, stage, Nk, N0, out, s);
         return launch_t<RPL_TRANSPORT_NONE, true>(p, stage, Nk, N0, out, s);
     }
     if (transport == RPL_TRANSPORT_REFRACTION)
-        return launch_phi<false>(p, stage, Nk, N0, out, s);
+        return launch_phi<RPL_TRANSPORT_REFRACTION, false>(p, stage, Nk, N0, out, s);
     if (transport == RPL_TRANSPORT_INTRINSIC)
-        return launch_t<RPL_TRANSPORT_INTRINSIC, false>(p, stage, Nk, N0, out, s);
+        return launch_phi<RPL_TRANSPORT_INTRINSIC, false>(p, stage, Nk, N0, out, s);
     return launch_t<RPL_TRANSPORT_NONE, false>(p, stage, Nk, N0, out, s);
 }
 

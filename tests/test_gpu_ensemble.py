@@ -24,13 +24,14 @@ def rel_l2(a, b):
     return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
 
 
-def build(wl, float_type=np.float32):
+def build(wl, float_type=np.float32, current=True):
     g, sg = wl.grid, wl.spectral_grid
     grid = grids.RectilinearGrid(float_type=float_type, size=(g.Nx, g.Ny, g.Nz), x=g.x_faces, y=g.y_faces,
                                  z=g.z_faces)
     sgrid = grids.PolarWaveVectorGrid(float_type, kappa=sg.kappa, phi=sg.phi, kappa_faces=sg.kappa_faces,
                                       phi_faces=sg.phi_faces)
-    m = ra.SpectralWaveModel(grid, sgrid, velocities=wl.velocities, **wl.model_kwargs)
+    # without the current the model takes the intrinsic WENO transport
+    m = ra.SpectralWaveModel(grid, sgrid, velocities=wl.velocities if current else None, **wl.model_kwargs)
     ra.set_(m, N=wl.N0)
     return m
 
@@ -63,6 +64,31 @@ def test_ensemble_equals_members_stepped_alone(sources):
             assert m.clock.iteration == nsteps
             assert np.array_equal(m.action, r), (ty, blocks, graph, k, rel_l2(m.action, r))
         say(f"ty={ty} blocks={blocks} graph={graph}: {ens.launch()} bitwise equal")
+        ens.close()
+
+
+def test_intrinsic_ensemble_equals_members_stepped_alone():
+    """Members without a current (intrinsic WENO transport, the per-bin work
+    list) with sources: bitwise equal to the members stepped alone."""
+    say("intrinsic ensemble vs single")
+    wls = members(5, True, seed_base=500)
+    dt, nsteps = 0.02, 3
+    ref = []
+    for w in wls:
+        m = build(w, current=False)
+        assert m.transport == ra._lib.RPL_TRANSPORT_INTRINSIC
+        ra.time_step_(m, dt, nsteps)
+        ref.append(m.action)
+    for ty, blocks, graph in ((4, 0, False), (8, 64, True)):
+        ms = [build(w, current=False) for w in wls]
+        ens = ra.Ensemble(ms)
+        ens.set_launch(ty, blocks)
+        ens.use_graph(graph)
+        ens.step(dt, nsteps)
+        ens.synchronize()
+        for k, (m, r) in enumerate(zip(ms, ref)):
+            assert np.array_equal(m.action, r), (ty, blocks, graph, k, rel_l2(m.action, r))
+        say(f"ty={ty} blocks={blocks} graph={graph}: bitwise equal")
         ens.close()
 
 
