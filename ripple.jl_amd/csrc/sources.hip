@@ -193,41 +193,68 @@ __device__ __forceinline__ void state_block_f32(int nx, int ny, int nk, int nphi
     }
     const bool wind = cap && U10 > 0.0f && ustar > 0.0f;
     if (active && (wind || ms || wc)) {
+        // What bin t reads; the next bin's values are loaded before this
+        // bin's arithmetic (the loop is latency-bound otherwise).
+        struct BinIn {
+            float a, k, w, invc, cph, sph, logk, sk, kpw, spw;
+        };
+        auto load = [&](int t, int m, int n) {
+            BinIn v;
+            v.a = N[ij + plane * (size_t)t];
+            v.k = gld<true>(tb.kappa, m);
+            v.w = gld<true>(tb.w, t);
+            if (wind) {
+                v.invc = gld<true>(tb.inv_c, m);
+                v.cph = gld<true>(tb.cos_phi, n);
+                v.sph = gld<true>(tb.sin_phi, n);
+                v.logk = gld<true>(tb.log_kappa, m);
+                v.sk = gld<true>(tb.sigma_k, m);
+            }
+            if (ms) {
+                v.kpw = gld<true>(tb.k_pow, m);
+                v.spw = gld<true>(tb.sigma_pow, m);
+            }
+            return v;
+        };
         int m = g, n = 0;
         while (m >= nk) {
             m -= nk;
             ++n;
         }
         const int nbins = nk * nphi;
+        BinIn cur{};
+        if (g < nbins)
+            cur = load(g, m, n);
         for (int t = g; t < nbins; t += G) {
-            const float a = N[ij + plane * (size_t)t];
-            const float k = gld<true>(tb.kappa, m);
-            if (wc)
-                m0 += a * gld<true>(tb.w, t);  // zeroth moment over every cell
-            if (a > 0.0f && k > 0.0f) {
-                if (wind) {
-                    const float invc = gld<true>(tb.inv_c, m);
-                    const float rate = jp_rate_f32(s, gld<true>(tb.cos_phi, n), gld<true>(tb.sin_phi, n),
-                                                   gld<true>(tb.log_kappa, m), invc, gld<true>(tb.sigma_k, m), ctu,
-                                                   stu, lz1, ustar);
-                    if (rate > 0.0f) {
-                        const float mflux = rate * a * invc * gld<true>(tb.w, t) * tb.rho_w_g;
-                        twx += mflux * gld<true>(tb.cos_phi, n);
-                        twy += mflux * gld<true>(tb.sin_phi, n);
-                    }
-                }
-                if (ms) {
-                    const float wt = gld<true>(tb.w, t);
-                    m0 += a * wt;
-                    kp += a * wt * gld<true>(tb.k_pow, m);
-                    sp += a * wt * gld<true>(tb.sigma_pow, m);
-                }
-            }
             m += G;
             while (m >= nk) {
                 m -= nk;
                 ++n;
             }
+            BinIn nxt = cur;
+            if (t + G < nbins)
+                nxt = load(t + G, m, n);
+            const float a = cur.a, k = cur.k;
+            if (wc)
+                m0 += a * cur.w;  // zeroth moment over every cell
+            if (a > 0.0f && k > 0.0f) {
+                if (wind) {
+                    const float rate = jp_rate_f32(s, cur.cph, cur.sph, cur.logk, cur.invc, cur.sk, ctu, stu, lz1,
+                                                   ustar);
+                    if (rate > 0.0f) {
+                        const float mflux = rate * a * cur.invc * cur.w * tb.rho_w_g;
+                        twx += mflux * cur.cph;
+                        twy += mflux * cur.sph;
+                    }
+                }
+                if (ms) {
+                    const float wt = cur.w;
+                    m0 += a * wt;
+                    kp += a * wt * cur.kpw;
+                    sp += a * wt * cur.spw;
+                }
+            }
+            cur = nxt;
         }
     }
     red[0][g][tx] = twx;
