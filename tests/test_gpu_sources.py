@@ -101,13 +101,22 @@ def test_source_tendency_and_steps_fp64(oracle, case):
     assert m.clock.time == pytest.approx(om.time)
 
 
-@pytest.mark.parametrize("case", ["fetch", "hurricane"])
+@pytest.mark.parametrize("case", ["fetch", "hurricane", "hurricane_powers"])
 def test_source_steps_fp32_fast(oracle, case):
+    """FP32 stage kernels (tabulated wind-input factors, sigma k^3 and sigma^q
+    tables, squares for exponent 2) against the FP64 oracle; the last case
+    takes the general-power branches (wind-input cosine power 3, saturation
+    exponent 1.5, sigma power 1.3)."""
     say(f"sources fp32 {case}")
     if case == "fetch":
         wl = workloads.fetch_limited_growth(float_type=np.float64, nx=24, ny=20, nz=4, nfreq=12, nphi=12, dt=10.0)
     else:
         wl = workloads.translating_hurricane(float_type=np.float64, nx=40, ny=36, nz=4, nkappa=10, nphi=12, dt=60.0)
+    if case == "hurricane_powers":
+        win = wl.model_kwargs["sources"].wind_input
+        wl.model_kwargs["sources"] = ra.PrecomputedSources(
+            wind_input=ra.PressureCorrelationInput(drag=win.drag, wind=win.wind, p_in=3.0),
+            dissipation=ra.LocalSaturationDissipation(B_r=1.05e-2, p=1.5, sigma_power=1.3))
     m = device(wl, np.float32, kernel="fast")
     om = oracle_model(oracle, wl)
     for _ in range(5):
