@@ -363,7 +363,9 @@ __global__ __launch_bounds__(256) void features_kernel(const T *__restrict__ N, 
         if (col < plane) {
             if (bin < nbins) {
                 const float a = (float)N[col + plane * bin];
-                v = (logf(fmaxf(a, 0.0f) + fs.floor) - fs.shift) * fs.scale;
+                // ln from the hardware log2 (about 1 ulp): the feature is
+                // stored in bf16, 8 bits of mantissa.
+                v = (__builtin_amdgcn_logf(fmaxf(a, 0.0f) + fs.floor) * 0.693147180559945f - fs.shift) * fs.scale;
                 mx = fmaxf(mx, a);
                 if (bin_hi)
                     over = over || !(a <= bin_hi[bin]);
@@ -382,9 +384,20 @@ __global__ __launch_bounds__(256) void features_kernel(const T *__restrict__ N, 
         red_over[ty][tx] = over ? 1 : 0;
     }
     __syncthreads();
-    for (int cc = ty; cc < 64; cc += 4) {
+    // 16-byte stores: entry q of the 512 takes column q / 8 and features
+    // 8 (q % 8) .. 8 (q % 8) + 7 of the tile.
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int q = threadIdx.x + 256 * h, cc = q >> 3, f0 = (q & 7) * 8;
         const size_t cl = c0 + cc;
-        X[(cl - col0) * (size_t)in_pad + b0 + tx] = f2bf(cl < plane ? tile[tx][cc] : 0.0f);
+        unsigned w[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const float lo = cl < plane ? tile[f0 + 2 * e][cc] : 0.0f;
+            const float hi = cl < plane ? tile[f0 + 2 * e + 1][cc] : 0.0f;
+            w[e] = (unsigned)f2bf(lo) | ((unsigned)f2bf(hi) << 16);
+        }
+        *reinterpret_cast<uint4 *>(X + (cl - col0) * (size_t)in_pad + b0 + f0) = make_uint4(w[0], w[1], w[2], w[3]);
     }
     if (flags && ty == 0 && col < plane) {
         const float m = fmaxf(fmaxf(red_max[0][tx], red_max[1][tx]), fmaxf(red_max[2][tx], red_max[3][tx]));
