@@ -193,13 +193,37 @@ __device__ __forceinline__ void store_tile(bf16 (*S)[WLD], const u32x4 (&r)[4], 
     }
 }
 
+// Block order of the wide GEMM: one-dimensional grid; workgroups are dealt
+// round-robin to the eight XCDs, so XCD x takes the contiguous range
+// [x T / 8, (x+1) T / 8) of the T tiles, and within it the tiles walk the
+// smaller tile dimension fastest.  The tiles that share a slice of the large
+// operand (the 65,536-column activations) then run together on one XCD and
+// read it once through that XCD's L2; the small operand (the weights) stays
+// in every L2.  (A two-dimensional grid with M fastest re-read the
+// activations from HBM once per tile of the other dimension.)
+__device__ __forceinline__ void wide_tile(int b, int mt, int nt, int &m0, int &n0)
+{
+    const int total = mt * nt;
+    const int xcd = b & 7, r = b >> 3;
+    const int per = total >> 3, rem = total & 7;
+    const int L = xcd < rem ? xcd * (per + 1) + r : rem * (per + 1) + (xcd - rem) * per + r;
+    if (nt <= mt) {
+        n0 = (L % nt) * WN;
+        m0 = (L / nt) * WM;
+    } else {
+        m0 = (L % mt) * WM;
+        n0 = (L / mt) * WN;
+    }
+}
+
 template <bool A_KC, bool B_KC, int EPI>
 __global__ __launch_bounds__(256) void gemm_wide_kernel(GemmArgs g)
 {
     __shared__ bf16 As[2][WM][WLD];
     __shared__ bf16 Bs[2][WN][WLD];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int m0 = blockIdx.x * WM, n0 = blockIdx.y * WN;
+    int m0, n0;
+    wide_tile(blockIdx.x, g.M / WM, g.N / WN, m0, n0);
     const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
     const bf16 *Ab = g.A + (size_t)m0 * g.sam;
     const bf16 *Bb = g.B + (size_t)n0 * g.sbn;
@@ -244,6 +268,56 @@ __global__ __launch_bounds__(256) void gemm_wide_kernel(GemmArgs g)
             store_tile<B_KC>(Bs[cur ^ 1], rb, tid);
         }
         __syncthreads();
+    }
+    if constexpr (EPI == EPI_BIAS_GELU_INFER) {
+        if (g.scn == 1 && (g.scm & 7) == 0 && ((size_t)g.Cb & 15) == 0) {
+            // Each wave stages its 64 x 64 bf16 results in its quarter of As
+            // (free after the last barrier of the K loop) and writes them as
+            // 16-byte row chunks, instead of 64 two-byte stores per lane.
+            float bn[4];
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+                bn[b] = g.bias[n0 + wn + b * 16 + r16];
+            bf16(*T)[WLD] = &As[0][0] + wave * 64;
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        T[a * 16 + 4 * (lane >> 4) + r][b * 16 + r16] = f2bf(gelu(acc[a][b][r] + bn[b]));
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const int c = lane + 64 * k, row = c >> 3, col = (c & 7) * 8;
+                const u32x4 v = *reinterpret_cast<const u32x4 *>(&T[row][col]);
+                *reinterpret_cast<u32x4 *>(g.Cb + (size_t)(m0 + wm + row) * g.scm + (n0 + wn + col)) = v;
+            }
+            return;
+        }
+    }
+    if constexpr (EPI == EPI_BIAS_ROW_F32) {
+        // the row bias of the thread's 16 rows, loaded once
+        float bm[4][4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                bm[a][r] = g.bias[m0 + wm + a * 16 + 4 * (lane >> 4) + r];
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int m = m0 + wm + a * 16 + 4 * (lane >> 4) + r;
+                    const int n = n0 + wn + b * 16 + r16;
+                    const float v = acc[a][b][r] + bm[a][r];
+                    g.Cf[(size_t)m * g.scm + (size_t)n * g.scn] = v;
+                    if (g.nonfinite && !(fabsf(v) < 3.0e38f))
+                        g.nonfinite[n] = 2;
+                }
+        return;
     }
 #pragma unroll
     for (int a = 0; a < 4; ++a)
@@ -484,7 +558,7 @@ template <bool A_KC, bool B_KC, int EPI>
 static void gemm_launch(const GemmArgs &g, hipStream_t s)
 {
     if (g.M % WM == 0 && g.N % WN == 0 && g.K % WK == 0)
-        hipLaunchKernelGGL((gemm_wide_kernel<A_KC, B_KC, EPI>), dim3(g.M / WM, g.N / WN), dim3(256), 0, s, g);
+        hipLaunchKernelGGL((gemm_wide_kernel<A_KC, B_KC, EPI>), dim3((g.M / WM) * (g.N / WN)), dim3(256), 0, s, g);
     else
         hipLaunchKernelGGL((gemm_kernel<A_KC, B_KC, EPI>), dim3(g.M / BM, g.N / BN), dim3(256), 0, s, g);
 }
