@@ -496,8 +496,12 @@ __device__ __forceinline__ void phi_tile(const float *__restrict__ Nk, const flo
             for (int d = 0; d < 7; ++d)
                 if (d != 3)
                     b.fk[d] = ldo<AS1>(Nk + plane * ((size_t)clampi(m + d - 3, nk) + (size_t)nk * n), bo);
-            // f(n + 4) wraps up to twice round a 3-bin circle: a full modulo
-            b.wnext = ldo<AS1>(Nk + plane * ((size_t)m + (size_t)nk * ((n + 4) % nphi)), bo);
+            // f(n + 4): n + 4 < 3 nphi for nphi >= 3 (it wraps twice round a
+            // 3-bin circle), so two conditional subtractions suffice
+            int nw = n + 4;
+            nw = nw >= nphi ? nw - nphi : nw;
+            nw = nw >= nphi ? nw - nphi : nw;
+            b.wnext = ldo<AS1>(Nk + plane * ((size_t)m + (size_t)nk * nw), bo);
         } else {
             b.x0 = ldo<AS1>(slab, bo);
         }
