@@ -162,6 +162,12 @@ def test_kappa_boundary_cases_fp64(oracle, nk):
     ra.time_step_(m, 0.02, 3)
     om.step(0.02, 3)
     assert rel_l2(m.action, om.action) <= 1e-12
+    # the FP32 phi-loop kernel on the same edge cases, against the FP32 oracle
+    m32 = device_model(wl, np.float32, kernel="fast")
+    o32 = oracle.OracleModel(wl, dtype=np.float32)
+    ra.time_step_(m32, 0.02, 3)
+    o32.step(0.02, 3)
+    assert rel_l2(m32.action, o32.action) <= 5e-5
 
 
 @pytest.mark.parametrize("shape", [(3, 3, 4, 3), (67, 5, 6, 7), (130, 66, 5, 36)])
