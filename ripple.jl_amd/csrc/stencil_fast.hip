@@ -683,9 +683,14 @@ __global__ __launch_bounds__(64 * TY) void ensemble_stage_kernel(const EnsembleM
 // intrinsic: tile column fastest, then tile row and kappa); each workgroup
 // walks the phi bins of its entry as the per-model kernel does (phi_tile),
 // so a member's results are those of the member stepped alone, bit for bit.
+// At least five waves per SIMD: the refraction-with-sources instantiation
+// needs a few registers more than the per-model kernel (descriptor-table
+// addressing) and would drop to four; held at five it spills a few values
+// and runs faster (measured on 32 config-5 members: 157.5 -> 149.9 ms per
+// step).
 template <int TRANSPORT, int STAGE, bool SRC, int TY>
-__global__ __launch_bounds__(64 * TY) void ensemble_phi_kernel(const EnsembleMember *__restrict__ mem, int members,
-                                                               long long total, const int *__restrict__ chunk_member)
+__global__ __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(5, 8))) void ensemble_phi_kernel(
+    const EnsembleMember *__restrict__ mem, int members, long long total, const int *__restrict__ chunk_member)
 {
     const int xcd = blockIdx.x & 7;
     const long long stride = (long long)(gridDim.x >> 3);
