@@ -144,7 +144,10 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g)
 
 // Large-tile GEMM for MFMA-bound shapes: block tile 128 x 128, K in steps of
 // 64, v_mfma_f32_16x16x32_bf16 (the gfx950 double-rate bf16 instruction).
-// Four wavefronts, each a 64 x 64 quadrant = 4 x 4 MFMA tiles.  Both tiles
+// Eight wavefronts, each a 64 x 32 block = 4 x 2 MFMA tiles (measured: the
+// K loop is bound by the latency of its global loads, and eight waves per
+// workgroup -- four per SIMD -- hide more of it than four waves of 64 x 64
+// each: 7 % faster inference).  Both tiles
 // are staged through LDS as [row][k] (k contiguous, rows padded to 144 bytes
 // so the 16-byte fragment reads of one wavefront spread over the banks) in
 // two buffers: the next K step is fetched into registers while the current
@@ -156,42 +159,7 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 constexpr int WM = 128, WN = 128, WK = 64, WLD = WK + 8;
 
-template <bool K_CONTIG>
-__device__ __forceinline__ void fetch_tile(u32x4 (&r)[4], const bf16 *__restrict__ G, size_t s_row, size_t s_k,
-                                           int tid)
-{
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int idx = tid + 256 * q;
-        if (K_CONTIG) {
-            const int row = idx >> 3, k = (idx & 7) * 8;
-            r[q] = *reinterpret_cast<const u32x4 *>(G + (size_t)row * s_row + k);
-        } else {
-            const int k = idx >> 4, row = (idx & 15) * 8;
-            r[q] = *reinterpret_cast<const u32x4 *>(G + (size_t)k * s_k + row);
-        }
-    }
-}
 
-template <bool K_CONTIG>
-__device__ __forceinline__ void store_tile(bf16 (*S)[WLD], const u32x4 (&r)[4], int tid)
-{
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int idx = tid + 256 * q;
-        if (K_CONTIG) {
-            const int row = idx >> 3, k = (idx & 7) * 8;
-            *reinterpret_cast<u32x4 *>(&S[row][k]) = r[q];
-        } else {
-            const int k = idx >> 4, row = (idx & 15) * 8;
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                S[row + 2 * u][k] = (bf16)(r[q][u] & 0xffffu);
-                S[row + 2 * u + 1][k] = (bf16)(r[q][u] >> 16);
-            }
-        }
-    }
-}
 
 // Block order of the wide GEMM: one-dimensional grid; workgroups are dealt
 // round-robin to the eight XCDs, so XCD x takes the contiguous range
@@ -216,25 +184,63 @@ __device__ __forceinline__ void wide_tile(int b, int mt, int nt, int &m0, int &n
     }
 }
 
+
+template <bool K_CONTIG>
+__device__ __forceinline__ void fetch_tile(u32x4 (&r)[2], const bf16 *__restrict__ G, size_t s_row, size_t s_k,
+                                            int tid)
+{
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int idx = tid + 512 * q;
+        if (K_CONTIG) {
+            const int row = idx >> 3, k = (idx & 7) * 8;
+            r[q] = *reinterpret_cast<const u32x4 *>(G + (size_t)row * s_row + k);
+        } else {
+            const int k = idx >> 4, row = (idx & 15) * 8;
+            r[q] = *reinterpret_cast<const u32x4 *>(G + (size_t)k * s_k + row);
+        }
+    }
+}
+
+template <bool K_CONTIG>
+__device__ __forceinline__ void store_tile(bf16 (*S)[WLD], const u32x4 (&r)[2], int tid)
+{
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int idx = tid + 512 * q;
+        if (K_CONTIG) {
+            const int row = idx >> 3, k = (idx & 7) * 8;
+            *reinterpret_cast<u32x4 *>(&S[row][k]) = r[q];
+        } else {
+            const int k = idx >> 4, row = (idx & 15) * 8;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                S[row + 2 * u][k] = (bf16)(r[q][u] & 0xffffu);
+                S[row + 2 * u + 1][k] = (bf16)(r[q][u] >> 16);
+            }
+        }
+    }
+}
+
 template <bool A_KC, bool B_KC, int EPI>
-__global__ __launch_bounds__(256) void gemm_wide_kernel(GemmArgs g)
+__global__ __launch_bounds__(512) void gemm_wide_kernel(GemmArgs g)
 {
     __shared__ bf16 As[2][WM][WLD];
     __shared__ bf16 Bs[2][WN][WLD];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     int m0, n0;
     wide_tile(blockIdx.x, g.M / WM, g.N / WN, m0, n0);
-    const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
+    const int wm = (wave & 1) * 64, wn = (wave >> 1) * 32;
     const bf16 *Ab = g.A + (size_t)m0 * g.sam;
     const bf16 *Bb = g.B + (size_t)n0 * g.sbn;
-    f32x4 acc[4][4];
+    f32x4 acc[4][2];
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
-        for (int b = 0; b < 4; ++b)
+        for (int b = 0; b < 2; ++b)
             acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
     const int r16 = lane & 15, q8 = (lane >> 4) * 8;
-    u32x4 ra[4], rb[4];
+    u32x4 ra[2], rb[2];
     fetch_tile<A_KC>(ra, Ab, g.sam, g.sak, tid);
     fetch_tile<B_KC>(rb, Bb, g.sbn, g.sbk, tid);
     store_tile<A_KC>(As[0], ra, tid);
@@ -250,17 +256,17 @@ __global__ __launch_bounds__(256) void gemm_wide_kernel(GemmArgs g)
         }
 #pragma unroll
         for (int ks = 0; ks < WK; ks += 32) {
-            bf16x8 af[4], bfr[4];
+            bf16x8 af[4], bfr[2];
 #pragma unroll
             for (int a = 0; a < 4; ++a)
                 af[a] = *reinterpret_cast<const bf16x8 *>(&As[cur][wm + a * 16 + r16][ks + q8]);
 #pragma unroll
-            for (int b = 0; b < 4; ++b)
+            for (int b = 0; b < 2; ++b)
                 bfr[b] = *reinterpret_cast<const bf16x8 *>(&Bs[cur][wn + b * 16 + r16][ks + q8]);
 #pragma unroll
             for (int a = 0; a < 4; ++a)
 #pragma unroll
-                for (int b = 0; b < 4; ++b)
+                for (int b = 0; b < 2; ++b)
                     acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a], bfr[b], acc[a][b], 0, 0, 0);
         }
         if (more) {
@@ -271,25 +277,26 @@ __global__ __launch_bounds__(256) void gemm_wide_kernel(GemmArgs g)
     }
     if constexpr (EPI == EPI_BIAS_GELU_INFER) {
         if (g.scn == 1 && (g.scm & 7) == 0 && ((size_t)g.Cb & 15) == 0) {
-            // Each wave stages its 64 x 64 bf16 results in its quarter of As
-            // (free after the last barrier of the K loop) and writes them as
-            // 16-byte row chunks, instead of 64 two-byte stores per lane.
-            float bn[4];
+            // Each wave stages its 64 x 32 bf16 results in LDS (As and Bs are
+            // free after the last barrier of the K loop; four waves in each)
+            // and writes them as 16-byte row chunks, instead of 32 two-byte
+            // stores per lane.
+            float bn[2];
 #pragma unroll
-            for (int b = 0; b < 4; ++b)
+            for (int b = 0; b < 2; ++b)
                 bn[b] = g.bias[n0 + wn + b * 16 + r16];
-            bf16(*T)[WLD] = &As[0][0] + wave * 64;
+            bf16(*T)[40] = reinterpret_cast<bf16(*)[40]>(wave < 4 ? &As[0][0][0] : &Bs[0][0][0]) + (wave & 3) * 64;
 #pragma unroll
             for (int a = 0; a < 4; ++a)
 #pragma unroll
-                for (int b = 0; b < 4; ++b)
+                for (int b = 0; b < 2; ++b)
 #pragma unroll
                     for (int r = 0; r < 4; ++r)
                         T[a * 16 + 4 * (lane >> 4) + r][b * 16 + r16] = f2bf(gelu(acc[a][b][r] + bn[b]));
             __syncthreads();
 #pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                const int c = lane + 64 * k, row = c >> 3, col = (c & 7) * 8;
+            for (int k = 0; k < 4; ++k) {
+                const int c = lane + 64 * k, row = c >> 2, col = (c & 3) * 8;
                 const u32x4 v = *reinterpret_cast<const u32x4 *>(&T[row][col]);
                 *reinterpret_cast<u32x4 *>(g.Cb + (size_t)(m0 + wm + row) * g.scm + (n0 + wn + col)) = v;
             }
@@ -307,7 +314,7 @@ __global__ __launch_bounds__(256) void gemm_wide_kernel(GemmArgs g)
 #pragma unroll
         for (int a = 0; a < 4; ++a)
 #pragma unroll
-            for (int b = 0; b < 4; ++b)
+            for (int b = 0; b < 2; ++b)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int m = m0 + wm + a * 16 + 4 * (lane >> 4) + r;
@@ -322,7 +329,7 @@ __global__ __launch_bounds__(256) void gemm_wide_kernel(GemmArgs g)
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
-        for (int b = 0; b < 4; ++b)
+        for (int b = 0; b < 2; ++b)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int m = m0 + wm + a * 16 + 4 * (lane >> 4) + r;
@@ -558,7 +565,7 @@ template <bool A_KC, bool B_KC, int EPI>
 static void gemm_launch(const GemmArgs &g, hipStream_t s)
 {
     if (g.M % WM == 0 && g.N % WN == 0 && g.K % WK == 0)
-        hipLaunchKernelGGL((gemm_wide_kernel<A_KC, B_KC, EPI>), dim3((g.M / WM) * (g.N / WN)), dim3(256), 0, s, g);
+        hipLaunchKernelGGL((gemm_wide_kernel<A_KC, B_KC, EPI>), dim3((g.M / WM) * (g.N / WN)), dim3(512), 0, s, g);
     else
         hipLaunchKernelGGL((gemm_kernel<A_KC, B_KC, EPI>), dim3(g.M / BM, g.N / BN), dim3(256), 0, s, g);
 }
