@@ -12,7 +12,11 @@
 //   * each reconstruction is v1 + weighted third differences with one
 //     reciprocal instead of four IEEE divides;
 //   * the current gradients are centred differences of U formed in-kernel
-//     (wave_current_refraction.jl:7-19) instead of four extra arrays.
+//     (wave_current_refraction.jl:7-19) instead of four extra arrays;
+//   * the refraction and intrinsic stages give each workgroup one (x, y)
+//     tile of one kappa bin and walk its phi bins (phi_tile): what depends on
+//     (i, j, kappa) only is formed once, the phi stencil is a register
+//     window, and a phi face is carried from one bin to the next.
 // Results differ from the reference-order kernel by rounding only.
 #include "rpl_internal.h"
 
