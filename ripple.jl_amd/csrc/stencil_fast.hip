@@ -596,14 +596,16 @@ __device__ __forceinline__ void phi_tile(const float *__restrict__ Nk, const flo
 // neighbours a tile reads are processed together on one XCD; intrinsic
 // transport has no spectral coupling and walks tile columns fastest, then
 // tile rows, then kappa, so the tiles in flight share their x and y halos.
-// Occupancy floors: intrinsic transport at seven waves per SIMD (72
+// Occupancy floors (waves per SIMD): intrinsic transport seven (72
 // registers, one spilled value: 1.71 -> 1.61 ms per launch on the 512^2
-// hurricane with sources), refraction at five, where it already is without
-// sources and which costs the with-sources instantiation six spilled values
-// (config-5 members one after another: 137.8 -> 134.7 ms per step).  More
-// waves for refraction spill more and run slower.
+// hurricane with sources); refraction without sources six (80 registers,
+// one spilled value: 2.216 -> 2.184 ms per launch on the 512^2 sheared
+// eddies); refraction with sources five (six spilled values; config-5
+// members one after another: 137.8 -> 134.7 ms per step).  Higher floors
+// spill more and run slower.
 template <int TRANSPORT, int STAGE, bool SRC>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TRANSPORT == RPL_TRANSPORT_INTRINSIC ? 7 : 5, 8)))
+__global__ __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(TRANSPORT == RPL_TRANSPORT_INTRINSIC ? 7 : (SRC ? 5 : 6), 8)))
 void stage_phi_kernel(const float *__restrict__ Nk, const float *__restrict__ N0, float *out, StageParams<float> p,
                       int nbx, int nby)
 {
