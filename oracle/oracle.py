@@ -20,7 +20,7 @@ _I = ctypes.c_int
 _P = ctypes.c_void_p
 _lib = None
 
-MODE_NONE, MODE_INTRINSIC, MODE_REFRACTION = 0, 1, 2
+MODE_NONE, MODE_INTRINSIC, MODE_REFRACTION, MODE_PSEUDOMOMENTUM = 0, 1, 2, 3
 STEPPERS = {"RK3": 0, "LSRK3": 1, "LowStorageRK3": 1, "ForwardEuler": 2}
 
 
@@ -36,6 +36,11 @@ def lib():
         _lib = ctypes.CDLL(LIB_PATH)
         _lib.orc_q_cdf.restype = _D
         _lib.orc_q_cdf.argtypes = [_D, _D, _D, _D]
+        _lib.orc_q_cdf_dkappa.restype = _D
+        _lib.orc_q_cdf_dkappa.argtypes = [_D, _D, _D, _D]
+        for f in ("orc_q_cell_integral", "orc_q_cell_integral_dkappa"):
+            getattr(_lib, f).restype = _D
+            getattr(_lib, f).argtypes = [_D, _D, _D, _D]
         _lib.orc_total_weighted_f64.restype = _D
         _lib.orc_total_weighted_f32.restype = _D
         _lib.orc_weno5_f64.restype = _D
@@ -113,11 +118,56 @@ def doppler(u, v, W):
     return Ux.reshape((nx, ny, nk), order="F"), Uy.reshape((nx, ny, nk), order="F")
 
 
+def q_cell_integral(kappa, z1, z2, depth):
+    return lib().orc_q_cell_integral(kappa, z1, z2, depth)
+
+
+def q_cdf_dkappa(kappa, z, depth, small_mu=1e-6):
+    return lib().orc_q_cdf_dkappa(kappa, z, depth, small_mu)
+
+
+def q_cell_integral_dkappa(kappa, z1, z2, depth):
+    return lib().orc_q_cell_integral_dkappa(kappa, z1, z2, depth)
+
+
+def pm_overlap(z_faces, kappa, depth):
+    """Pseudomomentum overlap and its kappa derivative, each (Nk, Nk) [target, source]."""
+    zf, kc = _c(z_faces), _c(kappa)
+    nk = len(kc)
+    O, dO = np.zeros(nk * nk), np.zeros(nk * nk)
+    lib().orc_pm_overlap(_I(len(zf) - 1), _p(zf), _I(nk), _p(kc), _D(depth), _p(O), _p(dO))
+    return O.reshape((nk, nk), order="F"), dO.reshape((nk, nk), order="F")
+
+
+def pm_doppler(N, O, dO, kx, ky, dtype=np.float64):
+    """Ux, Uy, dUx/dkappa, dUy/dkappa (each (Nx, Ny, Nk)) of the pseudomomentum coupling."""
+    nx, ny, nk, npf = N.shape
+    Nf, Of, dOf, kxf, kyf = (_c(a, dtype) for a in (N, O, dO, kx, ky))
+    out = [np.zeros(nx * ny * nk, dtype=dtype) for _ in range(4)]
+    sfx = "f64" if np.dtype(dtype) == np.float64 else "f32"
+    getattr(lib(), f"orc_pm_doppler_{sfx}")(_I(nx), _I(ny), _I(nk), _I(npf), _p(Nf), _p(Of), _p(dOf), _p(kxf),
+                                            _p(kyf), *[_p(a) for a in out])
+    return [a.reshape((nx, ny, nk), order="F") for a in out]
+
+
+def pm_cells(N, W, z_faces, kx, ky, cell_average=True, dtype=np.float64):
+    """Pseudomomentum cell averages (or integrals) px, py, each (Nx, Ny, Nz)."""
+    nx, ny, nk, npf = N.shape
+    nz = W.shape[0]
+    Nf, Wf, kxf, kyf = (_c(a, dtype) for a in (N, W, kx, ky))
+    zf = _c(z_faces)
+    px, py = np.zeros(nx * ny * nz, dtype=dtype), np.zeros(nx * ny * nz, dtype=dtype)
+    sfx = "f64" if np.dtype(dtype) == np.float64 else "f32"
+    getattr(lib(), f"orc_pm_cells_{sfx}")(_I(nx), _I(ny), _I(nk), _I(npf), _I(nz), _p(Nf), _p(Wf), _p(zf), _p(kxf),
+                                          _p(kyf), _I(int(cell_average)), _p(px), _p(py))
+    return px.reshape((nx, ny, nz), order="F"), py.reshape((nx, ny, nz), order="F")
+
+
 class OracleModel:
     """The oracle's version of SpectralWaveModel over a Workload's inputs."""
 
     def __init__(self, workload, dtype=np.float64, depth=None, gravity=9.81, doppler_override=None,
-                 transport=None, timestepper=None):
+                 transport=None, timestepper=None, pseudomomentum=None):
         wl = workload
         self.dtype = np.dtype(dtype)
         g, sg = wl.grid, wl.spectral_grid
@@ -161,6 +211,15 @@ class OracleModel:
         else:
             Ux = Uy = np.zeros((nx, ny, nk))
         self.Ux, self.Uy = _c(Ux, t), _c(Uy, t)
+        # Self-coupled pseudomomentum current: pseudomomentum=(z_faces, q_depth)
+        # (Coupling/current_coupling.jl:93-118); U follows every stage input.
+        self.pm = None
+        if pseudomomentum is not None:
+            zf, q_depth = pseudomomentum
+            O, _ = pm_overlap(zf, sg.kappa, q_depth)
+            self.pm = [_c(O, t), _c(tabs["mu_x"], t), _c(tabs["mu_y"], t)]
+            self.mode = MODE_PSEUDOMOMENTUM
+            self.Ux, self.Uy = np.zeros(nx * ny * nk, dtype=t), np.zeros(nx * ny * nk, dtype=t)
         self.N = _c(wl.N0, t).copy()  # never alias the workload's array
         self.time, self.iteration = 0.0, 0
 
@@ -180,7 +239,7 @@ class OracleModel:
         inv = [self._scalar(v) for v in self._inv()]
         rc = fn(_I(nx), _I(ny), _I(nk), _I(npf), _I(self.mode), _I(self.stepper), _p(self.N),
                 _p(self.Ux), _p(self.Uy), _p(self.kappa), _p(self.cgx), _p(self.cgy), _p(self.cos), _p(self.sin),
-                *inv, _D(dt), _I(nsteps), hook, user)
+                *inv, _D(dt), _I(nsteps), hook, user, *self._pm_args())
         if rc != 0:
             raise MemoryError("oracle run failed to allocate")
         self.time += dt * nsteps
@@ -194,8 +253,16 @@ class OracleModel:
         inv = [self._scalar(v) for v in self._inv()]
         getattr(L, f"orc_tendency_{sfx}")(_I(nx), _I(ny), _I(nk), _I(npf), _I(self.mode), _p(self.N), _p(G),
                                           _p(self.Ux), _p(self.Uy), _p(self.kappa), _p(self.cgx), _p(self.cgy),
-                                          _p(self.cos), _p(self.sin), *inv)
+                                          _p(self.cos), _p(self.sin), *inv, *self._pm_args())
         return G.reshape(self.size, order="F")
+
+    def _pm_args(self):
+        return [_p(a) for a in self.pm] if self.pm is not None else [None, None, None]
+
+    def doppler_velocity(self):
+        """(Ux, Uy): the coupling's velocity (in mode 3, of the last tendency input)."""
+        nx, ny, nk, _ = self.size
+        return self.Ux.reshape((nx, ny, nk), order="F"), self.Uy.reshape((nx, ny, nk), order="F")
 
     @property
     def action(self):

@@ -224,30 +224,124 @@ void SFX(orc_column_sums)(int nx, int ny, int nk, int nphi, const REAL *N,
     }
 }
 
+/* Coupling/pseudomomentum.jl:316-341 (_compute_pseudomomentum_doppler_velocity_kernel!):
+ *   U[i,j,t] = sum_{n, s} O[t,s] * N[i,j,s,n] * kx[s,n]
+ * (and Uy with ky; dU/dkappa with the derivative overlap dO when non-NULL),
+ * accumulated n outer, s inner, each term formed as (O * N) * k.  O and dO
+ * are Nk x Nk, target fastest (Julia overlap[target_m, source_m]); kx, ky
+ * are the first-moment measures [n][m]. */
+void SFX(orc_pm_doppler)(int nx, int ny, int nk, int nphi, const REAL *N, const REAL *O, const REAL *dO,
+                         const REAL *kx, const REAL *ky, REAL *Ux, REAL *Uy, REAL *dUx, REAL *dUy)
+{
+#pragma omp parallel for schedule(static)
+    for (long jt = 0; jt < (long)ny * nk; ++jt) {
+        int j = (int)(jt % ny), t = (int)(jt / ny);
+        for (int i = 0; i < nx; ++i) {
+            REAL ax = 0, ay = 0, dax = 0, day = 0;
+            for (int n = 0; n < nphi; ++n)
+                for (int s = 0; s < nk; ++s) {
+                    REAL a = N[IDX4(i, j, s, n)];
+                    size_t st = (size_t)s + (size_t)nk * n;
+                    REAL q = O[(size_t)t + (size_t)nk * s];
+                    ax += q * a * kx[st];
+                    ay += q * a * ky[st];
+                    if (dO) {
+                        REAL dq = dO[(size_t)t + (size_t)nk * s];
+                        dax += dq * a * kx[st];
+                        day += dq * a * ky[st];
+                    }
+                }
+            Ux[IDX3(i, j, t)] = ax;
+            Uy[IDX3(i, j, t)] = ay;
+            if (dO) {
+                dUx[IDX3(i, j, t)] = dax;
+                dUy[IDX3(i, j, t)] = day;
+            }
+        }
+    }
+}
+
+/* Coupling/pseudomomentum.jl:375-399 (_compute_pseudomomentum_cells_kernel!):
+ *   p[i,j,k] = scale_k * sum_{n, m} W[k,m] * N[i,j,m,n] * kx[m,n]
+ * with W the Q cell weights (nz x Nk, k fastest), n outer, m inner, and
+ * scale_k = 1 / |z_{k+1} - z_k| for cell averages, 1 for cell integrals. */
+void SFX(orc_pm_cells)(int nx, int ny, int nk, int nphi, int nz, const REAL *N, const REAL *W, const double *zf,
+                       const REAL *kx, const REAL *ky, int cell_average, REAL *px, REAL *py)
+{
+#pragma omp parallel for schedule(static)
+    for (long jk = 0; jk < (long)ny * nz; ++jk) {
+        int j = (int)(jk % ny), k = (int)(jk / ny);
+        REAL scale = cell_average ? (REAL)1 / (REAL)fabs(zf[k + 1] - zf[k]) : (REAL)1;
+        for (int i = 0; i < nx; ++i) {
+            REAL ax = 0, ay = 0;
+            for (int n = 0; n < nphi; ++n)
+                for (int m = 0; m < nk; ++m) {
+                    REAL q = W[(size_t)k + (size_t)nz * m];
+                    REAL a = N[IDX4(i, j, m, n)];
+                    size_t mn = (size_t)m + (size_t)nk * n;
+                    ax += q * a * kx[mn];
+                    ay += q * a * ky[mn];
+                }
+            size_t o = (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k);
+            px[o] = ax * scale;
+            py[o] = ay * scale;
+        }
+    }
+}
+
 /* Models/time_step.jl:119-170 (time_step!) for :RK3, :LSRK3 and
  * :ForwardEuler, driving the tendency of Models/tendencies.jl:18-109:
  *   mode 0: no transport and no sources (G = 0, tendencies.jl:60-62)
  *   mode 1: fused intrinsic transport (tendencies.jl:28-41)
  *   mode 2: fused Doppler transport + refraction (tendencies.jl:82-101)
+ *   mode 3: mode 2 with the self-coupled pseudomomentum current
+ *           (tendencies.jl:103-107): before every tendency evaluation U is
+ *           recomputed from the stage input (current_coupling.jl:140-143)
+ *           through the overlap pm_O and the measures pm_kx, pm_ky, into
+ *           Ux, Uy, which must then be writable.
  * stepper 0 = RK3, 1 = LSRK3 (identical arithmetic, time_step.jl:107-115),
- * 2 = ForwardEuler.  The current gradients are formed once: the reference
- * recomputes them every call from the same static Ux, Uy, giving the same
- * values.  extra_tendency, when non-NULL, is called after the transport
- * tendency with (N, G, user) and adds source terms (tendencies.jl:88-94). */
+ * 2 = ForwardEuler.  In mode 2 the current gradients are formed once: the
+ * reference recomputes them every call from the same static Ux, Uy, giving
+ * the same values; in mode 3 they follow U every evaluation.  extra_tendency,
+ * when non-NULL, is called after the transport tendency with (N, G, user) and
+ * adds source terms (tendencies.jl:88-94). */
 typedef void (*SFX(orc_tendency_hook))(const REAL *N, REAL *G, void *user);
 
+static void SFX(transport_tendency)(int nx, int ny, int nk, int nphi, int mode, const REAL *N, REAL *G,
+                                    const REAL *Ux, const REAL *Uy, REAL *gr, int fresh_gradients,
+                                    const REAL *pm_O, const REAL *pm_kx, const REAL *pm_ky,
+                                    const REAL *kappa_c, const REAL *cgx, const REAL *cgy,
+                                    const REAL *cos_t, const REAL *sin_t,
+                                    REAL dx_inv, REAL dy_inv, REAL dk_inv, REAL dp_inv)
+{
+    size_t cnt = (size_t)nx * ny * nk * nphi, c3 = (size_t)nx * ny * nk;
+    if (mode == 3)
+        SFX(orc_pm_doppler)(nx, ny, nk, nphi, N, pm_O, NULL, pm_kx, pm_ky, (REAL *)Ux, (REAL *)Uy, NULL, NULL);
+    if (mode >= 2) {
+        if (fresh_gradients || mode == 3)
+            SFX(orc_current_gradients)(nx, ny, nk, Ux, Uy, dx_inv, dy_inv, gr, gr + c3, gr + 2 * c3, gr + 3 * c3);
+        SFX(orc_refraction_tendency)(nx, ny, nk, nphi, N, G, Ux, Uy, gr, gr + c3, gr + 2 * c3, gr + 3 * c3,
+                                     kappa_c, cgx, cgy, cos_t, sin_t, dx_inv, dy_inv, dk_inv, dp_inv);
+    } else if (mode == 1) {
+        SFX(orc_intrinsic_tendency)(nx, ny, nk, nphi, N, G, cgx, cgy, dx_inv, dy_inv);
+    } else {
+        memset(G, 0, cnt * sizeof(REAL));
+    }
+}
+
 int SFX(orc_run)(int nx, int ny, int nk, int nphi, int mode, int stepper, REAL *N,
-                 const REAL *Ux, const REAL *Uy, const REAL *kappa_c,
+                 REAL *Ux, REAL *Uy, const REAL *kappa_c,
                  const REAL *cgx, const REAL *cgy, const REAL *cos_t, const REAL *sin_t,
                  REAL dx_inv, REAL dy_inv, REAL dk_inv, REAL dp_inv, double dt, int nsteps,
-                 SFX(orc_tendency_hook) extra_tendency, void *user)
+                 SFX(orc_tendency_hook) extra_tendency, void *user,
+                 const REAL *pm_O, const REAL *pm_kx, const REAL *pm_ky)
 {
     size_t cnt = (size_t)nx * ny * nk * nphi, c3 = (size_t)nx * ny * nk;
     REAL *G = (REAL *)malloc(cnt * sizeof(REAL));
     REAL *N0 = (REAL *)malloc(cnt * sizeof(REAL));
     REAL *S = (REAL *)malloc(cnt * sizeof(REAL));
-    REAL *gr = mode == 2 ? (REAL *)malloc(4 * c3 * sizeof(REAL)) : NULL;
-    if (!G || !N0 || !S || (mode == 2 && !gr)) {
+    REAL *gr = mode >= 2 ? (REAL *)malloc(4 * c3 * sizeof(REAL)) : NULL;
+    if (!G || !N0 || !S || (mode >= 2 && !gr)) {
         free(G); free(N0); free(S); free(gr);
         return -1;
     }
@@ -259,13 +353,8 @@ int SFX(orc_run)(int nx, int ny, int nk, int nphi, int mode, int stepper, REAL *
         if (nstage == 3)
             memcpy(N0, N, cnt * sizeof(REAL));
         for (int stage = 0; stage < nstage; ++stage) {
-            if (mode == 2)
-                SFX(orc_refraction_tendency)(nx, ny, nk, nphi, N, G, Ux, Uy, gr, gr + c3, gr + 2 * c3, gr + 3 * c3,
-                                             kappa_c, cgx, cgy, cos_t, sin_t, dx_inv, dy_inv, dk_inv, dp_inv);
-            else if (mode == 1)
-                SFX(orc_intrinsic_tendency)(nx, ny, nk, nphi, N, G, cgx, cgy, dx_inv, dy_inv);
-            else
-                memset(G, 0, cnt * sizeof(REAL));
+            SFX(transport_tendency)(nx, ny, nk, nphi, mode, N, G, Ux, Uy, gr, 0, pm_O, pm_kx, pm_ky, kappa_c, cgx,
+                                    cgy, cos_t, sin_t, dx_inv, dy_inv, dk_inv, dp_inv);
             if (extra_tendency)
                 extra_tendency(N, G, user);
             if (stage == 0) {
@@ -284,26 +373,21 @@ int SFX(orc_run)(int nx, int ny, int nk, int nphi, int mode, int stepper, REAL *
     return 0;
 }
 
-/* One tendency evaluation (the parity hook). */
+/* One tendency evaluation (the parity hook); in mode 3 Ux, Uy receive the
+ * coupling's velocity of N. */
 void SFX(orc_tendency)(int nx, int ny, int nk, int nphi, int mode, const REAL *N, REAL *G,
-                       const REAL *Ux, const REAL *Uy, const REAL *kappa_c,
+                       REAL *Ux, REAL *Uy, const REAL *kappa_c,
                        const REAL *cgx, const REAL *cgy, const REAL *cos_t, const REAL *sin_t,
-                       REAL dx_inv, REAL dy_inv, REAL dk_inv, REAL dp_inv)
+                       REAL dx_inv, REAL dy_inv, REAL dk_inv, REAL dp_inv,
+                       const REAL *pm_O, const REAL *pm_kx, const REAL *pm_ky)
 {
-    size_t cnt = (size_t)nx * ny * nk * nphi, c3 = (size_t)nx * ny * nk;
-    if (mode == 2) {
-        REAL *gr = (REAL *)malloc(4 * c3 * sizeof(REAL));
-        if (!gr)
-            return;
-        SFX(orc_current_gradients)(nx, ny, nk, Ux, Uy, dx_inv, dy_inv, gr, gr + c3, gr + 2 * c3, gr + 3 * c3);
-        SFX(orc_refraction_tendency)(nx, ny, nk, nphi, N, G, Ux, Uy, gr, gr + c3, gr + 2 * c3, gr + 3 * c3,
-                                     kappa_c, cgx, cgy, cos_t, sin_t, dx_inv, dy_inv, dk_inv, dp_inv);
-        free(gr);
-    } else if (mode == 1) {
-        SFX(orc_intrinsic_tendency)(nx, ny, nk, nphi, N, G, cgx, cgy, dx_inv, dy_inv);
-    } else {
-        memset(G, 0, cnt * sizeof(REAL));
-    }
+    size_t c3 = (size_t)nx * ny * nk;
+    REAL *gr = NULL;
+    if (mode >= 2 && !(gr = (REAL *)malloc(4 * c3 * sizeof(REAL))))
+        return;
+    SFX(transport_tendency)(nx, ny, nk, nphi, mode, N, G, Ux, Uy, gr, 1, pm_O, pm_kx, pm_ky, kappa_c, cgx, cgy,
+                            cos_t, sin_t, dx_inv, dy_inv, dk_inv, dp_inv);
+    free(gr);
 }
 
 /* Exposed for known-answer tests of the reconstruction itself. */

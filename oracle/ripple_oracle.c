@@ -188,6 +188,63 @@ void orc_q_weights(int nz, const double *zf, int nk, const double *kappa, double
         }
 }
 
+/* q_kernel.jl:30-33 (q_cell_integral): CDF(zhigh) - CDF(zlow). */
+double orc_q_cell_integral(double kappa, double z1, double z2, double depth)
+{
+    double lo = z1 < z2 ? z1 : z2, hi = z1 < z2 ? z2 : z1;
+    return orc_q_cdf(kappa, hi, depth, 1e-6) - orc_q_cdf(kappa, lo, depth, 1e-6);
+}
+
+/* q_kernel.jl:35-53 (q_cdf_kappa_derivative): d CDF / d kappa, with the
+ * small-mu branch depth * 4 mu s (s^2 - 1) / 3. */
+double orc_q_cdf_dkappa(double kappa, double z, double depth, double small_mu)
+{
+    double mu = kappa * depth;
+    double s = (z + depth) / depth;
+    s = s < 0 ? 0 : (s > 1 ? 1 : s);
+    if (fabs(mu) < small_mu)
+        return depth * (4 * mu * s * (s * s - 1) / 3);
+    double a = exp(-2 * mu * (1 - s));
+    double b = exp(-2 * mu * (1 + s));
+    double e = exp(-4 * mu);
+    double num = a - b, den = 1 - e;
+    double dnum = -2 * (1 - s) * a + 2 * (1 + s) * b;
+    double dden = 4 * e;
+    double dF = (dnum * den - num * dden) / (den * den);
+    return depth * dF;
+}
+
+/* q_kernel.jl:55-59 (q_cell_integral_kappa_derivative). */
+double orc_q_cell_integral_dkappa(double kappa, double z1, double z2, double depth)
+{
+    double lo = z1 < z2 ? z1 : z2, hi = z1 < z2 ? z2 : z1;
+    return orc_q_cdf_dkappa(kappa, hi, depth, 1e-6) - orc_q_cdf_dkappa(kappa, lo, depth, 1e-6);
+}
+
+/* Coupling/pseudomomentum.jl:103-122 (pseudomomentum_overlap) for a constant
+ * depth (pseudomomentum.jl:93-100):
+ *   O[t,s]  = sum_k W(kappa_t, k) W(kappa_s, k) / |dz_k|
+ *   dO[t,s] = sum_k dW/dkappa(kappa_t, k) W(kappa_s, k) / |dz_k|
+ * stored target fastest. */
+void orc_pm_overlap(int nz, const double *zf, int nk, const double *kappa, double depth, double *O, double *dO)
+{
+    for (int s = 0; s < nk; ++s)
+        for (int t = 0; t < nk; ++t) {
+            double total = 0, dtotal = 0;
+            for (int k = 0; k < nz; ++k) {
+                double z1 = zf[k], z2 = zf[k + 1];
+                double dz = fabs(z2 - z1);
+                double sw = orc_q_cell_integral(kappa[s], z1, z2, depth);
+                double tw = orc_q_cell_integral(kappa[t], z1, z2, depth);
+                double td = orc_q_cell_integral_dkappa(kappa[t], z1, z2, depth);
+                total += tw * sw / dz;
+                dtotal += td * sw / dz;
+            }
+            O[(size_t)t + (size_t)nk * s] = total;
+            dO[(size_t)t + (size_t)nk * s] = dtotal;
+        }
+}
+
 /* doppler_velocity.jl:45-62: U[i,j,m] = sum_k uL[i,j,k] * W[k,m], sequential
  * accumulation over k. */
 void orc_doppler(int nx, int ny, int nz, const double *uL, const double *vL, int nk,

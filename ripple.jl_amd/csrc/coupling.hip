@@ -1,6 +1,8 @@
-// Coupling precompute: the Q-transform Doppler contraction and the current
-// gradients.  Both run once per set_current (the current is prescribed and
-// static, Coupling/current_coupling.jl:127-138).
+// Coupling: the Q-transform Doppler contraction and the current gradients
+// (once per set_current: the prescribed current is static,
+// Coupling/current_coupling.jl:127-138), and the pseudomomentum contractions
+// of the action (before every tendency evaluation for the self-coupled
+// current, current_coupling.jl:140-143, and for the pseudomomentum fields).
 #include "rpl_internal.h"
 
 #pragma clang fp contract(off)
@@ -111,6 +113,133 @@ __global__ __launch_bounds__(256) void gradients_kernel(int nx, int ny, int nk, 
     Uyy[c] = (Uy[b + i + (size_t)nx * jp] - Uy[b + i + (size_t)nx * jm]) * hy;
 }
 
+// Pseudomomentum contractions of the action column (i, j):
+//   X[i,j,o] = scale_o * sum_{n, s} A[o,s] N[i,j,s,n] kx[s,n]     (Y with ky)
+// and, with a second matrix B, dX, dY the same with B in place of A.  A and B
+// are (nout x nk), o fastest.  With A the overlap O[t,s] (and B its kappa
+// derivative) this is the self-coupled Doppler velocity
+// (pseudomomentum.jl:316-341); with A[k,m] the Q cell weights W[k,m] and
+// scale_k = 1/|dz_k| (or none) it is the pseudomomentum cell average (or
+// integral) (pseudomomentum.jl:375-399).
+//
+// Reference order: per output, n outer, s inner, each term (A * N) * k, a
+// final product by scale_o when given.  One thread per (i, j) and a chunk of
+// PM_CHUNK outputs; consecutive threads take consecutive (i, j), so every
+// N[., ., s, n] load is coalesced and A, B, kx, ky are wave-uniform.
+constexpr int PM_CHUNK = 8;
+
+template <typename T>
+__global__ __launch_bounds__(256) void pm_ordered_kernel(size_t plane, int nk, int nphi, int nout,
+                                                         const T *__restrict__ N, const T *__restrict__ A,
+                                                         const T *__restrict__ B, const T *__restrict__ kx,
+                                                         const T *__restrict__ ky, const T *__restrict__ scale, T *X,
+                                                         T *Y, T *dX, T *dY)
+{
+    const size_t ij = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const int o0 = blockIdx.y * PM_CHUNK;
+    if (ij >= plane)
+        return;
+    T ax[PM_CHUNK], ay[PM_CHUNK], bx[PM_CHUNK], by[PM_CHUNK];
+#pragma unroll
+    for (int c = 0; c < PM_CHUNK; ++c)
+        ax[c] = ay[c] = bx[c] = by[c] = T(0);
+    for (int n = 0; n < nphi; ++n)
+        for (int s = 0; s < nk; ++s) {
+            const size_t sn = (size_t)s + (size_t)nk * n;
+            const T a = N[ij + plane * sn];
+            const T kxv = kx[sn], kyv = ky[sn];
+#pragma unroll
+            for (int c = 0; c < PM_CHUNK; ++c) {
+                const int o = o0 + c;
+                if (o < nout) {
+                    const T q = A[o + (size_t)nout * s];
+                    ax[c] += q * a * kxv;
+                    ay[c] += q * a * kyv;
+                    if (B) {
+                        const T dq = B[o + (size_t)nout * s];
+                        bx[c] += dq * a * kxv;
+                        by[c] += dq * a * kyv;
+                    }
+                }
+            }
+        }
+#pragma unroll
+    for (int c = 0; c < PM_CHUNK; ++c) {
+        const int o = o0 + c;
+        if (o < nout) {
+            const size_t out = ij + plane * o;
+            X[out] = scale ? ax[c] * scale[o] : ax[c];
+            Y[out] = scale ? ay[c] * scale[o] : ay[c];
+            if (B) {
+                dX[out] = bx[c];
+                dY[out] = by[c];
+            }
+        }
+    }
+}
+
+// The same contractions factored through the column's directional moments,
+// M_x[s] = sum_n N[s,n] kx[s,n] (and M_y), then X[o] = scale_o sum_s A[o,s]
+// M_x[s]: N is read once and the work per cell-bin is two products, so the
+// launch runs at the HBM rate of reading N.  A workgroup holds 64 columns;
+// its four wavefronts split the source bins s for the moments, which meet in
+// LDS (2 nk x 64 values), and then split the outputs o.  The sums are
+// regrouped, so the bits differ from the reference order by rounding only.
+template <typename T>
+__global__ __launch_bounds__(256) void pm_moment_kernel(size_t plane, int nk, int nphi, int nout,
+                                                        const T *__restrict__ N, const T *__restrict__ A,
+                                                        const T *__restrict__ B, const T *__restrict__ kx,
+                                                        const T *__restrict__ ky, const T *__restrict__ scale, T *X,
+                                                        T *Y, T *dX, T *dY)
+{
+    extern __shared__ unsigned char pm_lds[];
+    T *Mx = reinterpret_cast<T *>(pm_lds);
+    T *My = Mx + (size_t)nk * 64;
+    const int col = threadIdx.x & 63, grp = threadIdx.x >> 6;
+    const size_t ij = (size_t)blockIdx.x * 64 + col;
+    const bool in = ij < plane;
+    const size_t sstride = plane * nk;
+    for (int s = grp; s < nk; s += 4) {
+        T mx = T(0), my = T(0);
+        if (in) {
+            const T *Ns = N + ij + plane * s;
+#pragma unroll 6
+            for (int n = 0; n < nphi; ++n) {
+                const T a = Ns[sstride * n];
+                const size_t sn = (size_t)s + (size_t)nk * n;
+                mx += a * kx[sn];
+                my += a * ky[sn];
+            }
+        }
+        Mx[s * 64 + col] = mx;
+        My[s * 64 + col] = my;
+    }
+    __syncthreads();
+    if (!in)
+        return;
+    for (int o = grp; o < nout; o += 4) {
+        T x = T(0), y = T(0), dx = T(0), dy = T(0);
+        for (int s = 0; s < nk; ++s) {
+            const T mx = Mx[s * 64 + col], my = My[s * 64 + col];
+            const T q = A[o + (size_t)nout * s];
+            x += q * mx;
+            y += q * my;
+            if (B) {
+                const T dq = B[o + (size_t)nout * s];
+                dx += dq * mx;
+                dy += dq * my;
+            }
+        }
+        const size_t out = ij + plane * o;
+        X[out] = scale ? x * scale[o] : x;
+        Y[out] = scale ? y * scale[o] : y;
+        if (B) {
+            dX[out] = dx;
+            dY[out] = dy;
+        }
+    }
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void cast_from_f64_kernel(const double *__restrict__ src, T *dst, size_t n)
 {
@@ -180,6 +309,31 @@ template hipError_t launch_gradients<double>(int, int, int, const double *, cons
                                              double *, double *, double *, hipStream_t);
 template hipError_t launch_gradients<float>(int, int, int, const float *, const float *, float, float, float *,
                                             float *, float *, float *, hipStream_t);
+
+template <typename T>
+hipError_t launch_pm_contract(bool ordered, int nx, int ny, int nk, int nphi, int nout, const T *N, const T *A,
+                              const T *B, const T *kx, const T *ky, const T *scale, T *X, T *Y, T *dX, T *dY,
+                              hipStream_t s)
+{
+    const size_t plane = (size_t)nx * ny;
+    const size_t lds = 2 * (size_t)nk * 64 * sizeof(T);
+    if (ordered || lds > PM_MOMENT_LDS_MAX) {
+        dim3 grid((unsigned)((plane + 255) / 256), (unsigned)((nout + PM_CHUNK - 1) / PM_CHUNK));
+        hipLaunchKernelGGL(pm_ordered_kernel<T>, grid, dim3(256), 0, s, plane, nk, nphi, nout, N, A, B, kx, ky,
+                           scale, X, Y, dX, dY);
+    } else {
+        hipLaunchKernelGGL(pm_moment_kernel<T>, dim3((unsigned)((plane + 63) / 64)), dim3(256), lds, s, plane, nk,
+                           nphi, nout, N, A, B, kx, ky, scale, X, Y, dX, dY);
+    }
+    return hipGetLastError();
+}
+
+template hipError_t launch_pm_contract<float>(bool, int, int, int, int, int, const float *, const float *,
+                                              const float *, const float *, const float *, const float *, float *,
+                                              float *, float *, float *, hipStream_t);
+template hipError_t launch_pm_contract<double>(bool, int, int, int, int, int, const double *, const double *,
+                                               const double *, const double *, const double *, const double *,
+                                               double *, double *, double *, double *, hipStream_t);
 
 template <typename T>
 hipError_t launch_cast_from_f64(const double *src, T *dst, size_t n, hipStream_t s)

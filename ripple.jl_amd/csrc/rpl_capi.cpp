@@ -52,6 +52,14 @@ struct rpl_ctx {
 
     double dx = 0, dy = 0, dk = 0, dp = 0;
     bool has_geometry = false, has_current = false;
+    std::vector<double> h_mux, h_muy;  // first-moment measures (Nk, Nphi), kept for the pseudomomentum tables
+
+    // Doppler velocity kappa derivatives (allocated by the first coupling that sets them)
+    void *dUx = nullptr, *dUy = nullptr;
+    bool has_dU = false;
+    // self-coupled pseudomomentum current (rpl_set_pseudomomentum_coupling)
+    bool pm = false;
+    void *pm_O = nullptr, *pm_dO = nullptr, *pm_kx = nullptr, *pm_ky = nullptr;
     double time = 0;
     int64_t iteration = 0;
 
@@ -424,6 +432,56 @@ int record_end(rpl_ctx *c)
     if (c->events_used >= 3072)
         return fold_timing(c);
     return RPL_OK;
+}
+
+// Pseudomomentum contractions take the reference order unless the context
+// asked for the fast kernels (FP32 AUTO or FAST, FP64 FAST).
+bool pm_ordered(const rpl_ctx *c)
+{
+    return !(c->cfg.kernel == RPL_KERNEL_FAST || (!is_f64(c) && c->cfg.kernel == RPL_KERNEL_AUTO));
+}
+
+// update_coupling! of the self-coupled current (current_coupling.jl:140-143,
+// run first by compute_tendencies!, tendencies.jl:103-107): U and dU/dkappa of
+// the stage input Nk, then the gradients the reference-order stencil reads
+// (the fast stencil forms its own from U).
+int launch_coupling(rpl_ctx *c, const void *Nk)
+{
+    if (!c->pm)
+        return RPL_OK;
+    const int nx = c->cfg.nx, ny = c->cfg.ny, nk = c->cfg.nkappa, np = c->cfg.nphi;
+    const bool ordered = pm_ordered(c);
+    hipError_t e;
+    if (is_f64(c)) {
+        e = rpl::launch_pm_contract<double>(ordered, nx, ny, nk, np, nk, (const double *)Nk, (const double *)c->pm_O,
+                                            (const double *)c->pm_dO, (const double *)c->pm_kx,
+                                            (const double *)c->pm_ky, nullptr, (double *)c->Ux, (double *)c->Uy,
+                                            (double *)c->dUx, (double *)c->dUy, c->stream);
+        if (e == hipSuccess && !use_fast(c))
+            e = rpl::launch_gradients<double>(nx, ny, nk, (const double *)c->Ux, (const double *)c->Uy, 1.0 / c->dx,
+                                              1.0 / c->dy, (double *)c->Uxx, (double *)c->Uxy, (double *)c->Uyx,
+                                              (double *)c->Uyy, c->stream);
+    } else {
+        e = rpl::launch_pm_contract<float>(ordered, nx, ny, nk, np, nk, (const float *)Nk, (const float *)c->pm_O,
+                                           (const float *)c->pm_dO, (const float *)c->pm_kx, (const float *)c->pm_ky,
+                                           nullptr, (float *)c->Ux, (float *)c->Uy, (float *)c->dUx, (float *)c->dUy,
+                                           c->stream);
+        if (e == hipSuccess && !use_fast(c))
+            e = rpl::launch_gradients<float>(nx, ny, nk, (const float *)c->Ux, (const float *)c->Uy,
+                                             1.0f / (float)c->dx, 1.0f / (float)c->dy, (float *)c->Uxx,
+                                             (float *)c->Uxy, (float *)c->Uyx, (float *)c->Uyy, c->stream);
+    }
+    if (e != hipSuccess)
+        return fail(RPL_ERR_HIP, std::string("pseudomomentum coupling: ") + hipGetErrorString(e));
+    return RPL_OK;
+}
+
+// Everything a tendency evaluation needs before the stage kernel: the
+// coupling's velocity and the source state of the stage input.
+int prepare_stage(rpl_ctx *c, const void *Nk)
+{
+    int rc = launch_coupling(c, Nk);
+    return rc ? rc : launch_sources(c, Nk);
 }
 
 int launch_stage(rpl_ctx *c, double dt, int stage, const void *Nk, const void *N0, void *out)
@@ -832,7 +890,8 @@ int rpl_destroy(rpl_ctx *c)
                     c->t_kpow, c->t_spow, c->t_wc, c->f_cth, c->f_sth, c->f_logk, c->f_invc, c->f_sk3, c->f_sq,
                     c->f_logkap, c->f_invck, c->f_ctu, c->f_stu, c->f_lz1, (void *)c->sX, (void *)c->sA, (void *)c->sB, (void *)c->rate,
                     (void *)c->out_scale, (void *)c->s_err, (void *)c->use_plain, (void *)c->s_counts,
-                    (void *)c->bin_hi, (void *)c->s_amax, (void *)c->s_flags})
+                    (void *)c->bin_hi, (void *)c->s_amax, (void *)c->s_flags, c->dUx, c->dUy, c->pm_O,
+                    c->pm_dO, c->pm_kx, c->pm_ky})
         free_ptr(p);
     for (auto &ev : c->events) {
         (void)hipEventDestroy(ev.first);
@@ -913,7 +972,13 @@ int rpl_set_geometry(rpl_ctx *c, double dx, double dy, const double *kappa_c, co
     c->h_kf.assign(kappa_f, kappa_f + nk + 1);
     c->h_pc.assign(phi_c, phi_c + np);
     c->h_pf.assign(phi_f, phi_f + np + 1);
+    c->h_mux = mx;
+    c->h_muy = my;
     c->has_sources = false;  // source tables depend on the geometry
+    if (c->pm) {             // so do the pseudomomentum tables
+        c->pm = false;
+        c->has_current = false;
+    }
     int rc;
     if ((rc = upload_real(c, kc.data(), c->kappa, nk)) || (rc = upload_real(c, cgx.data(), c->cgx, tab)) ||
         (rc = upload_real(c, cgy.data(), c->cgy, tab)) || (rc = upload_real(c, cs.data(), c->cosp, np)) ||
@@ -963,6 +1028,48 @@ static double q_cdf(double kappa, double z, double depth)
     return std::fabs(mu) < small_mu ? s : fin;
 }
 
+// q_kernel.jl:30-33 (q_cell_integral): CDF(zhigh) - CDF(zlow).
+static double q_cell(double kappa, double z1, double z2, double depth)
+{
+    return q_cdf(kappa, std::max(z1, z2), depth) - q_cdf(kappa, std::min(z1, z2), depth);
+}
+
+// q_kernel.jl:35-53 (q_cdf_kappa_derivative): d CDF / d kappa of
+// sinh(2 mu s) / sinh(2 mu), mu = kappa d, in the reference's form; below
+// |mu| = 1e-6 its series depth * 4 mu s (s^2 - 1) / 3.
+static double q_cdf_dkappa(double kappa, double z, double depth)
+{
+    const double mu = kappa * depth;
+    double s = (z + depth) / depth;
+    s = s < 0 ? 0 : (s > 1 ? 1 : s);
+    if (std::fabs(mu) < 1e-6)
+        return depth * (4 * mu * s * (s * s - 1) / 3);
+    const double a = std::exp(-2 * mu * (1 - s)), b = std::exp(-2 * mu * (1 + s)), e = std::exp(-4 * mu);
+    const double num = a - b, den = 1 - e;
+    const double dnum = -2 * (1 - s) * a + 2 * (1 + s) * b, dden = 4 * e;
+    const double dF = (dnum * den - num * dden) / (den * den);
+    return depth * dF;
+}
+
+// q_kernel.jl:55-59 (q_cell_integral_kappa_derivative).
+static double q_cell_dkappa(double kappa, double z1, double z2, double depth)
+{
+    return q_cdf_dkappa(kappa, std::max(z1, z2), depth) - q_cdf_dkappa(kappa, std::min(z1, z2), depth);
+}
+
+static std::vector<double> host_kappa(rpl_ctx *c) { return c->h_kc; }
+
+static int ensure_dU(rpl_ctx *c)
+{
+    const size_t p3 = c->plane * c->cfg.nkappa;
+    int rc;
+    if (!c->dUx && (rc = alloc(&c->dUx, p3 * c->esize)))
+        return rc;
+    if (!c->dUy && (rc = alloc(&c->dUy, p3 * c->esize)))
+        return rc;
+    return RPL_OK;
+}
+
 int rpl_set_current(rpl_ctx *c, int32_t nz, const double *z_faces, const double *u, const double *v, double q_depth)
 {
     if (!c || !z_faces || !u || !v)
@@ -975,29 +1082,28 @@ int rpl_set_current(rpl_ctx *c, int32_t nz, const double *z_faces, const double 
         return fail(RPL_ERR_ARG, "depth must be positive");  // q_kernel.jl:19
     const int nx = c->cfg.nx, ny = c->cfg.ny, nk = c->cfg.nkappa;
     RPL_HIP(hipSetDevice(c->device));
-    std::vector<double> kc(nk);
-    if (is_f64(c)) {
-        RPL_HIP(hipMemcpy(kc.data(), c->kappa, nk * 8, hipMemcpyDeviceToHost));
-    } else {
-        std::vector<float> kf(nk);
-        RPL_HIP(hipMemcpy(kf.data(), c->kappa, nk * 4, hipMemcpyDeviceToHost));
-        for (int m = 0; m < nk; ++m)
-            kc[m] = kf[m];
-    }
-    // q_kernel.jl:87-92: W[k,m] = CDF(zhigh) - CDF(zlow), k fastest.
-    std::vector<double> W((size_t)nz * nk);
+    // The kappa centres in the context's element type, as the reference's
+    // coupling.kappa (current_coupling.jl:80).
+    std::vector<double> kc = host_kappa(c);
+    if (!is_f64(c))
+        for (double &k : kc)
+            k = (float)k;
+    // q_kernel.jl:87-92: W[k,m] = CDF(zhigh) - CDF(zlow), k fastest, and its
+    // kappa derivative (doppler_velocity.jl:25-43, 64-82).
+    std::vector<double> W((size_t)nz * nk), dWk((size_t)nz * nk);
     for (int m = 0; m < nk; ++m)
         for (int k = 0; k < nz; ++k) {
-            const double lo = std::min(z_faces[k], z_faces[k + 1]), hi = std::max(z_faces[k], z_faces[k + 1]);
-            W[k + (size_t)nz * m] = q_cdf(kc[m], hi, q_depth) - q_cdf(kc[m], lo, q_depth);
+            W[k + (size_t)nz * m] = q_cell(kc[m], z_faces[k], z_faces[k + 1], q_depth);
+            dWk[k + (size_t)nz * m] = q_cell_dkappa(kc[m], z_faces[k], z_faces[k + 1], q_depth);
         }
     const size_t nu = c->plane * nz;
-    void *du = nullptr, *dv = nullptr, *dW = nullptr;
+    void *du = nullptr, *dv = nullptr, *dW = nullptr, *dD = nullptr;
     int rc;
-    if ((rc = alloc(&du, nu * c->esize)) || (rc = alloc(&dv, nu * c->esize)) ||
-        (rc = alloc(&dW, W.size() * c->esize))) {
+    if ((rc = ensure_dU(c)) || (rc = alloc(&du, nu * c->esize)) || (rc = alloc(&dv, nu * c->esize)) ||
+        (rc = alloc(&dW, W.size() * c->esize)) || (rc = alloc(&dD, W.size() * c->esize))) {
         free_ptr(du);
         free_ptr(dv);
+        free_ptr(dW);
         return rc;
     }
     rc = upload_real(c, u, du, nu);
@@ -1005,6 +1111,8 @@ int rpl_set_current(rpl_ctx *c, int32_t nz, const double *z_faces, const double 
         rc = upload_real(c, v, dv, nu);
     if (!rc)
         rc = upload_real(c, W.data(), dW, W.size());
+    if (!rc)
+        rc = upload_real(c, dWk.data(), dD, dWk.size());
     if (!rc) {
         hipError_t e;
         if (is_f64(c) && c->cfg.kernel == RPL_KERNEL_FAST)
@@ -1016,6 +1124,16 @@ int rpl_set_current(rpl_ctx *c, int32_t nz, const double *z_faces, const double 
         else
             e = rpl::launch_doppler(nx, ny, nz, nk, (const float *)du, (const float *)dv, (const float *)dW,
                                     (float *)c->Ux, (float *)c->Uy, c->stream);
+        // dU/dkappa: the same contraction with the derivative weights.
+        if (e == hipSuccess && is_f64(c) && c->cfg.kernel == RPL_KERNEL_FAST)
+            e = rpl::launch_doppler_mfma<double>(nx, ny, nz, nk, (const double *)du, (const double *)dv,
+                                                 (const double *)dD, (double *)c->dUx, (double *)c->dUy, c->stream);
+        else if (e == hipSuccess && is_f64(c))
+            e = rpl::launch_doppler(nx, ny, nz, nk, (const double *)du, (const double *)dv, (const double *)dD,
+                                    (double *)c->dUx, (double *)c->dUy, c->stream);
+        else if (e == hipSuccess)
+            e = rpl::launch_doppler(nx, ny, nz, nk, (const float *)du, (const float *)dv, (const float *)dD,
+                                    (float *)c->dUx, (float *)c->dUy, c->stream);
         if (e != hipSuccess)
             rc = fail(RPL_ERR_HIP, std::string("doppler: ") + hipGetErrorString(e));
         else if ((e = hipStreamSynchronize(c->stream)) != hipSuccess)
@@ -1024,9 +1142,186 @@ int rpl_set_current(rpl_ctx *c, int32_t nz, const double *z_faces, const double 
     free_ptr(du);
     free_ptr(dv);
     free_ptr(dW);
+    free_ptr(dD);
     if (rc)
         return rc;
+    c->pm = false;
+    c->has_dU = true;
     return compute_gradients(c);
+}
+
+int rpl_set_pseudomomentum_coupling(rpl_ctx *c, int32_t nz, const double *z_faces, double q_depth)
+{
+    if (!c || !z_faces)
+        return fail(RPL_ERR_ARG, "null argument");
+    if (!c->has_geometry)
+        return fail(RPL_ERR_STATE, "rpl_set_geometry has not been called");
+    if (c->cfg.transport != RPL_TRANSPORT_REFRACTION)
+        return fail(RPL_ERR_STATE, "the pseudomomentum coupling drives the refraction transport (RPL_TRANSPORT_REFRACTION)");
+    if (nz < 1)
+        return fail(RPL_ERR_ARG, "nz must be >= 1");
+    if (!(q_depth > 0))
+        return fail(RPL_ERR_ARG, "depth must be positive");  // q_kernel.jl:19
+    for (int k = 0; k < nz; ++k)
+        if (!(z_faces[k] != z_faces[k + 1]))
+            return fail(RPL_ERR_ARG, "vertical cells must have nonzero thickness");
+    const int nk = c->cfg.nkappa;
+    RPL_HIP(hipSetDevice(c->device));
+    std::vector<double> kc = host_kappa(c);
+    if (!is_f64(c))
+        for (double &k : kc)
+            k = (float)k;
+    // pseudomomentum.jl:93-122: overlap O[t,s] = sum_k W_t W_s / |dz| and its
+    // derivative in the target kappa, target fastest.
+    std::vector<double> O((size_t)nk * nk), dO((size_t)nk * nk);
+    for (int src = 0; src < nk; ++src)
+        for (int t = 0; t < nk; ++t) {
+            double total = 0, dtotal = 0;
+            for (int k = 0; k < nz; ++k) {
+                const double z1 = z_faces[k], z2 = z_faces[k + 1], dz = std::fabs(z2 - z1);
+                const double sw = q_cell(kc[src], z1, z2, q_depth);
+                const double tw = q_cell(kc[t], z1, z2, q_depth);
+                const double td = q_cell_dkappa(kc[t], z1, z2, q_depth);
+                total += tw * sw / dz;
+                dtotal += td * sw / dz;
+            }
+            O[t + (size_t)nk * src] = total;
+            dO[t + (size_t)nk * src] = dtotal;
+        }
+    const size_t tab = (size_t)nk * c->cfg.nphi, p3 = c->plane * nk;
+    int rc;
+    if ((rc = ensure_dU(c)))
+        return rc;
+    for (void **b : {&c->pm_O, &c->pm_dO})
+        if (!*b && (rc = alloc(b, (size_t)nk * nk * c->esize)))
+            return rc;
+    for (void **b : {&c->pm_kx, &c->pm_ky})
+        if (!*b && (rc = alloc(b, tab * c->esize)))
+            return rc;
+    if ((rc = upload_real(c, O.data(), c->pm_O, O.size())) || (rc = upload_real(c, dO.data(), c->pm_dO, dO.size())) ||
+        (rc = upload_real(c, c->h_mux.data(), c->pm_kx, tab)) || (rc = upload_real(c, c->h_muy.data(), c->pm_ky, tab)))
+        return rc;
+    // A new coupling holds zero velocities until its first update
+    // (current_coupling.jl:104-109).
+    for (void *b : {c->Ux, c->Uy, c->dUx, c->dUy})
+        RPL_HIP(hipMemsetAsync(b, 0, p3 * c->esize, c->stream));
+    RPL_HIP(hipStreamSynchronize(c->stream));
+    c->pm = true;
+    c->has_dU = true;
+    c->has_current = true;
+    return RPL_OK;
+}
+
+int rpl_update_coupling(rpl_ctx *c)
+{
+    int rc = check_ready(c);
+    if (rc)
+        return rc;
+    if ((rc = launch_coupling(c, c->A)))
+        return rc;
+    RPL_HIP(hipStreamSynchronize(c->stream));
+    return RPL_OK;
+}
+
+int rpl_get_doppler_kappa_derivative(rpl_ctx *c, double *dUx, double *dUy)
+{
+    if (!c || !dUx || !dUy)
+        return fail(RPL_ERR_ARG, "null argument");
+    if (!c->has_dU)
+        return fail(RPL_ERR_STATE, "no coupling has set dU/dkappa (rpl_set_current or rpl_set_pseudomomentum_coupling)");
+    RPL_HIP(hipSetDevice(c->device));
+    RPL_HIP(hipStreamSynchronize(c->stream));
+    const size_t p3 = c->plane * c->cfg.nkappa;
+    if (is_f64(c)) {
+        RPL_HIP(hipMemcpy(dUx, c->dUx, p3 * 8, hipMemcpyDeviceToHost));
+        RPL_HIP(hipMemcpy(dUy, c->dUy, p3 * 8, hipMemcpyDeviceToHost));
+    } else {
+        std::vector<float> t(p3);
+        RPL_HIP(hipMemcpy(t.data(), c->dUx, p3 * 4, hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < p3; ++k)
+            dUx[k] = t[k];
+        RPL_HIP(hipMemcpy(t.data(), c->dUy, p3 * 4, hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < p3; ++k)
+            dUy[k] = t[k];
+    }
+    return RPL_OK;
+}
+
+int rpl_pseudomomentum(rpl_ctx *c, int32_t nz, const double *z_faces, double q_depth, int32_t cell_average,
+                       double *px, double *py)
+{
+    if (!c || !z_faces || !px || !py)
+        return fail(RPL_ERR_ARG, "null argument");
+    if (!c->has_geometry)
+        return fail(RPL_ERR_STATE, "rpl_set_geometry has not been called");
+    if (nz < 1)
+        return fail(RPL_ERR_ARG, "nz must be >= 1");
+    if (!(q_depth > 0))
+        return fail(RPL_ERR_ARG, "depth must be positive");
+    for (int k = 0; k < nz; ++k)
+        if (!(z_faces[k] != z_faces[k + 1]))
+            return fail(RPL_ERR_ARG, "vertical cells must have nonzero thickness");
+    const int nx = c->cfg.nx, ny = c->cfg.ny, nk = c->cfg.nkappa, np = c->cfg.nphi;
+    RPL_HIP(hipSetDevice(c->device));
+    std::vector<double> kc = host_kappa(c);
+    if (!is_f64(c))
+        for (double &k : kc)
+            k = (float)k;
+    // pseudomomentum.jl:375-399: W[k,m] (k fastest) and 1/|dz_k|.
+    std::vector<double> W((size_t)nz * nk), sc(nz);
+    for (int m = 0; m < nk; ++m)
+        for (int k = 0; k < nz; ++k)
+            W[k + (size_t)nz * m] = q_cell(kc[m], z_faces[k], z_faces[k + 1], q_depth);
+    for (int k = 0; k < nz; ++k)
+        sc[k] = 1 / std::fabs(z_faces[k + 1] - z_faces[k]);
+    const size_t tab = (size_t)nk * np, out = c->plane * nz;
+    void *dW = nullptr, *dS = nullptr, *dkx = nullptr, *dky = nullptr, *dpx = nullptr, *dpy = nullptr;
+    int rc = RPL_OK;
+    if ((rc = alloc(&dW, W.size() * c->esize)) || (rc = alloc(&dS, sc.size() * c->esize)) ||
+        (rc = alloc(&dkx, tab * c->esize)) || (rc = alloc(&dky, tab * c->esize)) ||
+        (rc = alloc(&dpx, out * c->esize)) || (rc = alloc(&dpy, out * c->esize)))
+        goto done;
+    if ((rc = upload_real(c, W.data(), dW, W.size())) || (rc = upload_real(c, sc.data(), dS, sc.size())) ||
+        (rc = upload_real(c, c->h_mux.data(), dkx, tab)) || (rc = upload_real(c, c->h_muy.data(), dky, tab)))
+        goto done;
+    {
+        hipError_t e;
+        const bool ordered = pm_ordered(c);
+        if (is_f64(c))
+            e = rpl::launch_pm_contract<double>(ordered, nx, ny, nk, np, nz, (const double *)c->A, (const double *)dW,
+                                                nullptr, (const double *)dkx, (const double *)dky,
+                                                cell_average ? (const double *)dS : nullptr, (double *)dpx,
+                                                (double *)dpy, nullptr, nullptr, c->stream);
+        else
+            e = rpl::launch_pm_contract<float>(ordered, nx, ny, nk, np, nz, (const float *)c->A, (const float *)dW,
+                                               nullptr, (const float *)dkx, (const float *)dky,
+                                               cell_average ? (const float *)dS : nullptr, (float *)dpx,
+                                               (float *)dpy, nullptr, nullptr, c->stream);
+        if (e == hipSuccess)
+            e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) {
+            rc = fail(RPL_ERR_HIP, std::string("pseudomomentum: ") + hipGetErrorString(e));
+            goto done;
+        }
+        if (is_f64(c)) {
+            if ((e = hipMemcpy(px, dpx, out * 8, hipMemcpyDeviceToHost)) == hipSuccess)
+                e = hipMemcpy(py, dpy, out * 8, hipMemcpyDeviceToHost);
+        } else {
+            std::vector<float> t(out);
+            if ((e = hipMemcpy(t.data(), dpx, out * 4, hipMemcpyDeviceToHost)) == hipSuccess)
+                for (size_t k = 0; k < out; ++k)
+                    px[k] = t[k];
+            if (e == hipSuccess && (e = hipMemcpy(t.data(), dpy, out * 4, hipMemcpyDeviceToHost)) == hipSuccess)
+                for (size_t k = 0; k < out; ++k)
+                    py[k] = t[k];
+        }
+        if (e != hipSuccess)
+            rc = fail(RPL_ERR_HIP, std::string("pseudomomentum copy: ") + hipGetErrorString(e));
+    }
+done:
+    for (void *p : {dW, dS, dkx, dky, dpx, dpy})
+        free_ptr(p);
+    return rc;
 }
 
 int rpl_set_doppler(rpl_ctx *c, const double *Ux, const double *Uy)
@@ -1040,6 +1335,8 @@ int rpl_set_doppler(rpl_ctx *c, const double *Ux, const double *Uy)
     int rc;
     if ((rc = upload_real(c, Ux, c->Ux, p3)) || (rc = upload_real(c, Uy, c->Uy, p3)))
         return rc;
+    c->pm = false;      // the velocities are now prescribed ...
+    c->has_dU = false;  // ... with no kappa derivative
     return compute_gradients(c);
 }
 
@@ -1092,7 +1389,7 @@ int rpl_tendency(rpl_ctx *c, void *G)
         return rc;
     if (!G)
         return fail(RPL_ERR_ARG, "null argument");
-    if ((rc = launch_wind(c)) || (rc = launch_sources(c, c->A)) ||
+    if ((rc = launch_wind(c)) || (rc = prepare_stage(c, c->A)) ||
         (rc = launch_stage(c, 0.0, rpl::STAGE_TENDENCY, c->A, c->A, c->B)))
         return rc;
     RPL_HIP(hipStreamSynchronize(c->stream));
@@ -1113,14 +1410,14 @@ int rpl_time_step(rpl_ctx *c, double dt, int32_t nsteps)
         if ((rc = launch_wind(c)))
             return rc;
         if (c->cfg.timestepper == RPL_FORWARD_EULER) {
-            if ((rc = launch_sources(c, c->A)) || (rc = launch_stage(c, dt, rpl::STAGE_1, c->A, c->A, c->B)))
+            if ((rc = prepare_stage(c, c->A)) || (rc = launch_stage(c, dt, rpl::STAGE_1, c->A, c->A, c->B)))
                 return rc;
             std::swap(c->A, c->B);
         } else {
             // SSP-RK3 over three buffers; A holds N0 and receives N3 in place.
-            if ((rc = launch_sources(c, c->A)) || (rc = launch_stage(c, dt, rpl::STAGE_1, c->A, c->A, c->B)) ||
-                (rc = launch_sources(c, c->B)) || (rc = launch_stage(c, dt, rpl::STAGE_2, c->B, c->A, c->C)) ||
-                (rc = launch_sources(c, c->C)) || (rc = launch_stage(c, dt, rpl::STAGE_3, c->C, c->A, c->A)))
+            if ((rc = prepare_stage(c, c->A)) || (rc = launch_stage(c, dt, rpl::STAGE_1, c->A, c->A, c->B)) ||
+                (rc = prepare_stage(c, c->B)) || (rc = launch_stage(c, dt, rpl::STAGE_2, c->B, c->A, c->C)) ||
+                (rc = prepare_stage(c, c->C)) || (rc = launch_stage(c, dt, rpl::STAGE_3, c->C, c->A, c->A)))
                 return rc;
         }
         c->time += dt;  // tick!(clock, dt), once per step
@@ -1652,6 +1949,8 @@ int rpl_ensemble_create(rpl_ensemble **out, rpl_ctx *const *members, int32_t cou
             return fail(RPL_ERR_UNSUPPORTED, "ensemble members need the same transport (intrinsic or refraction)");
         if (c->has_sources != first->has_sources)
             return fail(RPL_ERR_UNSUPPORTED, "either every member or no member has source terms");
+        if (c->pm)
+            return fail(RPL_ERR_UNSUPPORTED, "ensemble members take prescribed currents only (no pseudomomentum coupling)");
         if (c->device != first->device)
             return fail(RPL_ERR_ARG, "ensemble members must live on one device");
     }

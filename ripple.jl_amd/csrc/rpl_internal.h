@@ -121,6 +121,16 @@ template <typename T>
 hipError_t launch_gradients(int nx, int ny, int nk, const T *Ux, const T *Uy, T dx_inv, T dy_inv,
                             T *Uxx, T *Uxy, T *Uyx, T *Uyy, hipStream_t s);
 
+// Pseudomomentum contractions X[i,j,o] = scale_o sum_{n,s} A[o,s] N[i,j,s,n] kx[s,n]
+// (Y with ky; dX, dY with B when B is non-null; scale may be null).  ordered
+// keeps the reference's accumulation order; otherwise the sums go through the
+// column's directional moments (used when 2 nk 64 values fit PM_MOMENT_LDS_MAX).
+constexpr size_t PM_MOMENT_LDS_MAX = 64 * 1024;
+template <typename T>
+hipError_t launch_pm_contract(bool ordered, int nx, int ny, int nk, int nphi, int nout, const T *N, const T *A,
+                              const T *B, const T *kx, const T *ky, const T *scale, T *X, T *Y, T *dX, T *dY,
+                              hipStream_t s);
+
 // Diagnostics: per-column sums with FP64 accumulation; compensated totals.
 template <typename T>
 hipError_t launch_column_diagnostics(int nx, int ny, int nk, int nphi, const T *N,

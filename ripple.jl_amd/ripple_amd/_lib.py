@@ -83,6 +83,10 @@ SIGNATURES = {
     "rpl_set_current": ([_P, _I32, _DP, _DP, _DP, _D], ctypes.c_int),
     "rpl_set_doppler": ([_P, _DP, _DP], ctypes.c_int),
     "rpl_get_doppler": ([_P, _DP, _DP], ctypes.c_int),
+    "rpl_set_pseudomomentum_coupling": ([_P, _I32, _DP, _D], ctypes.c_int),
+    "rpl_update_coupling": ([_P], ctypes.c_int),
+    "rpl_get_doppler_kappa_derivative": ([_P, _DP, _DP], ctypes.c_int),
+    "rpl_pseudomomentum": ([_P, _I32, _DP, _D, _I32, _DP, _DP], ctypes.c_int),
     "rpl_set_action": ([_P, _P], ctypes.c_int),
     "rpl_get_action": ([_P, _P], ctypes.c_int),
     "rpl_tendency": ([_P, _P], ctypes.c_int),

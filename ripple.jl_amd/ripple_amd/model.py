@@ -12,7 +12,7 @@ import math
 import numpy as np
 
 from . import _lib
-from .grids import PERIODIC, RectilinearGrid, PolarWaveVectorGrid, FrequencyDirectionGrid, xnodes, ynodes
+from .grids import PERIODIC, BOUNDED, RectilinearGrid, PolarWaveVectorGrid, FrequencyDirectionGrid, xnodes, ynodes
 
 
 class InfiniteDepth:
@@ -26,6 +26,51 @@ WENO = "WENO"
 _TIMESTEPPERS = {"RK3": _lib.RPL_RK3, "LSRK3": _lib.RPL_LSRK3, "LowStorageRK3": _lib.RPL_LSRK3,
                  "ForwardEuler": _lib.RPL_FORWARD_EULER}
 _KERNELS = {"auto": _lib.RPL_KERNEL_AUTO, "reference": _lib.RPL_KERNEL_REFERENCE, "fast": _lib.RPL_KERNEL_FAST}
+
+
+class PseudomomentumVelocities:
+    """`PseudomomentumVelocities(; q_grid, vertical_grid, Nz, surface_spacing,
+    vertical_stretching)` (Coupling/lagrangian_velocities.jl:65-81): the
+    current is the wave field's own pseudomomentum, recomputed from the action
+    before every tendency evaluation.  The Q grid is `q_grid` (or
+    `vertical_grid`), else the model grid when it has vertical cells, else an
+    automatic stretched grid over the finite model depth."""
+
+    def __init__(self, *, q_grid=None, vertical_grid=None, depth=None, Nz=None, surface_spacing=None,
+                 vertical_stretching=2):
+        if depth is not None:
+            raise ValueError("`depth` is a SpectralWaveModel kwarg; pass `SpectralWaveModel(...; depth=...)` "
+                             "instead of putting `depth` in `velocities`")
+        if q_grid is not None and vertical_grid is not None:
+            raise ValueError("PseudomomentumVelocities cannot contain both `q_grid` and `vertical_grid`")
+        self.q_grid = q_grid if q_grid is not None else vertical_grid
+        self.Nz = Nz
+        self.surface_spacing = surface_spacing
+        self.vertical_stretching = vertical_stretching
+
+
+def automatic_q_grid(grid, spectral_grid, depth, Nz=None, surface_spacing=None, vertical_stretching=2):
+    """lagrangian_velocities.jl:196-225: Nz faces z_k = -H (1 - k/Nz)^stretch
+    over the finite depth H, with Nz = max(16, ceil((H / dz_surface)^(1/stretch)))
+    and dz_surface = 1 / (20 kappa_max) unless given."""
+    if not isinstance(spectral_grid, PolarWaveVectorGrid):
+        raise ValueError("automatic Q-grid selection requires a PolarWaveVectorGrid")
+    if depth is None or isinstance(depth, InfiniteDepth):
+        raise ValueError("automatic Q-grid selection requires finite model `depth`")
+    H = float(depth)
+    dz_surface = 1.0 / (20 * float(np.max(spectral_grid.kappa))) if surface_spacing is None else float(surface_spacing)
+    if not dz_surface > 0:
+        raise ValueError("automatic Q-grid `surface_spacing` must be positive")
+    stretch = float(vertical_stretching)
+    if not stretch > 1:
+        raise ValueError("automatic Q-grid `vertical_stretching` must be greater than 1")
+    Nz = max(16, math.ceil((H / dz_surface) ** (1 / stretch))) if Nz is None else int(Nz)
+    if Nz <= 0:
+        raise ValueError("automatic Q-grid `Nz` must be positive")
+    z = np.array([-H * (1 - k / Nz) ** stretch for k in range(Nz + 1)])
+    return RectilinearGrid(float_type=grid.float_type, size=(grid.Nx, grid.Ny, Nz),
+                           x=np.asarray(grid.x_faces), y=np.asarray(grid.y_faces), z=z,
+                           topology=(grid.topology[0], grid.topology[1], BOUNDED))
 
 
 class Clock:
@@ -95,6 +140,7 @@ class SpectralWaveModel:
         self.size = (grid.Nx, grid.Ny, Nk, Np)
 
         has_coupling = velocities is not None
+        self.pseudomomentum = isinstance(velocities, PseudomomentumVelocities)
         if has_coupling and not isinstance(spectral_grid, PolarWaveVectorGrid):
             raise ValueError("CWCM current coupling requires a PolarWaveVectorGrid")
         # Models/tendencies.jl:18-109.
@@ -123,7 +169,10 @@ class SpectralWaveModel:
                                         0.0 if depth_kind == _lib.RPL_DEPTH_INFINITE else float(self.depth),
                                         float(gravity)))
         self.velocities = None
-        if has_coupling:
+        self.q_grid = None
+        if self.pseudomomentum:
+            self._set_pseudomomentum(velocities)
+        elif has_coupling:
             self._set_velocities(velocities)
         _lib.check(lib.rpl_set_clock(ctx, self.clock.time, self.clock.iteration))
         if sources is not None:
@@ -153,6 +202,35 @@ class SpectralWaveModel:
                                                  _lib.dptr(vf), q_depth))
         self.velocities = (fields[0], fields[1])
 
+    def _q_depth(self, q_grid):
+        # lagrangian_velocities.jl:113-115: the Q grid's column depth for
+        # InfiniteDepth, else the model depth.
+        return q_grid.Lz if isinstance(self.depth, InfiniteDepth) else float(self.depth)
+
+    def _set_pseudomomentum(self, pv):
+        """build_coupling(::PseudomomentumVelocities) (lagrangian_velocities.jl:300-316)."""
+        g = self.grid
+        if pv.q_grid is not None:
+            q_grid = pv.q_grid
+            if (q_grid.Nx, q_grid.Ny) != (g.Nx, g.Ny) or not np.array_equal(q_grid.x_faces, g.x_faces) or \
+                    not np.array_equal(q_grid.y_faces, g.y_faces):
+                raise ValueError("the Q grid must share the model grid's horizontal cells")
+        elif g.Nz < 1:
+            if isinstance(self.depth, InfiniteDepth):
+                raise ValueError("Flat wave grids require finite model `depth` or `q_grid` for "
+                                 "PseudomomentumVelocities")
+            q_grid = automatic_q_grid(g, self.spectral_grid, self.depth, pv.Nz, pv.surface_spacing,
+                                      pv.vertical_stretching)
+        else:
+            q_grid = g
+        if q_grid.Nz < 1:
+            raise ValueError("QTransform requires a grid with a resolved vertical coordinate")
+        zf = np.ascontiguousarray(q_grid.z_faces, dtype=np.float64)
+        _lib.check(self._ctx.lib.rpl_set_pseudomomentum_coupling(self._ctx.ptr, q_grid.Nz, _lib.dptr(zf),
+                                                                 self._q_depth(q_grid)))
+        self.q_grid = q_grid
+        self.velocities = pv
+
     def doppler_velocity(self):
         """(Ux, Uy) caches, each (Nx, Ny, Nkappa)."""
         Nx, Ny, Nk, _ = self.size
@@ -160,6 +238,13 @@ class SpectralWaveModel:
         Uy = np.zeros(Nx * Ny * Nk)
         _lib.check(self._ctx.lib.rpl_get_doppler(self._ctx.ptr, _lib.dptr(Ux), _lib.dptr(Uy)))
         return Ux.reshape((Nx, Ny, Nk), order="F"), Uy.reshape((Nx, Ny, Nk), order="F")
+
+    def doppler_velocity_kappa_derivative(self):
+        """(dUx/dkappa, dUy/dkappa) caches, each (Nx, Ny, Nkappa)."""
+        Nx, Ny, Nk, _ = self.size
+        a, b = np.zeros(Nx * Ny * Nk), np.zeros(Nx * Ny * Nk)
+        _lib.check(self._ctx.lib.rpl_get_doppler_kappa_derivative(self._ctx.ptr, _lib.dptr(a), _lib.dptr(b)))
+        return a.reshape((Nx, Ny, Nk), order="F"), b.reshape((Nx, Ny, Nk), order="F")
 
     def set_doppler_velocity(self, Ux, Uy):
         a = [np.ascontiguousarray(np.asarray(x, dtype=np.float64).ravel(order="F")) for x in (Ux, Uy)]
@@ -238,6 +323,45 @@ def time_step_(model, dt, nsteps=1):
     _lib.check(model._ctx.lib.rpl_get_clock(model._ctx.ptr, ctypes.byref(t), ctypes.byref(it)))
     model.clock.time, model.clock.iteration = t.value, it.value
     return model
+
+
+def update_coupling_(model):
+    """`update_coupling!(model)` (Coupling/current_coupling.jl:118-147): the
+    pseudomomentum coupling's U and dU/dkappa from the current action; a no-op
+    for a prescribed current."""
+    _lib.check(model._ctx.lib.rpl_update_coupling(model._ctx.ptr))
+    return model
+
+
+def _pseudomomentum(model, depth, q_grid, cell_average):
+    q_grid = q_grid if q_grid is not None else (model.q_grid if model.q_grid is not None else model.grid)
+    if q_grid.Nz < 1:
+        raise ValueError("QTransform requires a grid with a resolved vertical coordinate")
+    d = model._q_depth(q_grid) if depth is None else float(depth)
+    zf = np.ascontiguousarray(q_grid.z_faces, dtype=np.float64)
+    Nx, Ny = model.size[:2]
+    px, py = np.zeros(Nx * Ny * q_grid.Nz), np.zeros(Nx * Ny * q_grid.Nz)
+    _lib.check(model._ctx.lib.rpl_pseudomomentum(model._ctx.ptr, q_grid.Nz, _lib.dptr(zf), d, int(cell_average),
+                                                 _lib.dptr(px), _lib.dptr(py)))
+    shp = (Nx, Ny, q_grid.Nz)
+    return px.reshape(shp, order="F"), py.reshape(shp, order="F")
+
+
+def compute_pseudomomentum_cell_averages(model, depth=None, q_grid=None):
+    """compute_pseudomomentum_cell_averages(N, depth, qtransform)
+    (Coupling/pseudomomentum.jl:190-200) of the model's action on the Q grid
+    (default: the coupling's, else the model grid) and depth (default: the
+    coupling's projection depth): (px, py), each (Nx, Ny, Nz)."""
+    return _pseudomomentum(model, depth, q_grid, True)
+
+
+def compute_pseudomomentum_cell_integrals(model, depth=None, q_grid=None):
+    """compute_pseudomomentum_cell_integrals (pseudomomentum.jl:180-188):
+    the cell averages times |dz_k|."""
+    return _pseudomomentum(model, depth, q_grid, False)
+
+
+pseudomomentum_fields = compute_pseudomomentum_cell_averages
 
 
 class Simulation:

@@ -215,3 +215,29 @@ def ensemble_member(index, float_type=np.float32, nz=16, sources=True, size_rang
             dissipation=LocalSaturationDissipation())
     name = f"ensemble_member_{index}_{n}x{n}x{nz}_k{nk}_p{nphi}_U{wind:.1f}"
     return Workload(name, grid, sg, (u, v), narrow_band_action(grid, sg), 0.02, 200, kw)
+
+
+def pseudomomentum_packet(float_type=np.float64, nx=64, ny=64, nz=16, nkappa=8, nphi=24, nsteps=20,
+                          amplitude=5.0, depth=10.0, direction=math.pi / 6):
+    """A swell packet that carries its own current: an nx x ny periodic box of
+    80 x 80 over z in [-depth, 0] with nz uniform levels (the Q grid), kappa
+    nkappa bins log-spaced 0.1-1.0, phi nphi bins; the narrow-band spectrum at
+    kappa0 = 0.4 (travelling toward phi = 0) times
+    amplitude * (0.2 + exp(-|x - c|^2 / (2 * 10^2))) about the box centre c;
+    the Doppler velocity is the packet's pseudomomentum projected through the
+    Q kernel (PseudomomentumVelocities), recomputed every stage; WENO5
+    refraction, SSP-RK3, dt = 0.05."""
+    from .model import PseudomomentumVelocities
+    Lx = Ly = 80.0
+    grid = RectilinearGrid(float_type=float_type, size=(nx, ny, nz), x=(0, Lx), y=(0, Ly), z=(-depth, 0),
+                           halo=(3, 3, 3), topology=(PERIODIC, PERIODIC, BOUNDED))
+    sg = PolarWaveVectorGrid(float_type, kappa=np.geomspace(0.1, 1.0, nkappa),
+                             phi=np.linspace(0, 2 * math.pi, nphi + 1)[:nphi])
+    x = xnodes(grid)[:, None]
+    y = ynodes(grid)[None, :]
+    env = amplitude * (0.2 + np.exp(-((x - Lx / 2) ** 2 + (y - Ly / 2) ** 2) / (2 * 10.0 ** 2)))
+    col = np.exp(-((sg.kappa[:, None] - 0.4) / 0.1) ** 2 - np.sin((sg.phi[None, :] - direction) / 2) ** 2 / 0.3 ** 2)
+    N = env[:, :, None, None] * col[None, None]
+    return Workload(f"pseudomomentum_packet_{nx}x{ny}x{nz}_k{nkappa}_p{nphi}", grid, sg, None,
+                    np.asfortranarray(N), 0.05, nsteps,
+                    dict(timestepper="RK3", velocities=PseudomomentumVelocities()))
