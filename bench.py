@@ -36,7 +36,9 @@ def build_model(wl, float_type, kernel="auto"):
                                  z=g.z_faces)
     sgrid = grids.PolarWaveVectorGrid(float_type, kappa=sg.kappa, phi=sg.phi, kappa_faces=sg.kappa_faces,
                                       phi_faces=sg.phi_faces)
-    m = ra.SpectralWaveModel(grid, sgrid, velocities=wl.velocities, kernel=kernel, **wl.model_kwargs)
+    kw = dict(wl.model_kwargs)
+    kw.setdefault("velocities", wl.velocities)
+    m = ra.SpectralWaveModel(grid, sgrid, kernel=kernel, **kw)
     ra.set_(m, N=wl.N0)
     return m
 
@@ -125,6 +127,28 @@ def storm_plain_vs_surrogate(steps, warmup):
                           "mlp_frac_of_bf16_dense_peak": flops / (t_inf * 1e-3) / 2.5e15,
                           "fallback_columns": st["plain"], "columns": st["columns"]},
             "step_speedup_surrogate_over_plain": t_plain / t_surr}
+
+
+def pseudomomentum_line(steps, warmup):
+    """The self-coupled pseudomomentum current at the headline shape (512 x 512
+    cells x 32 kappa x 36 phi, FP32): a swell packet carrying its own current,
+    Q grid of 32 levels over 10 m.  Each stage first contracts its input action
+    into U and dU/dkappa (pm_moment_kernel: reads N once, writes four
+    (nx, ny, nkappa) planes), then runs the refraction stage kernel."""
+    import ctypes
+    wl = workloads.pseudomomentum_packet(float_type=np.float32, nx=512, ny=512, nz=32, nkappa=32, nphi=36,
+                                         dt=0.01)
+    nx, ny, nk, nphi = wl.N0.shape
+    m = build_model(wl, np.float32)
+    t_step, l_stage, _ = time_model(m, wl.dt, steps, warmup)
+    ms, n = ctypes.c_double(), ctypes.c_int64()
+    _lib.check(m._ctx.lib.rpl_get_coupling_timing(m._ctx.ptr, ctypes.byref(ms), ctypes.byref(n)))
+    l_pm = ms.value / max(n.value, 1)
+    pm_bytes = 4 * (nx * ny * nk * nphi + 4 * nx * ny * nk)
+    return {"workload": wl.name, "value": wl.cell_bins / t_step, "ms_per_step": t_step * 1e3,
+            "stage_kernel_ms": l_stage, "coupling_update_ms": l_pm, "coupling_updates_timed": n.value,
+            "coupling_bytes_per_update": pm_bytes,
+            "coupling_achieved_GBs": pm_bytes / (l_pm * 1e-3) / 1e9 if l_pm > 0 else None}
 
 
 def ensemble_line(count, steps, with_surrogate=True, **member_kw):
@@ -243,6 +267,8 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="skip the FP64 and larger-grid lines")
     ap.add_argument("--no-storm", action="store_true", help="skip the plain vs surrogate hurricane line")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble line")
+    ap.add_argument("--only-pseudomomentum", action="store_true",
+                    help="print only the self-coupled pseudomomentum line (for profiling its kernels)")
     ap.add_argument("--ensemble-members", type=int, default=64)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy (reference for HBM byte counters) and exit")
@@ -260,6 +286,10 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_read": a.numel() * 4, "copy_bytes_written": b.numel() * 4, "launches": 3}))
+        return
+
+    if args.only_pseudomomentum:
+        print(json.dumps({"pseudomomentum_coupling": pseudomomentum_line(args.steps, args.warmup)}), flush=True)
         return
 
     wl = workloads.sheared_eddies()
@@ -307,6 +337,7 @@ def main():
             "achieved_GBs": stage_bytes_per_launch(1024, 1024, nk, nphi, 4) / (lb * 1e-3) / 1e9}
         del mb
         extra["err_vs_fp64"] = err_vs_fp64()
+        extra["pseudomomentum_coupling"] = pseudomomentum_line(max(3, args.steps // 2), 1)
     if not args.no_storm:
         extra["storm_plain_vs_surrogate"] = storm_plain_vs_surrogate(max(2, args.steps // 4), 1)
     if not args.no_ensemble:

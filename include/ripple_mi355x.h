@@ -293,6 +293,11 @@ int rpl_get_kernel_timing(rpl_ctx *ctx, double *total_ms, int64_t *launches);
  * same enable switch.  No counterpart in the reference. */
 int rpl_get_surrogate_timing(rpl_ctx *ctx, double *total_ms, int64_t *evaluations);
 
+/* Device time of the pseudomomentum coupling updates (one per tendency
+ * evaluation) since the last call, under the same enable switch.  No
+ * counterpart in the reference. */
+int rpl_get_coupling_timing(rpl_ctx *ctx, double *total_ms, int64_t *updates);
+
 /* Ensemble batching.  No counterpart in the reference, which runs one model
  * per process.  Members are ordinary contexts -- FP32 on the fast kernel,
  * SSP-RK3, one transport kind (intrinsic or refraction) and either all with

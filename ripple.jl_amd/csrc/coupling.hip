@@ -127,6 +127,7 @@ __global__ __launch_bounds__(256) void gradients_kernel(int nx, int ny, int nk, 
 // PM_CHUNK outputs; consecutive threads take consecutive (i, j), so every
 // N[., ., s, n] load is coalesced and A, B, kx, ky are wave-uniform.
 constexpr int PM_CHUNK = 8;
+constexpr int PM_OUT = 8;
 
 template <typename T>
 __global__ __launch_bounds__(256) void pm_ordered_kernel(size_t plane, int nk, int nphi, int nout,
@@ -183,22 +184,33 @@ __global__ __launch_bounds__(256) void pm_ordered_kernel(size_t plane, int nk, i
 // M_x[s]: N is read once and the work per cell-bin is two products, so the
 // launch runs at the HBM rate of reading N.  A workgroup holds 64 columns;
 // its four wavefronts split the source bins s for the moments, which meet in
-// LDS (2 nk x 64 values), and then split the outputs o.  The sums are
-// regrouped, so the bits differ from the reference order by rounding only.
+// LDS (2 nk x 64 values) next to the matrices, and then split the outputs o.
+// The sums are regrouped, so the bits differ from the reference order by
+// rounding only.
 template <typename T>
 __global__ __launch_bounds__(256) void pm_moment_kernel(size_t plane, int nk, int nphi, int nout,
                                                         const T *__restrict__ N, const T *__restrict__ A,
-                                                        const T *__restrict__ B, const T *__restrict__ kx,
-                                                        const T *__restrict__ ky, const T *__restrict__ scale, T *X,
-                                                        T *Y, T *dX, T *dY)
+                                                        const T *__restrict__ B,
+                                                        const T *__restrict__ kx, const T *__restrict__ ky,
+                                                        const T *__restrict__ scale, T *X, T *Y, T *dX, T *dY)
 {
     extern __shared__ unsigned char pm_lds[];
     T *Mx = reinterpret_cast<T *>(pm_lds);
     T *My = Mx + (size_t)nk * 64;
-    const int col = threadIdx.x & 63, grp = threadIdx.x >> 6;
+    // The matrices, A then B, [s][o] as in global memory.
+    T *Ash = My + (size_t)nk * 64;
+    T *Bsh = Ash + (size_t)nk * nout;
+    // The wavefront index, held in a scalar register so that the source bins
+    // and outputs each wavefront walks are scalar (uniform) values.
+    const int col = threadIdx.x & 63, grp = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const size_t ij = (size_t)blockIdx.x * 64 + col;
     const bool in = ij < plane;
     const size_t sstride = plane * nk;
+    for (int e = threadIdx.x; e < nk * nout; e += 256) {
+        Ash[e] = A[e];
+        if (B)
+            Bsh[e] = B[e];
+    }
     for (int s = grp; s < nk; s += 4) {
         T mx = T(0), my = T(0);
         if (in) {
@@ -217,25 +229,42 @@ __global__ __launch_bounds__(256) void pm_moment_kernel(size_t plane, int nk, in
     __syncthreads();
     if (!in)
         return;
-    for (int o = grp; o < nout; o += 4) {
-        T x = T(0), y = T(0), dx = T(0), dy = T(0);
+    // Each wavefront takes PM_OUT consecutive outputs at a time; the matrix
+    // entries it reads are the same for all its lanes (LDS broadcasts).
+    for (int o0 = grp * PM_OUT; o0 < nout; o0 += 4 * PM_OUT) {
+        T x[PM_OUT], y[PM_OUT], dx[PM_OUT], dy[PM_OUT];
+#pragma unroll
+        for (int c = 0; c < PM_OUT; ++c)
+            x[c] = y[c] = dx[c] = dy[c] = T(0);
         for (int s = 0; s < nk; ++s) {
             const T mx = Mx[s * 64 + col], my = My[s * 64 + col];
-            const T q = A[o + (size_t)nout * s];
-            x += q * mx;
-            y += q * my;
-            if (B) {
-                const T dq = B[o + (size_t)nout * s];
-                dx += dq * mx;
-                dy += dq * my;
+            const T *As = Ash + (size_t)nout * s + o0, *Bs = Bsh + (size_t)nout * s + o0;
+#pragma unroll
+            for (int c = 0; c < PM_OUT; ++c) {
+                if (o0 + c < nout) {  // uniform
+                    const T q = As[c];  // one LDS address for the whole wavefront
+                    x[c] += q * mx;
+                    y[c] += q * my;
+                    if (B) {
+                        const T dq = Bs[c];
+                        dx[c] += dq * mx;
+                        dy[c] += dq * my;
+                    }
+                }
             }
         }
-        const size_t out = ij + plane * o;
-        X[out] = scale ? x * scale[o] : x;
-        Y[out] = scale ? y * scale[o] : y;
-        if (B) {
-            dX[out] = dx;
-            dY[out] = dy;
+#pragma unroll
+        for (int c = 0; c < PM_OUT; ++c) {
+            const int o = o0 + c;
+            if (o < nout) {
+                const size_t out = ij + plane * o;
+                X[out] = scale ? x[c] * scale[o] : x[c];
+                Y[out] = scale ? y[c] * scale[o] : y[c];
+                if (B) {
+                    dX[out] = dx[c];
+                    dY[out] = dy[c];
+                }
+            }
         }
     }
 }
@@ -316,7 +345,7 @@ hipError_t launch_pm_contract(bool ordered, int nx, int ny, int nk, int nphi, in
                               hipStream_t s)
 {
     const size_t plane = (size_t)nx * ny;
-    const size_t lds = 2 * (size_t)nk * 64 * sizeof(T);
+    const size_t lds = pm_factored_lds(nk, nout, B != nullptr, sizeof(T));
     if (ordered || lds > PM_MOMENT_LDS_MAX) {
         dim3 grid((unsigned)((plane + 255) / 256), (unsigned)((nout + PM_CHUNK - 1) / PM_CHUNK));
         hipLaunchKernelGGL(pm_ordered_kernel<T>, grid, dim3(256), 0, s, plane, nk, nphi, nout, N, A, B, kx, ky,

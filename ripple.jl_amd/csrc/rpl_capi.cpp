@@ -95,13 +95,14 @@ struct rpl_ctx {
     int64_t s_columns = 0;
 
     // event timing: phase 0 = stage kernel launches, 1 = surrogate inference
-    // (features and network forward of one stage input)
+    // (features and network forward of one stage input), 2 = pseudomomentum
+    // coupling update (contraction and, on the reference path, gradients)
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
     std::vector<int> event_phase;
     size_t events_used = 0;
-    double timing_ms[2] = {0, 0};
-    int64_t timing_launches[2] = {0, 0};
+    double timing_ms[3] = {0, 0, 0};
+    int64_t timing_launches[3] = {0, 0, 0};
 };
 
 namespace {
@@ -451,9 +452,13 @@ int launch_coupling(rpl_ctx *c, const void *Nk)
         return RPL_OK;
     const int nx = c->cfg.nx, ny = c->cfg.ny, nk = c->cfg.nkappa, np = c->cfg.nphi;
     const bool ordered = pm_ordered(c);
+    int rc = record_begin(c, 2);
+    if (rc != RPL_OK)
+        return rc;
     hipError_t e;
     if (is_f64(c)) {
-        e = rpl::launch_pm_contract<double>(ordered, nx, ny, nk, np, nk, (const double *)Nk, (const double *)c->pm_O,
+        e = rpl::launch_pm_contract<double>(ordered, nx, ny, nk, np, nk, (const double *)Nk,
+                                            (const double *)c->pm_O,
                                             (const double *)c->pm_dO, (const double *)c->pm_kx,
                                             (const double *)c->pm_ky, nullptr, (double *)c->Ux, (double *)c->Uy,
                                             (double *)c->dUx, (double *)c->dUy, c->stream);
@@ -462,10 +467,9 @@ int launch_coupling(rpl_ctx *c, const void *Nk)
                                               1.0 / c->dy, (double *)c->Uxx, (double *)c->Uxy, (double *)c->Uyx,
                                               (double *)c->Uyy, c->stream);
     } else {
-        e = rpl::launch_pm_contract<float>(ordered, nx, ny, nk, np, nk, (const float *)Nk, (const float *)c->pm_O,
-                                           (const float *)c->pm_dO, (const float *)c->pm_kx, (const float *)c->pm_ky,
-                                           nullptr, (float *)c->Ux, (float *)c->Uy, (float *)c->dUx, (float *)c->dUy,
-                                           c->stream);
+        e = rpl::launch_pm_contract<float>(ordered, nx, ny, nk, np, nk, (const float *)Nk, (const float *)c->pm_O, (const float *)c->pm_dO, (const float *)c->pm_kx,
+                                           (const float *)c->pm_ky, nullptr, (float *)c->Ux, (float *)c->Uy,
+                                           (float *)c->dUx, (float *)c->dUy, c->stream);
         if (e == hipSuccess && !use_fast(c))
             e = rpl::launch_gradients<float>(nx, ny, nk, (const float *)c->Ux, (const float *)c->Uy,
                                              1.0f / (float)c->dx, 1.0f / (float)c->dy, (float *)c->Uxx,
@@ -473,7 +477,7 @@ int launch_coupling(rpl_ctx *c, const void *Nk)
     }
     if (e != hipSuccess)
         return fail(RPL_ERR_HIP, std::string("pseudomomentum coupling: ") + hipGetErrorString(e));
-    return RPL_OK;
+    return record_end(c);
 }
 
 // Everything a tendency evaluation needs before the stage kernel: the
@@ -1288,12 +1292,14 @@ int rpl_pseudomomentum(rpl_ctx *c, int32_t nz, const double *z_faces, double q_d
         hipError_t e;
         const bool ordered = pm_ordered(c);
         if (is_f64(c))
-            e = rpl::launch_pm_contract<double>(ordered, nx, ny, nk, np, nz, (const double *)c->A, (const double *)dW,
+            e = rpl::launch_pm_contract<double>(ordered, nx, ny, nk, np, nz, (const double *)c->A,
+                                                (const double *)dW,
                                                 nullptr, (const double *)dkx, (const double *)dky,
                                                 cell_average ? (const double *)dS : nullptr, (double *)dpx,
                                                 (double *)dpy, nullptr, nullptr, c->stream);
         else
-            e = rpl::launch_pm_contract<float>(ordered, nx, ny, nk, np, nz, (const float *)c->A, (const float *)dW,
+            e = rpl::launch_pm_contract<float>(ordered, nx, ny, nk, np, nz, (const float *)c->A,
+                                               (const float *)dW,
                                                nullptr, (const float *)dkx, (const float *)dky,
                                                cell_average ? (const float *)dS : nullptr, (float *)dpx,
                                                (float *)dpy, nullptr, nullptr, c->stream);
@@ -1798,6 +1804,22 @@ int rpl_get_kernel_timing(rpl_ctx *c, double *total_ms, int64_t *launches)
         *launches = c->timing_launches[0];
     c->timing_ms[0] = 0;
     c->timing_launches[0] = 0;
+    return RPL_OK;
+}
+
+int rpl_get_coupling_timing(rpl_ctx *c, double *total_ms, int64_t *updates)
+{
+    if (!c)
+        return fail(RPL_ERR_ARG, "null context");
+    int rc = fold_timing(c);
+    if (rc)
+        return rc;
+    if (total_ms)
+        *total_ms = c->timing_ms[2];
+    if (updates)
+        *updates = c->timing_launches[2];
+    c->timing_ms[2] = 0;
+    c->timing_launches[2] = 0;
     return RPL_OK;
 }
 

@@ -124,8 +124,14 @@ hipError_t launch_gradients(int nx, int ny, int nk, const T *Ux, const T *Uy, T 
 // Pseudomomentum contractions X[i,j,o] = scale_o sum_{n,s} A[o,s] N[i,j,s,n] kx[s,n]
 // (Y with ky; dX, dY with B when B is non-null; scale may be null).  ordered
 // keeps the reference's accumulation order; otherwise the sums go through the
-// column's directional moments (used when 2 nk 64 values fit PM_MOMENT_LDS_MAX).
+// column's directional moments.  The factored kernel holds the moments of 64
+// columns and the matrices in LDS (pm_factored_lds bytes, at most
+// PM_MOMENT_LDS_MAX); shapes beyond that take the ordered kernel.
 constexpr size_t PM_MOMENT_LDS_MAX = 64 * 1024;
+inline size_t pm_factored_lds(int nk, int nout, bool with_b, size_t esize)
+{
+    return (2 * (size_t)nk * 64 + (size_t)nk * nout * (with_b ? 2 : 1)) * esize;
+}
 template <typename T>
 hipError_t launch_pm_contract(bool ordered, int nx, int ny, int nk, int nphi, int nout, const T *N, const T *A,
                               const T *B, const T *kx, const T *ky, const T *scale, T *X, T *Y, T *dX, T *dY,

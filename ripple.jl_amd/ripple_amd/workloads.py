@@ -218,7 +218,7 @@ def ensemble_member(index, float_type=np.float32, nz=16, sources=True, size_rang
 
 
 def pseudomomentum_packet(float_type=np.float64, nx=64, ny=64, nz=16, nkappa=8, nphi=24, nsteps=20,
-                          amplitude=5.0, depth=10.0, direction=math.pi / 6):
+                          amplitude=5.0, depth=10.0, direction=math.pi / 6, dt=0.05):
     """A swell packet that carries its own current: an nx x ny periodic box of
     80 x 80 over z in [-depth, 0] with nz uniform levels (the Q grid), kappa
     nkappa bins log-spaced 0.1-1.0, phi nphi bins; the narrow-band spectrum at
@@ -226,7 +226,9 @@ def pseudomomentum_packet(float_type=np.float64, nx=64, ny=64, nz=16, nkappa=8, 
     amplitude * (0.2 + exp(-|x - c|^2 / (2 * 10^2))) about the box centre c;
     the Doppler velocity is the packet's pseudomomentum projected through the
     Q kernel (PseudomomentumVelocities), recomputed every stage; WENO5
-    refraction, SSP-RK3, dt = 0.05."""
+    refraction, SSP-RK3, time step dt (0.05 keeps the Courant number of the
+    fastest group velocity, about 5 m/s at kappa = 0.1, at 0.2 for 64 cells;
+    scale it with the cell size)."""
     from .model import PseudomomentumVelocities
     Lx = Ly = 80.0
     grid = RectilinearGrid(float_type=float_type, size=(nx, ny, nz), x=(0, Lx), y=(0, Ly), z=(-depth, 0),
@@ -239,5 +241,5 @@ def pseudomomentum_packet(float_type=np.float64, nx=64, ny=64, nz=16, nkappa=8, 
     col = np.exp(-((sg.kappa[:, None] - 0.4) / 0.1) ** 2 - np.sin((sg.phi[None, :] - direction) / 2) ** 2 / 0.3 ** 2)
     N = env[:, :, None, None] * col[None, None]
     return Workload(f"pseudomomentum_packet_{nx}x{ny}x{nz}_k{nkappa}_p{nphi}", grid, sg, None,
-                    np.asfortranarray(N), 0.05, nsteps,
+                    np.asfortranarray(N), dt, nsteps,
                     dict(timestepper="RK3", velocities=PseudomomentumVelocities()))
