@@ -106,20 +106,20 @@ int rpl_set_doppler(rpl_ctx *ctx, const double *Ux, const double *Uy);
 int rpl_get_doppler(rpl_ctx *ctx, double *Ux, double *Uy);
 
 /* Self-coupled pseudomomentum current: CWCMPseudomomentumCoupling
- * (Coupling/current_coupling.jl:93-118), built by build_coupling for
- * PseudomomentumVelocities (lagrangian_velocities.jl:300-316).  Builds the
+ * (Coupling/current_coupling.jl:93-121), built by build_coupling for
+ * PseudomomentumVelocities (lagrangian_velocities.jl:294-311).  Builds the
  * overlap O[t,s] = sum_k W_t,k W_s,k / |dz_k| and its target-kappa derivative
- * (pseudomomentum.jl:79-122, constant depth) and clears U and dU/dkappa.  From
+ * (pseudomomentum.jl:68-122, constant depth) and clears U and dU/dkappa.  From
  * then on every tendency evaluation first recomputes, from its own input N,
  *   U[i,j,t] = sum_{n,s} O[t,s] N[i,j,s,n] mu_x[s,n]  (Uy with mu_y)
  * and dU/dkappa with the derivative overlap (compute_tendencies!,
- * Models/tendencies.jl:103-107; pseudomomentum.jl:124-140, 316-341), then the
+ * Models/tendencies.jl:103-107; pseudomomentum.jl:124-140, 316-344), then the
  * current gradients and the refraction stage.  Needs RPL_TRANSPORT_REFRACTION.
  * rpl_set_current / rpl_set_doppler return the context to a prescribed
  * current.  Ensembles reject members with this coupling. */
 int rpl_set_pseudomomentum_coupling(rpl_ctx *ctx, int32_t nz, const double *z_faces, double q_depth);
 
-/* update_coupling!(model) (current_coupling.jl:118-147): recomputes the
+/* update_coupling!(model) (current_coupling.jl:123-148): recomputes the
  * pseudomomentum coupling's U and dU/dkappa from the current action; a no-op
  * for a prescribed current (static). */
 int rpl_update_coupling(rpl_ctx *ctx);
@@ -132,7 +132,7 @@ int rpl_update_coupling(rpl_ctx *ctx);
 int rpl_get_doppler_kappa_derivative(rpl_ctx *ctx, double *dUx, double *dUy);
 
 /* compute_pseudomomentum_cell_averages / _cell_integrals
- * (pseudomomentum.jl:159-200, 375-399) of the current action on the vertical
+ * (pseudomomentum.jl:162-210, 375-400) of the current action on the vertical
  * cells z_faces[0..nz]:
  *   p[i,j,k] = scale_k sum_{n,m} W[k,m] N[i,j,m,n] mu_x[m,n]  (py with mu_y)
  * with scale_k = 1/|dz_k| when cell_average is nonzero, else 1.  px, py:

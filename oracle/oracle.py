@@ -212,7 +212,7 @@ class OracleModel:
             Ux = Uy = np.zeros((nx, ny, nk))
         self.Ux, self.Uy = _c(Ux, t), _c(Uy, t)
         # Self-coupled pseudomomentum current: pseudomomentum=(z_faces, q_depth)
-        # (Coupling/current_coupling.jl:93-118); U follows every stage input.
+        # (Coupling/current_coupling.jl:93-121); U follows every stage input.
         self.pm = None
         if pseudomomentum is not None:
             zf, q_depth = pseudomomentum

@@ -224,7 +224,7 @@ void SFX(orc_column_sums)(int nx, int ny, int nk, int nphi, const REAL *N,
     }
 }
 
-/* Coupling/pseudomomentum.jl:316-341 (_compute_pseudomomentum_doppler_velocity_kernel!):
+/* Coupling/pseudomomentum.jl:316-344 (_compute_pseudomomentum_doppler_velocity_kernel!):
  *   U[i,j,t] = sum_{n, s} O[t,s] * N[i,j,s,n] * kx[s,n]
  * (and Uy with ky; dU/dkappa with the derivative overlap dO when non-NULL),
  * accumulated n outer, s inner, each term formed as (O * N) * k.  O and dO
@@ -261,7 +261,7 @@ void SFX(orc_pm_doppler)(int nx, int ny, int nk, int nphi, const REAL *N, const 
     }
 }
 
-/* Coupling/pseudomomentum.jl:375-399 (_compute_pseudomomentum_cells_kernel!):
+/* Coupling/pseudomomentum.jl:375-400 (_compute_pseudomomentum_cells_kernel!):
  *   p[i,j,k] = scale_k * sum_{n, m} W[k,m] * N[i,j,m,n] * kx[m,n]
  * with W the Q cell weights (nz x Nk, k fastest), n outer, m inner, and
  * scale_k = 1 / |z_{k+1} - z_k| for cell averages, 1 for cell integrals. */

@@ -222,7 +222,7 @@ double orc_q_cell_integral_dkappa(double kappa, double z1, double z2, double dep
 }
 
 /* Coupling/pseudomomentum.jl:103-122 (pseudomomentum_overlap) for a constant
- * depth (pseudomomentum.jl:93-100):
+ * depth (pseudomomentum.jl:89-101):
  *   O[t,s]  = sum_k W(kappa_t, k) W(kappa_s, k) / |dz_k|
  *   dO[t,s] = sum_k dW/dkappa(kappa_t, k) W(kappa_s, k) / |dz_k|
  * stored target fastest. */

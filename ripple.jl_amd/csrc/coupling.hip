@@ -118,9 +118,9 @@ __global__ __launch_bounds__(256) void gradients_kernel(int nx, int ny, int nk, 
 // and, with a second matrix B, dX, dY the same with B in place of A.  A and B
 // are (nout x nk), o fastest.  With A the overlap O[t,s] (and B its kappa
 // derivative) this is the self-coupled Doppler velocity
-// (pseudomomentum.jl:316-341); with A[k,m] the Q cell weights W[k,m] and
+// (pseudomomentum.jl:316-344); with A[k,m] the Q cell weights W[k,m] and
 // scale_k = 1/|dz_k| (or none) it is the pseudomomentum cell average (or
-// integral) (pseudomomentum.jl:375-399).
+// integral) (pseudomomentum.jl:375-400).
 //
 // Reference order: per output, n outer, s inner, each term (A * N) * k, a
 // final product by scale_o when given.  One thread per (i, j) and a chunk of

@@ -1175,7 +1175,7 @@ int rpl_set_pseudomomentum_coupling(rpl_ctx *c, int32_t nz, const double *z_face
     if (!is_f64(c))
         for (double &k : kc)
             k = (float)k;
-    // pseudomomentum.jl:93-122: overlap O[t,s] = sum_k W_t W_s / |dz| and its
+    // pseudomomentum.jl:89-122: overlap O[t,s] = sum_k W_t W_s / |dz| and its
     // derivative in the target kappa, target fastest.
     std::vector<double> O((size_t)nk * nk), dO((size_t)nk * nk);
     for (int src = 0; src < nk; ++src)
@@ -1271,7 +1271,7 @@ int rpl_pseudomomentum(rpl_ctx *c, int32_t nz, const double *z_faces, double q_d
     if (!is_f64(c))
         for (double &k : kc)
             k = (float)k;
-    // pseudomomentum.jl:375-399: W[k,m] (k fastest) and 1/|dz_k|.
+    // pseudomomentum.jl:375-400: W[k,m] (k fastest) and 1/|dz_k|.
     std::vector<double> W((size_t)nz * nk), sc(nz);
     for (int m = 0; m < nk; ++m)
         for (int k = 0; k < nz; ++k)

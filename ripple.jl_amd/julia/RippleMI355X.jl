@@ -24,6 +24,8 @@ module RippleMI355X
 using Libdl
 
 export MI355X, MI355XWaveModel, set!, time_step!, interior_action, doppler_velocity,
+       PseudomomentumVelocities, update_coupling!, doppler_velocity_kappa_derivative,
+       compute_pseudomomentum_cell_averages, compute_pseudomomentum_cell_integrals,
        m0, root_mean_square_wavenumber, mean_direction, significant_wave_height,
        total_action, total_deep_water_energy, Simulation, run!
 
@@ -63,6 +65,19 @@ function check(code)
     msg = unsafe_string(ccall(sym(:rpl_last_error), Cstring, ()))
     code == -1 ? throw(ArgumentError(msg)) : error("ripple_mi355x error $code: $msg")
 end
+
+"""
+    PseudomomentumVelocities(; q_grid=nothing)
+
+The self-coupled current (Coupling/lagrangian_velocities.jl:65-81): the
+Doppler velocity is the wave field's own pseudomomentum, recomputed from the
+action before every tendency evaluation.  `q_grid` (anything with
+`z_faces`) sets the vertical cells; by default the model grid's.
+"""
+struct PseudomomentumVelocities{G}
+    q_grid::G
+end
+PseudomomentumVelocities(; q_grid=nothing) = PseudomomentumVelocities(q_grid)
 
 mutable struct Clock
     time::Float64
@@ -117,7 +132,13 @@ function MI355XWaveModel(grid, spectral_grid; velocities=nothing, horizontal_adv
                 (Ptr{Cvoid}, Cdouble, Cdouble, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble},
                  Int32, Cdouble, Cdouble),
                 model.ctx, xf[2] - xf[1], yf[2] - yf[1], κ, κf, φ, φf, depth_kind, d, gravity))
-    if velocities !== nothing
+    if velocities isa PseudomomentumVelocities
+        qz = velocities.q_grid === nothing ? zf : collect(Float64, velocities.q_grid.z_faces)
+        length(qz) >= 2 || throw(ArgumentError("PseudomomentumVelocities needs vertical cells (a q_grid or a 3-D grid)"))
+        q_depth = depth === nothing ? qz[end] - qz[1] : Float64(depth)
+        check(ccall(sym(:rpl_set_pseudomomentum_coupling), Cint, (Ptr{Cvoid}, Int32, Ptr{Cdouble}, Cdouble),
+                    model.ctx, length(qz) - 1, qz, q_depth))
+    elseif velocities !== nothing
         u = collect(Float64, velocities.u)
         v = collect(Float64, velocities.v)
         Nz = length(zf) - 1
@@ -172,6 +193,30 @@ function doppler_velocity(model::MI355XWaveModel)
     check(ccall(sym(:rpl_get_doppler), Cint, (Ptr{Cvoid}, Ptr{Cdouble}, Ptr{Cdouble}), model.ctx, Ux, Uy))
     return Ux, Uy
 end
+
+"`update_coupling!(model)` (Coupling/current_coupling.jl:123-148)."
+update_coupling!(model::MI355XWaveModel) = (check(ccall(sym(:rpl_update_coupling), Cint, (Ptr{Cvoid},), model.ctx)); model)
+
+function doppler_velocity_kappa_derivative(model::MI355XWaveModel)
+    Nx, Ny, Nκ, _ = model.size
+    dUx, dUy = zeros(Nx, Ny, Nκ), zeros(Nx, Ny, Nκ)
+    check(ccall(sym(:rpl_get_doppler_kappa_derivative), Cint, (Ptr{Cvoid}, Ptr{Cdouble}, Ptr{Cdouble}),
+                model.ctx, dUx, dUy))
+    return dUx, dUy
+end
+
+"Pseudomomentum cell averages (or integrals) on the vertical cells `z_faces` (Coupling/pseudomomentum.jl:162-210)."
+function _pseudomomentum(model::MI355XWaveModel, z_faces, depth, average)
+    zf = collect(Float64, z_faces)
+    Nx, Ny = model.size[1], model.size[2]
+    px, py = zeros(Nx, Ny, length(zf) - 1), zeros(Nx, Ny, length(zf) - 1)
+    check(ccall(sym(:rpl_pseudomomentum), Cint,
+                (Ptr{Cvoid}, Int32, Ptr{Cdouble}, Cdouble, Int32, Ptr{Cdouble}, Ptr{Cdouble}),
+                model.ctx, length(zf) - 1, zf, Float64(depth), Int32(average), px, py))
+    return px, py
+end
+compute_pseudomomentum_cell_averages(model, z_faces, depth) = _pseudomomentum(model, z_faces, depth, true)
+compute_pseudomomentum_cell_integrals(model, z_faces, depth) = _pseudomomentum(model, z_faces, depth, false)
 
 function _columns(model, which)
     Nx, Ny = model.size[1], model.size[2]

@@ -208,7 +208,7 @@ class SpectralWaveModel:
         return q_grid.Lz if isinstance(self.depth, InfiniteDepth) else float(self.depth)
 
     def _set_pseudomomentum(self, pv):
-        """build_coupling(::PseudomomentumVelocities) (lagrangian_velocities.jl:300-316)."""
+        """build_coupling(::PseudomomentumVelocities) (lagrangian_velocities.jl:294-311)."""
         g = self.grid
         if pv.q_grid is not None:
             q_grid = pv.q_grid
@@ -326,7 +326,7 @@ def time_step_(model, dt, nsteps=1):
 
 
 def update_coupling_(model):
-    """`update_coupling!(model)` (Coupling/current_coupling.jl:118-147): the
+    """`update_coupling!(model)` (Coupling/current_coupling.jl:123-148): the
     pseudomomentum coupling's U and dU/dkappa from the current action; a no-op
     for a prescribed current."""
     _lib.check(model._ctx.lib.rpl_update_coupling(model._ctx.ptr))
@@ -349,14 +349,14 @@ def _pseudomomentum(model, depth, q_grid, cell_average):
 
 def compute_pseudomomentum_cell_averages(model, depth=None, q_grid=None):
     """compute_pseudomomentum_cell_averages(N, depth, qtransform)
-    (Coupling/pseudomomentum.jl:190-200) of the model's action on the Q grid
+    (Coupling/pseudomomentum.jl:200-210) of the model's action on the Q grid
     (default: the coupling's, else the model grid) and depth (default: the
     coupling's projection depth): (px, py), each (Nx, Ny, Nz)."""
     return _pseudomomentum(model, depth, q_grid, True)
 
 
 def compute_pseudomomentum_cell_integrals(model, depth=None, q_grid=None):
-    """compute_pseudomomentum_cell_integrals (pseudomomentum.jl:180-188):
+    """compute_pseudomomentum_cell_integrals (pseudomomentum.jl:189-199):
     the cell averages times |dz_k|."""
     return _pseudomomentum(model, depth, q_grid, False)
 

@@ -202,7 +202,7 @@ def test_pseudomomentum_error_paths():
     with pytest.raises(ValueError):
         ra.PseudomomentumVelocities(q_grid=g, vertical_grid=g)
     flat = grids.RectilinearGrid(size=(16, 16), x=(0, 80), y=(0, 80))
-    with pytest.raises(ValueError):  # lagrangian_velocities.jl:303-305
+    with pytest.raises(ValueError):  # lagrangian_velocities.jl:297-299
         ra.SpectralWaveModel(flat, sg, velocities=ra.PseudomomentumVelocities())
     # A flat grid with a finite depth takes the automatic Q grid (lagrangian_velocities.jl:196-225).
     m = ra.SpectralWaveModel(flat, sg, velocities=ra.PseudomomentumVelocities(Nz=7), depth=2.0,
