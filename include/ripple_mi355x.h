@@ -119,6 +119,18 @@ int rpl_get_doppler(rpl_ctx *ctx, double *Ux, double *Uy);
  * current.  Ensembles reject members with this coupling. */
 int rpl_set_pseudomomentum_coupling(rpl_ctx *ctx, int32_t nz, const double *z_faces, double q_depth);
 
+/* SpatialAveraging(; alpha_s, alpha_n) (Models/propagation_smoothing.jl:22-51),
+ * the Tolman (2002) garden-sprinkler alleviation that
+ * SpectralWaveModel(...; propagation_smoothing) applies after every time step
+ * (time_step.jl:167; propagation_smoothing.jl:160-199): each (kappa, phi)
+ * bin's x-y field becomes the mean of four bilinear interpolations at the
+ * corners of a rectangle with half-widths alpha_s |dcg| dt along the bin's
+ * direction and alpha_n cg dphi dt across it (cg = sqrt(9.81/kappa)/2,
+ * propagation_smoothing.jl:117-135).  enable = 0 turns it off.  Needs
+ * rpl_set_geometry first; a new geometry turns it off.  Ensembles reject
+ * members with smoothing. */
+int rpl_set_propagation_smoothing(rpl_ctx *ctx, int32_t enable, double alpha_s, double alpha_n);
+
 /* update_coupling!(model) (current_coupling.jl:123-148): recomputes the
  * pseudomomentum coupling's U and dU/dkappa from the current action; a no-op
  * for a prescribed current (static). */

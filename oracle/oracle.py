@@ -163,11 +163,38 @@ def pm_cells(N, W, z_faces, kx, ky, cell_average=True, dtype=np.float64):
     return px.reshape((nx, ny, nz), order="F"), py.reshape((nx, ny, nz), order="F")
 
 
+def gse_tables(kappa, dtype=np.float64):
+    """SpatialAveraging's per-kappa group speed and its half neighbour difference."""
+    t = np.dtype(dtype)
+    k = _c(kappa, t)
+    cg, dcg = np.zeros(len(k), dtype=t), np.zeros(len(k), dtype=t)
+    sfx = "f64" if t == np.float64 else "f32"
+    getattr(lib(), f"orc_gse_tables_{sfx}")(_I(len(k)), _p(k), _p(cg), _p(dcg))
+    return cg, dcg
+
+
+def spatial_averaging(N, kappa, phi, dphi, dx, dy, dt, alpha_s=0.5, alpha_n=0.5, dtype=np.float64):
+    """One SpatialAveraging pass (Models/propagation_smoothing.jl:160-199) of N (Nx, Ny, Nk, Nphi)."""
+    t = np.dtype(dtype)
+    nx, ny, nk, npf = N.shape
+    cg, dcg = gse_tables(kappa, t)
+    cs = _c([math.cos(p) for p in phi], t)
+    sn = _c([math.sin(p) for p in phi], t)
+    Nin = _c(N, t)
+    out = np.zeros_like(Nin)
+    sc = _D if t == np.float64 else _F
+    one = t.type(1)
+    getattr(lib(), f"orc_spatial_averaging_{'f64' if t == np.float64 else 'f32'}")(
+        _I(nx), _I(ny), _I(nk), _I(npf), _p(Nin), _p(out), _p(cg), _p(dcg), _p(cs), _p(sn), sc(dphi),
+        sc(alpha_s), sc(alpha_n), sc(dt), sc(one / t.type(dx)), sc(one / t.type(dy)))
+    return out.reshape((nx, ny, nk, npf), order="F")
+
+
 class OracleModel:
     """The oracle's version of SpectralWaveModel over a Workload's inputs."""
 
     def __init__(self, workload, dtype=np.float64, depth=None, gravity=9.81, doppler_override=None,
-                 transport=None, timestepper=None, pseudomomentum=None):
+                 transport=None, timestepper=None, pseudomomentum=None, smoothing=None):
         wl = workload
         self.dtype = np.dtype(dtype)
         g, sg = wl.grid, wl.spectral_grid
@@ -220,6 +247,9 @@ class OracleModel:
             self.pm = [_c(O, t), _c(tabs["mu_x"], t), _c(tabs["mu_y"], t)]
             self.mode = MODE_PSEUDOMOMENTUM
             self.Ux, self.Uy = np.zeros(nx * ny * nk, dtype=t), np.zeros(nx * ny * nk, dtype=t)
+        # SpatialAveraging after every step: smoothing=(alpha_s, alpha_n).
+        self.smoothing = smoothing
+        self._sg = sg
         self.N = _c(wl.N0, t).copy()  # never alias the workload's array
         self.time, self.iteration = 0.0, 0
 
@@ -232,6 +262,17 @@ class OracleModel:
         return _D(x) if self.dtype == np.float64 else _F(x)
 
     def step(self, dt, nsteps=1, hook=None, user=None):
+        if self.smoothing is not None:
+            # apply_propagation_smoothing! closes every step (time_step.jl:167).
+            for _ in range(nsteps):
+                self._step(dt, 1, hook, user)
+                sg = self._sg
+                self.N = _c(spatial_averaging(self.action, sg.kappa, sg.phi, self.dp, self.dx, self.dy, dt,
+                                              *self.smoothing, dtype=self.dtype), self.dtype)
+            return
+        self._step(dt, nsteps, hook, user)
+
+    def _step(self, dt, nsteps=1, hook=None, user=None):
         L = lib()
         nx, ny, nk, npf = self.size
         sfx = "f64" if self.dtype == np.float64 else "f32"

@@ -390,6 +390,76 @@ void SFX(orc_tendency)(int nx, int ny, int nk, int nphi, int mode, const REAL *N
     free(gr);
 }
 
+/* Models/propagation_smoothing.jl:117-157 (ensure_spatial_averaging_tables!):
+ * deep-water group speed per kappa bin, cg[m] = 1/2 sqrt(g / kappa[m]) with
+ * g = 9.81 in the element type, and half the difference of its neighbours,
+ * dcg[m] = |cg[m-1] - cg[m+1]| / 2 (the bin itself at the two ends). */
+void SFX(orc_gse_tables)(int nk, const REAL *kappa, REAL *cg, REAL *dcg)
+{
+    const REAL g = (REAL)9.81, half = (REAL)1 / (REAL)2;
+    for (int m = 0; m < nk; ++m)
+        cg[m] = half * (REAL)sqrt(g / kappa[m]);
+    for (int m = 0; m < nk; ++m) {
+        REAL lo = m == 0 ? cg[0] : half * (REAL)sqrt(g / kappa[m - 1]);
+        REAL hi = m == nk - 1 ? cg[nk - 1] : half * (REAL)sqrt(g / kappa[m + 1]);
+        REAL d = lo - hi;
+        dcg[m] = (d < 0 ? -d : d) / (REAL)2;
+    }
+}
+
+/* Wrap of a 1-based index onto 1..n: one period either way, as the
+ * reference's _gse_periodic (propagation_smoothing.jl:53), then further
+ * periods for displacements the reference would read out of range. */
+static inline int SFX(gse_wrap)(int i, int n)
+{
+    i = i < 1 ? i + n : (i > n ? i - n : i);
+    if (i < 1 || i > n)
+        i = ((i - 1) % n + n) % n + 1;
+    return i;
+}
+
+/* Models/propagation_smoothing.jl:56-110 (_spatial_averaging_kernel!) on a
+ * periodic x-y grid: for every cell the mean of four bilinear
+ * interpolations at the corners (s1 Ls e_s + s2 Ln e_n), s1, s2 = -1, +1
+ * (s1 outer), of a rectangle with half-widths Ls = as dcg dt along the bin's
+ * direction and Ln = an cg dphi dt across it, in cell units.  Indices are
+ * formed 1-based as in the reference so the fractional weights round the
+ * same way. */
+void SFX(orc_spatial_averaging)(int nx, int ny, int nk, int nphi, const REAL *Nin, REAL *Nout,
+                                const REAL *cg, const REAL *dcg, const REAL *cos_t, const REAL *sin_t,
+                                REAL dphi, REAL as, REAL an, REAL dt, REAL dx_inv, REAL dy_inv)
+{
+#pragma omp parallel for schedule(static)
+    for (long mn = 0; mn < (long)nk * nphi; ++mn) {
+        int m = (int)(mn % nk), n = (int)(mn / nk);
+        REAL Ls = as * dcg[m] * dt;
+        REAL Ln = an * cg[m] * dphi * dt;
+        REAL c = cos_t[n], s = sin_t[n];
+        REAL Ls_x = Ls * c * dx_inv, Ls_y = Ls * s * dy_inv;
+        REAL Ln_x = -Ln * s * dx_inv, Ln_y = Ln * c * dy_inv;
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                REAL acc = 0;
+                for (int s1 = -1; s1 <= 1; s1 += 2)
+                    for (int s2 = -1; s2 <= 1; s2 += 2) {
+                        REAL di = (REAL)s1 * Ls_x + (REAL)s2 * Ln_x;
+                        REAL dj = (REAL)s1 * Ls_y + (REAL)s2 * Ln_y;
+                        REAL ir = (REAL)(i + 1) + di, jr = (REAL)(j + 1) + dj;
+                        int i0 = (int)floor(ir), j0 = (int)floor(jr);
+                        REAL wi = ir - (REAL)i0, wj = jr - (REAL)j0;
+                        int i0c = SFX(gse_wrap)(i0, nx) - 1, i1c = SFX(gse_wrap)(i0 + 1, nx) - 1;
+                        int j0c = SFX(gse_wrap)(j0, ny) - 1, j1c = SFX(gse_wrap)(j0 + 1, ny) - 1;
+                        REAL f00 = Nin[IDX4(i0c, j0c, m, n)], f10 = Nin[IDX4(i1c, j0c, m, n)];
+                        REAL f01 = Nin[IDX4(i0c, j1c, m, n)], f11 = Nin[IDX4(i1c, j1c, m, n)];
+                        REAL one = 1;
+                        acc += (one - wi) * (one - wj) * f00 + wi * (one - wj) * f10 + (one - wi) * wj * f01 +
+                               wi * wj * f11;
+                    }
+                Nout[IDX4(i, j, m, n)] = acc / (REAL)4;
+            }
+    }
+}
+
 /* Exposed for known-answer tests of the reconstruction itself. */
 REAL SFX(orc_weno5)(REAL fm2, REAL fm1, REAL f0, REAL fp1, REAL fp2)
 {

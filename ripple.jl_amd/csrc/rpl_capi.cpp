@@ -60,6 +60,10 @@ struct rpl_ctx {
     // self-coupled pseudomomentum current (rpl_set_pseudomomentum_coupling)
     bool pm = false;
     void *pm_O = nullptr, *pm_dO = nullptr, *pm_kx = nullptr, *pm_ky = nullptr;
+    // SpatialAveraging after every step (rpl_set_propagation_smoothing)
+    bool gse = false;
+    double gse_as = 0.5, gse_an = 0.5;
+    void *gse_cg = nullptr, *gse_dcg = nullptr;
     double time = 0;
     int64_t iteration = 0;
 
@@ -512,6 +516,32 @@ int launch_stage(rpl_ctx *c, double dt, int stage, const void *Nk, const void *N
     return record_end(c);
 }
 
+// apply_propagation_smoothing! (Models/propagation_smoothing.jl:160-199):
+// the averaged field goes to B, which then becomes the action (the
+// reference copies it back; a buffer swap does the same here).
+int launch_smoothing(rpl_ctx *c, double dt)
+{
+    if (!c->gse)
+        return RPL_OK;
+    const int nx = c->cfg.nx, ny = c->cfg.ny, nk = c->cfg.nkappa, np = c->cfg.nphi;
+    hipError_t e;
+    if (is_f64(c))
+        e = rpl::launch_spatial_averaging<double>(nx, ny, nk, np, (const double *)c->A, (double *)c->B,
+                                                  (const double *)c->gse_cg, (const double *)c->gse_dcg,
+                                                  (const double *)c->cosp, (const double *)c->sinp, c->dp,
+                                                  c->gse_as, c->gse_an, dt, 1 / c->dx, 1 / c->dy, c->stream);
+    else
+        e = rpl::launch_spatial_averaging<float>(nx, ny, nk, np, (const float *)c->A, (float *)c->B,
+                                                 (const float *)c->gse_cg, (const float *)c->gse_dcg,
+                                                 (const float *)c->cosp, (const float *)c->sinp, (float)c->dp,
+                                                 (float)c->gse_as, (float)c->gse_an, (float)dt,
+                                                 1.0f / (float)c->dx, 1.0f / (float)c->dy, c->stream);
+    if (e != hipSuccess)
+        return fail(RPL_ERR_HIP, std::string("spatial averaging: ") + hipGetErrorString(e));
+    std::swap(c->A, c->B);
+    return RPL_OK;
+}
+
 int check_ready(rpl_ctx *c)
 {
     if (!c)
@@ -895,7 +925,7 @@ int rpl_destroy(rpl_ctx *c)
                     c->f_logkap, c->f_invck, c->f_ctu, c->f_stu, c->f_lz1, (void *)c->sX, (void *)c->sA, (void *)c->sB, (void *)c->rate,
                     (void *)c->out_scale, (void *)c->s_err, (void *)c->use_plain, (void *)c->s_counts,
                     (void *)c->bin_hi, (void *)c->s_amax, (void *)c->s_flags, c->dUx, c->dUy, c->pm_O,
-                    c->pm_dO, c->pm_kx, c->pm_ky})
+                    c->pm_dO, c->pm_kx, c->pm_ky, c->gse_cg, c->gse_dcg})
         free_ptr(p);
     for (auto &ev : c->events) {
         (void)hipEventDestroy(ev.first);
@@ -983,6 +1013,7 @@ int rpl_set_geometry(rpl_ctx *c, double dx, double dy, const double *kappa_c, co
         c->pm = false;
         c->has_current = false;
     }
+    c->gse = false;          // and the smoothing tables
     int rc;
     if ((rc = upload_real(c, kc.data(), c->kappa, nk)) || (rc = upload_real(c, cgx.data(), c->cgx, tab)) ||
         (rc = upload_real(c, cgy.data(), c->cgy, tab)) || (rc = upload_real(c, cs.data(), c->cosp, np)) ||
@@ -1227,6 +1258,50 @@ int rpl_update_coupling(rpl_ctx *c)
     return RPL_OK;
 }
 
+int rpl_set_propagation_smoothing(rpl_ctx *c, int32_t enable, double alpha_s, double alpha_n)
+{
+    if (!c)
+        return fail(RPL_ERR_ARG, "null context");
+    if (!c->has_geometry)
+        return fail(RPL_ERR_STATE, "rpl_set_geometry has not been called");
+    if (!enable) {
+        c->gse = false;
+        return RPL_OK;
+    }
+    if (!(alpha_s >= 0) || !(alpha_n >= 0) || !std::isfinite(alpha_s) || !std::isfinite(alpha_n))
+        return fail(RPL_ERR_ARG, "SpatialAveraging alphas must be finite and non-negative");
+    const int nk = c->cfg.nkappa;
+    RPL_HIP(hipSetDevice(c->device));
+    // propagation_smoothing.jl:117-135 in the element type: cg = 1/2 sqrt(g/kappa)
+    // with g = 9.81, dcg = |cg[m-1] - cg[m+1]| / 2 (the bin itself at the ends).
+    std::vector<double> cg(nk), dcg(nk);
+    if (is_f64(c)) {
+        auto speed = [&](int m) { return 1.0 / 2.0 * std::sqrt(9.81 / c->h_kc[m]); };
+        for (int m = 0; m < nk; ++m)
+            cg[m] = speed(m);
+        for (int m = 0; m < nk; ++m)
+            dcg[m] = std::fabs((m == 0 ? cg[0] : speed(m - 1)) - (m == nk - 1 ? cg[nk - 1] : speed(m + 1))) / 2.0;
+    } else {
+        auto speed = [&](int m) { return 1.0f / 2.0f * std::sqrt(9.81f / (float)c->h_kc[m]); };
+        for (int m = 0; m < nk; ++m)
+            cg[m] = speed(m);
+        for (int m = 0; m < nk; ++m)
+            dcg[m] = std::fabs((m == 0 ? (float)cg[0] : speed(m - 1)) - (m == nk - 1 ? (float)cg[nk - 1] : speed(m + 1))) /
+                     2.0f;
+    }
+    int rc;
+    for (void **b : {&c->gse_cg, &c->gse_dcg})
+        if (!*b && (rc = alloc(b, nk * c->esize)))
+            return rc;
+    if ((rc = upload_real(c, cg.data(), c->gse_cg, nk)) || (rc = upload_real(c, dcg.data(), c->gse_dcg, nk)))
+        return rc;
+    RPL_HIP(hipStreamSynchronize(c->stream));
+    c->gse_as = alpha_s;
+    c->gse_an = alpha_n;
+    c->gse = true;
+    return RPL_OK;
+}
+
 int rpl_get_doppler_kappa_derivative(rpl_ctx *c, double *dUx, double *dUy)
 {
     if (!c || !dUx || !dUy)
@@ -1426,6 +1501,8 @@ int rpl_time_step(rpl_ctx *c, double dt, int32_t nsteps)
                 (rc = prepare_stage(c, c->C)) || (rc = launch_stage(c, dt, rpl::STAGE_3, c->C, c->A, c->A)))
                 return rc;
         }
+        if ((rc = launch_smoothing(c, dt)))  // apply_propagation_smoothing!, time_step.jl:167
+            return rc;
         c->time += dt;  // tick!(clock, dt), once per step
         c->iteration += 1;
     }
@@ -1973,6 +2050,8 @@ int rpl_ensemble_create(rpl_ensemble **out, rpl_ctx *const *members, int32_t cou
             return fail(RPL_ERR_UNSUPPORTED, "either every member or no member has source terms");
         if (c->pm)
             return fail(RPL_ERR_UNSUPPORTED, "ensemble members take prescribed currents only (no pseudomomentum coupling)");
+        if (c->gse)
+            return fail(RPL_ERR_UNSUPPORTED, "ensemble members run without propagation smoothing");
         if (c->device != first->device)
             return fail(RPL_ERR_ARG, "ensemble members must live on one device");
     }

@@ -137,6 +137,14 @@ hipError_t launch_pm_contract(bool ordered, int nx, int ny, int nk, int nphi, in
                               const T *B, const T *kx, const T *ky, const T *scale, T *X, T *Y, T *dX, T *dY,
                               hipStream_t s);
 
+// SpatialAveraging (Models/propagation_smoothing.jl:56-110): Nout = mean of
+// four bilinear corner interpolations of Nin per (kappa, phi) bin, periodic
+// x and y; cg, dcg per kappa bin, cos/sin of the phi centres.
+template <typename T>
+hipError_t launch_spatial_averaging(int nx, int ny, int nk, int nphi, const T *Nin, T *Nout, const T *cg,
+                                    const T *dcg, const T *cosp, const T *sinp, T dphi, T as, T an, T dt, T dx_inv,
+                                    T dy_inv, hipStream_t s);
+
 // Diagnostics: per-column sums with FP64 accumulation; compensated totals.
 template <typename T>
 hipError_t launch_column_diagnostics(int nx, int ny, int nk, int nphi, const T *N,

@@ -11,7 +11,7 @@ from .grids import (RectilinearGrid, PolarWaveVectorGrid, FrequencyDirectionGrid
                     spectral_cell_measures, xnodes, ynodes, znodes, xfaces, yfaces, zfaces,
                     xspacings, yspacings, zspacings, PERIODIC, BOUNDED, FLAT)
 from .model import (SpectralWaveModel, InfiniteDepth, Clock, Simulation, WENO, set_, time_step_, run_,
-                    interior, PseudomomentumVelocities, automatic_q_grid, update_coupling_,
+                    interior, PseudomomentumVelocities, SpatialAveraging, automatic_q_grid, update_coupling_,
                     compute_pseudomomentum_cell_averages, compute_pseudomomentum_cell_integrals,
                     pseudomomentum_fields)
 from .sources import (BulkWindDrag, LinearStormTrack, HollandHurricaneWind, PressureCorrelationInput,
@@ -29,7 +29,7 @@ __all__ = [
     "spectral_cell_measures", "xnodes", "ynodes", "znodes", "xfaces", "yfaces", "zfaces",
     "xspacings", "yspacings", "zspacings", "PERIODIC", "BOUNDED", "FLAT",
     "SpectralWaveModel", "InfiniteDepth", "Clock", "Simulation", "WENO", "set_", "time_step_", "run_",
-    "interior", "PseudomomentumVelocities", "automatic_q_grid", "update_coupling_",
+    "interior", "PseudomomentumVelocities", "SpatialAveraging", "automatic_q_grid", "update_coupling_",
     "compute_pseudomomentum_cell_averages", "compute_pseudomomentum_cell_integrals", "pseudomomentum_fields",
     "m0", "root_mean_square_wavenumber", "mean_direction", "significant_wave_height",
     "total_action", "total_deep_water_energy",

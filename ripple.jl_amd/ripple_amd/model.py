@@ -73,6 +73,16 @@ def automatic_q_grid(grid, spectral_grid, depth, Nz=None, surface_spacing=None, 
                            topology=(grid.topology[0], grid.topology[1], BOUNDED))
 
 
+class SpatialAveraging:
+    """`SpatialAveraging(; αs=0.5, αn=0.5)` (Models/propagation_smoothing.jl:22-51):
+    Tolman's garden-sprinkler alleviation, applied to the action after every
+    time step."""
+
+    def __init__(self, alpha_s=0.5, alpha_n=0.5):
+        self.alpha_s = float(alpha_s)
+        self.alpha_n = float(alpha_n)
+
+
 class Clock:
     def __init__(self, time=0.0, iteration=0):
         self.time = float(time)
@@ -116,7 +126,7 @@ class SpectralWaveModel:
 
     def __init__(self, grid, spectral_grid, *, velocities=None, horizontal_advection=WENO,
                  spectral_advection=WENO, sources=None, depth=None, timestepper="ForwardEuler",
-                 clock=None, kernel="auto", gravity=9.81):
+                 clock=None, kernel="auto", gravity=9.81, propagation_smoothing=None):
         if not isinstance(grid, RectilinearGrid):
             raise TypeError(f"grid must be a RectilinearGrid; got {type(grid).__name__}")
         if not isinstance(spectral_grid, (PolarWaveVectorGrid, FrequencyDirectionGrid)):
@@ -175,6 +185,12 @@ class SpectralWaveModel:
         elif has_coupling:
             self._set_velocities(velocities)
         _lib.check(lib.rpl_set_clock(ctx, self.clock.time, self.clock.iteration))
+        self.propagation_smoothing = propagation_smoothing
+        if propagation_smoothing is not None:
+            if not isinstance(propagation_smoothing, SpatialAveraging):
+                raise TypeError("propagation_smoothing must be a SpatialAveraging (or None)")
+            _lib.check(lib.rpl_set_propagation_smoothing(ctx, 1, propagation_smoothing.alpha_s,
+                                                         propagation_smoothing.alpha_n))
         if sources is not None:
             from .sources import attach
             attach(self, sources)

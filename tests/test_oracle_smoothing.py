@@ -1,0 +1,103 @@
+"""Pins the oracle's SpatialAveraging (Models/propagation_smoothing.jl:22-206),
+which the reference's test suite does not exercise: against an independent
+bilinear interpolator (scipy.ndimage.map_coordinates with periodic wrap),
+exact identities (no displacement, uniform fields, linear fields, per-bin
+conservation on a periodic grid) and the table formulas (CPU only)."""
+import math
+
+import numpy as np
+import pytest
+from scipy import ndimage
+
+from ripple_amd import workloads
+
+
+def _setup(nx=24, ny=18, nk=5, nphi=8, seed=0):
+    rng = np.random.default_rng(seed)
+    kappa = np.geomspace(0.05, 1.5, nk)
+    phi = np.linspace(0, 2 * math.pi, nphi + 1)[:nphi] + 0.1
+    dphi = 2 * math.pi / nphi
+    N = np.asfortranarray(rng.uniform(0.1, 1.0, size=(nx, ny, nk, nphi)))
+    return N, kappa, phi, dphi
+
+
+def _scipy_average(N, kappa, phi, dphi, dx, dy, dt, a_s, a_n):
+    """The four-corner mean computed with scipy's bilinear interpolation
+    (order=1, periodic 'grid-wrap'), displacements in cell units."""
+    nx, ny, nk, nphi = N.shape
+    g = 9.81
+    cg = 0.5 * np.sqrt(g / kappa)
+    lo = np.concatenate(([cg[0]], cg[:-1]))
+    hi = np.concatenate((cg[1:], [cg[-1]]))
+    dcg = np.abs(lo - hi) / 2
+    ii, jj = np.meshgrid(np.arange(nx, dtype=float), np.arange(ny, dtype=float), indexing="ij")
+    out = np.zeros_like(N)
+    for m in range(nk):
+        Ls, Ln = a_s * dcg[m] * dt, a_n * cg[m] * dphi * dt
+        for n in range(nphi):
+            es = np.array([math.cos(phi[n]) / dx, math.sin(phi[n]) / dy])
+            en = np.array([-math.sin(phi[n]) / dx, math.cos(phi[n]) / dy])
+            acc = np.zeros((nx, ny))
+            for s1 in (-1, 1):
+                for s2 in (-1, 1):
+                    d = s1 * Ls * es + s2 * Ln * en
+                    acc += ndimage.map_coordinates(N[:, :, m, n], [ii + d[0], jj + d[1]], order=1, mode="grid-wrap")
+            out[:, :, m, n] = acc / 4
+    return out
+
+
+@pytest.mark.parametrize("dt,a_s,a_n", [(40.0, 0.5, 0.5), (90.0, 1.5, 0.75), (7.0, 0.0, 1.0)])
+def test_matches_independent_bilinear_interpolation(oracle, dt, a_s, a_n):
+    N, kappa, phi, dphi = _setup()
+    dx, dy = 2.0, 3.0
+    got = oracle.spatial_averaging(N, kappa, phi, dphi, dx, dy, dt, a_s, a_n)
+    want = _scipy_average(N, kappa, phi, dphi, dx, dy, dt, a_s, a_n)
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-13)
+    # The displacement reaches several cells for the slow bins here.
+    assert np.abs(got - N).max() > 1e-3
+
+
+def test_identities(oracle):
+    N, kappa, phi, dphi = _setup(seed=3)
+    dx, dy, dt = 1.5, 1.5, 30.0
+    # No displacement: each corner weight is exactly (1, 0, 0, 0).
+    assert np.array_equal(oracle.spatial_averaging(N, kappa, phi, dphi, dx, dy, 0.0), N)
+    assert np.array_equal(oracle.spatial_averaging(N, kappa, phi, dphi, dx, dy, dt, 0.0, 0.0), N)
+    # A uniform field stays uniform (the weights sum to one).
+    U = np.full_like(N, 0.37)
+    np.testing.assert_allclose(oracle.spatial_averaging(U, kappa, phi, dphi, dx, dy, dt), U, rtol=1e-15)
+    # Per-bin totals are conserved on the periodic grid: every cell is a
+    # shifted copy with weights summing to one.
+    out = oracle.spatial_averaging(N, kappa, phi, dphi, dx, dy, dt)
+    np.testing.assert_allclose(out.sum(axis=(0, 1)), N.sum(axis=(0, 1)), rtol=1e-13)
+    # A field linear in x and y keeps its value away from the wrap (the four
+    # corners sit symmetrically about the cell).
+    nx, ny, nk, nphi = N.shape
+    x = np.arange(nx)[:, None, None, None]
+    y = np.arange(ny)[None, :, None, None]
+    L = np.asfortranarray(np.broadcast_to(1.0 + 0.2 * x + 0.1 * y, N.shape))
+    small_dt = 0.3  # displacements under one cell (the fastest bin moves 0.55 cells here)
+    outL = oracle.spatial_averaging(L, kappa, phi, dphi, dx, dy, small_dt)
+    np.testing.assert_allclose(outL[2:-2, 2:-2], L[2:-2, 2:-2], rtol=0, atol=1e-13)
+
+
+def test_tables(oracle):
+    kappa = np.array([0.1, 0.4, 1.6])
+    cg, dcg = oracle.gse_tables(kappa)
+    np.testing.assert_allclose(cg, 0.5 * np.sqrt(9.81 / kappa), rtol=1e-15)
+    assert dcg[0] == pytest.approx(abs(cg[0] - cg[1]) / 2, rel=1e-15)
+    assert dcg[1] == pytest.approx(abs(cg[0] - cg[2]) / 2, rel=1e-15)
+    assert dcg[2] == pytest.approx(abs(cg[1] - cg[2]) / 2, rel=1e-15)
+
+
+def test_model_step_applies_smoothing_after_the_step(oracle):
+    """time_step.jl:138-168: the RK3 step, then the averaging, every step."""
+    wl = workloads.pseudomomentum_packet(nx=16, ny=12, nz=4, nkappa=4, nphi=8)
+    sg = wl.spectral_grid
+    plain = oracle.OracleModel(wl, transport=oracle.MODE_INTRINSIC, timestepper="RK3")
+    sm = oracle.OracleModel(wl, transport=oracle.MODE_INTRINSIC, timestepper="RK3", smoothing=(0.5, 0.5))
+    plain.step(wl.dt, 1)
+    sm.step(wl.dt, 1)
+    want = oracle.spatial_averaging(plain.action, sg.kappa, sg.phi, plain.dp, plain.dx, plain.dy, wl.dt)
+    assert np.array_equal(sm.action, want)
+    assert sm.time == plain.time == wl.dt
