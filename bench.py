@@ -29,7 +29,7 @@ METRIC = "cell-bin updates/s per RK3 step (plain vs surrogate), HBM GB/s; err vs
 HBM_PEAK_GBS = 8000.0
 
 
-def build_model(wl, float_type, kernel="auto"):
+def build_model(wl, float_type, kernel="auto", **extra):
     from ripple_amd import grids
     g, sg = wl.grid, wl.spectral_grid
     grid = grids.RectilinearGrid(float_type=float_type, size=(g.Nx, g.Ny, g.Nz), x=g.x_faces, y=g.y_faces,
@@ -38,6 +38,7 @@ def build_model(wl, float_type, kernel="auto"):
                                       phi_faces=sg.phi_faces)
     kw = dict(wl.model_kwargs)
     kw.setdefault("velocities", wl.velocities)
+    kw.update(extra)
     m = ra.SpectralWaveModel(grid, sgrid, kernel=kernel, **kw)
     ra.set_(m, N=wl.N0)
     return m
@@ -149,6 +150,24 @@ def pseudomomentum_line(steps, warmup):
             "stage_kernel_ms": l_stage, "coupling_update_ms": l_pm, "coupling_updates_timed": n.value,
             "coupling_bytes_per_update": pm_bytes,
             "coupling_achieved_GBs": pm_bytes / (l_pm * 1e-3) / 1e9 if l_pm > 0 else None}
+
+
+def smoothing_line(steps, warmup):
+    """SpatialAveraging (alpha_s = alpha_n = 0.5) after every step of the
+    headline workload: the step time with it, and its one pass per step
+    (spatial_averaging_kernel: N read, the averaged field written)."""
+    import ctypes
+    wl = workloads.sheared_eddies()
+    m = build_model(wl, np.float32, propagation_smoothing=ra.SpatialAveraging(0.5, 0.5))
+    t_step, l_stage, _ = time_model(m, wl.dt, steps, warmup)
+    ms, n = ctypes.c_double(), ctypes.c_int64()
+    _lib.check(m._ctx.lib.rpl_get_smoothing_timing(m._ctx.ptr, ctypes.byref(ms), ctypes.byref(n)))
+    l_sm = ms.value / max(n.value, 1)
+    sm_bytes = 2 * 4 * wl.cell_bins
+    return {"workload": wl.name + "_spatial_averaging", "value": wl.cell_bins / t_step, "ms_per_step": t_step * 1e3,
+            "stage_kernel_ms": l_stage, "smoothing_ms": l_sm, "smoothing_passes_timed": n.value,
+            "smoothing_bytes_per_pass": sm_bytes,
+            "smoothing_achieved_GBs": sm_bytes / (l_sm * 1e-3) / 1e9 if l_sm > 0 else None}
 
 
 def ensemble_line(count, steps, with_surrogate=True, **member_kw):
@@ -269,6 +288,8 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble line")
     ap.add_argument("--only-pseudomomentum", action="store_true",
                     help="print only the self-coupled pseudomomentum line (for profiling its kernels)")
+    ap.add_argument("--only-smoothing", action="store_true",
+                    help="print only the propagation-smoothing line (for profiling its kernel)")
     ap.add_argument("--ensemble-members", type=int, default=64)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy (reference for HBM byte counters) and exit")
@@ -290,6 +311,9 @@ def main():
 
     if args.only_pseudomomentum:
         print(json.dumps({"pseudomomentum_coupling": pseudomomentum_line(args.steps, args.warmup)}), flush=True)
+        return
+    if args.only_smoothing:
+        print(json.dumps({"propagation_smoothing": smoothing_line(args.steps, args.warmup)}), flush=True)
         return
 
     wl = workloads.sheared_eddies()
@@ -338,6 +362,7 @@ def main():
         del mb
         extra["err_vs_fp64"] = err_vs_fp64()
         extra["pseudomomentum_coupling"] = pseudomomentum_line(max(3, args.steps // 2), 1)
+        extra["propagation_smoothing"] = smoothing_line(max(3, args.steps // 2), 1)
     if not args.no_storm:
         extra["storm_plain_vs_surrogate"] = storm_plain_vs_surrogate(max(2, args.steps // 4), 1)
     if not args.no_ensemble:

@@ -310,6 +310,10 @@ int rpl_get_surrogate_timing(rpl_ctx *ctx, double *total_ms, int64_t *evaluation
  * counterpart in the reference. */
 int rpl_get_coupling_timing(rpl_ctx *ctx, double *total_ms, int64_t *updates);
 
+/* Device time of the propagation-smoothing passes (one per step) since the
+ * last call, under the same enable switch.  No counterpart in the reference. */
+int rpl_get_smoothing_timing(rpl_ctx *ctx, double *total_ms, int64_t *passes);
+
 /* Ensemble batching.  No counterpart in the reference, which runs one model
  * per process.  Members are ordinary contexts -- FP32 on the fast kernel,
  * SSP-RK3, one transport kind (intrinsic or refraction) and either all with

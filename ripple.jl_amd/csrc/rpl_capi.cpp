@@ -62,7 +62,7 @@ struct rpl_ctx {
     void *pm_O = nullptr, *pm_dO = nullptr, *pm_kx = nullptr, *pm_ky = nullptr;
     // SpatialAveraging after every step (rpl_set_propagation_smoothing)
     bool gse = false;
-    double gse_as = 0.5, gse_an = 0.5;
+    double gse_as = 0.5, gse_an = 0.5, gse_cg_max = 0, gse_dcg_max = 0;
     void *gse_cg = nullptr, *gse_dcg = nullptr;
     double time = 0;
     int64_t iteration = 0;
@@ -100,13 +100,14 @@ struct rpl_ctx {
 
     // event timing: phase 0 = stage kernel launches, 1 = surrogate inference
     // (features and network forward of one stage input), 2 = pseudomomentum
-    // coupling update (contraction and, on the reference path, gradients)
+    // coupling update (contraction and, on the reference path, gradients),
+    // 3 = propagation smoothing
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
     std::vector<int> event_phase;
     size_t events_used = 0;
-    double timing_ms[3] = {0, 0, 0};
-    int64_t timing_launches[3] = {0, 0, 0};
+    double timing_ms[4] = {0, 0, 0, 0};
+    int64_t timing_launches[4] = {0, 0, 0, 0};
 };
 
 namespace {
@@ -524,14 +525,23 @@ int launch_smoothing(rpl_ctx *c, double dt)
     if (!c->gse)
         return RPL_OK;
     const int nx = c->cfg.nx, ny = c->cfg.ny, nk = c->cfg.nkappa, np = c->cfg.nphi;
+    // The kernels index the corners with 32-bit integers: keep displacements
+    // far inside that range.
+    const double reach = (c->gse_as * c->gse_dcg_max + c->gse_an * c->gse_cg_max * std::fabs(c->dp)) * dt *
+                         std::max(1 / c->dx, 1 / c->dy);
+    if (!(reach < 1048576.0))
+        return fail(RPL_ERR_ARG, "propagation smoothing would move cells by more than 2^20 cells in one step");
+    int rc = record_begin(c, 3);
+    if (rc != RPL_OK)
+        return rc;
     hipError_t e;
     if (is_f64(c))
-        e = rpl::launch_spatial_averaging<double>(nx, ny, nk, np, (const double *)c->A, (double *)c->B,
+        e = rpl::launch_spatial_averaging<double>(false, nx, ny, nk, np, (const double *)c->A, (double *)c->B,
                                                   (const double *)c->gse_cg, (const double *)c->gse_dcg,
                                                   (const double *)c->cosp, (const double *)c->sinp, c->dp,
                                                   c->gse_as, c->gse_an, dt, 1 / c->dx, 1 / c->dy, c->stream);
     else
-        e = rpl::launch_spatial_averaging<float>(nx, ny, nk, np, (const float *)c->A, (float *)c->B,
+        e = rpl::launch_spatial_averaging<float>(use_fast(c), nx, ny, nk, np, (const float *)c->A, (float *)c->B,
                                                  (const float *)c->gse_cg, (const float *)c->gse_dcg,
                                                  (const float *)c->cosp, (const float *)c->sinp, (float)c->dp,
                                                  (float)c->gse_as, (float)c->gse_an, (float)dt,
@@ -539,7 +549,7 @@ int launch_smoothing(rpl_ctx *c, double dt)
     if (e != hipSuccess)
         return fail(RPL_ERR_HIP, std::string("spatial averaging: ") + hipGetErrorString(e));
     std::swap(c->A, c->B);
-    return RPL_OK;
+    return record_end(c);
 }
 
 int check_ready(rpl_ctx *c)
@@ -1298,6 +1308,8 @@ int rpl_set_propagation_smoothing(rpl_ctx *c, int32_t enable, double alpha_s, do
     RPL_HIP(hipStreamSynchronize(c->stream));
     c->gse_as = alpha_s;
     c->gse_an = alpha_n;
+    c->gse_cg_max = *std::max_element(cg.begin(), cg.end());
+    c->gse_dcg_max = *std::max_element(dcg.begin(), dcg.end());
     c->gse = true;
     return RPL_OK;
 }
@@ -1897,6 +1909,22 @@ int rpl_get_coupling_timing(rpl_ctx *c, double *total_ms, int64_t *updates)
         *updates = c->timing_launches[2];
     c->timing_ms[2] = 0;
     c->timing_launches[2] = 0;
+    return RPL_OK;
+}
+
+int rpl_get_smoothing_timing(rpl_ctx *c, double *total_ms, int64_t *passes)
+{
+    if (!c)
+        return fail(RPL_ERR_ARG, "null context");
+    int rc = fold_timing(c);
+    if (rc)
+        return rc;
+    if (total_ms)
+        *total_ms = c->timing_ms[3];
+    if (passes)
+        *passes = c->timing_launches[3];
+    c->timing_ms[3] = 0;
+    c->timing_launches[3] = 0;
     return RPL_OK;
 }
 

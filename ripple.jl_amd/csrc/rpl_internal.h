@@ -139,9 +139,10 @@ hipError_t launch_pm_contract(bool ordered, int nx, int ny, int nk, int nphi, in
 
 // SpatialAveraging (Models/propagation_smoothing.jl:56-110): Nout = mean of
 // four bilinear corner interpolations of Nin per (kappa, phi) bin, periodic
-// x and y; cg, dcg per kappa bin, cos/sin of the phi centres.
+// x and y; cg, dcg per kappa bin, cos/sin of the phi centres.  fast (FP32
+// only) forms the corner weights once per bin instead of per cell.
 template <typename T>
-hipError_t launch_spatial_averaging(int nx, int ny, int nk, int nphi, const T *Nin, T *Nout, const T *cg,
+hipError_t launch_spatial_averaging(bool fast, int nx, int ny, int nk, int nphi, const T *Nin, T *Nout, const T *cg,
                                     const T *dcg, const T *cosp, const T *sinp, T dphi, T as, T an, T dt, T dx_inv,
                                     T dy_inv, hipStream_t s);
 

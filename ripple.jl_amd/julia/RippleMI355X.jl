@@ -24,7 +24,7 @@ module RippleMI355X
 using Libdl
 
 export MI355X, MI355XWaveModel, set!, time_step!, interior_action, doppler_velocity,
-       PseudomomentumVelocities, update_coupling!, doppler_velocity_kappa_derivative,
+       PseudomomentumVelocities, SpatialAveraging, update_coupling!, doppler_velocity_kappa_derivative,
        compute_pseudomomentum_cell_averages, compute_pseudomomentum_cell_integrals,
        m0, root_mean_square_wavenumber, mean_direction, significant_wave_height,
        total_action, total_deep_water_energy, Simulation, run!
@@ -79,6 +79,13 @@ struct PseudomomentumVelocities{G}
 end
 PseudomomentumVelocities(; q_grid=nothing) = PseudomomentumVelocities(q_grid)
 
+"`SpatialAveraging(; αs=0.5, αn=0.5)` (Models/propagation_smoothing.jl:22-51), applied after every step."
+struct SpatialAveraging
+    αs::Float64
+    αn::Float64
+end
+SpatialAveraging(; αs=0.5, αn=0.5) = SpatialAveraging(Float64(αs), Float64(αn))
+
 mutable struct Clock
     time::Float64
     iteration::Int
@@ -106,7 +113,7 @@ Models/tendencies.jl:18-109.  `depth === nothing` is InfiniteDepth().
 """
 function MI355XWaveModel(grid, spectral_grid; velocities=nothing, horizontal_advection=:WENO,
                          spectral_advection=:WENO, depth=nothing, timestepper=:ForwardEuler,
-                         float_type=Float64, gravity=9.81, kernel=0)
+                         float_type=Float64, gravity=9.81, kernel=0, propagation_smoothing=nothing)
     xf, yf, zf = collect(Float64, grid.x_faces), collect(Float64, grid.y_faces), collect(Float64, grid.z_faces)
     κ, κf = collect(Float64, spectral_grid.κ), collect(Float64, spectral_grid.κ_faces)
     φ, φf = collect(Float64, spectral_grid.φ), collect(Float64, spectral_grid.φ_faces)
@@ -147,6 +154,10 @@ function MI355XWaveModel(grid, spectral_grid; velocities=nothing, horizontal_adv
         check(ccall(sym(:rpl_set_current), Cint,
                     (Ptr{Cvoid}, Int32, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble}, Cdouble),
                     model.ctx, Nz, zf, u, v, q_depth))
+    end
+    if propagation_smoothing isa SpatialAveraging
+        check(ccall(sym(:rpl_set_propagation_smoothing), Cint, (Ptr{Cvoid}, Int32, Cdouble, Cdouble),
+                    model.ctx, Int32(1), propagation_smoothing.αs, propagation_smoothing.αn))
     end
     return model
 end
