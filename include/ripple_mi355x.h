@@ -218,7 +218,17 @@ typedef struct rpl_sources {
      * constant rate: lambda = rate * max(m0 / threshold - 1, 0)^saturation_power
      * * <kappa^wavenumber_power>_cell / reference_wavenumber^wavenumber_power. */
     double wc_rate, wc_threshold, wc_saturation_power, wc_wavenumber_power, wc_reference_wavenumber;
+    /* DiscreteInteractionApproximation(SymmetricQuadruplet(; coefficient,
+     * frequency_offset, gravity)) as the bundle's nonlinear member
+     * (Physics/NonlinearInteractions/symmetric_quadruplet.jl:30-238,
+     * precomputed_sources.jl:227-233, 256-259): RPL_NONLINEAR_*.  Meant for
+     * FrequencyDirectionGrid spectra (geometric frequencies, uniform phi
+     * from phi[0]), as the reference's bilinear lookup assumes. */
+    int32_t nonlinear;
+    double dia_coefficient, dia_frequency_offset, dia_gravity;
 } rpl_sources;
+
+enum { RPL_NONLINEAR_NONE = 0, RPL_NONLINEAR_SYMMETRIC_QUADRUPLET = 1 };
 
 /* Installs (or, with NULL, removes) the source terms.  x_nodes / y_nodes are
  * the nx / ny cell-centre coordinates the winds are evaluated at. */

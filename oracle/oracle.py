@@ -345,7 +345,9 @@ class OrcSources(ctypes.Structure):
                    "wc_rate", "wc_threshold", "wc_saturation_power", "wc_wavenumber_power",
                    "wc_reference_wavenumber")] + \
                [("track_n", ctypes.c_int), ("track_t", ctypes.c_void_p), ("track_x", ctypes.c_void_p),
-                ("track_y", ctypes.c_void_p)]
+                ("track_y", ctypes.c_void_p),
+                ("nonlinear", ctypes.c_int), ("dia_coefficient", ctypes.c_double),
+                ("dia_frequency_offset", ctypes.c_double), ("dia_gravity", ctypes.c_double)]
 
 
 def set_track(src, times, centers):
@@ -378,6 +380,22 @@ def sources_from(struct):
         if hasattr(struct, name):
             setattr(s, name, getattr(struct, name))
     return s
+
+
+def dia_offsets(frequency_offset=0.25):
+    dp, dm = ctypes.c_double(), ctypes.c_double()
+    lib().orc_dia_offsets(_D(frequency_offset), ctypes.byref(dp), ctypes.byref(dm))
+    return dp.value, dm.value
+
+
+def dia_transfer(N, kappa, phi, coefficient=2.78e7, frequency_offset=0.25, gravity=9.81):
+    """SymmetricQuadruplet transfer T (Nx, Ny, Nk, Nphi) of an action field (FP64)."""
+    nx, ny, nk, npf = N.shape
+    Nf, kc, pc = _c(N), _c(kappa), _c(phi)
+    T = np.zeros_like(Nf)
+    lib().orc_dia_transfer(_I(nx), _I(ny), _I(nk), _I(npf), _p(Nf), _p(kc), _p(pc), _D(coefficient),
+                           _D(frequency_offset), _D(gravity), _p(T))
+    return T.reshape((nx, ny, nk, npf), order="F")
 
 
 def wind_columns(src, x_nodes, y_nodes, t):

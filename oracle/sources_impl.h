@@ -32,6 +32,8 @@ typedef struct orc_sources {
     double wc_rate, wc_threshold, wc_saturation_power, wc_wavenumber_power, wc_reference_wavenumber;
     int track_n;  /* > 0: LinearStormTrack of track_n points, replacing the two-point track */
     const double *track_t, *track_x, *track_y;
+    int nonlinear;  /* 0 none, 1 DiscreteInteractionApproximation(SymmetricQuadruplet) */
+    double dia_coefficient, dia_frequency_offset, dia_gravity;
 } orc_sources;
 
 /* drag.jl:25-34. */
@@ -147,6 +149,152 @@ static double orc_jp_rate(const orc_sources *s, double k, double sigma, double C
            (inv_age * inv_age) * cp * sigma;
 }
 
+/* ------------------------------------------------------------------------ */
+/* DiscreteInteractionApproximation{SymmetricQuadruplet}                     */
+/* (Physics/NonlinearInteractions/symmetric_quadruplet.jl:30-238).            */
+/* ------------------------------------------------------------------------ */
+
+/* symmetric_quadruplet.jl:38-49: the direction offsets of the two receivers
+ * from momentum conservation at lambda. */
+void orc_dia_offsets(double lambda, double *dtheta_plus, double *dtheta_minus)
+{
+    double a = (1 + lambda) * (1 + lambda), b = (1 - lambda) * (1 - lambda);
+    double cp = (a * a - b * b + 4) / (4 * a), cm = (b * b - a * a + 4) / (4 * b);
+    cp = cp < -1 ? -1 : (cp > 1 ? 1 : cp);
+    cm = cm < -1 ? -1 : (cm > 1 ? 1 : cm);
+    *dtheta_plus = acos(cp);
+    *dtheta_minus = acos(cm);
+}
+
+/* symmetric_quadruplet.jl:54-77 (_bilinear_lookup), 0-based bins out. */
+static int orc_dia_lookup(double s, double th, double smin, double smax, double log_xfr, double phimin, double dphi,
+                          int nk, int nphi, int *mlo, int *mhi, int *nlo, int *nhi, double w[4])
+{
+    if (s < smin || s > smax)
+        return 0;
+    double mr = 1.0 + log(s / smin) / log_xfr;
+    long t = (long)mr;
+    int ml = (int)(t < 1 ? 1 : (t > nk - 1 ? nk - 1 : t));
+    double wf = mr - ml;
+    double twopi = 2 * M_PI;
+    double r = fmod(th - phimin, twopi);
+    if (r != 0 && (r < 0) != (twopi < 0))
+        r += twopi;
+    double nr = r / dphi;
+    long nraw = (long)nr;
+    long nm = nraw % nphi;
+    if (nm < 0)
+        nm += nphi;
+    int nl = (int)nm + 1;
+    double wt = nr - (double)nraw;
+    int nh = nl == nphi ? 1 : nl + 1;
+    w[0] = (1 - wf) * (1 - wt);
+    w[1] = wf * (1 - wt);
+    w[2] = (1 - wf) * wt;
+    w[3] = wf * wt;
+    *mlo = ml - 1;
+    *mhi = ml;
+    *nlo = nl - 1;
+    *nhi = nh - 1;
+    return 1;
+}
+
+static inline double orc_dia_energy_factor(double sigma, double g) { return 4 * M_PI * (sigma * sigma) / g; }
+
+/* symmetric_quadruplet.jl:82-208: the transfer T[i,j,m,n] of every column,
+ * donors visited n outer, m inner, orientation +1 then -1 (the order in
+ * which a sequential backend runs the kernel's atomic updates).  f_a^11 and
+ * the constants DAL1..3, C/g^4 use libm pow (Julia's integer powers may
+ * differ in the last bit). */
+void orc_dia_transfer(int nx, int ny, int nk, int nphi, const double *N, const double *kappa, const double *phi,
+                      double coefficient, double lambda, double g, double *T)
+{
+    double dtp, dtm;
+    orc_dia_offsets(lambda, &dtp, &dtm);
+    const double DAL1 = 1 / pow(1 + lambda, 4), DAL2 = 1 / pow(1 - lambda, 4), DAL3 = 2 * DAL1 * DAL2;
+    const double Ceff = coefficient / pow(g, 4);
+    const double smin = sqrt(g * kappa[0]), smax = sqrt(g * kappa[nk - 1]);
+    const double log_xfr = log(sqrt(g * kappa[1]) / sqrt(g * kappa[0]));
+    const double phimin = phi[0], dphi = 2 * M_PI / nphi;
+    const size_t plane = (size_t)nx * ny;
+    memset(T, 0, plane * nk * nphi * sizeof(double));
+#pragma omp parallel for schedule(static)
+    for (long ij = 0; ij < (long)plane; ++ij) {
+        const double *Nc = N + ij;
+        double *Tc = T + ij;
+#define ORC_N(m, n) Nc[plane * ((size_t)(m) + (size_t)nk * (n))]
+#define ORC_T(m, n) Tc[plane * ((size_t)(m) + (size_t)nk * (n))]
+        for (int na = 0; na < nphi; ++na)
+            for (int ma = 0; ma < nk; ++ma) {
+                double Na = ORC_N(ma, na);
+                if (!(Na > 0))
+                    continue;
+                double sa_ = sqrt(g * kappa[ma]);
+                double fa = sa_ / (2 * M_PI);
+                double tha = phi[na];
+                double Efa = orc_dia_energy_factor(sa_, g);
+                double Ea = Efa * Na;
+                double sp = (1 + lambda) * sa_, sm = (1 - lambda) * sa_;
+                for (int o = 1; o >= -1; o -= 2) {
+                    double thp = tha + o * dtp, thm = tha - o * dtm;
+                    int mlp, mhp, nlp, nhp, mlm, mhm, nlm, nhm;
+                    double wp[4], wm[4];
+                    int inp = orc_dia_lookup(sp, thp, smin, smax, log_xfr, phimin, dphi, nk, nphi, &mlp, &mhp, &nlp,
+                                             &nhp, wp);
+                    int inm = orc_dia_lookup(sm, thm, smin, smax, log_xfr, phimin, dphi, nk, nphi, &mlm, &mhm, &nlm,
+                                             &nhm, wm);
+                    if (!inp && !inm)
+                        continue;
+                    double Ep = 0, Em = 0, Efp1 = 0, Efp2 = 0, Efm1 = 0, Efm2 = 0;
+                    if (inp) {
+                        Efp1 = orc_dia_energy_factor(sqrt(g * kappa[mlp]), g);
+                        Efp2 = orc_dia_energy_factor(sqrt(g * kappa[mhp]), g);
+                        double a0 = ORC_N(mlp, nlp), a1 = ORC_N(mhp, nlp), a2 = ORC_N(mlp, nhp), a3 = ORC_N(mhp, nhp);
+                        Ep = wp[0] * Efp1 * (a0 > 0 ? a0 : 0) + wp[1] * Efp2 * (a1 > 0 ? a1 : 0) +
+                             wp[2] * Efp1 * (a2 > 0 ? a2 : 0) + wp[3] * Efp2 * (a3 > 0 ? a3 : 0);
+                    }
+                    if (inm) {
+                        Efm1 = orc_dia_energy_factor(sqrt(g * kappa[mlm]), g);
+                        Efm2 = orc_dia_energy_factor(sqrt(g * kappa[mhm]), g);
+                        double a0 = ORC_N(mlm, nlm), a1 = ORC_N(mhm, nlm), a2 = ORC_N(mlm, nhm), a3 = ORC_N(mhm, nhm);
+                        Em = wm[0] * Efm1 * (a0 > 0 ? a0 : 0) + wm[1] * Efm2 * (a1 > 0 ? a1 : 0) +
+                             wm[2] * Efm1 * (a2 > 0 ? a2 : 0) + wm[3] * Efm2 * (a3 > 0 ? a3 : 0);
+                    }
+                    double s = Ea * (Ep * DAL1 + Em * DAL2) - Ep * Em * DAL3;
+                    double TE = Ceff * pow(fa, 11) * s / 2;
+                    double donor = (inp && inm) ? 2.0 : 1.0;
+                    ORC_T(ma, na) -= donor * TE / Efa;
+                    if (inp) {
+                        ORC_T(mlp, nlp) += wp[0] * TE / Efp1;
+                        ORC_T(mhp, nlp) += wp[1] * TE / Efp2;
+                        ORC_T(mlp, nhp) += wp[2] * TE / Efp1;
+                        ORC_T(mhp, nhp) += wp[3] * TE / Efp2;
+                    }
+                    if (inm) {
+                        ORC_T(mlm, nlm) += wm[0] * TE / Efm1;
+                        ORC_T(mhm, nlm) += wm[1] * TE / Efm2;
+                        ORC_T(mlm, nhm) += wm[2] * TE / Efm1;
+                        ORC_T(mhm, nhm) += wm[3] * TE / Efm2;
+                    }
+                }
+            }
+#undef ORC_N
+#undef ORC_T
+    }
+}
+
+/* symmetric_quadruplet.jl:210-222 (source_split): T >= 0 is a positive
+ * source, T < 0 a damping rate min(-T / max(N, eps), 1). */
+static inline void orc_dia_split(double T, double Na, double *positive, double *damping)
+{
+    if (T >= 0) {
+        *positive += T;
+    } else {
+        double d = -T / (Na > DBL_EPSILON ? Na : DBL_EPSILON);
+        *damping += d < 1 ? d : 1;
+    }
+}
+
 typedef struct orc_source_ctx {
     const orc_sources *src;
     int nx, ny, nk, nphi;
@@ -178,6 +326,15 @@ void orc_sources_hook_f64(const double *N, double *G, void *user)
     const orc_sources *s = c->src;
     const int nx = c->nx, ny = c->ny, nk = c->nk, nphi = c->nphi;
     const double g = s->gravity;
+    /* The nonlinear state of the bundle (precomputed_sources.jl:227-233). */
+    double *Tdia = NULL;
+    if (s->nonlinear == 1) {
+        Tdia = (double *)malloc((size_t)nx * ny * nk * nphi * sizeof(double));
+        if (!Tdia)
+            return;
+        orc_dia_transfer(nx, ny, nk, nphi, N, c->kappa, c->phi, s->dia_coefficient, s->dia_frequency_offset,
+                         s->dia_gravity, Tdia);
+    }
 #pragma omp parallel for schedule(static)
     for (long ij = 0; ij < (long)nx * ny; ++ij) {
         int i = (int)(ij % nx), j = (int)(ij / nx);
@@ -295,9 +452,12 @@ void orc_sources_hook_f64(const double *N, double *G, void *user)
                     else
                         positive += -rate * a;
                 }
+                if (Tdia)  /* precomputed_sources.jl:256-259 */
+                    orc_dia_split(Tdia[cidx], a, &positive, &damping);
                 G[cidx] += positive - damping * a;
             }
     }
+    free(Tdia);
 }
 
 /* Per-column wind fields at time t, for tests of the device wind kernel. */

@@ -1,0 +1,145 @@
+// Nonlinear four-wave transfer by the discrete interaction approximation
+// with one symmetric quadruplet (the reference's
+// DiscreteInteractionApproximation{SymmetricQuadruplet},
+// Physics/NonlinearInteractions/symmetric_quadruplet.jl:30-238).
+//
+// Each donor bin (m, n) of a column interacts with two receivers at
+// (1 +- lambda) sigma and directions theta +- offsets, read and updated
+// through bilinear corners.  Where the receivers fall depends only on the
+// spectral grid, so the host tabulates, per donor and orientation, the
+// corner bins and weights (DiaTables); a workgroup then holds the action and
+// the transfer of a few adjacent columns in LDS, walks every (donor, column)
+// pair, and scatters the transfer with LDS atomic adds.  The last phase turns
+// the transfer into the source the stage adds, T where T >= 0 and
+// -min(-T / max(N, eps), 1) N otherwise (symmetric_quadruplet.jl:210-222).
+#include "rpl_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace rpl {
+namespace {
+
+__device__ __forceinline__ long long dia_xcd_order(long long b, long long total)
+{
+    const long long xcd = b & 7, r = b >> 3, per = total >> 3, rem = total & 7;
+    return xcd < rem ? xcd * (per + 1) + r : rem * (per + 1) + (xcd - rem) * per + r;
+}
+
+template <typename T>
+__device__ __forceinline__ T eps_of();
+template <>
+__device__ __forceinline__ float eps_of<float>() { return 1.1920928955078125e-7f; }
+template <>
+__device__ __forceinline__ double eps_of<double>() { return 2.220446049250313e-16; }
+
+template <typename T>
+__global__ __launch_bounds__(256) void dia_source_kernel(size_t plane, int nbins, int cols, long long tiles,
+                                                         const T *__restrict__ N, DiaTables<T> tb, T *__restrict__ out)
+{
+    extern __shared__ unsigned char dia_lds[];
+    T *Nl = reinterpret_cast<T *>(dia_lds);
+    T *Tl = Nl + (size_t)nbins * cols;
+    const size_t ij0 = (size_t)dia_xcd_order(blockIdx.x, tiles) * cols;
+    const int ncol = (int)min((size_t)cols, plane - ij0);
+    const int items = nbins * cols;
+    // The columns' action, bin-major ([bin][column]), and a zero transfer.
+    for (int e = threadIdx.x; e < items; e += 256) {
+        const int bin = e / cols, cc = e - bin * cols;
+        Nl[e] = cc < ncol ? N[ij0 + cc + plane * (size_t)bin] : T(0);
+        Tl[e] = T(0);
+    }
+    __syncthreads();
+    // symmetric_quadruplet.jl:92-173 for every (donor, column) pair.
+    for (int e = threadIdx.x; e < items; e += 256) {
+        const int d = e / cols, cc = e - d * cols;
+        const T Na = Nl[e];
+        if (!(Na > T(0)))
+            continue;
+        const T Efa = tb.ef_a[d];
+        const T Ea = Efa * Na;
+#pragma unroll
+        for (int o = 0; o < 2; ++o) {
+            const int q = 2 * d + o;
+            const int fl = tb.flags[q];
+            if (!fl)
+                continue;
+            const bool inp = fl & 1, inm = fl & 2;
+            const int *bp = tb.bins + 8 * (size_t)q;
+            const T *wef = tb.wef + 8 * (size_t)q;
+            T Ep = T(0), Em = T(0);
+            if (inp)
+                Ep = wef[0] * fmax(Nl[bp[0] * cols + cc], T(0)) + wef[1] * fmax(Nl[bp[1] * cols + cc], T(0)) +
+                     wef[2] * fmax(Nl[bp[2] * cols + cc], T(0)) + wef[3] * fmax(Nl[bp[3] * cols + cc], T(0));
+            if (inm)
+                Em = wef[4] * fmax(Nl[bp[4] * cols + cc], T(0)) + wef[5] * fmax(Nl[bp[5] * cols + cc], T(0)) +
+                     wef[6] * fmax(Nl[bp[6] * cols + cc], T(0)) + wef[7] * fmax(Nl[bp[7] * cols + cc], T(0));
+            const T s = Ea * (Ep * tb.dal1 + Em * tb.dal2) - Ep * Em * tb.dal3;
+            const T TE = tb.cf_a[d] * s / T(2);
+            const T donor = (inp && inm) ? T(2) : T(1);
+            atomicAdd(&Tl[e], -(donor * TE / Efa));
+            const T *w = tb.w + 8 * (size_t)q, *ef = tb.ef + 8 * (size_t)q;
+            if (inp) {
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    atomicAdd(&Tl[bp[k] * cols + cc], w[k] * TE / ef[k]);
+            }
+            if (inm) {
+#pragma unroll
+                for (int k = 4; k < 8; ++k)
+                    atomicAdd(&Tl[bp[k] * cols + cc], w[k] * TE / ef[k]);
+            }
+        }
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < items; e += 256) {
+        const int bin = e / cols, cc = e - bin * cols;
+        if (cc >= ncol)
+            continue;
+        const T t = Tl[e], a = Nl[e];
+        T src;
+        if (t >= T(0)) {
+            src = t;
+        } else {
+            const T rate = -t / fmax(a, eps_of<T>());
+            src = -(fmin(rate, T(1)) * a);
+        }
+        out[ij0 + cc + plane * (size_t)bin] = src;
+    }
+}
+
+}  // namespace
+
+template <typename T>
+int dia_columns_per_workgroup(int nbins)
+{
+    const size_t per_col = 2 * (size_t)nbins * sizeof(T);
+    const size_t cols = (48 * 1024) / per_col;
+    return (int)std::min<size_t>(cols, 8);
+}
+
+template <typename T>
+hipError_t launch_dia_source(int nx, int ny, int nk, int nphi, const T *N, const DiaTables<T> &tb, T *out,
+                             hipStream_t s)
+{
+    const size_t plane = (size_t)nx * ny;
+    const int nbins = nk * nphi;
+    const int cols = dia_columns_per_workgroup<T>(nbins);
+    if (cols < 1)
+        return hipErrorInvalidValue;
+    const long long tiles = (long long)((plane + cols - 1) / cols);
+    if (tiles > 0x7fffffffLL)
+        return hipErrorInvalidValue;
+    const size_t lds = 2 * (size_t)nbins * cols * sizeof(T);
+    hipLaunchKernelGGL(dia_source_kernel<T>, dim3((unsigned)tiles), dim3(256), lds, s, plane, nbins, cols, tiles, N,
+                       tb, out);
+    return hipGetLastError();
+}
+
+template int dia_columns_per_workgroup<float>(int);
+template int dia_columns_per_workgroup<double>(int);
+template hipError_t launch_dia_source<float>(int, int, int, int, const float *, const DiaTables<float> &, float *,
+                                             hipStream_t);
+template hipError_t launch_dia_source<double>(int, int, int, int, const double *, const DiaTables<double> &,
+                                              double *, hipStream_t);
+
+}  // namespace rpl

@@ -60,6 +60,12 @@ struct rpl_ctx {
     // self-coupled pseudomomentum current (rpl_set_pseudomomentum_coupling)
     bool pm = false;
     void *pm_O = nullptr, *pm_dO = nullptr, *pm_kx = nullptr, *pm_ky = nullptr;
+    // DIA nonlinear member of the sources: column-independent tables and the
+    // per-stage source plane the stage kernels add (StageParams::src_add)
+    bool dia = false;
+    void *dia_flags = nullptr, *dia_bins = nullptr, *dia_wef = nullptr, *dia_w = nullptr, *dia_ef = nullptr,
+         *dia_efa = nullptr, *dia_cfa = nullptr, *dia_add = nullptr;
+    double dia_dal[3] = {0, 0, 0};
     // SpatialAveraging after every step (rpl_set_propagation_smoothing)
     bool gse = false;
     double gse_as = 0.5, gse_an = 0.5, gse_cg_max = 0, gse_dcg_max = 0;
@@ -177,9 +183,9 @@ rpl::StageParams<T> stage_params(const rpl_ctx *c, double dt)
     p.dk_inv = T(1) / (T)c->dk;
     p.dp_inv = T(1) / (T)c->dp;
     p.dt = (T)dt;
-    p.src_add = nullptr;
+    p.src_add = c->dia ? (const T *)c->dia_add : nullptr;
     p.src = rpl::SourceParams<T>{};
-    if (c->has_sources) {
+    if (c->has_sources && (c->src.wind_input || c->src.dissipation)) {
         const rpl_sources &s = c->src;
         rpl::SourceParams<T> &q = p.src;
         q.on = 1;
@@ -373,8 +379,32 @@ int record_end(rpl_ctx *c);
 // Source preparation of one stage input: surrogate inference (if attached),
 // the bundle's column state (only on plain and audit columns when the
 // surrogate runs), then the audit.
+// The DIA source of the stage input into the additive plane.
+template <typename T>
+int launch_dia_t(rpl_ctx *c, const T *Nk)
+{
+    rpl::DiaTables<T> tb{(const int *)c->dia_flags, (const int *)c->dia_bins, (const T *)c->dia_wef,
+                         (const T *)c->dia_w, (const T *)c->dia_ef, (const T *)c->dia_efa, (const T *)c->dia_cfa,
+                         (T)c->dia_dal[0], (T)c->dia_dal[1], (T)c->dia_dal[2]};
+    hipError_t e = rpl::launch_dia_source<T>(c->cfg.nx, c->cfg.ny, c->cfg.nkappa, c->cfg.nphi, Nk, tb,
+                                             (T *)c->dia_add, c->stream);
+    if (e != hipSuccess)
+        return fail(RPL_ERR_HIP, std::string("symmetric quadruplet: ") + hipGetErrorString(e));
+    return RPL_OK;
+}
+
+int launch_dia(rpl_ctx *c, const void *Nk)
+{
+    if (!c->dia)
+        return RPL_OK;
+    return is_f64(c) ? launch_dia_t<double>(c, (const double *)Nk) : launch_dia_t<float>(c, (const float *)Nk);
+}
+
 int launch_sources(rpl_ctx *c, const void *Nk)
 {
+    int rc0 = launch_dia(c, Nk);
+    if (rc0)
+        return rc0;
     if (!c->s_active)
         return launch_state(c, Nk);
     int rc = record_begin(c, 1);
@@ -935,7 +965,8 @@ int rpl_destroy(rpl_ctx *c)
                     c->f_logkap, c->f_invck, c->f_ctu, c->f_stu, c->f_lz1, (void *)c->sX, (void *)c->sA, (void *)c->sB, (void *)c->rate,
                     (void *)c->out_scale, (void *)c->s_err, (void *)c->use_plain, (void *)c->s_counts,
                     (void *)c->bin_hi, (void *)c->s_amax, (void *)c->s_flags, c->dUx, c->dUy, c->pm_O,
-                    c->pm_dO, c->pm_kx, c->pm_ky, c->gse_cg, c->gse_dcg})
+                    c->pm_dO, c->pm_kx, c->pm_ky, c->gse_cg, c->gse_dcg, c->dia_flags, c->dia_bins, c->dia_wef,
+                    c->dia_w, c->dia_ef, c->dia_efa, c->dia_cfa, c->dia_add})
         free_ptr(p);
     for (auto &ev : c->events) {
         (void)hipEventDestroy(ev.first);
@@ -1019,6 +1050,7 @@ int rpl_set_geometry(rpl_ctx *c, double dx, double dy, const double *kappa_c, co
     c->h_mux = mx;
     c->h_muy = my;
     c->has_sources = false;  // source tables depend on the geometry
+    c->dia = false;
     if (c->pm) {             // so do the pseudomomentum tables
         c->pm = false;
         c->has_current = false;
@@ -1624,12 +1656,134 @@ int rpl_conservation(rpl_ctx *c, double *total_action, double *total_energy)
     return RPL_OK;
 }
 
+// symmetric_quadruplet.jl:54-77 (_bilinear_lookup), 0-based bins.
+static bool dia_lookup(double s, double th, double smin, double smax, double log_xfr, double phimin, double dphi,
+                       int nk, int nphi, int bins[4], double w[4], int *mlo, int *mhi)
+{
+    if (s < smin || s > smax)
+        return false;
+    const double mr = 1.0 + std::log(s / smin) / log_xfr;
+    const long t = (long)mr;  // unsafe_trunc
+    const int ml = (int)std::min<long>(std::max<long>(t, 1), nk - 1);
+    const double wf = mr - ml;
+    const double twopi = 2 * M_PI;
+    double r = std::fmod(th - phimin, twopi);  // Julia mod: the sign of the divisor
+    if (r != 0 && (r < 0) != (twopi < 0))
+        r += twopi;
+    const double nr = r / dphi;
+    const long nraw = (long)nr;
+    long nm = nraw % nphi;
+    if (nm < 0)
+        nm += nphi;
+    const int nl = (int)nm;                        // 0-based n_lo
+    const int nh = nl + 1 == nphi ? 0 : nl + 1;    // 0-based n_hi
+    const double wt = nr - (double)nraw;
+    w[0] = (1 - wf) * (1 - wt);
+    w[1] = wf * (1 - wt);
+    w[2] = (1 - wf) * wt;
+    w[3] = wf * wt;
+    *mlo = ml - 1;
+    *mhi = ml;
+    bins[0] = *mlo + nk * nl;
+    bins[1] = *mhi + nk * nl;
+    bins[2] = *mlo + nk * nh;
+    bins[3] = *mhi + nk * nh;
+    return true;
+}
+
+// The column-independent DIA tables (symmetric_quadruplet.jl:38-49, 82-208).
+static int build_dia_tables(rpl_ctx *c, const rpl_sources &s)
+{
+    c->dia = false;
+    if (s.nonlinear != RPL_NONLINEAR_SYMMETRIC_QUADRUPLET)
+        return RPL_OK;
+    const int nk = c->cfg.nkappa, np = c->cfg.nphi, nb = nk * np;
+    const double lam = s.dia_frequency_offset, g = s.dia_gravity;
+    const double a = (1 + lam) * (1 + lam), b = (1 - lam) * (1 - lam);
+    const double dtp = std::acos(std::min(1.0, std::max(-1.0, (a * a - b * b + 4) / (4 * a))));
+    const double dtm = std::acos(std::min(1.0, std::max(-1.0, (b * b - a * a + 4) / (4 * b))));
+    const double ceff = s.dia_coefficient / std::pow(g, 4);
+    std::vector<double> kc = c->h_kc;
+    if (!is_f64(c))
+        for (double &k : kc)
+            k = (float)k;
+    auto energy_factor = [&](double sigma) { return 4 * M_PI * (sigma * sigma) / g; };
+    const double smin = std::sqrt(g * kc[0]), smax = std::sqrt(g * kc[nk - 1]);
+    const double log_xfr = std::log(std::sqrt(g * kc[1]) / std::sqrt(g * kc[0]));
+    const double phimin = c->h_pc[0], dphi = 2 * M_PI / np;
+    std::vector<int> flags(2 * (size_t)nb, 0), bins(16 * (size_t)nb, 0);
+    std::vector<double> wef(16 * (size_t)nb, 0), w(16 * (size_t)nb, 0), ef(16 * (size_t)nb, 1), efa(nb), cfa(nb);
+    for (int n = 0; n < np; ++n)
+        for (int m = 0; m < nk; ++m) {
+            const int d = m + nk * n;
+            const double sa = std::sqrt(g * kc[m]);
+            efa[d] = energy_factor(sa);
+            cfa[d] = ceff * std::pow(sa / (2 * M_PI), 11);
+            const double sp = (1 + lam) * sa, sm = (1 - lam) * sa, tha = c->h_pc[n];
+            for (int o = 0; o < 2; ++o) {
+                const int sign = o == 0 ? 1 : -1, q = 2 * d + o;
+                int mlo, mhi;
+                double wq[4];
+                if (dia_lookup(sp, tha + sign * dtp, smin, smax, log_xfr, phimin, dphi, nk, np, &bins[8 * (size_t)q], wq, &mlo,
+                               &mhi)) {
+                    flags[q] |= 1;
+                    const double e1 = energy_factor(std::sqrt(g * kc[mlo])), e2 = energy_factor(std::sqrt(g * kc[mhi]));
+                    const double efk[4] = {e1, e2, e1, e2};
+                    for (int k = 0; k < 4; ++k) {
+                        wef[8 * (size_t)q + k] = wq[k] * efk[k];
+                        w[8 * (size_t)q + k] = wq[k];
+                        ef[8 * (size_t)q + k] = efk[k];
+                    }
+                }
+                if (dia_lookup(sm, tha - sign * dtm, smin, smax, log_xfr, phimin, dphi, nk, np, &bins[8 * (size_t)q + 4], wq,
+                               &mlo, &mhi)) {
+                    flags[q] |= 2;
+                    const double e1 = energy_factor(std::sqrt(g * kc[mlo])), e2 = energy_factor(std::sqrt(g * kc[mhi]));
+                    const double efk[4] = {e1, e2, e1, e2};
+                    for (int k = 0; k < 4; ++k) {
+                        wef[8 * (size_t)q + 4 + k] = wq[k] * efk[k];
+                        w[8 * (size_t)q + 4 + k] = wq[k];
+                        ef[8 * (size_t)q + 4 + k] = efk[k];
+                    }
+                }
+            }
+        }
+    int rc;
+    for (void **p : {&c->dia_flags})
+        if (!*p && (rc = alloc(p, 2 * (size_t)nb * sizeof(int))))
+            return rc;
+    for (void **p : {&c->dia_bins})
+        if (!*p && (rc = alloc(p, 16 * (size_t)nb * sizeof(int))))
+            return rc;
+    for (void **p : {&c->dia_wef, &c->dia_w, &c->dia_ef})
+        if (!*p && (rc = alloc(p, 16 * (size_t)nb * c->esize)))
+            return rc;
+    for (void **p : {&c->dia_efa, &c->dia_cfa})
+        if (!*p && (rc = alloc(p, (size_t)nb * c->esize)))
+            return rc;
+    if (!c->dia_add && (rc = alloc(&c->dia_add, c->cnt * c->esize)))
+        return rc;
+    RPL_HIP(hipMemcpy(c->dia_flags, flags.data(), flags.size() * sizeof(int), hipMemcpyHostToDevice));
+    RPL_HIP(hipMemcpy(c->dia_bins, bins.data(), bins.size() * sizeof(int), hipMemcpyHostToDevice));
+    if ((rc = upload_real(c, wef.data(), c->dia_wef, wef.size())) || (rc = upload_real(c, w.data(), c->dia_w, w.size())) ||
+        (rc = upload_real(c, ef.data(), c->dia_ef, ef.size())) || (rc = upload_real(c, efa.data(), c->dia_efa, nb)) ||
+        (rc = upload_real(c, cfa.data(), c->dia_cfa, nb)))
+        return rc;
+    c->dia_dal[0] = 1 / std::pow(1 + lam, 4);
+    c->dia_dal[1] = 1 / std::pow(1 - lam, 4);
+    c->dia_dal[2] = 2 * c->dia_dal[0] * c->dia_dal[1];
+    RPL_HIP(hipStreamSynchronize(c->stream));
+    c->dia = true;
+    return RPL_OK;
+}
+
 int rpl_set_sources(rpl_ctx *c, const rpl_sources *src, const double *x_nodes, const double *y_nodes)
 {
     if (!c)
         return fail(RPL_ERR_ARG, "null context");
     if (!src) {
         c->has_sources = false;
+        c->dia = false;
         return RPL_OK;
     }
     if (!c->has_geometry)
@@ -1661,6 +1815,19 @@ int rpl_set_sources(rpl_ctx *c, const rpl_sources *src, const double *x_nodes, c
             if (!(s.track_t1 > s.track_t0))
                 return fail(RPL_ERR_ARG, "storm-track times must be strictly increasing");
         }
+    }
+    if (s.nonlinear != RPL_NONLINEAR_NONE && s.nonlinear != RPL_NONLINEAR_SYMMETRIC_QUADRUPLET)
+        return fail(RPL_ERR_ARG, "unsupported nonlinear term");
+    if (s.nonlinear == RPL_NONLINEAR_SYMMETRIC_QUADRUPLET) {
+        if (c->cfg.nkappa < 2)
+            return fail(RPL_ERR_ARG, "the symmetric quadruplet needs at least two frequency bins");
+        if (!(s.dia_frequency_offset > 0 && s.dia_frequency_offset < 1))
+            return fail(RPL_ERR_ARG, "frequency_offset must lie in (0, 1)");
+        if (!(s.dia_gravity > 0) || !std::isfinite(s.dia_coefficient))
+            return fail(RPL_ERR_ARG, "symmetric-quadruplet gravity must be positive and its coefficient finite");
+        const size_t lds = 2 * (size_t)c->cfg.nkappa * c->cfg.nphi * c->esize;
+        if (lds > 48 * 1024)
+            return fail(RPL_ERR_UNSUPPORTED, "the symmetric quadruplet holds a column in 48 KiB of LDS: too many bins");
     }
     if (s.dissipation == RPL_DISSIPATION_MEAN_SPECTRUM && !(s.p_mean != 0))
         return fail(RPL_ERR_ARG, "mean-spectrum exponent p must be nonzero");
@@ -1814,6 +1981,8 @@ int rpl_set_sources(rpl_ctx *c, const rpl_sources *src, const double *x_nodes, c
     wp.y0 = s.track_y0;
     wp.x1 = s.track_x1;
     wp.y1 = s.track_y1;
+    if ((rc = build_dia_tables(c, s)))
+        return rc;
     c->src = s;
     c->has_sources = s.wind_input != RPL_WIND_INPUT_NONE || s.dissipation != RPL_DISSIPATION_NONE;
     RPL_HIP(hipStreamSynchronize(c->stream));
@@ -2080,6 +2249,8 @@ int rpl_ensemble_create(rpl_ensemble **out, rpl_ctx *const *members, int32_t cou
             return fail(RPL_ERR_UNSUPPORTED, "ensemble members take prescribed currents only (no pseudomomentum coupling)");
         if (c->gse)
             return fail(RPL_ERR_UNSUPPORTED, "ensemble members run without propagation smoothing");
+        if (c->dia)
+            return fail(RPL_ERR_UNSUPPORTED, "ensemble members run without the nonlinear (DIA) term");
         if (c->device != first->device)
             return fail(RPL_ERR_ARG, "ensemble members must live on one device");
     }

@@ -146,6 +146,27 @@ hipError_t launch_spatial_averaging(bool fast, int nx, int ny, int nk, int nphi,
                                     const T *dcg, const T *cosp, const T *sinp, T dphi, T as, T an, T dt, T dx_inv,
                                     T dy_inv, hipStream_t s);
 
+// DiscreteInteractionApproximation{SymmetricQuadruplet}: per donor bin d and
+// orientation o (entry q = 2 d + o, o = 0 for +1, 1 for -1) the receivers'
+// corner bins (four for sigma+, four for sigma-), weight times energy
+// factor (the gather), weight and energy factor (the scatter), flags (bit 0:
+// sigma+ on the grid, bit 1: sigma- on the grid); per donor its energy
+// factor and C/g^4 f^11.  Column-independent; built on the host.
+template <typename T>
+struct DiaTables {
+    const int *flags;  // 2 nbins
+    const int *bins;   // 16 nbins
+    const T *wef, *w, *ef;  // 16 nbins each
+    const T *ef_a, *cf_a;   // nbins each
+    T dal1, dal2, dal3;
+};
+template <typename T>
+int dia_columns_per_workgroup(int nbins);
+// out = the DIA source (T or -min(-T/max(N, eps), 1) N) of the action N.
+template <typename T>
+hipError_t launch_dia_source(int nx, int ny, int nk, int nphi, const T *N, const DiaTables<T> &tb, T *out,
+                             hipStream_t s);
+
 // Diagnostics: per-column sums with FP64 accumulation; compensated totals.
 template <typename T>
 hipError_t launch_column_diagnostics(int nx, int ny, int nk, int nphi, const T *N,

@@ -14,7 +14,8 @@ from .model import (SpectralWaveModel, InfiniteDepth, Clock, Simulation, WENO, s
                     interior, PseudomomentumVelocities, SpatialAveraging, automatic_q_grid, update_coupling_,
                     compute_pseudomomentum_cell_averages, compute_pseudomomentum_cell_integrals,
                     pseudomomentum_fields)
-from .sources import (BulkWindDrag, LinearStormTrack, HollandHurricaneWind, PressureCorrelationInput,
+from .sources import (SymmetricQuadruplet, DiscreteInteractionApproximation,
+                      BulkWindDrag, LinearStormTrack, HollandHurricaneWind, PressureCorrelationInput,
                       MeanSpectrumWhitecapping, LocalSaturationDissipation, WhitecappingDissipation,
                       PrecomputedSources, wind_state)
 from .surrogate import (SourceNetwork, SurrogateConfig, training_pairs, attach_surrogate, surrogate_stats,
@@ -33,6 +34,7 @@ __all__ = [
     "compute_pseudomomentum_cell_averages", "compute_pseudomomentum_cell_integrals", "pseudomomentum_fields",
     "m0", "root_mean_square_wavenumber", "mean_direction", "significant_wave_height",
     "total_action", "total_deep_water_energy",
+    "SymmetricQuadruplet", "DiscreteInteractionApproximation",
     "BulkWindDrag", "LinearStormTrack", "HollandHurricaneWind", "PressureCorrelationInput",
     "MeanSpectrumWhitecapping", "LocalSaturationDissipation", "WhitecappingDissipation", "PrecomputedSources",
     "wind_state",

@@ -59,7 +59,9 @@ class rpl_sources(ctypes.Structure):
                    "cds", "delta1", "delta2", "p_mean",
                    "ls_cds", "ls_br", "ls_p", "ls_sigma_power",
                    "wc_rate", "wc_threshold", "wc_saturation_power", "wc_wavenumber_power",
-                   "wc_reference_wavenumber")]
+                   "wc_reference_wavenumber")] + \
+               [("nonlinear", ctypes.c_int32), ("dia_coefficient", ctypes.c_double),
+                ("dia_frequency_offset", ctypes.c_double), ("dia_gravity", ctypes.c_double)]
 
 
 class rpl_surrogate(ctypes.Structure):

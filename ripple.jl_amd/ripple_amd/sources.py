@@ -131,12 +131,36 @@ class WhitecappingDissipation:
         self.reference_wavenumber = float(reference_wavenumber)
 
 
-class PrecomputedSources:
-    """`PrecomputedSources(; wind_input, dissipation)`: the bundle whose wind
-    input carries the wave-supported-stress cap."""
+class SymmetricQuadruplet:
+    """`SymmetricQuadruplet(; coefficient=2.78e7, frequency_offset=0.25, gravity=9.81)`
+    (Physics/NonlinearInteractions/symmetric_quadruplet.jl:30-49): WAVEWATCH
+    III's single-quadruplet DIA, receivers at (1 +- lambda) sigma."""
 
-    def __init__(self, wind_input=None, dissipation=None):
-        self.wind_input, self.dissipation = wind_input, dissipation
+    def __init__(self, coefficient=2.78e7, frequency_offset=0.25, gravity=9.81):
+        self.coefficient, self.frequency_offset, self.gravity = (float(v) for v in (coefficient, frequency_offset,
+                                                                                     gravity))
+        lam = self.frequency_offset
+        a, b = (1 + lam) ** 2, (1 - lam) ** 2
+        self.direction_offset_plus = math.acos(min(1.0, max(-1.0, (a * a - b * b + 4) / (4 * a))))
+        self.direction_offset_minus = math.acos(min(1.0, max(-1.0, (b * b - a * a + 4) / (4 * b))))
+
+
+class DiscreteInteractionApproximation:
+    """`DiscreteInteractionApproximation(method)` (Sources/Sources.jl:419-428)
+    with a SymmetricQuadruplet method, the nonlinear four-wave transfer."""
+
+    def __init__(self, method=None):
+        self.method = method if method is not None else SymmetricQuadruplet()
+        if not isinstance(self.method, SymmetricQuadruplet):
+            raise NotImplementedError("only the SymmetricQuadruplet method is built")
+
+
+class PrecomputedSources:
+    """`PrecomputedSources(; wind_input, dissipation, nonlinear)`: the bundle
+    whose wind input carries the wave-supported-stress cap."""
+
+    def __init__(self, wind_input=None, dissipation=None, nonlinear=None):
+        self.wind_input, self.dissipation, self.nonlinear = wind_input, dissipation, nonlinear
 
 
 def to_struct(sources):
@@ -148,14 +172,23 @@ def to_struct(sources):
     s.rho_water = 1025.0
     s.von_karman = 0.4
     s.p_mean = 0.5
+    nl = None
     if isinstance(sources, PrecomputedSources):
-        wi, diss, cap = sources.wind_input, sources.dissipation, 1
+        wi, diss, cap, nl = sources.wind_input, sources.dissipation, 1, sources.nonlinear
     elif isinstance(sources, PressureCorrelationInput):
         wi, diss, cap = sources, None, 0
     elif isinstance(sources, (MeanSpectrumWhitecapping, LocalSaturationDissipation, WhitecappingDissipation)):
         wi, diss, cap = None, sources, 0
+    elif isinstance(sources, DiscreteInteractionApproximation):
+        wi, diss, cap, nl = None, None, 0, sources
     else:
         raise TypeError(f"unsupported sources {type(sources).__name__}")
+    if nl is not None:
+        if not isinstance(nl, DiscreteInteractionApproximation):
+            raise TypeError("the nonlinear member must be a DiscreteInteractionApproximation")
+        q = nl.method
+        s.nonlinear = 1
+        s.dia_coefficient, s.dia_frequency_offset, s.dia_gravity = q.coefficient, q.frequency_offset, q.gravity
     if isinstance(diss, MeanSpectrumWhitecapping) and not isinstance(sources, PrecomputedSources):
         raise NotImplementedError("MeanSpectrumWhitecapping runs through the PrecomputedSources bundle state")
     if wi is not None:
@@ -194,8 +227,12 @@ def to_struct(sources):
 
 def attach(model, sources):
     """Install `sources` on the model's device context."""
+    from .grids import FrequencyDirectionGrid
     lib = _lib.load()
     s = to_struct(sources)
+    if s.nonlinear and not isinstance(model.spectral_grid, FrequencyDirectionGrid):
+        # symmetric_quadruplet.jl:177: the transfer is defined on frequency-direction grids.
+        raise ValueError("the symmetric-quadruplet DIA needs a FrequencyDirectionGrid")
     xn = np.ascontiguousarray(xnodes(model.grid), dtype=np.float64)
     yn = np.ascontiguousarray(ynodes(model.grid), dtype=np.float64)
     _lib.check(lib.rpl_set_sources(model._ctx.ptr, ctypes.byref(s), _lib.dptr(xn), _lib.dptr(yn)))
