@@ -34,8 +34,12 @@ def build_model(wl, float_type, kernel="auto", **extra):
     g, sg = wl.grid, wl.spectral_grid
     grid = grids.RectilinearGrid(float_type=float_type, size=(g.Nx, g.Ny, g.Nz), x=g.x_faces, y=g.y_faces,
                                  z=g.z_faces)
-    sgrid = grids.PolarWaveVectorGrid(float_type, kappa=sg.kappa, phi=sg.phi, kappa_faces=sg.kappa_faces,
-                                      phi_faces=sg.phi_faces)
+    if isinstance(sg, grids.FrequencyDirectionGrid):
+        sgrid = grids.FrequencyDirectionGrid(float_type, frequency=sg.frequency, phi=sg.phi,
+                                             frequency_faces=sg.frequency_faces, phi_faces=sg.phi_faces)
+    else:
+        sgrid = grids.PolarWaveVectorGrid(float_type, kappa=sg.kappa, phi=sg.phi, kappa_faces=sg.kappa_faces,
+                                          phi_faces=sg.phi_faces)
     kw = dict(wl.model_kwargs)
     kw.setdefault("velocities", wl.velocities)
     kw.update(extra)
@@ -170,6 +174,32 @@ def smoothing_line(steps, warmup):
             "smoothing_achieved_GBs": sm_bytes / (l_sm * 1e-3) / 1e9 if l_sm > 0 else None}
 
 
+def dia_line(steps, warmup):
+    """The wind sea with the full source bundle including the DIA nonlinear
+    transfer (256 x 256 x 40 frequencies x 36 directions, FP32, intrinsic
+    transport): the RK3 step with and without the nonlinear member, and the
+    DIA source launch (dia_source_kernel: the columns' action and transfer in
+    LDS, the lookups tabulated)."""
+    import ctypes
+    from ripple_amd import sources as S
+    wl = workloads.wind_sea_with_dia()
+    m = build_model(wl, np.float32)
+    t_step, l_stage, _ = time_model(m, wl.dt, steps, warmup)
+    ms, n = ctypes.c_double(), ctypes.c_int64()
+    _lib.check(m._ctx.lib.rpl_get_nonlinear_timing(m._ctx.ptr, ctypes.byref(ms), ctypes.byref(n)))
+    l_dia = ms.value / max(n.value, 1)
+    del m
+    src = wl.model_kwargs["sources"]
+    plain = S.PrecomputedSources(wind_input=src.wind_input, dissipation=src.dissipation)
+    m2 = build_model(wl, np.float32, sources=plain)
+    t_plain, _, _ = time_model(m2, wl.dt, steps, warmup)
+    cells = wl.cell_bins
+    return {"workload": wl.name, "value": cells / t_step, "ms_per_step": t_step * 1e3, "stage_kernel_ms": l_stage,
+            "dia_source_ms": l_dia, "dia_launches_timed": n.value,
+            "without_dia_ms_per_step": t_plain * 1e3,
+            "dia_donor_pairs_per_s": 2 * cells / (l_dia * 1e-3) if l_dia > 0 else None}
+
+
 def ensemble_line(count, steps, with_surrogate=True, **member_kw):
     """The batched ensemble: `count` members drawn as ensemble_member(0..count-1)
     (64^2-512^2 cells x 16 levels, 16-40 kappa x 24-48 phi bins, own eddy
@@ -288,6 +318,8 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble line")
     ap.add_argument("--only-pseudomomentum", action="store_true",
                     help="print only the self-coupled pseudomomentum line (for profiling its kernels)")
+    ap.add_argument("--only-dia", action="store_true",
+                    help="print only the DIA nonlinear-source line (for profiling its kernel)")
     ap.add_argument("--only-smoothing", action="store_true",
                     help="print only the propagation-smoothing line (for profiling its kernel)")
     ap.add_argument("--ensemble-members", type=int, default=64)
@@ -311,6 +343,9 @@ def main():
 
     if args.only_pseudomomentum:
         print(json.dumps({"pseudomomentum_coupling": pseudomomentum_line(args.steps, args.warmup)}), flush=True)
+        return
+    if args.only_dia:
+        print(json.dumps({"nonlinear_dia": dia_line(args.steps, args.warmup)}), flush=True)
         return
     if args.only_smoothing:
         print(json.dumps({"propagation_smoothing": smoothing_line(args.steps, args.warmup)}), flush=True)
@@ -363,6 +398,7 @@ def main():
         extra["err_vs_fp64"] = err_vs_fp64()
         extra["pseudomomentum_coupling"] = pseudomomentum_line(max(3, args.steps // 2), 1)
         extra["propagation_smoothing"] = smoothing_line(max(3, args.steps // 2), 1)
+        extra["nonlinear_dia"] = dia_line(max(3, args.steps // 4), 1)
     if not args.no_storm:
         extra["storm_plain_vs_surrogate"] = storm_plain_vs_surrogate(max(2, args.steps // 4), 1)
     if not args.no_ensemble:

@@ -324,6 +324,11 @@ int rpl_get_coupling_timing(rpl_ctx *ctx, double *total_ms, int64_t *updates);
  * last call, under the same enable switch.  No counterpart in the reference. */
 int rpl_get_smoothing_timing(rpl_ctx *ctx, double *total_ms, int64_t *passes);
 
+/* Device time of the nonlinear (DIA) source launches (one per tendency
+ * evaluation) since the last call, under the same enable switch.  No
+ * counterpart in the reference. */
+int rpl_get_nonlinear_timing(rpl_ctx *ctx, double *total_ms, int64_t *launches);
+
 /* Ensemble batching.  No counterpart in the reference, which runs one model
  * per process.  Members are ordinary contexts -- FP32 on the fast kernel,
  * SSP-RK3, one transport kind (intrinsic or refraction) and either all with

@@ -49,46 +49,60 @@ __global__ __launch_bounds__(256) void dia_source_kernel(size_t plane, int nbins
         Tl[e] = T(0);
     }
     __syncthreads();
-    // symmetric_quadruplet.jl:92-173 for every (donor, column) pair.
+    // symmetric_quadruplet.jl:92-173 for every (donor, column) pair.  Both
+    // orientations gather first and scatter after: an LDS atomic may alias a
+    // later LDS read, so interleaving them would serialize every access.
     for (int e = threadIdx.x; e < items; e += 256) {
         const int d = e / cols, cc = e - d * cols;
         const T Na = Nl[e];
         if (!(Na > T(0)))
             continue;
-        const T Efa = tb.ef_a[d];
-        const T Ea = Efa * Na;
+        const T Ea = tb.ef_a[d] * Na;
+        int fl[2];
+        T TE[2];
 #pragma unroll
         for (int o = 0; o < 2; ++o) {
             const int q = 2 * d + o;
-            const int fl = tb.flags[q];
-            if (!fl)
-                continue;
-            const bool inp = fl & 1, inm = fl & 2;
+            fl[o] = tb.flags[q];
             const int *bp = tb.bins + 8 * (size_t)q;
             const T *wef = tb.wef + 8 * (size_t)q;
             T Ep = T(0), Em = T(0);
-            if (inp)
+            if (fl[o] & 1)
                 Ep = wef[0] * fmax(Nl[bp[0] * cols + cc], T(0)) + wef[1] * fmax(Nl[bp[1] * cols + cc], T(0)) +
                      wef[2] * fmax(Nl[bp[2] * cols + cc], T(0)) + wef[3] * fmax(Nl[bp[3] * cols + cc], T(0));
-            if (inm)
+            if (fl[o] & 2)
                 Em = wef[4] * fmax(Nl[bp[4] * cols + cc], T(0)) + wef[5] * fmax(Nl[bp[5] * cols + cc], T(0)) +
                      wef[6] * fmax(Nl[bp[6] * cols + cc], T(0)) + wef[7] * fmax(Nl[bp[7] * cols + cc], T(0));
-            const T s = Ea * (Ep * tb.dal1 + Em * tb.dal2) - Ep * Em * tb.dal3;
-            const T TE = tb.cf_a[d] * s / T(2);
-            const T donor = (inp && inm) ? T(2) : T(1);
-            atomicAdd(&Tl[e], -(donor * TE / Efa));
-            const T *w = tb.w + 8 * (size_t)q, *ef = tb.ef + 8 * (size_t)q;
-            if (inp) {
+            const T sa = Ea * (Ep * tb.dal1 + Em * tb.dal2) - Ep * Em * tb.dal3;
+            TE[o] = tb.cf_a[d] * sa * T(0.5);
+        }
+        // Action changes: the reference divides by the energy factors; their
+        // reciprocals (and w / E) are tabulated (rounding-level difference,
+        // no division in the loop).
+        const T inv_efa = tb.inv_ef_a[d];
+        T own = T(0);
+#pragma unroll
+        for (int o = 0; o < 2; ++o) {
+            if (!fl[o])
+                continue;
+            const T donor = fl[o] == 3 ? T(2) : T(1);
+            own = own - donor * TE[o] * inv_efa;
+            const int q = 2 * d + o;
+            const int *bp = tb.bins + 8 * (size_t)q;
+            const T *wd = tb.wdiv + 8 * (size_t)q;
+            if (fl[o] & 1) {
 #pragma unroll
                 for (int k = 0; k < 4; ++k)
-                    atomicAdd(&Tl[bp[k] * cols + cc], w[k] * TE / ef[k]);
+                    atomicAdd(&Tl[bp[k] * cols + cc], TE[o] * wd[k]);
             }
-            if (inm) {
+            if (fl[o] & 2) {
 #pragma unroll
                 for (int k = 4; k < 8; ++k)
-                    atomicAdd(&Tl[bp[k] * cols + cc], w[k] * TE / ef[k]);
+                    atomicAdd(&Tl[bp[k] * cols + cc], TE[o] * wd[k]);
             }
         }
+        if (fl[0] | fl[1])
+            atomicAdd(&Tl[e], own);
     }
     __syncthreads();
     for (int e = threadIdx.x; e < items; e += 256) {

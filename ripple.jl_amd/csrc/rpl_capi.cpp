@@ -63,8 +63,8 @@ struct rpl_ctx {
     // DIA nonlinear member of the sources: column-independent tables and the
     // per-stage source plane the stage kernels add (StageParams::src_add)
     bool dia = false;
-    void *dia_flags = nullptr, *dia_bins = nullptr, *dia_wef = nullptr, *dia_w = nullptr, *dia_ef = nullptr,
-         *dia_efa = nullptr, *dia_cfa = nullptr, *dia_add = nullptr;
+    void *dia_flags = nullptr, *dia_bins = nullptr, *dia_wef = nullptr, *dia_wdiv = nullptr, *dia_efa = nullptr,
+         *dia_inv_efa = nullptr, *dia_cfa = nullptr, *dia_add = nullptr;
     double dia_dal[3] = {0, 0, 0};
     // SpatialAveraging after every step (rpl_set_propagation_smoothing)
     bool gse = false;
@@ -107,13 +107,13 @@ struct rpl_ctx {
     // event timing: phase 0 = stage kernel launches, 1 = surrogate inference
     // (features and network forward of one stage input), 2 = pseudomomentum
     // coupling update (contraction and, on the reference path, gradients),
-    // 3 = propagation smoothing
+    // 3 = propagation smoothing, 4 = nonlinear (DIA) source
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
     std::vector<int> event_phase;
     size_t events_used = 0;
-    double timing_ms[4] = {0, 0, 0, 0};
-    int64_t timing_launches[4] = {0, 0, 0, 0};
+    double timing_ms[5] = {0, 0, 0, 0, 0};
+    int64_t timing_launches[5] = {0, 0, 0, 0, 0};
 };
 
 namespace {
@@ -383,9 +383,10 @@ int record_end(rpl_ctx *c);
 template <typename T>
 int launch_dia_t(rpl_ctx *c, const T *Nk)
 {
-    rpl::DiaTables<T> tb{(const int *)c->dia_flags, (const int *)c->dia_bins, (const T *)c->dia_wef,
-                         (const T *)c->dia_w, (const T *)c->dia_ef, (const T *)c->dia_efa, (const T *)c->dia_cfa,
-                         (T)c->dia_dal[0], (T)c->dia_dal[1], (T)c->dia_dal[2]};
+    rpl::DiaTables<T> tb{(const int *)c->dia_flags, (const int *)c->dia_bins,  (const T *)c->dia_wef,
+                         (const T *)c->dia_wdiv,       (const T *)c->dia_efa,  (const T *)c->dia_inv_efa,
+                         (const T *)c->dia_cfa,        (T)c->dia_dal[0],       (T)c->dia_dal[1],
+                         (T)c->dia_dal[2]};
     hipError_t e = rpl::launch_dia_source<T>(c->cfg.nx, c->cfg.ny, c->cfg.nkappa, c->cfg.nphi, Nk, tb,
                                              (T *)c->dia_add, c->stream);
     if (e != hipSuccess)
@@ -397,7 +398,10 @@ int launch_dia(rpl_ctx *c, const void *Nk)
 {
     if (!c->dia)
         return RPL_OK;
-    return is_f64(c) ? launch_dia_t<double>(c, (const double *)Nk) : launch_dia_t<float>(c, (const float *)Nk);
+    int rc = record_begin(c, 4);
+    if (rc == RPL_OK)
+        rc = is_f64(c) ? launch_dia_t<double>(c, (const double *)Nk) : launch_dia_t<float>(c, (const float *)Nk);
+    return rc == RPL_OK ? record_end(c) : rc;
 }
 
 int launch_sources(rpl_ctx *c, const void *Nk)
@@ -966,7 +970,7 @@ int rpl_destroy(rpl_ctx *c)
                     (void *)c->out_scale, (void *)c->s_err, (void *)c->use_plain, (void *)c->s_counts,
                     (void *)c->bin_hi, (void *)c->s_amax, (void *)c->s_flags, c->dUx, c->dUy, c->pm_O,
                     c->pm_dO, c->pm_kx, c->pm_ky, c->gse_cg, c->gse_dcg, c->dia_flags, c->dia_bins, c->dia_wef,
-                    c->dia_w, c->dia_ef, c->dia_efa, c->dia_cfa, c->dia_add})
+                    c->dia_wdiv, c->dia_efa, c->dia_inv_efa, c->dia_cfa, c->dia_add})
         free_ptr(p);
     for (auto &ev : c->events) {
         (void)hipEventDestroy(ev.first);
@@ -1712,12 +1716,14 @@ static int build_dia_tables(rpl_ctx *c, const rpl_sources &s)
     const double log_xfr = std::log(std::sqrt(g * kc[1]) / std::sqrt(g * kc[0]));
     const double phimin = c->h_pc[0], dphi = 2 * M_PI / np;
     std::vector<int> flags(2 * (size_t)nb, 0), bins(16 * (size_t)nb, 0);
-    std::vector<double> wef(16 * (size_t)nb, 0), w(16 * (size_t)nb, 0), ef(16 * (size_t)nb, 1), efa(nb), cfa(nb);
+    std::vector<double> wef(16 * (size_t)nb, 0), w(16 * (size_t)nb, 0), ef(16 * (size_t)nb, 1), efa(nb), cfa(nb),
+        inv_efa(nb), wdiv(16 * (size_t)nb, 0);
     for (int n = 0; n < np; ++n)
         for (int m = 0; m < nk; ++m) {
             const int d = m + nk * n;
             const double sa = std::sqrt(g * kc[m]);
             efa[d] = energy_factor(sa);
+            inv_efa[d] = 1 / efa[d];
             cfa[d] = ceff * std::pow(sa / (2 * M_PI), 11);
             const double sp = (1 + lam) * sa, sm = (1 - lam) * sa, tha = c->h_pc[n];
             for (int o = 0; o < 2; ++o) {
@@ -1755,19 +1761,21 @@ static int build_dia_tables(rpl_ctx *c, const rpl_sources &s)
     for (void **p : {&c->dia_bins})
         if (!*p && (rc = alloc(p, 16 * (size_t)nb * sizeof(int))))
             return rc;
-    for (void **p : {&c->dia_wef, &c->dia_w, &c->dia_ef})
+    for (size_t k = 0; k < wdiv.size(); ++k)
+        wdiv[k] = w[k] / ef[k];
+    for (void **p : {&c->dia_wef, &c->dia_wdiv})
         if (!*p && (rc = alloc(p, 16 * (size_t)nb * c->esize)))
             return rc;
-    for (void **p : {&c->dia_efa, &c->dia_cfa})
+    for (void **p : {&c->dia_efa, &c->dia_inv_efa, &c->dia_cfa})
         if (!*p && (rc = alloc(p, (size_t)nb * c->esize)))
             return rc;
     if (!c->dia_add && (rc = alloc(&c->dia_add, c->cnt * c->esize)))
         return rc;
     RPL_HIP(hipMemcpy(c->dia_flags, flags.data(), flags.size() * sizeof(int), hipMemcpyHostToDevice));
     RPL_HIP(hipMemcpy(c->dia_bins, bins.data(), bins.size() * sizeof(int), hipMemcpyHostToDevice));
-    if ((rc = upload_real(c, wef.data(), c->dia_wef, wef.size())) || (rc = upload_real(c, w.data(), c->dia_w, w.size())) ||
-        (rc = upload_real(c, ef.data(), c->dia_ef, ef.size())) || (rc = upload_real(c, efa.data(), c->dia_efa, nb)) ||
-        (rc = upload_real(c, cfa.data(), c->dia_cfa, nb)))
+    if ((rc = upload_real(c, wef.data(), c->dia_wef, wef.size())) ||
+        (rc = upload_real(c, wdiv.data(), c->dia_wdiv, wdiv.size())) || (rc = upload_real(c, efa.data(), c->dia_efa, nb)) ||
+        (rc = upload_real(c, inv_efa.data(), c->dia_inv_efa, nb)) || (rc = upload_real(c, cfa.data(), c->dia_cfa, nb)))
         return rc;
     c->dia_dal[0] = 1 / std::pow(1 + lam, 4);
     c->dia_dal[1] = 1 / std::pow(1 - lam, 4);
@@ -2078,6 +2086,22 @@ int rpl_get_coupling_timing(rpl_ctx *c, double *total_ms, int64_t *updates)
         *updates = c->timing_launches[2];
     c->timing_ms[2] = 0;
     c->timing_launches[2] = 0;
+    return RPL_OK;
+}
+
+int rpl_get_nonlinear_timing(rpl_ctx *c, double *total_ms, int64_t *launches)
+{
+    if (!c)
+        return fail(RPL_ERR_ARG, "null context");
+    int rc = fold_timing(c);
+    if (rc)
+        return rc;
+    if (total_ms)
+        *total_ms = c->timing_ms[4];
+    if (launches)
+        *launches = c->timing_launches[4];
+    c->timing_ms[4] = 0;
+    c->timing_launches[4] = 0;
     return RPL_OK;
 }
 

@@ -148,16 +148,16 @@ hipError_t launch_spatial_averaging(bool fast, int nx, int ny, int nk, int nphi,
 
 // DiscreteInteractionApproximation{SymmetricQuadruplet}: per donor bin d and
 // orientation o (entry q = 2 d + o, o = 0 for +1, 1 for -1) the receivers'
-// corner bins (four for sigma+, four for sigma-), weight times energy
-// factor (the gather), weight and energy factor (the scatter), flags (bit 0:
-// sigma+ on the grid, bit 1: sigma- on the grid); per donor its energy
-// factor and C/g^4 f^11.  Column-independent; built on the host.
+// corner bins (four for sigma+, four for sigma-), weight times energy factor
+// (the gather) and weight over energy factor (the scatter), and flags (bit 0:
+// sigma+ on the grid, bit 1: sigma- on the grid); per donor its energy factor,
+// its reciprocal and C/g^4 f^11.  Column-independent; built on the host.
 template <typename T>
 struct DiaTables {
-    const int *flags;  // 2 nbins
-    const int *bins;   // 16 nbins
-    const T *wef, *w, *ef;  // 16 nbins each
-    const T *ef_a, *cf_a;   // nbins each
+    const int *flags;        // 2 nbins
+    const int *bins;         // 16 nbins
+    const T *wef, *wdiv;     // 16 nbins each
+    const T *ef_a, *inv_ef_a, *cf_a;  // nbins each
     T dal1, dal2, dal3;
 };
 template <typename T>

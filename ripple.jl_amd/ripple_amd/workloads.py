@@ -153,6 +153,35 @@ def fetch_limited_growth(float_type=np.float32, nx=256, ny=256, nz=16, nfreq=40,
                     dict(timestepper="RK3", sources=sources, depth=50.0))
 
 
+def wind_sea_with_dia(float_type=np.float32, nx=256, ny=256, nz=4, nfreq=40, nphi=36, dt=60.0, nsteps=10,
+                      hs=2.0, fp=0.12, seed=DEFAULT_SEED):
+    """The fetch-limited box (100 km x 100 km periodic, depth 50 m) with the
+    full source bundle including the nonlinear transfer:
+    PrecomputedSources(PressureCorrelationInput (U10 = 15 m/s along 0.3 rad,
+    linear drag, stress cap), LocalSaturationDissipation,
+    DiscreteInteractionApproximation(SymmetricQuadruplet())); 40 geometric
+    frequencies 0.05-0.4 Hz x 36 directions (the frequency range of the
+    reference's DIA tests); a developed JONSWAP sea (Hs = 2 m, fp = 0.12 Hz,
+    cos^2 spreading) with +-30 % per-cell noise; WENO5 intrinsic transport,
+    SSP-RK3, dt = 60 s."""
+    from .sources import (PrecomputedSources, PressureCorrelationInput, LocalSaturationDissipation, BulkWindDrag,
+                          DiscreteInteractionApproximation, SymmetricQuadruplet)
+    L = 100e3
+    grid = RectilinearGrid(float_type=float_type, size=(nx, ny, nz), x=(0, L), y=(0, L), z=(-50.0, 0.0),
+                           topology=(PERIODIC, PERIODIC, BOUNDED))
+    sg = FrequencyDirectionGrid(float_type, frequency=np.geomspace(0.05, 0.4, nfreq),
+                                phi=np.linspace(0, 2 * math.pi, nphi + 1)[:nphi])
+    sources = PrecomputedSources(
+        wind_input=PressureCorrelationInput(drag=BulkWindDrag("linear"), wind=15.0, direction=0.3),
+        dissipation=LocalSaturationDissipation(),
+        nonlinear=DiscreteInteractionApproximation(SymmetricQuadruplet()))
+    rng = np.random.Generator(np.random.Philox(seed))
+    N0 = jonswap_action(grid, sg, hs, fp)
+    N0 = np.asfortranarray(N0 * rng.uniform(0.7, 1.3, size=N0.shape))
+    return Workload(f"wind_sea_with_dia_{nx}x{ny}_f{nfreq}_p{nphi}", grid, sg, None, N0, dt, nsteps,
+                    dict(timestepper="RK3", sources=sources, depth=50.0))
+
+
 def translating_hurricane(float_type=np.float32, nx=1024, ny=1024, nz=32, nkappa=32, nphi=36, dt=60.0,
                           nsteps=60):
     """1024^2 x 32 periodic box of 800 km x 800 km, depth 200 m; polar kappa
