@@ -307,6 +307,32 @@ def cpu_baseline(threads):
                       f"warm-up, OpenMP {threads} threads, {t * 1e3:.0f} ms/step"}
 
 
+def write_performance_tsv(path, record):
+    """The measured lines as Validation.jl's performance summary
+    (case, operation, seconds, bytes, description): seconds per RK3 step,
+    bytes = the line's algorithmic HBM bytes per step where it has them."""
+    from ripple_amd.validation import PerformanceMetric, write_performance_summary
+    rows = [PerformanceMetric(record["config"]["workload"], "rk3_step", record["ms_per_step"] * 1e-3,
+                              3 * record["roofline"]["bytes_per_launch"], "FP32 sheared-eddy RK3 step (headline)")]
+    e = record.get("extra", {})
+    if "fp64_reference_kernel" in e:
+        rows.append(PerformanceMetric(record["config"]["workload"], "rk3_step_fp64",
+                                      e["fp64_reference_kernel"]["ms_per_step"] * 1e-3, 0.0,
+                                      "FP64 reference-order RK3 step"))
+    for key, op, desc in (("pseudomomentum_coupling", "rk3_step", "RK3 step with the self-coupled pseudomomentum current"),
+                          ("propagation_smoothing", "rk3_step", "RK3 step plus SpatialAveraging"),
+                          ("nonlinear_dia", "rk3_step", "RK3 step with wind input, dissipation and DIA")):
+        if key in e:
+            rows.append(PerformanceMetric(e[key]["workload"], op, e[key]["ms_per_step"] * 1e-3, 0.0, desc))
+    if "storm_plain_vs_surrogate" in e:
+        st = e["storm_plain_vs_surrogate"]
+        rows.append(PerformanceMetric(st["workload"], "rk3_step_plain", st["plain"]["ms_per_step"] * 1e-3, 0.0,
+                                      "hurricane RK3 step, plain sources"))
+        rows.append(PerformanceMetric(st["workload"], "rk3_step_surrogate", st["surrogate"]["ms_per_step"] * 1e-3,
+                                      0.0, "hurricane RK3 step, learned surrogate sources"))
+    write_performance_summary(path, rows)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -323,6 +349,8 @@ def main():
     ap.add_argument("--only-smoothing", action="store_true",
                     help="print only the propagation-smoothing line (for profiling its kernel)")
     ap.add_argument("--ensemble-members", type=int, default=64)
+    ap.add_argument("--performance-summary", default=None,
+                    help="also write the measured lines as a reference-format performance summary (TSV)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy (reference for HBM byte counters) and exit")
     args = ap.parse_args()
@@ -406,6 +434,8 @@ def main():
         extra["ensemble_small_members"] = ensemble_line(256, 10, with_surrogate=False, size_range=(16, 64),
                                                         kappa_range=(8, 16), phi_range=(12, 24))
     record["extra"] = extra
+    if args.performance_summary:
+        write_performance_tsv(args.performance_summary, record)
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         record["cpu_baseline"] = cpu_baseline(threads)

@@ -21,6 +21,7 @@ from .sources import (SymmetricQuadruplet, DiscreteInteractionApproximation,
 from .surrogate import (SourceNetwork, SurrogateConfig, training_pairs, attach_surrogate, surrogate_stats,
                         train as train_surrogate)
 from .ensemble import Ensemble
+from . import validation
 from .diagnostics import (m0, root_mean_square_wavenumber, mean_direction, significant_wave_height,
                           total_action, total_deep_water_energy)
 
