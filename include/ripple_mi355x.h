@@ -152,6 +152,16 @@ int rpl_get_doppler_kappa_derivative(rpl_ctx *ctx, double *dUx, double *dUy);
 int rpl_pseudomomentum(rpl_ctx *ctx, int32_t nz, const double *z_faces, double q_depth, int32_t cell_average,
                        double *px, double *py);
 
+/* compute_pseudomomentum_tendency_cell_averages! and
+ * cwcm_momentum_tendency_fields! (pseudomomentum.jl:219-283, 402-436): the
+ * cell-averaged pseudomomentum of the action tendency (N - old_N) / dt,
+ * with N the current action and old_N a host array of the context's element
+ * type and layout, times coefficient (1 for the tendency fields, -1 for the
+ * CWCM momentum forcing).  dt > 0, else RPL_ERR_ARG (pseudomomentum.jl:226).
+ * Uses the stage buffers: call between steps.  ptx, pty: (nx, ny, nz) FP64. */
+int rpl_pseudomomentum_tendency(rpl_ctx *ctx, const void *old_N, double dt, int32_t nz, const double *z_faces,
+                                double q_depth, double coefficient, double *ptx, double *pty);
+
 /* set!(model; N=array) (ProductFields/set.jl:20-30, 44-48) and interior(N)
  * (product_field.jl:221-230).  Element type is the context dtype. */
 int rpl_set_action(rpl_ctx *ctx, const void *N);

@@ -269,6 +269,16 @@ __global__ __launch_bounds__(256) void pm_moment_kernel(size_t plane, int nk, in
     }
 }
 
+// pseudomomentum.jl:412: the action tendency (new - old) / dt per cell.
+template <typename T>
+__global__ __launch_bounds__(256) void action_tendency_kernel(size_t n, const T *__restrict__ Nnew,
+                                                              const T *__restrict__ Nold, T dt, T *out)
+{
+    const size_t k = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (k < n)
+        out[k] = (Nnew[k] - Nold[k]) / dt;
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void cast_from_f64_kernel(const double *__restrict__ src, T *dst, size_t n)
 {
@@ -363,6 +373,20 @@ template hipError_t launch_pm_contract<float>(bool, int, int, int, int, int, con
 template hipError_t launch_pm_contract<double>(bool, int, int, int, int, int, const double *, const double *,
                                                const double *, const double *, const double *, const double *,
                                                double *, double *, double *, double *, hipStream_t);
+
+template <typename T>
+hipError_t launch_action_tendency(size_t n, const T *Nnew, const T *Nold, T dt, T *out, hipStream_t s)
+{
+    if (n == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(action_tendency_kernel<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, Nnew, Nold,
+                       dt, out);
+    return hipGetLastError();
+}
+
+template hipError_t launch_action_tendency<float>(size_t, const float *, const float *, float, float *, hipStream_t);
+template hipError_t launch_action_tendency<double>(size_t, const double *, const double *, double, double *,
+                                                   hipStream_t);
 
 template <typename T>
 hipError_t launch_cast_from_f64(const double *src, T *dst, size_t n, hipStream_t s)

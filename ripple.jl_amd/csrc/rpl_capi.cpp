@@ -1374,8 +1374,40 @@ int rpl_get_doppler_kappa_derivative(rpl_ctx *c, double *dUx, double *dUy)
     return RPL_OK;
 }
 
+// The pseudomomentum cell contraction of a device action field N (the
+// action itself, or its tendency), scaled by coefficient when it is not 1.
+static int pseudomomentum_cells(rpl_ctx *c, const void *Nsrc, int32_t nz, const double *z_faces, double q_depth,
+                                int32_t cell_average, double coefficient, double *px, double *py);
+
 int rpl_pseudomomentum(rpl_ctx *c, int32_t nz, const double *z_faces, double q_depth, int32_t cell_average,
                        double *px, double *py)
+{
+    if (!c)
+        return fail(RPL_ERR_ARG, "null argument");
+    return pseudomomentum_cells(c, c->A, nz, z_faces, q_depth, cell_average, 1.0, px, py);
+}
+
+int rpl_pseudomomentum_tendency(rpl_ctx *c, const void *old_N, double dt, int32_t nz, const double *z_faces,
+                                double q_depth, double coefficient, double *ptx, double *pty)
+{
+    if (!c || !old_N)
+        return fail(RPL_ERR_ARG, "null argument");
+    if (!(dt > 0))
+        return fail(RPL_ERR_ARG, "pseudomomentum tendency timestep must be positive");  // pseudomomentum.jl:226
+    RPL_HIP(hipSetDevice(c->device));
+    // The stage buffers are free between steps: old action into B, tendency into C.
+    RPL_HIP(hipMemcpyAsync(c->B, old_N, c->cnt * c->esize, hipMemcpyHostToDevice, c->stream));
+    hipError_t e = is_f64(c) ? rpl::launch_action_tendency<double>(c->cnt, (const double *)c->A, (const double *)c->B,
+                                                                    dt, (double *)c->C, c->stream)
+                             : rpl::launch_action_tendency<float>(c->cnt, (const float *)c->A, (const float *)c->B,
+                                                                   (float)dt, (float *)c->C, c->stream);
+    if (e != hipSuccess)
+        return fail(RPL_ERR_HIP, std::string("action tendency: ") + hipGetErrorString(e));
+    return pseudomomentum_cells(c, c->C, nz, z_faces, q_depth, 1, coefficient, ptx, pty);
+}
+
+static int pseudomomentum_cells(rpl_ctx *c, const void *Nsrc, int32_t nz, const double *z_faces, double q_depth,
+                                int32_t cell_average, double coefficient, double *px, double *py)
 {
     if (!c || !z_faces || !px || !py)
         return fail(RPL_ERR_ARG, "null argument");
@@ -1415,13 +1447,13 @@ int rpl_pseudomomentum(rpl_ctx *c, int32_t nz, const double *z_faces, double q_d
         hipError_t e;
         const bool ordered = pm_ordered(c);
         if (is_f64(c))
-            e = rpl::launch_pm_contract<double>(ordered, nx, ny, nk, np, nz, (const double *)c->A,
+            e = rpl::launch_pm_contract<double>(ordered, nx, ny, nk, np, nz, (const double *)Nsrc,
                                                 (const double *)dW,
                                                 nullptr, (const double *)dkx, (const double *)dky,
                                                 cell_average ? (const double *)dS : nullptr, (double *)dpx,
                                                 (double *)dpy, nullptr, nullptr, c->stream);
         else
-            e = rpl::launch_pm_contract<float>(ordered, nx, ny, nk, np, nz, (const float *)c->A,
+            e = rpl::launch_pm_contract<float>(ordered, nx, ny, nk, np, nz, (const float *)Nsrc,
                                                (const float *)dW,
                                                nullptr, (const float *)dkx, (const float *)dky,
                                                cell_average ? (const float *)dS : nullptr, (float *)dpx,
@@ -1432,17 +1464,25 @@ int rpl_pseudomomentum(rpl_ctx *c, int32_t nz, const double *z_faces, double q_d
             rc = fail(RPL_ERR_HIP, std::string("pseudomomentum: ") + hipGetErrorString(e));
             goto done;
         }
+        // _scale_field_pair_kernel! (pseudomomentum.jl:430-436) in the element
+        // type, when a coefficient other than one is asked for.
         if (is_f64(c)) {
             if ((e = hipMemcpy(px, dpx, out * 8, hipMemcpyDeviceToHost)) == hipSuccess)
                 e = hipMemcpy(py, dpy, out * 8, hipMemcpyDeviceToHost);
+            if (coefficient != 1.0)
+                for (size_t k = 0; k < out; ++k) {
+                    px[k] *= coefficient;
+                    py[k] *= coefficient;
+                }
         } else {
+            const float cf = (float)coefficient;
             std::vector<float> t(out);
             if ((e = hipMemcpy(t.data(), dpx, out * 4, hipMemcpyDeviceToHost)) == hipSuccess)
                 for (size_t k = 0; k < out; ++k)
-                    px[k] = t[k];
+                    px[k] = coefficient != 1.0 ? t[k] * cf : t[k];
             if (e == hipSuccess && (e = hipMemcpy(t.data(), dpy, out * 4, hipMemcpyDeviceToHost)) == hipSuccess)
                 for (size_t k = 0; k < out; ++k)
-                    py[k] = t[k];
+                    py[k] = coefficient != 1.0 ? t[k] * cf : t[k];
         }
         if (e != hipSuccess)
             rc = fail(RPL_ERR_HIP, std::string("pseudomomentum copy: ") + hipGetErrorString(e));

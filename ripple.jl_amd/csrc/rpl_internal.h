@@ -137,6 +137,10 @@ hipError_t launch_pm_contract(bool ordered, int nx, int ny, int nk, int nphi, in
                               const T *B, const T *kx, const T *ky, const T *scale, T *X, T *Y, T *dX, T *dY,
                               hipStream_t s);
 
+// out = (Nnew - Nold) / dt elementwise (pseudomomentum.jl:412).
+template <typename T>
+hipError_t launch_action_tendency(size_t n, const T *Nnew, const T *Nold, T dt, T *out, hipStream_t s);
+
 // SpatialAveraging (Models/propagation_smoothing.jl:56-110): Nout = mean of
 // four bilinear corner interpolations of Nin per (kappa, phi) bin, periodic
 // x and y; cg, dcg per kappa bin, cos/sin of the phi centres.  fast (FP32

@@ -13,7 +13,8 @@ from .grids import (RectilinearGrid, PolarWaveVectorGrid, FrequencyDirectionGrid
 from .model import (SpectralWaveModel, InfiniteDepth, Clock, Simulation, WENO, set_, time_step_, run_,
                     interior, PseudomomentumVelocities, SpatialAveraging, automatic_q_grid, update_coupling_,
                     compute_pseudomomentum_cell_averages, compute_pseudomomentum_cell_integrals,
-                    pseudomomentum_fields)
+                    pseudomomentum_fields, compute_pseudomomentum_tendency_cell_averages,
+                    pseudomomentum_tendency_fields, cwcm_momentum_tendency_fields)
 from .sources import (SymmetricQuadruplet, DiscreteInteractionApproximation,
                       BulkWindDrag, LinearStormTrack, HollandHurricaneWind, PressureCorrelationInput,
                       MeanSpectrumWhitecapping, LocalSaturationDissipation, WhitecappingDissipation,
@@ -33,6 +34,7 @@ __all__ = [
     "SpectralWaveModel", "InfiniteDepth", "Clock", "Simulation", "WENO", "set_", "time_step_", "run_",
     "interior", "PseudomomentumVelocities", "SpatialAveraging", "automatic_q_grid", "update_coupling_",
     "compute_pseudomomentum_cell_averages", "compute_pseudomomentum_cell_integrals", "pseudomomentum_fields",
+    "compute_pseudomomentum_tendency_cell_averages", "pseudomomentum_tendency_fields", "cwcm_momentum_tendency_fields",
     "m0", "root_mean_square_wavenumber", "mean_direction", "significant_wave_height",
     "total_action", "total_deep_water_energy",
     "SymmetricQuadruplet", "DiscreteInteractionApproximation",

@@ -380,6 +380,41 @@ def compute_pseudomomentum_cell_integrals(model, depth=None, q_grid=None):
 pseudomomentum_fields = compute_pseudomomentum_cell_averages
 
 
+def compute_pseudomomentum_tendency_cell_averages(model, old_N, dt, depth=None, q_grid=None, coefficient=1.0):
+    """compute_pseudomomentum_tendency_cell_averages!(ptx, pty, new_N, old_N, dt,
+    depth, qtransform) (Coupling/pseudomomentum.jl:219-252) with new_N the
+    model's action: the cell-averaged pseudomomentum of (new_N - old_N) / dt,
+    times `coefficient`."""
+    if not dt > 0:
+        raise _lib.RippleArgumentError(-1, "pseudomomentum tendency timestep must be positive")
+    q_grid = q_grid if q_grid is not None else (model.q_grid if model.q_grid is not None else model.grid)
+    if q_grid.Nz < 1:
+        raise ValueError("QTransform requires a grid with a resolved vertical coordinate")
+    old = np.asarray(old_N, dtype=model.dtype)
+    if old.shape != model.size:
+        raise ValueError("new and old wave-action fields must have matching sizes")
+    d = model._q_depth(q_grid) if depth is None else float(depth)
+    zf = np.ascontiguousarray(q_grid.z_faces, dtype=np.float64)
+    flat = np.ascontiguousarray(old.ravel(order="F"))
+    Nx, Ny = model.size[:2]
+    px, py = np.zeros(Nx * Ny * q_grid.Nz), np.zeros(Nx * Ny * q_grid.Nz)
+    _lib.check(model._ctx.lib.rpl_pseudomomentum_tendency(model._ctx.ptr, _lib.vptr(flat), float(dt), q_grid.Nz,
+                                                          _lib.dptr(zf), d, float(coefficient), _lib.dptr(px),
+                                                          _lib.dptr(py)))
+    shp = (Nx, Ny, q_grid.Nz)
+    return px.reshape(shp, order="F"), py.reshape(shp, order="F")
+
+
+pseudomomentum_tendency_fields = compute_pseudomomentum_tendency_cell_averages
+
+
+def cwcm_momentum_tendency_fields(model, old_N, dt, depth=None, q_grid=None, coefficient=-1.0):
+    """cwcm_momentum_tendency_fields! (pseudomomentum.jl:254-270): the
+    pseudomomentum tendency scaled by coefficient (default -1), the momentum
+    forcing a current model receives."""
+    return compute_pseudomomentum_tendency_cell_averages(model, old_N, dt, depth, q_grid, coefficient)
+
+
 class Simulation:
     """`Simulation(model; Δt, stop_iteration, stop_time)` (Oceananigans)."""
 

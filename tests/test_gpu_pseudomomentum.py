@@ -167,6 +167,34 @@ def test_pseudomomentum_cells(oracle, average):
         assert np.array_equal(a, b)
 
 
+def test_pseudomomentum_tendency_fields(oracle):
+    """pseudomomentum.jl:219-283: cell averages of (new - old) / dt, and the
+    CWCM momentum forcing (coefficient -1), against the oracle's contraction
+    of the same difference (FP64 bitwise)."""
+    say("pseudomomentum tendency fields")
+    wl = packet(nx=24, ny=20, nz=10)
+    g, sg = wl.grid, wl.spectral_grid
+    m = device_model(wl, np.float64)
+    old = m.action.copy()
+    ra.time_step_(m, wl.dt, 1)
+    new = m.action
+    dt = wl.dt
+    t = oracle.geometry(sg.kappa, sg.kappa_faces, sg.phi, sg.phi_faces)
+    W = oracle.q_weights(g.z_faces, sg.kappa, g.Lz)
+    want = oracle.pm_cells((new - old) / dt, W, g.z_faces, t["mu_x"], t["mu_y"], cell_average=True)
+    got = ra.compute_pseudomomentum_tendency_cell_averages(m, old, dt)
+    for a, b in zip(got, want):
+        assert np.array_equal(a, b)
+        assert np.abs(b).max() > 0
+    forcing = ra.cwcm_momentum_tendency_fields(m, old, dt)
+    for a, b in zip(forcing, want):
+        assert np.array_equal(a, -b)
+    with pytest.raises(ra.RippleArgumentError):
+        ra.compute_pseudomomentum_tendency_cell_averages(m, old, 0.0)
+    # The action is untouched by the call (the stage buffers hold old and the tendency).
+    assert np.array_equal(m.action, new)
+
+
 def test_prescribed_current_kappa_derivative(oracle):
     """doppler_velocity.jl:25-43: dU/dkappa of a prescribed current, and the
     reference's check that a depth-uniform current has none
