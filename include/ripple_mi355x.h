@@ -157,7 +157,7 @@ int rpl_pseudomomentum(rpl_ctx *ctx, int32_t nz, const double *z_faces, double q
  * cell-averaged pseudomomentum of the action tendency (N - old_N) / dt,
  * with N the current action and old_N a host array of the context's element
  * type and layout, times coefficient (1 for the tendency fields, -1 for the
- * CWCM momentum forcing).  dt > 0, else RPL_ERR_ARG (pseudomomentum.jl:226).
+ * CWCM momentum forcing).  dt > 0, else RPL_ERR_ARG (pseudomomentum.jl:225).
  * Uses the stage buffers: call between steps.  ptx, pty: (nx, ny, nz) FP64. */
 int rpl_pseudomomentum_tendency(rpl_ctx *ctx, const void *old_N, double dt, int32_t nz, const double *z_faces,
                                 double q_depth, double coefficient, double *ptx, double *pty);

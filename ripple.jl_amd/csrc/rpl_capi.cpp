@@ -1393,7 +1393,7 @@ int rpl_pseudomomentum_tendency(rpl_ctx *c, const void *old_N, double dt, int32_
     if (!c || !old_N)
         return fail(RPL_ERR_ARG, "null argument");
     if (!(dt > 0))
-        return fail(RPL_ERR_ARG, "pseudomomentum tendency timestep must be positive");  // pseudomomentum.jl:226
+        return fail(RPL_ERR_ARG, "pseudomomentum tendency timestep must be positive");  // pseudomomentum.jl:225
     RPL_HIP(hipSetDevice(c->device));
     // The stage buffers are free between steps: old action into B, tendency into C.
     RPL_HIP(hipMemcpyAsync(c->B, old_N, c->cnt * c->esize, hipMemcpyHostToDevice, c->stream));
