@@ -249,6 +249,18 @@ int rpl_set_sources(rpl_ctx *ctx, const rpl_sources *src, const double *x_nodes,
  * instead of the two track points of rpl_sources.  n = 0 removes it.  Times
  * must increase strictly (RPL_ERR_ARG otherwise, as the reference's
  * ArgumentError). */
+/* DiscreteInteractionApproximation(interactions; power, resonance_tolerance)
+ * with the ListedQuadruplets method (Sources/Sources.jl:374-428, 1268-1385):
+ * n explicit interactions, bins[8 q .. 8 q + 7] = donor1 (m, n), donor2,
+ * receiver1, receiver2 (0-based), constant rates[q] >= 0; frequency holds
+ * the nkappa frequency centers of the FrequencyDirectionGrid.  Checked as the
+ * reference checks them (indices, distinct donor and receiver pairs,
+ * frequency and wavevector resonance within the tolerance, positive power).
+ * The source enters every tendency evaluation; n = 0 removes it.  One
+ * nonlinear term at a time (not with rpl_sources.nonlinear). */
+int rpl_set_quadruplets(rpl_ctx *ctx, int32_t n, const int32_t *bins, const double *rates,
+                        const double *frequency, double power, double resonance_tolerance);
+
 int rpl_set_storm_track(rpl_ctx *ctx, int32_t n, const double *times, const double *x, const double *y);
 
 /* Per-column wind state of the last step: U10, wind angle, friction velocity,

@@ -471,3 +471,53 @@ void orc_wind_columns(const orc_sources *s, int nx, int ny, const double *x_node
             u_star[o] = orc_friction_velocity(s, U10[o]);
         }
 }
+
+/* DiscreteInteractionApproximation{ListedQuadruplets} (Sources/Sources.jl:
+ * 374-428, 1332-1385) with constant rates: for every cell and every listed
+ * interaction (donors d1, d2, receivers r1, r2; 0-based (m, n) bins),
+ *   flux = rate max(N_d1, 0)^p max(N_d2, 0)^p,
+ *   receivers: positive += flux / w,  donors: damping += flux / (w N) (N > 0),
+ * in list order, then G += positive - damping N. */
+typedef struct orc_listed_ctx {
+    int nx, ny, nk, nphi;
+    int count;
+    const int *bins;      /* 8 per interaction: d1m d1n d2m d2n r1m r1n r2m r2n */
+    const double *rates;  /* per interaction */
+    double power;
+    const double *w;      /* (nk, nphi) cell measures */
+} orc_listed_ctx;
+
+void orc_listed_hook_f64(const double *N, double *G, void *user)
+{
+    const orc_listed_ctx *c = (const orc_listed_ctx *)user;
+    const size_t plane = (size_t)c->nx * c->ny;
+    const int nk = c->nk;
+#pragma omp parallel for schedule(static)
+    for (long ij = 0; ij < (long)plane; ++ij)
+        for (int n = 0; n < c->nphi; ++n)
+            for (int m = 0; m < nk; ++m) {
+                const size_t cell = ij + plane * ((size_t)m + (size_t)nk * n);
+                const double weight = c->w[m + (size_t)nk * n];
+                const double a = N[cell];
+                double positive = 0, damping = 0;
+                for (int q = 0; q < c->count; ++q) {
+                    const int *b = c->bins + 8 * (size_t)q;
+                    double a1 = N[ij + plane * ((size_t)b[0] + (size_t)nk * b[1])];
+                    double a2 = N[ij + plane * ((size_t)b[2] + (size_t)nk * b[3])];
+                    a1 = a1 > 0 ? a1 : 0;
+                    a2 = a2 > 0 ? a2 : 0;
+                    const double flux = c->rates[q] * pow(a1, c->power) * pow(a2, c->power);
+                    if (flux == 0)
+                        continue;
+                    if (m == b[4] && n == b[5])
+                        positive += flux / weight;
+                    if (m == b[6] && n == b[7])
+                        positive += flux / weight;
+                    if (m == b[0] && n == b[1] && a > 0)
+                        damping += flux / (weight * a);
+                    if (m == b[2] && n == b[3] && a > 0)
+                        damping += flux / (weight * a);
+                }
+                G[cell] += positive - damping * a;
+            }
+}

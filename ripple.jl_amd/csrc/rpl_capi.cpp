@@ -66,6 +66,11 @@ struct rpl_ctx {
     void *dia_flags = nullptr, *dia_bins = nullptr, *dia_wef = nullptr, *dia_wdiv = nullptr, *dia_efa = nullptr,
          *dia_inv_efa = nullptr, *dia_cfa = nullptr, *dia_add = nullptr;
     double dia_dal[3] = {0, 0, 0};
+    // ListedQuadruplets (rpl_set_quadruplets)
+    bool listed = false;
+    int32_t listed_n = 0;
+    double listed_power = 1;
+    void *listed_bins = nullptr, *listed_rates = nullptr, *listed_w = nullptr;
     // SpatialAveraging after every step (rpl_set_propagation_smoothing)
     bool gse = false;
     double gse_as = 0.5, gse_an = 0.5, gse_cg_max = 0, gse_dcg_max = 0;
@@ -183,7 +188,7 @@ rpl::StageParams<T> stage_params(const rpl_ctx *c, double dt)
     p.dk_inv = T(1) / (T)c->dk;
     p.dp_inv = T(1) / (T)c->dp;
     p.dt = (T)dt;
-    p.src_add = c->dia ? (const T *)c->dia_add : nullptr;
+    p.src_add = (c->dia || c->listed) ? (const T *)c->dia_add : nullptr;
     p.src = rpl::SourceParams<T>{};
     if (c->has_sources && (c->src.wind_input || c->src.dissipation)) {
         const rpl_sources &s = c->src;
@@ -396,6 +401,26 @@ int launch_dia_t(rpl_ctx *c, const T *Nk)
 
 int launch_dia(rpl_ctx *c, const void *Nk)
 {
+    if (c->listed) {
+        int rc = record_begin(c, 4);
+        if (rc)
+            return rc;
+        hipError_t e;
+        const int nx = c->cfg.nx, ny = c->cfg.ny, nk = c->cfg.nkappa, np = c->cfg.nphi;
+        if (is_f64(c))
+            e = rpl::launch_listed_source<double>(nx, ny, nk, np, c->listed_n, (const int *)c->listed_bins,
+                                                  (const double *)c->listed_rates, c->listed_power,
+                                                  (const double *)c->listed_w, (const double *)Nk,
+                                                  (double *)c->dia_add, c->stream);
+        else
+            e = rpl::launch_listed_source<float>(nx, ny, nk, np, c->listed_n, (const int *)c->listed_bins,
+                                                 (const float *)c->listed_rates, (float)c->listed_power,
+                                                 (const float *)c->listed_w, (const float *)Nk, (float *)c->dia_add,
+                                                 c->stream);
+        if (e != hipSuccess)
+            return fail(RPL_ERR_HIP, std::string("listed quadruplets: ") + hipGetErrorString(e));
+        return record_end(c);
+    }
     if (!c->dia)
         return RPL_OK;
     int rc = record_begin(c, 4);
@@ -970,7 +995,8 @@ int rpl_destroy(rpl_ctx *c)
                     (void *)c->out_scale, (void *)c->s_err, (void *)c->use_plain, (void *)c->s_counts,
                     (void *)c->bin_hi, (void *)c->s_amax, (void *)c->s_flags, c->dUx, c->dUy, c->pm_O,
                     c->pm_dO, c->pm_kx, c->pm_ky, c->gse_cg, c->gse_dcg, c->dia_flags, c->dia_bins, c->dia_wef,
-                    c->dia_wdiv, c->dia_efa, c->dia_inv_efa, c->dia_cfa, c->dia_add})
+                    c->dia_wdiv, c->dia_efa, c->dia_inv_efa, c->dia_cfa, c->dia_add, c->listed_bins,
+                    c->listed_rates, c->listed_w})
         free_ptr(p);
     for (auto &ev : c->events) {
         (void)hipEventDestroy(ev.first);
@@ -1055,6 +1081,7 @@ int rpl_set_geometry(rpl_ctx *c, double dx, double dy, const double *kappa_c, co
     c->h_muy = my;
     c->has_sources = false;  // source tables depend on the geometry
     c->dia = false;
+    c->listed = false;
     if (c->pm) {             // so do the pseudomomentum tables
         c->pm = false;
         c->has_current = false;
@@ -1867,6 +1894,8 @@ int rpl_set_sources(rpl_ctx *c, const rpl_sources *src, const double *x_nodes, c
     if (s.nonlinear != RPL_NONLINEAR_NONE && s.nonlinear != RPL_NONLINEAR_SYMMETRIC_QUADRUPLET)
         return fail(RPL_ERR_ARG, "unsupported nonlinear term");
     if (s.nonlinear == RPL_NONLINEAR_SYMMETRIC_QUADRUPLET) {
+        if (c->listed)
+            return fail(RPL_ERR_STATE, "listed quadruplets are set: one nonlinear term at a time");
         if (c->cfg.nkappa < 2)
             return fail(RPL_ERR_ARG, "the symmetric quadruplet needs at least two frequency bins");
         if (!(s.dia_frequency_offset > 0 && s.dia_frequency_offset < 1))
@@ -2034,6 +2063,85 @@ int rpl_set_sources(rpl_ctx *c, const rpl_sources *src, const double *x_nodes, c
     c->src = s;
     c->has_sources = s.wind_input != RPL_WIND_INPUT_NONE || s.dissipation != RPL_DISSIPATION_NONE;
     RPL_HIP(hipStreamSynchronize(c->stream));
+    return RPL_OK;
+}
+
+int rpl_set_quadruplets(rpl_ctx *c, int32_t n, const int32_t *bins, const double *rates, const double *frequency,
+                        double power, double resonance_tolerance)
+{
+    if (!c)
+        return fail(RPL_ERR_ARG, "null context");
+    if (!c->has_geometry)
+        return fail(RPL_ERR_STATE, "rpl_set_geometry has not been called");
+    if (n <= 0) {
+        c->listed = false;
+        return RPL_OK;
+    }
+    if (!bins || !rates || !frequency)
+        return fail(RPL_ERR_ARG, "null argument");
+    if (c->dia)
+        return fail(RPL_ERR_STATE, "the symmetric quadruplet is set: one nonlinear term at a time");
+    if (!(power > 0))
+        return fail(RPL_ERR_ARG, "DIA transfer power must be positive");  // Sources.jl:1334
+    if (!(resonance_tolerance >= 0))
+        return fail(RPL_ERR_ARG, "quadruplet resonance tolerance must be nonnegative");  // Sources.jl:1297
+    const int nk = c->cfg.nkappa, np = c->cfg.nphi;
+    auto freq = [&](int m) { return frequency[m]; };  // diagnostics/moments.jl:15
+    for (int q = 0; q < n; ++q) {
+        const int32_t *b = bins + 8 * (size_t)q;
+        for (int k = 0; k < 8; k += 2)
+            if (b[k] < 0 || b[k] >= nk || b[k + 1] < 0 || b[k + 1] >= np)
+                return fail(RPL_ERR_ARG, "quadruplet transfer index outside the spectral grid");  // Sources.jl:1274-1285
+        // Sources.jl:1287-1292: the donor and receiver bin pairs differ.
+        std::pair<int, int> d1{b[0], b[1]}, d2{b[2], b[3]}, r1{b[4], b[5]}, r2{b[6], b[7]};
+        if (std::minmax(d1, d2) == std::minmax(r1, r2))
+            return fail(RPL_ERR_ARG, "quadruplet transfer donor and receiver bin pairs must differ");
+        // Sources.jl:1296-1329: resonance in frequency and wavevector.
+        const double fd = freq(b[0]) + freq(b[2]), fr = freq(b[4]) + freq(b[6]);
+        const double fscale = std::max({std::fabs(fd), std::fabs(fr), 2.220446049250313e-16});
+        if (!(std::fabs(fr - fd) <= resonance_tolerance * fscale))
+            return fail(RPL_ERR_ARG, "quadruplet transfer requires receiver frequencies to match donor frequencies within tolerance");
+        double kx[4], ky[4];
+        for (int k = 0; k < 4; ++k) {
+            kx[k] = c->h_kc[b[2 * k]] * std::cos(c->h_pc[b[2 * k + 1]]);
+            ky[k] = c->h_kc[b[2 * k]] * std::sin(c->h_pc[b[2 * k + 1]]);
+        }
+        const double dkx = kx[0] + kx[1], dky = ky[0] + ky[1], rkx = kx[2] + kx[3], rky = ky[2] + ky[3];
+        double vscale = std::max(std::hypot(dkx, dky), std::hypot(rkx, rky));
+        for (int k = 0; k < 4; ++k)
+            vscale = std::max({vscale, std::fabs(kx[k]), std::fabs(ky[k])});
+        vscale = std::max(vscale, 2.220446049250313e-16);
+        if (!(std::hypot(rkx - dkx, rky - dky) <= resonance_tolerance * vscale))
+            return fail(RPL_ERR_ARG, "quadruplet transfer requires receiver wavevectors to match donor wavevectors within tolerance");
+        if (!(rates[q] >= 0))
+            return fail(RPL_ERR_ARG, "DIA transfer rates must be nonnegative");  // Sources.jl:1347
+    }
+    const size_t tab = (size_t)nk * np;
+    std::vector<double> w(tab);
+    for (int pn = 0; pn < np; ++pn)
+        for (int m = 0; m < nk; ++m) {
+            const double k1 = c->h_kf[m], k2 = c->h_kf[m + 1];
+            w[m + (size_t)nk * pn] = ((k2 * k2 - k1 * k1) / 2) * (c->h_pf[pn + 1] - c->h_pf[pn]);
+            if (!(w[m + (size_t)nk * pn] > 0))
+                return fail(RPL_ERR_ARG, "DIA transfer requires positive spectral cell measures");
+        }
+    RPL_HIP(hipSetDevice(c->device));
+    free_ptr(c->listed_bins);
+    free_ptr(c->listed_rates);
+    c->listed_bins = c->listed_rates = nullptr;
+    int rc;
+    if ((rc = alloc(&c->listed_bins, 8 * (size_t)n * sizeof(int32_t))) ||
+        (rc = alloc(&c->listed_rates, (size_t)n * c->esize)) ||
+        (!c->listed_w && (rc = alloc(&c->listed_w, tab * c->esize))) ||
+        (!c->dia_add && (rc = alloc(&c->dia_add, c->cnt * c->esize))))
+        return rc;
+    RPL_HIP(hipMemcpy(c->listed_bins, bins, 8 * (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice));
+    if ((rc = upload_real(c, rates, c->listed_rates, n)) || (rc = upload_real(c, w.data(), c->listed_w, tab)))
+        return rc;
+    RPL_HIP(hipStreamSynchronize(c->stream));
+    c->listed_n = n;
+    c->listed_power = power;
+    c->listed = true;
     return RPL_OK;
 }
 
@@ -2313,7 +2421,7 @@ int rpl_ensemble_create(rpl_ensemble **out, rpl_ctx *const *members, int32_t cou
             return fail(RPL_ERR_UNSUPPORTED, "ensemble members take prescribed currents only (no pseudomomentum coupling)");
         if (c->gse)
             return fail(RPL_ERR_UNSUPPORTED, "ensemble members run without propagation smoothing");
-        if (c->dia)
+        if (c->dia || c->listed)
             return fail(RPL_ERR_UNSUPPORTED, "ensemble members run without the nonlinear (DIA) term");
         if (c->device != first->device)
             return fail(RPL_ERR_ARG, "ensemble members must live on one device");

@@ -15,7 +15,8 @@ from .model import (SpectralWaveModel, InfiniteDepth, Clock, Simulation, WENO, s
                     compute_pseudomomentum_cell_averages, compute_pseudomomentum_cell_integrals,
                     pseudomomentum_fields, compute_pseudomomentum_tendency_cell_averages,
                     pseudomomentum_tendency_fields, cwcm_momentum_tendency_fields)
-from .sources import (SymmetricQuadruplet, DiscreteInteractionApproximation,
+from .sources import (SymmetricQuadruplet, DiscreteInteractionApproximation, QuadrupletTransferInteraction,
+                      ListedQuadruplets,
                       BulkWindDrag, LinearStormTrack, HollandHurricaneWind, PressureCorrelationInput,
                       MeanSpectrumWhitecapping, LocalSaturationDissipation, WhitecappingDissipation,
                       PrecomputedSources, wind_state)
@@ -37,7 +38,7 @@ __all__ = [
     "compute_pseudomomentum_tendency_cell_averages", "pseudomomentum_tendency_fields", "cwcm_momentum_tendency_fields",
     "m0", "root_mean_square_wavenumber", "mean_direction", "significant_wave_height",
     "total_action", "total_deep_water_energy",
-    "SymmetricQuadruplet", "DiscreteInteractionApproximation",
+    "SymmetricQuadruplet", "DiscreteInteractionApproximation", "QuadrupletTransferInteraction", "ListedQuadruplets",
     "BulkWindDrag", "LinearStormTrack", "HollandHurricaneWind", "PressureCorrelationInput",
     "MeanSpectrumWhitecapping", "LocalSaturationDissipation", "WhitecappingDissipation", "PrecomputedSources",
     "wind_state",

@@ -145,14 +145,40 @@ class SymmetricQuadruplet:
         self.direction_offset_minus = math.acos(min(1.0, max(-1.0, (b * b - a * a + 4) / (4 * b))))
 
 
-class DiscreteInteractionApproximation:
-    """`DiscreteInteractionApproximation(method)` (Sources/Sources.jl:419-428)
-    with a SymmetricQuadruplet method, the nonlinear four-wave transfer."""
+class QuadrupletTransferInteraction:
+    """`QuadrupletTransferInteraction(donor1, donor2, receiver1, receiver2;
+    rate)` (Sources/Sources.jl:378-402): an explicit quadruplet, each bin a
+    1-based (frequency m, direction n) pair as in the reference, with a
+    constant rate."""
 
-    def __init__(self, method=None):
-        self.method = method if method is not None else SymmetricQuadruplet()
-        if not isinstance(self.method, SymmetricQuadruplet):
-            raise NotImplementedError("only the SymmetricQuadruplet method is built")
+    def __init__(self, donor1, donor2, receiver1, receiver2, rate=0.0):
+        self.bins = tuple((int(b[0]), int(b[1])) for b in (donor1, donor2, receiver1, receiver2))
+        if callable(rate) or np.ndim(rate) != 0:
+            raise NotImplementedError("quadruplet transfer rates are constants here")
+        self.rate = float(rate)
+
+
+class ListedQuadruplets:
+    """Sources.jl:413-417: explicit interactions with a transfer power and a
+    resonance tolerance."""
+
+    def __init__(self, interactions=(), power=1.0, resonance_tolerance=1e-12):
+        self.interactions = tuple(interactions)
+        self.power, self.resonance_tolerance = float(power), float(resonance_tolerance)
+
+
+class DiscreteInteractionApproximation:
+    """`DiscreteInteractionApproximation(method)` or
+    `DiscreteInteractionApproximation(interactions; power, resonance_tolerance)`
+    (Sources/Sources.jl:419-428): the nonlinear four-wave transfer with a
+    SymmetricQuadruplet method or, as the reference's default, a list of
+    QuadrupletTransferInteractions."""
+
+    def __init__(self, method=(), power=1.0, resonance_tolerance=1e-12):
+        if isinstance(method, (SymmetricQuadruplet, ListedQuadruplets)):
+            self.method = method
+        else:
+            self.method = ListedQuadruplets(method, power, resonance_tolerance)
 
 
 class PrecomputedSources:
@@ -187,6 +213,10 @@ def to_struct(sources):
         if not isinstance(nl, DiscreteInteractionApproximation):
             raise TypeError("the nonlinear member must be a DiscreteInteractionApproximation")
         q = nl.method
+        if isinstance(q, ListedQuadruplets):
+            if isinstance(sources, PrecomputedSources):
+                raise NotImplementedError("listed quadruplets run as a source term of their own, not a bundle member")
+            return s  # installed by attach through rpl_set_quadruplets
         s.nonlinear = 1
         s.dia_coefficient, s.dia_frequency_offset, s.dia_gravity = q.coefficient, q.frequency_offset, q.gravity
     if isinstance(diss, MeanSpectrumWhitecapping) and not isinstance(sources, PrecomputedSources):
@@ -230,9 +260,11 @@ def attach(model, sources):
     from .grids import FrequencyDirectionGrid
     lib = _lib.load()
     s = to_struct(sources)
-    if s.nonlinear and not isinstance(model.spectral_grid, FrequencyDirectionGrid):
-        # symmetric_quadruplet.jl:177: the transfer is defined on frequency-direction grids.
-        raise ValueError("the symmetric-quadruplet DIA needs a FrequencyDirectionGrid")
+    listed = sources.method if isinstance(sources, DiscreteInteractionApproximation) and \
+        isinstance(sources.method, ListedQuadruplets) else None
+    if (s.nonlinear or listed) and not isinstance(model.spectral_grid, FrequencyDirectionGrid):
+        # symmetric_quadruplet.jl:177, Sources.jl:1275: the transfer is defined on frequency-direction grids.
+        raise ValueError("DiscreteInteractionApproximation requires a FrequencyDirectionGrid")
     xn = np.ascontiguousarray(xnodes(model.grid), dtype=np.float64)
     yn = np.ascontiguousarray(ynodes(model.grid), dtype=np.float64)
     _lib.check(lib.rpl_set_sources(model._ctx.ptr, ctypes.byref(s), _lib.dptr(xn), _lib.dptr(yn)))
@@ -245,6 +277,15 @@ def attach(model, sources):
         _lib.check(lib.rpl_set_storm_track(model._ctx.ptr, len(t), _lib.dptr(t), _lib.dptr(cx), _lib.dptr(cy)))
     else:
         _lib.check(lib.rpl_set_storm_track(model._ctx.ptr, 0, None, None, None))
+    if listed is not None and listed.interactions:
+        bins = np.ascontiguousarray([[v - 1 for b in q.bins for v in b] for q in listed.interactions], dtype=np.int32)
+        rates = np.ascontiguousarray([q.rate for q in listed.interactions], dtype=np.float64)
+        freq = np.ascontiguousarray(model.spectral_grid.frequency, dtype=np.float64)
+        _lib.check(lib.rpl_set_quadruplets(model._ctx.ptr, len(rates),
+                                           bins.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), _lib.dptr(rates),
+                                           _lib.dptr(freq), listed.power, listed.resonance_tolerance))
+    else:
+        _lib.check(lib.rpl_set_quadruplets(model._ctx.ptr, 0, None, None, None, 1.0, 0.0))
     return s
 
 
