@@ -25,7 +25,7 @@ using Libdl
 
 export MI355X, MI355XWaveModel, set!, time_step!, interior_action, doppler_velocity,
        PseudomomentumVelocities, SpatialAveraging, update_coupling!, doppler_velocity_kappa_derivative,
-       compute_pseudomomentum_cell_averages, compute_pseudomomentum_cell_integrals,
+       compute_pseudomomentum_cell_averages, compute_pseudomomentum_cell_integrals, set_quadruplets!,
        m0, root_mean_square_wavenumber, mean_direction, significant_wave_height,
        total_action, total_deep_water_energy, Simulation, run!
 
@@ -228,6 +228,32 @@ function _pseudomomentum(model::MI355XWaveModel, z_faces, depth, average)
 end
 compute_pseudomomentum_cell_averages(model, z_faces, depth) = _pseudomomentum(model, z_faces, depth, true)
 compute_pseudomomentum_cell_integrals(model, z_faces, depth) = _pseudomomentum(model, z_faces, depth, false)
+
+"""
+    set_quadruplets!(model, interactions, frequency; power=1.0, resonance_tolerance=1e-12)
+
+Installs `DiscreteInteractionApproximation(interactions; power,
+resonance_tolerance)` with the `ListedQuadruplets` method
+(Sources/Sources.jl:378-428, 1268-1385).  `interactions` are objects with
+Ripple's `QuadrupletTransferInteraction` fields (1-based `donor1_m`, …,
+`receiver2_n`, constant `rate`); `frequency` is the grid's frequency
+centres.  The library runs the reference's index, resonance and rate checks
+and throws on the first failure.
+"""
+function set_quadruplets!(model::MI355XWaveModel, interactions, frequency; power=1.0, resonance_tolerance=1e-12)
+    bins = Int32[]
+    rates = Float64[]
+    for q in interactions
+        append!(bins, Int32.((q.donor1_m, q.donor1_n, q.donor2_m, q.donor2_n,
+                              q.receiver1_m, q.receiver1_n, q.receiver2_m, q.receiver2_n)) .- Int32(1))
+        push!(rates, Float64(q.rate))
+    end
+    check(ccall(sym(:rpl_set_quadruplets), Cint,
+                (Ptr{Cvoid}, Int32, Ptr{Int32}, Ptr{Cdouble}, Ptr{Cdouble}, Cdouble, Cdouble),
+                model.ctx, length(rates), bins, rates, collect(Float64, frequency), Float64(power),
+                Float64(resonance_tolerance)))
+    return model
+end
 
 function _columns(model, which)
     Nx, Ny = model.size[1], model.size[2]
