@@ -2125,7 +2125,10 @@ int rpl_set_quadruplets(rpl_ctx *c, int32_t n, const int32_t *bins, const double
             if (!(w[m + (size_t)nk * pn] > 0))
                 return fail(RPL_ERR_ARG, "DIA transfer requires positive spectral cell measures");
         }
+    if ((size_t)nk * np > 65535)
+        return fail(RPL_ERR_UNSUPPORTED, "listed quadruplets support at most 65535 spectral bins");
     RPL_HIP(hipSetDevice(c->device));
+    c->listed = false;  // until the new list is on the device
     free_ptr(c->listed_bins);
     free_ptr(c->listed_rates);
     c->listed_bins = c->listed_rates = nullptr;

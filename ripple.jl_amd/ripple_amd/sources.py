@@ -267,6 +267,7 @@ def attach(model, sources):
         raise ValueError("DiscreteInteractionApproximation requires a FrequencyDirectionGrid")
     xn = np.ascontiguousarray(xnodes(model.grid), dtype=np.float64)
     yn = np.ascontiguousarray(ynodes(model.grid), dtype=np.float64)
+    _lib.check(lib.rpl_set_quadruplets(model._ctx.ptr, 0, None, None, None, 1.0, 0.0))  # one nonlinear term at a time
     _lib.check(lib.rpl_set_sources(model._ctx.ptr, ctypes.byref(s), _lib.dptr(xn), _lib.dptr(yn)))
     wi = sources.wind_input if isinstance(sources, PrecomputedSources) else sources
     track = getattr(getattr(wi, "wind", None), "track", None)
@@ -284,8 +285,6 @@ def attach(model, sources):
         _lib.check(lib.rpl_set_quadruplets(model._ctx.ptr, len(rates),
                                            bins.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), _lib.dptr(rates),
                                            _lib.dptr(freq), listed.power, listed.resonance_tolerance))
-    else:
-        _lib.check(lib.rpl_set_quadruplets(model._ctx.ptr, 0, None, None, None, 1.0, 0.0))
     return s
 
 
