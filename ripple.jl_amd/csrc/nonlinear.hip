@@ -38,78 +38,58 @@ __global__ __launch_bounds__(256) void dia_source_kernel(size_t plane, int nbins
 {
     extern __shared__ unsigned char dia_lds[];
     T *Nl = reinterpret_cast<T *>(dia_lds);
-    T *Tl = Nl + (size_t)nbins * cols;
+    T *TEl = Nl + (size_t)nbins * cols;  // [2 nbins][column]
     const size_t ij0 = (size_t)dia_xcd_order(blockIdx.x, tiles) * cols;
     const int ncol = (int)min((size_t)cols, plane - ij0);
     const int items = nbins * cols;
-    // The columns' action, bin-major ([bin][column]), and a zero transfer.
+    // The columns' action, bin-major ([bin][column]).
     for (int e = threadIdx.x; e < items; e += 256) {
         const int bin = e / cols, cc = e - bin * cols;
         Nl[e] = cc < ncol ? N[ij0 + cc + plane * (size_t)bin] : T(0);
-        Tl[e] = T(0);
     }
     __syncthreads();
-    // symmetric_quadruplet.jl:92-173 for every (donor, column) pair.  Both
-    // orientations gather first and scatter after: an LDS atomic may alias a
-    // later LDS read, so interleaving them would serialize every access.
+    // symmetric_quadruplet.jl:92-173 for every (donor, column) pair: the
+    // transfer of both orientations, gathered from the receivers' corners.
     for (int e = threadIdx.x; e < items; e += 256) {
         const int d = e / cols, cc = e - d * cols;
         const T Na = Nl[e];
-        if (!(Na > T(0)))
-            continue;
-        const T Ea = tb.ef_a[d] * Na;
-        int fl[2];
-        T TE[2];
+        T TE[2] = {T(0), T(0)};
+        if (Na > T(0)) {
+            const T Ea = tb.ef_a[d] * Na;
 #pragma unroll
-        for (int o = 0; o < 2; ++o) {
-            const int q = 2 * d + o;
-            fl[o] = tb.flags[q];
-            const int *bp = tb.bins + 8 * (size_t)q;
-            const T *wef = tb.wef + 8 * (size_t)q;
-            T Ep = T(0), Em = T(0);
-            if (fl[o] & 1)
-                Ep = wef[0] * fmax(Nl[bp[0] * cols + cc], T(0)) + wef[1] * fmax(Nl[bp[1] * cols + cc], T(0)) +
-                     wef[2] * fmax(Nl[bp[2] * cols + cc], T(0)) + wef[3] * fmax(Nl[bp[3] * cols + cc], T(0));
-            if (fl[o] & 2)
-                Em = wef[4] * fmax(Nl[bp[4] * cols + cc], T(0)) + wef[5] * fmax(Nl[bp[5] * cols + cc], T(0)) +
-                     wef[6] * fmax(Nl[bp[6] * cols + cc], T(0)) + wef[7] * fmax(Nl[bp[7] * cols + cc], T(0));
-            const T sa = Ea * (Ep * tb.dal1 + Em * tb.dal2) - Ep * Em * tb.dal3;
-            TE[o] = tb.cf_a[d] * sa * T(0.5);
-        }
-        // Action changes: the reference divides by the energy factors; their
-        // reciprocals (and w / E) are tabulated (rounding-level difference,
-        // no division in the loop).
-        const T inv_efa = tb.inv_ef_a[d];
-        T own = T(0);
-#pragma unroll
-        for (int o = 0; o < 2; ++o) {
-            if (!fl[o])
-                continue;
-            const T donor = fl[o] == 3 ? T(2) : T(1);
-            own = own - donor * TE[o] * inv_efa;
-            const int q = 2 * d + o;
-            const int *bp = tb.bins + 8 * (size_t)q;
-            const T *wd = tb.wdiv + 8 * (size_t)q;
-            if (fl[o] & 1) {
-#pragma unroll
-                for (int k = 0; k < 4; ++k)
-                    atomicAdd(&Tl[bp[k] * cols + cc], TE[o] * wd[k]);
-            }
-            if (fl[o] & 2) {
-#pragma unroll
-                for (int k = 4; k < 8; ++k)
-                    atomicAdd(&Tl[bp[k] * cols + cc], TE[o] * wd[k]);
+            for (int o = 0; o < 2; ++o) {
+                const int q = 2 * d + o;
+                const int fl = tb.flags[q];
+                const int *bp = tb.bins + 8 * (size_t)q;
+                const T *wef = tb.wef + 8 * (size_t)q;
+                T Ep = T(0), Em = T(0);
+                if (fl & 1)
+                    Ep = wef[0] * fmax(Nl[bp[0] * cols + cc], T(0)) + wef[1] * fmax(Nl[bp[1] * cols + cc], T(0)) +
+                         wef[2] * fmax(Nl[bp[2] * cols + cc], T(0)) + wef[3] * fmax(Nl[bp[3] * cols + cc], T(0));
+                if (fl & 2)
+                    Em = wef[4] * fmax(Nl[bp[4] * cols + cc], T(0)) + wef[5] * fmax(Nl[bp[5] * cols + cc], T(0)) +
+                         wef[6] * fmax(Nl[bp[6] * cols + cc], T(0)) + wef[7] * fmax(Nl[bp[7] * cols + cc], T(0));
+                const T sa = Ea * (Ep * tb.dal1 + Em * tb.dal2) - Ep * Em * tb.dal3;
+                TE[o] = fl ? tb.cf_a[d] * sa * T(0.5) : T(0);
             }
         }
-        if (fl[0] | fl[1])
-            atomicAdd(&Tl[e], own);
+        TEl[(2 * d) * cols + cc] = TE[0];
+        TEl[(2 * d + 1) * cols + cc] = TE[1];
     }
     __syncthreads();
+    // Each bin gathers what its donors send it (the scatter transposed on the
+    // host; no atomics), then the split of symmetric_quadruplet.jl:210-222.
+    // The action changes use tabulated w / E and 1 / E_a instead of the
+    // reference's divisions (rounding-level difference).
     for (int e = threadIdx.x; e < items; e += 256) {
         const int bin = e / cols, cc = e - bin * cols;
         if (cc >= ncol)
             continue;
-        const T t = Tl[e], a = Nl[e];
+        T t = T(0);
+        const int c1 = tb.gat_off[bin + 1];
+        for (int c = tb.gat_off[bin]; c < c1; ++c)
+            t = t + TEl[tb.gat_q[c] * cols + cc] * tb.gat_w[c];
+        const T a = Nl[e];
         T src;
         if (t >= T(0)) {
             src = t;
@@ -169,8 +149,8 @@ __global__ __launch_bounds__(256) void listed_source_kernel(size_t plane, int nk
 template <typename T>
 int dia_columns_per_workgroup(int nbins)
 {
-    const size_t per_col = 2 * (size_t)nbins * sizeof(T);
-    const size_t cols = (48 * 1024) / per_col;
+    const size_t per_col = 3 * (size_t)nbins * sizeof(T);  // action and two transfers
+    const size_t cols = (64 * 1024) / per_col;
     return (int)std::min<size_t>(cols, 8);
 }
 
@@ -186,7 +166,7 @@ hipError_t launch_dia_source(int nx, int ny, int nk, int nphi, const T *N, const
     const long long tiles = (long long)((plane + cols - 1) / cols);
     if (tiles > 0x7fffffffLL)
         return hipErrorInvalidValue;
-    const size_t lds = 2 * (size_t)nbins * cols * sizeof(T);
+    const size_t lds = 3 * (size_t)nbins * cols * sizeof(T);
     hipLaunchKernelGGL(dia_source_kernel<T>, dim3((unsigned)tiles), dim3(256), lds, s, plane, nbins, cols, tiles, N,
                        tb, out);
     return hipGetLastError();

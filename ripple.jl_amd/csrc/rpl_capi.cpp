@@ -64,7 +64,8 @@ struct rpl_ctx {
     // per-stage source plane the stage kernels add (StageParams::src_add)
     bool dia = false;
     void *dia_flags = nullptr, *dia_bins = nullptr, *dia_wef = nullptr, *dia_wdiv = nullptr, *dia_efa = nullptr,
-         *dia_inv_efa = nullptr, *dia_cfa = nullptr, *dia_add = nullptr;
+         *dia_inv_efa = nullptr, *dia_cfa = nullptr, *dia_add = nullptr, *dia_gat_off = nullptr,
+         *dia_gat_q = nullptr, *dia_gat_w = nullptr;
     double dia_dal[3] = {0, 0, 0};
     // ListedQuadruplets (rpl_set_quadruplets)
     bool listed = false;
@@ -391,7 +392,8 @@ int launch_dia_t(rpl_ctx *c, const T *Nk)
     rpl::DiaTables<T> tb{(const int *)c->dia_flags, (const int *)c->dia_bins,  (const T *)c->dia_wef,
                          (const T *)c->dia_wdiv,       (const T *)c->dia_efa,  (const T *)c->dia_inv_efa,
                          (const T *)c->dia_cfa,        (T)c->dia_dal[0],       (T)c->dia_dal[1],
-                         (T)c->dia_dal[2]};
+                         (T)c->dia_dal[2],             (const int *)c->dia_gat_off, (const int *)c->dia_gat_q,
+                         (const T *)c->dia_gat_w};
     hipError_t e = rpl::launch_dia_source<T>(c->cfg.nx, c->cfg.ny, c->cfg.nkappa, c->cfg.nphi, Nk, tb,
                                              (T *)c->dia_add, c->stream);
     if (e != hipSuccess)
@@ -995,7 +997,7 @@ int rpl_destroy(rpl_ctx *c)
                     (void *)c->out_scale, (void *)c->s_err, (void *)c->use_plain, (void *)c->s_counts,
                     (void *)c->bin_hi, (void *)c->s_amax, (void *)c->s_flags, c->dUx, c->dUy, c->pm_O,
                     c->pm_dO, c->pm_kx, c->pm_ky, c->gse_cg, c->gse_dcg, c->dia_flags, c->dia_bins, c->dia_wef,
-                    c->dia_wdiv, c->dia_efa, c->dia_inv_efa, c->dia_cfa, c->dia_add, c->listed_bins,
+                    c->dia_wdiv, c->dia_efa, c->dia_inv_efa, c->dia_cfa, c->dia_add, c->dia_gat_off, c->dia_gat_q, c->dia_gat_w, c->listed_bins,
                     c->listed_rates, c->listed_w})
         free_ptr(p);
     for (auto &ev : c->events) {
@@ -1844,6 +1846,41 @@ static int build_dia_tables(rpl_ctx *c, const rpl_sources &s)
         (rc = upload_real(c, wdiv.data(), c->dia_wdiv, wdiv.size())) || (rc = upload_real(c, efa.data(), c->dia_efa, nb)) ||
         (rc = upload_real(c, inv_efa.data(), c->dia_inv_efa, nb)) || (rc = upload_real(c, cfa.data(), c->dia_cfa, nb)))
         return rc;
+    // The scatter transposed, per receiving bin: every receiver corner and
+    // the donor's own loss (twice when both receivers are in range).
+    std::vector<std::vector<std::pair<int, double>>> into(nb);
+    for (int d = 0; d < nb; ++d)
+        for (int o = 0; o < 2; ++o) {
+            const int q = 2 * d + o, fl = flags[q];
+            if (!fl)
+                continue;
+            for (int k = (fl & 1) ? 0 : 4; k < ((fl & 2) ? 8 : 4); ++k)
+                into[bins[8 * (size_t)q + k]].push_back({q, wdiv[8 * (size_t)q + k]});
+            into[d].push_back({q, -(fl == 3 ? 2.0 : 1.0) * inv_efa[d]});
+        }
+    std::vector<int> gat_off(nb + 1, 0), gat_q;
+    std::vector<double> gat_w;
+    for (int r = 0; r < nb; ++r) {
+        for (auto &pr : into[r]) {
+            gat_q.push_back(pr.first);
+            gat_w.push_back(pr.second);
+        }
+        gat_off[r + 1] = (int)gat_q.size();
+    }
+    for (void **p : {&c->dia_gat_off, &c->dia_gat_q, &c->dia_gat_w}) {
+        free_ptr(*p);
+        *p = nullptr;
+    }
+    const size_t nnz = std::max<size_t>(gat_q.size(), 1);
+    if ((rc = alloc(&c->dia_gat_off, gat_off.size() * sizeof(int))) || (rc = alloc(&c->dia_gat_q, nnz * sizeof(int))) ||
+        (rc = alloc(&c->dia_gat_w, nnz * c->esize)))
+        return rc;
+    RPL_HIP(hipMemcpy(c->dia_gat_off, gat_off.data(), gat_off.size() * sizeof(int), hipMemcpyHostToDevice));
+    if (!gat_q.empty()) {
+        RPL_HIP(hipMemcpy(c->dia_gat_q, gat_q.data(), gat_q.size() * sizeof(int), hipMemcpyHostToDevice));
+        if ((rc = upload_real(c, gat_w.data(), c->dia_gat_w, gat_w.size())))
+            return rc;
+    }
     c->dia_dal[0] = 1 / std::pow(1 + lam, 4);
     c->dia_dal[1] = 1 / std::pow(1 - lam, 4);
     c->dia_dal[2] = 2 * c->dia_dal[0] * c->dia_dal[1];

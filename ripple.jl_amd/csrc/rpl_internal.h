@@ -163,6 +163,12 @@ struct DiaTables {
     const T *wef, *wdiv;     // 16 nbins each
     const T *ef_a, *inv_ef_a, *cf_a;  // nbins each
     T dal1, dal2, dal3;
+    // The scatter transposed: bin r's transfer is the sum over
+    // gat_q[gat_off[r] .. gat_off[r + 1]) of TE[q] gat_w (receiver corners
+    // and the donor's own loss), TE[q = 2 d + o] the donor d's transfer of
+    // orientation o.
+    const int *gat_off, *gat_q;
+    const T *gat_w;
 };
 template <typename T>
 int dia_columns_per_workgroup(int nbins);
