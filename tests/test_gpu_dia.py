@@ -1,7 +1,8 @@
 """Device DIA (DiscreteInteractionApproximation{SymmetricQuadruplet},
 Physics/NonlinearInteractions/symmetric_quadruplet.jl) against the FP64
 oracle, alone and as the nonlinear member of the PrecomputedSources bundle.
-The device scatters with LDS atomic adds, whose order is free, so FP64 agrees
+The device gathers each bin's transfer in a host-built list order, with
+tabulated reciprocals in place of the reference's divisions, so FP64 agrees
 to rounding (bound written in each test) rather than bitwise."""
 import math
 import sys
