@@ -97,6 +97,16 @@ def geometry(kc, kf, pc, pf, finite_depth=False, depth=0.0, gravity=9.81):
     return {k: v.reshape((nk, npf), order="F") for k, v in out.items()}
 
 
+def energy_measures(kf, pf, gravity=9.81):
+    """bulk_statistics.jl:23-26: sqrt(g) * the annular measure of kappa^(1/2),
+    (Nk, Nphi)."""
+    kf, pf = _c(kf), _c(pf)
+    nk, npf = len(kf) - 1, len(pf) - 1
+    e = np.zeros(nk * npf)
+    lib().orc_energy_measures(_I(nk), _I(npf), _p(kf), _p(pf), _D(gravity), _p(e))
+    return e.reshape((nk, npf), order="F")
+
+
 def q_cdf(kappa, z, depth, small_mu=1e-6):
     return lib().orc_q_cdf(kappa, z, depth, small_mu)
 

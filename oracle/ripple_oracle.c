@@ -122,6 +122,23 @@ void orc_moment_measures(int nk, int nphi, const double *kf, const double *pf,
         }
 }
 
+/* Diagnostics/bulk_statistics.jl:23-26 (_deep_water_frequency_cell on polar
+ * and frequency-direction grids) = sqrt(g) * spectral_radial_power_measure(
+ * grid, m, n, 1//2), with CoordinateGrids/finite_volume_integration.jl:24-27
+ * (radial_power_annular_measure): (phi2 - phi1) * (k2^(p+2) - k1^(p+2)) /
+ * (p + 2) at p = 1/2.  Julia evaluates 1//2 + 2 as the exact rational 5//2,
+ * so the exponent and divisor are the double 2.5. */
+void orc_energy_measures(int nk, int nphi, const double *kf, const double *pf, double gravity, double *e)
+{
+    const double sg = sqrt(gravity);
+    for (int n = 0; n < nphi; ++n)
+        for (int m = 0; m < nk; ++m) {
+            double k1 = kf[m], k2 = kf[m + 1], p1 = pf[n], p2 = pf[n + 1];
+            double annular = (p2 - p1) * (pow(k2, 2.5) - pow(k1, 2.5)) / 2.5;
+            e[(size_t)m + (size_t)nk * n] = sg * annular;
+        }
+}
+
 /* Models/transport.jl:61-70: deep-water group velocity averaged over the
  * spectral cell; :72-83: finite depth, point-sampled at (kappa_m, phi_n). */
 void orc_group_velocity_tables(int nk, int nphi, const double *kc, const double *kf,

@@ -37,6 +37,10 @@ int rpl::set_error(int code, const char *msg) { return fail(code, msg); }
 struct rpl_ctx {
     rpl_config cfg{};
     int device = 0;
+    // Bumped by every call that changes what a step launches (geometry,
+    // current, sources, storm track, nonlinear terms, smoothing, surrogate),
+    // so an ensemble knows when its captured graph is stale.
+    uint64_t generation = 0;
     hipStream_t stream = nullptr;
     size_t cnt = 0;    // interior cell-bins
     size_t plane = 0;  // nx * ny
@@ -728,6 +732,7 @@ struct rpl_ensemble {
     hipGraphExec_t exec = nullptr;
     double graph_dt = -1;
     int graph_ty = 0, graph_blocks = 0;
+    std::vector<uint64_t> graph_gen;  // the members' generations at capture
 };
 
 namespace {
@@ -892,12 +897,37 @@ int ensemble_stages(rpl_ensemble *e)
     return RPL_OK;
 }
 
+// The rules a context must meet to step in an ensemble with `first`
+// (checked at creation and again before every ensemble step, since a
+// member's configuration may change in between).
+int ensemble_member_check(rpl_ctx *c, rpl_ctx *first)
+{
+    if (c->cfg.dtype != RPL_F32 || !use_fast(c))
+        return fail(RPL_ERR_UNSUPPORTED, "ensemble members must be FP32 contexts on the fast kernel");
+    if (c->cfg.timestepper == RPL_FORWARD_EULER)
+        return fail(RPL_ERR_UNSUPPORTED, "ensemble members must use SSP-RK3");
+    if (c->cfg.transport != first->cfg.transport || c->cfg.transport == RPL_TRANSPORT_NONE)
+        return fail(RPL_ERR_UNSUPPORTED, "ensemble members need the same transport (intrinsic or refraction)");
+    if (c->has_sources != first->has_sources)
+        return fail(RPL_ERR_UNSUPPORTED, "either every member or no member has source terms");
+    if (c->pm)
+        return fail(RPL_ERR_UNSUPPORTED, "ensemble members take prescribed currents only (no pseudomomentum coupling)");
+    if (c->gse)
+        return fail(RPL_ERR_UNSUPPORTED, "ensemble members run without propagation smoothing");
+    if (c->dia || c->listed)
+        return fail(RPL_ERR_UNSUPPORTED, "ensemble members run without the nonlinear (DIA) term");
+    if (c->device != first->device)
+        return fail(RPL_ERR_ARG, "ensemble members must live on one device");
+    return RPL_OK;
+}
+
 void drop_graph(rpl_ensemble *e)
 {
     if (e->exec)
         (void)hipGraphExecDestroy(e->exec);
     e->exec = nullptr;
     e->graph_dt = -1;
+    e->graph_gen.clear();
 }
 
 }  // namespace
@@ -1015,6 +1045,7 @@ int rpl_set_geometry(rpl_ctx *c, double dx, double dy, const double *kappa_c, co
 {
     if (!c || !kappa_c || !kappa_f || !phi_c || !phi_f)
         return fail(RPL_ERR_ARG, "null argument");
+    ++c->generation;
     if (!(dx > 0) || !(dy > 0))
         return fail(RPL_ERR_ARG, "dx and dy must be positive");
     if (!(gravity > 0))
@@ -1184,6 +1215,7 @@ int rpl_set_current(rpl_ctx *c, int32_t nz, const double *z_faces, const double 
 {
     if (!c || !z_faces || !u || !v)
         return fail(RPL_ERR_ARG, "null argument");
+    ++c->generation;
     if (!c->has_geometry)
         return fail(RPL_ERR_STATE, "rpl_set_geometry has not been called");
     if (nz < 1)
@@ -1264,6 +1296,7 @@ int rpl_set_pseudomomentum_coupling(rpl_ctx *c, int32_t nz, const double *z_face
 {
     if (!c || !z_faces)
         return fail(RPL_ERR_ARG, "null argument");
+    ++c->generation;
     if (!c->has_geometry)
         return fail(RPL_ERR_STATE, "rpl_set_geometry has not been called");
     if (c->cfg.transport != RPL_TRANSPORT_REFRACTION)
@@ -1337,6 +1370,7 @@ int rpl_set_propagation_smoothing(rpl_ctx *c, int32_t enable, double alpha_s, do
 {
     if (!c)
         return fail(RPL_ERR_ARG, "null context");
+    ++c->generation;
     if (!c->has_geometry)
         return fail(RPL_ERR_STATE, "rpl_set_geometry has not been called");
     if (!enable) {
@@ -1526,6 +1560,7 @@ int rpl_set_doppler(rpl_ctx *c, const double *Ux, const double *Uy)
 {
     if (!c || !Ux || !Uy)
         return fail(RPL_ERR_ARG, "null argument");
+    ++c->generation;
     if (!c->has_geometry)
         return fail(RPL_ERR_STATE, "rpl_set_geometry has not been called");
     RPL_HIP(hipSetDevice(c->device));
@@ -1893,6 +1928,7 @@ int rpl_set_sources(rpl_ctx *c, const rpl_sources *src, const double *x_nodes, c
 {
     if (!c)
         return fail(RPL_ERR_ARG, "null context");
+    ++c->generation;
     if (!src) {
         c->has_sources = false;
         c->dia = false;
@@ -1939,9 +1975,14 @@ int rpl_set_sources(rpl_ctx *c, const rpl_sources *src, const double *x_nodes, c
             return fail(RPL_ERR_ARG, "frequency_offset must lie in (0, 1)");
         if (!(s.dia_gravity > 0) || !std::isfinite(s.dia_coefficient))
             return fail(RPL_ERR_ARG, "symmetric-quadruplet gravity must be positive and its coefficient finite");
-        const size_t lds = 2 * (size_t)c->cfg.nkappa * c->cfg.nphi * c->esize;
-        if (lds > 48 * 1024)
-            return fail(RPL_ERR_UNSUPPORTED, "the symmetric quadruplet holds a column in 48 KiB of LDS: too many bins");
+        // The DIA workgroup holds a column's action and both orientations'
+        // transfers (3 arrays of nkappa * nphi elements) in 64 KiB of LDS.
+        const int nbins = c->cfg.nkappa * c->cfg.nphi;
+        const int cols = is_f64(c) ? rpl::dia_columns_per_workgroup<double>(nbins)
+                                   : rpl::dia_columns_per_workgroup<float>(nbins);
+        if (cols < 1)
+            return fail(RPL_ERR_UNSUPPORTED, "the symmetric quadruplet holds a column's action and two transfers "
+                                             "(3 * nkappa * nphi elements) in 64 KiB of LDS: too many bins");
     }
     if (s.dissipation == RPL_DISSIPATION_MEAN_SPECTRUM && !(s.p_mean != 0))
         return fail(RPL_ERR_ARG, "mean-spectrum exponent p must be nonzero");
@@ -2108,6 +2149,7 @@ int rpl_set_quadruplets(rpl_ctx *c, int32_t n, const int32_t *bins, const double
 {
     if (!c)
         return fail(RPL_ERR_ARG, "null context");
+    ++c->generation;
     if (!c->has_geometry)
         return fail(RPL_ERR_STATE, "rpl_set_geometry has not been called");
     if (n <= 0) {
@@ -2189,6 +2231,7 @@ int rpl_set_storm_track(rpl_ctx *c, int32_t n, const double *times, const double
 {
     if (!c)
         return fail(RPL_ERR_ARG, "null context");
+    ++c->generation;
     if (n == 0) {
         c->track_t.clear();
         c->track_x.clear();
@@ -2329,6 +2372,7 @@ int rpl_attach_surrogate(rpl_ctx *c, rpl_mlp *net, const rpl_surrogate *cfg, con
 {
     if (!c)
         return fail(RPL_ERR_ARG, "null context");
+    ++c->generation;
     RPL_HIP(hipSetDevice(c->device));
     RPL_HIP(hipStreamSynchronize(c->stream));
     if (!net) {
@@ -2449,22 +2493,8 @@ int rpl_ensemble_create(rpl_ensemble **out, rpl_ctx *const *members, int32_t cou
         for (int q = 0; q < k; ++q)
             if (members[q] == c)
                 return fail(RPL_ERR_ARG, "a context appears twice in the ensemble");
-        if (c->cfg.dtype != RPL_F32 || !use_fast(c))
-            return fail(RPL_ERR_UNSUPPORTED, "ensemble members must be FP32 contexts on the fast kernel");
-        if (c->cfg.timestepper == RPL_FORWARD_EULER)
-            return fail(RPL_ERR_UNSUPPORTED, "ensemble members must use SSP-RK3");
-        if (c->cfg.transport != first->cfg.transport || c->cfg.transport == RPL_TRANSPORT_NONE)
-            return fail(RPL_ERR_UNSUPPORTED, "ensemble members need the same transport (intrinsic or refraction)");
-        if (c->has_sources != first->has_sources)
-            return fail(RPL_ERR_UNSUPPORTED, "either every member or no member has source terms");
-        if (c->pm)
-            return fail(RPL_ERR_UNSUPPORTED, "ensemble members take prescribed currents only (no pseudomomentum coupling)");
-        if (c->gse)
-            return fail(RPL_ERR_UNSUPPORTED, "ensemble members run without propagation smoothing");
-        if (c->dia || c->listed)
-            return fail(RPL_ERR_UNSUPPORTED, "ensemble members run without the nonlinear (DIA) term");
-        if (c->device != first->device)
-            return fail(RPL_ERR_ARG, "ensemble members must live on one device");
+        if ((rc = ensemble_member_check(c, first)))
+            return rc;
     }
     rpl_ensemble *e = new rpl_ensemble();
     e->device = first->device;
@@ -2573,15 +2603,18 @@ int rpl_ensemble_step(rpl_ensemble *e, double dt, int32_t nsteps)
     RPL_HIP(hipSetDevice(e->device));
     int rc;
     for (rpl_ctx *c : e->members)
-        if ((rc = check_ready(c)))
+        if ((rc = check_ready(c)) || (rc = ensemble_member_check(c, e->members[0])))
             return rc;
     // Parameters may change between calls (sources, surrogate attachment);
     // the descriptors are small, so they are rebuilt on every call.
     RPL_HIP(hipStreamSynchronize(e->stream));
     if ((rc = ensemble_descriptors(e, dt)))
         return rc;
+    std::vector<uint64_t> gen;
+    for (rpl_ctx *c : e->members)
+        gen.push_back(c->generation);
     if (e->use_graph && (e->exec == nullptr || e->graph_dt != dt || e->graph_ty != e->ty ||
-                         e->graph_blocks != e->blocks)) {
+                         e->graph_blocks != e->blocks || e->graph_gen != gen)) {
         drop_graph(e);
         hipGraph_t g = nullptr;
         RPL_HIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
@@ -2600,6 +2633,7 @@ int rpl_ensemble_step(rpl_ensemble *e, double dt, int32_t nsteps)
         e->graph_dt = dt;
         e->graph_ty = e->ty;
         e->graph_blocks = e->blocks;
+        e->graph_gen = gen;
     }
     for (int s = 0; s < nsteps; ++s) {
         for (rpl_ctx *c : e->members)

@@ -2,6 +2,7 @@
 bulk_statistics.jl).  Each returns host numpy arrays of shape (Nx, Ny) or a
 float for the global integrals."""
 import ctypes
+import math
 
 import numpy as np
 
@@ -48,5 +49,9 @@ def total_action(model):
 
 
 def total_deep_water_energy(model, gravity=9.81):
-    """sum_ij dx dy sum_mn N sqrt(g) * (radial power 1/2 measure) (bulk_statistics.jl:93-99)."""
-    return _totals(model)[1]
+    """sum_ij dx dy sum_mn N sqrt(g) * (radial power 1/2 measure) (bulk_statistics.jl:93-99).
+
+    The device measure carries the model's gravity; the measure is linear in
+    sqrt(g), so another `gravity` rescales the compensated total."""
+    e = _totals(model)[1]
+    return e if float(gravity) == model.gravity else e * math.sqrt(float(gravity) / model.gravity)

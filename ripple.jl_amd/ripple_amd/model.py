@@ -144,6 +144,7 @@ class SpectralWaveModel:
         self.sources = sources
         self.timestepper = timestepper
         self.clock = clock if clock is not None else Clock()
+        self.gravity = float(gravity)
         self.dtype = np.dtype(np.float64 if grid.float_type == np.float64 or
                               spectral_grid.float_type == np.float64 else np.float32)
         Nk, Np = spectral_grid.shape

@@ -130,3 +130,34 @@ def test_dia_error_paths():
     with pytest.raises(ra.RippleArgumentError):
         device(workload(ra.DiscreteInteractionApproximation(ra.SymmetricQuadruplet(frequency_offset=1.5))),
                np.float64)
+
+
+@pytest.mark.parametrize("float_type,nfreq,ok", [(np.float64, 45, True), (np.float64, 46, False),
+                                                 (np.float32, 91, True), (np.float32, 92, False)])
+def test_lds_capacity_boundary(float_type, nfreq, ok):
+    """A DIA workgroup holds a column's action and two transfers, 3 * nbins
+    elements, in 64 KiB of LDS: 60 directions allow 45 frequencies in FP64
+    (64800 B) but not 46 (66240 B), 91 in FP32 but not 92.  Accepted grids
+    step; rejected ones fail when the sources are set, not at every step."""
+    say(f"dia lds boundary {np.dtype(float_type).name} {nfreq}x60")
+    wl = workload(DIA, nx=3, ny=3, nfreq=nfreq, nphi=60)
+    if ok:
+        m = device(wl, float_type)
+        ra.time_step_(m, 1.0)
+        assert np.all(np.isfinite(m.action))
+    else:
+        with pytest.raises(ra.RippleError):
+            device(wl, float_type)
+
+
+@pytest.mark.parametrize("float_type,tol", [(np.float64, 1e-13), (np.float32, 2e-4)])
+def test_partial_last_tile(oracle, float_type, tol):
+    """13 x 7 = 91 columns leave a partial last workgroup of columns in both
+    precisions (4 columns per workgroup in FP64, 8 in FP32 at 24 x 24 bins)."""
+    say(f"dia partial tile {np.dtype(float_type).name}")
+    wl = workload(DIA, nx=13, ny=7)
+    m = device(wl, float_type)
+    om = oracle_model(oracle, wl, False)
+    err = rel_l2(m.tendency(), om.tendency())
+    say(f"partial-tile tendency rel-L2 {err:.3e}")
+    assert err < tol

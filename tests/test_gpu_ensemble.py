@@ -150,3 +150,122 @@ def test_ensemble_error_paths():
     with pytest.raises(ra.RippleArgumentError):
         ens.step(-1.0)
     ens.close()
+
+
+def _attach_networks(models, bins, seed=0, audit_stride=3):
+    """Per-member 1155-style networks (bins + 3 -> 512 x 3 -> bins, GELU),
+    random weights, with a wind envelope that sends some members to the plain
+    bundle and an audit on every audit_stride-th column."""
+    from ripple_amd import surrogate as sg
+    nets = []
+    cfg = sg.SurrogateConfig(rate_unit=1e-7, wind_lo=0.0, wind_hi=15.0, audit_stride=audit_stride,
+                             audit_bound=0.5)
+    for k, (m, nb) in enumerate(zip(models, bins)):
+        net = sg.SourceNetwork(nb + 3, nb, hidden=512, hidden_layers=3, batch_capacity=64, seed=seed + k)
+        sg.attach_surrogate(m, net, cfg)
+        nets.append(net)
+    return nets
+
+
+def test_ensemble_with_surrogate_equals_members_stepped_alone():
+    """Config 5's defining mode: every member takes its sources from its own
+    512 x 3 network (plain bundle outside the wind envelope and on failed
+    audits).  The ensemble equals the members stepped alone with the same
+    networks, bitwise, for two launch shapes, graph on and off."""
+    say("ensemble with surrogate vs single")
+    wls = members(5, True, seed_base=700)
+    bins = [w.N0.shape[2] * w.N0.shape[3] for w in wls]
+    dt, nsteps = 0.02, 3
+    ref = []
+    for k, w in enumerate(wls):
+        m = build(w)
+        nets = _attach_networks([m], [bins[k]], seed=k)
+        ra.time_step_(m, dt, nsteps)
+        ref.append(m.action)
+        del nets
+    for ty, blocks, graph in ((4, 0, False), (2, 64, True)):
+        ms = [build(w) for w in wls]
+        nets = []
+        for k, m in enumerate(ms):
+            nets += _attach_networks([m], [bins[k]], seed=k)
+        ens = ra.Ensemble(ms)
+        ens.set_launch(ty, blocks)
+        ens.use_graph(graph)
+        ens.step(dt, nsteps)
+        ens.synchronize()
+        for k, (m, r) in enumerate(zip(ms, ref)):
+            assert np.array_equal(m.action, r), (ty, blocks, graph, k, rel_l2(m.action, r))
+        say(f"surrogate ty={ty} blocks={blocks} graph={graph}: bitwise equal")
+        ens.close()
+
+
+def test_sampled_members_with_sources_against_fp64_oracle(oracle):
+    """Four sampled config-5 members with the plain source bundle, stepped in
+    the ensemble in FP32, against FP64 oracle runs of the same members."""
+    from ripple_amd.sources import to_struct
+    say("ensemble with sources vs fp64 oracle")
+    wls = members(4, True, size_range=(12, 40), seed_base=800)
+    ms = [build(w) for w in wls]
+    ens = ra.Ensemble(ms)
+    ens.use_graph(True)
+    ens.step(0.02, 5)
+    ens.close()
+    for w, m in zip(wls, ms):
+        src = oracle.sources_from(to_struct(w.model_kwargs["sources"]))
+        om = oracle.OracleSourceModel(w, src, transport=oracle.MODE_REFRACTION)
+        om.step(0.02, 5)
+        err = rel_l2(m.action, om.action)
+        say(f"{w.name}: FP32 ensemble with sources vs FP64 oracle rel-L2 {err:.2e}")
+        assert err < 5e-3
+
+
+def test_graph_recaptured_after_member_changes():
+    """A member's sources change between graph-replayed ensemble steps, and a
+    surrogate is attached to another: the ensemble recaptures and still
+    equals the members stepped alone with the same changes."""
+    from ripple_amd import sources as srcmod
+    say("ensemble graph after member changes")
+    wls = members(3, True, seed_base=900)
+    bins = [w.N0.shape[2] * w.N0.shape[3] for w in wls]
+    dt = 0.02
+    calmer = ra.PrecomputedSources(
+        wind_input=ra.PressureCorrelationInput(drag=ra.BulkWindDrag("linear"), wind=3.0, direction=1.0),
+        dissipation=ra.LocalSaturationDissipation())
+
+    def change(models, keep):
+        srcmod.attach(models[0], calmer)
+        keep += _attach_networks([models[1]], [bins[1]], seed=11)
+
+    ref, keep = [build(w) for w in wls], []
+    for m in ref:
+        ra.time_step_(m, dt, 2)
+    change(ref, keep)
+    for m in ref:
+        ra.time_step_(m, dt, 2)
+    ms = [build(w) for w in wls]
+    ens = ra.Ensemble(ms)
+    ens.use_graph(True)
+    ens.step(dt, 2)
+    change(ms, keep)
+    ens.step(dt, 2)
+    ens.synchronize()
+    for k, (m, r) in enumerate(zip(ms, ref)):
+        assert np.array_equal(m.action, r.action), (k, rel_l2(m.action, r.action))
+    ens.close()
+
+
+def test_member_rules_rechecked_at_every_step():
+    """A member that gains propagation smoothing after the ensemble was made
+    is refused at the next ensemble step (the stage kernels cannot run it)."""
+    say("ensemble member rules at step")
+    wls = members(2, True, seed_base=950)
+    ms = [build(w) for w in wls]
+    ens = ra.Ensemble(ms)
+    ens.step(0.02, 1)
+    lib = ra._lib.load()
+    ra._lib.check(lib.rpl_set_propagation_smoothing(ms[1]._ctx.ptr, 1, 0.5, 0.5))
+    with pytest.raises(ra.RippleError):
+        ens.step(0.02, 1)
+    ra._lib.check(lib.rpl_set_propagation_smoothing(ms[1]._ctx.ptr, 0, 0.5, 0.5))
+    ens.step(0.02, 1)
+    ens.close()

@@ -209,8 +209,48 @@ def test_conservation_sums(oracle):
     # Intrinsic periodic transport telescopes: drift <= 1e-10 over 20 RK3 steps.
     ra.time_step_(m, 0.01, 20)
     assert abs(ra.total_action(m) - ta) <= 1e-10 * abs(ta)
+    # total_deep_water_energy (bulk_statistics.jl:93-99) against the oracle's
+    # energy measure on the same state.
+    om.N = np.ascontiguousarray(m.action.ravel(order="F"))
+    sg = wl.spectral_grid
+    E = oracle.energy_measures(sg.kappa_faces, sg.phi_faces)
     energy = ra.total_deep_water_energy(m)
-    assert energy > 0
+    assert energy == pytest.approx(om.total(E), rel=1e-13)
+    E2 = oracle.energy_measures(sg.kappa_faces, sg.phi_faces, gravity=2.0)
+    assert ra.total_deep_water_energy(m, gravity=2.0) == pytest.approx(om.total(E2), rel=1e-13)
+
+
+def _exact_weighted_total(N, measure, dx, dy):
+    """Exact rational sum_ij dx dy sum_mn N w of the given (FP32 or FP64) state."""
+    from fractions import Fraction
+    Nd = np.asarray(N, dtype=np.float64)
+    total = Fraction(0)
+    for m in range(Nd.shape[2]):
+        for n in range(Nd.shape[3]):
+            plane = sum((Fraction(float(v)) for v in Nd[:, :, m, n].ravel()), Fraction(0))
+            total += plane * Fraction(float(measure[m, n]))
+    return float(total * Fraction(float(dx)) * Fraction(float(dy)))
+
+
+def test_conservation_sums_fp32_state(oracle):
+    """The FP32 path keeps FP64 only for the compensated action and energy
+    sums; on an FP32 state they must match the exact sum of that same state
+    to <= 1e-10 relative (the device sums FP32 values times FP64 measures)."""
+    say("conservation sums fp32 state")
+    wl = random_workload(40, 24, 5, 8, seed=9, current=False)
+    m = device_model(wl, np.float32)
+    ra.time_step_(m, 0.01, 5)
+    N = m.action
+    assert N.dtype == np.float32
+    sg = wl.spectral_grid
+    dx = float(wl.grid.x_faces[1] - wl.grid.x_faces[0])
+    dy = float(wl.grid.y_faces[1] - wl.grid.y_faces[0])
+    w = oracle.geometry(sg.kappa, sg.kappa_faces, sg.phi, sg.phi_faces)["w"]
+    E = oracle.energy_measures(sg.kappa_faces, sg.phi_faces)
+    exact_a = _exact_weighted_total(N, w, dx, dy)
+    exact_e = _exact_weighted_total(N, E, dx, dy)
+    assert abs(ra.total_action(m) - exact_a) <= 1e-10 * abs(exact_a)
+    assert abs(ra.total_deep_water_energy(m) - exact_e) <= 1e-10 * abs(exact_e)
 
 
 def test_error_paths():

@@ -1,6 +1,7 @@
 """Pins the CPU oracle against the reference's own known answers and against
 closed forms that a transcription slip would break (CPU only)."""
 import ctypes
+from fractions import Fraction
 import math
 
 import numpy as np
@@ -55,6 +56,45 @@ def test_measures_against_quadrature(oracle):
             cgy = q(lambda k, p: 0.5 * math.sqrt(9.81 / k) * math.sin(p)) / t["w"][m, n]
             assert t["cgx"][m, n] == pytest.approx(cgx, rel=1e-10, abs=1e-12)
             assert t["cgy"][m, n] == pytest.approx(cgy, rel=1e-10, abs=1e-12)
+
+
+def test_energy_measures_against_quadrature(oracle):
+    """total_deep_water_energy's measure (bulk_statistics.jl:23-26) is the
+    integral of the deep-water frequency sqrt(g kappa) over the annular cell,
+    area element kappa dkappa dphi."""
+    kf, pf = np.array([0.0, 0.2, 0.4, 0.7, 1.0, 3.5]), np.array([-0.4, 0.8, 2.0, 3.5])
+    for g in (9.81, 1.0):
+        E = oracle.energy_measures(kf, pf, gravity=g)
+        assert E.shape == (5, 3)
+        for m in range(5):
+            for n in range(3):
+                ref = integrate.dblquad(lambda k, p: math.sqrt(g * k) * k, pf[n], pf[n + 1], kf[m], kf[m + 1],
+                                        epsabs=1e-14, epsrel=1e-13)[0]
+                assert E[m, n] == pytest.approx(ref, rel=1e-11)
+        # The cells tile the annulus: the sum telescopes to the disc's
+        # sqrt(g) * 2 pi * kf_max^(5/2) / (5/2) (lower face 0).
+        total = math.sqrt(g) * (pf[-1] - pf[0]) * kf[-1] ** 2.5 / 2.5
+        assert E.sum() == pytest.approx(total, rel=1e-14)
+
+
+def test_total_energy_against_exact_sum(oracle):
+    """total_deep_water_energy (bulk_statistics.jl:93-99) of a state is the
+    sum over columns and bins of N * measure * dx * dy; the oracle's
+    compensated total matches the exact rational sum of the same products."""
+    rng = np.random.default_rng(4)
+    nx, ny, nk, npf = 5, 4, 3, 6
+    kf = np.array([0.1, 0.3, 0.6, 1.2])
+    pf = np.linspace(0, 2 * math.pi, npf + 1)
+    E = oracle.energy_measures(kf, pf)
+    N = rng.random((nx, ny, nk, npf)) * 10.0 ** rng.integers(-6, 3, size=(nx, ny, nk, npf))
+    Nf = np.asfortranarray(N)
+    L = oracle.lib()
+    Ef = np.asfortranarray(E)
+    got = L.orc_total_weighted_f64(nx, ny, nk, npf, oracle._p(Nf), oracle._p(Ef), ctypes.c_double(0.7),
+                                   ctypes.c_double(1.3))
+    exact = sum(Fraction(float(N[i, j, m, n])) * Fraction(float(E[m, n])) for i in range(nx) for j in range(ny)
+                for m in range(nk) for n in range(npf)) * Fraction(0.7) * Fraction(1.3)
+    assert got == pytest.approx(float(exact), rel=1e-14)
 
 
 def test_finite_depth_group_velocity_limits(oracle):
@@ -141,6 +181,158 @@ def test_weno5_fifth_order_on_smooth_data(oracle):
         f = [(math.cos(xi - h / 2) - math.cos(xi + h / 2)) / h for xi in x]  # averages of sin
         errs.append(abs(oracle.weno5(f) - math.sin(h / 2)))
     assert errs[0] / errs[1] > 20 and errs[1] / errs[2] > 20
+
+
+class _Poly:
+    """Polynomial with exact rational coefficients (lowest order first)."""
+
+    def __init__(self, c):
+        self.c = [Fraction(x) for x in c]
+
+    def __add__(self, o):
+        n = max(len(self.c), len(o.c))
+        return _Poly([(self.c[k] if k < len(self.c) else 0) + (o.c[k] if k < len(o.c) else 0) for k in range(n)])
+
+    def __mul__(self, o):
+        if not isinstance(o, _Poly):
+            return _Poly([x * o for x in self.c])
+        out = [Fraction(0)] * (len(self.c) + len(o.c) - 1)
+        for a, x in enumerate(self.c):
+            for b, y in enumerate(o.c):
+                out[a + b] += x * y
+        return _Poly(out)
+
+    def deriv(self):
+        return _Poly([k * x for k, x in enumerate(self.c)][1:] or [0])
+
+    def integral(self, a, b):
+        return sum(x * (Fraction(b) ** (k + 1) - Fraction(a) ** (k + 1)) / (k + 1) for k, x in enumerate(self.c))
+
+    def __call__(self, x):
+        return sum(c * Fraction(x) ** k for k, c in enumerate(self.c))
+
+
+def _reconstruction(cells, averages):
+    """The polynomial whose averages over the unit cells [i-1/2, i+1/2],
+    i in `cells`, are `averages`: the derivative of the Lagrange interpolant of
+    the primitive through the cell faces (Jiang & Shu 1996, section 2)."""
+    faces = [Fraction(2 * cells[0] - 1, 2) + k for k in range(len(cells) + 1)]
+    prim = [Fraction(0)]
+    for v in averages:
+        prim.append(prim[-1] + Fraction(v))
+    P = _Poly([0])
+    for a, xa in enumerate(faces):
+        basis = _Poly([1])
+        for b, xb in enumerate(faces):
+            if b != a:
+                basis = basis * _Poly([-xb, 1]) * (1 / (xa - xb))
+        P = P + basis * prim[a]
+    return P.deriv()
+
+
+def _weno5_from_definitions(f, eps):
+    """Value at x = 1/2 of the WENO5-JS reconstruction of the cell averages f
+    on cells -2..2, from the definitions: candidate polynomials on the three
+    3-cell stencils, smoothness indicators
+    IS_k = sum_l int_{-1/2}^{1/2} (d^l p_k / dx^l)^2 dx (l = 1, 2; dx = 1),
+    linear weights d_k solving sum_k d_k p_k(1/2) = p_5(1/2) for all data,
+    alpha_k = d_k / (eps + IS_k)^2 (Jiang & Shu 1996, eqs. 2.10, 3.2, 3.9)."""
+    half = Fraction(1, 2)
+    stencils = [(-2, -1, 0), (-1, 0, 1), (0, 1, 2)]
+    # Linear weights: the face value of each candidate and of the 5-cell
+    # polynomial is linear in the data; match the coefficients of f_-2..f_2.
+    rows = []
+    for s in stencils:
+        rows.append([_reconstruction(s, [1 if c == e else 0 for c in s])(half) if e in s else Fraction(0)
+                     for e in range(-2, 3)])
+    big = [_reconstruction(list(range(-2, 3)), [1 if c == e else 0 for c in range(-2, 3)])(half)
+           for e in range(-2, 3)]
+    # d0 is fixed by f_-2 (only stencil 0 holds it), d2 by f_2, d1 by f_0's row.
+    d0 = big[0] / rows[0][0]
+    d2 = big[4] / rows[2][4]
+    d1 = (big[1] - d0 * rows[0][1]) / rows[1][1]
+    d = [d0, d1, d2]
+    for e in range(5):  # the weights reproduce the 5-cell reconstruction
+        assert sum(d[k] * rows[k][e] for k in range(3)) == big[e]
+    num, den = Fraction(0), Fraction(0)
+    for k, s in enumerate(stencils):
+        p = _reconstruction(s, [f[c + 2] for c in s])
+        p1 = p.deriv()
+        p2 = p1.deriv()
+        IS = (p1 * p1).integral(-half, half) + (p2 * p2).integral(-half, half)
+        alpha = d[k] / (eps + IS) ** 2
+        num += alpha * p(half)
+        den += alpha
+    return num / den, d
+
+
+def test_weno5_linear_weights_from_definitions():
+    _, d = _weno5_from_definitions([Fraction(0)] * 5, Fraction(1))
+    assert d == [Fraction(1, 10), Fraction(6, 10), Fraction(3, 10)]
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_weno5_known_answers_from_definitions(oracle, dtype):
+    """The reconstruction (wave_current_refraction.jl:41-55) against the
+    exact rational evaluation of WENO5-JS from its definitions, with the
+    reference's eps = 1e4 eps(FT), on non-smooth stencils.  The stencils make
+    the smoothness indicators differ by orders of magnitude (a wrong 13/12 or
+    1/4 moves the weights), put one indicator near eps (a wrong eps moves
+    the result), and put the most weight on each candidate in turn (a wrong
+    linear weight moves the result)."""
+    ft = np.dtype(dtype).type
+    eps = Fraction(float(ft(10000) * np.finfo(dtype).eps))
+    if dtype == np.float64:
+        tiny = 1e-6  # indicators ~1e-12, next to eps = 2.2e-12
+        rel = 1e-13
+    else:
+        tiny = 2e-2  # indicators ~4e-4, next to eps = 1.2e-3
+        rel = 2e-6
+    cases = [
+        [0.0, 0.0, 0.0, 1.0, 1.0],
+        [1.0, 1.0, 0.0, 0.0, 0.0],
+        [0.3, 1.7, 0.2, 0.9, 2.5],
+        [2.0, -1.0, 4.0, 0.5, 0.25],
+        [1.0, 1.0 + tiny, 1.0 - tiny, 1.0 + 2 * tiny, 1.0],
+        [0.0, 0.0, tiny, 3 * tiny, 0.0],
+        [5.0, 5.0 + tiny, 5.0, 5.0 + 40 * tiny, 5.0 - 30 * tiny],
+    ]
+    for f in cases:
+        fr = [Fraction(float(ft(x))) for x in f]
+        exact, _ = _weno5_from_definitions(fr, eps)
+        got = oracle.weno5([float(ft(x)) for x in f], dtype)
+        scale = max(abs(float(x)) for x in fr)
+        assert got == pytest.approx(float(exact), rel=rel, abs=rel * scale), f
+
+
+def test_weno5_known_answers_detect_constant_slips():
+    """The definitions-based evaluation differs from one built with any of
+    the plausible slips (13/12 -> 13/11, 1/4 -> 1/3, linear weights permuted,
+    eps off by 10x) on the stencils above, so the known-answer test pins them."""
+    f = [Fraction(x) for x in (0.3, 1.7, 0.2, 0.9, 2.5)]
+    eps = Fraction(1e4 * 2.0 ** -52)
+    good, _ = _weno5_from_definitions(f, eps)
+
+    def variant(c1, c2, d, e, data):
+        v = [Fraction(1, 3) * data[0] - Fraction(7, 6) * data[1] + Fraction(11, 6) * data[2],
+             -Fraction(1, 6) * data[1] + Fraction(5, 6) * data[2] + Fraction(1, 3) * data[3],
+             Fraction(1, 3) * data[2] + Fraction(5, 6) * data[3] - Fraction(1, 6) * data[4]]
+        b = [c1 * (data[0] - 2 * data[1] + data[2]) ** 2 + c2 * (data[0] - 4 * data[1] + 3 * data[2]) ** 2,
+             c1 * (data[1] - 2 * data[2] + data[3]) ** 2 + c2 * (data[1] - data[3]) ** 2,
+             c1 * (data[2] - 2 * data[3] + data[4]) ** 2 + c2 * (3 * data[2] - 4 * data[3] + data[4]) ** 2]
+        a = [d[k] / (e + b[k]) ** 2 for k in range(3)]
+        return sum(a[k] * v[k] for k in range(3)) / sum(a)
+
+    std = (Fraction(13, 12), Fraction(1, 4), [Fraction(1, 10), Fraction(6, 10), Fraction(3, 10)], eps)
+    assert variant(*std, f) == good  # the typed-out form agrees with the derived one
+    slips = [(Fraction(13, 11),) + std[1:], (std[0], Fraction(1, 3)) + std[2:],
+             std[:2] + ([Fraction(3, 10), Fraction(6, 10), Fraction(1, 10)], eps)]
+    for s in slips:
+        assert abs(variant(*s, f) - good) > 1e-6
+    ft = [Fraction(x) for x in (5.0, 5.0 + 1e-6, 5.0, 5.0 + 4e-5, 5.0 - 3e-5)]
+    good_t, _ = _weno5_from_definitions(ft, eps)
+    assert variant(*std, ft) == good_t
+    assert abs(variant(*(std[:3] + (10 * eps,)), ft) - good_t) > 1e-12
 
 
 def test_weno5_constant_and_mirror(oracle):

@@ -72,23 +72,81 @@ def test_wind_input_growth_signs(oracle):
             assert np.all(G == 0)
 
 
-def test_jp_rate_closed_form(oracle):
-    """Single cell, uniform wind along the bin: the tendency equals the
-    Janssen rate written out here from pressure_correlation.jl:10-16 times N."""
-    wl = tiny(nfreq=1, nphi=3, nx=1, ny=1, polar=True)
-    wl.spectral_grid = grids.PolarWaveVectorGrid(kappa=[0.3], phi=[0.0, 2.0, 4.0])
-    U10, N = 20.0, 2e-3
-    wl.N0 = np.full((1, 1, 1, 3), N, order="F")
-    src = src_struct(oracle, ra.PressureCorrelationInput(drag=ra.BulkWindDrag(), wind=U10))
-    G = oracle.OracleSourceModel(wl, src, transport=oracle.MODE_NONE).tendency()
-    k, g = 0.3, 9.81
-    us = U10 * math.sqrt((0.8 + 0.065 * U10) * 1e-3)
-    C = math.sqrt(g * k) / k
-    x = us / C + 0.011
-    Z = math.log(k * 0.0095 * us * us / g) + 0.4 / x
-    assert Z < 0
-    rate = (1.225 / 1025.0) * (1.2 / 0.16) * math.exp(Z) * Z ** 4 * x * x * 1.0 * math.sqrt(g * k)
-    assert G[0, 0, 0, 0] == pytest.approx(rate * N, rel=1e-13)
+def _jp_rates(oracle, U10, kappa, phi, N=2e-3, **params):
+    """Per-cell growth rate G / N of PressureCorrelationInput alone on one
+    column of a polar grid, wind along phi = 0."""
+    g = grids.RectilinearGrid(size=(1, 1, 1), x=(0, 1.0), y=(0, 1.0), z=(0, 1))
+    sg = grids.PolarWaveVectorGrid(kappa=list(kappa), phi=list(phi))
+    N0 = np.full((1, 1, len(kappa), len(phi)), N, order="F")
+    wl = workloads.Workload("jp", g, sg, None, N0, 1.0, 1, dict(horizontal_advection=None, timestepper="ForwardEuler"))
+    src = src_struct(oracle, ra.PressureCorrelationInput(drag=ra.BulkWindDrag(), wind=U10, **params))
+    return oracle.OracleSourceModel(wl, src, transport=oracle.MODE_NONE).tendency()[0, 0] / N
+
+
+def test_jp_rate_published_form(oracle):
+    """Janssen (1991) / Bidlot et al. (2005) quasi-linear growth in its
+    published form: gamma = eps (beta_max / kappa_vk^2) mu ln^4(mu) x^2 sigma
+    for mu = k z0 exp(kappa_vk / x) < 1, with x = (u*/c + z_alpha) cos(dtheta),
+    Charnock z0 = alpha0 u*^2 / g, eps = rho_a / rho_w, and the reference's
+    direction factor cos^p_in of which x^2 holds cos^2
+    (pressure_correlation.jl:109-136).  The product-and-logarithm form here
+    is not the oracle's exponent-of-a-sum form, so a slip in either differs."""
+    phi = list(np.arange(9) * (2 * math.pi / 9))
+    for U10, kappa in ((20.0, [0.05, 0.1, 0.3, 1.0]), (8.0, [0.2, 0.6, 2.0])):
+        R = _jp_rates(oracle, U10, kappa, phi)
+        g, kvk = 9.81, 0.4
+        us = U10 * math.sqrt((0.8 + 0.065 * U10) * 1e-3)  # BulkWindDrag(:linear), Cd(10) = 1.45e-3
+        z0 = 0.0095 * us ** 2 / g
+        grew = 0
+        for m, k in enumerate(kappa):
+            c = math.sqrt(g / k)
+            for n, th in enumerate(phi):
+                cs = math.cos(th)
+                if cs <= 0:
+                    assert R[m, n] == 0.0
+                    continue
+                x = (us / c + 0.011) * cs
+                mu = k * z0 * math.exp(kvk / x)
+                if mu >= 1:
+                    assert R[m, n] == 0.0
+                    continue
+                gamma = (1.225 / 1025.0) * (1.2 / kvk ** 2) * mu * math.log(mu) ** 4 * x ** 2 * math.sqrt(g * k)
+                assert R[m, n] == pytest.approx(gamma, rel=1e-12)
+                grew += 1
+        assert grew > 0
+
+
+def test_jp_rate_scalings(oracle):
+    """Exact consequences of the rate's structure: linear in beta_max and in
+    rho_air / rho_water; raising p_in by one multiplies every growing bin by
+    its cos(theta - theta_u); the growth window depends on none of them."""
+    kappa, phi = [0.05, 0.1, 0.3], list(np.arange(16) * (2 * math.pi / 16))
+    base = _jp_rates(oracle, 15.0, kappa, phi)
+    assert np.count_nonzero(base) >= 6
+    np.testing.assert_allclose(_jp_rates(oracle, 15.0, kappa, phi, beta_max=2.4), 2 * base, rtol=1e-14)
+    np.testing.assert_allclose(_jp_rates(oracle, 15.0, kappa, phi, rho_air=2.45), 2 * base, rtol=1e-14)
+    p3 = _jp_rates(oracle, 15.0, kappa, phi, p_in=3.0)
+    np.testing.assert_allclose(p3, base * np.cos(phi)[None, :], rtol=1e-13)
+
+
+def test_jp_rate_vanishes_quartically_at_the_window_edge(oracle):
+    """Inside the window the rate carries ln^4(mu) with ln(mu) linear in the
+    distance to the edge, so just inside the edge halving the distance divides
+    the rate by 2^4.  The edge wind is found by bisection on the oracle's
+    own output (zero outside, positive inside)."""
+    kappa, phi = [0.05], [0.0]
+    lo, hi = 5.0, 40.0  # zero at 5 m/s (waves outrun the wind), positive at 40 m/s
+    assert _jp_rates(oracle, lo, kappa, phi)[0, 0] == 0.0 and _jp_rates(oracle, hi, kappa, phi)[0, 0] > 0
+    for _ in range(80):
+        mid = 0.5 * (lo + hi)
+        if _jp_rates(oracle, mid, kappa, phi)[0, 0] > 0:
+            hi = mid
+        else:
+            lo = mid
+    edge = hi
+    d = 1e-5 * edge
+    r1, r2 = _jp_rates(oracle, edge + d, kappa, phi)[0, 0], _jp_rates(oracle, edge + 2 * d, kappa, phi)[0, 0]
+    assert r2 / r1 == pytest.approx(16.0, rel=1e-3)
 
 
 def test_mean_spectrum_single_bin_closed_form(oracle):
