@@ -191,6 +191,24 @@ int rpl_diagnostics(rpl_ctx *ctx, double *m0, double *kappa_rms, double *mean_di
  * the device in FP64 with compensated (double-double) summation. */
 int rpl_conservation(rpl_ctx *ctx, double *total_action, double *total_energy);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the compensated FP64 total of rpl_conservation,
+ *   *out = sum_ij area * sum_mn N[i,j,m,n] * measure[m,n]
+ * (total_action with the cell measures w, total_deep_water_energy with the
+ * energy measures), over a caller-owned device action N (nx*ny*nkappa*nphi
+ * elements of dtype, i fastest) and caller-owned device measures (nkappa*nphi
+ * FP64, m fastest), written to one device double `out`, enqueued on `stream`
+ * (a hipStream_t; NULL for the null stream) of the current device without
+ * synchronizing.  Nothing is allocated: the per-workgroup partial sums go to
+ * a library-held device array of 1024 (hi, lo) pairs (16 KiB per GPU, held
+ * until the process ends; the launch grid never exceeds it).  Every call on a
+ * GPU shares that GPU's array, so calls that overlap on one GPU (other
+ * streams or threads) race; calls in order on one stream do not.  Limits:
+ * nx * ny < 2^30 and nkappa * nphi < 2^30 (RPL_ERR_ARG beyond). */
+int rpl_weighted_total_noalloc(const void *N, int32_t dtype, int32_t nx, int32_t ny, int32_t nkappa, int32_t nphi,
+                               const double *measure, double area, double *out, void *stream);
+
 /* Source terms of the PrecomputedSources bundle
  * (Physics/Packages/precomputed_sources.jl:23-283), evaluated on the device on
  * every stage input and added to the transport tendency

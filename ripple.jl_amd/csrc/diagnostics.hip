@@ -2,6 +2,8 @@
 // direction) and the compensated global totals used for conservation checks.
 #include "rpl_internal.h"
 
+#include <mutex>
+
 #pragma clang fp contract(off)
 
 namespace rpl {
@@ -99,7 +101,58 @@ __global__ __launch_bounds__(256) void weighted_total_kernel(int nx, int ny, int
     }
 }
 
+// Folds the per-workgroup (hi, lo) pairs in workgroup order into one
+// double, the same operations in the same order as the host-side fold of
+// rpl_conservation, so the two totals are bit-identical.
+__global__ __launch_bounds__(64) void fold_partials_kernel(const double *__restrict__ partials, int nblocks,
+                                                           double *__restrict__ out)
+{
+    if (threadIdx.x != 0)
+        return;
+    double hi = 0, lo = 0;
+    for (int b = 0; b < nblocks; ++b) {
+        const double x = partials[2 * b];
+        const double s = hi + x, bv = s - hi, av = s - bv;
+        lo += (hi - av) + (x - bv) + partials[2 * b + 1];
+        hi = s;
+    }
+    out[0] = hi + lo;
+}
+
 }  // namespace
+
+// Workspace of the allocation-free total: one (hi, lo) pair per workgroup of
+// weighted_total_kernel, whose grid is capped at NOALLOC_MAX_PARTIALS
+// workgroups (16 KiB on every GPU the library runs on, held until the process
+// ends).  Host code never passes this name to a kernel: it takes the
+// device's address from hipGetSymbolAddress once per device.
+constexpr int NOALLOC_MAX_PARTIALS = 1024;
+static __device__ __align__(256) double scratch[2 * NOALLOC_MAX_PARTIALS];
+
+// The device address of `scratch` on the current device, looked up once per
+// device with that device current.
+static hipError_t scratch_address(double **out)
+{
+    constexpr int MAX_DEVICES = 64;
+    static std::mutex lock;
+    static double *addr[MAX_DEVICES] = {};
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess)
+        return e;
+    if (dev < 0 || dev >= MAX_DEVICES)
+        return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> g(lock);
+    if (!addr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess)
+            return e;
+        addr[dev] = (double *)p;
+    }
+    *out = addr[dev];
+    return hipSuccess;
+}
 
 int weighted_total_blocks(int nx, int ny)
 {
@@ -128,6 +181,29 @@ hipError_t launch_weighted_total(int nx, int ny, int nk, int nphi, const T *N, c
     return hipGetLastError();
 }
 
+template <typename T>
+hipError_t launch_weighted_total_noalloc(int nx, int ny, int nk, int nphi, const T *N, const double *w, double area,
+                                         double *out, hipStream_t s)
+{
+    const int nblocks = weighted_total_blocks(nx, ny);
+    if (nblocks > NOALLOC_MAX_PARTIALS)
+        return hipErrorInvalidValue;
+    double *partials = nullptr;
+    hipError_t e = scratch_address(&partials);
+    if (e != hipSuccess)
+        return e;
+    hipLaunchKernelGGL(weighted_total_kernel<T>, dim3(nblocks), dim3(256), 0, s, nx, ny, nk, nphi, N, w, area,
+                       partials);
+    if ((e = hipGetLastError()) != hipSuccess)
+        return e;
+    hipLaunchKernelGGL(fold_partials_kernel, dim3(1), dim3(64), 0, s, (const double *)partials, nblocks, out);
+    return hipGetLastError();
+}
+
+template hipError_t launch_weighted_total_noalloc<float>(int, int, int, int, const float *, const double *, double,
+                                                         double *, hipStream_t);
+template hipError_t launch_weighted_total_noalloc<double>(int, int, int, int, const double *, const double *, double,
+                                                          double *, hipStream_t);
 template hipError_t launch_column_diagnostics<float>(int, int, int, int, const float *, const double *,
                                                      const double *, const double *, const double *, double *,
                                                      double *, double *, hipStream_t);

@@ -1764,6 +1764,30 @@ int rpl_conservation(rpl_ctx *c, double *total_action, double *total_energy)
     return RPL_OK;
 }
 
+int rpl_weighted_total_noalloc(const void *N, int32_t dtype, int32_t nx, int32_t ny, int32_t nkappa, int32_t nphi,
+                               const double *measure, double area, double *out, void *stream)
+{
+    if (!N || !measure || !out)
+        return fail(RPL_ERR_ARG, "null argument");
+    if (dtype != RPL_F32 && dtype != RPL_F64)
+        return fail(RPL_ERR_ARG, "dtype must be RPL_F32 or RPL_F64");
+    if (nx < 1 || ny < 1 || nkappa < 1 || nphi < 1)
+        return fail(RPL_ERR_ARG, "nx, ny, nkappa and nphi must be positive");
+    if ((int64_t)nx * ny >= ((int64_t)1 << 30) || (int64_t)nkappa * nphi >= ((int64_t)1 << 30))
+        return fail(RPL_ERR_ARG, "rpl_weighted_total_noalloc: nx * ny and nkappa * nphi must each stay below 2^30");
+    if (!std::isfinite(area))
+        return fail(RPL_ERR_ARG, "area must be finite");
+    hipStream_t s = (hipStream_t)stream;
+    hipError_t e = dtype == RPL_F64
+                       ? rpl::launch_weighted_total_noalloc<double>(nx, ny, nkappa, nphi, (const double *)N, measure,
+                                                                    area, out, s)
+                       : rpl::launch_weighted_total_noalloc<float>(nx, ny, nkappa, nphi, (const float *)N, measure,
+                                                                   area, out, s);
+    if (e != hipSuccess)
+        return fail(RPL_ERR_HIP, std::string("weighted total (no allocation): ") + hipGetErrorString(e));
+    return RPL_OK;
+}
+
 // symmetric_quadruplet.jl:54-77 (_bilinear_lookup), 0-based bins.
 static bool dia_lookup(double s, double th, double smin, double smax, double log_xfr, double phimin, double dphi,
                        int nk, int nphi, int bins[4], double w[4], int *mlo, int *mhi)

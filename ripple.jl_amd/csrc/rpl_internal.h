@@ -192,6 +192,11 @@ hipError_t launch_column_diagnostics(int nx, int ny, int nk, int nphi, const T *
 template <typename T>
 hipError_t launch_weighted_total(int nx, int ny, int nk, int nphi, const T *N, const double *w,
                                  double area, double *partials, int nblocks, hipStream_t s);
+// The same total into one device double `out`, with the workgroup partials
+// in the library's static device scratch (no allocation, no host sync).
+template <typename T>
+hipError_t launch_weighted_total_noalloc(int nx, int ny, int nk, int nphi, const T *N, const double *w, double area,
+                                         double *out, hipStream_t s);
 int weighted_total_blocks(int nx, int ny);
 
 template <typename T>

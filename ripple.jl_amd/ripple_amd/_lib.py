@@ -100,6 +100,7 @@ SIGNATURES = {
     "rpl_set_clock": ([_P, _D, ctypes.c_int64], ctypes.c_int),
     "rpl_diagnostics": ([_P, _DP, _DP, _DP], ctypes.c_int),
     "rpl_conservation": ([_P, _DP, _DP], ctypes.c_int),
+    "rpl_weighted_total_noalloc": ([_P, _I32, _I32, _I32, _I32, _I32, _P, _D, _P, _P], ctypes.c_int),
     "rpl_set_sources": ([_P, ctypes.POINTER(rpl_sources), _DP, _DP], ctypes.c_int),
     "rpl_get_wind": ([_P, _DP, _DP, _DP], ctypes.c_int),
     "rpl_set_storm_track": ([_P, _I32, _DP, _DP, _DP], ctypes.c_int),
