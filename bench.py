@@ -1,14 +1,18 @@
 """Benchmark of the wave-action time step on one MI355X.
 
-A step is one SSP-RK3 time step (three fused tendency+stage kernel launches)
-of the sheared-eddy configuration: 512 x 512 cells x 32 kappa x 36 phi bins,
-FP32 state, WENO5 transport with Doppler shift and spectral refraction,
-synthetic eddy current.  The action is resident in HBM before timing starts.
+A step is one SSP-RK3 time step (three fused tendency+stage kernel launches
+plus the source preparation of each stage) of the translating-hurricane
+configuration: 1024 x 1024 cells x 32 kappa x 36 phi bins, FP32 state, WENO5
+intrinsic transport, the plain source bundle (Holland wind input with the
+stress cap, local-saturation whitecapping); then the same step with the
+learned 1155 -> 512 x 3 -> 1152 surrogate supplying the sources.  The action
+is resident in HBM before timing starts.
 
-Prints one JSON line: cell-bin updates per second per RK3 step (the value),
-the roofline of the stage kernel measured with HIP events on the library's
-stream, the FP32 error against the FP64 path, and the CPU oracle timed on a
-bounded sample of the same workload on this host.
+Prints one JSON line: cell-bin updates per second per RK3 step with the plain
+sources (the value) and with the surrogate, the roofline of the dominant
+kernel (the stage kernel) measured with HIP events on the library's stream,
+the other configurations' lines, the FP32 error against the FP64 path, and
+the CPU oracle timed on a bounded sample of the same workload on this host.
 """
 import argparse
 import json
@@ -57,19 +61,29 @@ def stage_bytes_per_launch(nx, ny, nk, nphi, esize):
     return per_step / 3.0
 
 
-def measured_traffic(workload):
-    """HBM bytes per stage launch from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary of this workload (scaled by the copy-kernel factors
-    recorded there), or None if there is none for this workload."""
-    path = os.path.join(ROOT, "profiles", "stage_phi_kernel_hbm_traffic.json")
+def intrinsic_stage_bytes_per_launch(nx, ny, nk, nphi, esize, column_fields=6):
+    """Algorithmic HBM bytes of one intrinsic-transport stage launch with the
+    source bundle, averaged over the three stages: N_k read and N_{k+1}
+    written every stage, N0 read in stages 2-3, and the per-column source
+    inputs (U10, u*, cos and sin of the wind angle, log z1, stress factor)."""
+    cells = nx * ny * nk * nphi
+    return esize * (8 * cells / 3.0 + column_fields * nx * ny)
+
+
+def measured_traffic(workload, kernel="stage_phi_kernel"):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary of this workload (scaled by the
+    copy-kernel factors recorded there), or None if there is none."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_hbm_traffic.json")
     try:
         with open(path) as f:
             rec = json.load(f)
     except (OSError, ValueError):
         return None
-    if rec.get("workload") != workload:
-        return None
-    return rec["traffic_bytes_per_launch"]
+    for r in rec if isinstance(rec, list) else [rec]:
+        if r.get("workload") == workload:
+            return r["traffic_bytes_per_launch"]
+    return None
 
 
 def time_model(model, dt, steps, warmup):
@@ -92,46 +106,72 @@ def time_model(model, dt, steps, warmup):
 MLP_FLOP_PER_COLUMN = 2 * (1155 * 512 + 512 * 512 * 2 + 512 * 1152)  # 1155 -> 512 x3 -> 1152
 
 
-def storm_plain_vs_surrogate(steps, warmup):
-    """The translating-hurricane configuration at full size (1024 x 1024 cells
-    x 32 kappa x 36 phi, FP32, intrinsic WENO5 transport, Holland wind input
-    with stress cap and local-saturation whitecapping): RK3 step time with the
-    plain source bundle and with the 1155 -> 512 x 3 -> 1152 network supplying
-    the sources on every column.  The network weights are random (a timing
+def phase_ms(model, getter):
+    """Average device milliseconds of one timing phase since its last read."""
+    import ctypes
+    ms, n = ctypes.c_double(), ctypes.c_int64()
+    _lib.check(getattr(model._ctx.lib, getter)(model._ctx.ptr, ctypes.byref(ms), ctypes.byref(n)))
+    return ms.value / max(n.value, 1), n.value
+
+
+def storm_plain_vs_surrogate(steps, warmup, nx=1024, ny=1024):
+    """The translating-hurricane configuration (1024 x 1024 cells x 32 kappa
+    x 36 phi, FP32, intrinsic WENO5 transport, Holland wind input with stress
+    cap and local-saturation whitecapping): RK3 step time with the plain
+    source bundle and with the 1155 -> 512 x 3 -> 1152 network supplying the
+    sources on every column, and the device time of each step's kernels (HIP
+    events on the library stream).  The network weights are random (a timing
     run: the work per step does not depend on them); the envelope is open so
     that no column falls back."""
-    import ctypes
     from ripple_amd import surrogate as sg
-    wl = workloads.translating_hurricane()
+    wl = workloads.translating_hurricane(nx=nx, ny=ny)
     nx, ny, nk, nphi = wl.N0.shape
     nb = nk * nphi
     m = build_model(wl, np.float32)
     del wl.N0
-    t_plain, l_plain, _ = time_model(m, wl.dt, steps, warmup)
+    t_plain, l_plain, n_plain = time_model(m, wl.dt, steps, warmup)
+    l_state, n_state = phase_ms(m, "rpl_get_source_timing")
     net = sg.SourceNetwork(nb + 3, nb, hidden=512, hidden_layers=3, batch_capacity=64, seed=1)
     cfg = sg.SurrogateConfig(rate_unit=1e-7)
     sg.attach_surrogate(m, net, cfg)
-    lib, ctx = m._ctx.lib, m._ctx.ptr
     ra.time_step_(m, wl.dt, 1)
     m.synchronize()
-    _lib.check(lib.rpl_get_surrogate_timing(ctx, None, None))
+    phase_ms(m, "rpl_get_surrogate_timing")
     t_surr, l_surr, _ = time_model(m, wl.dt, steps, 0)
-    ms, n = ctypes.c_double(), ctypes.c_int64()
-    _lib.check(lib.rpl_get_surrogate_timing(ctx, ctypes.byref(ms), ctypes.byref(n)))
+    t_inf, _ = phase_ms(m, "rpl_get_surrogate_timing")
     st = sg.surrogate_stats(m)
-    t_inf = ms.value / max(n.value, 1)
     flops = MLP_FLOP_PER_COLUMN * nx * ny
     cells = nx * ny * nk * nphi
     sg.attach_surrogate(m, None, None)
     net.close()
-    return {"workload": f"translating_hurricane_{nx}x{ny}_k{nk}_p{nphi}", "dtype": "f32 state, bf16 network",
-            "plain": {"value": cells / t_plain, "ms_per_step": t_plain * 1e3, "stage_kernel_ms": l_plain},
+    return {"workload": wl.name, "cells": cells, "dt": wl.dt,
+            "plain": {"value": cells / t_plain, "ms_per_step": t_plain * 1e3, "stage_kernel_ms": l_plain,
+                      "stage_launches_timed": n_plain, "source_state_ms": l_state,
+                      "source_state_launches_timed": n_state,
+                      "other_ms_per_step": t_plain * 1e3 - 3 * (l_plain + l_state)},
             "surrogate": {"value": cells / t_surr, "ms_per_step": t_surr * 1e3, "stage_kernel_ms": l_surr,
                           "inference_ms_per_stage": t_inf,
                           "mlp_TFLOPs": flops / (t_inf * 1e-3) / 1e12,
                           "mlp_frac_of_bf16_dense_peak": flops / (t_inf * 1e-3) / 2.5e15,
                           "fallback_columns": st["plain"], "columns": st["columns"]},
             "step_speedup_surrogate_over_plain": t_plain / t_surr}
+
+
+def refraction_line(steps, warmup, nx, ny, float_type=np.float32):
+    """The sheared-eddy refraction configuration at nx x ny x 32 kappa x 36
+    phi (fused WENO5 Doppler transport + spectral refraction, no sources):
+    RK3 step time and the stage kernel's roofline."""
+    wl = workloads.sheared_eddies(nx=nx, ny=ny)
+    m = build_model(wl, float_type)
+    esize = np.dtype(float_type).itemsize
+    t, l, n = time_model(m, wl.dt, steps, warmup)
+    nk, nphi = wl.N0.shape[2], wl.N0.shape[3]
+    bpl = stage_bytes_per_launch(nx, ny, nk, nphi, esize)
+    ach = bpl / (l * 1e-3) / 1e9
+    return {"workload": wl.name, "dtype": np.dtype(float_type).name, "value": wl.cell_bins / t,
+            "ms_per_step": t * 1e3, "avg_launch_ms": l, "launches_timed": n, "bytes_per_launch": bpl,
+            "achieved_GBs": ach, "frac_of_hbm_peak": ach / HBM_PEAK_GBS,
+            "traffic": measured_traffic(wl.name) if float_type == np.float32 else None}
 
 
 def pseudomomentum_line(steps, warmup):
@@ -289,22 +329,85 @@ def err_vs_fp64():
     return out
 
 
-def cpu_baseline(threads):
-    """The CPU oracle (FP32, reference operation order, OpenMP) on a bounded
-    sample: the same spectral grid, current and physics on a 96 x 96 sub-box."""
+def err_vs_fp64_sheared_eddies(nsteps=200):
+    """FP32 against FP64 after the sheared-eddy run at its full 512 x 512 x
+    32 x 36 size (200 steps of dt = 0.02): N, m0, kappa_rms and mean
+    direction."""
+    wl = workloads.sheared_eddies()
+    out = {"workload": f"{wl.name}_{nsteps}_steps"}
+    m64 = build_model(wl, np.float64)
+    ra.time_step_(m64, wl.dt, nsteps)
+    N64 = m64.action
+    d64 = (ra.m0(m64), ra.root_mean_square_wavenumber(m64), ra.mean_direction(m64), ra.total_action(m64))
+    del m64
+    m32 = build_model(wl, np.float32)
+    ra.time_step_(m32, wl.dt, nsteps)
+    N32 = m32.action
+
+    def rl2(a, b):
+        a = np.asarray(a, np.float64)
+        return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+
+    out["N_rel_l2"] = rl2(N32, N64)
+    del N32, N64
+    out["m0_rel_l2"] = rl2(ra.m0(m32), d64[0])
+    out["kappa_rms_rel_l2"] = rl2(ra.root_mean_square_wavenumber(m32), d64[1])
+    d = np.angle(np.exp(1j * (ra.mean_direction(m32) - d64[2])))
+    out["mean_dir_max_abs_rad"] = float(np.max(np.abs(d)))
+    out["total_action_rel_diff"] = abs(ra.total_action(m32) - d64[3]) / abs(d64[3])
+    return out
+
+
+def cpu_threads():
+    """Host threads this process may use: its CPU affinity, capped at the
+    OMP_NUM_THREADS the host sets for one GPU's share (when it sets one)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(1, n)
+
+
+def cpu_baseline():
+    """The CPU oracle (FP64, the reference's operation order, OpenMP over
+    (j, m, n)) on bounded samples of the headline workload: the hurricane's
+    spectral grid (32 kappa x 36 phi), physics and source bundle on a
+    32 x 32-column box with one thread and on a 128 x 128-column box with
+    every host thread available; two RK3 steps after one warm-up each."""
+    import platform
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
-    oracle.set_threads(threads)
-    wl = workloads.sheared_eddies(nx=96, ny=96)
-    om = oracle.OracleModel(wl, dtype=np.float32)
-    om.step(wl.dt, 1)  # warm-up
-    steps = 2
-    t0 = time.perf_counter()
-    om.step(wl.dt, steps)
-    t = (time.perf_counter() - t0) / steps
-    return {"value": wl.cell_bins / t, "unit": "cell-bin updates/s", "cores": threads, "kind": "port",
-            "sample": f"{wl.name} (96x96 sub-box of the 512x512 workload), FP32, {steps} RK3 steps after 1 "
-                      f"warm-up, OpenMP {threads} threads, {t * 1e3:.0f} ms/step"}
+    from ripple_amd.sources import to_struct
+    runs = {}
+    for label, threads, n in (("single_thread", 1, 32), ("all_threads", cpu_threads(), 128)):
+        oracle.set_threads(threads)
+        wl = workloads.translating_hurricane(nx=n, ny=n, float_type=np.float64)
+        src = oracle.sources_from(to_struct(wl.model_kwargs["sources"]))
+        om = oracle.OracleSourceModel(wl, src, depth=200.0)
+        om.step(wl.dt, 1)  # warm-up
+        steps = 2
+        t0 = time.perf_counter()
+        om.step(wl.dt, steps)
+        t = (time.perf_counter() - t0) / steps
+        runs[label] = {"threads": threads, "columns": f"{n}x{n}", "ms_per_step": t * 1e3,
+                       "value": wl.cell_bins / t}
+        del om
+    cpu = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    cpu = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    a = runs["all_threads"]
+    return {"value": a["value"], "unit": "cell-bin updates/s", "cores": a["threads"], "kind": "port",
+            "sample": (f"translating hurricane physics and 32x36 spectral grid on {a['columns']} columns, FP64 "
+                       f"oracle with the source bundle, 2 RK3 steps after 1 warm-up, OpenMP {a['threads']} "
+                       f"threads, {a['ms_per_step']:.0f} ms/step; Julia threads: n/a (reference not runnable)"),
+            "single_thread": runs["single_thread"], "cpu_model": cpu, "host": platform.node(),
+            "compiler": "gcc -O3 -march=x86-64-v3 -fopenmp -ffp-contract=off"}
 
 
 def write_performance_tsv(path, record):
@@ -312,24 +415,24 @@ def write_performance_tsv(path, record):
     (case, operation, seconds, bytes, description): seconds per RK3 step,
     bytes = the line's algorithmic HBM bytes per step where it has them."""
     from ripple_amd.validation import PerformanceMetric, write_performance_summary
-    rows = [PerformanceMetric(record["config"]["workload"], "rk3_step", record["ms_per_step"] * 1e-3,
-                              3 * record["roofline"]["bytes_per_launch"], "FP32 sheared-eddy RK3 step (headline)")]
+    rows = [PerformanceMetric(record["config"]["workload"], "rk3_step_plain", record["ms_per_step"] * 1e-3,
+                              3 * record["roofline"]["bytes_per_launch"],
+                              "FP32 hurricane RK3 step, plain sources (headline)"),
+            PerformanceMetric(record["config"]["workload"], "rk3_step_surrogate",
+                              record["surrogate"]["ms_per_step"] * 1e-3, 0.0,
+                              "FP32 hurricane RK3 step, learned surrogate sources")]
     e = record.get("extra", {})
-    if "fp64_reference_kernel" in e:
-        rows.append(PerformanceMetric(record["config"]["workload"], "rk3_step_fp64",
-                                      e["fp64_reference_kernel"]["ms_per_step"] * 1e-3, 0.0,
-                                      "FP64 reference-order RK3 step"))
+    for key, desc in (("refraction_1024x1024", "FP32 sheared-eddy refraction RK3 step"),
+                      ("refraction_512x512", "FP32 sheared-eddy refraction RK3 step"),
+                      ("refraction_512x512_fp64", "FP64 sheared-eddy refraction RK3 step")):
+        if key in e:
+            rows.append(PerformanceMetric(e[key]["workload"], "rk3_step_" + e[key]["dtype"],
+                                          e[key]["ms_per_step"] * 1e-3, 3 * e[key]["bytes_per_launch"], desc))
     for key, op, desc in (("pseudomomentum_coupling", "rk3_step", "RK3 step with the self-coupled pseudomomentum current"),
                           ("propagation_smoothing", "rk3_step", "RK3 step plus SpatialAveraging"),
                           ("nonlinear_dia", "rk3_step", "RK3 step with wind input, dissipation and DIA")):
         if key in e:
             rows.append(PerformanceMetric(e[key]["workload"], op, e[key]["ms_per_step"] * 1e-3, 0.0, desc))
-    if "storm_plain_vs_surrogate" in e:
-        st = e["storm_plain_vs_surrogate"]
-        rows.append(PerformanceMetric(st["workload"], "rk3_step_plain", st["plain"]["ms_per_step"] * 1e-3, 0.0,
-                                      "hurricane RK3 step, plain sources"))
-        rows.append(PerformanceMetric(st["workload"], "rk3_step_surrogate", st["surrogate"]["ms_per_step"] * 1e-3,
-                                      0.0, "hurricane RK3 step, learned surrogate sources"))
     write_performance_summary(path, rows)
 
 
@@ -340,7 +443,6 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-extra", action="store_true", help="skip the FP64 and larger-grid lines")
-    ap.add_argument("--no-storm", action="store_true", help="skip the plain vs surrogate hurricane line")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble line")
     ap.add_argument("--only-pseudomomentum", action="store_true",
                     help="print only the self-coupled pseudomomentum line (for profiling its kernels)")
@@ -348,6 +450,10 @@ def main():
                     help="print only the DIA nonlinear-source line (for profiling its kernel)")
     ap.add_argument("--only-smoothing", action="store_true",
                     help="print only the propagation-smoothing line (for profiling its kernel)")
+    ap.add_argument("--only-headline", action="store_true",
+                    help="print only the plain vs surrogate hurricane line (for profiling its kernels)")
+    ap.add_argument("--only-refraction", type=int, default=0, metavar="N",
+                    help="print only the N x N sheared-eddy refraction line (for profiling its kernel)")
     ap.add_argument("--ensemble-members", type=int, default=64)
     ap.add_argument("--performance-summary", default=None,
                     help="also write the measured lines as a reference-format performance summary (TSV)")
@@ -378,57 +484,60 @@ def main():
     if args.only_smoothing:
         print(json.dumps({"propagation_smoothing": smoothing_line(args.steps, args.warmup)}), flush=True)
         return
+    if args.only_headline:
+        print(json.dumps({"storm_plain_vs_surrogate": storm_plain_vs_surrogate(args.steps, args.warmup)}), flush=True)
+        return
+    if args.only_refraction:
+        n = args.only_refraction
+        print(json.dumps({"refraction": refraction_line(args.steps, args.warmup, n, n)}), flush=True)
+        return
 
-    wl = workloads.sheared_eddies()
-    nx, ny, nk, nphi = wl.N0.shape
-    model = build_model(wl, np.float32)
-    t_step, t_launch, launches = time_model(model, wl.dt, args.steps, args.warmup)
-    value = wl.cell_bins / t_step
-    bpl = stage_bytes_per_launch(nx, ny, nk, nphi, 4)
-    achieved = bpl / (t_launch * 1e-3) / 1e9
+    st = storm_plain_vs_surrogate(args.steps, args.warmup)
+    nx, ny = 1024, 1024
+    nk, nphi = 32, 36
+    plain, surr = st["plain"], st["surrogate"]
+    bpl = intrinsic_stage_bytes_per_launch(nx, ny, nk, nphi, 4)
+    achieved = bpl / (plain["stage_kernel_ms"] * 1e-3) / 1e9
     record = {
         "metric": METRIC,
-        "value": value,
+        "value": plain["value"],
         "unit": "cell-bin updates/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": t_step * 1e3,
+        "ms_per_step": plain["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": wl.name, "nx": nx, "ny": ny, "nkappa": nk, "nphi": nphi, "nz": wl.grid.Nz,
-                   "dt": wl.dt, "transport": "fused WENO5 Doppler transport + refraction",
-                   "sources": None, "timestepper": "SSP-RK3", "kernel": "fast"},
+        "config": {"workload": st["workload"], "nx": nx, "ny": ny, "nkappa": nk, "nphi": nphi, "nz": 32,
+                   "dt": st["dt"], "transport": "fused WENO5 intrinsic transport",
+                   "sources": "PrecomputedSources(PressureCorrelationInput(BulkWindDrag(:linear), Holland "
+                              "hurricane on a LinearStormTrack) with stress cap, LocalSaturationDissipation)",
+                   "timestepper": "SSP-RK3", "kernel": "fast"},
+        "surrogate": {"value": surr["value"], "ms_per_step": surr["ms_per_step"],
+                      "speedup_over_plain": st["step_speedup_surrogate_over_plain"],
+                      "network": "1155 -> 512 x 3 -> 1152 GELU, bf16 weights and activations, fp32 accumulation",
+                      "inference_ms_per_stage": surr["inference_ms_per_stage"], "mlp_TFLOPs": surr["mlp_TFLOPs"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(wl.name),
-                     "kernel": "stage_phi_kernel", "bytes_per_launch": bpl,
-                     "avg_launch_ms": t_launch, "launches_timed": launches},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(st["workload"]),
+                     "kernel": "stage_phi_kernel (intrinsic transport + source bundle + RK3 stage)",
+                     "bytes_per_launch": bpl, "avg_launch_ms": plain["stage_kernel_ms"],
+                     "launches_timed": plain["stage_launches_timed"]},
+        "kernels_per_stage_ms": {"stage_phi_kernel": plain["stage_kernel_ms"],
+                                 "source_state_kernel": plain["source_state_ms"]},
     }
-    extra = {}
+    extra = {"storm_plain_vs_surrogate": st}
     if not args.no_extra:
-        del model
-        m64 = build_model(wl, np.float64)
-        t64, l64, _ = time_model(m64, wl.dt, max(3, args.steps // 4), 1)
-        extra["fp64_reference_kernel"] = {
-            "value": wl.cell_bins / t64, "ms_per_step": t64 * 1e3, "avg_launch_ms": l64,
-            "achieved_GBs": stage_bytes_per_launch(nx, ny, nk, nphi, 8) / (l64 * 1e-3) / 1e9}
-        del m64
-        big = workloads.sheared_eddies(nx=1024, ny=1024)
-        mb = build_model(big, np.float32)
-        tb, lb, _ = time_model(mb, big.dt, max(3, args.steps // 2), 1)
-        extra["fp32_1024x1024"] = {
-            "workload": big.name, "value": big.cell_bins / tb, "ms_per_step": tb * 1e3, "avg_launch_ms": lb,
-            "achieved_GBs": stage_bytes_per_launch(1024, 1024, nk, nphi, 4) / (lb * 1e-3) / 1e9}
-        del mb
+        extra["refraction_1024x1024"] = refraction_line(max(3, args.steps // 2), 1, 1024, 1024)
+        extra["refraction_512x512"] = refraction_line(args.steps, args.warmup, 512, 512)
+        extra["refraction_512x512_fp64"] = refraction_line(max(3, args.steps // 4), 1, 512, 512, np.float64)
         extra["err_vs_fp64"] = err_vs_fp64()
+        extra["err_vs_fp64_sheared_eddies"] = err_vs_fp64_sheared_eddies()
         extra["pseudomomentum_coupling"] = pseudomomentum_line(max(3, args.steps // 2), 1)
         extra["propagation_smoothing"] = smoothing_line(max(3, args.steps // 2), 1)
         extra["nonlinear_dia"] = dia_line(max(3, args.steps // 4), 1)
-    if not args.no_storm:
-        extra["storm_plain_vs_surrogate"] = storm_plain_vs_surrogate(max(2, args.steps // 4), 1)
     if not args.no_ensemble:
         extra["ensemble"] = ensemble_line(args.ensemble_members, max(2, args.steps // 5))
         extra["ensemble_small_members"] = ensemble_line(256, 10, with_surrogate=False, size_range=(16, 64),
@@ -437,8 +546,7 @@ def main():
     if args.performance_summary:
         write_performance_tsv(args.performance_summary, record)
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        record["cpu_baseline"] = cpu_baseline(threads)
+        record["cpu_baseline"] = cpu_baseline()
     print(json.dumps(record), flush=True)
 
 

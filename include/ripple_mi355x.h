@@ -364,6 +364,12 @@ int rpl_get_coupling_timing(rpl_ctx *ctx, double *total_ms, int64_t *updates);
  * last call, under the same enable switch.  No counterpart in the reference. */
 int rpl_get_smoothing_timing(rpl_ctx *ctx, double *total_ms, int64_t *passes);
 
+/* Device time of the source bundle's per-column state launches (one per
+ * tendency evaluation: the stress cap's wave-supported stress and the
+ * dissipation's column moments, precomputed_sources.jl:41-218) since the last
+ * call, under the same enable switch.  No counterpart in the reference. */
+int rpl_get_source_timing(rpl_ctx *ctx, double *total_ms, int64_t *launches);
+
 /* Device time of the nonlinear (DIA) source launches (one per tendency
  * evaluation) since the last call, under the same enable switch.  No
  * counterpart in the reference. */

@@ -117,13 +117,14 @@ struct rpl_ctx {
     // event timing: phase 0 = stage kernel launches, 1 = surrogate inference
     // (features and network forward of one stage input), 2 = pseudomomentum
     // coupling update (contraction and, on the reference path, gradients),
-    // 3 = propagation smoothing, 4 = nonlinear (DIA) source
+    // 3 = propagation smoothing, 4 = nonlinear (DIA) source, 5 = the source
+    // bundle's per-column state of one stage input
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
     std::vector<int> event_phase;
     size_t events_used = 0;
-    double timing_ms[5] = {0, 0, 0, 0, 0};
-    int64_t timing_launches[5] = {0, 0, 0, 0, 0};
+    double timing_ms[6] = {0, 0, 0, 0, 0, 0};
+    int64_t timing_launches[6] = {0, 0, 0, 0, 0, 0};
 };
 
 namespace {
@@ -224,6 +225,7 @@ rpl::StageParams<T> stage_params(const rpl_ctx *c, double dt)
         q.d2 = (T)s.delta2;
         q.ls_cds = (T)s.ls_cds;
         q.ls_br = (T)s.ls_br;
+        q.ls_inv_br = (T)(1.0 / s.ls_br);
         q.ls_p = (T)s.ls_p;
         q.ls_q = (T)s.ls_sigma_power;
         q.wc_rate = (T)s.wc_rate;
@@ -312,6 +314,9 @@ int launch_wind(rpl_ctx *c)
     return RPL_OK;
 }
 
+int record_begin(rpl_ctx *c, int phase);
+int record_end(rpl_ctx *c);
+
 // Bundle state of one stage input (prepare_sources, precomputed_sources.jl:230-238).
 int launch_state(rpl_ctx *c, const void *Nk)
 {
@@ -322,6 +327,9 @@ int launch_state(rpl_ctx *c, const void *Nk)
     if (!needs)
         return RPL_OK;
     const int nx = c->cfg.nx, ny = c->cfg.ny, nk = c->cfg.nkappa, np = c->cfg.nphi;
+    int rc = record_begin(c, 5);
+    if (rc != RPL_OK)
+        return rc;
     hipError_t e;
     if (is_f64(c)) {
         auto p = stage_params<double>(c, 0.0);
@@ -336,7 +344,7 @@ int launch_state(rpl_ctx *c, const void *Nk)
     }
     if (e != hipSuccess)
         return fail(RPL_ERR_HIP, std::string("source state kernel: ") + hipGetErrorString(e));
-    return RPL_OK;
+    return record_end(c);
 }
 
 // Network inference of one stage input: features and forward pass chunk by
@@ -718,6 +726,7 @@ struct rpl_ensemble {
     hipStream_t stream = nullptr;
     int transport = 0;
     bool src = false;
+    int src_kind = 0;  // rpl::SOURCE_* of the members' stage launches
     int ty = 4, blocks = 0, cus = 0;
     rpl::EnsembleMember *desc = nullptr;  // 3 stages x members, device
     long long total = 0;
@@ -794,6 +803,15 @@ int ensemble_descriptors(rpl_ensemble *e, double dt)
         t0 += (long long)nbx * nby * c->cfg.nkappa * (phi_loop ? 1 : c->cfg.nphi);
     }
     e->total = t0;
+    // One source kind for the whole launch: the specialised bundle path when
+    // every member qualifies (the two paths give the same bits).
+    e->src_kind = rpl::SOURCE_NONE;
+    if (e->src) {
+        e->src_kind = rpl::SOURCE_JP_LS;
+        for (int k = 0; k < M; ++k)
+            if (!rpl::source_is_jp_ls(h[k].p.src))
+                e->src_kind = rpl::SOURCE_GENERIC;
+    }
     RPL_HIP(hipMemcpy(e->desc, h.data(), h.size() * sizeof(rpl::EnsembleMember), hipMemcpyHostToDevice));
     const size_t nchunks = (size_t)((t0 + (1LL << rpl::ENSEMBLE_CHUNK_SHIFT) - 1) >> rpl::ENSEMBLE_CHUNK_SHIFT);
     std::vector<int> cm(nchunks);
@@ -890,7 +908,7 @@ int ensemble_stages(rpl_ensemble *e)
             }
         }
         hipError_t err = rpl::launch_ensemble_stage(e->desc + (size_t)(st - 1) * M, M, e->total, e->chunk_member,
-                                                    e->transport, e->src, st, e->ty, launch_blocks(e), e->stream);
+                                                    e->transport, e->src_kind, st, e->ty, launch_blocks(e), e->stream);
         if (err != hipSuccess)
             return fail(RPL_ERR_HIP, std::string("ensemble stage launch: ") + hipGetErrorString(err));
     }
@@ -2376,6 +2394,23 @@ int rpl_get_smoothing_timing(rpl_ctx *c, double *total_ms, int64_t *passes)
     return RPL_OK;
 }
 
+int rpl_get_source_timing(rpl_ctx *c, double *total_ms, int64_t *launches)
+{
+    if (!c)
+        return fail(RPL_ERR_ARG, "null context");
+    RPL_HIP(hipSetDevice(c->device));
+    int rc = fold_timing(c);
+    if (rc)
+        return rc;
+    if (total_ms)
+        *total_ms = c->timing_ms[5];
+    if (launches)
+        *launches = c->timing_launches[5];
+    c->timing_ms[5] = 0;
+    c->timing_launches[5] = 0;
+    return RPL_OK;
+}
+
 int rpl_get_surrogate_timing(rpl_ctx *c, double *total_ms, int64_t *evaluations)
 {
     if (!c)
@@ -2597,9 +2632,13 @@ int rpl_ensemble_get_launch(rpl_ensemble *e, int32_t *tile_rows, int32_t *blocks
         *blocks = e->blocks;  // 0: one workgroup per tile
     if (tiles) {
         long long t = 0;
-        for (rpl_ctx *c : e->members)
-            t += (long long)((c->cfg.nx + 63) / 64) * ((c->cfg.ny + e->ty - 1) / e->ty) * c->cfg.nkappa *
-                 c->cfg.nphi;
+        for (rpl_ctx *c : e->members) {
+            if (e->transport == RPL_TRANSPORT_INTRINSIC || e->transport == RPL_TRANSPORT_REFRACTION)
+                t += (long long)((c->cfg.nx + 63) / 64) * ((c->cfg.ny + e->ty - 1) / e->ty) * c->cfg.nkappa;
+            else
+                t += (long long)((c->cfg.nx + 63) / 64) * ((c->cfg.ny + e->ty - 1) / e->ty) * c->cfg.nkappa *
+                     c->cfg.nphi;
+        }
         *tiles = t;
     }
     return RPL_OK;
@@ -2717,12 +2756,12 @@ int rpl_ensemble_autotune(rpl_ensemble *e, double dt, int32_t reps, int32_t *til
         for (int mu : mult) {
             long long g = mu ? (long long)e->cus * mu : (e->total + 7) / 8 * 8;
             g = std::max<long long>(8, std::min<long long>(g, 0x7ffffff8LL) / 8 * 8);
-            err = rpl::launch_ensemble_stage(e->desc, M, e->total, e->chunk_member, e->transport, e->src,
+            err = rpl::launch_ensemble_stage(e->desc, M, e->total, e->chunk_member, e->transport, e->src_kind,
                                              rpl::STAGE_1, ty, (int)g, e->stream);  // warm-up
             if (err == hipSuccess)
                 err = hipEventRecord(a, e->stream);
             for (int r = 0; r < reps && err == hipSuccess; ++r)
-                err = rpl::launch_ensemble_stage(e->desc, M, e->total, e->chunk_member, e->transport, e->src,
+                err = rpl::launch_ensemble_stage(e->desc, M, e->total, e->chunk_member, e->transport, e->src_kind,
                                                  rpl::STAGE_1, ty, (int)g, e->stream);
             if (err == hipSuccess)
                 err = hipEventRecord(b, e->stream);

@@ -103,8 +103,9 @@ hipError_t launch_ensemble_state(const EnsembleState *mem, int members, long lon
 // `blocks` (a multiple of 8) workgroups of 64 x ty threads; chunk_member[c]
 // is the member holding tile c << ENSEMBLE_CHUNK_SHIFT.
 constexpr int ENSEMBLE_CHUNK_SHIFT = 8;
+
 hipError_t launch_ensemble_stage(const EnsembleMember *mem, int members, long long total, const int *chunk_member,
-                                 int transport, bool src, int stage, int ty, int blocks, hipStream_t s);
+                                 int transport, int src_kind, int stage, int ty, int blocks, hipStream_t s);
 hipError_t launch_stage_fast(const StageParams<float> &p, int transport, int stage,
                              const float *Nk, const float *N0, float *out, hipStream_t s);
 

@@ -36,6 +36,7 @@ struct SourceParams {
     T z_alpha, p_in, alpha0, kvk, g;
     T cds, d1, d2;              // mean-spectrum whitecapping
     T ls_cds, ls_br, ls_p, ls_q; // local saturation
+    T ls_inv_br;                 // 1 / ls_br (the FP32 path multiplies by it)
     T wc_rate, wc_thr, wc_sp;    // whitecapping: rate, saturation threshold and power
     const T *wc_factor;          // whitecapping: <kappa^p>_cell / kappa_ref^p per bin
     // FP32 forms (null in FP64, which keeps the reference's operation order):
@@ -66,6 +67,15 @@ __device__ __forceinline__ T gld(const T *p, size_t i)
     if (AS1)
         return ((const __attribute__((address_space(1))) T *)p)[i];
     return p[i];
+}
+
+// A table entry read through the constant address space: at a wave-uniform
+// index this is a scalar (s_load) load, which costs no vector memory or VALU
+// instruction.  Only for tables that no kernel writes while it reads them.
+template <typename T>
+__device__ __forceinline__ T sld(const T *p, size_t i)
+{
+    return ((const __attribute__((address_space(4))) T *)p)[i];
 }
 
 template <typename T>
@@ -126,7 +136,10 @@ __device__ __forceinline__ float jp_rate_f32(const SourceParams<float> &s, float
         return 0.0f;
     const float cp = s.p_in == 2.0f ? cos_tu * cos_tu : powf(cos_tu, s.p_in);
     const float Z2 = Z * Z;
-    return s.c0 * expf(Z) * (Z2 * Z2) * (inv_age * inv_age) * cp * sigma;
+    // Z < 0 here, so exp(Z) lies in (0, 1): the hardware exponential
+    // (__expf, a few ulp) keeps the FP32 rate well inside the FP32 path's
+    // tolerance and costs a tenth of expf
+    return s.c0 * __expf(Z) * (Z2 * Z2) * (inv_age * inv_age) * cp * sigma;
 }
 
 // The per-column inputs of the per-cell source (what the wind kernel and the
@@ -185,30 +198,30 @@ __device__ __forceinline__ SourceBin<T> source_bin(const SourceParams<T> &s, int
     // outside its envelope still take the plain source.
     SourceBin<T> b{};
     if (s.surrogate)
-        b.out_scale = gld<AS1>(s.out_scale, t);
-    b.k = gld<AS1>(s.k_cell, t);
+        b.out_scale = sld(s.out_scale, t);
+    b.k = sld(s.k_cell, t);
     if (s.wind_input) {
         if constexpr (std::is_same<T, float>::value) {
-            b.cth = gld<AS1>(s.cth_cell, t);
-            b.sth = gld<AS1>(s.sth_cell, t);
-            b.logk = gld<AS1>(s.logk_cell, t);
-            b.invc = gld<AS1>(s.invc_cell, t);
-            b.sigma = gld<AS1>(s.sigma_cell, t);
+            b.cth = sld(s.cth_cell, t);
+            b.sth = sld(s.sth_cell, t);
+            b.logk = sld(s.logk_cell, t);
+            b.invc = sld(s.invc_cell, t);
+            b.sigma = sld(s.sigma_cell, t);
         } else {
-            b.sigma = gld<AS1>(s.sigma_cell, t);
-            b.C = gld<AS1>(s.C_cell, t);
-            b.theta = gld<AS1>(s.theta_cell, t);
+            b.sigma = sld(s.sigma_cell, t);
+            b.C = sld(s.C_cell, t);
+            b.theta = sld(s.theta_cell, t);
         }
     }
     if (s.dissipation == 2) {
         if constexpr (std::is_same<T, float>::value) {
-            b.sk3 = gld<AS1>(s.sk3_cell, t);
-            b.sq = gld<AS1>(s.sq_cell, t);
+            b.sk3 = sld(s.sk3_cell, t);
+            b.sq = sld(s.sq_cell, t);
         } else {
-            b.sigma = gld<AS1>(s.sigma_cell, t);
+            b.sigma = sld(s.sigma_cell, t);
         }
     } else if (s.dissipation == 3) {
-        b.wcf = gld<AS1>(s.wc_factor, t);
+        b.wcf = sld(s.wc_factor, t);
     }
     return b;
 }
@@ -268,7 +281,8 @@ __device__ __forceinline__ T source_term_cb(const SourceParams<T> &s, const Sour
                 B = a * b.sigma * (k * k * k);
                 sq = dpow(b.sigma, s.ls_q);
             }
-            T over = B / s.ls_br - T(1);
+            // FP32 multiplies by the tabulated 1 / B_r; FP64 divides as the reference
+            T over = (std::is_same<T, float>::value ? B * s.ls_inv_br : B / s.ls_br) - T(1);
             over = over > T(0) ? over : T(0);
             if (over > T(0)) {
                 over = over < T(1e6) ? over : T(1e6);
@@ -292,6 +306,85 @@ __device__ __forceinline__ T source_term_cb(const SourceParams<T> &s, const Sour
     }
     if (net)
         return (positive - damping * a) + learned;
+    return positive - damping * a;
+}
+
+// Source kinds a stage kernel is compiled for: none, the common bundle of
+// the wind input (any drag, with or without the stress cap) and
+// local-saturation or no dissipation, without a network (SOURCE_JP_LS), or
+// every configuration through source_term_cb (SOURCE_GENERIC).
+enum { SOURCE_NONE = 0, SOURCE_JP_LS = 1, SOURCE_GENERIC = 2 };
+
+// Whether a source configuration takes the SOURCE_JP_LS path.
+template <typename T>
+__host__ __device__ inline bool source_is_jp_ls(const SourceParams<T> &s)
+{
+    return s.on && !s.surrogate && (s.wind_input == 0 || s.wind_input == 1) &&
+           (s.dissipation == 0 || s.dissipation == 2);
+}
+
+// The per-column inputs of SOURCE_JP_LS: u* (0 where U10 <= 0, which the
+// bundle skips), cos and sin of the wind angle, log z1 and the stress factor
+// (1 without the cap).  byte_off is the column's byte offset in an (nx, ny)
+// FP32 plane.
+struct JpColumn {
+    float us, ctu, stu, lz1, fac;
+};
+
+template <bool AS1 = false>
+__device__ __forceinline__ JpColumn jp_column(const SourceParams<float> &s, unsigned byte_off)
+{
+    JpColumn c{0.0f, 0.0f, 0.0f, 0.0f, 1.0f};
+    if (!s.wind_input)
+        return c;
+    typedef const __attribute__((address_space(1))) char gchar;
+    auto ld = [&](const float *base) {
+        return *(const __attribute__((address_space(1))) float *)((gchar *)base + byte_off);
+    };
+    const float U10 = ld(s.U10);
+    c.us = U10 > 0.0f ? ld(s.ustar) : 0.0f;
+    c.ctu = ld(s.cos_tu);
+    c.stu = ld(s.sin_tu);
+    c.lz1 = ld(s.log_z1);
+    if (s.stress_cap)
+        c.fac = ld(s.factor);
+    return c;
+}
+
+// source_term_cb for a SOURCE_JP_LS configuration, in its operation order
+// (so the two paths give the same bits): the bundle's wind input times the
+// stress factor plus local-saturation dissipation (precomputed_sources.jl:
+// 196-218, local_saturation.jl:18-52).
+__device__ __forceinline__ float source_jp_ls(const SourceParams<float> &s, const JpColumn &c,
+                                              const SourceBin<float> &b, float a)
+{
+#pragma clang fp contract(off)
+    float positive = 0.0f, damping = 0.0f;
+    const float k = b.k;
+    if (s.wind_input && c.us > 0.0f && k > 0.0f) {
+        const float raw = jp_rate_f32(s, b.cth, b.sth, b.logk, b.invc, b.sigma, c.ctu, c.stu, c.lz1, c.us);
+        if (raw > 0.0f) {
+            const float rate = s.stress_cap ? raw * c.fac : raw;
+            if (rate >= 0.0f)
+                positive += rate * a;
+            else
+                damping += -rate;
+        }
+    }
+    if (s.dissipation == 2 && a > 0.0f && k > 0.0f) {
+        const float B = a * b.sk3;
+        float over = B * s.ls_inv_br - 1.0f;
+        over = over > 0.0f ? over : 0.0f;
+        if (over > 0.0f) {
+            over = over < 1e6f ? over : 1e6f;
+            const float op = s.ls_p == 2.0f ? over * over : powf(over, s.ls_p);
+            const float rate = -s.ls_cds * b.sq * op;
+            if (rate >= 0.0f)
+                damping += rate;
+            else
+                positive += -rate * a;
+        }
+    }
     return positive - damping * a;
 }
 

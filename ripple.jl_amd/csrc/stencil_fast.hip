@@ -413,7 +413,7 @@ __global__ __launch_bounds__(64 * STAGE_ROWS) void stage_fast_kernel(const float
 // that of stage_tile, so the results are the same bit for bit.  Shared by the
 // per-model kernel and the ensemble work-list kernel (AS1: p and the slab
 // pointers came from memory).
-template <int TRANSPORT, int STAGE, bool SRC, int TY, bool AS1>
+template <int TRANSPORT, int STAGE, int SK, int TY, bool AS1>
 __device__ __forceinline__ void phi_tile(const float *__restrict__ Nk, const float *__restrict__ N0, float *out,
                                          const StageParams<float> &p, int ib, int jb, int m)
 {
@@ -449,8 +449,13 @@ __device__ __forceinline__ void phi_tile(const float *__restrict__ Nk, const flo
         Uyc = ldo<AS1>(Uy, bo);
         kap = gld<AS1>(p.kappa, m);
     }
+    constexpr bool SRC = SK != SOURCE_NONE;
+    // the column's source inputs, loaded once for every bin of the tile
     SourceColumn<float> scol{};
-    if (SRC)
+    JpColumn jcol{};
+    if constexpr (SK == SOURCE_JP_LS)
+        jcol = jp_column<AS1>(p.src, bo);
+    else if constexpr (SK == SOURCE_GENERIC)
         scol = source_column<float, AS1>(p.src, off);
     // What bin n reads: its x and y stencils, for refraction its kappa
     // stencil and the phi window's next value f(n + 4), otherwise the cell
@@ -566,7 +571,10 @@ __device__ __forceinline__ void phi_tile(const float *__restrict__ Nk, const flo
             const float dk = kappa_difference(cur.fk, x0, ck, m, nk);
             L = L - fmaf(dk, p.dk_inv, dp * p.dp_inv);
         }
-        if (SRC)  // tendencies.jl:88-94, G += source_tendency
+        // tendencies.jl:88-94, G += source_tendency
+        if constexpr (SK == SOURCE_JP_LS)
+            L += source_jp_ls(p.src, jcol, cur.sb, x0);
+        else if constexpr (SK == SOURCE_GENERIC)
             L += source_term_cb<float, AS1>(p.src, scol, cur.sb, off, m + nk * n, x0);
         if (p.src_add)
             L += cur.add;
@@ -603,9 +611,9 @@ __device__ __forceinline__ void phi_tile(const float *__restrict__ Nk, const flo
 // eddies); refraction with sources five (six spilled values; config-5
 // members one after another: 137.8 -> 134.7 ms per step).  Higher floors
 // spill more and run slower.
-template <int TRANSPORT, int STAGE, bool SRC>
+template <int TRANSPORT, int STAGE, int SK>
 __global__ __launch_bounds__(256)
-__attribute__((amdgpu_waves_per_eu(TRANSPORT == RPL_TRANSPORT_INTRINSIC ? 7 : (SRC ? 5 : 6), 8)))
+__attribute__((amdgpu_waves_per_eu(TRANSPORT == RPL_TRANSPORT_INTRINSIC ? 7 : (SK != SOURCE_NONE ? 5 : 6), 8)))
 void stage_phi_kernel(const float *__restrict__ Nk, const float *__restrict__ N0, float *out, StageParams<float> p,
                       int nbx, int nby)
 {
@@ -614,10 +622,10 @@ void stage_phi_kernel(const float *__restrict__ Nk, const float *__restrict__ N0
         t = tile_of(blockIdx.x, nbx * nby * p.nk, nbx, nby, p.nk, 1);
     else
         t = tile_of_spatial(blockIdx.x, nbx * nby * p.nk, nbx, nby, p.nk);
-    phi_tile<TRANSPORT, STAGE, SRC, 4, false>(Nk, N0, out, p, t.ib, t.jb, t.m);
+    phi_tile<TRANSPORT, STAGE, SK, 4, false>(Nk, N0, out, p, t.ib, t.jb, t.m);
 }
 
-template <int TRANSPORT, bool SRC>
+template <int TRANSPORT, int SK>
 hipError_t launch_phi(const StageParams<float> &p, int stage, const float *Nk, const float *N0, float *out,
                       hipStream_t s)
 {
@@ -628,17 +636,17 @@ hipError_t launch_phi(const StageParams<float> &p, int stage, const float *Nk, c
     dim3 block(64, 4, 1), grid((unsigned)total, 1, 1);
     switch (stage) {
     case STAGE_TENDENCY:
-        hipLaunchKernelGGL((stage_phi_kernel<TRANSPORT, STAGE_TENDENCY, SRC>), grid, block, 0, s, Nk, N0, out, p,
+        hipLaunchKernelGGL((stage_phi_kernel<TRANSPORT, STAGE_TENDENCY, SK>), grid, block, 0, s, Nk, N0, out, p,
                            nbx, nby);
         break;
     case STAGE_1:
-        hipLaunchKernelGGL((stage_phi_kernel<TRANSPORT, STAGE_1, SRC>), grid, block, 0, s, Nk, N0, out, p, nbx, nby);
+        hipLaunchKernelGGL((stage_phi_kernel<TRANSPORT, STAGE_1, SK>), grid, block, 0, s, Nk, N0, out, p, nbx, nby);
         break;
     case STAGE_2:
-        hipLaunchKernelGGL((stage_phi_kernel<TRANSPORT, STAGE_2, SRC>), grid, block, 0, s, Nk, N0, out, p, nbx, nby);
+        hipLaunchKernelGGL((stage_phi_kernel<TRANSPORT, STAGE_2, SK>), grid, block, 0, s, Nk, N0, out, p, nbx, nby);
         break;
     default:
-        hipLaunchKernelGGL((stage_phi_kernel<TRANSPORT, STAGE_3, SRC>), grid, block, 0, s, Nk, N0, out, p, nbx, nby);
+        hipLaunchKernelGGL((stage_phi_kernel<TRANSPORT, STAGE_3, SK>), grid, block, 0, s, Nk, N0, out, p, nbx, nby);
         break;
     }
     return hipGetLastError();
@@ -701,7 +709,7 @@ __global__ __launch_bounds__(64 * TY) void ensemble_stage_kernel(const EnsembleM
 // addressing) and would drop to four; held at five it spills a few values
 // and runs faster (measured on 32 config-5 members: 157.5 -> 149.9 ms per
 // step).
-template <int TRANSPORT, int STAGE, bool SRC, int TY>
+template <int TRANSPORT, int STAGE, int SK, int TY>
 __global__ __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(5, 8))) void ensemble_phi_kernel(
     const EnsembleMember *__restrict__ mem, int members, long long total, const int *__restrict__ chunk_member)
 {
@@ -733,7 +741,7 @@ __global__ __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(5, 8)))
             jb = q % e.nby;
             m = q / e.nby;
         }
-        phi_tile<TRANSPORT, STAGE, SRC, TY, true>(e.Nk, e.N0, e.out, e.p, ib, jb, m);
+        phi_tile<TRANSPORT, STAGE, SK, TY, true>(e.Nk, e.N0, e.out, e.p, ib, jb, m);
     }
 }
 
@@ -764,22 +772,22 @@ hipError_t launch_t(const StageParams<float> &p, int stage, const float *Nk, con
     return hipGetLastError();
 }
 
-template <int TRANSPORT, int STAGE, bool SRC>
+template <int TRANSPORT, int STAGE, int SK>
 hipError_t launch_ens_ty(const EnsembleMember *mem, int members, long long total, const int *chunk_member, int ty,
                          int blocks, hipStream_t s)
 {
     if constexpr (TRANSPORT == RPL_TRANSPORT_REFRACTION || TRANSPORT == RPL_TRANSPORT_INTRINSIC) {
         switch (ty) {
         case 2:
-            hipLaunchKernelGGL((ensemble_phi_kernel<TRANSPORT, STAGE, SRC, 2>), dim3(blocks), dim3(64, 2), 0, s, mem,
+            hipLaunchKernelGGL((ensemble_phi_kernel<TRANSPORT, STAGE, SK, 2>), dim3(blocks), dim3(64, 2), 0, s, mem,
                                members, total, chunk_member);
             break;
         case 8:
-            hipLaunchKernelGGL((ensemble_phi_kernel<TRANSPORT, STAGE, SRC, 8>), dim3(blocks), dim3(64, 8), 0, s, mem,
+            hipLaunchKernelGGL((ensemble_phi_kernel<TRANSPORT, STAGE, SK, 8>), dim3(blocks), dim3(64, 8), 0, s, mem,
                                members, total, chunk_member);
             break;
         default:
-            hipLaunchKernelGGL((ensemble_phi_kernel<TRANSPORT, STAGE, SRC, 4>), dim3(blocks), dim3(64, 4), 0, s, mem,
+            hipLaunchKernelGGL((ensemble_phi_kernel<TRANSPORT, STAGE, SK, 4>), dim3(blocks), dim3(64, 4), 0, s, mem,
                                members, total, chunk_member);
             break;
         }
@@ -787,50 +795,58 @@ hipError_t launch_ens_ty(const EnsembleMember *mem, int members, long long total
     }
     switch (ty) {
     case 2:
-        hipLaunchKernelGGL((ensemble_stage_kernel<TRANSPORT, STAGE, SRC, 2>), dim3(blocks), dim3(64, 2), 0, s, mem,
+        hipLaunchKernelGGL((ensemble_stage_kernel<TRANSPORT, STAGE, SK != SOURCE_NONE, 2>), dim3(blocks), dim3(64, 2), 0, s, mem,
                            members, total, chunk_member);
         break;
     case 8:
-        hipLaunchKernelGGL((ensemble_stage_kernel<TRANSPORT, STAGE, SRC, 8>), dim3(blocks), dim3(64, 8), 0, s, mem,
+        hipLaunchKernelGGL((ensemble_stage_kernel<TRANSPORT, STAGE, SK != SOURCE_NONE, 8>), dim3(blocks), dim3(64, 8), 0, s, mem,
                            members, total, chunk_member);
         break;
     default:
-        hipLaunchKernelGGL((ensemble_stage_kernel<TRANSPORT, STAGE, SRC, 4>), dim3(blocks), dim3(64, 4), 0, s, mem,
+        hipLaunchKernelGGL((ensemble_stage_kernel<TRANSPORT, STAGE, SK != SOURCE_NONE, 4>), dim3(blocks), dim3(64, 4), 0, s, mem,
                            members, total, chunk_member);
         break;
     }
     return hipGetLastError();
 }
 
-template <int TRANSPORT, bool SRC>
+template <int TRANSPORT, int SK>
 hipError_t launch_ens_stage(const EnsembleMember *mem, int members, long long total, const int *chunk_member,
                             int stage, int ty, int blocks, hipStream_t s)
 {
     switch (stage) {
     case STAGE_1:
-        return launch_ens_ty<TRANSPORT, STAGE_1, SRC>(mem, members, total, chunk_member, ty, blocks, s);
+        return launch_ens_ty<TRANSPORT, STAGE_1, SK>(mem, members, total, chunk_member, ty, blocks, s);
     case STAGE_2:
-        return launch_ens_ty<TRANSPORT, STAGE_2, SRC>(mem, members, total, chunk_member, ty, blocks, s);
+        return launch_ens_ty<TRANSPORT, STAGE_2, SK>(mem, members, total, chunk_member, ty, blocks, s);
     default:
-        return launch_ens_ty<TRANSPORT, STAGE_3, SRC>(mem, members, total, chunk_member, ty, blocks, s);
+        return launch_ens_ty<TRANSPORT, STAGE_3, SK>(mem, members, total, chunk_member, ty, blocks, s);
     }
 }
 
 }  // namespace
 
 hipError_t launch_ensemble_stage(const EnsembleMember *mem, int members, long long total, const int *chunk_member,
-                                 int transport, bool src, int stage, int ty, int blocks, hipStream_t s)
+                                 int transport, int src_kind, int stage, int ty, int blocks, hipStream_t s)
 {
     if (members < 1 || total < 1 || !chunk_member || blocks < 8 || blocks % 8 || (ty != 2 && ty != 4 && ty != 8) ||
-        stage < STAGE_1 || stage > STAGE_3)
+        stage < STAGE_1 || stage > STAGE_3 || src_kind < SOURCE_NONE || src_kind > SOURCE_GENERIC)
         return hipErrorInvalidValue;
     const int *cm = chunk_member;
-    if (transport == RPL_TRANSPORT_REFRACTION)
-        return src ? launch_ens_stage<RPL_TRANSPORT_REFRACTION, true>(mem, members, total, cm, stage, ty, blocks, s)
-                   : launch_ens_stage<RPL_TRANSPORT_REFRACTION, false>(mem, members, total, cm, stage, ty, blocks, s);
-    if (transport == RPL_TRANSPORT_INTRINSIC)
-        return src ? launch_ens_stage<RPL_TRANSPORT_INTRINSIC, true>(mem, members, total, cm, stage, ty, blocks, s)
-                   : launch_ens_stage<RPL_TRANSPORT_INTRINSIC, false>(mem, members, total, cm, stage, ty, blocks, s);
+    if (transport == RPL_TRANSPORT_REFRACTION) {
+        if (src_kind == SOURCE_JP_LS)
+            return launch_ens_stage<RPL_TRANSPORT_REFRACTION, SOURCE_JP_LS>(mem, members, total, cm, stage, ty, blocks, s);
+        return src_kind == SOURCE_GENERIC
+                   ? launch_ens_stage<RPL_TRANSPORT_REFRACTION, SOURCE_GENERIC>(mem, members, total, cm, stage, ty, blocks, s)
+                   : launch_ens_stage<RPL_TRANSPORT_REFRACTION, SOURCE_NONE>(mem, members, total, cm, stage, ty, blocks, s);
+    }
+    if (transport == RPL_TRANSPORT_INTRINSIC) {
+        if (src_kind == SOURCE_JP_LS)
+            return launch_ens_stage<RPL_TRANSPORT_INTRINSIC, SOURCE_JP_LS>(mem, members, total, cm, stage, ty, blocks, s);
+        return src_kind == SOURCE_GENERIC
+                   ? launch_ens_stage<RPL_TRANSPORT_INTRINSIC, SOURCE_GENERIC>(mem, members, total, cm, stage, ty, blocks, s)
+                   : launch_ens_stage<RPL_TRANSPORT_INTRINSIC, SOURCE_NONE>(mem, members, total, cm, stage, ty, blocks, s);
+    }
     return hipErrorInvalidValue;
 }
 
@@ -838,16 +854,19 @@ hipError_t launch_stage_fast(const StageParams<float> &p, int transport, int sta
                              float *out, hipStream_t s)
 {
     if (p.src.on) {
+        const bool jp = source_is_jp_ls(p.src);  // the specialised bundle path (same bits as the generic one)
         if (transport == RPL_TRANSPORT_REFRACTION)
-            return launch_phi<RPL_TRANSPORT_REFRACTION, true>(p, stage, Nk, N0, out, s);
+            return jp ? launch_phi<RPL_TRANSPORT_REFRACTION, SOURCE_JP_LS>(p, stage, Nk, N0, out, s)
+                      : launch_phi<RPL_TRANSPORT_REFRACTION, SOURCE_GENERIC>(p, stage, Nk, N0, out, s);
         if (transport == RPL_TRANSPORT_INTRINSIC)
-            return launch_phi<RPL_TRANSPORT_INTRINSIC, true>(p, stage, Nk, N0, out, s);
+            return jp ? launch_phi<RPL_TRANSPORT_INTRINSIC, SOURCE_JP_LS>(p, stage, Nk, N0, out, s)
+                      : launch_phi<RPL_TRANSPORT_INTRINSIC, SOURCE_GENERIC>(p, stage, Nk, N0, out, s);
         return launch_t<RPL_TRANSPORT_NONE, true>(p, stage, Nk, N0, out, s);
     }
     if (transport == RPL_TRANSPORT_REFRACTION)
-        return launch_phi<RPL_TRANSPORT_REFRACTION, false>(p, stage, Nk, N0, out, s);
+        return launch_phi<RPL_TRANSPORT_REFRACTION, SOURCE_NONE>(p, stage, Nk, N0, out, s);
     if (transport == RPL_TRANSPORT_INTRINSIC)
-        return launch_phi<RPL_TRANSPORT_INTRINSIC, false>(p, stage, Nk, N0, out, s);
+        return launch_phi<RPL_TRANSPORT_INTRINSIC, SOURCE_NONE>(p, stage, Nk, N0, out, s);
     return launch_t<RPL_TRANSPORT_NONE, false>(p, stage, Nk, N0, out, s);
 }
 

@@ -111,6 +111,7 @@ SIGNATURES = {
     "rpl_get_coupling_timing": ([_P, _DP, _I64P], ctypes.c_int),
     "rpl_get_smoothing_timing": ([_P, _DP, _I64P], ctypes.c_int),
     "rpl_get_nonlinear_timing": ([_P, _DP, _I64P], ctypes.c_int),
+    "rpl_get_source_timing": ([_P, _DP, _I64P], ctypes.c_int),
     "rpl_stream": ([_P], ctypes.c_void_p),
     "rpl_ensemble_create": ([ctypes.POINTER(_P), ctypes.POINTER(_P), _I32], ctypes.c_int),
     "rpl_ensemble_destroy": ([_P], ctypes.c_int),
