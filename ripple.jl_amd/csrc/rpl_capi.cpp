@@ -90,6 +90,11 @@ struct rpl_ctx {
     double *xn = nullptr, *yn = nullptr;                       // cell centres (FP64)
     void *U10 = nullptr, *theta_u = nullptr, *ustar = nullptr;  // per column
     void *factor = nullptr, *cm0 = nullptr, *kbar = nullptr, *sbar = nullptr;
+    // FP32 stress cap fused into the stage kernels: per (column, kappa bin)
+    // wave-supported stress sums of a stage's output, the per-bin weights,
+    // and the buffer those sums belong to (null: none pending)
+    void *tau_x = nullptr, *tau_y = nullptr, *tau_c = nullptr, *tau_s = nullptr;
+    const void *tau_for = nullptr;
     void *k_cell = nullptr, *theta_cell = nullptr, *sigma_cell = nullptr, *C_cell = nullptr;  // per bin
     void *t_phi = nullptr, *t_w = nullptr, *t_sigma_k = nullptr, *t_cos = nullptr, *t_sin = nullptr;
     void *t_kpow = nullptr, *t_spow = nullptr, *t_wc = nullptr;
@@ -243,6 +248,10 @@ rpl::StageParams<T> stage_params(const rpl_ctx *c, double dt)
             q.sin_tu = (const T *)c->f_stu;
             q.log_z1 = (const T *)c->f_lz1;
         }
+        if constexpr (std::is_same<T, float>::value) {
+            p.tau_c = (const T *)c->tau_c;
+            p.tau_s = (const T *)c->tau_s;
+        }
         if (c->s_active) {
             q.surrogate = 1;
             q.surrogate_terms = c->scfg.terms;
@@ -316,6 +325,21 @@ int launch_wind(rpl_ctx *c)
 
 int record_begin(rpl_ctx *c, int phase);
 int record_end(rpl_ctx *c);
+bool use_fast(const rpl_ctx *c);
+
+// Whether the stress cap of the next RK3 stage comes from the wave-supported
+// stress sums the stage kernels write for their output (FP32 fast kernels,
+// wind input with the cap, local-saturation or no dissipation, no network:
+// the stage kernel evaluates every cell's raw wind-input rate anyway), instead
+// of a second pass over the stage input.  The first stage of every step still
+// runs the column-state kernel: the wind changes between steps.
+bool fuse_cap(const rpl_ctx *c)
+{
+    return !is_f64(c) && use_fast(c) && c->has_sources && c->src.wind_input && c->src.stress_cap && !c->s_active &&
+           (c->src.dissipation == RPL_DISSIPATION_NONE || c->src.dissipation == RPL_DISSIPATION_LOCAL_SATURATION) &&
+           (c->cfg.transport == RPL_TRANSPORT_INTRINSIC || c->cfg.transport == RPL_TRANSPORT_REFRACTION) &&
+           c->cfg.timestepper != RPL_FORWARD_EULER && c->tau_x && c->tau_c;
+}
 
 // Bundle state of one stage input (prepare_sources, precomputed_sources.jl:230-238).
 int launch_state(rpl_ctx *c, const void *Nk)
@@ -331,6 +355,17 @@ int launch_state(rpl_ctx *c, const void *Nk)
     if (rc != RPL_OK)
         return rc;
     hipError_t e;
+    if (c->tau_for == Nk && fuse_cap(c)) {
+        // the previous stage kernel summed this input's wave-supported stress
+        c->tau_for = nullptr;
+        e = rpl::launch_stress_factor_from_sums(nx, ny, nk, (const float *)c->tau_x, (const float *)c->tau_y,
+                                                stage_params<float>(c, 0.0).src, state_tables<float>(c),
+                                                (float *)c->factor, c->stream);
+        if (e != hipSuccess)
+            return fail(RPL_ERR_HIP, std::string("stress factor kernel: ") + hipGetErrorString(e));
+        return record_end(c);
+    }
+    c->tau_for = nullptr;
     if (is_f64(c)) {
         auto p = stage_params<double>(c, 0.0);
         e = rpl::launch_source_state<double>(nx, ny, nk, np, (const double *)Nk, p.src, state_tables<double>(c),
@@ -579,6 +614,13 @@ int launch_stage(rpl_ctx *c, double dt, int stage, const void *Nk, const void *N
                                         c->stream);
     } else {
         auto p = stage_params<float>(c, dt);
+        // stages 1 and 2 sum their output's wave-supported stress for the next stage's cap
+        c->tau_for = nullptr;
+        if ((stage == rpl::STAGE_1 || stage == rpl::STAGE_2) && fuse_cap(c)) {
+            p.tau_x = (float *)c->tau_x;
+            p.tau_y = (float *)c->tau_y;
+            c->tau_for = out;
+        }
         if (use_fast(c))
             e = rpl::launch_stage_fast(p, tr, stage, (const float *)Nk, (const float *)N0, (float *)out, c->stream);
         else
@@ -786,6 +828,10 @@ int ensemble_descriptors(rpl_ensemble *e, double dt)
         for (int st = 0; st < 3; ++st) {
             rpl::EnsembleMember &d = h[(size_t)st * M + k];
             d.p = stage_params<float>(c, dt);
+            if (st < 2 && fuse_cap(c)) {  // as launch_stage: stages 1 and 2 sum their output's stress
+                d.p.tau_x = (float *)c->tau_x;
+                d.p.tau_y = (float *)c->tau_y;
+            }
             const void *Nk, *N0;
             void *out;
             stage_buffers(c, st + 1, &Nk, &N0, &out);
@@ -858,6 +904,11 @@ int ensemble_descriptors(rpl_ensemble *e, double dt)
             d.m0 = (float *)c->cm0;
             d.kbar = (float *)c->kbar;
             d.sbar = (float *)c->sbar;
+            d.tau_x = d.tau_y = nullptr;
+            if (st > 0 && fuse_cap(c)) {  // stages 2 and 3: the factor from the sums
+                d.tau_x = (const float *)c->tau_x;
+                d.tau_y = (const float *)c->tau_y;
+            }
             d.blk0 = b0;
         }
         b0 += (long long)((c->plane + 63) / 64);
@@ -1041,6 +1092,7 @@ int rpl_destroy(rpl_ctx *c)
                     (void *)c->yn, c->U10, c->theta_u, c->ustar, c->factor, c->cm0, c->kbar, c->sbar, c->k_cell,
                     c->theta_cell, c->sigma_cell, c->C_cell, c->t_phi, c->t_w, c->t_sigma_k, c->t_cos, c->t_sin,
                     c->t_kpow, c->t_spow, c->t_wc, c->f_cth, c->f_sth, c->f_logk, c->f_invc, c->f_sk3, c->f_sq,
+                    c->tau_x, c->tau_y, c->tau_c, c->tau_s,
                     c->f_logkap, c->f_invck, c->f_ctu, c->f_stu, c->f_lz1, (void *)c->sX, (void *)c->sA, (void *)c->sB, (void *)c->rate,
                     (void *)c->out_scale, (void *)c->s_err, (void *)c->use_plain, (void *)c->s_counts,
                     (void *)c->bin_hi, (void *)c->s_amax, (void *)c->s_flags, c->dUx, c->dUy, c->pm_O,
@@ -2129,6 +2181,26 @@ int rpl_set_sources(rpl_ctx *c, const rpl_sources *src, const double *x_nodes, c
         for (int m = 0; m < nk; ++m) {
             flkap[m] = c->h_kc[m] > 0 ? std::log(c->h_kc[m]) : 0.0;
             fick[m] = sk[m] > 0 ? c->h_kc[m] / sk[m] : 0.0;
+        }
+        if (s.wind_input && s.stress_cap) {
+            // stress-cap weights of the fused path: the state kernel's
+            // (k / sigma) w rho_w g per bin times cos and sin of the direction
+            std::vector<double> tc(tab), ts(tab);
+            for (size_t t = 0; t < tab; ++t) {
+                const double wt = fic[t] * w[t] * (s.rho_water * s.gravity);
+                tc[t] = wt * fc[t];
+                ts[t] = wt * fs[t];
+            }
+            void **tt[] = {&c->tau_c, &c->tau_s};
+            for (void **b : tt)
+                if (!*b && (rc = alloc(b, tab * sizeof(float))))
+                    return rc;
+            void **tp[] = {&c->tau_x, &c->tau_y};
+            for (void **b : tp)
+                if (!*b && (rc = alloc(b, plane * (size_t)nk * sizeof(float))))
+                    return rc;
+            if ((rc = upload_real(c, tc.data(), c->tau_c, tab)) || (rc = upload_real(c, ts.data(), c->tau_s, tab)))
+                return rc;
         }
         if ((rc = upload_real(c, fc.data(), c->f_cth, tab)) || (rc = upload_real(c, fs.data(), c->f_sth, tab)) ||
             (rc = upload_real(c, flk.data(), c->f_logk, tab)) || (rc = upload_real(c, fic.data(), c->f_invc, tab)) ||

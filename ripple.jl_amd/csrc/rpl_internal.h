@@ -28,6 +28,12 @@ struct StageParams {
     T dt;                             // time step in T (time_step.jl:16-20)
     const T *src_add;                 // optional additive tendency (nx,ny,nk,nphi) or null
     SourceParams<T> src;              // source terms (src.on == 0: none)
+    // Wave-supported stress of the stage's OUTPUT for the next stage's
+    // stress cap (FP32 wind-input bundle): when tau_x is set, the kernel
+    // writes per (column, kappa bin) the sums over phi of raw * N_out *
+    // tau_c[t] (and tau_s) into tau_x / tau_y, (nx, ny, nk) planes.
+    T *tau_x, *tau_y;
+    const T *tau_c, *tau_s;           // (k / sigma) w rho_w g (cos theta, sin theta) per bin
 };
 
 // One ensemble member's stage launch in the work-list kernel: its stage
@@ -65,14 +71,26 @@ struct StateTables {
 
 // One member's column-state launch in an ensemble (FP32): inputs, bundle
 // parameters, outputs, and its first 64-column block in the work list.
+// tau_x set: the stress factor comes from the previous stage's
+// wave-supported stress sums (stress_factor_from_sums) instead of a pass
+// over N.
 struct EnsembleState {
     int nx, ny, nk, nphi;
     const float *N;
     SourceParams<float> s;
     StateTables<float> tb;
     float *factor, *m0, *kbar, *sbar;
+    const float *tau_x, *tau_y;
     long long blk0;
 };
+
+// The stress factor of every column from the per-kappa wave-supported stress
+// sums a stage kernel wrote for its output (StageParams::tau_x / tau_y),
+// summed over kappa in bin order: the bundle cap (precomputed_sources.jl:
+// 126-170) without a second pass over N.
+hipError_t launch_stress_factor_from_sums(int nx, int ny, int nk, const float *tau_x, const float *tau_y,
+                                          const SourceParams<float> &s, const StateTables<float> &tb, float *factor,
+                                          hipStream_t st);
 
 
 template <typename T>

@@ -296,6 +296,37 @@ __device__ __forceinline__ void state_block_f32(int nx, int ny, int nk, int nphi
     }
 }
 
+// The bundle's stress cap of one column (precomputed_sources.jl:126-170)
+// from the per-kappa wave-supported stress sums of a stage kernel, summed
+// over kappa in bin order.  Columns without wind have zero sums (factor 1).
+__device__ __forceinline__ void stress_factor_column(size_t ij, size_t plane, int nk, const float *__restrict__ tx,
+                                                     const float *__restrict__ ty, const SourceParams<float> &s,
+                                                     const StateTables<float> &tb, float *factor)
+{
+    float twx = 0.0f, twy = 0.0f;
+    for (int m = 0; m < nk; ++m) {
+        twx += gld<true>(tx, ij + plane * (size_t)m);
+        twy += gld<true>(ty, ij + plane * (size_t)m);
+    }
+    const float ustar = gld<true>(s.ustar, ij);
+    const float ctu = gld<true>(s.cos_tu, ij), stu = gld<true>(s.sin_tu, ij);
+    const float tau_max = tb.rho_air * (ustar * ustar);
+    const float par = twx * ctu + twy * stu;
+    const float eff = par > 0.0f ? par : 0.0f;
+    const float f = (eff > tb.cap_eps * tau_max && tau_max > 0.0f) ? tb.cap_eps * tau_max / eff : 1.0f;
+    gp_store(factor, ij, f);
+}
+
+__global__ __launch_bounds__(256) void stress_factor_kernel(int nx, int ny, int nk, const float *__restrict__ tx,
+                                                            const float *__restrict__ ty, SourceParams<float> s,
+                                                            StateTables<float> tb, float *factor)
+{
+    const size_t plane = (size_t)nx * ny;
+    const size_t ij = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (ij < plane)
+        stress_factor_column(ij, plane, nk, tx, ty, s, tb, factor);
+}
+
 __global__ __launch_bounds__(256) void source_state_f32_kernel(int nx, int ny, int nk, int nphi,
                                                                const float *__restrict__ N, SourceParams<float> s,
                                                                StateTables<float> tb, float *factor, float *m0o,
@@ -319,6 +350,12 @@ __global__ __launch_bounds__(256) void ensemble_state_kernel(const EnsembleState
     }
     const EnsembleState &e = mem[__builtin_amdgcn_readfirstlane(u)];
     const size_t col0 = (size_t)__builtin_amdgcn_readfirstlane((int)(b - e.blk0)) * 64;
+    if (e.tau_x) {  // the stress factor from the previous stage's sums
+        const size_t plane = (size_t)e.nx * e.ny, ij = col0 + threadIdx.x;
+        if (threadIdx.x < 64 && ij < plane)
+            stress_factor_column(ij, plane, e.nk, e.tau_x, e.tau_y, e.s, e.tb, e.factor);
+        return;
+    }
     state_block_f32(e.nx, e.ny, e.nk, e.nphi, e.N, e.s, e.tb, e.factor, e.m0, e.kbar, e.sbar, col0);
 }
 
@@ -394,6 +431,16 @@ template hipError_t launch_wind_columns<float>(int, int, const double *, const d
 template hipError_t launch_wind_columns<double>(int, int, const double *, const double *, double, const WindParams &,
                                                double *, double *, double *, double *, double *, double *,
                                                hipStream_t);
+hipError_t launch_stress_factor_from_sums(int nx, int ny, int nk, const float *tau_x, const float *tau_y,
+                                          const SourceParams<float> &sp, const StateTables<float> &tb, float *factor,
+                                          hipStream_t s)
+{
+    const size_t plane = (size_t)nx * ny;
+    hipLaunchKernelGGL(stress_factor_kernel, dim3((unsigned)((plane + 255) / 256)), dim3(256), 0, s, nx, ny, nk,
+                       tau_x, tau_y, sp, tb, factor);
+    return hipGetLastError();
+}
+
 hipError_t launch_ensemble_state(const EnsembleState *mem, int members, long long blocks, hipStream_t s)
 {
     if (members < 1 || blocks < 1 || blocks > 0x7fffffffLL)

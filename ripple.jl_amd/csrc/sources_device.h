@@ -230,7 +230,7 @@ __device__ __forceinline__ SourceBin<T> source_bin(const SourceParams<T> &s, int
 // positive - damping * a.
 template <typename T, bool AS1 = false>
 __device__ __forceinline__ T source_term_cb(const SourceParams<T> &s, const SourceColumn<T> &c, const SourceBin<T> &b,
-                                            unsigned col, int t, T a)
+                                            unsigned col, int t, T a, T *raw_out = nullptr)
 {
 #pragma clang fp contract(off)
     const bool net = c.net;
@@ -250,6 +250,8 @@ __device__ __forceinline__ T source_term_cb(const SourceParams<T> &s, const Sour
                 raw = jp_rate_f32(s, b.cth, b.sth, b.logk, b.invc, b.sigma, c.cos_tu, c.sin_tu, c.log_z1, ustar);
             else
                 raw = jp_rate<T>(s, k, b.sigma, b.C, b.theta, c.theta_u, ustar);
+            if (raw_out)
+                *raw_out = raw;
             if (raw > T(0)) {
                 const T rate = s.stress_cap ? raw * c.factor : raw;
                 if (rate >= T(0))
@@ -356,13 +358,15 @@ __device__ __forceinline__ JpColumn jp_column(const SourceParams<float> &s, unsi
 // stress factor plus local-saturation dissipation (precomputed_sources.jl:
 // 196-218, local_saturation.jl:18-52).
 __device__ __forceinline__ float source_jp_ls(const SourceParams<float> &s, const JpColumn &c,
-                                              const SourceBin<float> &b, float a)
+                                              const SourceBin<float> &b, float a, float &raw_out)
 {
 #pragma clang fp contract(off)
     float positive = 0.0f, damping = 0.0f;
     const float k = b.k;
+    raw_out = 0.0f;  // the raw wind-input rate (0 outside the growth window)
     if (s.wind_input && c.us > 0.0f && k > 0.0f) {
         const float raw = jp_rate_f32(s, b.cth, b.sth, b.logk, b.invc, b.sigma, c.ctu, c.stu, c.lz1, c.us);
+        raw_out = raw;
         if (raw > 0.0f) {
             const float rate = s.stress_cap ? raw * c.fac : raw;
             if (rate >= 0.0f)

@@ -529,6 +529,7 @@ __device__ __forceinline__ void phi_tile(const float *__restrict__ Nk, const flo
     }
     Face6 carry = {0.0f, 0.0f};  // phi face n - 1/2 (as (v, f) of weno_face_of5)
     bool cpos = true;            // its upwind side
+    float twx = 0.0f, twy = 0.0f;  // wave-supported stress sums of the output (SOURCE_JP_LS)
     for (int n = 0; n < nphi; ++n) {
         BinLoads cur;
         load_bin(n, cur);
@@ -572,10 +573,11 @@ __device__ __forceinline__ void phi_tile(const float *__restrict__ Nk, const flo
             L = L - fmaf(dk, p.dk_inv, dp * p.dp_inv);
         }
         // tendencies.jl:88-94, G += source_tendency
+        float raw = 0.0f;
         if constexpr (SK == SOURCE_JP_LS)
-            L += source_jp_ls(p.src, jcol, cur.sb, x0);
+            L += source_jp_ls(p.src, jcol, cur.sb, x0, raw);
         else if constexpr (SK == SOURCE_GENERIC)
-            L += source_term_cb<float, AS1>(p.src, scol, cur.sb, off, m + nk * n, x0);
+            L += source_term_cb<float, AS1>(p.src, scol, cur.sb, off, m + nk * n, x0, &raw);
         if (p.src_add)
             L += cur.add;
         float r;
@@ -590,11 +592,27 @@ __device__ __forceinline__ void phi_tile(const float *__restrict__ Nk, const flo
             r = clip0(fmaf(a, cur.n0, b * st));
         }
         sto<AS1>(out + plane * bin, bo, r);
+        if constexpr (SK != SOURCE_NONE) {
+            // the output's wave-supported stress for the next stage's cap:
+            // raw is this bin's rate at this column's wind, r the next input
+            // (both source paths give raw the same bits)
+            if (p.tau_x) {
+                const float mf = raw * r;
+                twx = fmaf(mf, sld(p.tau_c, bin), twx);
+                twy = fmaf(mf, sld(p.tau_s, bin), twy);
+            }
+        }
         if constexpr (REF) {
 #pragma unroll
             for (int d = 0; d < 6; ++d)
                 w[d] = w[d + 1];
             w[6] = cur.wnext;
+        }
+    }
+    if constexpr (SK != SOURCE_NONE) {
+        if (p.tau_x) {
+            sto<AS1>(p.tau_x + plane * (size_t)m, bo, twx);
+            sto<AS1>(p.tau_y + plane * (size_t)m, bo, twy);
         }
     }
 }

@@ -170,3 +170,37 @@ def test_fetch_limited_growth_full_size_properties():
     say(f"fetch-limited: mean Hs {hs0:.4f} -> {hs:.4f} after 10 min")
     assert hs > hs0
     assert abs(float(np.mean(ra.mean_direction(m)))) < 0.05
+
+
+def test_fused_stress_cap_fp32(oracle):
+    """The FP32 fast kernels take the stress cap of RK3 stages 2 and 3 from
+    the wave-supported stress their previous stage summed for its output
+    (stage 1 runs the column-state kernel).  On a strong sea under 25 m/s,
+    where the cap scales the growth down, the FP32 run matches the FP32
+    reference-order kernel (column-state kernel every stage) to rounding and
+    the FP64 oracle to the FP32 tolerance."""
+    say("fused stress cap fp32")
+    g = grids.RectilinearGrid(size=(70, 9, 2), x=(0, 70e3), y=(0, 9e3), z=(-50.0, 0.0))
+    sg = grids.PolarWaveVectorGrid(kappa=np.geomspace(0.02, 0.6, 10), phi=np.linspace(0, 2 * math.pi, 13)[:12])
+    inp = ra.PressureCorrelationInput(drag=ra.BulkWindDrag("linear"), wind=25.0, direction=0.4)
+    src = ra.PrecomputedSources(wind_input=inp, dissipation=ra.LocalSaturationDissipation(B_r=1.05e-2))
+    rng = np.random.default_rng(11)
+    N0 = np.asfortranarray(40.0 * (1 + 0.5 * rng.random((70, 9, 10, 12))))
+    wl = workloads.Workload("cap", g, sg, None, N0, 20.0, 3, dict(timestepper="RK3", sources=src))
+    # the cap is active: the capped growth is well below the raw growth
+    om = oracle_model(oracle, wl)
+    raw = oracle.OracleSourceModel(wl, oracle.sources_from(to_struct(inp)), transport=oracle.MODE_NONE).tendency()
+    capped = oracle.OracleSourceModel(
+        wl, oracle.sources_from(to_struct(ra.PrecomputedSources(wind_input=inp))),
+        transport=oracle.MODE_NONE).tendency()
+    assert capped.sum() < 0.7 * raw.sum()
+    fast = device(wl, np.float32, kernel="fast")
+    ref = device(wl, np.float32, kernel="reference")
+    for _ in range(3):
+        ra.time_step_(fast, wl.dt)
+        ra.time_step_(ref, wl.dt)
+        om.step(wl.dt)
+    d_ref = rel_l2(fast.action, ref.action)
+    d_orc = rel_l2(fast.action, om.action)
+    say(f"fused cap fp32: vs fp32 reference kernel {d_ref:.2e}, vs fp64 oracle {d_orc:.2e}")
+    assert d_ref <= 2e-5 and d_orc <= 1e-4

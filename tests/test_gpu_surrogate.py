@@ -168,7 +168,11 @@ def _model(wl, float_type, sources=True):
 @pytest.mark.parametrize("float_type", [np.float32, np.float64])
 def test_fallbacks_reproduce_the_plain_bundle(float_type):
     """Envelope fallback on every column, and failed audits on every column,
-    both give the plain-source run exactly; the counters say so."""
+    both give the plain-source run; the counters say so.  FP64 is exact.  In
+    FP32 the plain run takes the stress cap of RK3 stages 2 and 3 from the
+    sums its stage kernels form (another summation order than the
+    column-state kernel the network path runs), so the two agree to FP32
+    rounding of the cap factor."""
     say(f"surrogate fallbacks {np.dtype(float_type).name}")
     wl = _storm(float_type)
     nb = wl.spectral_grid.kappa.size * wl.spectral_grid.phi.size
@@ -189,7 +193,10 @@ def test_fallbacks_reproduce_the_plain_bundle(float_type):
         else:
             assert st["audited"] == 9 * plane and st["plain"] == 0
             assert st["failed"] >= 9 * plane - 9 * np.count_nonzero(np.all(wl.N0 == 0, axis=(2, 3)))
-        assert np.array_equal(m.action, ref.action), rel_l2(m.action, ref.action)
+        if float_type == np.float64:
+            assert np.array_equal(m.action, ref.action), rel_l2(m.action, ref.action)
+        else:
+            assert rel_l2(m.action, ref.action) <= 1e-6, rel_l2(m.action, ref.action)
         sg.attach_surrogate(m, None, None)
     net.close()
 
