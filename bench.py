@@ -157,18 +157,18 @@ def storm_plain_vs_surrogate(steps, warmup, nx=1024, ny=1024):
             "step_speedup_surrogate_over_plain": t_plain / t_surr}
 
 
-def refraction_line(steps, warmup, nx, ny, float_type=np.float32):
+def refraction_line(steps, warmup, nx, ny, float_type=np.float32, kernel="auto"):
     """The sheared-eddy refraction configuration at nx x ny x 32 kappa x 36
     phi (fused WENO5 Doppler transport + spectral refraction, no sources):
     RK3 step time and the stage kernel's roofline."""
     wl = workloads.sheared_eddies(nx=nx, ny=ny)
-    m = build_model(wl, float_type)
+    m = build_model(wl, float_type, kernel=kernel)
     esize = np.dtype(float_type).itemsize
     t, l, n = time_model(m, wl.dt, steps, warmup)
     nk, nphi = wl.N0.shape[2], wl.N0.shape[3]
     bpl = stage_bytes_per_launch(nx, ny, nk, nphi, esize)
     ach = bpl / (l * 1e-3) / 1e9
-    return {"workload": wl.name, "dtype": np.dtype(float_type).name, "value": wl.cell_bins / t,
+    return {"workload": wl.name, "dtype": np.dtype(float_type).name, "kernel": kernel, "value": wl.cell_bins / t,
             "ms_per_step": t * 1e3, "avg_launch_ms": l, "launches_timed": n, "bytes_per_launch": bpl,
             "achieved_GBs": ach, "frac_of_hbm_peak": ach / HBM_PEAK_GBS,
             "traffic": measured_traffic(wl.name) if float_type == np.float32 else None}
@@ -454,6 +454,7 @@ def main():
                     help="print only the plain vs surrogate hurricane line (for profiling its kernels)")
     ap.add_argument("--only-refraction", type=int, default=0, metavar="N",
                     help="print only the N x N sheared-eddy refraction line (for profiling its kernel)")
+    ap.add_argument("--fp64-fast", action="store_true", help="with --only-refraction: FP64 on the fast kernel")
     ap.add_argument("--ensemble-members", type=int, default=64)
     ap.add_argument("--performance-summary", default=None,
                     help="also write the measured lines as a reference-format performance summary (TSV)")
@@ -489,7 +490,8 @@ def main():
         return
     if args.only_refraction:
         n = args.only_refraction
-        print(json.dumps({"refraction": refraction_line(args.steps, args.warmup, n, n)}), flush=True)
+        ft, kern = (np.float64, "fast") if args.fp64_fast else (np.float32, "auto")
+        print(json.dumps({"refraction": refraction_line(args.steps, args.warmup, n, n, ft, kern)}), flush=True)
         return
 
     st = storm_plain_vs_surrogate(args.steps, args.warmup)
@@ -533,6 +535,8 @@ def main():
         extra["refraction_1024x1024"] = refraction_line(max(3, args.steps // 2), 1, 1024, 1024)
         extra["refraction_512x512"] = refraction_line(args.steps, args.warmup, 512, 512)
         extra["refraction_512x512_fp64"] = refraction_line(max(3, args.steps // 4), 1, 512, 512, np.float64)
+        extra["refraction_512x512_fp64_fast"] = refraction_line(max(3, args.steps // 2), 1, 512, 512, np.float64,
+                                                                kernel="fast")
         extra["err_vs_fp64"] = err_vs_fp64()
         extra["err_vs_fp64_sheared_eddies"] = err_vs_fp64_sheared_eddies()
         extra["pseudomomentum_coupling"] = pseudomomentum_line(max(3, args.steps // 2), 1)

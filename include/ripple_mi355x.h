@@ -55,10 +55,12 @@ enum { RPL_DEPTH_INFINITE = 0, RPL_DEPTH_CONSTANT = 1 };
 
 /* Implementation selector.  REFERENCE evaluates the reference formulas in
  * their order (the FP64 parity path, bit-identical to the CPU restatement);
- * FAST is the restructured FP32 stencil.  The Doppler contraction of
- * rpl_set_current runs on the matrix cores (MFMA) for FP32 always and for FP64
- * under FAST; AUTO is FAST for FP32 and REFERENCE for FP64.  An FP64 context
- * always uses the reference-order stencil. */
+ * FAST is the restructured stencil (one upwind reconstruction per face,
+ * shared smoothness indicators, the phi bins walked inside the workgroup),
+ * in FP32 and, for refraction and intrinsic transport, in FP64, where it
+ * agrees with the reference order to rounding.  The Doppler contraction of
+ * rpl_set_current runs on the matrix cores (MFMA) for FP32 always and for
+ * FP64 under FAST; AUTO is FAST for FP32 and REFERENCE for FP64. */
 enum { RPL_KERNEL_AUTO = 0, RPL_KERNEL_REFERENCE = 1, RPL_KERNEL_FAST = 2 };
 
 typedef struct rpl_config {

@@ -610,8 +610,15 @@ int launch_stage(rpl_ctx *c, double dt, int stage, const void *Nk, const void *N
     const int tr = c->cfg.transport;
     if (is_f64(c)) {
         auto p = stage_params<double>(c, dt);
-        e = rpl::launch_stage_reference(p, tr, stage, (const double *)Nk, (const double *)N0, (double *)out,
-                                        c->stream);
+        // FP64 under RPL_KERNEL_FAST: the phi-loop kernel (rounding-level
+        // differences from the reference order); slabs of 2^29 cells and
+        // more stay on the reference-order kernel (32-bit byte offsets)
+        if (c->cfg.kernel == RPL_KERNEL_FAST && tr != RPL_TRANSPORT_NONE && c->plane < ((size_t)1 << 29))
+            e = rpl::launch_stage_fast(p, tr, stage, (const double *)Nk, (const double *)N0, (double *)out,
+                                       c->stream);
+        else
+            e = rpl::launch_stage_reference(p, tr, stage, (const double *)Nk, (const double *)N0, (double *)out,
+                                            c->stream);
     } else {
         auto p = stage_params<float>(c, dt);
         // stages 1 and 2 sum their output's wave-supported stress for the next stage's cap

@@ -126,6 +126,10 @@ hipError_t launch_ensemble_stage(const EnsembleMember *mem, int members, long lo
                                  int transport, int src_kind, int stage, int ty, int blocks, hipStream_t s);
 hipError_t launch_stage_fast(const StageParams<float> &p, int transport, int stage,
                              const float *Nk, const float *N0, float *out, hipStream_t s);
+// The phi-loop stage in FP64 (refraction and intrinsic transport): the FP32
+// kernel's restructured arithmetic, not the reference operation order.
+hipError_t launch_stage_fast(const StageParams<double> &p, int transport, int stage, const double *Nk,
+                             const double *N0, double *out, hipStream_t s);
 
 // Coupling: U[i,j,m] = sum_k u[i,j,k] W[k,m] (W is (nz, nk), k fastest) and the
 // centred periodic gradients of U.

@@ -28,7 +28,36 @@
 namespace rpl {
 namespace {
 
+// Element-type helpers.  The FP32 forms are the instructions the kernels
+// were built on; FP64 takes the hardware reciprocal with one Newton step
+// (e = 1 - x r, r + r e), which brings it to the last bit or two.
+__device__ __forceinline__ float fm(float a, float b, float c) { return fmaf(a, b, c); }
+__device__ __forceinline__ double fm(double a, double b, double c) { return fma(a, b, c); }
+__device__ __forceinline__ float fab(float x) { return fabsf(x); }
+__device__ __forceinline__ double fab(double x) { return fabs(x); }
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double frcp(double x)
+{
+    const double r = __builtin_amdgcn_rcp(x);
+    return fma(r, fma(-x, r, 1.0), r);
+}
+
+// WENO5 constants of the element type: 4 * epsilon with the reference's
+// epsilon 1e4 eps(FT) (wave_current_refraction.jl:43; the smoothness
+// indicators are carried times 4) and the largest denominator the product
+// form may reach before the ratio form takes over.
+template <typename T>
+struct Weno;
+template <>
+struct Weno<float> {
+    static constexpr float eps4 = 4.0f * (10000.0f * 1.1920928955078125e-7f);
+    static constexpr float big = 3.0e38f;
+};
+template <>
+struct Weno<double> {
+    static constexpr double eps4 = 4.0 * (10000.0 * 2.220446049250313080847263336181640625e-16);
+    static constexpr double big = 1.0e307;
+};
 
 // The two upwind-left WENO5 face values of one cell along one direction.
 //
@@ -45,60 +74,60 @@ __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x)
 // denominators, so one reciprocal replaces the reference's four divides.  The
 // smoothness indicators are scaled by 4 together with epsilon, which leaves
 // the weights unchanged, and every value is carried times 6 (the 1/6 is
-// applied once to the flux).  If the product form overflows (differences of
-// the action above ~1e4) the lane falls back to the ratio form.
+// applied once to the flux).  If the product form overflows (FP32:
+// differences of the action above ~1e4) the lane falls back to the ratio form.
 //
 // weno_face6 returns 6 * (value - v1) for the face with smoothness q0, q1, q2
 // and third differences U0 = D_l - D_c, U1 = D_c - D_r.
-__device__ __forceinline__ float weno_face6(float q0, float q1, float q2, float U0, float U1)
+template <typename T>
+__device__ __forceinline__ T weno_face6(T q0, T q1, T q2, T U0, T U1)
 {
-    const float s0 = q0 * q0, s1 = q1 * q1, s2 = q2 * q2;
+    const T s0 = q0 * q0, s1 = q1 * q1, s2 = q2 * q2;
     // 6 (v0 - v1) = 2 U0, 6 (v2 - v1) = U1; weights 1 : 6 : 3 over s0, s1, s2.
-    float num = s1 * fmaf(s2 * 2.0f, U0, 3.0f * s0 * U1);
-    float den = fmaf(s2, fmaf(6.0f, s0, s1), 3.0f * s0 * s1);
-    if (!(den <= 3.0e38f)) {
-        const float r0 = q1 * frcp(q0), r2 = q1 * frcp(q2);
-        const float t0 = r0 * r0, t2 = r2 * r2;
-        num = fmaf(2.0f * t0, U0, 3.0f * t2 * U1);
-        den = fmaf(6.0f, t0, fmaf(18.0f, t2, 36.0f)) * (1.0f / 6.0f);
+    T num = s1 * fm(s2 * T(2), U0, T(3) * s0 * U1);
+    T den = fm(s2, fm(T(6), s0, s1), T(3) * s0 * s1);
+    if (!(den <= Weno<T>::big)) {
+        const T r0 = q1 * frcp(q0), r2 = q1 * frcp(q2);
+        const T t0 = r0 * r0, t2 = r2 * r2;
+        num = fm(T(2) * t0, U0, T(3) * t2 * U1);
+        den = fm(T(6), t0, fm(T(18), t2, T(36))) * (T(1) / T(6));
     }
     return num * frcp(den);
 }
 
 // 6 * (B - A) for the window w0..w5, and optionally A and B themselves.
-template <bool VALUES>
-__device__ __forceinline__ float weno_pair6(float w0, float w1, float w2, float w3, float w4, float w5, float *A,
-                                           float *B)
+template <bool VALUES, typename T>
+__device__ __forceinline__ T weno_pair6(T w0, T w1, T w2, T w3, T w4, T w5, T *A, T *B)
 {
-    const float eps4 = 4.0f * (10000.0f * 1.1920928955078125e-7f);  // 4 * 1e4 eps(Float32)
-    const float c13_3 = 13.0f / 3.0f;
-    const float g0 = w1 - w0, g1 = w2 - w1, g2 = w3 - w2, g3 = w4 - w3, g4 = w5 - w4;
-    const float D1 = g1 - g0, D2 = g2 - g1, D3 = g3 - g2, D4 = g4 - g3;
-    const float t1 = fmaf(c13_3 * D1, D1, eps4), t2 = fmaf(c13_3 * D2, D2, eps4);
-    const float t3 = fmaf(c13_3 * D3, D3, eps4), t4 = fmaf(c13_3 * D4, D4, eps4);
+    const T eps4 = Weno<T>::eps4;
+    const T c13_3 = T(13) / T(3);
+    const T g0 = w1 - w0, g1 = w2 - w1, g2 = w3 - w2, g3 = w4 - w3, g4 = w5 - w4;
+    const T D1 = g1 - g0, D2 = g2 - g1, D3 = g3 - g2, D4 = g4 - g3;
+    const T t1 = fm(c13_3 * D1, D1, eps4), t2 = fm(c13_3 * D2, D2, eps4);
+    const T t3 = fm(c13_3 * D3, D3, eps4), t4 = fm(c13_3 * D4, D4, eps4);
     // beta0 of a face uses the right-edge form on its first triplet, beta1 the
     // centred form, beta2 the left-edge form on its last triplet.
-    const float eA1 = fmaf(2.0f, g1, D1), eA2 = fmaf(2.0f, g2, D2);
-    const float eC2 = g1 + g2, eC3 = g2 + g3;
-    const float eB3 = fmaf(-2.0f, g2, D3), eB4 = fmaf(-2.0f, g3, D4);
-    const float qA1 = fmaf(eA1, eA1, t1), qA2 = fmaf(eA2, eA2, t2);
-    const float qC2 = fmaf(eC2, eC2, t2), qC3 = fmaf(eC3, eC3, t3);
-    const float qB3 = fmaf(eB3, eB3, t3), qB4 = fmaf(eB4, eB4, t4);
-    const float U1 = D1 - D2, U2 = D2 - D3, U3 = D3 - D4;
-    const float fa = weno_face6(qA1, qC2, qB3, U1, U2);
-    const float fb = weno_face6(qA2, qC3, qB4, U2, U3);
+    const T eA1 = fm(T(2), g1, D1), eA2 = fm(T(2), g2, D2);
+    const T eC2 = g1 + g2, eC3 = g2 + g3;
+    const T eB3 = fm(T(-2), g2, D3), eB4 = fm(T(-2), g3, D4);
+    const T qA1 = fm(eA1, eA1, t1), qA2 = fm(eA2, eA2, t2);
+    const T qC2 = fm(eC2, eC2, t2), qC3 = fm(eC3, eC3, t3);
+    const T qB3 = fm(eB3, eB3, t3), qB4 = fm(eB4, eB4, t4);
+    const T U1 = D1 - D2, U2 = D2 - D3, U3 = D3 - D4;
+    const T fa = weno_face6(qA1, qC2, qB3, U1, U2);
+    const T fb = weno_face6(qA2, qC3, qB4, U2, U3);
     // 6 v1 of each face: 5 w_c + 2 w_r - w_l.
-    const float va = fmaf(5.0f, w2, fmaf(2.0f, w3, -w1));
-    const float vb = fmaf(5.0f, w3, fmaf(2.0f, w4, -w2));
+    const T va = fm(T(5), w2, fm(T(2), w3, -w1));
+    const T vb = fm(T(5), w3, fm(T(2), w4, -w2));
     if (VALUES) {
-        *A = (va + fa) * (1.0f / 6.0f);
-        *B = (vb + fb) * (1.0f / 6.0f);
+        *A = (va + fa) * (T(1) / T(6));
+        *B = (vb + fb) * (T(1) / T(6));
     }
     return (vb - va) + (fb - fa);
 }
 
-__device__ __forceinline__ void weno_pair(float w0, float w1, float w2, float w3, float w4, float w5, float &A,
-                                          float &B)
+template <typename T>
+__device__ __forceinline__ void weno_pair(T w0, T w1, T w2, T w3, T w4, T w5, T &A, T &B)
 {
     (void)weno_pair6<true>(w0, w1, w2, w3, w4, w5, &A, &B);
 }
@@ -108,110 +137,114 @@ __device__ __forceinline__ void weno_pair(float w0, float w1, float w2, float w3
 // its A face on w0..w4 (and for its B face on w1..w5), in the same order, so a
 // face carried from one cell to its neighbour has the same bits as the pair
 // evaluated at the neighbour.
+template <typename T>
 struct Face6 {
-    float v, f;
+    T v, f;
 };
 
-__device__ __forceinline__ Face6 weno_face_of5(float u0, float u1, float u2, float u3, float u4)
+template <typename T>
+__device__ __forceinline__ Face6<T> weno_face_of5(T u0, T u1, T u2, T u3, T u4)
 {
-    const float eps4 = 4.0f * (10000.0f * 1.1920928955078125e-7f);
-    const float c13_3 = 13.0f / 3.0f;
-    const float h0 = u1 - u0, h1 = u2 - u1, h2 = u3 - u2, h3 = u4 - u3;
-    const float E1 = h1 - h0, E2 = h2 - h1, E3 = h3 - h2;
-    const float s1 = fmaf(c13_3 * E1, E1, eps4), s2 = fmaf(c13_3 * E2, E2, eps4);
-    const float s3 = fmaf(c13_3 * E3, E3, eps4);
-    const float eA = fmaf(2.0f, h1, E1), eC = h1 + h2, eB = fmaf(-2.0f, h2, E3);
-    const float qA = fmaf(eA, eA, s1), qC = fmaf(eC, eC, s2), qB = fmaf(eB, eB, s3);
-    Face6 r;
+    const T eps4 = Weno<T>::eps4;
+    const T c13_3 = T(13) / T(3);
+    const T h0 = u1 - u0, h1 = u2 - u1, h2 = u3 - u2, h3 = u4 - u3;
+    const T E1 = h1 - h0, E2 = h2 - h1, E3 = h3 - h2;
+    const T s1 = fm(c13_3 * E1, E1, eps4), s2 = fm(c13_3 * E2, E2, eps4);
+    const T s3 = fm(c13_3 * E3, E3, eps4);
+    const T eA = fm(T(2), h1, E1), eC = h1 + h2, eB = fm(T(-2), h2, E3);
+    const T qA = fm(eA, eA, s1), qC = fm(eC, eC, s2), qB = fm(eB, eB, s3);
+    Face6<T> r;
     r.f = weno_face6(qA, qC, qB, E1 - E2, E2 - E3);
-    r.v = fmaf(5.0f, u2, fmaf(2.0f, u3, -u1));
+    r.v = fm(T(5), u2, fm(T(2), u3, -u1));
     return r;
 }
 
 // weno_pair6's two faces as (v, f) parts; (b.v - a.v) + (b.f - a.f) is its
 // return value.
-__device__ __forceinline__ void weno_pair_parts(float w0, float w1, float w2, float w3, float w4, float w5, Face6 &a,
-                                                Face6 &b)
+template <typename T>
+__device__ __forceinline__ void weno_pair_parts(T w0, T w1, T w2, T w3, T w4, T w5, Face6<T> &a, Face6<T> &b)
 {
-    const float eps4 = 4.0f * (10000.0f * 1.1920928955078125e-7f);
-    const float c13_3 = 13.0f / 3.0f;
-    const float g0 = w1 - w0, g1 = w2 - w1, g2 = w3 - w2, g3 = w4 - w3, g4 = w5 - w4;
-    const float D1 = g1 - g0, D2 = g2 - g1, D3 = g3 - g2, D4 = g4 - g3;
-    const float t1 = fmaf(c13_3 * D1, D1, eps4), t2 = fmaf(c13_3 * D2, D2, eps4);
-    const float t3 = fmaf(c13_3 * D3, D3, eps4), t4 = fmaf(c13_3 * D4, D4, eps4);
-    const float eA1 = fmaf(2.0f, g1, D1), eA2 = fmaf(2.0f, g2, D2);
-    const float eC2 = g1 + g2, eC3 = g2 + g3;
-    const float eB3 = fmaf(-2.0f, g2, D3), eB4 = fmaf(-2.0f, g3, D4);
-    const float qA1 = fmaf(eA1, eA1, t1), qA2 = fmaf(eA2, eA2, t2);
-    const float qC2 = fmaf(eC2, eC2, t2), qC3 = fmaf(eC3, eC3, t3);
-    const float qB3 = fmaf(eB3, eB3, t3), qB4 = fmaf(eB4, eB4, t4);
-    const float U1 = D1 - D2, U2 = D2 - D3, U3 = D3 - D4;
+    const T eps4 = Weno<T>::eps4;
+    const T c13_3 = T(13) / T(3);
+    const T g0 = w1 - w0, g1 = w2 - w1, g2 = w3 - w2, g3 = w4 - w3, g4 = w5 - w4;
+    const T D1 = g1 - g0, D2 = g2 - g1, D3 = g3 - g2, D4 = g4 - g3;
+    const T t1 = fm(c13_3 * D1, D1, eps4), t2 = fm(c13_3 * D2, D2, eps4);
+    const T t3 = fm(c13_3 * D3, D3, eps4), t4 = fm(c13_3 * D4, D4, eps4);
+    const T eA1 = fm(T(2), g1, D1), eA2 = fm(T(2), g2, D2);
+    const T eC2 = g1 + g2, eC3 = g2 + g3;
+    const T eB3 = fm(T(-2), g2, D3), eB4 = fm(T(-2), g3, D4);
+    const T qA1 = fm(eA1, eA1, t1), qA2 = fm(eA2, eA2, t2);
+    const T qC2 = fm(eC2, eC2, t2), qC3 = fm(eC3, eC3, t3);
+    const T qB3 = fm(eB3, eB3, t3), qB4 = fm(eB4, eB4, t4);
+    const T U1 = D1 - D2, U2 = D2 - D3, U3 = D3 - D4;
     a.f = weno_face6(qA1, qC2, qB3, U1, U2);
     b.f = weno_face6(qA2, qC3, qB4, U2, U3);
-    a.v = fmaf(5.0f, w2, fmaf(2.0f, w3, -w1));
-    b.v = fmaf(5.0f, w3, fmaf(2.0f, w4, -w2));
+    a.v = fm(T(5), w2, fm(T(2), w3, -w1));
+    b.v = fm(T(5), w3, fm(T(2), w4, -w2));
 }
 
 // |vel| * (N_plus - N_minus) for a direction with a full stencil on both faces.
 // The 1/6 folds into the product with |vel|.
-__device__ __forceinline__ float upwind_difference(float fm3, float fm2, float fm1, float f0, float fp1, float fp2,
-                                                   float fp3, float vel)
+template <typename T>
+__device__ __forceinline__ T upwind_difference(T fm3, T fm2, T fm1, T f0, T fp1, T fp2, T fp3, T vel)
 {
-    const bool pos = vel >= 0.0f;
-    const float d6 = weno_pair6<false>(pos ? fm3 : fp3, pos ? fm2 : fp2, pos ? fm1 : fp1, f0, pos ? fp1 : fm1,
-                                       pos ? fp2 : fm2, nullptr, nullptr);
-    return fabsf(vel) * (1.0f / 6.0f) * d6;
+    const bool pos = vel >= T(0);
+    const T d6 = weno_pair6<false>(pos ? fm3 : fp3, pos ? fm2 : fp2, pos ? fm1 : fp1, f0, pos ? fp1 : fm1,
+                                   pos ? fp2 : fm2, (T *)nullptr, (T *)nullptr);
+    return fab(vel) * (T(1) / T(6)) * d6;
 }
 
 // Load at a 32-bit byte offset from a block-uniform base, so the address is a
 // scalar base plus a 32-bit lane offset (global_load ... saddr form) and costs
 // no 64-bit vector arithmetic.  Offsets stay below 4 GiB: a slab is nx*ny
-// elements and the host rejects nx*ny >= 2^30.
+// elements and the host rejects nx*ny >= 2^30 (2^29 for FP64 slabs).
 // AS1: the base came from memory (ensemble descriptors) and is marked global
 // so that the load stays a global (not flat) load.
-template <bool AS1 = false>
-__device__ __forceinline__ float ldo(const float *base, unsigned byte_off)
+template <bool AS1 = false, typename T>
+__device__ __forceinline__ T ldo(const T *base, unsigned byte_off)
 {
     if (AS1) {
         typedef const __attribute__((address_space(1))) char gchar;
-        typedef const __attribute__((address_space(1))) float gfloat;
-        return *(gfloat *)((gchar *)base + byte_off);
+        typedef const __attribute__((address_space(1))) T gT;
+        return *(gT *)((gchar *)base + byte_off);
     }
-    return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(base) + byte_off);
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + byte_off);
 }
 
-template <bool AS1 = false>
-__device__ __forceinline__ void sto(float *base, unsigned byte_off, float v)
+template <bool AS1 = false, typename T>
+__device__ __forceinline__ void sto(T *base, unsigned byte_off, T v)
 {
     if (AS1) {
         typedef __attribute__((address_space(1))) char gchar;
-        typedef __attribute__((address_space(1))) float gfloat;
-        *(gfloat *)((gchar *)base + byte_off) = v;
+        typedef __attribute__((address_space(1))) T gT;
+        *(gT *)((gchar *)base + byte_off) = v;
         return;
     }
-    *reinterpret_cast<float *>(reinterpret_cast<char *>(base) + byte_off) = v;
+    *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + byte_off) = v;
 }
 
 __device__ __forceinline__ int wrap(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 __device__ __forceinline__ int clampi(int i, int n) { return i < 0 ? 0 : (i >= n ? n - 1 : i); }
-__device__ __forceinline__ float clip0(float x) { return x < 0.0f ? 0.0f : x; }
+template <typename T>
+__device__ __forceinline__ T clip0(T x) { return x < T(0) ? T(0) : x; }
 
 // Kappa flux difference of one cell (wave_current_refraction.jl:125-137):
 // WENO5 where the six-point stencil fits, first-order upwind faces where it
 // does not, zero flux through the outer faces.  m is uniform across the
 // workgroup, so the edge path does not diverge.  fk[3] is unused (x0).
-__device__ __forceinline__ float kappa_difference(const float (&fk)[7], float x0, float ck, int m, int nk)
+template <typename T>
+__device__ __forceinline__ T kappa_difference(const T (&fk)[7], T x0, T ck, int m, int nk)
 {
     if (m >= 3 && m + 3 <= nk - 1)
         return upwind_difference(fk[0], fk[1], fk[2], x0, fk[4], fk[5], fk[6], ck);
-    const float km3 = fk[0], km2 = fk[1], km1 = fk[2], kp1 = fk[4], kp2 = fk[5], kp3 = fk[6];
-    const bool pos = ck >= 0.0f;
-    const float w2 = pos ? km1 : kp1, w4 = pos ? kp1 : km1;
-    float A = w2, B = x0;  // first-order upstream / downstream faces
+    const T km3 = fk[0], km2 = fk[1], km1 = fk[2], kp1 = fk[4], kp2 = fk[5], kp3 = fk[6];
+    const bool pos = ck >= T(0);
+    const T w2 = pos ? km1 : kp1, w4 = pos ? kp1 : km1;
+    T A = w2, B = x0;  // first-order upstream / downstream faces
     const bool has_p = (m - 2 >= 0) && (m + 3 <= nk - 1);
     const bool has_m = (m - 3 >= 0) && (m + 2 <= nk - 1);
     if (has_p || has_m) {
-        float Aw, Bw;
+        T Aw, Bw;
         weno_pair(pos ? km3 : kp3, pos ? km2 : kp2, w2, x0, w4, pos ? kp2 : km2, Aw, Bw);
         // For ck >= 0 the downstream face is the plus face, else the minus face.
         if (pos ? has_m : has_p)
@@ -219,9 +252,9 @@ __device__ __forceinline__ float kappa_difference(const float (&fk)[7], float x0
         if (pos ? has_p : has_m)
             B = Bw;
     }
-    const float plus = pos ? B : A, minus = pos ? A : B;
-    const float fkp = (m == nk - 1) ? 0.0f : ck * plus;
-    const float fkm = (m == 0) ? 0.0f : ck * minus;
+    const T plus = pos ? B : A, minus = pos ? A : B;
+    const T fkp = (m == nk - 1) ? T(0) : ck * plus;
+    const T fkm = (m == 0) ? T(0) : ck * minus;
     return fkp - fkm;
 }
 
@@ -413,9 +446,9 @@ __global__ __launch_bounds__(64 * STAGE_ROWS) void stage_fast_kernel(const float
 // that of stage_tile, so the results are the same bit for bit.  Shared by the
 // per-model kernel and the ensemble work-list kernel (AS1: p and the slab
 // pointers came from memory).
-template <int TRANSPORT, int STAGE, int SK, int TY, bool AS1>
-__device__ __forceinline__ void phi_tile(const float *__restrict__ Nk, const float *__restrict__ N0, float *out,
-                                         const StageParams<float> &p, int ib, int jb, int m)
+template <typename T, int TRANSPORT, int STAGE, int SK, int TY, bool AS1>
+__device__ __forceinline__ void phi_tile(const T *__restrict__ Nk, const T *__restrict__ N0, T *out,
+                                         const StageParams<T> &p, int ib, int jb, int m)
 {
     constexpr bool REF = TRANSPORT == RPL_TRANSPORT_REFRACTION;
     static_assert(REF || TRANSPORT == RPL_TRANSPORT_INTRINSIC, "phi_tile: refraction or intrinsic transport");
@@ -427,20 +460,21 @@ __device__ __forceinline__ void phi_tile(const float *__restrict__ Nk, const flo
     const size_t plane = (size_t)nx * ny;
     const unsigned nxu = (unsigned)nx;
     const unsigned off = (unsigned)i + nxu * (unsigned)j;
-    const unsigned bo = off << 2;
+    constexpr unsigned ES = (unsigned)sizeof(T), SH = ES == 8 ? 3u : 2u;  // element bytes, log2
+    const unsigned bo = off << SH;
     const bool x_inner = i0 >= 3 && i0 + 64 + 3 <= nx;  // block-uniform
     const bool y_inner = j0 >= 3 && j0 + TY + 3 <= ny;
-    const unsigned rowb = (nxu * (unsigned)j) << 2, colb = (unsigned)i << 2;
-    const unsigned sx = 4u, sy = nxu << 2;
-    float Uxx = 0.0f, Uxy = 0.0f, Uyx = 0.0f, Uyy = 0.0f, Uxc = 0.0f, Uyc = 0.0f, kap = 0.0f;
+    const unsigned rowb = (nxu * (unsigned)j) << SH, colb = (unsigned)i << SH;
+    const unsigned sx = ES, sy = nxu << SH;
+    T Uxx = T(0), Uxy = T(0), Uyx = T(0), Uyy = T(0), Uxc = T(0), Uyc = T(0), kap = T(0);
     if constexpr (REF) {
         // Doppler velocity and gradients at (i, j, m) (wave_current_refraction.jl:7-19).
-        const float *Ux = p.Ux + plane * m, *Uy = p.Uy + plane * m;
-        const unsigned xp = x_inner ? bo + sx : rowb + ((unsigned)wrap(i + 1, nx) << 2);
-        const unsigned xm = x_inner ? bo - sx : rowb + ((unsigned)wrap(i - 1, nx) << 2);
+        const T *Ux = p.Ux + plane * m, *Uy = p.Uy + plane * m;
+        const unsigned xp = x_inner ? bo + sx : rowb + ((unsigned)wrap(i + 1, nx) << SH);
+        const unsigned xm = x_inner ? bo - sx : rowb + ((unsigned)wrap(i - 1, nx) << SH);
         const unsigned yp = y_inner ? bo + sy : colb + sy * (unsigned)wrap(j + 1, ny);
         const unsigned ym = y_inner ? bo - sy : colb + sy * (unsigned)wrap(j - 1, ny);
-        const float hx = 0.5f * p.dx_inv, hy = 0.5f * p.dy_inv;
+        const T hx = T(0.5) * p.dx_inv, hy = T(0.5) * p.dy_inv;
         Uxx = (ldo<AS1>(Ux, xp) - ldo<AS1>(Ux, xm)) * hx;
         Uxy = (ldo<AS1>(Ux, yp) - ldo<AS1>(Ux, ym)) * hy;
         Uyx = (ldo<AS1>(Uy, xp) - ldo<AS1>(Uy, xm)) * hx;
@@ -451,12 +485,13 @@ __device__ __forceinline__ void phi_tile(const float *__restrict__ Nk, const flo
     }
     constexpr bool SRC = SK != SOURCE_NONE;
     // the column's source inputs, loaded once for every bin of the tile
-    SourceColumn<float> scol{};
+    SourceColumn<T> scol{};
     JpColumn jcol{};
+    static_assert(SK != SOURCE_JP_LS || std::is_same<T, float>::value, "the specialised source path is FP32");
     if constexpr (SK == SOURCE_JP_LS)
         jcol = jp_column<AS1>(p.src, bo);
     else if constexpr (SK == SOURCE_GENERIC)
-        scol = source_column<float, AS1>(p.src, off);
+        scol = source_column<T, AS1>(p.src, off);
     // What bin n reads: its x and y stencils, for refraction its kappa
     // stencil and the phi window's next value f(n + 4), otherwise the cell
     // itself, N0 and the added source at the cell, and the bin's source
@@ -465,8 +500,8 @@ __device__ __forceinline__ void phi_tile(const float *__restrict__ Nk, const flo
     // per lane) decides the speed: issuing bin n + 1's loads ahead of bin n's
     // arithmetic costs 20 more registers and runs slower.
     struct BinLoads {
-        float fx[7], fy[7], fk[7], wnext, x0, n0, add;
-        SourceBin<float> sb;
+        T fx[7], fy[7], fk[7], wnext, x0, n0, add;
+        SourceBin<T> sb;
     };
     const unsigned cell_off = bo;
     auto load_bin = [&](int n, BinLoads &b) {
@@ -477,17 +512,17 @@ __device__ __forceinline__ void phi_tile(const float *__restrict__ Nk, const flo
         // SIMD).
         unsigned bo = cell_off;
         asm volatile("" : "+v"(bo));
-        const float *slab = Nk + plane * ((size_t)m + (size_t)nk * n);
+        const T *slab = Nk + plane * ((size_t)m + (size_t)nk * n);
         if (x_inner) {
 #pragma unroll
             for (int d = 0; d < 7; ++d)
                 if (d != 3)
-                    b.fx[d] = ldo<AS1>(slab, bo + (unsigned)(4 * (d - 3)));
+                    b.fx[d] = ldo<AS1>(slab, bo + (unsigned)((int)ES * (d - 3)));
         } else {
 #pragma unroll
             for (int d = 0; d < 7; ++d)
                 if (d != 3)
-                    b.fx[d] = ldo<AS1>(slab, rowb + ((unsigned)wrap(i + d - 3, nx) << 2));
+                    b.fx[d] = ldo<AS1>(slab, rowb + ((unsigned)wrap(i + d - 3, nx) << SH));
         }
         if (y_inner) {
 #pragma unroll
@@ -515,81 +550,81 @@ __device__ __forceinline__ void phi_tile(const float *__restrict__ Nk, const flo
             b.x0 = ldo<AS1>(slab, bo);
         }
         const size_t bin = (size_t)m + (size_t)nk * n;
-        b.n0 = (STAGE == STAGE_2 || STAGE == STAGE_3) ? ldo<AS1>(N0 + plane * bin, bo) : 0.0f;
-        b.add = p.src_add ? ldo<AS1>(p.src_add + plane * bin, bo) : 0.0f;
+        b.n0 = (STAGE == STAGE_2 || STAGE == STAGE_3) ? ldo<AS1>(N0 + plane * bin, bo) : T(0);
+        b.add = p.src_add ? ldo<AS1>(p.src_add + plane * bin, bo) : T(0);
         if (SRC)
-            b.sb = source_bin<float, AS1>(p.src, (int)bin);
+            b.sb = source_bin<T, AS1>(p.src, (int)bin);
     };
     // phi window f(n - 3 .. n + 3) at this cell, periodic in phi (refraction)
-    float w[7] = {};
+    T w[7] = {};
     if constexpr (REF) {
 #pragma unroll
         for (int d = 0; d < 7; ++d)
             w[d] = ldo<AS1>(Nk + plane * ((size_t)m + (size_t)nk * wrap(d - 3, nphi)), bo);
     }
-    Face6 carry = {0.0f, 0.0f};  // phi face n - 1/2 (as (v, f) of weno_face_of5)
+    Face6<T> carry = {T(0), T(0)};  // phi face n - 1/2 (as (v, f) of weno_face_of5)
     bool cpos = true;            // its upwind side
-    float twx = 0.0f, twy = 0.0f;  // wave-supported stress sums of the output (SOURCE_JP_LS)
+    T twx = T(0), twy = T(0);  // wave-supported stress sums of the output (FP32 stress cap)
     for (int n = 0; n < nphi; ++n) {
         BinLoads cur;
         load_bin(n, cur);
         const size_t bin = (size_t)m + (size_t)nk * n;
-        const float x0 = REF ? w[3] : cur.x0;
-        float ux = gld<AS1>(p.cgx, m + nk * n), uy = gld<AS1>(p.cgy, m + nk * n);
-        float ck = 0.0f, cp = 0.0f;
+        const T x0 = REF ? w[3] : cur.x0;
+        T ux = gld<AS1>(p.cgx, m + nk * n), uy = gld<AS1>(p.cgy, m + nk * n);
+        T ck = T(0), cp = T(0);
         if constexpr (REF) {
-            const float cs = gld<AS1>(p.cosp, n), sn = gld<AS1>(p.sinp, n);
-            const float cc = cs * cs, ss = sn * sn, csn = cs * sn;
+            const T cs = gld<AS1>(p.cosp, n), sn = gld<AS1>(p.sinp, n);
+            const T cc = cs * cs, ss = sn * sn, csn = cs * sn;
             ck = -kap * (cc * Uxx + csn * (Uyx + Uxy) + ss * Uyy);
             cp = csn * (Uxx - Uyy) + ss * Uyx - cc * Uxy;
             ux += Uxc;
             uy += Uyc;
         }
-        const float *fx = cur.fx, *fy = cur.fy;
-        const float dx = upwind_difference(fx[0], fx[1], fx[2], x0, fx[4], fx[5], fx[6], ux);
-        const float dy = upwind_difference(fy[0], fy[1], fy[2], x0, fy[4], fy[5], fy[6], uy);
-        float L = -fmaf(dx, p.dx_inv, dy * p.dy_inv);
+        const T *fx = cur.fx, *fy = cur.fy;
+        const T dx = upwind_difference(fx[0], fx[1], fx[2], x0, fx[4], fx[5], fx[6], ux);
+        const T dy = upwind_difference(fy[0], fy[1], fy[2], x0, fy[4], fy[5], fy[6], uy);
+        T L = -fm(dx, p.dx_inv, dy * p.dy_inv);
         if constexpr (REF) {
             // phi faces: the face n + 1/2 of this bin is the face n - 1/2 of
             // the next one.  When every lane keeps its upwind side from the
             // previous bin, only the new face is evaluated.
-            const bool ppos = cp >= 0.0f;
-            float d6;
+            const bool ppos = cp >= T(0);
+            T d6;
             if (n > 0 && __all(ppos == cpos)) {
-                const Face6 nf = weno_face_of5(ppos ? w[1] : w[6], ppos ? w[2] : w[5], ppos ? x0 : w[4],
+                const Face6<T> nf = weno_face_of5(ppos ? w[1] : w[6], ppos ? w[2] : w[5], ppos ? x0 : w[4],
                                                ppos ? w[4] : x0, ppos ? w[5] : w[2]);
                 d6 = ppos ? (nf.v - carry.v) + (nf.f - carry.f) : (carry.v - nf.v) + (carry.f - nf.f);
                 carry = nf;
             } else {
-                Face6 a, b;
+                Face6<T> a, b;
                 weno_pair_parts(ppos ? w[0] : w[6], ppos ? w[1] : w[5], ppos ? w[2] : w[4], x0,
                                 ppos ? w[4] : w[2], ppos ? w[5] : w[1], a, b);
                 d6 = (b.v - a.v) + (b.f - a.f);
                 carry = ppos ? b : a;
             }
             cpos = ppos;
-            const float dp = fabsf(cp) * (1.0f / 6.0f) * d6;
-            const float dk = kappa_difference(cur.fk, x0, ck, m, nk);
-            L = L - fmaf(dk, p.dk_inv, dp * p.dp_inv);
+            const T dp = fab(cp) * (T(1) / T(6)) * d6;
+            const T dk = kappa_difference(cur.fk, x0, ck, m, nk);
+            L = L - fm(dk, p.dk_inv, dp * p.dp_inv);
         }
         // tendencies.jl:88-94, G += source_tendency
-        float raw = 0.0f;
+        T raw = T(0);
         if constexpr (SK == SOURCE_JP_LS)
             L += source_jp_ls(p.src, jcol, cur.sb, x0, raw);
         else if constexpr (SK == SOURCE_GENERIC)
-            L += source_term_cb<float, AS1>(p.src, scol, cur.sb, off, m + nk * n, x0, &raw);
+            L += source_term_cb<T, AS1>(p.src, scol, cur.sb, off, m + nk * n, x0, &raw);
         if (p.src_add)
             L += cur.add;
-        float r;
+        T r;
         if (STAGE == STAGE_TENDENCY) {
             r = L;
         } else if (STAGE == STAGE_1) {
-            r = clip0(fmaf(p.dt, L, x0));
+            r = clip0(fm(p.dt, L, x0));
         } else {
-            const float a = STAGE == STAGE_2 ? 0.75f : (float)(1.0 / 3.0);
-            const float b = STAGE == STAGE_2 ? 0.25f : (float)(2.0 / 3.0);
-            const float st = clip0(fmaf(p.dt, L, x0));
-            r = clip0(fmaf(a, cur.n0, b * st));
+            const T a = STAGE == STAGE_2 ? T(0.75) : (T)(1.0 / 3.0);
+            const T b = STAGE == STAGE_2 ? T(0.25) : (T)(2.0 / 3.0);
+            const T st = clip0(fm(p.dt, L, x0));
+            r = clip0(fm(a, cur.n0, b * st));
         }
         sto<AS1>(out + plane * bin, bo, r);
         if constexpr (SK != SOURCE_NONE) {
@@ -597,9 +632,9 @@ __device__ __forceinline__ void phi_tile(const float *__restrict__ Nk, const flo
             // raw is this bin's rate at this column's wind, r the next input
             // (both source paths give raw the same bits)
             if (p.tau_x) {
-                const float mf = raw * r;
-                twx = fmaf(mf, sld(p.tau_c, bin), twx);
-                twy = fmaf(mf, sld(p.tau_s, bin), twy);
+                const T mf = raw * r;
+                twx = fm(mf, sld(p.tau_c, bin), twx);
+                twy = fm(mf, sld(p.tau_s, bin), twy);
             }
         }
         if constexpr (REF) {
@@ -629,10 +664,15 @@ __device__ __forceinline__ void phi_tile(const float *__restrict__ Nk, const flo
 // eddies); refraction with sources five (six spilled values; config-5
 // members one after another: 137.8 -> 134.7 ms per step).  Higher floors
 // spill more and run slower.
-template <int TRANSPORT, int STAGE, int SK>
+#ifndef RPL_F64_MIN_WAVES
+#define RPL_F64_MIN_WAVES 2
+#endif
+template <typename T, int TRANSPORT, int STAGE, int SK>
 __global__ __launch_bounds__(256)
-__attribute__((amdgpu_waves_per_eu(TRANSPORT == RPL_TRANSPORT_INTRINSIC ? 7 : (SK != SOURCE_NONE ? 5 : 6), 8)))
-void stage_phi_kernel(const float *__restrict__ Nk, const float *__restrict__ N0, float *out, StageParams<float> p,
+__attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? RPL_F64_MIN_WAVES
+                                                  : (TRANSPORT == RPL_TRANSPORT_INTRINSIC ? 7 : (SK != SOURCE_NONE ? 5 : 6)),
+                                   8)))
+void stage_phi_kernel(const T *__restrict__ Nk, const T *__restrict__ N0, T *out, StageParams<T> p,
                       int nbx, int nby)
 {
     Tile t;
@@ -640,11 +680,11 @@ void stage_phi_kernel(const float *__restrict__ Nk, const float *__restrict__ N0
         t = tile_of(blockIdx.x, nbx * nby * p.nk, nbx, nby, p.nk, 1);
     else
         t = tile_of_spatial(blockIdx.x, nbx * nby * p.nk, nbx, nby, p.nk);
-    phi_tile<TRANSPORT, STAGE, SK, 4, false>(Nk, N0, out, p, t.ib, t.jb, t.m);
+    phi_tile<T, TRANSPORT, STAGE, SK, 4, false>(Nk, N0, out, p, t.ib, t.jb, t.m);
 }
 
-template <int TRANSPORT, int SK>
-hipError_t launch_phi(const StageParams<float> &p, int stage, const float *Nk, const float *N0, float *out,
+template <typename T, int TRANSPORT, int SK>
+hipError_t launch_phi(const StageParams<T> &p, int stage, const T *Nk, const T *N0, T *out,
                       hipStream_t s)
 {
     const int nbx = (p.nx + 63) / 64, nby = (p.ny + 3) / 4;
@@ -654,17 +694,17 @@ hipError_t launch_phi(const StageParams<float> &p, int stage, const float *Nk, c
     dim3 block(64, 4, 1), grid((unsigned)total, 1, 1);
     switch (stage) {
     case STAGE_TENDENCY:
-        hipLaunchKernelGGL((stage_phi_kernel<TRANSPORT, STAGE_TENDENCY, SK>), grid, block, 0, s, Nk, N0, out, p,
+        hipLaunchKernelGGL((stage_phi_kernel<T, TRANSPORT, STAGE_TENDENCY, SK>), grid, block, 0, s, Nk, N0, out, p,
                            nbx, nby);
         break;
     case STAGE_1:
-        hipLaunchKernelGGL((stage_phi_kernel<TRANSPORT, STAGE_1, SK>), grid, block, 0, s, Nk, N0, out, p, nbx, nby);
+        hipLaunchKernelGGL((stage_phi_kernel<T, TRANSPORT, STAGE_1, SK>), grid, block, 0, s, Nk, N0, out, p, nbx, nby);
         break;
     case STAGE_2:
-        hipLaunchKernelGGL((stage_phi_kernel<TRANSPORT, STAGE_2, SK>), grid, block, 0, s, Nk, N0, out, p, nbx, nby);
+        hipLaunchKernelGGL((stage_phi_kernel<T, TRANSPORT, STAGE_2, SK>), grid, block, 0, s, Nk, N0, out, p, nbx, nby);
         break;
     default:
-        hipLaunchKernelGGL((stage_phi_kernel<TRANSPORT, STAGE_3, SK>), grid, block, 0, s, Nk, N0, out, p, nbx, nby);
+        hipLaunchKernelGGL((stage_phi_kernel<T, TRANSPORT, STAGE_3, SK>), grid, block, 0, s, Nk, N0, out, p, nbx, nby);
         break;
     }
     return hipGetLastError();
@@ -759,7 +799,7 @@ __global__ __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(5, 8)))
             jb = q % e.nby;
             m = q / e.nby;
         }
-        phi_tile<TRANSPORT, STAGE, SK, TY, true>(e.Nk, e.N0, e.out, e.p, ib, jb, m);
+        phi_tile<float, TRANSPORT, STAGE, SK, TY, true>(e.Nk, e.N0, e.out, e.p, ib, jb, m);
     }
 }
 
@@ -874,18 +914,33 @@ hipError_t launch_stage_fast(const StageParams<float> &p, int transport, int sta
     if (p.src.on) {
         const bool jp = source_is_jp_ls(p.src);  // the specialised bundle path (same bits as the generic one)
         if (transport == RPL_TRANSPORT_REFRACTION)
-            return jp ? launch_phi<RPL_TRANSPORT_REFRACTION, SOURCE_JP_LS>(p, stage, Nk, N0, out, s)
-                      : launch_phi<RPL_TRANSPORT_REFRACTION, SOURCE_GENERIC>(p, stage, Nk, N0, out, s);
+            return jp ? launch_phi<float, RPL_TRANSPORT_REFRACTION, SOURCE_JP_LS>(p, stage, Nk, N0, out, s)
+                      : launch_phi<float, RPL_TRANSPORT_REFRACTION, SOURCE_GENERIC>(p, stage, Nk, N0, out, s);
         if (transport == RPL_TRANSPORT_INTRINSIC)
-            return jp ? launch_phi<RPL_TRANSPORT_INTRINSIC, SOURCE_JP_LS>(p, stage, Nk, N0, out, s)
-                      : launch_phi<RPL_TRANSPORT_INTRINSIC, SOURCE_GENERIC>(p, stage, Nk, N0, out, s);
+            return jp ? launch_phi<float, RPL_TRANSPORT_INTRINSIC, SOURCE_JP_LS>(p, stage, Nk, N0, out, s)
+                      : launch_phi<float, RPL_TRANSPORT_INTRINSIC, SOURCE_GENERIC>(p, stage, Nk, N0, out, s);
         return launch_t<RPL_TRANSPORT_NONE, true>(p, stage, Nk, N0, out, s);
     }
     if (transport == RPL_TRANSPORT_REFRACTION)
-        return launch_phi<RPL_TRANSPORT_REFRACTION, SOURCE_NONE>(p, stage, Nk, N0, out, s);
+        return launch_phi<float, RPL_TRANSPORT_REFRACTION, SOURCE_NONE>(p, stage, Nk, N0, out, s);
     if (transport == RPL_TRANSPORT_INTRINSIC)
-        return launch_phi<RPL_TRANSPORT_INTRINSIC, SOURCE_NONE>(p, stage, Nk, N0, out, s);
+        return launch_phi<float, RPL_TRANSPORT_INTRINSIC, SOURCE_NONE>(p, stage, Nk, N0, out, s);
     return launch_t<RPL_TRANSPORT_NONE, false>(p, stage, Nk, N0, out, s);
+}
+
+hipError_t launch_stage_fast(const StageParams<double> &p, int transport, int stage, const double *Nk,
+                             const double *N0, double *out, hipStream_t s)
+{
+    // 32-bit byte offsets within a slab: FP64 slabs below 2^29 cells
+    if ((size_t)p.nx * p.ny >= ((size_t)1 << 29))
+        return hipErrorInvalidValue;
+    if (transport == RPL_TRANSPORT_REFRACTION)
+        return p.src.on ? launch_phi<double, RPL_TRANSPORT_REFRACTION, SOURCE_GENERIC>(p, stage, Nk, N0, out, s)
+                        : launch_phi<double, RPL_TRANSPORT_REFRACTION, SOURCE_NONE>(p, stage, Nk, N0, out, s);
+    if (transport == RPL_TRANSPORT_INTRINSIC)
+        return p.src.on ? launch_phi<double, RPL_TRANSPORT_INTRINSIC, SOURCE_GENERIC>(p, stage, Nk, N0, out, s)
+                        : launch_phi<double, RPL_TRANSPORT_INTRINSIC, SOURCE_NONE>(p, stage, Nk, N0, out, s);
+    return hipErrorInvalidValue;
 }
 
 }  // namespace rpl

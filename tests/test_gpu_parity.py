@@ -291,3 +291,46 @@ def test_full_size_sheared_eddies_properties():
     m.set_doppler_velocity(np.full(Ux.shape, 0.1), np.full(Ux.shape, -0.2))
     ra.set_(m, N=0.5)
     assert np.all(m.tendency() == 0.0)
+
+
+def test_vortex_refraction_fp64_fast_kernel(oracle):
+    """The restructured phi-loop kernel in FP64 (kernel="fast"): it differs
+    from the reference operation order by rounding only, so the tendency
+    matches the oracle to 1e-13 and the whole 400-step config-1 run to the
+    FP64 bar of 1e-12 relative L2."""
+    say("vortex fp64 fast kernel 400 steps")
+    wl = workloads.vortex_refraction()
+    m = device_model(wl, np.float64, kernel="fast")
+    om = oracle.OracleModel(wl)
+    assert rel_l2(m.tendency(), om.tendency()) <= 1e-13
+    ra.run_(ra.Simulation(m, dt=wl.dt, stop_iteration=wl.nsteps))
+    om.step(wl.dt, wl.nsteps)
+    err = rel_l2(m.action, om.action)
+    say(f"vortex fp64 fast kernel rel-L2 = {err:.3e}")
+    assert err <= 1e-12
+    m0_d, m0_o = ra.m0(m), om.diagnostics()[0]
+    assert rel_l2(m0_d, m0_o) <= 1e-12
+
+
+@pytest.mark.parametrize("nk", [1, 2, 3, 4, 5, 7, 9])
+def test_kappa_boundary_cases_fp64_fast(oracle, nk):
+    say(f"kappa edge fp64 fast nk={nk}")
+    wl = random_workload(11, 9, nk, 5, seed=nk)
+    m = device_model(wl, np.float64, kernel="fast")
+    om = oracle.OracleModel(wl)
+    assert rel_l2(m.tendency(), om.tendency()) <= 1e-13
+    ra.time_step_(m, 0.02, 3)
+    om.step(0.02, 3)
+    assert rel_l2(m.action, om.action) <= 1e-12
+
+
+@pytest.mark.parametrize("shape", [(3, 3, 4, 3), (67, 5, 6, 7), (130, 66, 5, 36)])
+def test_ragged_shapes_fp64_fast(oracle, shape):
+    say(f"fp64 fast shape {shape}")
+    for current in (True, False):
+        wl = random_workload(*shape, seed=17, current=current)
+        m = device_model(wl, np.float64, kernel="fast")
+        om = oracle.OracleModel(wl)
+        ra.time_step_(m, 0.01, 2)
+        om.step(0.01, 2)
+        assert rel_l2(m.action, om.action) <= 1e-12, (shape, current)

@@ -204,3 +204,22 @@ def test_fused_stress_cap_fp32(oracle):
     d_orc = rel_l2(fast.action, om.action)
     say(f"fused cap fp32: vs fp32 reference kernel {d_ref:.2e}, vs fp64 oracle {d_orc:.2e}")
     assert d_ref <= 2e-5 and d_orc <= 1e-4
+
+
+def test_sources_fp64_fast_kernel(oracle):
+    """Intrinsic transport with the source bundle in FP64 on the phi-loop
+    kernel: within 1e-12 of the oracle after three RK3 steps."""
+    say("sources fp64 fast kernel")
+    wl = workloads.translating_hurricane(float_type=np.float64, nx=40, ny=36, nz=4, nkappa=10, nphi=12, dt=60.0)
+    s = wl.model_kwargs["sources"].wind_input.wind
+    s.rmax, s.radius = 3e3, 30e3
+    s.track.centers = [(200e3, 0.0), (200e3 + 5.0 * s.track.times[1], 0.0)]
+    m = device(wl, np.float64, kernel="fast")
+    om = oracle_model(oracle, wl)
+    assert rel_l2(m.tendency(), om.tendency()) <= 1e-12
+    for _ in range(3):
+        ra.time_step_(m, wl.dt)
+        om.step(wl.dt)
+    err = rel_l2(m.action, om.action)
+    say(f"sources fp64 fast: rel-L2 after 3 steps {err:.2e}")
+    assert err <= 1e-12
