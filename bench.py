@@ -245,7 +245,9 @@ def ensemble_line(count, steps, with_surrogate=True, **member_kw):
     (64^2-512^2 cells x 16 levels, 16-40 kappa x 24-48 phi bins, own eddy
     current and uniform wind, wind input + local-saturation sources), FP32.
     Times the members stepped one after another (rpl_time_step each) and the
-    ensemble (persistent launch per stage, autotuned shape, hipGraph replay),
+    ensemble (large members' own stage kernels on branch streams beside one
+    persistent work-list launch per stage for the small ones; autotuned split
+    and shape; hipGraph replay),
     first with the plain sources, then with every member's sources from a
     512 x 3 network of its own bin count (random weights, open envelope: a
     timing run)."""
@@ -456,6 +458,8 @@ def main():
                     help="print only the N x N sheared-eddy refraction line (for profiling its kernel)")
     ap.add_argument("--fp64-fast", action="store_true", help="with --only-refraction: FP64 on the fast kernel")
     ap.add_argument("--ensemble-members", type=int, default=64)
+    ap.add_argument("--only-ensemble", action="store_true",
+                    help="print only the batched-ensemble lines (members stepped one by one vs batched)")
     ap.add_argument("--performance-summary", default=None,
                     help="also write the measured lines as a reference-format performance summary (TSV)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -478,6 +482,12 @@ def main():
 
     if args.only_pseudomomentum:
         print(json.dumps({"pseudomomentum_coupling": pseudomomentum_line(args.steps, args.warmup)}), flush=True)
+        return
+    if args.only_ensemble:
+        print(json.dumps({"ensemble": ensemble_line(args.ensemble_members, max(2, args.steps // 5))}), flush=True)
+        print(json.dumps({"ensemble_small_members": ensemble_line(256, 10, with_surrogate=False, size_range=(16, 64),
+                                                                  kappa_range=(8, 16), phi_range=(12, 24))}),
+              flush=True)
         return
     if args.only_dia:
         print(json.dumps({"nonlinear_dia": dia_line(args.steps, args.warmup)}), flush=True)

@@ -382,8 +382,9 @@ int rpl_get_nonlinear_timing(rpl_ctx *ctx, double *total_ms, int64_t *launches);
  * SSP-RK3, one transport kind (intrinsic or refraction) and either all with
  * or all without source terms -- whose sizes and spectral grids may differ.
  * rpl_ensemble_step advances every member together: per RK3 stage, the
- * members' source preparation (column state, surrogate inference) and then
- * ONE persistent-kernel launch over a work list of all members' tiles (64 x
+ * members' source preparation (column state, surrogate inference), then the
+ * stage kernel of each large member (as rpl_time_step launches it) and ONE
+ * persistent-kernel launch over a work list of the other members' tiles (64 x
  * tile_rows cells of one bin).  While in an ensemble, a member's work runs on
  * the ensemble's stream; members must outlive the ensemble, and are used
  * through the ensemble only (their own calls still work and are ordered on
@@ -398,10 +399,17 @@ int rpl_ensemble_synchronize(rpl_ensemble *ens);
  * list; 0: one workgroup per tile, the default). */
 int rpl_ensemble_set_launch(rpl_ensemble *ens, int32_t tile_rows, int32_t blocks);
 int rpl_ensemble_get_launch(rpl_ensemble *ens, int32_t *tile_rows, int32_t *blocks, int64_t *tiles);
-/* Times the stage-1 launch (which leaves the members' state unchanged) for
- * tile_rows in {2, 4, 8} and grids of {2, 4, 8, 16} workgroups per CU or one
- * workgroup per tile, reps launches each; keeps and returns the fastest
- * (blocks 0: one workgroup per tile). */
+/* Which members are large: those with at least min_tiles tiles of 64 x 4
+ * cells x one kappa bin (0: the default, 4096; negative: none, every member
+ * joins the work list).  The getter returns the threshold in force and how
+ * many members run their own stage kernel. */
+int rpl_ensemble_set_split(rpl_ensemble *ens, int64_t min_tiles);
+int rpl_ensemble_get_split(rpl_ensemble *ens, int64_t *min_tiles, int32_t *own_members);
+/* Times stage 1 (which leaves the members' state unchanged) for the splits
+ * {none, 16384, 4096, 1024} and, for the work list, tile_rows in {2, 4, 8}
+ * and grids of {2, 4, 8, 16} workgroups per CU or one workgroup per tile,
+ * reps launches each; keeps the fastest and returns its shape (blocks 0: one
+ * workgroup per tile; the split through rpl_ensemble_get_split). */
 int rpl_ensemble_autotune(rpl_ensemble *ens, double dt, int32_t reps, int32_t *tile_rows, int32_t *blocks,
                           double *best_ms);
 /* Capture the three stages of a step (source preparation + persistent

@@ -784,12 +784,23 @@ struct rpl_ensemble {
     size_t chunk_cap = 0;
     int smembers = 0;
     long long sblocks = 0;
+    // Members with at least `split` tiles of 64 x 4 cells x one kappa bin
+    // run their own stage kernel (as rpl_time_step does) instead of joining
+    // the work list: 0 the default threshold, negative never.
+    long long split = 0;
+    std::vector<rpl_ctx *> own;            // those members ...
+    std::vector<int> own_branch;           // ... and the branch stream each runs on
+    hipStream_t branch[4] = {};            // branch streams (default priority)
+    hipEvent_t fork = nullptr, join[4] = {};
+    std::vector<rpl_ctx *> list;           // the members in the work list
+    int lmembers = 0;
     double desc_dt = -1;
     int desc_ty = 0;
     bool use_graph = false;
     hipGraphExec_t exec = nullptr;
     double graph_dt = -1;
     int graph_ty = 0, graph_blocks = 0;
+    long long graph_split = 0;
     std::vector<uint64_t> graph_gen;  // the members' generations at capture
 };
 
@@ -815,6 +826,15 @@ void stage_buffers(rpl_ctx *c, int stage, const void **Nk, const void **N0, void
 // of the current launch shape when launching).
 int default_blocks(const rpl_ensemble *) { return 0; }
 
+// Default size (tiles of 64 x 4 cells x one kappa bin) from which a member
+// runs its own stage kernel: two rounds of workgroups over 256 CUs at the
+// stage kernel's occupancy, where the per-model launch fills the chip.
+constexpr long long ENSEMBLE_OWN_TILES = 4096;
+// Branch streams the large members' step chains are spread over (the HIP
+// runtime's default of four hardware queues per process).
+constexpr int ENSEMBLE_BRANCHES = 4;
+static_assert(sizeof(rpl_ensemble::branch) / sizeof(hipStream_t) == ENSEMBLE_BRANCHES, "branch streams");
+
 int launch_blocks(const rpl_ensemble *e)
 {
     if (e->blocks > 0)
@@ -824,13 +844,47 @@ int launch_blocks(const rpl_ensemble *e)
 }
 
 
+// Tiles of 64 x 4 cells x one kappa bin of a member: the size by which a
+// member is sent to its own stage kernel.
+long long member_tiles(const rpl_ctx *c)
+{
+    return (long long)((c->cfg.nx + 63) / 64) * ((c->cfg.ny + 3) / 4) * c->cfg.nkappa;
+}
+
+bool own_kernel(const rpl_ensemble *e, const rpl_ctx *c)
+{
+    const long long thr = e->split == 0 ? ENSEMBLE_OWN_TILES : e->split;
+    return thr > 0 && member_tiles(c) >= thr;
+}
+
 int ensemble_descriptors(rpl_ensemble *e, double dt)
 {
-    const int M = (int)e->members.size();
+    // Work-list members first (k < M), then the members with a kernel of their own.
+    std::vector<rpl_ctx *> order;
+    for (rpl_ctx *c : e->members)
+        if (!own_kernel(e, c))
+            order.push_back(c);
+    const int M = (int)order.size();
+    for (rpl_ctx *c : e->members)
+        if (own_kernel(e, c))
+            order.push_back(c);
+    // The large members' step chains go to the branch streams, largest
+    // first onto the least loaded one.
+    e->own.assign(order.begin() + M, order.end());
+    std::sort(e->own.begin(), e->own.end(),
+              [](const rpl_ctx *a, const rpl_ctx *b) { return member_tiles(a) > member_tiles(b); });
+    e->own_branch.assign(e->own.size(), 0);
+    long long load[ENSEMBLE_BRANCHES] = {};
+    for (size_t k = 0; k < e->own.size(); ++k) {
+        const int b = (int)(std::min_element(load, load + ENSEMBLE_BRANCHES) - load);
+        e->own_branch[k] = b;
+        load[b] += member_tiles(e->own[k]);
+    }
+    order.resize(M);
     std::vector<rpl::EnsembleMember> h(3 * (size_t)M);
     long long t0 = 0;
     for (int k = 0; k < M; ++k) {
-        rpl_ctx *c = e->members[k];
+        rpl_ctx *c = order[k];
         const int nbx = (c->cfg.nx + 63) / 64, nby = (c->cfg.ny + e->ty - 1) / e->ty;
         for (int st = 0; st < 3; ++st) {
             rpl::EnsembleMember &d = h[(size_t)st * M + k];
@@ -856,6 +910,8 @@ int ensemble_descriptors(rpl_ensemble *e, double dt)
         t0 += (long long)nbx * nby * c->cfg.nkappa * (phi_loop ? 1 : c->cfg.nphi);
     }
     e->total = t0;
+    e->lmembers = M;
+    e->list = order;
     // One source kind for the whole launch: the specialised bundle path when
     // every member qualifies (the two paths give the same bits).
     e->src_kind = rpl::SOURCE_NONE;
@@ -865,7 +921,8 @@ int ensemble_descriptors(rpl_ensemble *e, double dt)
             if (!rpl::source_is_jp_ls(h[k].p.src))
                 e->src_kind = rpl::SOURCE_GENERIC;
     }
-    RPL_HIP(hipMemcpy(e->desc, h.data(), h.size() * sizeof(rpl::EnsembleMember), hipMemcpyHostToDevice));
+    if (M)
+        RPL_HIP(hipMemcpy(e->desc, h.data(), h.size() * sizeof(rpl::EnsembleMember), hipMemcpyHostToDevice));
     const size_t nchunks = (size_t)((t0 + (1LL << rpl::ENSEMBLE_CHUNK_SHIFT) - 1) >> rpl::ENSEMBLE_CHUNK_SHIFT);
     std::vector<int> cm(nchunks);
     for (size_t q = 0, k = 0; q < nchunks; ++q) {
@@ -882,10 +939,11 @@ int ensemble_descriptors(rpl_ensemble *e, double dt)
         RPL_HIP(hipMalloc((void **)&e->chunk_member, nchunks * sizeof(int)));
         e->chunk_cap = nchunks;
     }
-    RPL_HIP(hipMemcpy(e->chunk_member, cm.data(), nchunks * sizeof(int), hipMemcpyHostToDevice));
-    // Column states (stress cap, mean-spectrum moments) of the members that need one.
+    if (nchunks)
+        RPL_HIP(hipMemcpy(e->chunk_member, cm.data(), nchunks * sizeof(int), hipMemcpyHostToDevice));
+    // Column states (stress cap, mean-spectrum moments) of the work-list members that need one.
     std::vector<rpl_ctx *> sm;
-    for (rpl_ctx *c : e->members)
+    for (rpl_ctx *c : order)
         if (c->has_sources && ((c->src.wind_input && c->src.stress_cap) ||
                                c->src.dissipation == RPL_DISSIPATION_MEAN_SPECTRUM ||
                                c->src.dissipation == RPL_DISSIPATION_WHITECAPPING))
@@ -929,16 +987,17 @@ int ensemble_descriptors(rpl_ensemble *e, double dt)
     return RPL_OK;
 }
 
-// Sources and persistent stage launches of one RK3 step (no wind: that is
-// evaluated at the clock time before each step).
-int ensemble_stages(rpl_ensemble *e)
+// Stages first..last of the work-list members on the ensemble stream: per
+// stage the network inference of surrogate members, the column states in one
+// launch, the surrogate audits, then one launch over the work list.
+int ensemble_list_stages(rpl_ensemble *e, int first, int last)
 {
-    const int M = (int)e->members.size();
-    for (int st = rpl::STAGE_1; st <= rpl::STAGE_3; ++st) {
+    const int M = e->lmembers;
+    if (M == 0)
+        return RPL_OK;
+    for (int st = first; st <= last; ++st) {
         if (e->src) {
-            // Network inference of surrogate members, then every member's
-            // column state in one launch, then the surrogate audits.
-            for (rpl_ctx *c : e->members) {
+            for (rpl_ctx *c : e->list) {
                 if (!c->s_active)
                     continue;
                 const void *Nk, *N0;
@@ -954,7 +1013,7 @@ int ensemble_stages(rpl_ensemble *e)
                 if (err != hipSuccess)
                     return fail(RPL_ERR_HIP, std::string("ensemble state launch: ") + hipGetErrorString(err));
             }
-            for (rpl_ctx *c : e->members) {
+            for (rpl_ctx *c : e->list) {
                 if (!c->s_active || c->scfg.audit_stride <= 0)
                     continue;
                 const void *Nk, *N0;
@@ -972,6 +1031,57 @@ int ensemble_stages(rpl_ensemble *e)
     }
     return RPL_OK;
 }
+
+// Stages first..last of a large member on its branch stream: the launches of
+// rpl_time_step (source preparation, stage kernel), so the member advances
+// exactly as when stepped alone.
+int ensemble_own_stages(rpl_ensemble *e, size_t k, int first, int last, double dt)
+{
+    rpl_ctx *c = e->own[k];
+    c->stream = e->branch[e->own_branch[k]];
+    int rc = RPL_OK;
+    for (int st = first; st <= last && rc == RPL_OK; ++st) {
+        const void *Nk, *N0;
+        void *out;
+        stage_buffers(c, st, &Nk, &N0, &out);
+        if ((rc = prepare_stage(c, Nk)) == RPL_OK)
+            rc = launch_stage(c, dt, st, Nk, N0, out);
+    }
+    c->stream = e->stream;
+    return rc;
+}
+
+// Stages first..last of every member (no wind: that is evaluated at the
+// clock time before each step).  The large members' chains fork from the
+// ensemble stream onto the branch streams and join it at the end, so they
+// overlap one another and the work-list launches.
+int ensemble_run(rpl_ensemble *e, int first, int last, double dt)
+{
+    const bool fork = !e->own.empty();
+    if (fork) {
+        RPL_HIP(hipEventRecord(e->fork, e->stream));
+        for (int b = 0; b < ENSEMBLE_BRANCHES; ++b)
+            RPL_HIP(hipStreamWaitEvent(e->branch[b], e->fork, 0));
+    }
+    int rc = RPL_OK;
+    for (size_t k = 0; k < e->own.size() && rc == RPL_OK; ++k)
+        rc = ensemble_own_stages(e, k, first, last, dt);
+    if (rc == RPL_OK)
+        rc = ensemble_list_stages(e, first, last);
+    if (fork) {
+        // join even after an error, so a capture in progress can end
+        for (int b = 0; b < ENSEMBLE_BRANCHES; ++b) {
+            hipError_t x = hipEventRecord(e->join[b], e->branch[b]);
+            if (x == hipSuccess)
+                x = hipStreamWaitEvent(e->stream, e->join[b], 0);
+            if (x != hipSuccess && rc == RPL_OK)
+                rc = fail(RPL_ERR_HIP, std::string("ensemble join: ") + hipGetErrorString(x));
+        }
+    }
+    return rc;
+}
+
+int ensemble_stages(rpl_ensemble *e, double dt) { return ensemble_run(e, rpl::STAGE_1, rpl::STAGE_3, dt); }
 
 // The rules a context must meet to step in an ensemble with `first`
 // (checked at creation and again before every ensemble step, since a
@@ -995,6 +1105,21 @@ int ensemble_member_check(rpl_ctx *c, rpl_ctx *first)
     if (c->device != first->device)
         return fail(RPL_ERR_ARG, "ensemble members must live on one device");
     return RPL_OK;
+}
+
+void destroy_branches(rpl_ensemble *e)
+{
+    for (int b = 0; b < ENSEMBLE_BRANCHES; ++b) {
+        if (e->branch[b])
+            (void)hipStreamDestroy(e->branch[b]);
+        if (e->join[b])
+            (void)hipEventDestroy(e->join[b]);
+        e->branch[b] = nullptr;
+        e->join[b] = nullptr;
+    }
+    if (e->fork)
+        (void)hipEventDestroy(e->fork);
+    e->fork = nullptr;
 }
 
 void drop_graph(rpl_ensemble *e)
@@ -2646,10 +2771,21 @@ int rpl_ensemble_create(rpl_ensemble **out, rpl_ctx *const *members, int32_t cou
     }
     e->cus = prop.multiProcessorCount;
     e->blocks = default_blocks(e);
+    bool ok = hipEventCreateWithFlags(&e->fork, hipEventDisableTiming) == hipSuccess;
+    for (int b = 0; b < ENSEMBLE_BRANCHES && ok; ++b)
+        ok = hipStreamCreateWithFlags(&e->branch[b], hipStreamNonBlocking) == hipSuccess &&
+             hipEventCreateWithFlags(&e->join[b], hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+        destroy_branches(e);
+        (void)hipStreamDestroy(e->stream);
+        delete e;
+        return fail(RPL_ERR_HIP, "ensemble branch streams");
+    }
     if (hipMalloc((void **)&e->desc, 3 * (size_t)count * sizeof(rpl::EnsembleMember)) != hipSuccess ||
         hipMalloc((void **)&e->sdesc, 3 * (size_t)count * sizeof(rpl::EnsembleState)) != hipSuccess) {
         if (e->desc)
             (void)hipFree(e->desc);
+        destroy_branches(e);
         (void)hipStreamDestroy(e->stream);
         delete e;
         return fail(RPL_ERR_NOMEM, "ensemble descriptors");
@@ -2682,6 +2818,10 @@ int rpl_ensemble_destroy(rpl_ensemble *e)
         (void)hipFree(e->sdesc);
     if (e->chunk_member)
         (void)hipFree(e->chunk_member);
+    for (int b = 0; b < ENSEMBLE_BRANCHES; ++b)
+        if (e->branch[b])
+            (void)hipStreamSynchronize(e->branch[b]);
+    destroy_branches(e);
     (void)hipStreamDestroy(e->stream);
     delete e;
     return RPL_OK;
@@ -2723,6 +2863,30 @@ int rpl_ensemble_get_launch(rpl_ensemble *e, int32_t *tile_rows, int32_t *blocks
     return RPL_OK;
 }
 
+int rpl_ensemble_set_split(rpl_ensemble *e, int64_t min_tiles)
+{
+    if (!e)
+        return fail(RPL_ERR_ARG, "null ensemble");
+    e->split = min_tiles;
+    e->desc_dt = -1;
+    return RPL_OK;
+}
+
+int rpl_ensemble_get_split(rpl_ensemble *e, int64_t *min_tiles, int32_t *own_members)
+{
+    if (!e)
+        return fail(RPL_ERR_ARG, "null ensemble");
+    if (min_tiles)
+        *min_tiles = e->split == 0 ? ENSEMBLE_OWN_TILES : e->split;
+    if (own_members) {
+        int n = 0;
+        for (rpl_ctx *c : e->members)
+            n += own_kernel(e, c) ? 1 : 0;
+        *own_members = n;
+    }
+    return RPL_OK;
+}
+
 int rpl_ensemble_use_graph(rpl_ensemble *e, int32_t enable)
 {
     if (!e)
@@ -2756,11 +2920,11 @@ int rpl_ensemble_step(rpl_ensemble *e, double dt, int32_t nsteps)
     for (rpl_ctx *c : e->members)
         gen.push_back(c->generation);
     if (e->use_graph && (e->exec == nullptr || e->graph_dt != dt || e->graph_ty != e->ty ||
-                         e->graph_blocks != e->blocks || e->graph_gen != gen)) {
+                         e->graph_blocks != e->blocks || e->graph_split != e->split || e->graph_gen != gen)) {
         drop_graph(e);
         hipGraph_t g = nullptr;
         RPL_HIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-        rc = ensemble_stages(e);
+        rc = ensemble_stages(e, dt);
         hipError_t err = hipStreamEndCapture(e->stream, &g);
         if (rc)
             return rc;
@@ -2775,6 +2939,7 @@ int rpl_ensemble_step(rpl_ensemble *e, double dt, int32_t nsteps)
         e->graph_dt = dt;
         e->graph_ty = e->ty;
         e->graph_blocks = e->blocks;
+        e->graph_split = e->split;
         e->graph_gen = gen;
     }
     for (int s = 0; s < nsteps; ++s) {
@@ -2783,7 +2948,7 @@ int rpl_ensemble_step(rpl_ensemble *e, double dt, int32_t nsteps)
                 return rc;
         if (e->use_graph) {
             RPL_HIP(hipGraphLaunch(e->exec, e->stream));
-        } else if ((rc = ensemble_stages(e))) {
+        } else if ((rc = ensemble_stages(e, dt))) {
             return rc;
         }
         for (rpl_ctx *c : e->members) {
@@ -2822,50 +2987,89 @@ int rpl_ensemble_autotune(rpl_ensemble *e, double dt, int32_t reps, int32_t *til
     hipEvent_t a, b;
     RPL_HIP(hipEventCreate(&a));
     RPL_HIP(hipEventCreate(&b));
-    const int M = (int)e->members.size();
+    // Mean time of `reps` runs of `fn` (after one warm-up) on the ensemble stream, in ms.
+    auto timed = [&](auto fn, float *ms) {
+        int r = fn();
+        if (r == RPL_OK && hipEventRecord(a, e->stream) != hipSuccess)
+            r = fail(RPL_ERR_HIP, "autotune: event record");
+        for (int q = 0; q < reps && r == RPL_OK; ++q)
+            r = fn();
+        if (r == RPL_OK && (hipEventRecord(b, e->stream) != hipSuccess || hipEventSynchronize(b) != hipSuccess ||
+                            hipEventElapsedTime(ms, a, b) != hipSuccess))
+            r = fail(RPL_ERR_HIP, "autotune: event timing");
+        *ms /= reps;
+        return r;
+    };
     double best = 1e300;
     int best_ty = e->ty, best_blocks = e->blocks;
+    long long best_split = e->split;
+    const long long splits[4] = {-1, 16384, ENSEMBLE_OWN_TILES, 1024};  // -1: every member in the work list
     const int tys[3] = {2, 4, 8};
     const int mult[5] = {2, 4, 8, 16, 0};  // workgroups per CU; 0: one workgroup per tile
-    hipError_t err = hipSuccess;
-    for (int ty : tys) {
-        e->ty = ty;
+    rc = RPL_OK;
+    for (long long split : splits) {
+        e->split = split;
+        // the work list's shape first, timing its launch alone ...
+        int lty = e->ty, lblocks = e->blocks;
         if ((rc = ensemble_descriptors(e, dt)))
             break;
-        for (int mu : mult) {
-            long long g = mu ? (long long)e->cus * mu : (e->total + 7) / 8 * 8;
-            g = std::max<long long>(8, std::min<long long>(g, 0x7ffffff8LL) / 8 * 8);
-            err = rpl::launch_ensemble_stage(e->desc, M, e->total, e->chunk_member, e->transport, e->src_kind,
-                                             rpl::STAGE_1, ty, (int)g, e->stream);  // warm-up
-            if (err == hipSuccess)
-                err = hipEventRecord(a, e->stream);
-            for (int r = 0; r < reps && err == hipSuccess; ++r)
-                err = rpl::launch_ensemble_stage(e->desc, M, e->total, e->chunk_member, e->transport, e->src_kind,
-                                                 rpl::STAGE_1, ty, (int)g, e->stream);
-            if (err == hipSuccess)
-                err = hipEventRecord(b, e->stream);
-            if (err == hipSuccess)
-                err = hipEventSynchronize(b);
-            float ms = 0;
-            if (err == hipSuccess)
-                err = hipEventElapsedTime(&ms, a, b);
-            if (err != hipSuccess)
-                break;
-            if (ms / reps < best) {
-                best = ms / reps;
-                best_ty = ty;
-                best_blocks = mu ? (int)g : 0;
+        if (e->lmembers) {
+            float lbest = 1e30f;
+            for (int ty : tys) {
+                e->ty = ty;
+                if ((rc = ensemble_descriptors(e, dt)))
+                    break;
+                for (int mu : mult) {
+                    long long g = mu ? (long long)e->cus * mu : (e->total + 7) / 8 * 8;
+                    g = std::max<long long>(8, std::min<long long>(g, 0x7ffffff8LL) / 8 * 8);
+                    float ms = 0;
+                    rc = timed(
+                        [&]() {
+                            hipError_t x = rpl::launch_ensemble_stage(e->desc, e->lmembers, e->total,
+                                                                      e->chunk_member, e->transport, e->src_kind,
+                                                                      rpl::STAGE_1, ty, (int)g, e->stream);
+                            return x == hipSuccess ? RPL_OK
+                                                   : fail(RPL_ERR_HIP, std::string("autotune: ") +
+                                                                           hipGetErrorString(x));
+                        },
+                        &ms);
+                    if (rc)
+                        break;
+                    if (ms < lbest) {
+                        lbest = ms;
+                        lty = ty;
+                        lblocks = mu ? (int)g : 0;
+                    }
+                }
+                if (rc)
+                    break;
             }
+            if (rc)
+                break;
+            e->ty = lty;
+            e->blocks = lblocks;
+            if ((rc = ensemble_descriptors(e, dt)))
+                break;
         }
-        if (err != hipSuccess)
+        // ... then the whole first stage: the large members' chains on the
+        // branch streams beside the work list
+        float ms = 0;
+        if ((rc = timed([&]() { return ensemble_run(e, rpl::STAGE_1, rpl::STAGE_1, dt); }, &ms)))
             break;
+        if (ms < best) {
+            best = ms;
+            best_ty = lty;
+            best_blocks = lblocks;
+            best_split = split;
+        }
     }
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
+    for (rpl_ctx *c : e->members)
+        c->tau_for = nullptr;
     if (rc)
         return rc;
-    if (err != hipSuccess)
-        return fail(RPL_ERR_HIP, std::string("autotune: ") + hipGetErrorString(err));
+    e->split = best_split;
     e->ty = best_ty;
     e->blocks = best_blocks;
     e->desc_dt = -1;

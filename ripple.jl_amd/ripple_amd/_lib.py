@@ -120,6 +120,8 @@ SIGNATURES = {
     "rpl_ensemble_set_launch": ([_P, _I32, _I32], ctypes.c_int),
     "rpl_ensemble_get_launch": ([_P, ctypes.POINTER(_I32), ctypes.POINTER(_I32), _I64P], ctypes.c_int),
     "rpl_ensemble_autotune": ([_P, _D, _I32, ctypes.POINTER(_I32), ctypes.POINTER(_I32), _DP], ctypes.c_int),
+    "rpl_ensemble_set_split": ([_P, ctypes.c_int64], ctypes.c_int),
+    "rpl_ensemble_get_split": ([_P, _I64P, ctypes.POINTER(_I32)], ctypes.c_int),
     "rpl_ensemble_use_graph": ([_P, _I32], ctypes.c_int),
     "rpl_mlp_create": ([ctypes.POINTER(_P), _I32, _I32, _I32, _I32, _I32, ctypes.c_uint64], ctypes.c_int),
     "rpl_mlp_destroy": ([_P], ctypes.c_int),

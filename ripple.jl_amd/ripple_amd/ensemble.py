@@ -49,16 +49,26 @@ class Ensemble:
     def set_launch(self, tile_rows=0, blocks=0):
         _lib.check(self._lib.rpl_ensemble_set_launch(self.ptr, int(tile_rows), int(blocks)))
 
+    def set_split(self, min_tiles=0):
+        """Members with at least `min_tiles` tiles of 64 x 4 cells x one kappa
+        bin run their own stage kernel; 0 the default, negative none."""
+        _lib.check(self._lib.rpl_ensemble_set_split(self.ptr, int(min_tiles)))
+
+    def split(self):
+        t, n = ctypes.c_int64(), ctypes.c_int32()
+        _lib.check(self._lib.rpl_ensemble_get_split(self.ptr, ctypes.byref(t), ctypes.byref(n)))
+        return {"min_tiles": t.value, "own_members": n.value}
+
     def launch(self):
         ty, b, t = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int64()
         _lib.check(self._lib.rpl_ensemble_get_launch(self.ptr, ctypes.byref(ty), ctypes.byref(b), ctypes.byref(t)))
-        return {"tile_rows": ty.value, "blocks": b.value, "tiles": t.value}
+        return {"tile_rows": ty.value, "blocks": b.value, "tiles": t.value, **self.split()}
 
     def autotune(self, dt, reps=3):
         ty, b, ms = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_double()
         _lib.check(self._lib.rpl_ensemble_autotune(self.ptr, float(dt), int(reps), ctypes.byref(ty),
                                                    ctypes.byref(b), ctypes.byref(ms)))
-        return {"tile_rows": ty.value, "blocks": b.value, "stage_ms": ms.value}
+        return {"tile_rows": ty.value, "blocks": b.value, "stage_ms": ms.value, **self.split()}
 
     def use_graph(self, enable=True):
         _lib.check(self._lib.rpl_ensemble_use_graph(self.ptr, 1 if enable else 0))

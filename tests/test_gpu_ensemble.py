@@ -52,18 +52,25 @@ def test_ensemble_equals_members_stepped_alone(sources):
         m = build(w)
         ra.time_step_(m, dt, nsteps)
         ref.append(m.action)
-    for ty, blocks, graph in ((4, 0, False), (2, 64, False), (8, 2048, True), (4, 8, True)):
+    # split: members from that many tiles on run their own stage kernel
+    # (-1 none, 1 every member, 40 the larger ones here)
+    for ty, blocks, graph, split in ((4, 0, False, -1), (2, 64, False, 40), (8, 2048, True, -1), (4, 8, True, 40),
+                                     (4, 0, True, 1)):
         ms = [build(w) for w in wls]
         ens = ra.Ensemble(ms)
         ens.set_launch(ty, blocks)
+        ens.set_split(split)
         ens.use_graph(graph)
         ens.step(dt, nsteps - 1)
         ens.step(dt, 1)
         ens.synchronize()
+        launch = ens.launch()
+        if split == 40:
+            assert 0 < launch["own_members"] < len(ms), launch
         for k, (m, r) in enumerate(zip(ms, ref)):
             assert m.clock.iteration == nsteps
-            assert np.array_equal(m.action, r), (ty, blocks, graph, k, rel_l2(m.action, r))
-        say(f"ty={ty} blocks={blocks} graph={graph}: {ens.launch()} bitwise equal")
+            assert np.array_equal(m.action, r), (ty, blocks, graph, split, k, rel_l2(m.action, r))
+        say(f"ty={ty} blocks={blocks} graph={graph} split={split}: {launch} bitwise equal")
         ens.close()
 
 
@@ -79,16 +86,17 @@ def test_intrinsic_ensemble_equals_members_stepped_alone():
         assert m.transport == ra._lib.RPL_TRANSPORT_INTRINSIC
         ra.time_step_(m, dt, nsteps)
         ref.append(m.action)
-    for ty, blocks, graph in ((4, 0, False), (8, 64, True)):
+    for ty, blocks, graph, split in ((4, 0, False, -1), (8, 64, True, 40)):
         ms = [build(w, current=False) for w in wls]
         ens = ra.Ensemble(ms)
         ens.set_launch(ty, blocks)
+        ens.set_split(split)
         ens.use_graph(graph)
         ens.step(dt, nsteps)
         ens.synchronize()
         for k, (m, r) in enumerate(zip(ms, ref)):
-            assert np.array_equal(m.action, r), (ty, blocks, graph, k, rel_l2(m.action, r))
-        say(f"ty={ty} blocks={blocks} graph={graph}: bitwise equal")
+            assert np.array_equal(m.action, r), (ty, blocks, graph, split, k, rel_l2(m.action, r))
+        say(f"ty={ty} blocks={blocks} graph={graph} split={split}: {ens.launch()} bitwise equal")
         ens.close()
 
 
@@ -101,7 +109,7 @@ def test_autotune_keeps_results_and_state():
     best = ens.autotune(0.02, reps=2)
     say(f"autotune {best}")
     assert best["tile_rows"] in (2, 4, 8) and best["blocks"] % 8 == 0 and best["blocks"] >= 0
-    assert best["stage_ms"] > 0
+    assert best["stage_ms"] > 0 and 0 <= best["own_members"] <= len(ms)
     for m, b in zip(ms, before):
         assert np.array_equal(m.action, b)  # tuning leaves the state alone
     ens.use_graph(True)
@@ -183,19 +191,20 @@ def test_ensemble_with_surrogate_equals_members_stepped_alone():
         ra.time_step_(m, dt, nsteps)
         ref.append(m.action)
         del nets
-    for ty, blocks, graph in ((4, 0, False), (2, 64, True)):
+    for ty, blocks, graph, split in ((4, 0, False, -1), (2, 64, True, 40)):
         ms = [build(w) for w in wls]
         nets = []
         for k, m in enumerate(ms):
             nets += _attach_networks([m], [bins[k]], seed=k)
         ens = ra.Ensemble(ms)
         ens.set_launch(ty, blocks)
+        ens.set_split(split)
         ens.use_graph(graph)
         ens.step(dt, nsteps)
         ens.synchronize()
         for k, (m, r) in enumerate(zip(ms, ref)):
-            assert np.array_equal(m.action, r), (ty, blocks, graph, k, rel_l2(m.action, r))
-        say(f"surrogate ty={ty} blocks={blocks} graph={graph}: bitwise equal")
+            assert np.array_equal(m.action, r), (ty, blocks, graph, split, k, rel_l2(m.action, r))
+        say(f"surrogate ty={ty} blocks={blocks} graph={graph} split={split}: {ens.launch()} bitwise equal")
         ens.close()
 
 
