@@ -119,21 +119,19 @@ def storm_plain_vs_surrogate(steps, warmup, nx=1024, ny=1024):
     x 36 phi, FP32, intrinsic WENO5 transport, Holland wind input with stress
     cap and local-saturation whitecapping): RK3 step time with the plain
     source bundle and with the 1155 -> 512 x 3 -> 1152 network supplying the
-    sources on every column, and the device time of each step's kernels (HIP
-    events on the library stream).  The network weights are random (a timing
-    run: the work per step does not depend on them); the envelope is open so
-    that no column falls back."""
-    from ripple_amd import surrogate as sg
+    sources (the trained network shipped with the package, storm_surrogate;
+    its configuration's terms and training envelope, outside which a column
+    keeps the plain bundle), and the device time of each step's kernels (HIP
+    events on the library stream)."""
+    from ripple_amd import surrogate as sg, storm_surrogate
     wl = workloads.translating_hurricane(nx=nx, ny=ny)
     nx, ny, nk, nphi = wl.N0.shape
-    nb = nk * nphi
     m = build_model(wl, np.float32)
     del wl.N0
     t_plain, l_plain, n_plain = time_model(m, wl.dt, steps, warmup)
     l_state, n_state = phase_ms(m, "rpl_get_source_timing")
-    net = sg.SourceNetwork(nb + 3, nb, hidden=512, hidden_layers=3, batch_capacity=64, seed=1)
-    cfg = sg.SurrogateConfig(rate_unit=1e-7)
-    sg.attach_surrogate(m, net, cfg)
+    net, cfg, hi, _ = sg.load_network(storm_surrogate.NETWORK_FILE, batch_capacity=64)
+    sg.attach_surrogate(m, net, cfg, bin_action_hi=hi)
     ra.time_step_(m, wl.dt, 1)
     m.synchronize()
     phase_ms(m, "rpl_get_surrogate_timing")
@@ -153,8 +151,62 @@ def storm_plain_vs_surrogate(steps, warmup, nx=1024, ny=1024):
                           "inference_ms_per_stage": t_inf,
                           "mlp_TFLOPs": flops / (t_inf * 1e-3) / 1e12,
                           "mlp_frac_of_bf16_dense_peak": flops / (t_inf * 1e-3) / 2.5e15,
-                          "fallback_columns": st["plain"], "columns": st["columns"]},
+                          "fallback_columns": st["plain"], "columns": st["columns"],
+                          "network_terms": "every source term" if cfg.terms == 0 else "wind input (plain dissipation)"},
             "step_speedup_surrogate_over_plain": t_plain / t_surr}
+
+
+def storm_surrogate_accuracy(nx=1024, ny=1024, nsteps=None, network_file=None):
+    """The translating hurricane (nx x ny x 32 kappa x 36 phi) run for its
+    nsteps (60 x 60 s) four ways from the same JONSWAP seed: FP32 with the
+    plain source bundle, FP32 with the trained 1155 -> 512 x 3 -> 1152 network
+    supplying every source term (plain bundle only where a column leaves the
+    training envelope), FP64 with the plain bundle (the FP64 path parity-pinned
+    to the oracle), and FP32 without sources (the scale of the source terms'
+    effect).  Reports the significant-wave-height field's relative L2 error
+    and the domain-mean m0's relative error of each run against the plain
+    FP32 and the FP64 runs."""
+    from ripple_amd import storm_surrogate, surrogate as sg, diagnostics as dg
+    wl = workloads.translating_hurricane(nx=nx, ny=ny)
+    nsteps = wl.nsteps if nsteps is None else nsteps
+    net, cfg, hi, meta = sg.load_network(network_file or storm_surrogate.NETWORK_FILE, batch_capacity=64)
+    out, fields = {}, {}
+
+    def run(label, float_type, surrogate=False, sources=True):
+        kw = {} if sources else {"sources": None}
+        m = build_model(wl, float_type, **kw)
+        if surrogate:
+            sg.attach_surrogate(m, net, cfg, bin_action_hi=hi)
+        t0 = time.perf_counter()
+        ra.time_step_(m, wl.dt, nsteps)
+        m.synchronize()
+        t = time.perf_counter() - t0
+        hs = np.asarray(dg.significant_wave_height(m), dtype=np.float64)
+        fields[label] = (hs, float(np.mean(np.asarray(dg.m0(m), dtype=np.float64))))
+        rec = {"wall_s": t}
+        if surrogate:
+            rec["stats"] = sg.surrogate_stats(m)
+            sg.attach_surrogate(m, None, None)
+        out[label] = rec
+        del m
+
+    run("plain_f32", np.float32)
+    run("surrogate_f32", np.float32, surrogate=True)
+    run("plain_f64", np.float64)
+    run("no_sources_f32", np.float32, sources=False)
+    net.close()
+    for label in ("surrogate_f32", "plain_f32", "no_sources_f32"):
+        hs, m0 = fields[label]
+        for ref in ("plain_f32", "plain_f64"):
+            if ref == label:
+                continue
+            hr, m0r = fields[ref]
+            out[label][f"Hs_rel_l2_vs_{ref}"] = float(np.linalg.norm(hs - hr) / np.linalg.norm(hr))
+            out[label][f"m0_rel_vs_{ref}"] = float(abs(m0 - m0r) / m0r)
+            out[label][f"m0_ratio_to_{ref}"] = float(m0 / m0r)
+    return {"workload": wl.name, "steps": nsteps, "network": os.path.basename(network_file or
+                                                                               storm_surrogate.NETWORK_FILE),
+            "training": {k: meta[k] for k in ("pairs", "held_out_rate_rel_l2") if k in meta}, **out}
 
 
 def refraction_line(steps, warmup, nx, ny, float_type=np.float32, kernel="auto"):
@@ -458,6 +510,8 @@ def main():
                     help="print only the N x N sheared-eddy refraction line (for profiling its kernel)")
     ap.add_argument("--fp64-fast", action="store_true", help="with --only-refraction: FP64 on the fast kernel")
     ap.add_argument("--ensemble-members", type=int, default=64)
+    ap.add_argument("--only-surrogate-accuracy", action="store_true",
+                    help="print only the trained surrogate's accuracy on the 1024^2 hurricane")
     ap.add_argument("--only-ensemble", action="store_true",
                     help="print only the batched-ensemble lines (members stepped one by one vs batched)")
     ap.add_argument("--performance-summary", default=None,
@@ -494,6 +548,9 @@ def main():
         return
     if args.only_smoothing:
         print(json.dumps({"propagation_smoothing": smoothing_line(args.steps, args.warmup)}), flush=True)
+        return
+    if args.only_surrogate_accuracy:
+        print(json.dumps({"storm_surrogate_accuracy": storm_surrogate_accuracy()}), flush=True)
         return
     if args.only_headline:
         print(json.dumps({"storm_plain_vs_surrogate": storm_plain_vs_surrogate(args.steps, args.warmup)}), flush=True)
@@ -549,6 +606,7 @@ def main():
                                                                 kernel="fast")
         extra["err_vs_fp64"] = err_vs_fp64()
         extra["err_vs_fp64_sheared_eddies"] = err_vs_fp64_sheared_eddies()
+        extra["storm_surrogate_accuracy"] = storm_surrogate_accuracy()
         extra["pseudomomentum_coupling"] = pseudomomentum_line(max(3, args.steps // 2), 1)
         extra["propagation_smoothing"] = smoothing_line(max(3, args.steps // 2), 1)
         extra["nonlinear_dia"] = dia_line(max(3, args.steps // 4), 1)

@@ -308,6 +308,15 @@ int rpl_mlp_forward(rpl_mlp *net, int32_t batch, const float *X, float *Y);
 int rpl_mlp_gradients(rpl_mlp *net, int32_t batch, const float *X, const float *T, float *loss, float *grads);
 /* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) on the batch. */
 int rpl_mlp_train_step(rpl_mlp *net, int32_t batch, const float *X, const float *T, float lr, float *loss);
+/* `steps` Adam steps on a device-resident training set: X (rows x in_dim)
+ * and T (rows x out_dim), FP32 row-major DEVICE pointers on the network's
+ * device.  Each step draws `batch` rows uniformly with replacement (a
+ * counter-based hash of seed and the network's step count), so no host
+ * traffic is involved; the learning rate decays geometrically from lr0 to
+ * lr1 over the call.  losses (host, `steps` values, may be null) receives
+ * each step's mean squared error (reading it synchronizes every step). */
+int rpl_mlp_train_device(rpl_mlp *net, int64_t rows, const float *X, const float *T, int32_t batch, int32_t steps,
+                         float lr0, float lr1, uint64_t seed, float *losses);
 
 /* Surrogate use in the time step: on every stage the network maps each
  * column's features to per-bin rates r (source = r * rate_unit * sigma_bin * N).

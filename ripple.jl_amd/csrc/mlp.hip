@@ -415,6 +415,33 @@ __global__ __launch_bounds__(256) void to_bf16_kernel(const float *__restrict__ 
         dst[k] = f2bf(src[k]);
 }
 
+// One mini-batch row per workgroup from a device-resident training set: row
+// r < batch takes set row splitmix64(seed + step * 2^32 + r) mod rows (uniform
+// draws with replacement), converted to bf16 features and FP32 targets with
+// zero padding; rows past the batch are zero.
+__global__ __launch_bounds__(256) void gather_rows_kernel(const float *__restrict__ X, const float *__restrict__ T,
+                                                          unsigned long long rows, int in_dim, int in_pad,
+                                                          int out_dim, int out_pad, int batch,
+                                                          unsigned long long key, bf16 *__restrict__ Xb,
+                                                          float *__restrict__ Tb)
+{
+    const int r = blockIdx.x;
+    unsigned long long src = 0;
+    const bool live = r < batch;
+    if (live) {
+        unsigned long long z = key + (unsigned long long)r + 0x9E3779B97F4A7C15ull;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        src = (z ^ (z >> 31)) % rows;
+    }
+    const float *xs = X + src * (unsigned long long)in_dim;
+    const float *ts = T + src * (unsigned long long)out_dim;
+    for (int c = threadIdx.x; c < in_pad; c += 256)
+        Xb[(size_t)r * in_pad + c] = f2bf(live && c < in_dim ? xs[c] : 0.0f);
+    for (int c = threadIdx.x; c < out_pad; c += 256)
+        Tb[(size_t)r * out_pad + c] = live && c < out_dim ? ts[c] : 0.0f;
+}
+
 // Surrogate features of one 64-column x 64-feature tile per workgroup:
 // x = (log(N + floor) - shift) * scale for every bin, then U10x/ref,
 // U10y/ref, depth/ref, zero padding.  The action planes are read along i
@@ -634,6 +661,17 @@ hipError_t adam(float *w, const float *g, float *m, float *v, bf16 *wb, size_t n
     const float c1 = 1.0f - powf(b1, (float)step), c2 = 1.0f - powf(b2, (float)step);
     hipLaunchKernelGGL(adam_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, w, g, m, v, wb, n, lr, b1, b2,
                        eps, c1, c2);
+    return hipGetLastError();
+}
+
+hipError_t gather_rows(const float *X, const float *T, unsigned long long rows, int in_dim, int in_pad, int out_dim,
+                       int out_pad, int batch, int rows_pad, unsigned long long seed, unsigned long long step,
+                       bf16 *Xb, float *Tb, hipStream_t s)
+{
+    if (rows == 0 || batch > rows_pad)
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL(gather_rows_kernel, dim3((unsigned)rows_pad), dim3(256), 0, s, X, T, rows, in_dim, in_pad,
+                       out_dim, out_pad, batch, seed + (step << 32), Xb, Tb);
     return hipGetLastError();
 }
 

@@ -179,14 +179,15 @@ int backward(rpl_mlp *n, int rows, int batch, float *loss)
             dZ = bufs[l & 1];
         }
     }
+    if (!loss)
+        return RPL_OK;
     std::vector<float> part(blocks);
     MLP_HIP(hipStreamSynchronize(n->stream));
     MLP_HIP(hipMemcpy(part.data(), n->partials, blocks * 4, hipMemcpyDeviceToHost));
     double s = 0;
     for (float v : part)
         s += v;
-    if (loss)
-        *loss = (float)(s / ((double)batch * n->out_dim));
+    *loss = (float)(s / ((double)batch * n->out_dim));
     return RPL_OK;
 }
 
@@ -426,6 +427,7 @@ int rpl_mlp_gradients(rpl_mlp *n, int32_t batch, const float *X, const float *T,
     if ((rc = load_inputs(n, batch, X)) || (rc = load_targets(n, batch, T)) || (rc = forward_train(n, rows)) ||
         (rc = backward(n, rows, batch, loss)))
         return rc;
+    MLP_HIP(hipStreamSynchronize(n->stream));  // the stream does not order with the copies below
     if (grads) {
         size_t off = 0;
         for (int l = 0; l < nlayers(n); ++l) {
@@ -436,6 +438,41 @@ int rpl_mlp_gradients(rpl_mlp *n, int32_t batch, const float *X, const float *T,
             off += nb;
         }
     }
+    return RPL_OK;
+}
+
+int rpl_mlp_train_device(rpl_mlp *n, int64_t rows, const float *X, const float *T, int32_t batch, int32_t steps,
+                         float lr0, float lr1, uint64_t seed, float *losses)
+{
+    int rc = check_batch(n, batch);
+    if (rc)
+        return rc;
+    if (!X || !T || rows < 1 || steps < 0 || !(lr0 > 0) || !(lr1 > 0))
+        return set_err(RPL_ERR_ARG, "need device rows, rows >= 1, steps >= 0 and positive learning rates");
+    hipPointerAttribute_t ax, at;
+    if (hipPointerGetAttributes(&ax, X) != hipSuccess || hipPointerGetAttributes(&at, T) != hipSuccess ||
+        ax.type != hipMemoryTypeDevice || at.type != hipMemoryTypeDevice || ax.device != n->device ||
+        at.device != n->device) {
+        (void)hipGetLastError();
+        return set_err(RPL_ERR_ARG, "training rows must be device memory on the network's device");
+    }
+    const int rp = round64(batch);
+    for (int k = 0; k < steps; ++k) {
+        const float lr = lr0 * (float)std::pow((double)lr1 / lr0, steps > 1 ? (double)k / (steps - 1) : 0.0);
+        MLP_HIP(rpl::mlp::gather_rows(X, T, (unsigned long long)rows, n->in_dim, n->dims[0], n->out_dim,
+                                      n->dims.back(), batch, rp, seed, (unsigned long long)n->step, n->X, n->T,
+                                      n->stream));
+        if ((rc = forward_train(n, rp)) || (rc = backward(n, rp, batch, losses ? losses + k : nullptr)))
+            return rc;
+        n->step += 1;
+        for (int l = 0; l < nlayers(n); ++l) {
+            MLP_HIP(rpl::mlp::adam(n->W[l], n->gW[l], n->mW[l], n->vW[l], n->Wb[l], layer_w(n, l), lr, 0.9f, 0.999f,
+                                   1e-8f, n->step, n->stream));
+            MLP_HIP(rpl::mlp::adam(n->b[l], n->gb[l], n->mb[l], n->vb[l], nullptr, n->dims[l + 1], lr, 0.9f, 0.999f,
+                                   1e-8f, n->step, n->stream));
+        }
+    }
+    MLP_HIP(hipStreamSynchronize(n->stream));
     return RPL_OK;
 }
 

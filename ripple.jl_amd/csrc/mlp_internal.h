@@ -54,6 +54,12 @@ hipError_t mse(const float *Y, const float *T, int rows, int rows_valid, int col
 hipError_t adam(float *w, const float *g, float *m, float *v, bf16 *wb, size_t n, float lr, float b1, float b2,
                 float eps, int step, hipStream_t s);
 hipError_t to_bf16(const float *src, bf16 *dst, size_t n, hipStream_t s);
+// Mini-batch of `batch` rows drawn on the device from a training set of
+// `rows` rows (X: rows x in_dim, T: rows x out_dim, FP32) into Xb (rows_pad x
+// in_pad, bf16) and Tb (rows_pad x out_pad, FP32), zero padded.
+hipError_t gather_rows(const float *X, const float *T, unsigned long long rows, int in_dim, int in_pad, int out_dim,
+                       int out_pad, int batch, int rows_pad, unsigned long long seed, unsigned long long step,
+                       bf16 *Xb, float *Tb, hipStream_t s);
 // Features of columns [col0, col0 + ncols) (ncols a multiple of 64) into X
 // rows 0..ncols-1; columns at or past `plane` get zero features.  With amax
 // and flags (both cleared by envelope_clear first), also the column's largest

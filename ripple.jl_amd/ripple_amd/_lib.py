@@ -131,6 +131,8 @@ SIGNATURES = {
     "rpl_mlp_forward": ([_P, _I32, _FP, _FP], ctypes.c_int),
     "rpl_mlp_gradients": ([_P, _I32, _FP, _FP, _FP, _FP], ctypes.c_int),
     "rpl_mlp_train_step": ([_P, _I32, _FP, _FP, ctypes.c_float, _FP], ctypes.c_int),
+    "rpl_mlp_train_device": ([_P, ctypes.c_int64, _P, _P, _I32, _I32, ctypes.c_float, ctypes.c_float, ctypes.c_uint64,
+                              _FP], ctypes.c_int),
     "rpl_attach_surrogate": ([_P, _P, ctypes.POINTER(rpl_surrogate), _FP], ctypes.c_int),
     "rpl_surrogate_stats": ([_P, _I64P, _I64P, _I64P, _I64P, _DP], ctypes.c_int),
     "rpl_surrogate_training_pairs": ([_P, ctypes.POINTER(rpl_surrogate), _FP, _FP], ctypes.c_int),

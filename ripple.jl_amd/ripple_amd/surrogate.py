@@ -233,6 +233,141 @@ def envelope_from_pairs(X, cfg, margin=0.02):
     return cfg
 
 
+def _device_rows(A, width, name, rows):
+    """(device address, rows) of a training array: a contiguous FP32 torch GPU
+    tensor of rows x width, or a raw device address (int or c_void_p) with
+    `rows` given."""
+    if isinstance(A, (int, ctypes.c_void_p)):
+        if rows is None or rows < 1:
+            raise ValueError(f"{name} given as a device address needs rows >= 1")
+        return ctypes.c_void_p(A.value if isinstance(A, ctypes.c_void_p) else A), int(rows)
+    import torch  # device memory for training sets
+    if not isinstance(A, torch.Tensor) or not A.is_cuda or A.dtype != torch.float32 or A.dim() != 2 \
+            or A.shape[1] != width or not A.is_contiguous():
+        raise ValueError(f"{name} must be a contiguous FP32 GPU tensor of rows x {width}")
+    torch.cuda.synchronize(A.device)  # the rows are written on torch's stream
+    return ctypes.c_void_p(A.data_ptr()), int(A.shape[0])
+
+
+def train_device(net, X, T, *, steps, batch=4096, lr=1e-3, lr_final=None, seed=0, log_every=0, rows=None):
+    """Adam on mini-batches drawn on the device (rpl_mlp_train_device) from a
+    training set held in GPU memory: X (rows x in_dim) and T (rows x out_dim),
+    FP32 row-major, either contiguous torch GPU tensors or device addresses
+    with `rows` given.  The learning rate decays geometrically from lr to
+    lr_final (default lr/10) over all steps.  Returns the loss history."""
+    xp, xr = _device_rows(X, net.in_dim, "X", rows)
+    tp, tr = _device_rows(T, net.out_dim, "T", rows)
+    if xr != tr:
+        raise ValueError("X and T need the same number of rows")
+    batch = min(batch, net.batch_capacity)
+    lr_final = lr / 10 if lr_final is None else lr_final
+    losses = np.zeros(steps, dtype=np.float32)
+    chunk = log_every if log_every else max(steps, 1)
+    span = max(steps - 1, 1)
+    done = 0
+    while done < steps:
+        n = min(chunk, steps - done)
+        lr0 = lr * (lr_final / lr) ** (done / span)
+        lr1 = lr * (lr_final / lr) ** ((done + n - 1) / span)
+        part = losses[done:done + n]
+        _lib.check(net._lib.rpl_mlp_train_device(net.ptr, xr, xp, tp, batch, n, lr0, lr1, ctypes.c_uint64(seed),
+                                                 _lib.fptr(part)))
+        done += n
+        if log_every:
+            print(f"step {done}: loss {float(np.mean(part)):.4e} (mean of the last {n}), lr {lr1:.2e}", flush=True)
+    return losses
+
+
+def _standardisation(T):
+    T = np.asarray(T, dtype=np.float64)
+    mt, st = T.mean(axis=0), T.std(axis=0)
+    return mt, np.maximum(st, 1e-3 * max(float(np.sqrt(np.mean(T * T))), 1e-30))
+
+
+def _fold_output_affine(net, mt, st):
+    """Last layer <- st * (last layer) + mt, so the network emits unstandardised rates."""
+    flat = net.params().astype(np.float64)
+    layers = net.layers(flat)
+    WL, bL = layers[-1]
+    n_out = net.out_dim
+    WL[:n_out, :] *= st[:, None]
+    bL[:n_out] = st * bL[:n_out] + mt
+    net.set_params(flat.astype(np.float32))
+
+
+def fit_device(net, X, T, *, steps, batch=4096, lr=1e-3, lr_final=None, seed=0, log_every=0, chunk=1 << 18):
+    """fit() on a GPU-resident training set (torch tensors as train_device
+    takes them).  T is standardised IN PLACE (per output bin, mean and std
+    accumulated in FP64 over row chunks) and the affine map is folded into
+    the last layer afterwards.  Returns the loss history (standardised units)."""
+    import torch
+    _device_rows(T, net.out_dim, "T", None)
+    rows = T.shape[0]
+    s1 = torch.zeros(net.out_dim, dtype=torch.float64, device=T.device)
+    s2 = torch.zeros_like(s1)
+    for a in range(0, rows, chunk):
+        c = T[a:a + chunk].double()
+        s1 += c.sum(0)
+        s2 += (c * c).sum(0)
+    mt = s1 / rows
+    var = torch.clamp(s2 / rows - mt * mt, min=0.0)
+    rms = float(torch.sqrt(torch.sum(s2) / (rows * net.out_dim)))
+    st = torch.clamp(torch.sqrt(var), min=1e-3 * max(rms, 1e-30))
+    for a in range(0, rows, chunk):
+        T[a:a + chunk].sub_(mt.float()).div_(st.float())
+    losses = train_device(net, X, T, steps=steps, batch=batch, lr=lr, lr_final=lr_final, seed=seed,
+                          log_every=log_every)
+    _fold_output_affine(net, mt.cpu().numpy(), st.cpu().numpy())
+    return losses
+
+
+def _bf16_bits(a):
+    """Round-to-nearest-even bf16 bit patterns of FP32 values (NaN kept NaN)."""
+    u = np.ascontiguousarray(a, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    nan = (u & 0x7FFFFFFF) > 0x7F800000
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+    r[nan] = ((u[nan] >> 16) | 0x40).astype(np.uint16)
+    return r
+
+
+def save_network(path, net, cfg, bin_action_hi=None, **meta):
+    """Write a trained network for inference: weights as bf16 (the precision
+    the inference GEMMs read them in, so nothing is lost), biases FP32, the
+    surrogate configuration and optional per-bin action bounds, plus free-form
+    metadata (JSON-serialisable), to an .npz file."""
+    import json
+    arrays = {}
+    for l, (W, b) in enumerate(net.layers()):
+        arrays[f"W{l}"] = _bf16_bits(W)
+        arrays[f"b{l}"] = np.asarray(b, dtype=np.float32)
+    head = {"in_dim": net.in_dim, "out_dim": net.out_dim, "hidden": net.hidden,
+            "hidden_layers": net.hidden_layers, "config": {f.name: getattr(cfg, f.name) for f in fields(cfg)},
+            "meta": meta}
+    arrays["header"] = np.frombuffer(json.dumps(head).encode(), dtype=np.uint8)
+    if bin_action_hi is not None:
+        arrays["bin_action_hi"] = np.asarray(bin_action_hi, dtype=np.float32)
+    np.savez_compressed(path, **arrays)
+
+
+def load_network(path, batch_capacity=64):
+    """(network, SurrogateConfig, bin_action_hi or None, metadata) from save_network's file."""
+    import json
+    with np.load(path, allow_pickle=False) as z:
+        head = json.loads(bytes(z["header"]).decode())
+        net = SourceNetwork(head["in_dim"], head["out_dim"], hidden=head["hidden"],
+                            hidden_layers=head["hidden_layers"], batch_capacity=batch_capacity)
+        parts = []
+        for l in range(len(net.dims) - 1):
+            W = (z[f"W{l}"].astype(np.uint32) << 16).view(np.float32)
+            if W.shape != (net.dims[l + 1], net.dims[l]):
+                net.close()
+                raise ValueError(f"layer {l} has shape {W.shape}, expected {(net.dims[l + 1], net.dims[l])}")
+            parts += [W.ravel(), z[f"b{l}"].astype(np.float32)]
+        net.set_params(np.concatenate(parts))
+        hi = z["bin_action_hi"].astype(np.float64) if "bin_action_hi" in z.files else None
+    return net, SurrogateConfig(**head["config"]), hi, head["meta"]
+
+
 def fit(net, X, T, *, steps, batch=2048, lr=1e-3, lr_final=None, seed=0, log_every=0):
     """Train `net` to map feature rows X to target rows T (both as returned by
     `training_pairs`), with every output bin standardised over the training
@@ -242,17 +377,9 @@ def fit(net, X, T, *, steps, batch=2048, lr=1e-3, lr_final=None, seed=0, log_eve
     folding 1/std into bf16 first-layer weights would cancel catastrophically
     against the folded mean; the feature scaling of the configuration keeps
     them O(1).  Returns the loss history (in standardised units)."""
-    T = np.asarray(T, dtype=np.float64)
-    mt, st = T.mean(axis=0), T.std(axis=0)
-    st = np.maximum(st, 1e-3 * max(float(np.sqrt(np.mean(T * T))), 1e-30))
-    Tn = ((T - mt) / st).astype(np.float32)
+    mt, st = _standardisation(T)
+    Tn = ((np.asarray(T, dtype=np.float64) - mt) / st).astype(np.float32)
     losses = train(net, np.asarray(X, dtype=np.float32), Tn, steps=steps, batch=batch, lr=lr, lr_final=lr_final,
                    seed=seed, log_every=log_every)
-    flat = net.params().astype(np.float64)
-    layers = net.layers(flat)
-    WL, bL = layers[-1]
-    n_out = net.out_dim
-    WL[:n_out, :] *= st[:, None]
-    bL[:n_out] = st * bL[:n_out] + mt
-    net.set_params(flat.astype(np.float32))
+    _fold_output_affine(net, mt, st)
     return losses

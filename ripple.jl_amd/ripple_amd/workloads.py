@@ -209,6 +209,68 @@ def translating_hurricane(float_type=np.float32, nx=1024, ny=1024, nz=32, nkappa
                     dict(timestepper="RK3", sources=sources, depth=200.0))
 
 
+def hurricane_training_storm(index, float_type=np.float32, nx=128, ny=128, nz=8, nkappa=32, nphi=36, dt=60.0,
+                             nsteps=120, seed_base=1000):
+    """A randomised relative of translating_hurricane for surrogate training,
+    drawn from seed seed_base + index: the same 800 km box, depth 200 m and
+    32 x 36 polar grid at nx x ny cells; Holland storm of vmax uniform in
+    25-60 m/s, rmax 20-60 km, radius 200-400 km, B 1.0-2.0, inflow 10-30 deg,
+    counter-clockwise, starting anywhere in the box and translating at 0-10
+    m/s in a random direction; JONSWAP seed of Hs 0.2-1.5 m and peak
+    frequency 0.1-0.25 Hz (cos^2 about phi = 0); same source bundle."""
+    from .sources import (PrecomputedSources, PressureCorrelationInput, LocalSaturationDissipation, BulkWindDrag,
+                          HollandHurricaneWind, LinearStormTrack)
+    rng = np.random.Generator(np.random.Philox(seed_base + index))
+    L = 800e3
+    grid = RectilinearGrid(float_type=float_type, size=(nx, ny, nz), x=(0, L), y=(-L / 2, L / 2),
+                           z=(-200.0, 0.0), topology=(PERIODIC, PERIODIC, BOUNDED))
+    sg = PolarWaveVectorGrid(float_type, kappa=np.geomspace(0.05, 2.0, nkappa),
+                             phi=np.linspace(0, 2 * math.pi, nphi + 1)[:nphi])
+    vmax = float(rng.uniform(25.0, 60.0))
+    rmax = float(rng.uniform(20e3, 60e3))
+    radius = float(rng.uniform(200e3, 400e3))
+    shape = float(rng.uniform(1.0, 2.0))
+    inflow = math.radians(float(rng.uniform(10.0, 30.0)))
+    speed, heading = float(rng.uniform(0.0, 10.0)), float(rng.uniform(0.0, 2 * math.pi))
+    x0, y0 = float(rng.uniform(0.0, L)), float(rng.uniform(-L / 2, L / 2))
+    Hs, fp = float(rng.uniform(0.2, 1.5)), float(rng.uniform(0.1, 0.25))
+    T = 10 * 86400.0
+    track = LinearStormTrack([0.0, T], [(x0, y0), (x0 + speed * T * math.cos(heading),
+                                                   y0 + speed * T * math.sin(heading))])
+    wind = HollandHurricaneWind(center=track, vmax=vmax, rmax=rmax, radius=radius, shape_parameter=shape,
+                                inflow_angle=inflow)
+    sources = PrecomputedSources(wind_input=PressureCorrelationInput(drag=BulkWindDrag("linear"), wind=wind),
+                                 dissipation=LocalSaturationDissipation(B_r=1.05e-2))
+    name = (f"hurricane_training_storm_{index}_{nx}x{ny}_v{vmax:.0f}_r{rmax / 1e3:.0f}km_"
+            f"u{speed:.1f}_Hs{Hs:.2f}")
+    return Workload(name, grid, sg, None, jonswap_action(grid, sg, Hs, fp), dt, nsteps,
+                    dict(timestepper="RK3", sources=sources, depth=200.0))
+
+
+def uniform_wind_sea(index, float_type=np.float32, nx=64, ny=64, nz=8, nkappa=32, nphi=36, dt=60.0, nsteps=60,
+                     wind_range=(5.0, 40.0), seed_base=2000):
+    """Surrogate training sea under a uniform wind, drawn from seed seed_base +
+    index: the hurricane's box, depth and 32 x 36 grid at nx x ny cells;
+    U10 uniform in wind_range from a random direction; JONSWAP seed of Hs
+    0.2-1.5 m, peak 0.1-0.25 Hz (cos^2 about phi = 0); same source bundle."""
+    from .sources import PrecomputedSources, PressureCorrelationInput, LocalSaturationDissipation, BulkWindDrag
+    rng = np.random.Generator(np.random.Philox(seed_base + index))
+    L = 800e3
+    grid = RectilinearGrid(float_type=float_type, size=(nx, ny, nz), x=(0, L), y=(-L / 2, L / 2),
+                           z=(-200.0, 0.0), topology=(PERIODIC, PERIODIC, BOUNDED))
+    sg = PolarWaveVectorGrid(float_type, kappa=np.geomspace(0.05, 2.0, nkappa),
+                             phi=np.linspace(0, 2 * math.pi, nphi + 1)[:nphi])
+    U = float(rng.uniform(*wind_range))
+    direction = float(rng.uniform(0.0, 2 * math.pi))
+    Hs, fp = float(rng.uniform(0.2, 1.5)), float(rng.uniform(0.1, 0.25))
+    sources = PrecomputedSources(
+        wind_input=PressureCorrelationInput(drag=BulkWindDrag("linear"), wind=U, direction=direction),
+        dissipation=LocalSaturationDissipation(B_r=1.05e-2))
+    return Workload(f"uniform_wind_sea_{index}_{nx}x{ny}_U{U:.1f}_Hs{Hs:.2f}", grid, sg, None,
+                    jonswap_action(grid, sg, Hs, fp), dt, nsteps, dict(timestepper="RK3", sources=sources,
+                                                                       depth=200.0))
+
+
 def ensemble_member(index, float_type=np.float32, nz=16, sources=True, size_range=(64, 512),
                     kappa_range=(16, 40), phi_range=(24, 48), seed_base=0):
     """One member of the batched ensemble, drawn from seed seed_base + index:

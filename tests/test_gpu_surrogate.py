@@ -12,6 +12,7 @@ import mlp_oracle as mo
 import ripple_amd as ra
 from ripple_amd import grids, workloads
 from ripple_amd import surrogate as sg
+from hip_buffers import HipBuffers
 
 pytestmark = pytest.mark.gpu
 
@@ -133,6 +134,122 @@ def test_training_reduces_loss():
     say(f"training loss {l0:.3e} -> {l1:.3e}")
     assert l1 < l0 / 10
     assert np.all(np.isfinite(losses))
+    net.close()
+
+
+def _drawn_rows(seed, step, batch, rows):
+    """The rows rpl_mlp_train_device draws for one step: splitmix64 of
+    seed + step * 2^32 + r + golden-ratio increment, modulo rows."""
+    m64 = (1 << 64) - 1
+    out = []
+    for r in range(batch):
+        z = (seed + (step << 32) + r + 0x9E3779B97F4A7C15) & m64
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & m64
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & m64
+        out.append((z ^ (z >> 31)) % rows)
+    return np.array(out)
+
+
+def test_device_training_equals_host_steps_on_the_drawn_rows():
+    """rpl_mlp_train_device draws its batches on the device; two steps of it
+    give the same parameters, bitwise, as two host train_step calls on the
+    rows the hash draws (same bf16 conversion, padding and Adam)."""
+    say("mlp device training vs host steps")
+    rng = np.random.default_rng(9)
+    rows, batch = 1000, 100
+    X = rng.normal(size=(rows, 70)).astype(np.float32)
+    T = (0.3 * rng.normal(size=(rows, 50))).astype(np.float32)
+    a = sg.SourceNetwork(70, 50, hidden=64, hidden_layers=2, batch_capacity=128, seed=4)
+    b = sg.SourceNetwork(70, 50, hidden=64, hidden_layers=2, batch_capacity=128, seed=4)
+    seed = 12345
+    hip = HipBuffers()
+    Xd, Td = hip.upload(X), hip.upload(T)
+    losses = sg.train_device(a, Xd, Td, rows=rows, steps=2, batch=batch, lr=2e-3, lr_final=1e-3, seed=seed)
+    hip.free()
+    host = []
+    for step, lr in ((0, 2e-3), (1, 1e-3)):
+        idx = _drawn_rows(seed, step, batch, rows)
+        host.append(b.train_step(X[idx], T[idx], lr))
+    pa, pb = a.params(), b.params()
+    say(f"losses device {losses} host {host}; max |dp| {np.max(np.abs(pa - pb)):.3e}")
+    assert np.array_equal(pa, pb)
+    assert np.allclose(losses, host, rtol=1e-6)
+    a.close()
+    b.close()
+
+
+def test_device_training_reduces_loss_and_checks_its_rows():
+    import ctypes
+    say("mlp device training")
+    rng = np.random.default_rng(5)
+    net = sg.SourceNetwork(48, 40, hidden=128, hidden_layers=2, batch_capacity=512, seed=3)
+    X = rng.uniform(-1, 1, size=(4096, 48)).astype(np.float32)
+    A = rng.normal(size=(48, 40)) / 7
+    T = np.tanh(X @ A).astype(np.float32) + 0.5 * np.sin(X[:, :40] * 2).astype(np.float32)
+    l0, _ = net.gradients(X[:512], T[:512])
+    hip = HipBuffers()
+    losses = sg.train_device(net, hip.upload(X), hip.upload(T), rows=X.shape[0], steps=400, batch=512, lr=2e-3,
+                             log_every=100)
+    hip.free()
+    l1, _ = net.gradients(X[:512], T[:512])
+    say(f"device training loss {l0:.3e} -> {l1:.3e}")
+    assert l1 < l0 / 10 and np.all(np.isfinite(losses))
+    # host memory is refused before any launch
+    rc = net._lib.rpl_mlp_train_device(net.ptr, X.shape[0], ctypes.c_void_p(X.ctypes.data),
+                                       ctypes.c_void_p(T.ctypes.data), 64, 1, 1e-3, 1e-3, ctypes.c_uint64(0), None)
+    assert rc == -1  # RPL_ERR_ARG
+    with pytest.raises(ValueError):
+        sg.train_device(net, 1 << 40, 1 << 41, steps=1)  # an address without its row count
+    net.close()
+
+
+def test_saved_network_infers_identically(tmp_path):
+    """save_network keeps bf16 weights (what the inference GEMMs read) and
+    FP32 biases: the reloaded network's outputs are bitwise the original's."""
+    say("mlp save / load")
+    net, X, T = _net_and_batch(6)
+    for _ in range(3):
+        net.train_step(X, T, 1e-3)
+    cfg = sg.SurrogateConfig(rate_unit=3e-6, scale=0.07, terms=1, wind_hi=33.0)
+    hi = np.linspace(1.0, 2.0, 70)
+    path = tmp_path / "net.npz"
+    sg.save_network(path, net, cfg, bin_action_hi=hi, note="test")
+    net2, cfg2, hi2, meta = sg.load_network(path, batch_capacity=256)
+    assert np.array_equal(net.forward(X), net2.forward(X))
+    assert cfg2 == cfg and meta == {"note": "test"}
+    assert np.array_equal(hi2, hi.astype(np.float32))
+    net.close()
+    net2.close()
+
+
+def test_shipped_storm_network_tracks_the_plain_bundle():
+    """The trained hurricane network shipped with the package, on a 128^2
+    translating hurricane (a coarser relative of the 1024^2 case it was
+    trained for) over 30 steps of 60 s: the significant wave height stays
+    within 5 % (relative L2) of the plain-source run, the domain-mean m0
+    within 2 %, while dropping the sources moves Hs by tens of percent."""
+    from ripple_amd import storm_surrogate, diagnostics as dg
+    say("shipped storm surrogate")
+    net, cfg, hi, meta = sg.load_network(storm_surrogate.NETWORK_FILE)
+    wl = workloads.translating_hurricane(nx=128, ny=128, nz=8)
+    runs = {}
+    for label in ("plain", "surrogate", "none"):
+        m = _model(wl, np.float32, sources=label != "none")
+        if label == "surrogate":
+            sg.attach_surrogate(m, net, cfg, bin_action_hi=hi)
+        ra.time_step_(m, wl.dt, 30)
+        runs[label] = (np.asarray(dg.significant_wave_height(m), np.float64),
+                       float(np.mean(np.asarray(dg.m0(m), np.float64))))
+        if label == "surrogate":
+            st = sg.surrogate_stats(m)
+            sg.attach_surrogate(m, None, None)
+    e = rel_l2(runs["surrogate"][0], runs["plain"][0])
+    e_none = rel_l2(runs["none"][0], runs["plain"][0])
+    e_m0 = abs(runs["surrogate"][1] / runs["plain"][1] - 1)
+    say(f"meta {dict((k, meta[k]) for k in ('pairs', 'terms', 'held_out_rate_rel_l2'))}; stats {st}; "
+        f"Hs rel-L2 {e:.3e} (no sources {e_none:.3e}); m0 {e_m0:.3e}")
+    assert st["plain"] < 0.1 * st["columns"]
+    assert e < 0.05 and e_m0 < 0.02 and e_none > 0.2
     net.close()
 
 

@@ -12,6 +12,7 @@ import pytest
 
 import ripple_amd as ra
 from ripple_amd import grids, workloads
+from hip_buffers import HipBuffers
 
 pytestmark = pytest.mark.gpu
 
@@ -32,31 +33,7 @@ def _model(nx, ny, nk, nphi, float_type, seed):
     return m, sg
 
 
-class _Hip:
-    """The caller's own device buffers, from the HIP runtime the library
-    itself links (loaded already by the library, so one runtime per process)."""
-
-    def __init__(self):
-        self.rt = ctypes.CDLL("libamdhip64.so.7")
-        self.bufs = []
-
-    def upload(self, host):
-        host = np.ascontiguousarray(host)
-        p = ctypes.c_void_p()
-        assert self.rt.hipMalloc(ctypes.byref(p), ctypes.c_size_t(max(host.nbytes, 8))) == 0
-        self.bufs.append(p)
-        assert self.rt.hipMemcpy(p, host.ctypes.data_as(ctypes.c_void_p), ctypes.c_size_t(host.nbytes), 1) == 0
-        return p
-
-    def download(self, p, out):
-        assert self.rt.hipDeviceSynchronize() == 0
-        assert self.rt.hipMemcpy(out.ctypes.data_as(ctypes.c_void_p), p, ctypes.c_size_t(out.nbytes), 2) == 0
-        return out
-
-    def free(self):
-        for p in self.bufs:
-            self.rt.hipFree(p)
-        self.bufs = []
+_Hip = HipBuffers
 
 
 def _noalloc_total(lib, hip, N, measure, area, dtype_code):
