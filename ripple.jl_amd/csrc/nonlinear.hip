@@ -7,11 +7,13 @@
 // (1 +- lambda) sigma and directions theta +- offsets, read and updated
 // through bilinear corners.  Where the receivers fall depends only on the
 // spectral grid, so the host tabulates, per donor and orientation, the
-// corner bins and weights (DiaTables); a workgroup then holds the action and
-// the transfer of a few adjacent columns in LDS, walks every (donor, column)
-// pair, and scatters the transfer with LDS atomic adds.  The last phase turns
-// the transfer into the source the stage adds, T where T >= 0 and
-// -min(-T / max(N, eps), 1) N otherwise (symmetric_quadruplet.jl:210-222).
+// corner bins and weights (DiaTables) and the transposed list of what each
+// bin receives; a workgroup then holds the action and the transfer of a few
+// adjacent columns in LDS, computes every donor's transfer from its
+// receivers' corners, and each bin gathers what its donors send it (no
+// atomics).  The last phase turns the transfer into the source the stage
+// adds, T where T >= 0 and -min(-T / max(N, eps), 1) N otherwise
+// (symmetric_quadruplet.jl:210-222).
 #include "rpl_internal.h"
 
 #pragma clang fp contract(off)
@@ -40,7 +42,7 @@ template <typename T>
 __global__ __launch_bounds__(DIA_THREADS) void dia_source_kernel(size_t plane, int nbins, int cols, long long tiles,
                                                          const T *__restrict__ N, DiaTables<T> tb, T *__restrict__ out)
 {
-    extern __shared__ unsigned char dia_lds[];
+    extern __shared__ __align__(16) unsigned char dia_lds[];
     T *Nl = reinterpret_cast<T *>(dia_lds);
     T *TEl = Nl + (size_t)nbins * cols;  // [2 nbins][column]
     const size_t ij0 = (size_t)dia_xcd_order(blockIdx.x, tiles) * cols;
@@ -105,6 +107,187 @@ __global__ __launch_bounds__(DIA_THREADS) void dia_source_kernel(size_t plane, i
     }
 }
 
+// The same transfer with the columns as a short vector: a workgroup takes
+// COLS adjacent columns (4 in FP32, 2 in FP64: 16 bytes per bin), and thread
+// t owns bins and donors t, t + 512, ... (DPT of them) in every phase, for
+// all COLS columns at once: one table load per donor instead of one per
+// (donor, column), one 16-byte LDS read per corner.  The transfer of the
+// thread's donors stays in registers until every corner read of the action
+// is done, then overwrites the action in LDS ([2 nbins][COLS]), so the tile
+// needs 2 nbins COLS elements of LDS instead of 3; the thread's own action
+// stays in registers for the last phase.  Every value follows the operation
+// order of dia_source_kernel, so both kernels give the same bits.
+template <typename T, int COLS, int DPT>
+__global__ __launch_bounds__(DIA_THREADS) void dia_source_vec_kernel(size_t plane, int nbins, long long tiles,
+                                                             const T *__restrict__ N, DiaTables<T> tb,
+                                                             T *__restrict__ out)
+{
+    extern __shared__ __align__(16) unsigned char dia_lds[];
+    T *L = reinterpret_cast<T *>(dia_lds);
+    const size_t ij0 = (size_t)dia_xcd_order(blockIdx.x, tiles) * COLS;
+    const int ncol = (int)min((size_t)COLS, plane - ij0);
+    const int tid = threadIdx.x;
+    typedef T VecT __attribute__((ext_vector_type(COLS)));
+    // whole 16-byte groups: every bin's COLS values are one aligned vector
+    const bool whole = ncol == COLS && plane % COLS == 0;
+    T own[DPT][COLS];
+#pragma unroll
+    for (int k = 0; k < DPT; ++k) {
+        const int bin = tid + DIA_THREADS * k;
+        if (bin < nbins) {
+            if (whole) {
+                const VecT v = *reinterpret_cast<const VecT *>(N + ij0 + plane * (size_t)bin);
+#pragma unroll
+                for (int c = 0; c < COLS; ++c)
+                    own[k][c] = v[c];
+            } else {
+#pragma unroll
+                for (int c = 0; c < COLS; ++c)
+                    own[k][c] = c < ncol ? N[ij0 + c + plane * (size_t)bin] : T(0);
+            }
+            VecT v;
+#pragma unroll
+            for (int c = 0; c < COLS; ++c)
+                v[c] = own[k][c];
+            *reinterpret_cast<VecT *>(L + bin * COLS) = v;
+        }
+    }
+    __syncthreads();
+    T te[DPT][2][COLS];
+#pragma unroll
+    for (int k = 0; k < DPT; ++k) {
+        const int d = tid + DIA_THREADS * k;
+#pragma unroll
+        for (int o = 0; o < 2; ++o)
+#pragma unroll
+            for (int c = 0; c < COLS; ++c)
+                te[k][o][c] = T(0);
+        if (d >= nbins)
+            continue;
+        bool any = false;
+#pragma unroll
+        for (int c = 0; c < COLS; ++c)
+            any = any || own[k][c] > T(0);
+        if (!any)
+            continue;
+        const T efa = tb.ef_a[d], cfa = tb.cf_a[d];
+        typedef int Int4 __attribute__((ext_vector_type(4)));
+#pragma unroll
+        for (int o = 0; o < 2; ++o) {
+            const int q = 2 * d + o;
+            const int fl = tb.flags[q];
+            const Int4 *bq = reinterpret_cast<const Int4 *>(tb.bins + 8 * (size_t)q);
+            const T *wef = tb.wef + 8 * (size_t)q;
+            T Ep[COLS], Em[COLS];
+#pragma unroll
+            for (int c = 0; c < COLS; ++c)
+                Ep[c] = Em[c] = T(0);
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                if (!(fl & (1 << h)))
+                    continue;
+                const Int4 b4 = bq[h];
+                T acc[COLS];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const T w = wef[4 * h + j];
+                    const VecT row = *reinterpret_cast<const VecT *>(L + b4[j] * COLS);
+#pragma unroll
+                    for (int c = 0; c < COLS; ++c) {
+                        const T v = w * fmax(row[c], T(0));
+                        acc[c] = j == 0 ? v : acc[c] + v;
+                    }
+                }
+#pragma unroll
+                for (int c = 0; c < COLS; ++c) {
+                    if (h == 0)
+                        Ep[c] = acc[c];
+                    else
+                        Em[c] = acc[c];
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < COLS; ++c) {
+                const T Ea = efa * own[k][c];
+                const T sa = Ea * (Ep[c] * tb.dal1 + Em[c] * tb.dal2) - Ep[c] * Em[c] * tb.dal3;
+                te[k][o][c] = (own[k][c] > T(0) && fl) ? cfa * sa * T(0.5) : T(0);
+            }
+        }
+    }
+    __syncthreads();  // every corner read of the action is done
+#pragma unroll
+    for (int k = 0; k < DPT; ++k) {
+        const int d = tid + DIA_THREADS * k;
+        if (d < nbins)
+#pragma unroll
+            for (int o = 0; o < 2; ++o) {
+                VecT v;
+#pragma unroll
+                for (int c = 0; c < COLS; ++c)
+                    v[c] = te[k][o][c];
+                *reinterpret_cast<VecT *>(L + (2 * d + o) * COLS) = v;
+            }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < DPT; ++k) {
+        const int bin = tid + DIA_THREADS * k;
+        if (bin >= nbins)
+            continue;
+        T t[COLS];
+#pragma unroll
+        for (int c = 0; c < COLS; ++c)
+            t[c] = T(0);
+        const int c1 = tb.gat_off[bin + 1];
+        // the list loads of four entries are issued before their sums
+        int e = tb.gat_off[bin];
+        for (; e + 4 <= c1; e += 4) {
+            int gq[4];
+            T gw[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                gq[u] = tb.gat_q[e + u];
+                gw[u] = tb.gat_w[e + u];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const VecT row = *reinterpret_cast<const VecT *>(L + gq[u] * COLS);
+#pragma unroll
+                for (int c = 0; c < COLS; ++c)
+                    t[c] = t[c] + row[c] * gw[u];
+            }
+        }
+        for (; e < c1; ++e) {
+            const T w = tb.gat_w[e];
+            const VecT row = *reinterpret_cast<const VecT *>(L + tb.gat_q[e] * COLS);
+#pragma unroll
+            for (int c = 0; c < COLS; ++c)
+                t[c] = t[c] + row[c] * w;
+        }
+        VecT sv;
+#pragma unroll
+        for (int c = 0; c < COLS; ++c) {
+            const T a = own[k][c];
+            T src;
+            if (t[c] >= T(0)) {
+                src = t[c];
+            } else {
+                const T rate = -t[c] / fmax(a, eps_of<T>());
+                src = -(fmin(rate, T(1)) * a);
+            }
+            sv[c] = src;
+        }
+        if (whole) {
+            *reinterpret_cast<VecT *>(out + ij0 + plane * (size_t)bin) = sv;
+        } else {
+#pragma unroll
+            for (int c = 0; c < COLS; ++c)
+                if (c < ncol)
+                    out[ij0 + c + plane * (size_t)bin] = sv[c];
+        }
+    }
+}
+
 // ListedQuadruplets (Sources/Sources.jl:1332-1385) with constant rates: one
 // thread per cell walks the listed interactions in order and sums, as the
 // reference, flux / w where the cell is a receiver and flux / (w N) (N > 0)
@@ -158,12 +341,35 @@ int dia_columns_per_workgroup(int nbins)
     return (int)std::min<size_t>(cols, 8);
 }
 
+template <typename T, int COLS, int DPT>
+hipError_t launch_dia_vec(size_t plane, int nbins, const T *N, const DiaTables<T> &tb, T *out, hipStream_t s)
+{
+    const long long tiles = (long long)((plane + COLS - 1) / COLS);
+    if (tiles > 0x7fffffffLL)
+        return hipErrorInvalidValue;
+    const size_t lds = 2 * (size_t)nbins * COLS * sizeof(T);
+    hipLaunchKernelGGL((dia_source_vec_kernel<T, COLS, DPT>), dim3((unsigned)tiles), dim3(DIA_THREADS), lds, s, plane,
+                       nbins, tiles, N, tb, out);
+    return hipGetLastError();
+}
+
 template <typename T>
 hipError_t launch_dia_source(int nx, int ny, int nk, int nphi, const T *N, const DiaTables<T> &tb, T *out,
                              hipStream_t s)
 {
     const size_t plane = (size_t)nx * ny;
     const int nbins = nk * nphi;
+    // Grids up to 2048 bins: the column-vector kernel (16 bytes per bin and
+    // column group, 2 nbins of them within 64 KiB of LDS).
+    constexpr int VC = 16 / sizeof(T);
+    if (nbins <= DIA_THREADS)
+        return launch_dia_vec<T, VC, 1>(plane, nbins, N, tb, out, s);
+    if (nbins <= 2 * DIA_THREADS)
+        return launch_dia_vec<T, VC, 2>(plane, nbins, N, tb, out, s);
+    if (nbins <= 3 * DIA_THREADS)
+        return launch_dia_vec<T, VC, 3>(plane, nbins, N, tb, out, s);
+    if (nbins <= 4 * DIA_THREADS)
+        return launch_dia_vec<T, VC, 4>(plane, nbins, N, tb, out, s);
     const int cols = dia_columns_per_workgroup<T>(nbins);
     if (cols < 1)
         return hipErrorInvalidValue;
