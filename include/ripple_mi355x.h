@@ -303,6 +303,8 @@ int rpl_mlp_destroy(rpl_mlp *net);
 int rpl_mlp_dims(rpl_mlp *net, int32_t *in_pad, int32_t *out_pad, int64_t *nparams);
 int rpl_mlp_set_params(rpl_mlp *net, const float *params);
 int rpl_mlp_get_params(rpl_mlp *net, float *params);
+/* Inference: the kernels the RK3 stage runs (hidden activations rounded to
+ * bf16 in the GEMM epilogue, the output layer computed transposed). */
 int rpl_mlp_forward(rpl_mlp *net, int32_t batch, const float *X, float *Y);
 /* Loss = mean over the batch and out_dim of (Y - T)^2; grads of the loss. */
 int rpl_mlp_gradients(rpl_mlp *net, int32_t batch, const float *X, const float *T, float *loss, float *grads);

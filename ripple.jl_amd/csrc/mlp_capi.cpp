@@ -405,14 +405,19 @@ int rpl_mlp_forward(rpl_mlp *n, int32_t batch, const float *X, float *Y)
         return rc;
     if (!X || !Y)
         return set_err(RPL_ERR_ARG, "null argument");
+    // The inference path of the RK3 stage (forward_rows): activations
+    // straight to bf16, the output transposed ([out_pad][rows]).
     const int rows = round64(batch), out_pad = n->dims.back();
-    if ((rc = load_inputs(n, batch, X)) || (rc = forward_train(n, rows)))
+    if ((rc = load_inputs(n, batch, X)))
         return rc;
+    bf16 *act_b = n->A[std::min(1, n->hidden_layers - 1)];
+    MLP_HIP(rpl::mlp::forward_rows(n, n->X, rows, n->A[0], act_b, n->Y, (size_t)rows, nullptr, n->stream));
     MLP_HIP(hipStreamSynchronize(n->stream));
     std::vector<float> h((size_t)rows * out_pad);
     MLP_HIP(hipMemcpy(h.data(), n->Y, h.size() * 4, hipMemcpyDeviceToHost));
     for (int r = 0; r < batch; ++r)
-        std::memcpy(Y + (size_t)r * n->out_dim, &h[(size_t)r * out_pad], n->out_dim * 4);
+        for (int o = 0; o < n->out_dim; ++o)
+            Y[(size_t)r * n->out_dim + o] = h[(size_t)o * rows + r];
     return RPL_OK;
 }
 

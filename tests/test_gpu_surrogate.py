@@ -103,6 +103,26 @@ def test_large_tile_gemm_forward_and_gradients_match_oracle():
     net.close()
 
 
+def test_inference_kernels_match_oracle():
+    """rpl_mlp_forward runs the kernels of the RK3 stage's inference: 256 rows
+    take the 128 x 128 GEMM in every layer (the output layer transposed, 256
+    units x 256 rows), 192 rows the 64 x 64 one where 128 does not divide.
+    Both against the oracle, and the rows they share agree to FP32 rounding."""
+    say("mlp inference kernels vs oracle")
+    net, X, T = _net_and_batch(8, in_dim=200, out_dim=250, hidden=512, layers=3, batch=256)
+    assert net.dims == [256, 512, 512, 512, 256]
+    layers, Xp, _ = _oracle_inputs(net, X, T)
+    Ye, _ = mo.forward(layers, Xp, emulate=True)
+    Y = net.forward(X)
+    Y192 = net.forward(X[:192])
+    e = rel_l2(Y, Ye[:256, :net.out_dim])
+    e192 = rel_l2(Y192, Ye[:192, :net.out_dim])
+    d = rel_l2(Y[:192], Y192)
+    say(f"forward rel-L2 vs oracle: 256 rows {e:.2e}, 192 rows {e192:.2e}; between kernels {d:.2e}")
+    assert e < 2e-3 and e192 < 2e-3 and d < 1e-3
+    net.close()
+
+
 def test_adam_step_matches_oracle():
     say("mlp adam step")
     net, X, T = _net_and_batch(2)
