@@ -214,11 +214,14 @@ def to_struct(sources):
             raise TypeError("the nonlinear member must be a DiscreteInteractionApproximation")
         q = nl.method
         if isinstance(q, ListedQuadruplets):
-            if isinstance(sources, PrecomputedSources):
-                raise NotImplementedError("listed quadruplets run as a source term of their own, not a bundle member")
-            return s  # installed by attach through rpl_set_quadruplets
-        s.nonlinear = 1
-        s.dia_coefficient, s.dia_frequency_offset, s.dia_gravity = q.coefficient, q.frequency_offset, q.gravity
+            # installed by attach through rpl_set_quadruplets; as a bundle
+            # member its positive - damping N joins the bundle's sum through
+            # the stage's additive source plane
+            if not isinstance(sources, PrecomputedSources):
+                return s
+        else:
+            s.nonlinear = 1
+            s.dia_coefficient, s.dia_frequency_offset, s.dia_gravity = q.coefficient, q.frequency_offset, q.gravity
     if isinstance(diss, MeanSpectrumWhitecapping) and not isinstance(sources, PrecomputedSources):
         raise NotImplementedError("MeanSpectrumWhitecapping runs through the PrecomputedSources bundle state")
     if wi is not None:
@@ -260,8 +263,9 @@ def attach(model, sources):
     from .grids import FrequencyDirectionGrid
     lib = _lib.load()
     s = to_struct(sources)
-    listed = sources.method if isinstance(sources, DiscreteInteractionApproximation) and \
-        isinstance(sources.method, ListedQuadruplets) else None
+    nl = sources.nonlinear if isinstance(sources, PrecomputedSources) else sources
+    listed = nl.method if isinstance(nl, DiscreteInteractionApproximation) and \
+        isinstance(nl.method, ListedQuadruplets) else None
     if (s.nonlinear or listed) and not isinstance(model.spectral_grid, FrequencyDirectionGrid):
         # symmetric_quadruplet.jl:177, Sources.jl:1275: the transfer is defined on frequency-direction grids.
         raise ValueError("DiscreteInteractionApproximation requires a FrequencyDirectionGrid")

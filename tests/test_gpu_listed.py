@@ -106,6 +106,56 @@ def test_fp64_steps_against_oracle(oracle, power):
     assert err < 1e-14
 
 
+def test_listed_as_bundle_member_against_oracle(oracle):
+    """PrecomputedSources(wind input, local saturation, nonlinear =
+    DiscreteInteractionApproximation(listed quadruplets)) (the bundle's
+    source_split, precomputed_sources.jl:240-261): the device adds the
+    listed source after the bundle's, so the oracle runs its bundle hook and
+    then its listed hook.  The reference sums every member's positive and
+    damping parts before forming positive - damping N, a rounding-level
+    difference (bound below)."""
+    from ripple_amd.sources import to_struct
+    say("listed quadruplets as a bundle member fp64")
+    freq = np.geomspace(0.08, 0.3, 8)
+    quads = _resonant_list()
+    src = ra.PrecomputedSources(
+        wind_input=ra.PressureCorrelationInput(drag=ra.BulkWindDrag("linear"), wind=12.0, direction=0.4),
+        dissipation=ra.LocalSaturationDissipation(), nonlinear=ra.DiscreteInteractionApproximation(quads))
+    m = model(src, freq, 24, nx=6, ny=5)
+    N0 = np.asfortranarray(np.random.default_rng(11).uniform(0.0, 1.0, m.size))
+    ra.set_(m, N=N0)
+    sg = m.spectral_grid
+    wl = workloads.Workload("listed_bundle", m.grid, sg, None, N0, 0.02, 3, dict(timestepper="RK3", sources=src))
+    om = oracle.OracleSourceModel(wl, oracle.sources_from(to_struct(src)), transport=oracle.MODE_NONE)
+    inter = [tuple((b[0] - 1, b[1] - 1) for b in q.bins) + (q.rate,) for q in quads]
+    _, luser, lctx = oracle.listed_hook(m.size, inter, 1.0, weights(sg))
+    L = oracle.lib()
+    V = ctypes.c_void_p
+
+    def both(N, G, user):
+        L.orc_sources_hook_f64(V(N), V(G), ctypes.pointer(om._ctx))
+        L.orc_listed_hook_f64(V(N), V(G), luser)
+
+    hook = ctypes.CFUNCTYPE(None, V, V, V)(both)
+    G = m.tendency()
+    Gw = oracle._c(om.tendency())
+    L.orc_listed_hook_f64(V(oracle._c(N0).ctypes.data), V(Gw.ctypes.data), luser)
+    Gw = Gw.reshape(m.size, order="F")
+    G_alone = model(src.nonlinear, freq, 24, nx=6, ny=5)
+    ra.set_(G_alone, N=N0)
+    err = rel_l2(G, Gw)
+    say(f"tendency rel-L2 {err:.3e}; listed part alone {rel_l2(G_alone.tendency(), Gw):.3e} away")
+    assert err < 1e-13 and rel_l2(G_alone.tendency(), Gw) > 1e-6  # the bundle terms are in it
+    ra.time_step_(m, wl.dt, 3)
+    for _ in range(3):
+        om._ctx.time = om.time
+        oracle.OracleModel.step(om, wl.dt, 1, ctypes.cast(hook, V), None)
+    err = rel_l2(m.action, om.action)
+    say(f"3 RK3 steps rel-L2 {err:.3e}")
+    assert err < 1e-12
+    del lctx
+
+
 def test_fp32_against_fp64_oracle(oracle):
     say("listed quadruplets fp32")
     freq = np.geomspace(0.08, 0.3, 8)
@@ -141,8 +191,14 @@ def test_argument_checks():
     with pytest.raises(ValueError):  # source_terms.jl:1184-1187
         ra.SpectralWaveModel(g, grids.PolarWaveVectorGrid(kappa=[1.0], phi=[0.0, 1.6, 3.1, 4.7]),
                              horizontal_advection=None, sources=ra.DiscreteInteractionApproximation(ok))
-    with pytest.raises(NotImplementedError):
-        model(ra.PrecomputedSources(nonlinear=ra.DiscreteInteractionApproximation(ok)), [0.2], 4)
+    # as the bundle's only member it is the listed source itself
+    mb = model(ra.PrecomputedSources(nonlinear=ra.DiscreteInteractionApproximation(ok)), [0.2], 4)
+    ma = model(ra.DiscreteInteractionApproximation(ok), [0.2], 4)
+    N = np.full(ma.size, 0.5, order="F")
+    N[:, :, 0, :] = [4.0, 0.25, 3.0, 0.75]
+    for mm in (mb, ma):
+        ra.set_(mm, N=N)
+    np.testing.assert_allclose(mb.tendency(), ma.tendency(), rtol=1e-15, atol=0)
     # An empty list is a no-op source.
     m = model(ra.DiscreteInteractionApproximation(), [0.2], 4)
     ra.set_(m, N=1.0)
