@@ -194,6 +194,21 @@ __device__ __forceinline__ T upwind_difference(T fm3, T fm2, T fm1, T f0, T fp1,
     return fab(vel) * (T(1) / T(6)) * d6;
 }
 
+// The same for a velocity that is uniform across the wavefront (intrinsic
+// transport: the group velocity of a bin): the upwind side is a branch taken
+// by the whole wave instead of a per-lane selection of the six stencil
+// values.  Same arithmetic, so the same bits.
+template <typename T>
+__device__ __forceinline__ T upwind_difference_uniform(T fm3, T fm2, T fm1, T f0, T fp1, T fp2, T fp3, T vel)
+{
+    T d6;
+    if (vel >= T(0))
+        d6 = weno_pair6<false>(fm3, fm2, fm1, f0, fp1, fp2, (T *)nullptr, (T *)nullptr);
+    else
+        d6 = weno_pair6<false>(fp3, fp2, fp1, f0, fm1, fm2, (T *)nullptr, (T *)nullptr);
+    return fab(vel) * (T(1) / T(6)) * d6;
+}
+
 // Load at a 32-bit byte offset from a block-uniform base, so the address is a
 // scalar base plus a 32-bit lane offset (global_load ... saddr form) and costs
 // no 64-bit vector arithmetic.  Offsets stay below 4 GiB: a slab is nx*ny
@@ -588,7 +603,14 @@ __device__ __forceinline__ void phi_tile(const T *__restrict__ Nk, const T *__re
         load_bin(n, cur, interior);
         const size_t bin = (size_t)m + (size_t)nk * n;
         const T x0 = REF ? w[3] : cur.x0;
-        T ux = gld<AS1>(p.cgx, m + nk * n), uy = gld<AS1>(p.cgy, m + nk * n);
+        T ux, uy;
+        if constexpr (REF) {
+            ux = gld<AS1>(p.cgx, m + nk * n);
+            uy = gld<AS1>(p.cgy, m + nk * n);
+        } else {
+            ux = sld(p.cgx, (size_t)(m + nk * n));  // the bin's group velocity: wave-uniform
+            uy = sld(p.cgy, (size_t)(m + nk * n));
+        }
         T ck = T(0), cp = T(0);
         if constexpr (REF) {
             const T cs = gld<AS1>(p.cosp, n), sn = gld<AS1>(p.sinp, n);
@@ -599,8 +621,14 @@ __device__ __forceinline__ void phi_tile(const T *__restrict__ Nk, const T *__re
             uy += Uyc;
         }
         const T *fx = cur.fx, *fy = cur.fy;
-        const T dx = upwind_difference(fx[0], fx[1], fx[2], x0, fx[4], fx[5], fx[6], ux);
-        const T dy = upwind_difference(fy[0], fy[1], fy[2], x0, fy[4], fy[5], fy[6], uy);
+        T dx, dy;
+        if constexpr (REF) {
+            dx = upwind_difference(fx[0], fx[1], fx[2], x0, fx[4], fx[5], fx[6], ux);
+            dy = upwind_difference(fy[0], fy[1], fy[2], x0, fy[4], fy[5], fy[6], uy);
+        } else {
+            dx = upwind_difference_uniform(fx[0], fx[1], fx[2], x0, fx[4], fx[5], fx[6], ux);
+            dy = upwind_difference_uniform(fy[0], fy[1], fy[2], x0, fy[4], fy[5], fy[6], uy);
+        }
         T L = -fm(dx, p.dx_inv, dy * p.dy_inv);
         if constexpr (REF) {
             // phi faces: the face n + 1/2 of this bin is the face n - 1/2 of
