@@ -133,6 +133,19 @@ int rpl_set_pseudomomentum_coupling(rpl_ctx *ctx, int32_t nz, const double *z_fa
  * members with smoothing. */
 int rpl_set_propagation_smoothing(rpl_ctx *ctx, int32_t enable, double alpha_s, double alpha_n);
 
+/* The x and y topology of the model's RectilinearGrid (Oceananigans'
+ * topology(grid)): 1 = Periodic, 0 = Bounded; a new context is periodic in
+ * both.  Replaces the periodic_x / periodic_y flags the reference's kernels
+ * read from the grid (propagation_smoothing.jl:175-177,
+ * intrinsic_transport.jl:109-111).  On a bounded axis SpatialAveraging
+ * clamps its corner indices to the edge cells (propagation_smoothing.jl:
+ * 56, 97-100).  The refraction kernel wraps x and y on either topology, as
+ * the reference's does (wave_current_refraction.jl:7-19, 97-111); the fused
+ * intrinsic kernel runs on Periodic x Periodic grids only (tendencies.jl:
+ * 45-50; the reference takes Oceananigans' per-bin advection otherwise), so
+ * a bounded axis with intrinsic transport returns RPL_ERR_UNSUPPORTED. */
+int rpl_set_topology(rpl_ctx *ctx, int32_t periodic_x, int32_t periodic_y);
+
 /* update_coupling!(model) (current_coupling.jl:123-148): recomputes the
  * pseudomomentum coupling's U and dU/dkappa from the current action; a no-op
  * for a prescribed current (static). */

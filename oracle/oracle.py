@@ -183,8 +183,10 @@ def gse_tables(kappa, dtype=np.float64):
     return cg, dcg
 
 
-def spatial_averaging(N, kappa, phi, dphi, dx, dy, dt, alpha_s=0.5, alpha_n=0.5, dtype=np.float64):
-    """One SpatialAveraging pass (Models/propagation_smoothing.jl:160-199) of N (Nx, Ny, Nk, Nphi)."""
+def spatial_averaging(N, kappa, phi, dphi, dx, dy, dt, alpha_s=0.5, alpha_n=0.5, dtype=np.float64,
+                      periodic=(True, True)):
+    """One SpatialAveraging pass (Models/propagation_smoothing.jl:160-199) of N (Nx, Ny, Nk, Nphi);
+    periodic = (x, y): a bounded axis clamps the corner indices to its edge cells."""
     t = np.dtype(dtype)
     nx, ny, nk, npf = N.shape
     cg, dcg = gse_tables(kappa, t)
@@ -196,7 +198,8 @@ def spatial_averaging(N, kappa, phi, dphi, dx, dy, dt, alpha_s=0.5, alpha_n=0.5,
     one = t.type(1)
     getattr(lib(), f"orc_spatial_averaging_{'f64' if t == np.float64 else 'f32'}")(
         _I(nx), _I(ny), _I(nk), _I(npf), _p(Nin), _p(out), _p(cg), _p(dcg), _p(cs), _p(sn), sc(dphi),
-        sc(alpha_s), sc(alpha_n), sc(dt), sc(one / t.type(dx)), sc(one / t.type(dy)))
+        sc(alpha_s), sc(alpha_n), sc(dt), sc(one / t.type(dx)), sc(one / t.type(dy)), _I(int(bool(periodic[0]))),
+        _I(int(bool(periodic[1]))))
     return out.reshape((nx, ny, nk, npf), order="F")
 
 
@@ -257,9 +260,12 @@ class OracleModel:
             self.pm = [_c(O, t), _c(tabs["mu_x"], t), _c(tabs["mu_y"], t)]
             self.mode = MODE_PSEUDOMOMENTUM
             self.Ux, self.Uy = np.zeros(nx * ny * nk, dtype=t), np.zeros(nx * ny * nk, dtype=t)
-        # SpatialAveraging after every step: smoothing=(alpha_s, alpha_n).
+        # SpatialAveraging after every step: smoothing=(alpha_s, alpha_n),
+        # clamped on the grid's bounded axes (propagation_smoothing.jl:175-177).
         self.smoothing = smoothing
         self._sg = sg
+        topo = getattr(g, "topology", ("Periodic", "Periodic"))
+        self._periodic = (topo[0] == "Periodic", topo[1] == "Periodic")
         self.N = _c(wl.N0, t).copy()  # never alias the workload's array
         self.time, self.iteration = 0.0, 0
 
@@ -278,7 +284,8 @@ class OracleModel:
                 self._step(dt, 1, hook, user)
                 sg = self._sg
                 self.N = _c(spatial_averaging(self.action, sg.kappa, sg.phi, self.dp, self.dx, self.dy, dt,
-                                              *self.smoothing, dtype=self.dtype), self.dtype)
+                                              *self.smoothing, dtype=self.dtype, periodic=self._periodic),
+                                    self.dtype)
             return
         self._step(dt, nsteps, hook, user)
 

@@ -418,16 +418,24 @@ static inline int SFX(gse_wrap)(int i, int n)
     return i;
 }
 
-/* Models/propagation_smoothing.jl:56-110 (_spatial_averaging_kernel!) on a
- * periodic x-y grid: for every cell the mean of four bilinear
- * interpolations at the corners (s1 Ls e_s + s2 Ln e_n), s1, s2 = -1, +1
- * (s1 outer), of a rectangle with half-widths Ls = as dcg dt along the bin's
- * direction and Ln = an cg dphi dt across it, in cell units.  Indices are
- * formed 1-based as in the reference so the fractional weights round the
- * same way. */
+/* The reference's _gse_clamp (propagation_smoothing.jl:56): a 1-based index
+ * held to 1..n on a Bounded axis. */
+static inline int SFX(gse_clamp)(int i, int n)
+{
+    return i < 1 ? 1 : (i > n ? n : i);
+}
+
+/* Models/propagation_smoothing.jl:56-110 (_spatial_averaging_kernel!): for
+ * every cell the mean of four bilinear interpolations at the corners
+ * (s1 Ls e_s + s2 Ln e_n), s1, s2 = -1, +1 (s1 outer), of a rectangle with
+ * half-widths Ls = as dcg dt along the bin's direction and Ln = an cg dphi dt
+ * across it, in cell units.  Indices are formed 1-based as in the reference
+ * so the fractional weights round the same way; each corner index wraps on
+ * a periodic axis (px, py nonzero) and is clamped to the edge cell on a
+ * bounded one (propagation_smoothing.jl:97-100). */
 void SFX(orc_spatial_averaging)(int nx, int ny, int nk, int nphi, const REAL *Nin, REAL *Nout,
                                 const REAL *cg, const REAL *dcg, const REAL *cos_t, const REAL *sin_t,
-                                REAL dphi, REAL as, REAL an, REAL dt, REAL dx_inv, REAL dy_inv)
+                                REAL dphi, REAL as, REAL an, REAL dt, REAL dx_inv, REAL dy_inv, int px, int py)
 {
 #pragma omp parallel for schedule(static)
     for (long mn = 0; mn < (long)nk * nphi; ++mn) {
@@ -447,8 +455,10 @@ void SFX(orc_spatial_averaging)(int nx, int ny, int nk, int nphi, const REAL *Ni
                         REAL ir = (REAL)(i + 1) + di, jr = (REAL)(j + 1) + dj;
                         int i0 = (int)floor(ir), j0 = (int)floor(jr);
                         REAL wi = ir - (REAL)i0, wj = jr - (REAL)j0;
-                        int i0c = SFX(gse_wrap)(i0, nx) - 1, i1c = SFX(gse_wrap)(i0 + 1, nx) - 1;
-                        int j0c = SFX(gse_wrap)(j0, ny) - 1, j1c = SFX(gse_wrap)(j0 + 1, ny) - 1;
+                        int i0c = (px ? SFX(gse_wrap)(i0, nx) : SFX(gse_clamp)(i0, nx)) - 1;
+                        int i1c = (px ? SFX(gse_wrap)(i0 + 1, nx) : SFX(gse_clamp)(i0 + 1, nx)) - 1;
+                        int j0c = (py ? SFX(gse_wrap)(j0, ny) : SFX(gse_clamp)(j0, ny)) - 1;
+                        int j1c = (py ? SFX(gse_wrap)(j0 + 1, ny) : SFX(gse_clamp)(j0 + 1, ny)) - 1;
                         REAL f00 = Nin[IDX4(i0c, j0c, m, n)], f10 = Nin[IDX4(i1c, j0c, m, n)];
                         REAL f01 = Nin[IDX4(i0c, j1c, m, n)], f11 = Nin[IDX4(i1c, j1c, m, n)];
                         REAL one = 1;

@@ -79,6 +79,8 @@ struct rpl_ctx {
     // SpatialAveraging after every step (rpl_set_propagation_smoothing)
     bool gse = false;
     double gse_as = 0.5, gse_an = 0.5, gse_cg_max = 0, gse_dcg_max = 0;
+    // the grid's x and y topology (rpl_set_topology): periodic or bounded
+    bool periodic_x = true, periodic_y = true;
     void *gse_cg = nullptr, *gse_dcg = nullptr;
     double time = 0;
     int64_t iteration = 0;
@@ -661,13 +663,15 @@ int launch_smoothing(rpl_ctx *c, double dt)
         e = rpl::launch_spatial_averaging<double>(false, nx, ny, nk, np, (const double *)c->A, (double *)c->B,
                                                   (const double *)c->gse_cg, (const double *)c->gse_dcg,
                                                   (const double *)c->cosp, (const double *)c->sinp, c->dp,
-                                                  c->gse_as, c->gse_an, dt, 1 / c->dx, 1 / c->dy, c->stream);
+                                                  c->gse_as, c->gse_an, dt, 1 / c->dx, 1 / c->dy, c->periodic_x,
+                                                  c->periodic_y, c->stream);
     else
         e = rpl::launch_spatial_averaging<float>(use_fast(c), nx, ny, nk, np, (const float *)c->A, (float *)c->B,
                                                  (const float *)c->gse_cg, (const float *)c->gse_dcg,
                                                  (const float *)c->cosp, (const float *)c->sinp, (float)c->dp,
                                                  (float)c->gse_as, (float)c->gse_an, (float)dt,
-                                                 1.0f / (float)c->dx, 1.0f / (float)c->dy, c->stream);
+                                                 1.0f / (float)c->dx, 1.0f / (float)c->dy, c->periodic_x,
+                                                 c->periodic_y, c->stream);
     if (e != hipSuccess)
         return fail(RPL_ERR_HIP, std::string("spatial averaging: ") + hipGetErrorString(e));
     std::swap(c->A, c->B);
@@ -1565,6 +1569,21 @@ int rpl_update_coupling(rpl_ctx *c)
     if ((rc = launch_coupling(c, c->A)))
         return rc;
     RPL_HIP(hipStreamSynchronize(c->stream));
+    return RPL_OK;
+}
+
+int rpl_set_topology(rpl_ctx *c, int32_t periodic_x, int32_t periodic_y)
+{
+    if (!c)
+        return fail(RPL_ERR_ARG, "null context");
+    if ((periodic_x != 0 && periodic_x != 1) || (periodic_y != 0 && periodic_y != 1))
+        return fail(RPL_ERR_ARG, "periodic_x and periodic_y must be 0 (Bounded) or 1 (Periodic)");
+    if (c->cfg.transport == RPL_TRANSPORT_INTRINSIC && !(periodic_x && periodic_y))
+        return fail(RPL_ERR_UNSUPPORTED, "the fused intrinsic transport kernel runs on Periodic x Periodic grids "
+                                         "only (the reference uses Oceananigans' per-bin advection otherwise)");
+    ++c->generation;
+    c->periodic_x = periodic_x != 0;
+    c->periodic_y = periodic_y != 0;
     return RPL_OK;
 }
 

@@ -165,13 +165,14 @@ template <typename T>
 hipError_t launch_action_tendency(size_t n, const T *Nnew, const T *Nold, T dt, T *out, hipStream_t s);
 
 // SpatialAveraging (Models/propagation_smoothing.jl:56-110): Nout = mean of
-// four bilinear corner interpolations of Nin per (kappa, phi) bin, periodic
-// x and y; cg, dcg per kappa bin, cos/sin of the phi centres.  fast (FP32
-// only) forms the corner weights once per bin instead of per cell.
+// four bilinear corner interpolations of Nin per (kappa, phi) bin, each axis
+// periodic (px, py) or clamped at its edges; cg, dcg per kappa bin, cos/sin
+// of the phi centres.  fast (FP32 only) forms the corner weights once per bin
+// instead of per cell.
 template <typename T>
 hipError_t launch_spatial_averaging(bool fast, int nx, int ny, int nk, int nphi, const T *Nin, T *Nout, const T *cg,
                                     const T *dcg, const T *cosp, const T *sinp, T dphi, T as, T an, T dt, T dx_inv,
-                                    T dy_inv, hipStream_t s);
+                                    T dy_inv, bool px, bool py, hipStream_t s);
 
 // DiscreteInteractionApproximation{SymmetricQuadruplet}: per donor bin d and
 // orientation o (entry q = 2 d + o, o = 0 for +1, 1 for -1) the receivers'

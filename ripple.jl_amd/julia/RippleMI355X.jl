@@ -109,11 +109,13 @@ end
 
 `grid` needs `x_faces`, `y_faces`, `z_faces` vectors (Oceananigans: `xfaces(grid)` ...);
 `spectral_grid` needs `κ`, `κ_faces`, `φ`, `φ_faces`.  Dispatch follows
-Models/tendencies.jl:18-109.  `depth === nothing` is InfiniteDepth().
+Models/tendencies.jl:18-109.  `depth === nothing` is InfiniteDepth().  `topology`
+is the grid's (x, y) topology, `:Periodic` or `:Bounded` (Oceananigans: `topology(grid)`).
 """
 function MI355XWaveModel(grid, spectral_grid; velocities=nothing, horizontal_advection=:WENO,
                          spectral_advection=:WENO, depth=nothing, timestepper=:ForwardEuler,
-                         float_type=Float64, gravity=9.81, kernel=0, propagation_smoothing=nothing)
+                         float_type=Float64, gravity=9.81, kernel=0, propagation_smoothing=nothing,
+                         topology=(:Periodic, :Periodic))
     xf, yf, zf = collect(Float64, grid.x_faces), collect(Float64, grid.y_faces), collect(Float64, grid.z_faces)
     κ, κf = collect(Float64, spectral_grid.κ), collect(Float64, spectral_grid.κ_faces)
     φ, φf = collect(Float64, spectral_grid.φ), collect(Float64, spectral_grid.φ_faces)
@@ -155,6 +157,8 @@ function MI355XWaveModel(grid, spectral_grid; velocities=nothing, horizontal_adv
                     (Ptr{Cvoid}, Int32, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble}, Cdouble),
                     model.ctx, Nz, zf, u, v, q_depth))
     end
+    check(ccall(sym(:rpl_set_topology), Cint, (Ptr{Cvoid}, Int32, Int32),
+                model.ctx, Int32(topology[1] === :Periodic), Int32(topology[2] === :Periodic)))
     if propagation_smoothing isa SpatialAveraging
         check(ccall(sym(:rpl_set_propagation_smoothing), Cint, (Ptr{Cvoid}, Int32, Cdouble, Cdouble),
                     model.ctx, Int32(1), propagation_smoothing.αs, propagation_smoothing.αn))

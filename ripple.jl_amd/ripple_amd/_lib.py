@@ -88,6 +88,7 @@ SIGNATURES = {
     "rpl_set_pseudomomentum_coupling": ([_P, _I32, _DP, _D], ctypes.c_int),
     "rpl_update_coupling": ([_P], ctypes.c_int),
     "rpl_set_propagation_smoothing": ([_P, _I32, _D, _D], ctypes.c_int),
+    "rpl_set_topology": ([_P, _I32, _I32], ctypes.c_int),
     "rpl_get_doppler_kappa_derivative": ([_P, _DP, _DP], ctypes.c_int),
     "rpl_pseudomomentum": ([_P, _I32, _DP, _D, _I32, _DP, _DP], ctypes.c_int),
     "rpl_pseudomomentum_tendency": ([_P, _P, _D, _I32, _DP, _D, _D, _DP, _DP], ctypes.c_int),

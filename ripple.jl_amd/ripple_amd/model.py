@@ -133,8 +133,8 @@ class SpectralWaveModel:
             raise TypeError("spectral_grid must be a PolarWaveVectorGrid or FrequencyDirectionGrid")
         if timestepper not in _TIMESTEPPERS:
             raise ValueError(f"unsupported timestepper {timestepper}")
-        if grid.topology[0] != PERIODIC or grid.topology[1] != PERIODIC:
-            raise NotImplementedError("the fused kernels cover Periodic x Periodic grids only")
+        if any(t not in (PERIODIC, BOUNDED) for t in grid.topology[:2]):
+            raise NotImplementedError("x and y must be Periodic or Bounded")
         self.grid, self.spectral_grid = grid, spectral_grid
         self.depth = InfiniteDepth() if depth is None else depth
         if not isinstance(self.depth, InfiniteDepth) and not float(self.depth) > 0:
@@ -160,6 +160,10 @@ class SpectralWaveModel:
         elif horizontal_advection is None:
             transport = _lib.RPL_TRANSPORT_NONE
         elif horizontal_advection == WENO and not has_coupling and Nk * Np > 1:
+            # tendencies.jl:45-50: the fused kernel on Periodic x Periodic grids
+            # only; the reference takes Oceananigans' per-bin advection otherwise
+            if grid.topology[0] != PERIODIC or grid.topology[1] != PERIODIC:
+                raise NotImplementedError("intrinsic WENO transport is built for Periodic x Periodic grids only")
             transport = _lib.RPL_TRANSPORT_INTRINSIC
         else:
             raise NotImplementedError("only the fused transport kernels are built (WENO, periodic, >1 bin)")
@@ -186,6 +190,10 @@ class SpectralWaveModel:
         elif has_coupling:
             self._set_velocities(velocities)
         _lib.check(lib.rpl_set_clock(ctx, self.clock.time, self.clock.iteration))
+        # the refraction kernel wraps x and y on either topology
+        # (wave_current_refraction.jl:97-111); SpatialAveraging clamps on a
+        # bounded axis (propagation_smoothing.jl:97-100)
+        _lib.check(lib.rpl_set_topology(ctx, int(grid.topology[0] == PERIODIC), int(grid.topology[1] == PERIODIC)))
         self.propagation_smoothing = propagation_smoothing
         if propagation_smoothing is not None:
             if not isinstance(propagation_smoothing, SpatialAveraging):

@@ -21,9 +21,10 @@ def _setup(nx=24, ny=18, nk=5, nphi=8, seed=0):
     return N, kappa, phi, dphi
 
 
-def _scipy_average(N, kappa, phi, dphi, dx, dy, dt, a_s, a_n):
+def _scipy_average(N, kappa, phi, dphi, dx, dy, dt, a_s, a_n, modes=("grid-wrap", "grid-wrap")):
     """The four-corner mean computed with scipy's bilinear interpolation
-    (order=1, periodic 'grid-wrap'), displacements in cell units."""
+    (order=1; per axis periodic 'grid-wrap' or edge-clamped 'nearest'),
+    displacements in cell units."""
     nx, ny, nk, nphi = N.shape
     g = 9.81
     cg = 0.5 * np.sqrt(g / kappa)
@@ -41,9 +42,30 @@ def _scipy_average(N, kappa, phi, dphi, dx, dy, dt, a_s, a_n):
             for s1 in (-1, 1):
                 for s2 in (-1, 1):
                     d = s1 * Ls * es + s2 * Ln * en
-                    acc += ndimage.map_coordinates(N[:, :, m, n], [ii + d[0], jj + d[1]], order=1, mode="grid-wrap")
+                    acc += _interp(N[:, :, m, n], ii + d[0], jj + d[1], modes)
             out[:, :, m, n] = acc / 4
     return out
+
+
+def _interp(f, x, y, modes):
+    """Bilinear interpolation of f at (x, y) with scipy; a 'nearest' axis is
+    handled by clamping the coordinate to [0, n - 1] first, which is what
+    clamping both corner indices to the edge cell gives."""
+    if modes == ("grid-wrap", "grid-wrap"):
+        return ndimage.map_coordinates(f, [x, y], order=1, mode="grid-wrap")
+    nx, ny = f.shape
+    if modes[0] == "nearest":
+        x = np.clip(x, 0, nx - 1)
+    if modes[1] == "nearest":
+        y = np.clip(y, 0, ny - 1)
+    # a wrapped axis: pad one period on each side, so grid-wrap is not needed
+    fx = np.concatenate([f, f, f], axis=0) if modes[0] == "grid-wrap" else f
+    fxy = np.concatenate([fx, fx, fx], axis=1) if modes[1] == "grid-wrap" else fx
+    if modes[0] == "grid-wrap":
+        x = np.mod(x, nx) + nx
+    if modes[1] == "grid-wrap":
+        y = np.mod(y, ny) + ny
+    return ndimage.map_coordinates(fxy, [x, y], order=1, mode="nearest")
 
 
 @pytest.mark.parametrize("dt,a_s,a_n", [(40.0, 0.5, 0.5), (90.0, 1.5, 0.75), (7.0, 0.0, 1.0)])
@@ -55,6 +77,32 @@ def test_matches_independent_bilinear_interpolation(oracle, dt, a_s, a_n):
     np.testing.assert_allclose(got, want, rtol=0, atol=1e-13)
     # The displacement reaches several cells for the slow bins here.
     assert np.abs(got - N).max() > 1e-3
+
+
+@pytest.mark.parametrize("periodic", [(False, False), (True, False), (False, True)])
+def test_bounded_axes_clamp_to_edge_cells(oracle, periodic):
+    """propagation_smoothing.jl:56, 97-100: on a Bounded axis every corner
+    index is held to the first or last cell; scipy's interpolation on the
+    clamped coordinate is the independent check."""
+    N, kappa, phi, dphi = _setup(seed=5)
+    dx, dy, dt = 2.0, 3.0, 90.0
+    got = oracle.spatial_averaging(N, kappa, phi, dphi, dx, dy, dt, 1.5, 0.75, periodic=periodic)
+    modes = tuple("grid-wrap" if p else "nearest" for p in periodic)
+    want = _scipy_average(N, kappa, phi, dphi, dx, dy, dt, 1.5, 0.75, modes)
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-13)
+    wrapped = oracle.spatial_averaging(N, kappa, phi, dphi, dx, dy, dt, 1.5, 0.75)
+    assert np.abs(got - wrapped).max() > 1e-3
+    # with displacements under two cells the clamp changes only the cells
+    # within three of a bounded edge
+    far = oracle.spatial_averaging(N, kappa, phi, dphi, 200.0, 200.0, dt, 1.5, 0.75, periodic=periodic)
+    far_w = oracle.spatial_averaging(N, kappa, phi, dphi, 200.0, 200.0, dt, 1.5, 0.75)
+    assert np.array_equal(far[3:-3, 3:-3], far_w[3:-3, 3:-3])
+    assert not np.array_equal(far, far_w)
+    # a uniform field stays uniform; no displacement is the identity
+    U = np.full_like(N, 0.37)
+    np.testing.assert_allclose(oracle.spatial_averaging(U, kappa, phi, dphi, dx, dy, dt, periodic=periodic), U,
+                               rtol=1e-15)
+    assert np.array_equal(oracle.spatial_averaging(N, kappa, phi, dphi, dx, dy, 0.0, periodic=periodic), N)
 
 
 def test_identities(oracle):
