@@ -294,6 +294,19 @@ int rpl_set_sources(rpl_ctx *ctx, const rpl_sources *src, const double *x_nodes,
 int rpl_set_quadruplets(rpl_ctx *ctx, int32_t n, const int32_t *bins, const double *rates,
                         const double *frequency, double power, double resonance_tolerance);
 
+/* Rates of the listed interactions given per column: the reference's
+ * source_value (Sources/Sources.jl:1450-1467) of a QuadrupletTransferInteraction
+ * rate that is an (Nx, Ny) array (value[i, j]) or a function of position and
+ * time (value(x, y, t), value(x, y), value(t), evaluated at the cell centre
+ * and model.clock.time, which stays at the step's start for all of its
+ * stages).  rates holds nx ny values per interaction, column i + nx j
+ * fastest, interaction q at offset q nx ny, all >= 0 (RPL_ERR_ARG
+ * otherwise, as the reference's ArgumentError); the host evaluates a
+ * time-dependent rate and calls this before each step.  NULL returns to the
+ * constant rates of rpl_set_quadruplets; a new list drops the field.  Needs
+ * rpl_set_quadruplets first. */
+int rpl_set_quadruplet_rate_field(rpl_ctx *ctx, const double *rates);
+
 int rpl_set_storm_track(rpl_ctx *ctx, int32_t n, const double *times, const double *x, const double *y);
 
 /* Per-column wind state of the last step: U10, wind angle, friction velocity,

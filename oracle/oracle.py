@@ -418,19 +418,22 @@ def dia_transfer(N, kappa, phi, coefficient=2.78e7, frequency_offset=0.25, gravi
 class OrcListedCtx(ctypes.Structure):
     _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nk", ctypes.c_int), ("nphi", ctypes.c_int),
                 ("count", ctypes.c_int), ("bins", ctypes.c_void_p), ("rates", ctypes.c_void_p),
-                ("power", ctypes.c_double), ("w", ctypes.c_void_p)]
+                ("power", ctypes.c_double), ("w", ctypes.c_void_p), ("rate_field", ctypes.c_void_p)]
 
 
-def listed_hook(size, interactions, power, w):
-    """(hook, user) for OracleModel.step/tendency: ListedQuadruplets with
-    constant rates; interactions = [((d1m, d1n), (d2m, d2n), (r1m, r1n), (r2m, r2n), rate)], 0-based."""
+def listed_hook(size, interactions, power, w, rate_field=None):
+    """(hook, user) for OracleModel.step/tendency: ListedQuadruplets;
+    interactions = [((d1m, d1n), (d2m, d2n), (r1m, r1n), (r2m, r2n), rate)], 0-based, with constant rates,
+    or per-column rates rate_field of shape (Nx, Ny, count)."""
     nx, ny, nk, npf = size
     bins = np.ascontiguousarray([v for it in interactions for pair in it[:4] for v in pair], dtype=np.int32)
     rates = _c([it[4] for it in interactions])
     wf = _c(w)
+    rf = None if rate_field is None else _c(np.asarray(rate_field, dtype=np.float64).reshape(
+        (nx, ny, len(interactions)), order="F"))
     ctx = OrcListedCtx(nx, ny, nk, npf, len(interactions), bins.ctypes.data, rates.ctypes.data, float(power),
-                       wf.ctypes.data)
-    ctx._keep = (bins, rates, wf)
+                       wf.ctypes.data, None if rf is None else rf.ctypes.data)
+    ctx._keep = (bins, rates, wf, rf)
     hook = ctypes.cast(lib().orc_listed_hook_f64, ctypes.c_void_p)
     return hook, ctypes.cast(ctypes.pointer(ctx), ctypes.c_void_p), ctx
 

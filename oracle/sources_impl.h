@@ -473,11 +473,14 @@ void orc_wind_columns(const orc_sources *s, int nx, int ny, const double *x_node
 }
 
 /* DiscreteInteractionApproximation{ListedQuadruplets} (Sources/Sources.jl:
- * 374-428, 1332-1385) with constant rates: for every cell and every listed
- * interaction (donors d1, d2, receivers r1, r2; 0-based (m, n) bins),
+ * 374-428, 1332-1385): for every cell and every listed interaction (donors
+ * d1, d2, receivers r1, r2; 0-based (m, n) bins),
  *   flux = rate max(N_d1, 0)^p max(N_d2, 0)^p,
  *   receivers: positive += flux / w,  donors: damping += flux / (w N) (N > 0),
- * in list order, then G += positive - damping N. */
+ * in list order, then G += positive - damping N.  The rate is the
+ * interaction's constant, or with rate_field set its column's value
+ * (source_value of an array or a function of position, Sources.jl:
+ * 1450-1467), rate_field[col + nx ny q]. */
 typedef struct orc_listed_ctx {
     int nx, ny, nk, nphi;
     int count;
@@ -485,6 +488,7 @@ typedef struct orc_listed_ctx {
     const double *rates;  /* per interaction */
     double power;
     const double *w;      /* (nk, nphi) cell measures */
+    const double *rate_field;  /* NULL, or (nx ny, count), column fastest */
 } orc_listed_ctx;
 
 void orc_listed_hook_f64(const double *N, double *G, void *user)
@@ -506,7 +510,8 @@ void orc_listed_hook_f64(const double *N, double *G, void *user)
                     double a2 = N[ij + plane * ((size_t)b[2] + (size_t)nk * b[3])];
                     a1 = a1 > 0 ? a1 : 0;
                     a2 = a2 > 0 ? a2 : 0;
-                    const double flux = c->rates[q] * pow(a1, c->power) * pow(a2, c->power);
+                    const double rate = c->rate_field ? c->rate_field[ij + plane * (size_t)q] : c->rates[q];
+                    const double flux = rate * pow(a1, c->power) * pow(a2, c->power);
                     if (flux == 0)
                         continue;
                     if (m == b[4] && n == b[5])

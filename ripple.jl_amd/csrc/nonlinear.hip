@@ -288,18 +288,22 @@ __global__ __launch_bounds__(DIA_THREADS) void dia_source_vec_kernel(size_t plan
     }
 }
 
-// ListedQuadruplets (Sources/Sources.jl:1332-1385) with constant rates: one
-// thread per cell walks the listed interactions in order and sums, as the
-// reference, flux / w where the cell is a receiver and flux / (w N) (N > 0)
-// where it is a donor, flux = rate max(N_d1, 0)^p max(N_d2, 0)^p; the source
-// is positive - damping N.  The list is uniform across the grid (scalar
-// loads); the donor values come from the thread's own column.
+// ListedQuadruplets (Sources/Sources.jl:1332-1385): one thread per cell
+// walks the listed interactions in order and sums, as the reference,
+// flux / w where the cell is a receiver and flux / (w N) (N > 0) where it is
+// a donor, flux = rate max(N_d1, 0)^p max(N_d2, 0)^p; the source is
+// positive - damping N.  The list is uniform across the grid (scalar loads);
+// the donor values come from the thread's own column.  Each interaction's
+// rate is a constant (rates[q]) or, with rate_field set, its column's value
+// rate_field[col + plane q] (a rate given per column or as a function of
+// position and time, evaluated by the host at the step's clock time).
 template <typename T>
 __global__ __launch_bounds__(256) void listed_source_kernel(size_t plane, int nk, int nphi, int count,
                                                             const int *__restrict__ bins,
-                                                            const T *__restrict__ rates, T power, bool unit_power,
-                                                            const T *__restrict__ w, const T *__restrict__ N,
-                                                            T *__restrict__ out)
+                                                            const T *__restrict__ rates,
+                                                            const T *__restrict__ rate_field, T power,
+                                                            bool unit_power, const T *__restrict__ w,
+                                                            const T *__restrict__ N, T *__restrict__ out)
 {
     const size_t ij = (size_t)blockIdx.x * 256 + threadIdx.x;
     const int mn = blockIdx.y;
@@ -316,7 +320,9 @@ __global__ __launch_bounds__(256) void listed_source_kernel(size_t plane, int nk
         T a2 = N[ij + plane * ((size_t)b[2] + (size_t)nk * b[3])];
         a1 = a1 > T(0) ? a1 : T(0);
         a2 = a2 > T(0) ? a2 : T(0);
-        const T flux = unit_power ? rates[q] * a1 * a2 : rates[q] * pow(a1, power) * pow(a2, power);
+        // source_value (Sources.jl:1450-1467): a constant, or the column's value
+        const T rate = rate_field ? rate_field[ij + plane * (size_t)q] : rates[q];
+        const T flux = unit_power ? rate * a1 * a2 : rate * pow(a1, power) * pow(a2, power);
         if (flux == T(0))
             continue;
         if (m == b[4] && n == b[5])
@@ -383,20 +389,20 @@ hipError_t launch_dia_source(int nx, int ny, int nk, int nphi, const T *N, const
 }
 
 template <typename T>
-hipError_t launch_listed_source(int nx, int ny, int nk, int nphi, int count, const int *bins, const T *rates, T power,
-                                const T *w, const T *N, T *out, hipStream_t s)
+hipError_t launch_listed_source(int nx, int ny, int nk, int nphi, int count, const int *bins, const T *rates,
+                                const T *rate_field, T power, const T *w, const T *N, T *out, hipStream_t s)
 {
     const size_t plane = (size_t)nx * ny;
     dim3 grid((unsigned)((plane + 255) / 256), (unsigned)(nk * nphi));
-    hipLaunchKernelGGL(listed_source_kernel<T>, grid, dim3(256), 0, s, plane, nk, nphi, count, bins, rates, power,
-                       power == T(1), w, N, out);
+    hipLaunchKernelGGL(listed_source_kernel<T>, grid, dim3(256), 0, s, plane, nk, nphi, count, bins, rates,
+                       rate_field, power, power == T(1), w, N, out);
     return hipGetLastError();
 }
 
-template hipError_t launch_listed_source<float>(int, int, int, int, int, const int *, const float *, float,
-                                                const float *, const float *, float *, hipStream_t);
-template hipError_t launch_listed_source<double>(int, int, int, int, int, const int *, const double *, double,
-                                                 const double *, const double *, double *, hipStream_t);
+template hipError_t launch_listed_source<float>(int, int, int, int, int, const int *, const float *, const float *,
+                                                float, const float *, const float *, float *, hipStream_t);
+template hipError_t launch_listed_source<double>(int, int, int, int, int, const int *, const double *, const double *,
+                                                 double, const double *, const double *, double *, hipStream_t);
 template int dia_columns_per_workgroup<float>(int);
 template int dia_columns_per_workgroup<double>(int);
 template hipError_t launch_dia_source<float>(int, int, int, int, const float *, const DiaTables<float> &, float *,

@@ -76,6 +76,9 @@ struct rpl_ctx {
     int32_t listed_n = 0;
     double listed_power = 1;
     void *listed_bins = nullptr, *listed_rates = nullptr, *listed_w = nullptr;
+    // per-column rates (rpl_set_quadruplet_rate_field): (nx ny, listed_n)
+    void *listed_field = nullptr;
+    bool listed_has_field = false;
     // SpatialAveraging after every step (rpl_set_propagation_smoothing)
     bool gse = false;
     double gse_as = 0.5, gse_an = 0.5, gse_cg_max = 0, gse_dcg_max = 0;
@@ -460,12 +463,16 @@ int launch_dia(rpl_ctx *c, const void *Nk)
         const int nx = c->cfg.nx, ny = c->cfg.ny, nk = c->cfg.nkappa, np = c->cfg.nphi;
         if (is_f64(c))
             e = rpl::launch_listed_source<double>(nx, ny, nk, np, c->listed_n, (const int *)c->listed_bins,
-                                                  (const double *)c->listed_rates, c->listed_power,
+                                                  (const double *)c->listed_rates,
+                                                  c->listed_has_field ? (const double *)c->listed_field : nullptr,
+                                                  c->listed_power,
                                                   (const double *)c->listed_w, (const double *)Nk,
                                                   (double *)c->dia_add, c->stream);
         else
             e = rpl::launch_listed_source<float>(nx, ny, nk, np, c->listed_n, (const int *)c->listed_bins,
-                                                 (const float *)c->listed_rates, (float)c->listed_power,
+                                                 (const float *)c->listed_rates,
+                                                 c->listed_has_field ? (const float *)c->listed_field : nullptr,
+                                                 (float)c->listed_power,
                                                  (const float *)c->listed_w, (const float *)Nk, (float *)c->dia_add,
                                                  c->stream);
         if (e != hipSuccess)
@@ -1234,7 +1241,7 @@ int rpl_destroy(rpl_ctx *c)
                     (void *)c->bin_hi, (void *)c->s_amax, (void *)c->s_flags, c->dUx, c->dUy, c->pm_O,
                     c->pm_dO, c->pm_kx, c->pm_ky, c->gse_cg, c->gse_dcg, c->dia_flags, c->dia_bins, c->dia_wef,
                     c->dia_wdiv, c->dia_efa, c->dia_inv_efa, c->dia_cfa, c->dia_add, c->dia_gat_off, c->dia_gat_q, c->dia_gat_w, c->listed_bins,
-                    c->listed_rates, c->listed_w})
+                    c->listed_rates, c->listed_w, c->listed_field})
         free_ptr(p);
     for (auto &ev : c->events) {
         (void)hipEventDestroy(ev.first);
@@ -2473,6 +2480,9 @@ int rpl_set_quadruplets(rpl_ctx *c, int32_t n, const int32_t *bins, const double
         return fail(RPL_ERR_UNSUPPORTED, "listed quadruplets support at most 65535 spectral bins");
     RPL_HIP(hipSetDevice(c->device));
     c->listed = false;  // until the new list is on the device
+    c->listed_has_field = false;  // a new list takes its constant rates
+    free_ptr(c->listed_field);    // and may have another length
+    c->listed_field = nullptr;
     free_ptr(c->listed_bins);
     free_ptr(c->listed_rates);
     c->listed_bins = c->listed_rates = nullptr;
@@ -2489,6 +2499,33 @@ int rpl_set_quadruplets(rpl_ctx *c, int32_t n, const int32_t *bins, const double
     c->listed_n = n;
     c->listed_power = power;
     c->listed = true;
+    return RPL_OK;
+}
+
+int rpl_set_quadruplet_rate_field(rpl_ctx *c, const double *rates)
+{
+    if (!c)
+        return fail(RPL_ERR_ARG, "null context");
+    ++c->generation;
+    if (!rates) {
+        c->listed_has_field = false;
+        return RPL_OK;
+    }
+    if (!c->listed)
+        return fail(RPL_ERR_STATE, "rpl_set_quadruplets has not set a list of interactions");
+    const size_t n = (size_t)c->cfg.nx * c->cfg.ny * c->listed_n;
+    for (size_t k = 0; k < n; ++k)
+        if (!(rates[k] >= 0))
+            return fail(RPL_ERR_ARG, "DIA transfer rates must be nonnegative");  // Sources.jl:1347
+    RPL_HIP(hipSetDevice(c->device));
+    int rc;
+    if (!c->listed_field && (rc = alloc(&c->listed_field, n * c->esize)))
+        return rc;
+    c->listed_has_field = false;  // until the field is on the device
+    if ((rc = upload_real(c, rates, c->listed_field, n)))
+        return rc;
+    RPL_HIP(hipStreamSynchronize(c->stream));
+    c->listed_has_field = true;
     return RPL_OK;
 }
 

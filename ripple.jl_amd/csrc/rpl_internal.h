@@ -196,11 +196,13 @@ struct DiaTables {
 };
 template <typename T>
 int dia_columns_per_workgroup(int nbins);
-// ListedQuadruplets source (constant rates): bins 8 per interaction
-// (d1m d1n d2m d2n r1m r1n r2m r2n, 0-based), w the (nk, nphi) cell measures.
+// ListedQuadruplets source: bins 8 per interaction (d1m d1n d2m d2n r1m r1n
+// r2m r2n, 0-based), constant rates[q], or per-column rates rate_field
+// (nx ny, count), column fastest, when rate_field is set; w the (nk, nphi)
+// cell measures.
 template <typename T>
-hipError_t launch_listed_source(int nx, int ny, int nk, int nphi, int count, const int *bins, const T *rates, T power,
-                                const T *w, const T *N, T *out, hipStream_t s);
+hipError_t launch_listed_source(int nx, int ny, int nk, int nphi, int count, const int *bins, const T *rates,
+                                const T *rate_field, T power, const T *w, const T *N, T *out, hipStream_t s);
 // out = the DIA source (T or -min(-T/max(N, eps), 1) N) of the action N.
 template <typename T>
 hipError_t launch_dia_source(int nx, int ny, int nk, int nphi, const T *N, const DiaTables<T> &tb, T *out,

@@ -26,6 +26,7 @@ using Libdl
 export MI355X, MI355XWaveModel, set!, time_step!, interior_action, doppler_velocity,
        PseudomomentumVelocities, SpatialAveraging, update_coupling!, doppler_velocity_kappa_derivative,
        compute_pseudomomentum_cell_averages, compute_pseudomomentum_cell_integrals, set_quadruplets!,
+       update_quadruplet_rates!,
        m0, root_mean_square_wavenumber, mean_direction, significant_wave_height,
        total_action, total_deep_water_energy, Simulation, run!
 
@@ -240,9 +241,13 @@ Installs `DiscreteInteractionApproximation(interactions; power,
 resonance_tolerance)` with the `ListedQuadruplets` method
 (Sources/Sources.jl:378-428, 1268-1385).  `interactions` are objects with
 Ripple's `QuadrupletTransferInteraction` fields (1-based `donor1_m`, …,
-`receiver2_n`, constant `rate`); `frequency` is the grid's frequency
-centres.  The library runs the reference's index, resonance and rate checks
-and throws on the first failure.
+`receiver2_n`, `rate`); `frequency` is the grid's frequency centres.  A
+`rate` may be a number, an `(Nx, Ny)` array or a function of `(x, y, t)`,
+`(x, y)` or `(t)` (Ripple's `source_value`, Sources.jl:1450-1467); the
+non-constant ones go to the device as a per-column field evaluated at
+`model.clock.time` (`update_quadruplet_rates!` re-evaluates it, before each
+step for a rate that depends on time).  The library runs the reference's
+index, resonance and rate checks and throws on the first failure.
 """
 function set_quadruplets!(model::MI355XWaveModel, interactions, frequency; power=1.0, resonance_tolerance=1e-12)
     bins = Int32[]
@@ -250,12 +255,39 @@ function set_quadruplets!(model::MI355XWaveModel, interactions, frequency; power
     for q in interactions
         append!(bins, Int32.((q.donor1_m, q.donor1_n, q.donor2_m, q.donor2_n,
                               q.receiver1_m, q.receiver1_n, q.receiver2_m, q.receiver2_n)) .- Int32(1))
-        push!(rates, Float64(q.rate))
+        push!(rates, q.rate isa Number ? Float64(q.rate) : 0.0)
     end
     check(ccall(sym(:rpl_set_quadruplets), Cint,
                 (Ptr{Cvoid}, Int32, Ptr{Int32}, Ptr{Cdouble}, Ptr{Cdouble}, Cdouble, Cdouble),
                 model.ctx, length(rates), bins, rates, collect(Float64, frequency), Float64(power),
                 Float64(resonance_tolerance)))
+    all(q -> q.rate isa Number, interactions) || update_quadruplet_rates!(model, interactions)
+    return model
+end
+
+function _source_value(value, x, y, t)
+    value isa Number && return Float64(value)
+    applicable(value, x, y, t) && return Float64(value(x, y, t))
+    applicable(value, x, y) && return Float64(value(x, y))
+    applicable(value, t) && return Float64(value(t))
+    throw(ArgumentError("callable source parameters must accept (x, y, t), (x, y), or (t)"))
+end
+
+"""
+    update_quadruplet_rates!(model, interactions)
+
+Evaluates every interaction's rate at the cell centres and `model.clock.time`
+and uploads them as the per-column rate field (`rpl_set_quadruplet_rate_field`).
+"""
+function update_quadruplet_rates!(model::MI355XWaveModel, interactions)
+    Nx, Ny = model.size[1], model.size[2]
+    xc, yc = xnodes(model), ynodes(model)
+    t = model.clock.time
+    field = zeros(Nx, Ny, length(interactions))
+    for (k, q) in enumerate(interactions), j in 1:Ny, i in 1:Nx
+        field[i, j, k] = q.rate isa AbstractArray ? Float64(q.rate[i, j]) : _source_value(q.rate, xc[i], yc[j], t)
+    end
+    check(ccall(sym(:rpl_set_quadruplet_rate_field), Cint, (Ptr{Cvoid}, Ptr{Cdouble}), model.ctx, field))
     return model
 end
 

@@ -106,6 +106,7 @@ SIGNATURES = {
     "rpl_get_wind": ([_P, _DP, _DP, _DP], ctypes.c_int),
     "rpl_set_storm_track": ([_P, _I32, _DP, _DP, _DP], ctypes.c_int),
     "rpl_set_quadruplets": ([_P, _I32, ctypes.POINTER(_I32), _DP, _DP, _D, _D], ctypes.c_int),
+    "rpl_set_quadruplet_rate_field": ([_P, _DP], ctypes.c_int),
     "rpl_enable_kernel_timing": ([_P, _I32], ctypes.c_int),
     "rpl_get_kernel_timing": ([_P, _DP, _I64P], ctypes.c_int),
     "rpl_get_surrogate_timing": ([_P, _DP, _I64P], ctypes.c_int),

@@ -290,6 +290,9 @@ class SpectralWaveModel:
 
     def tendency(self):
         """compute_tendencies!(model) followed by interior(model.tendencies)."""
+        rates = getattr(self, "_listed_rates", None)
+        if rates is not None and rates.time_dependent:
+            rates.upload(self.clock.time)
         G = np.empty(int(np.prod(self.size)), dtype=self.dtype)
         _lib.check(self._ctx.lib.rpl_tendency(self._ctx.ptr, _lib.vptr(G)))
         return G.reshape(self.size, order="F")
@@ -338,15 +341,29 @@ def set_(model, N=None):
     return model
 
 
-def time_step_(model, dt, nsteps=1):
-    """`time_step!(model, dt)`, nsteps times, on the device."""
-    if not dt > 0:
-        raise _lib.RippleArgumentError(-1, "time step must be positive")
-    _lib.check(model._ctx.lib.rpl_time_step(model._ctx.ptr, float(dt), int(nsteps)))
+def _read_clock(model):
     t = ctypes.c_double()
     it = ctypes.c_int64()
     _lib.check(model._ctx.lib.rpl_get_clock(model._ctx.ptr, ctypes.byref(t), ctypes.byref(it)))
     model.clock.time, model.clock.iteration = t.value, it.value
+
+
+def time_step_(model, dt, nsteps=1):
+    """`time_step!(model, dt)`, nsteps times, on the device.  Listed
+    quadruplet rates that depend on time are evaluated at the clock before
+    each step (every stage of a step sees the step's start time, as
+    model.clock.time in the reference)."""
+    if not dt > 0:
+        raise _lib.RippleArgumentError(-1, "time step must be positive")
+    rates = getattr(model, "_listed_rates", None)
+    if rates is not None and rates.time_dependent:
+        for _ in range(int(nsteps)):
+            rates.upload(model.clock.time)
+            _lib.check(model._ctx.lib.rpl_time_step(model._ctx.ptr, float(dt), 1))
+            _read_clock(model)
+        return model
+    _lib.check(model._ctx.lib.rpl_time_step(model._ctx.ptr, float(dt), int(nsteps)))
+    _read_clock(model)
     return model
 
 

@@ -148,14 +148,88 @@ class SymmetricQuadruplet:
 class QuadrupletTransferInteraction:
     """`QuadrupletTransferInteraction(donor1, donor2, receiver1, receiver2;
     rate)` (Sources/Sources.jl:378-402): an explicit quadruplet, each bin a
-    1-based (frequency m, direction n) pair as in the reference, with a
-    constant rate."""
+    1-based (frequency m, direction n) pair as in the reference.  The rate is
+    what the reference's source_value accepts (Sources.jl:1450-1467): a
+    number, an (Nx, Ny) array (its [i, j] value per column), or a function of
+    (x, y, t), (x, y) or (t) evaluated at the cell centres and the model
+    clock."""
 
     def __init__(self, donor1, donor2, receiver1, receiver2, rate=0.0):
         self.bins = tuple((int(b[0]), int(b[1])) for b in (donor1, donor2, receiver1, receiver2))
-        if callable(rate) or np.ndim(rate) != 0:
-            raise NotImplementedError("quadruplet transfer rates are constants here")
-        self.rate = float(rate)
+        if callable(rate):
+            self.rate = rate
+        elif np.ndim(rate) == 0:
+            self.rate = float(rate)
+        elif np.ndim(rate) == 2:
+            self.rate = np.array(rate, dtype=np.float64)
+        else:
+            raise ValueError("a quadruplet rate is a number, an (Nx, Ny) array or a function of (x, y, t), "
+                             "(x, y) or (t)")
+
+
+def _arity(fun):
+    """How a callable source parameter is called, in the reference's order of
+    `applicable` tests (Sources.jl:1458-1466): 3 for (x, y, t), 2 for (x, y),
+    1 for (t)."""
+    import inspect
+    try:
+        sig = inspect.signature(fun)
+    except (TypeError, ValueError):
+        return 3
+    for k in (3, 2, 1):
+        try:
+            sig.bind(*range(k))
+            return k
+        except TypeError:
+            pass
+    raise _lib.RippleArgumentError(-1, "callable source parameters must accept (x, y, t), (x, y), or (t)")
+
+
+def source_value_field(value, x_nodes, y_nodes, t):
+    """source_value (Sources/Sources.jl:1450-1467) of a parameter at every
+    column: (Nx, Ny) values at the cell centres and time t."""
+    nx, ny = len(x_nodes), len(y_nodes)
+    if callable(value):
+        k = _arity(value)
+        X, Y = np.meshgrid(np.asarray(x_nodes, dtype=np.float64), np.asarray(y_nodes, dtype=np.float64),
+                           indexing="ij")
+        args = {3: (X, Y, t), 2: (X, Y), 1: (t,)}[k]
+        try:
+            v = np.broadcast_to(np.asarray(value(*args), dtype=np.float64), (nx, ny))
+        except Exception:  # a function of scalars only
+            if k == 1:
+                v = np.full((nx, ny), float(value(t)))
+            else:
+                v = np.array([[float(value(x, y, t) if k == 3 else value(x, y)) for y in y_nodes]
+                              for x in x_nodes])
+        return np.array(v, dtype=np.float64)
+    if np.ndim(value) == 0:
+        return np.full((nx, ny), float(value))
+    a = np.asarray(value, dtype=np.float64)
+    if a.shape != (nx, ny):
+        raise ValueError(f"a rate array must have the grid's (Nx, Ny) = ({nx}, {ny}) shape; got {a.shape}")
+    return a
+
+
+class ListedRateField:
+    """Uploads the per-column rates of a list whose rates are not all
+    constants (rpl_set_quadruplet_rate_field), at the model clock; a rate
+    that depends on time is evaluated again before every step."""
+
+    def __init__(self, model, interactions, x_nodes, y_nodes):
+        self.model, self.interactions = model, interactions
+        self.x_nodes, self.y_nodes = x_nodes, y_nodes
+        self.time_dependent = any(callable(q.rate) and _arity(q.rate) != 2 for q in interactions)
+
+    def field(self, t):
+        """(Nx Ny, count) rates, column i + Nx j fastest."""
+        cols = [source_value_field(q.rate, self.x_nodes, self.y_nodes, t).ravel(order="F")
+                for q in self.interactions]
+        return np.ascontiguousarray(np.stack(cols, axis=1).ravel(order="F"))
+
+    def upload(self, t):
+        f = self.field(t)
+        _lib.check(self.model._ctx.lib.rpl_set_quadruplet_rate_field(self.model._ctx.ptr, _lib.dptr(f)))
 
 
 class ListedQuadruplets:
@@ -282,13 +356,20 @@ def attach(model, sources):
         _lib.check(lib.rpl_set_storm_track(model._ctx.ptr, len(t), _lib.dptr(t), _lib.dptr(cx), _lib.dptr(cy)))
     else:
         _lib.check(lib.rpl_set_storm_track(model._ctx.ptr, 0, None, None, None))
+    model._listed_rates = None
     if listed is not None and listed.interactions:
         bins = np.ascontiguousarray([[v - 1 for b in q.bins for v in b] for q in listed.interactions], dtype=np.int32)
-        rates = np.ascontiguousarray([q.rate for q in listed.interactions], dtype=np.float64)
+        # rates given per column or as functions go in a field after the list
+        rates = np.ascontiguousarray([q.rate if isinstance(q.rate, float) else 0.0 for q in listed.interactions],
+                                     dtype=np.float64)
         freq = np.ascontiguousarray(model.spectral_grid.frequency, dtype=np.float64)
         _lib.check(lib.rpl_set_quadruplets(model._ctx.ptr, len(rates),
                                            bins.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), _lib.dptr(rates),
                                            _lib.dptr(freq), listed.power, listed.resonance_tolerance))
+        if not all(isinstance(q.rate, float) for q in listed.interactions):
+            field = ListedRateField(model, listed.interactions, xn, yn)
+            field.upload(model.clock.time)
+            model._listed_rates = field
     return s
 
 

@@ -203,3 +203,73 @@ def test_argument_checks():
     m = model(ra.DiscreteInteractionApproximation(), [0.2], 4)
     ra.set_(m, N=1.0)
     assert not m.tendency().any()
+
+
+def _field(quads, m, t):
+    """(Nx, Ny, count) rates of the listed interactions at time t."""
+    from ripple_amd.sources import source_value_field
+    from ripple_amd.grids import xnodes, ynodes
+    return np.stack([source_value_field(q.rate, xnodes(m.grid), ynodes(m.grid), t) for q in quads], axis=2)
+
+
+@pytest.mark.parametrize("power", [1.0, 1.5])
+def test_rates_per_column_and_in_time_against_oracle(oracle, power):
+    """Rates as the reference's source_value takes them (Sources.jl:
+    1450-1467): an (Nx, Ny) array, a function of (x, y) and a function of
+    (x, y, t), which the host evaluates at the clock before every step."""
+    say(f"listed quadruplets fp64 with per-column and time-dependent rates, power={power}")
+    freq = np.geomspace(0.08, 0.3, 8)
+    base = _resonant_list()
+    nx, ny = 6, 5
+    rng = np.random.default_rng(11)
+    arr = rng.uniform(0.5, 2.5, (nx, ny))
+    forms = [arr, lambda x, y: 1.0 + 0.5 * np.sin(7 * x) * np.cos(5 * y),
+             lambda x, y, t: 2.0 + np.cos(3 * x + 40 * t) + 0.1 * y]
+    quads = tuple(Q(*q.bins, rate=forms[k % 3]) if k % 2 == 0 else q for k, q in enumerate(base))
+    dia = ra.DiscreteInteractionApproximation(quads, power=power)
+    m = model(dia, freq, 24, nx=nx, ny=ny)
+    N0 = np.asfortranarray(rng.uniform(0.0, 1.0, m.size))
+    ra.set_(m, N=N0)
+    sg = m.spectral_grid
+    inter = [tuple((b[0] - 1, b[1] - 1) for b in q.bins) + (0.0,) for q in quads]
+    G = m.tendency()
+    hook, user, ctx = oracle.listed_hook(m.size, inter, power, weights(sg), rate_field=_field(quads, m, 0.0))
+    Gw = np.zeros(N0.size)
+    fn = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p)(hook.value)
+    Nf = oracle._c(N0)
+    fn(Nf.ctypes.data, Gw.ctypes.data, user)
+    Gw = Gw.reshape(m.size, order="F")
+    err = rel_l2(G, Gw)
+    say(f"tendency rel-L2 {err:.3e}")
+    assert err < (1e-300 if power == 1.0 else 1e-15)
+    wl = workloads.Workload("listed-field", m.grid, sg, None, N0, 0.02, 3, dict(timestepper="RK3"))
+    om = oracle.OracleModel(wl, transport=oracle.MODE_NONE, timestepper="RK3")
+    dt = 0.02
+    ra.time_step_(m, dt, 3)
+    for k in range(3):
+        hook, user, ctx = oracle.listed_hook(m.size, inter, power, weights(sg), rate_field=_field(quads, m, k * dt))
+        om.step(dt, 1, hook, user)
+    err = rel_l2(m.action, om.action)
+    say(f"3 RK3 steps rel-L2 {err:.3e}")
+    assert err < (1e-300 if power == 1.0 else 1e-14)
+    assert m.clock.iteration == 3
+    # the time dependence matters: rates frozen at t = 0 give another answer
+    frozen = oracle.OracleModel(wl, transport=oracle.MODE_NONE, timestepper="RK3")
+    hook, user, ctx = oracle.listed_hook(m.size, inter, power, weights(sg), rate_field=_field(quads, m, 0.0))
+    frozen.step(dt, 3, hook, user)
+    assert rel_l2(frozen.action, om.action) > 1e-8
+
+
+def test_rate_field_errors():
+    say("listed quadruplet rate-field errors")
+    freq = np.geomspace(0.08, 0.3, 8)
+    quads = _resonant_list()
+    neg = tuple(Q(*q.bins, rate=(lambda x, y: x - 0.5)) if k == 0 else q for k, q in enumerate(quads))
+    with pytest.raises(ra.RippleArgumentError):
+        model(ra.DiscreteInteractionApproximation(neg), freq, 24, nx=6, ny=5)  # Sources.jl:1347
+    m = model(ra.DiscreteInteractionApproximation(quads), freq, 24, nx=6, ny=5)
+    lib = m._ctx.lib
+    ra._lib.check(lib.rpl_set_quadruplet_rate_field(m._ctx.ptr, None))  # back to the constants
+    bad = np.full(6 * 5 * len(quads), np.nan)
+    with pytest.raises(ra.RippleArgumentError):
+        ra._lib.check(lib.rpl_set_quadruplet_rate_field(m._ctx.ptr, ra._lib.dptr(bad)))

@@ -74,3 +74,38 @@ def test_power_and_clipping(oracle):
                     want[d] -= flux / w[d]
         np.testing.assert_allclose(G[i, 0], want, rtol=1e-14, atol=1e-300)
     assert G[1, 0, 2, 1] == 0  # column 1 lost the second interaction
+
+
+def test_rate_field_is_each_columns_constant_rate(oracle):
+    """Rates given per column (the reference's source_value of an array or a
+    function of position, Sources.jl:1450-1467): every column's tendency is
+    the constant-rate tendency of that column alone with its own rates, a
+    field equal to the constants everywhere changes nothing, and the
+    tendency is linear in each column's rate."""
+    rng = np.random.default_rng(4)
+    nx, ny, nk, nphi = 3, 2, 3, 4
+    w = rng.uniform(0.5, 2.0, (nk, nphi))
+    N = np.asfortranarray(rng.uniform(0.1, 1.0, (nx, ny, nk, nphi)))
+    quads = [((0, 0), (0, 2), (0, 1), (0, 3), 0.2), ((2, 0), (2, 2), (2, 1), (0, 3), 0.7)]
+    field = rng.uniform(0.0, 2.0, (nx, ny, len(quads)))
+    field[1, 0, 0] = 0.0  # a column where the first interaction is off
+    G = _hook_tendency_field(oracle, N, quads, 1.5, w, field)
+    for i in range(nx):
+        for j in range(ny):
+            own = [q[:4] + (field[i, j, k],) for k, q in enumerate(quads)]
+            want = _hook_tendency(oracle, N[i:i + 1, j:j + 1], own, 1.5, w)[0, 0]
+            np.testing.assert_array_equal(G[i, j], want)
+    const = np.broadcast_to(np.array([q[4] for q in quads]), (nx, ny, len(quads)))
+    np.testing.assert_array_equal(_hook_tendency_field(oracle, N, quads, 1.5, w, const),
+                                  _hook_tendency(oracle, N, quads, 1.5, w))
+    np.testing.assert_allclose(_hook_tendency_field(oracle, N, quads, 1.5, w, 2 * field), 2 * G, rtol=1e-14,
+                               atol=1e-300)
+
+
+def _hook_tendency_field(oracle, N, interactions, power, w, field):
+    hook, user, ctx = oracle.listed_hook(N.shape, interactions, power, w, rate_field=field)
+    Nf = oracle._c(N)
+    G = np.zeros_like(Nf)
+    fn = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p)(hook.value)
+    fn(Nf.ctypes.data, G.ctypes.data, user)
+    return G.reshape(N.shape, order="F")

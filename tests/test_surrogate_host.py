@@ -20,3 +20,31 @@ def test_bf16_bits_round_to_nearest_even_like_torch():
     assert _bf16_bits(np.array([1.01171875], np.float32))[0] == 0x3F82
     nan = _bf16_bits(np.array([np.nan], np.float32))[0]
     assert (nan & 0x7F80) == 0x7F80 and (nan & 0x7F) != 0
+
+
+def test_source_value_field_forms():
+    """source_value (Sources/Sources.jl:1450-1467): a number, an (Nx, Ny)
+    array, or a function of (x, y, t), (x, y) or (t), tried in that order."""
+    import numpy as np
+    import pytest
+    import ripple_amd as ra
+    from ripple_amd.sources import source_value_field, _arity
+    xn, yn = np.array([0.5, 1.5, 2.5]), np.array([10.0, 20.0])
+    np.testing.assert_array_equal(source_value_field(0.3, xn, yn, 7.0), np.full((3, 2), 0.3))
+    a = np.arange(6.0).reshape(3, 2)
+    np.testing.assert_array_equal(source_value_field(a, xn, yn, 7.0), a)
+    np.testing.assert_array_equal(source_value_field(lambda x, y, t: x + y + t, xn, yn, 7.0),
+                                  xn[:, None] + yn[None, :] + 7.0)
+    np.testing.assert_array_equal(source_value_field(lambda x, y: x * y, xn, yn, 7.0), xn[:, None] * yn[None, :])
+    np.testing.assert_array_equal(source_value_field(lambda t: 2 * t, xn, yn, 7.0), np.full((3, 2), 14.0))
+    import math
+    # a function of scalars only is evaluated column by column
+    np.testing.assert_allclose(source_value_field(lambda x, y: math.hypot(x, y), xn, yn, 0.0),
+                               np.hypot(xn[:, None], yn[None, :]), rtol=1e-15)
+    assert _arity(lambda x, y, t=0.0: 0) == 3 and _arity(lambda *a: 0) == 3
+    with pytest.raises(ra.RippleArgumentError):
+        _arity(lambda a, b, c, d: 0)
+    with pytest.raises(ValueError):
+        source_value_field(np.zeros((2, 2)), xn, yn, 0.0)
+    with pytest.raises(ValueError):
+        ra.QuadrupletTransferInteraction((1, 1), (1, 2), (1, 3), (1, 4), rate=np.zeros((2, 2, 2)))
