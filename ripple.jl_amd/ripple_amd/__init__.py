@@ -24,6 +24,7 @@ from .surrogate import (SourceNetwork, SurrogateConfig, training_pairs, attach_s
                         train as train_surrogate)
 from .ensemble import Ensemble
 from . import validation
+from .output import JLD2Writer, IterationInterval, TimeInterval, fetch_output
 from .diagnostics import (m0, root_mean_square_wavenumber, mean_direction, significant_wave_height,
                           total_action, total_deep_water_energy)
 
@@ -43,5 +44,5 @@ __all__ = [
     "MeanSpectrumWhitecapping", "LocalSaturationDissipation", "WhitecappingDissipation", "PrecomputedSources",
     "wind_state",
     "SourceNetwork", "SurrogateConfig", "training_pairs", "attach_surrogate", "surrogate_stats",
-    "train_surrogate", "Ensemble",
+    "train_surrogate", "Ensemble", "JLD2Writer", "IterationInterval", "TimeInterval", "fetch_output",
 ]

@@ -442,17 +442,23 @@ def cwcm_momentum_tendency_fields(model, old_N, dt, depth=None, q_grid=None, coe
 
 
 class Simulation:
-    """`Simulation(model; Δt, stop_iteration, stop_time)` (Oceananigans)."""
+    """`Simulation(model; Δt, stop_iteration, stop_time)` (Oceananigans), with
+    `output_writers`, a dict of writers (output.JLD2Writer) that `run_` calls
+    on their schedules."""
 
     def __init__(self, model, *, dt, stop_iteration=None, stop_time=None):
         if not dt > 0:
             raise ValueError("Δt must be positive")
         self.model, self.dt = model, float(dt)
         self.stop_iteration, self.stop_time = stop_iteration, stop_time
+        self.output_writers = {}
 
 
 def run_(sim):
-    """`run!(simulation)`: advance until stop_iteration / stop_time."""
+    """`run!(simulation)`: advance until stop_iteration / stop_time.  The
+    output writers write at the starting iteration and after every step
+    their schedule actuates on; the steps between two writes run as one
+    device call."""
     m = sim.model
     if sim.stop_iteration is not None:
         n = int(sim.stop_iteration) - m.clock.iteration
@@ -460,7 +466,20 @@ def run_(sim):
         n = int(math.ceil((sim.stop_time - m.clock.time) / sim.dt - 1e-9))
     else:
         raise ValueError("Simulation needs stop_iteration or stop_time")
-    if n > 0:
-        time_step_(m, sim.dt, n)
+    writers = list(sim.output_writers.values())
+    try:
+        for w in writers:
+            if w.schedule(m):
+                w.write(m)
+        while n > 0:
+            k = min([n] + [w.schedule.steps_to_next(m, sim.dt) for w in writers])
+            time_step_(m, sim.dt, k)
+            n -= k
+            for w in writers:
+                if w.schedule(m):
+                    w.write(m)
+    finally:
+        for w in writers:
+            w.close()
     m.synchronize()
     return sim
