@@ -196,6 +196,19 @@ int rpl_synchronize(rpl_ctx *ctx);
 int rpl_get_clock(rpl_ctx *ctx, double *time, int64_t *iteration);
 int rpl_set_clock(rpl_ctx *ctx, double time, int64_t iteration);
 
+/* A non-finite check of the action after every interval-th step (a flag,
+ * not a repair): one pass over N counts its NaN and infinite values on the
+ * step's stream, and the first checked iteration that found any is kept
+ * with its count.  interval = 0 turns it off; setting it clears the record.
+ * Steps of ensemble members (rpl_ensemble_step) are not checked.  No
+ * counterpart in the reference, which clips negative action
+ * (time_step.jl:72,103) and detects nothing; Oceananigans' NaNChecker
+ * callback is the nearest analogue. */
+int rpl_set_nonfinite_check(rpl_ctx *ctx, int32_t interval);
+/* The record: first_iteration = -1 (and count = 0) if no check has found a
+ * non-finite value since rpl_set_nonfinite_check.  Waits for the stream. */
+int rpl_get_nonfinite(rpl_ctx *ctx, int64_t *first_iteration, int64_t *count);
+
 /* m0, root_mean_square_wavenumber and mean_direction
  * (Diagnostics/moments.jl:33-123, diagnostic_operations.jl:56-165): three
  * (nx, ny) FP64 arrays; any pointer may be NULL. */

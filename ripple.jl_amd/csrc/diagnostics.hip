@@ -2,6 +2,8 @@
 // direction) and the compensated global totals used for conservation checks.
 #include "rpl_internal.h"
 
+#include <algorithm>
+
 #include <mutex>
 
 #pragma clang fp contract(off)
@@ -200,6 +202,73 @@ hipError_t launch_weighted_total_noalloc(int nx, int ny, int nk, int nphi, const
     return hipGetLastError();
 }
 
+// Non-finite check of the action (a flag, not a repair): one pass over N
+// counts its NaN and infinite values into counter[0] (16-byte loads, one
+// atomic per workgroup that found any), then one thread records the first
+// checked iteration with a nonzero count, record = {iteration, count}, and
+// clears the counter for the next check.  Both kernels run on the step's
+// stream; the host reads the record when asked.
+template <typename T>
+__global__ __launch_bounds__(256) void nonfinite_count_kernel(size_t n, const T *__restrict__ N,
+                                                              unsigned long long *counter)
+{
+    constexpr int V = 16 / sizeof(T);
+    unsigned long long local = 0;
+    const size_t nv = n / V, stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < nv; k += stride) {
+        typedef T Vec __attribute__((ext_vector_type(V)));
+        const Vec v = *reinterpret_cast<const Vec *>(N + k * V);
+#pragma unroll
+        for (int q = 0; q < V; ++q)
+            local += !isfinite(v[q]);
+    }
+    for (size_t k = nv * V + (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride)
+        local += !isfinite(N[k]);
+    __shared__ unsigned long long part[256 / 64];
+    for (int off = 32; off > 0; off >>= 1)
+        local += __shfl_down(local, off, 64);
+    if ((threadIdx.x & 63) == 0)
+        part[threadIdx.x >> 6] = local;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long tot = 0;
+        for (int w = 0; w < 256 / 64; ++w)
+            tot += part[w];
+        if (tot)
+            atomicAdd(counter, tot);
+    }
+}
+
+__global__ void nonfinite_commit_kernel(unsigned long long *counter, long long *record, long long iteration)
+{
+    const unsigned long long c = *counter;
+    if (c && record[0] < 0) {
+        record[0] = iteration;
+        record[1] = (long long)c;
+    }
+    *counter = 0;
+}
+
+template <typename T>
+hipError_t launch_nonfinite_check(size_t n, const T *N, unsigned long long *counter, long long *record,
+                                  long long iteration, hipStream_t s)
+{
+    if ((reinterpret_cast<uintptr_t>(N) & 15) != 0)
+        return hipErrorInvalidValue;
+    const size_t want = (n / (16 / sizeof(T)) + 255) / 256;
+    const unsigned blocks = (unsigned)std::max<size_t>(1, std::min<size_t>(want, 8192));
+    hipLaunchKernelGGL(nonfinite_count_kernel<T>, dim3(blocks), dim3(256), 0, s, n, N, counter);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        return e;
+    hipLaunchKernelGGL(nonfinite_commit_kernel, dim3(1), dim3(1), 0, s, counter, record, iteration);
+    return hipGetLastError();
+}
+
+template hipError_t launch_nonfinite_check<float>(size_t, const float *, unsigned long long *, long long *, long long,
+                                                  hipStream_t);
+template hipError_t launch_nonfinite_check<double>(size_t, const double *, unsigned long long *, long long *,
+                                                   long long, hipStream_t);
 template hipError_t launch_weighted_total_noalloc<float>(int, int, int, int, const float *, const double *, double,
                                                          double *, hipStream_t);
 template hipError_t launch_weighted_total_noalloc<double>(int, int, int, int, const double *, const double *, double,

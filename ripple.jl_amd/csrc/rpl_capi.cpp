@@ -84,6 +84,10 @@ struct rpl_ctx {
     double gse_as = 0.5, gse_an = 0.5, gse_cg_max = 0, gse_dcg_max = 0;
     // the grid's x and y topology (rpl_set_topology): periodic or bounded
     bool periodic_x = true, periodic_y = true;
+    // non-finite check after every nf_interval-th step (rpl_set_nonfinite_check):
+    // a device counter and the record {first iteration, count}
+    int32_t nf_interval = 0;
+    void *nf_buf = nullptr;
     void *gse_cg = nullptr, *gse_dcg = nullptr;
     double time = 0;
     int64_t iteration = 0;
@@ -450,6 +454,20 @@ int launch_dia_t(rpl_ctx *c, const T *Nk)
                                              (T *)c->dia_add, c->stream);
     if (e != hipSuccess)
         return fail(RPL_ERR_HIP, std::string("symmetric quadruplet: ") + hipGetErrorString(e));
+    return RPL_OK;
+}
+
+// The non-finite check of the action after a step (rpl_set_nonfinite_check).
+int launch_nonfinite(rpl_ctx *c)
+{
+    auto *counter = (unsigned long long *)c->nf_buf;
+    auto *record = (long long *)c->nf_buf + 1;
+    hipError_t e = is_f64(c) ? rpl::launch_nonfinite_check<double>(c->cnt, (const double *)c->A, counter, record,
+                                                                    c->iteration, c->stream)
+                             : rpl::launch_nonfinite_check<float>(c->cnt, (const float *)c->A, counter, record,
+                                                                   c->iteration, c->stream);
+    if (e != hipSuccess)
+        return fail(RPL_ERR_HIP, std::string("non-finite check: ") + hipGetErrorString(e));
     return RPL_OK;
 }
 
@@ -1241,7 +1259,7 @@ int rpl_destroy(rpl_ctx *c)
                     (void *)c->bin_hi, (void *)c->s_amax, (void *)c->s_flags, c->dUx, c->dUy, c->pm_O,
                     c->pm_dO, c->pm_kx, c->pm_ky, c->gse_cg, c->gse_dcg, c->dia_flags, c->dia_bins, c->dia_wef,
                     c->dia_wdiv, c->dia_efa, c->dia_inv_efa, c->dia_cfa, c->dia_add, c->dia_gat_off, c->dia_gat_q, c->dia_gat_w, c->listed_bins,
-                    c->listed_rates, c->listed_w, c->listed_field})
+                    c->listed_rates, c->listed_w, c->listed_field, c->nf_buf})
         free_ptr(p);
     for (auto &ev : c->events) {
         (void)hipEventDestroy(ev.first);
@@ -1885,7 +1903,45 @@ int rpl_time_step(rpl_ctx *c, double dt, int32_t nsteps)
             return rc;
         c->time += dt;  // tick!(clock, dt), once per step
         c->iteration += 1;
+        if (c->nf_interval > 0 && c->iteration % c->nf_interval == 0 && (rc = launch_nonfinite(c)))
+            return rc;
     }
+    return RPL_OK;
+}
+
+int rpl_set_nonfinite_check(rpl_ctx *c, int32_t interval)
+{
+    if (!c)
+        return fail(RPL_ERR_ARG, "null context");
+    if (interval < 0)
+        return fail(RPL_ERR_ARG, "the non-finite check interval must be >= 0");
+    RPL_HIP(hipSetDevice(c->device));
+    int rc;
+    if (interval > 0 && !c->nf_buf && (rc = alloc(&c->nf_buf, 3 * sizeof(long long))))
+        return rc;
+    if (c->nf_buf) {
+        const long long init[3] = {0, -1, 0};  // counter, first iteration, count
+        RPL_HIP(hipStreamSynchronize(c->stream));
+        RPL_HIP(hipMemcpy(c->nf_buf, init, sizeof init, hipMemcpyHostToDevice));
+    }
+    c->nf_interval = interval;
+    return RPL_OK;
+}
+
+int rpl_get_nonfinite(rpl_ctx *c, int64_t *first_iteration, int64_t *count)
+{
+    if (!c)
+        return fail(RPL_ERR_ARG, "null context");
+    long long rec[3] = {0, -1, 0};
+    if (c->nf_buf) {
+        RPL_HIP(hipSetDevice(c->device));
+        RPL_HIP(hipStreamSynchronize(c->stream));
+        RPL_HIP(hipMemcpy(rec, c->nf_buf, sizeof rec, hipMemcpyDeviceToHost));
+    }
+    if (first_iteration)
+        *first_iteration = rec[1];
+    if (count)
+        *count = rec[2];
     return RPL_OK;
 }
 

@@ -224,6 +224,12 @@ template <typename T>
 hipError_t launch_weighted_total_noalloc(int nx, int ny, int nk, int nphi, const T *N, const double *w, double area,
                                          double *out, hipStream_t s);
 int weighted_total_blocks(int nx, int ny);
+// Counts the non-finite values of N (n elements, 16-byte aligned) and, when
+// there are any and record[0] < 0, sets record = {iteration, count};
+// counter (one device word) is left at zero.
+template <typename T>
+hipError_t launch_nonfinite_check(size_t n, const T *N, unsigned long long *counter, long long *record,
+                                  long long iteration, hipStream_t s);
 
 template <typename T>
 hipError_t launch_cast_from_f64(const double *src, T *dst, size_t n, hipStream_t s);

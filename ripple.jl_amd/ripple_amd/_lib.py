@@ -99,6 +99,8 @@ SIGNATURES = {
     "rpl_synchronize": ([_P], ctypes.c_int),
     "rpl_get_clock": ([_P, _DP, _I64P], ctypes.c_int),
     "rpl_set_clock": ([_P, _D, ctypes.c_int64], ctypes.c_int),
+    "rpl_set_nonfinite_check": ([_P, _I32], ctypes.c_int),
+    "rpl_get_nonfinite": ([_P, _I64P, _I64P], ctypes.c_int),
     "rpl_diagnostics": ([_P, _DP, _DP, _DP], ctypes.c_int),
     "rpl_conservation": ([_P, _DP, _DP], ctypes.c_int),
     "rpl_weighted_total_noalloc": ([_P, _I32, _I32, _I32, _I32, _I32, _P, _D, _P, _P], ctypes.c_int),

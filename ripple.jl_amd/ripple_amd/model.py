@@ -300,6 +300,23 @@ class SpectralWaveModel:
     def synchronize(self):
         _lib.check(self._ctx.lib.rpl_synchronize(self._ctx.ptr))
 
+    def set_nonfinite_check(self, interval):
+        """Check the action for NaN and infinite values after every
+        interval-th step (0: off); clears the record."""
+        _lib.check(self._ctx.lib.rpl_set_nonfinite_check(self._ctx.ptr, int(interval)))
+
+    def nonfinite(self):
+        """(first checked iteration with a non-finite value or -1, their count then)."""
+        it, n = ctypes.c_int64(), ctypes.c_int64()
+        _lib.check(self._ctx.lib.rpl_get_nonfinite(self._ctx.ptr, ctypes.byref(it), ctypes.byref(n)))
+        return it.value, n.value
+
+    def set_clock(self, time, iteration):
+        """model.clock = (time, iteration): with set_(model, N=...) the state
+        a run resumes from."""
+        _lib.check(self._ctx.lib.rpl_set_clock(self._ctx.ptr, float(time), int(iteration)))
+        self.clock.time, self.clock.iteration = float(time), int(iteration)
+
     @property
     def stream(self):
         return self._ctx.lib.rpl_stream(self._ctx.ptr)
