@@ -12,7 +12,9 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liboracle.so")
+# RIPPLE_ORACLE_LIB selects another build of the same sources (the
+# address/undefined-behaviour sanitizer build, `make -C oracle sanitize`).
+LIB_PATH = os.environ.get("RIPPLE_ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
 
 _D = ctypes.c_double
 _F = ctypes.c_float
