@@ -93,6 +93,21 @@ int rpl_set_geometry(rpl_ctx *ctx, double dx, double dy,
                      const double *phi_c, const double *phi_f,
                      int32_t depth_kind, double depth, double gravity);
 
+/* A spatially varying depth (Oceananigans Field or function on the
+ * horizontal grid as SpectralWaveModel's depth; the reference's
+ * is_spatially_varying_depth, Models/transport.jl:102-139): depth holds
+ * nx ny positive values, i fastest.  The group velocity becomes the
+ * point-sampled finite-depth one of every column at its own depth
+ * (transport.jl:72-83, 111-139; intrinsic_velocity_component of the 4-D
+ * table, :105-109), and a prescribed current set afterwards takes each
+ * column's depth as its Q depth (q_depth_at, q_transform.jl:124-125;
+ * q_projection_depth, lagrangian_velocities.jl:192-193).  Call after
+ * rpl_set_geometry and before rpl_set_current (a current set before is
+ * dropped); NULL returns to rpl_set_geometry's depth.  Non-positive values:
+ * RPL_ERR_ARG (lagrangian_velocities.jl:123-124).  The pseudomomentum
+ * coupling and ensembles take a constant depth (RPL_ERR_UNSUPPORTED). */
+int rpl_set_depth_field(rpl_ctx *ctx, const double *depth);
+
 /* Prescribed Lagrangian-mean current (Coupling/current_coupling.jl:73-91,
  * 127-138): builds the Q weights W[k,m] = CDF(z_{k+1}) - CDF(z_k)
  * (q_kernel.jl:77-92) and runs the Doppler contraction

@@ -130,6 +130,43 @@ def doppler(u, v, W):
     return Ux.reshape((nx, ny, nk), order="F"), Uy.reshape((nx, ny, nk), order="F")
 
 
+def group_velocity_field(kappa, phi, depth, gravity=9.81):
+    """(cgx, cgy) of shape (Nx, Ny, Nk, Nphi): the point-sampled finite-depth
+    group velocity of every column at its own depth (transport.jl:111-139)."""
+    d = np.asarray(depth, dtype=np.float64)
+    nx, ny = d.shape
+    kc, pc = _c(kappa), _c(phi)
+    nk, npf = len(kc), len(pc)
+    cgx, cgy = np.zeros(nx * ny * nk * npf), np.zeros(nx * ny * nk * npf)
+    df = _c(d)  # held: a temporary would be freed before the call
+    lib().orc_group_velocity_field(_I(nx), _I(ny), _I(nk), _I(npf), _p(kc), _p(pc), _p(df), _D(gravity),
+                                   _p(cgx), _p(cgy))
+    shape = (nx, ny, nk, npf)
+    return cgx.reshape(shape, order="F"), cgy.reshape(shape, order="F")
+
+
+def q_weights_columns(z_faces, kappa, depth):
+    """Q weights of every column at its own depth, shape (Nx, Ny, Nz, Nk)."""
+    d = np.asarray(depth, dtype=np.float64)
+    nx, ny = d.shape
+    zf, kc = _c(z_faces), _c(kappa)
+    nz, nk = len(zf) - 1, len(kc)
+    W = np.zeros(nx * ny * nz * nk)
+    df = _c(d)
+    lib().orc_q_weights_columns(_I(nx * ny), _I(nz), _p(zf), _I(nk), _p(kc), _p(df), _p(W))
+    return W.reshape((nx, ny, nz, nk), order="F")
+
+
+def doppler_columns(u, v, Wc):
+    """U[i,j,m] = sum_k u[i,j,k] Wc[i,j,k,m], sequential over k."""
+    nx, ny, nz = u.shape
+    nk = Wc.shape[3]
+    Ux, Uy = np.zeros(nx * ny * nk), np.zeros(nx * ny * nk)
+    uf, vf, wf = _c(u), _c(v), _c(Wc)
+    lib().orc_doppler_columns(_I(nx), _I(ny), _I(nz), _p(uf), _p(vf), _I(nk), _p(wf), _p(Ux), _p(Uy))
+    return Ux.reshape((nx, ny, nk), order="F"), Uy.reshape((nx, ny, nk), order="F")
+
+
 def q_cell_integral(kappa, z1, z2, depth):
     return lib().orc_q_cell_integral(kappa, z1, z2, depth)
 
@@ -225,7 +262,14 @@ class OracleModel:
         else:
             self.mode = MODE_INTRINSIC
         finite = depth is not None
-        tabs = geometry(sg.kappa, sg.kappa_faces, sg.phi, sg.phi_faces, finite, depth or 0.0, gravity)
+        # depth: None (deep water), a number, or an (Nx, Ny) array (a
+        # spatially varying depth: group velocity and Q depth per column)
+        field = depth is not None and np.ndim(depth) == 2
+        tabs = geometry(sg.kappa, sg.kappa_faces, sg.phi, sg.phi_faces, finite, 1.0 if field else (depth or 0.0),
+                        gravity)
+        if field:
+            tabs["cgx"], tabs["cgy"] = group_velocity_field(sg.kappa, sg.phi, depth, gravity)
+        self._cg_cells = field
         self.tables = tabs
         self.dx = float(g.x_faces[1] - g.x_faces[0])
         self.dy = float(g.y_faces[1] - g.y_faces[0])
@@ -242,14 +286,17 @@ class OracleModel:
             Ux, Uy = doppler_override
         elif wl.velocities is not None:
             q_depth = g.Lz if depth is None else depth
-            W = q_weights(g.z_faces, sg.kappa, q_depth)
+            if field:
+                W = q_weights_columns(g.z_faces, sg.kappa, depth)
+            else:
+                W = q_weights(g.z_faces, sg.kappa, q_depth)
             if t == np.float32:
                 W = W.astype(np.float32).astype(np.float64)
                 u = wl.velocities[0].astype(np.float32).astype(np.float64)
                 v = wl.velocities[1].astype(np.float32).astype(np.float64)
             else:
                 u, v = wl.velocities
-            Ux, Uy = doppler(u, v, W)
+            Ux, Uy = doppler_columns(u, v, W) if field else doppler(u, v, W)
         else:
             Ux = Uy = np.zeros((nx, ny, nk))
         self.Ux, self.Uy = _c(Ux, t), _c(Uy, t)
@@ -257,6 +304,8 @@ class OracleModel:
         # (Coupling/current_coupling.jl:93-121); U follows every stage input.
         self.pm = None
         if pseudomomentum is not None:
+            if field:
+                raise NotImplementedError("the oracle's pseudomomentum coupling takes a constant depth")
             zf, q_depth = pseudomomentum
             O, _ = pm_overlap(zf, sg.kappa, q_depth)
             self.pm = [_c(O, t), _c(tabs["mu_x"], t), _c(tabs["mu_y"], t)]
@@ -293,6 +342,7 @@ class OracleModel:
 
     def _step(self, dt, nsteps=1, hook=None, user=None):
         L = lib()
+        L.orc_set_cg_cells(_I(int(self._cg_cells)))
         nx, ny, nk, npf = self.size
         sfx = "f64" if self.dtype == np.float64 else "f32"
         fn = getattr(L, f"orc_run_{sfx}")
@@ -307,6 +357,7 @@ class OracleModel:
 
     def tendency(self):
         L = lib()
+        L.orc_set_cg_cells(_I(int(self._cg_cells)))
         nx, ny, nk, npf = self.size
         sfx = "f64" if self.dtype == np.float64 else "f32"
         G = np.zeros_like(self.N)

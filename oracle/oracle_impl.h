@@ -101,8 +101,8 @@ void SFX(orc_refraction_tendency)(int nx, int ny, int nk, int nphi, const REAL *
             REAL Uxx = Ux_x[c3], Uxy = Ux_y[c3], Uyx = Uy_x[c3], Uyy = Uy_y[c3];
             REAL ck = -kap * (cph * cph * Uxx + cph * sph * (Uyx + Uxy) + sph * sph * Uyy);
             REAL cp = cph * sph * (Uxx - Uyy) + sph * sph * Uyx - cph * cph * Uxy;
-            REAL ux = cgx[m + (size_t)nk * n] + Uxc;
-            REAL uy = cgy[m + (size_t)nk * n] + Uyc;
+            REAL ux = (g_cg_cells ? cgx[IDX4(i, j, m, n)] : cgx[m + (size_t)nk * n]) + Uxc;
+            REAL uy = (g_cg_cells ? cgy[IDX4(i, j, m, n)] : cgy[m + (size_t)nk * n]) + Uyc;
 
             REAL xm3 = N[IDX4(orc_periodic(i - 3, nx), j, m, n)], xm2 = N[IDX4(orc_periodic(i - 2, nx), j, m, n)];
             REAL xm1 = N[IDX4(orc_periodic(i - 1, nx), j, m, n)], x0 = N[IDX4(i, j, m, n)];
@@ -157,8 +157,10 @@ void SFX(orc_intrinsic_tendency)(int nx, int ny, int nk, int nphi, const REAL *N
         int j = (int)(jmn % ny);
         int m = (int)((jmn / ny) % nk);
         int n = (int)(jmn / ((long)ny * nk));
-        REAL ux = cgx[m + (size_t)nk * n], uy = cgy[m + (size_t)nk * n];
+        const REAL ux_bin = cgx[m + (size_t)nk * n], uy_bin = cgy[m + (size_t)nk * n];
         for (int i = 0; i < nx; ++i) {
+            const REAL ux = g_cg_cells ? cgx[IDX4(i, j, m, n)] : ux_bin;
+            const REAL uy = g_cg_cells ? cgy[IDX4(i, j, m, n)] : uy_bin;
             REAL xm3 = N[IDX4(orc_periodic(i - 3, nx), j, m, n)], xm2 = N[IDX4(orc_periodic(i - 2, nx), j, m, n)];
             REAL xm1 = N[IDX4(orc_periodic(i - 1, nx), j, m, n)], x0 = N[IDX4(i, j, m, n)];
             REAL xp1 = N[IDX4(orc_periodic(i + 1, nx), j, m, n)], xp2 = N[IDX4(orc_periodic(i + 2, nx), j, m, n)];

@@ -35,6 +35,16 @@ static inline int orc_clamp(int i, int n) { return i < 0 ? 0 : (i >= n ? n - 1 :
 #define REAL double
 #define REAL_EPS DBL_EPSILON
 #define SFX(name) name##_f64
+/* Group velocities per cell (a spatially varying depth, transport.jl:
+ * 105-109 intrinsic_velocity_component of a 4-D table) when set, per (kappa,
+ * phi) bin otherwise.  Set by the front end before each transport call. */
+static int g_cg_cells = 0;
+int orc_set_cg_cells(int on)
+{
+    g_cg_cells = on != 0;
+    return 0;
+}
+
 #include "oracle_impl.h"
 #undef REAL
 #undef REAL_EPS
@@ -174,6 +184,33 @@ void orc_group_velocity_tables(int nk, int nphi, const double *kc, const double 
         }
 }
 
+/* Models/transport.jl:111-139 with a spatially varying depth: every column's
+ * point-sampled finite-depth group velocity (transport.jl:72-83) at its own
+ * depth[i + nx j], tables (nx, ny, nk, nphi), i fastest. */
+void orc_group_velocity_field(int nx, int ny, int nk, int nphi, const double *kc, const double *pc,
+                              const double *depth, double gravity, double *cgx, double *cgy)
+{
+    const size_t plane = (size_t)nx * ny;
+    for (int n = 0; n < nphi; ++n)
+        for (int m = 0; m < nk; ++m)
+            for (size_t ij = 0; ij < plane; ++ij) {
+                size_t o = ij + plane * ((size_t)m + (size_t)nk * n);
+                double k = kc[m];
+                double kx = k * cos(pc[n]), ky = k * sin(pc[n]);
+                if (k == 0) {
+                    cgx[o] = 0;
+                    cgy[o] = 0;
+                    continue;
+                }
+                double mu = k * depth[ij];
+                double phase = sqrt(gravity * tanh(mu) / k);
+                double factor = (1 + 2 * mu / sinh(2 * mu)) / 2;
+                double cg = factor * phase;
+                cgx[o] = cg * kx / k;
+                cgy[o] = cg * ky / k;
+            }
+}
+
 /* ------------------------------------------------------------------------ */
 /* Q-transform (Coupling/q_kernel.jl:60-92, doppler_velocity.jl:45-62).       */
 /* ------------------------------------------------------------------------ */
@@ -203,6 +240,22 @@ void orc_q_weights(int nz, const double *zf, int nk, const double *kappa, double
             double lo = z1 < z2 ? z1 : z2, hi = z1 < z2 ? z2 : z1;
             W[k + (size_t)nz * m] = orc_q_cdf(kappa[m], hi, depth, 1e-6) - orc_q_cdf(kappa[m], lo, depth, 1e-6);
         }
+}
+
+/* The weights of every column at its own depth (q_depth_at(depth, i, j),
+ * q_transform.jl:124-125): Wc[ij + nx ny (k + nz m)]. */
+void orc_q_weights_columns(int ncol, int nz, const double *zf, int nk, const double *kappa, const double *depth,
+                           double *Wc)
+{
+#pragma omp parallel for schedule(static)
+    for (long ij = 0; ij < (long)ncol; ++ij)
+        for (int m = 0; m < nk; ++m)
+            for (int k = 0; k < nz; ++k) {
+                double z1 = zf[k], z2 = zf[k + 1];
+                double lo = z1 < z2 ? z1 : z2, hi = z1 < z2 ? z2 : z1;
+                Wc[(size_t)ij + (size_t)ncol * ((size_t)k + (size_t)nz * m)] =
+                    orc_q_cdf(kappa[m], hi, depth[ij], 1e-6) - orc_q_cdf(kappa[m], lo, depth[ij], 1e-6);
+            }
 }
 
 /* q_kernel.jl:30-33 (q_cell_integral): CDF(zhigh) - CDF(zlow). */
@@ -281,6 +334,30 @@ void orc_doppler(int nx, int ny, int nz, const double *uL, const double *vL, int
             size_t o = (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * m);
             Ux[o] = ax;
             Uy[o] = ay;
+        }
+    }
+}
+
+/* doppler_velocity.jl:45-62 with a depth per column: the weights of column
+ * ij are Wc[ij + nx ny (k + nz m)]; sequential accumulation over k. */
+void orc_doppler_columns(int nx, int ny, int nz, const double *uL, const double *vL, int nk, const double *Wc,
+                         double *Ux, double *Uy)
+{
+    const size_t plane = (size_t)nx * ny;
+#pragma omp parallel for schedule(static)
+    for (long jm = 0; jm < (long)ny * nk; ++jm) {
+        int j = (int)(jm % ny), m = (int)(jm / ny);
+        for (int i = 0; i < nx; ++i) {
+            const size_t ij = (size_t)i + (size_t)nx * j;
+            double ax = 0, ay = 0;
+            for (int k = 0; k < nz; ++k) {
+                double q = Wc[ij + plane * ((size_t)k + (size_t)nz * m)];
+                size_t c = ij + plane * k;
+                ax += uL[c] * q;
+                ay += vL[c] * q;
+            }
+            Ux[ij + plane * m] = ax;
+            Uy[ij + plane * m] = ay;
         }
     }
 }

@@ -33,6 +33,69 @@ __global__ __launch_bounds__(256) void doppler_sequential_kernel(int nx, int ny,
     Uy[ij + plane * m] = ay;
 }
 
+// q_kernel.jl:60-92 (q_cdf_kernel) in FP64 on the device: the CDF of the
+// vertical projection kernel, sinh(2 mu s) / sinh(2 mu) written with
+// exponentials that cannot overflow, s itself below |mu| = 1e-6; and
+// q_kernel.jl:35-53, its kappa derivative.  The host forms the same
+// expressions for the tables of a constant depth.
+__device__ __forceinline__ double q_cdf_dev(double kappa, double z, double depth)
+{
+    const double small_mu = 1e-6;
+    const double mu = kappa * depth;
+    const double mus = fabs(mu) < small_mu ? (mu < 0 ? -small_mu : small_mu) : mu;
+    double s = (z + depth) / depth;
+    s = s < 0 ? 0 : (s > 1 ? 1 : s);
+    const double a = 2 * mus * s, b = 2 * mus;
+    const double fin = (exp(a - b) - exp(-a - b)) / (1 - exp(-2 * b));
+    return fabs(mu) < small_mu ? s : fin;
+}
+
+__device__ __forceinline__ double q_cdf_dkappa_dev(double kappa, double z, double depth)
+{
+    const double mu = kappa * depth;
+    double s = (z + depth) / depth;
+    s = s < 0 ? 0 : (s > 1 ? 1 : s);
+    if (fabs(mu) < 1e-6)
+        return depth * (4 * mu * s * (s * s - 1) / 3);
+    const double a = exp(-2 * mu * (1 - s)), b = exp(-2 * mu * (1 + s)), e = exp(-4 * mu);
+    const double num = a - b, den = 1 - e;
+    const double dnum = -2 * (1 - s) * a + 2 * (1 + s) * b, dden = 4 * e;
+    const double dF = (dnum * den - num * dden) / (den * den);
+    return depth * dF;
+}
+
+// doppler_velocity.jl:45-62 with a depth per column (q_depth_at(depth, i,
+// j), q_transform.jl:124-125): thread per (i, j, m) forms its column's
+// weights W[k, m] = CDF(z_high) - CDF(z_low) (or their kappa derivative,
+// doppler_velocity.jl:64-82) at depth[i + nx j] and accumulates over k in
+// ascending order, as the reference.
+template <typename T>
+__global__ __launch_bounds__(256) void doppler_columns_kernel(size_t plane, int nz, const T *__restrict__ u,
+                                                              const T *__restrict__ v,
+                                                              const double *__restrict__ zf,
+                                                              const double *__restrict__ kc,
+                                                              const double *__restrict__ depth, bool derivative,
+                                                              T *Ux, T *Uy)
+{
+    const size_t ij = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const int m = blockIdx.y;
+    if (ij >= plane)
+        return;
+    const double d = depth[ij], k = kc[m];
+    T ax = T(0), ay = T(0);
+    for (int q = 0; q < nz; ++q) {
+        const double z1 = zf[q], z2 = zf[q + 1];
+        const double hi = z1 < z2 ? z2 : z1, lo = z1 < z2 ? z1 : z2;
+        const double w = derivative ? q_cdf_dkappa_dev(k, hi, d) - q_cdf_dkappa_dev(k, lo, d)
+                                    : q_cdf_dev(k, hi, d) - q_cdf_dev(k, lo, d);
+        const T wt = (T)w;
+        ax += u[ij + plane * q] * wt;
+        ay += v[ij + plane * q] * wt;
+    }
+    Ux[ij + plane * m] = ax;
+    Uy[ij + plane * m] = ay;
+}
+
 // The same contraction on the matrix cores: U[ij, m] = sum_k u[ij, k] W[k, m]
 // is an [nx*ny x nz] x [nz x nk] product (M = nx*ny rows, K = nz, N = nk).
 // One wavefront owns a 16 x 16 tile of U (16 columns ij, 16 bins m) for both
@@ -344,6 +407,24 @@ hipError_t launch_gradients(int nx, int ny, int nk, const T *Ux, const T *Uy, T 
     return hipGetLastError();
 }
 
+template <typename T>
+hipError_t launch_doppler_columns(int nx, int ny, int nz, int nk, const T *u, const T *v, const double *zf,
+                                  const double *kc, const double *depth, bool derivative, T *Ux, T *Uy, hipStream_t s)
+{
+    const size_t plane = (size_t)nx * ny;
+    if (nk > 65535 || (plane + 255) / 256 > 0x7fffffffULL)
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL(doppler_columns_kernel<T>, dim3((unsigned)((plane + 255) / 256), (unsigned)nk), dim3(256), 0,
+                       s, plane, nz, u, v, zf, kc, depth, derivative, Ux, Uy);
+    return hipGetLastError();
+}
+
+template hipError_t launch_doppler_columns<double>(int, int, int, int, const double *, const double *,
+                                                   const double *, const double *, const double *, bool, double *,
+                                                   double *, hipStream_t);
+template hipError_t launch_doppler_columns<float>(int, int, int, int, const float *, const float *, const double *,
+                                                  const double *, const double *, bool, float *, float *,
+                                                  hipStream_t);
 template hipError_t launch_gradients<double>(int, int, int, const double *, const double *, double, double, double *,
                                              double *, double *, double *, hipStream_t);
 template hipError_t launch_gradients<float>(int, int, int, const float *, const float *, float, float, float *,

@@ -84,6 +84,13 @@ struct rpl_ctx {
     double gse_as = 0.5, gse_an = 0.5, gse_cg_max = 0, gse_dcg_max = 0;
     // the grid's x and y topology (rpl_set_topology): periodic or bounded
     bool periodic_x = true, periodic_y = true;
+    // a depth per column (rpl_set_depth_field): the host values, the group
+    // speed per (column, kappa bin) and the bins' direction tables (kx, ky, k
+    // for the reference-order kernel; kx / k, ky / k for the phi walk)
+    bool has_depth_field = false;
+    double gravity = 9.81;  // rpl_set_geometry's
+    std::vector<double> h_depth;
+    void *cgm = nullptr, *cg_vec = nullptr, *cg_dir = nullptr;
     // non-finite check after every nf_interval-th step (rpl_set_nonfinite_check):
     // a device counter and the record {first iteration, count}
     int32_t nf_interval = 0;
@@ -183,6 +190,8 @@ int upload_f64(rpl_ctx *c, const std::vector<double> &host, double *dst)
     return RPL_OK;
 }
 
+bool phi_walk_kernel(const rpl_ctx *c);
+
 template <typename T>
 rpl::StageParams<T> stage_params(const rpl_ctx *c, double dt)
 {
@@ -202,6 +211,18 @@ rpl::StageParams<T> stage_params(const rpl_ctx *c, double dt)
     p.cgy = (const T *)c->cgy;
     p.cosp = (const T *)c->cosp;
     p.sinp = (const T *)c->sinp;
+    if (c->has_depth_field) {
+        const size_t tab = (size_t)c->cfg.nkappa * c->cfg.nphi;
+        p.cgm = (const T *)c->cgm;
+        if (phi_walk_kernel(c)) {
+            p.cdx = (const T *)c->cg_dir;
+            p.cdy = (const T *)c->cg_dir + tab;
+        } else {
+            p.cdx = (const T *)c->cg_vec;
+            p.cdy = (const T *)c->cg_vec + tab;
+            p.cdk = (const T *)c->cg_vec + 2 * tab;
+        }
+    }
     // 1 / first spacing, formed in the element type (wave_current_refraction.jl:213).
     p.dx_inv = T(1) / (T)c->dx;
     p.dy_inv = T(1) / (T)c->dy;
@@ -533,6 +554,16 @@ bool use_fast(const rpl_ctx *c)
     return true;
 }
 
+// Whether launch_stage takes the phi-walk kernel (launch_stage_fast) rather
+// than the reference-order one.
+bool phi_walk_kernel(const rpl_ctx *c)
+{
+    if (is_f64(c))
+        return c->cfg.kernel == RPL_KERNEL_FAST && c->cfg.transport != RPL_TRANSPORT_NONE &&
+               c->plane < ((size_t)1 << 29);
+    return use_fast(c);
+}
+
 int record_begin(rpl_ctx *c, int phase)
 {
     if (!c->timing)
@@ -640,7 +671,7 @@ int launch_stage(rpl_ctx *c, double dt, int stage, const void *Nk, const void *N
         // FP64 under RPL_KERNEL_FAST: the phi-loop kernel (rounding-level
         // differences from the reference order); slabs of 2^29 cells and
         // more stay on the reference-order kernel (32-bit byte offsets)
-        if (c->cfg.kernel == RPL_KERNEL_FAST && tr != RPL_TRANSPORT_NONE && c->plane < ((size_t)1 << 29))
+        if (phi_walk_kernel(c))
             e = rpl::launch_stage_fast(p, tr, stage, (const double *)Nk, (const double *)N0, (double *)out,
                                        c->stream);
         else
@@ -1131,6 +1162,8 @@ int ensemble_member_check(rpl_ctx *c, rpl_ctx *first)
         return fail(RPL_ERR_UNSUPPORTED, "ensemble members run without propagation smoothing");
     if (c->dia || c->listed)
         return fail(RPL_ERR_UNSUPPORTED, "ensemble members run without the nonlinear (DIA) term");
+    if (c->has_depth_field)
+        return fail(RPL_ERR_UNSUPPORTED, "ensemble members take a constant or infinite depth");
     if (c->device != first->device)
         return fail(RPL_ERR_ARG, "ensemble members must live on one device");
     return RPL_OK;
@@ -1259,7 +1292,7 @@ int rpl_destroy(rpl_ctx *c)
                     (void *)c->bin_hi, (void *)c->s_amax, (void *)c->s_flags, c->dUx, c->dUy, c->pm_O,
                     c->pm_dO, c->pm_kx, c->pm_ky, c->gse_cg, c->gse_dcg, c->dia_flags, c->dia_bins, c->dia_wef,
                     c->dia_wdiv, c->dia_efa, c->dia_inv_efa, c->dia_cfa, c->dia_add, c->dia_gat_off, c->dia_gat_q, c->dia_gat_w, c->listed_bins,
-                    c->listed_rates, c->listed_w, c->listed_field, c->nf_buf})
+                    c->listed_rates, c->listed_w, c->listed_field, c->nf_buf, c->cgm, c->cg_vec, c->cg_dir})
         free_ptr(p);
     for (auto &ev : c->events) {
         (void)hipEventDestroy(ev.first);
@@ -1351,6 +1384,7 @@ int rpl_set_geometry(rpl_ctx *c, double dx, double dy, const double *kappa_c, co
         c->has_current = false;
     }
     c->gse = false;          // and the smoothing tables
+    c->has_depth_field = false;  // and a depth field's group speeds
     int rc;
     if ((rc = upload_real(c, kc.data(), c->kappa, nk)) || (rc = upload_real(c, cgx.data(), c->cgx, tab)) ||
         (rc = upload_real(c, cgy.data(), c->cgy, tab)) || (rc = upload_real(c, cs.data(), c->cosp, np)) ||
@@ -1362,6 +1396,7 @@ int rpl_set_geometry(rpl_ctx *c, double dx, double dy, const double *kappa_c, co
     RPL_HIP(hipStreamSynchronize(c->stream));
     c->dx = dx;
     c->dy = dy;
+    c->gravity = gravity;
     c->dk = kappa_f[1] - kappa_f[0];  // first(coordinate_spacings(cgrid, 1))
     c->dp = phi_f[1] - phi_f[0];
     c->has_geometry = true;
@@ -1442,6 +1477,73 @@ static int ensure_dU(rpl_ctx *c)
     return RPL_OK;
 }
 
+int rpl_set_depth_field(rpl_ctx *c, const double *depth)
+{
+    if (!c)
+        return fail(RPL_ERR_ARG, "null context");
+    ++c->generation;
+    if (!c->has_geometry)
+        return fail(RPL_ERR_STATE, "rpl_set_geometry has not been called");
+    const size_t plane = c->plane;
+    if (!depth) {
+        c->has_depth_field = false;
+        c->h_depth.clear();
+        c->has_current = false;  // the coupling's Q depth came from the field
+        c->pm = false;
+        return RPL_OK;
+    }
+    for (size_t k = 0; k < plane; ++k)
+        if (!(depth[k] > 0) || !std::isfinite(depth[k]))
+            return fail(RPL_ERR_ARG, "depth values must be positive");  // lagrangian_velocities.jl:123-124
+    const int nk = c->cfg.nkappa, np = c->cfg.nphi;
+    const size_t tab = (size_t)nk * np;
+    const double g = c->gravity;
+    // transport.jl:72-83 at each column's depth: cg = (1 + 2 mu / sinh 2mu) / 2
+    // * sqrt(g tanh(mu) / k), mu = k d; the velocity is (cg kx) / k.
+    std::vector<double> mag(plane * nk);
+    for (int m = 0; m < nk; ++m) {
+        const double k = c->h_kc[m];
+        for (size_t ij = 0; ij < plane; ++ij) {
+            if (k == 0) {
+                mag[ij + plane * m] = 0;
+                continue;
+            }
+            const double mu = k * depth[ij];
+            const double phase = std::sqrt(g * std::tanh(mu) / k);
+            const double factor = (1 + 2 * mu / std::sinh(2 * mu)) / 2;
+            mag[ij + plane * m] = factor * phase;
+        }
+    }
+    std::vector<double> vec(3 * tab), dir(2 * tab);
+    for (int n = 0; n < np; ++n)
+        for (int m = 0; m < nk; ++m) {
+            const size_t t = (size_t)m + (size_t)nk * n;
+            const double k = c->h_kc[m];
+            const double kx = k * std::cos(c->h_pc[n]), ky = k * std::sin(c->h_pc[n]);
+            vec[t] = k == 0 ? 0 : kx;
+            vec[tab + t] = k == 0 ? 0 : ky;
+            vec[2 * tab + t] = k == 0 ? 1 : k;  // a zero wavenumber has zero velocity
+            dir[t] = k == 0 ? 0 : kx / k;
+            dir[tab + t] = k == 0 ? 0 : ky / k;
+        }
+    RPL_HIP(hipSetDevice(c->device));
+    int rc;
+    if ((!c->cgm && (rc = alloc(&c->cgm, plane * nk * c->esize))) ||
+        (!c->cg_vec && (rc = alloc(&c->cg_vec, 3 * tab * c->esize))) ||
+        (!c->cg_dir && (rc = alloc(&c->cg_dir, 2 * tab * c->esize))))
+        return rc;
+    c->has_depth_field = false;  // until the tables are on the device
+    if ((rc = upload_real(c, mag.data(), c->cgm, plane * nk)) || (rc = upload_real(c, vec.data(), c->cg_vec, 3 * tab)) ||
+        (rc = upload_real(c, dir.data(), c->cg_dir, 2 * tab)))
+        return rc;
+    RPL_HIP(hipStreamSynchronize(c->stream));
+    c->h_depth.assign(depth, depth + plane);
+    c->has_depth_field = true;
+    c->has_current = false;  // a current set before takes the field's Q depth when set again
+    c->pm = false;
+    return RPL_OK;
+}
+
 int rpl_set_current(rpl_ctx *c, int32_t nz, const double *z_faces, const double *u, const double *v, double q_depth)
 {
     if (!c || !z_faces || !u || !v)
@@ -1486,7 +1588,41 @@ int rpl_set_current(rpl_ctx *c, int32_t nz, const double *z_faces, const double 
         rc = upload_real(c, W.data(), dW, W.size());
     if (!rc)
         rc = upload_real(c, dWk.data(), dD, dWk.size());
-    if (!rc) {
+    if (!rc && c->has_depth_field) {
+        // q_projection_depth of a depth field (lagrangian_velocities.jl:192-193):
+        // every column's weights at its own depth, formed on the device
+        void *dzf = nullptr, *dkc = nullptr, *ddep = nullptr;
+        if ((rc = alloc(&dzf, (nz + 1) * sizeof(double))) || (rc = alloc(&dkc, nk * sizeof(double))) ||
+            (rc = alloc(&ddep, c->plane * sizeof(double)))) {
+            free_ptr(dzf);
+            free_ptr(dkc);
+        } else {
+            RPL_HIP(hipMemcpy(dzf, z_faces, (nz + 1) * sizeof(double), hipMemcpyHostToDevice));
+            RPL_HIP(hipMemcpy(dkc, kc.data(), nk * sizeof(double), hipMemcpyHostToDevice));
+            RPL_HIP(hipMemcpy(ddep, c->h_depth.data(), c->plane * sizeof(double), hipMemcpyHostToDevice));
+            hipError_t e = hipSuccess;
+            for (int d = 0; d < 2 && e == hipSuccess; ++d) {
+                if (is_f64(c))
+                    e = rpl::launch_doppler_columns<double>(nx, ny, nz, nk, (const double *)du, (const double *)dv,
+                                                            (const double *)dzf, (const double *)dkc,
+                                                            (const double *)ddep, d == 1,
+                                                            (double *)(d ? c->dUx : c->Ux),
+                                                            (double *)(d ? c->dUy : c->Uy), c->stream);
+                else
+                    e = rpl::launch_doppler_columns<float>(nx, ny, nz, nk, (const float *)du, (const float *)dv,
+                                                           (const double *)dzf, (const double *)dkc,
+                                                           (const double *)ddep, d == 1, (float *)(d ? c->dUx : c->Ux),
+                                                           (float *)(d ? c->dUy : c->Uy), c->stream);
+            }
+            if (e == hipSuccess)
+                e = hipStreamSynchronize(c->stream);
+            if (e != hipSuccess)
+                rc = fail(RPL_ERR_HIP, std::string("doppler (depth per column): ") + hipGetErrorString(e));
+            free_ptr(dzf);
+            free_ptr(dkc);
+            free_ptr(ddep);
+        }
+    } else if (!rc) {
         hipError_t e;
         if (is_f64(c) && c->cfg.kernel == RPL_KERNEL_FAST)
             e = rpl::launch_doppler_mfma<double>(nx, ny, nz, nk, (const double *)du, (const double *)dv,
@@ -1532,6 +1668,9 @@ int rpl_set_pseudomomentum_coupling(rpl_ctx *c, int32_t nz, const double *z_face
         return fail(RPL_ERR_STATE, "rpl_set_geometry has not been called");
     if (c->cfg.transport != RPL_TRANSPORT_REFRACTION)
         return fail(RPL_ERR_STATE, "the pseudomomentum coupling drives the refraction transport (RPL_TRANSPORT_REFRACTION)");
+    if (c->has_depth_field)
+        return fail(RPL_ERR_UNSUPPORTED, "the pseudomomentum coupling takes a constant depth (the overlap tables "
+                                         "of a depth per column are not built)");
     if (nz < 1)
         return fail(RPL_ERR_ARG, "nz must be >= 1");
     if (!(q_depth > 0))

@@ -23,6 +23,13 @@ struct StageParams {
     const T *Uxx, *Uxy, *Uyx, *Uyy;   // current gradients (nx, ny, nk)
     const T *kappa;                   // kappa centres (nk)
     const T *cgx, *cgy;               // intrinsic group velocity (nk, nphi)
+    // A spatially varying depth (transport.jl:102-139): when cgm is set the
+    // group velocity of cell (i, j, m, n) is the point-sampled finite-depth
+    // speed cgm[i + nx j + nx ny m] of the column's depth times the bin's
+    // direction: (cgm cdx) / cdk in the reference-order kernel (cdx, cdy =
+    // kappa cos, kappa sin; cdk = kappa) and cgm cdx in the others (cdx, cdy
+    // = cos, sin).  (nk, nphi) tables.
+    const T *cgm, *cdx, *cdy, *cdk;
     const T *cosp, *sinp;             // cos/sin of phi centres (nphi)
     T dx_inv, dy_inv, dk_inv, dp_inv; // 1 / first spacing
     T dt;                             // time step in T (time_step.jl:16-20)
@@ -140,6 +147,14 @@ hipError_t launch_doppler(int nx, int ny, int nz, int nk, const float *u, const 
 template <typename T>
 hipError_t launch_doppler_mfma(int nx, int ny, int nz, int nk, const T *u, const T *v, const T *W, T *Ux, T *Uy,
                                hipStream_t s);
+// Doppler velocity of a prescribed current with a depth per column
+// (doppler_velocity.jl:45-62 with q_depth_at(depth, i, j), q_transform.jl:
+// 124-125): each column's Q weights (or their kappa derivative) formed on
+// the device at its depth, kc the kappa centres, zf the nz + 1 faces.
+template <typename T>
+hipError_t launch_doppler_columns(int nx, int ny, int nz, int nk, const T *u, const T *v, const double *zf,
+                                  const double *kc, const double *depth, bool derivative, T *Ux, T *Uy,
+                                  hipStream_t s);
 template <typename T>
 hipError_t launch_gradients(int nx, int ny, int nk, const T *Ux, const T *Uy, T dx_inv, T dy_inv,
                             T *Uxx, T *Uxy, T *Uyx, T *Uyy, hipStream_t s);

@@ -461,7 +461,7 @@ __global__ __launch_bounds__(64 * STAGE_ROWS) void stage_fast_kernel(const float
 // that of stage_tile, so the results are the same bit for bit.  Shared by the
 // per-model kernel and the ensemble work-list kernel (AS1: p and the slab
 // pointers came from memory).
-template <typename T, int TRANSPORT, int STAGE, int SK, int TY, bool AS1>
+template <typename T, int TRANSPORT, int STAGE, int SK, int TY, bool AS1, bool CGF = false>
 __device__ __forceinline__ void phi_tile(const T *__restrict__ Nk, const T *__restrict__ N0, T *out,
                                          const StageParams<T> &p, int ib, int jb, int m)
 {
@@ -499,6 +499,12 @@ __device__ __forceinline__ void phi_tile(const T *__restrict__ Nk, const T *__re
         kap = gld<AS1>(p.kappa, m);
     }
     constexpr bool SRC = SK != SOURCE_NONE;
+    // CGF (a depth per column): the cell's group speed at its column's depth,
+    // the same for every phi bin of the walk; the velocity is that speed
+    // times the bin's direction (cos, sin)
+    T cgm = T(0);
+    if constexpr (CGF)
+        cgm = ldo<AS1>(p.cgm + plane * (size_t)m, bo);
     // the column's source inputs, loaded once for every bin of the tile
     SourceColumn<T> scol{};
     JpColumn jcol{};
@@ -604,7 +610,11 @@ __device__ __forceinline__ void phi_tile(const T *__restrict__ Nk, const T *__re
         const size_t bin = (size_t)m + (size_t)nk * n;
         const T x0 = REF ? w[3] : cur.x0;
         T ux, uy;
-        if constexpr (REF) {
+        if constexpr (CGF) {
+            // the speed is nonnegative, so the upwind side is still the bin's
+            ux = cgm * sld(p.cdx, (size_t)(m + nk * n));
+            uy = cgm * sld(p.cdy, (size_t)(m + nk * n));
+        } else if constexpr (REF) {
             ux = gld<AS1>(p.cgx, m + nk * n);
             uy = gld<AS1>(p.cgy, m + nk * n);
         } else {
@@ -722,7 +732,7 @@ __device__ __forceinline__ void phi_tile(const T *__restrict__ Nk, const T *__re
 #ifndef RPL_F64_MIN_WAVES
 #define RPL_F64_MIN_WAVES 2
 #endif
-template <typename T, int TRANSPORT, int STAGE, int SK>
+template <typename T, int TRANSPORT, int STAGE, int SK, bool CGF>
 __global__ __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? RPL_F64_MIN_WAVES
                                                   : (TRANSPORT == RPL_TRANSPORT_INTRINSIC ? 7 : (SK != SOURCE_NONE ? 5 : 6)),
@@ -735,10 +745,10 @@ void stage_phi_kernel(const T *__restrict__ Nk, const T *__restrict__ N0, T *out
         t = tile_of(blockIdx.x, nbx * nby * p.nk, nbx, nby, p.nk, 1);
     else
         t = tile_of_spatial(blockIdx.x, nbx * nby * p.nk, nbx, nby, p.nk);
-    phi_tile<T, TRANSPORT, STAGE, SK, 4, false>(Nk, N0, out, p, t.ib, t.jb, t.m);
+    phi_tile<T, TRANSPORT, STAGE, SK, 4, false, CGF>(Nk, N0, out, p, t.ib, t.jb, t.m);
 }
 
-template <typename T, int TRANSPORT, int SK>
+template <typename T, int TRANSPORT, int SK, bool CGF = false>
 hipError_t launch_phi(const StageParams<T> &p, int stage, const T *Nk, const T *N0, T *out,
                       hipStream_t s)
 {
@@ -749,17 +759,20 @@ hipError_t launch_phi(const StageParams<T> &p, int stage, const T *Nk, const T *
     dim3 block(64, 4, 1), grid((unsigned)total, 1, 1);
     switch (stage) {
     case STAGE_TENDENCY:
-        hipLaunchKernelGGL((stage_phi_kernel<T, TRANSPORT, STAGE_TENDENCY, SK>), grid, block, 0, s, Nk, N0, out, p,
-                           nbx, nby);
+        hipLaunchKernelGGL((stage_phi_kernel<T, TRANSPORT, STAGE_TENDENCY, SK, CGF>), grid, block, 0, s, Nk, N0, out,
+                           p, nbx, nby);
         break;
     case STAGE_1:
-        hipLaunchKernelGGL((stage_phi_kernel<T, TRANSPORT, STAGE_1, SK>), grid, block, 0, s, Nk, N0, out, p, nbx, nby);
+        hipLaunchKernelGGL((stage_phi_kernel<T, TRANSPORT, STAGE_1, SK, CGF>), grid, block, 0, s, Nk, N0, out, p, nbx,
+                           nby);
         break;
     case STAGE_2:
-        hipLaunchKernelGGL((stage_phi_kernel<T, TRANSPORT, STAGE_2, SK>), grid, block, 0, s, Nk, N0, out, p, nbx, nby);
+        hipLaunchKernelGGL((stage_phi_kernel<T, TRANSPORT, STAGE_2, SK, CGF>), grid, block, 0, s, Nk, N0, out, p, nbx,
+                           nby);
         break;
     default:
-        hipLaunchKernelGGL((stage_phi_kernel<T, TRANSPORT, STAGE_3, SK>), grid, block, 0, s, Nk, N0, out, p, nbx, nby);
+        hipLaunchKernelGGL((stage_phi_kernel<T, TRANSPORT, STAGE_3, SK, CGF>), grid, block, 0, s, Nk, N0, out, p, nbx,
+                           nby);
         break;
     }
     return hipGetLastError();
@@ -966,6 +979,14 @@ hipError_t launch_ensemble_stage(const EnsembleMember *mem, int members, long lo
 hipError_t launch_stage_fast(const StageParams<float> &p, int transport, int stage, const float *Nk, const float *N0,
                              float *out, hipStream_t s)
 {
+    if (p.cgm && transport != RPL_TRANSPORT_NONE) {
+        // a depth per column: generic source path
+        if (transport == RPL_TRANSPORT_REFRACTION)
+            return p.src.on ? launch_phi<float, RPL_TRANSPORT_REFRACTION, SOURCE_GENERIC, true>(p, stage, Nk, N0, out, s)
+                            : launch_phi<float, RPL_TRANSPORT_REFRACTION, SOURCE_NONE, true>(p, stage, Nk, N0, out, s);
+        return p.src.on ? launch_phi<float, RPL_TRANSPORT_INTRINSIC, SOURCE_GENERIC, true>(p, stage, Nk, N0, out, s)
+                        : launch_phi<float, RPL_TRANSPORT_INTRINSIC, SOURCE_NONE, true>(p, stage, Nk, N0, out, s);
+    }
     if (p.src.on) {
         const bool jp = source_is_jp_ls(p.src);  // the specialised bundle path (same bits as the generic one)
         if (transport == RPL_TRANSPORT_REFRACTION)
@@ -989,6 +1010,15 @@ hipError_t launch_stage_fast(const StageParams<double> &p, int transport, int st
     // 32-bit byte offsets within a slab: FP64 slabs below 2^29 cells
     if ((size_t)p.nx * p.ny >= ((size_t)1 << 29))
         return hipErrorInvalidValue;
+    if (p.cgm) {
+        if (transport == RPL_TRANSPORT_REFRACTION)
+            return p.src.on ? launch_phi<double, RPL_TRANSPORT_REFRACTION, SOURCE_GENERIC, true>(p, stage, Nk, N0, out, s)
+                            : launch_phi<double, RPL_TRANSPORT_REFRACTION, SOURCE_NONE, true>(p, stage, Nk, N0, out, s);
+        if (transport == RPL_TRANSPORT_INTRINSIC)
+            return p.src.on ? launch_phi<double, RPL_TRANSPORT_INTRINSIC, SOURCE_GENERIC, true>(p, stage, Nk, N0, out, s)
+                            : launch_phi<double, RPL_TRANSPORT_INTRINSIC, SOURCE_NONE, true>(p, stage, Nk, N0, out, s);
+        return hipErrorInvalidValue;
+    }
     if (transport == RPL_TRANSPORT_REFRACTION)
         return p.src.on ? launch_phi<double, RPL_TRANSPORT_REFRACTION, SOURCE_GENERIC>(p, stage, Nk, N0, out, s)
                         : launch_phi<double, RPL_TRANSPORT_REFRACTION, SOURCE_NONE>(p, stage, Nk, N0, out, s);

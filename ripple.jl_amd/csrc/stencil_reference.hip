@@ -80,7 +80,17 @@ __global__ __launch_bounds__(256) void stage_reference_kernel(const T *__restric
     if (TRANSPORT != RPL_TRANSPORT_NONE) {
         const size_t row = c - i;
         const size_t c3 = (size_t)i + (size_t)nx * j + plane * m;
-        T ux = p.cgx[m + nk * n], uy = p.cgy[m + nk * n];
+        T ux, uy;
+        if (p.cgm) {
+            // transport.jl:72-83 at the column's depth: (cg kx) / k
+            const T mg = p.cgm[(size_t)i + (size_t)nx * j + plane * m];
+            const int t = m + nk * n;
+            ux = (mg * p.cdx[t]) / p.cdk[t];
+            uy = (mg * p.cdy[t]) / p.cdk[t];
+        } else {
+            ux = p.cgx[m + nk * n];
+            uy = p.cgy[m + nk * n];
+        }
         T ck = T(0), cp = T(0);
         if (TRANSPORT == RPL_TRANSPORT_REFRACTION) {
             const T kap = p.kappa[m], cph = p.cosp[n], sph = p.sinp[n];

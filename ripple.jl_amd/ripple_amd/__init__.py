@@ -10,7 +10,7 @@ from ._lib import RippleError, RippleArgumentError, load as load_library, LIB_PA
 from .grids import (RectilinearGrid, PolarWaveVectorGrid, FrequencyDirectionGrid, centers_to_faces,
                     spectral_cell_measures, xnodes, ynodes, znodes, xfaces, yfaces, zfaces,
                     xspacings, yspacings, zspacings, PERIODIC, BOUNDED, FLAT)
-from .model import (SpectralWaveModel, InfiniteDepth, Clock, Simulation, WENO, set_, time_step_, run_,
+from .model import (SpectralWaveModel, InfiniteDepth, HorizontalField, Clock, Simulation, WENO, set_, time_step_, run_,
                     interior, PseudomomentumVelocities, SpatialAveraging, automatic_q_grid, update_coupling_,
                     compute_pseudomomentum_cell_averages, compute_pseudomomentum_cell_integrals,
                     pseudomomentum_fields, compute_pseudomomentum_tendency_cell_averages,
@@ -33,7 +33,7 @@ __all__ = [
     "RectilinearGrid", "PolarWaveVectorGrid", "FrequencyDirectionGrid", "centers_to_faces",
     "spectral_cell_measures", "xnodes", "ynodes", "znodes", "xfaces", "yfaces", "zfaces",
     "xspacings", "yspacings", "zspacings", "PERIODIC", "BOUNDED", "FLAT",
-    "SpectralWaveModel", "InfiniteDepth", "Clock", "Simulation", "WENO", "set_", "time_step_", "run_",
+    "SpectralWaveModel", "InfiniteDepth", "HorizontalField", "Clock", "Simulation", "WENO", "set_", "time_step_", "run_",
     "interior", "PseudomomentumVelocities", "SpatialAveraging", "automatic_q_grid", "update_coupling_",
     "compute_pseudomomentum_cell_averages", "compute_pseudomomentum_cell_integrals", "pseudomomentum_fields",
     "compute_pseudomomentum_tendency_cell_averages", "pseudomomentum_tendency_fields", "cwcm_momentum_tendency_fields",

@@ -82,6 +82,7 @@ SIGNATURES = {
     "rpl_create": ([ctypes.POINTER(_P), ctypes.POINTER(rpl_config)], ctypes.c_int),
     "rpl_destroy": ([_P], ctypes.c_int),
     "rpl_set_geometry": ([_P, _D, _D, _DP, _DP, _DP, _DP, _I32, _D, _D], ctypes.c_int),
+    "rpl_set_depth_field": ([_P, _DP], ctypes.c_int),
     "rpl_set_current": ([_P, _I32, _DP, _DP, _DP, _D], ctypes.c_int),
     "rpl_set_doppler": ([_P, _DP, _DP], ctypes.c_int),
     "rpl_get_doppler": ([_P, _DP, _DP], ctypes.c_int),

@@ -22,6 +22,17 @@ class InfiniteDepth:
         return "InfiniteDepth()"
 
 
+class HorizontalField:
+    """Values on the horizontal wave grid, (Nx, Ny): the stand-in for the
+    Oceananigans Field a reference user passes as a spatially varying
+    `depth` (lagrangian_velocities.jl:128-147)."""
+
+    def __init__(self, values):
+        self.data = np.array(values, dtype=np.float64)
+        if self.data.ndim != 2:
+            raise ValueError("a HorizontalField holds an (Nx, Ny) array")
+
+
 WENO = "WENO"
 _TIMESTEPPERS = {"RK3": _lib.RPL_RK3, "LSRK3": _lib.RPL_LSRK3, "LowStorageRK3": _lib.RPL_LSRK3,
                  "ForwardEuler": _lib.RPL_FORWARD_EULER}
@@ -136,9 +147,33 @@ class SpectralWaveModel:
         if any(t not in (PERIODIC, BOUNDED) for t in grid.topology[:2]):
             raise NotImplementedError("x and y must be Periodic or Bounded")
         self.grid, self.spectral_grid = grid, spectral_grid
-        self.depth = InfiniteDepth() if depth is None else depth
-        if not isinstance(self.depth, InfiniteDepth) and not float(self.depth) > 0:
-            raise ValueError("depth must be positive")
+        # lagrangian_velocities.jl:174-189 (validate_model_depth): InfiniteDepth,
+        # a positive number, a function of (x, y) or a Field on the horizontal
+        # grid; a function or Field is a depth per column
+        self.depth_field = None
+        if callable(depth) and not isinstance(depth, InfiniteDepth):
+            X, Y = np.meshgrid(xnodes(grid), ynodes(grid), indexing="ij")
+            try:
+                vals = np.broadcast_to(np.asarray(depth(X, Y), dtype=np.float64), (grid.Nx, grid.Ny))
+            except Exception:  # a function of scalars only
+                vals = np.array([[float(depth(x, y)) for y in ynodes(grid)] for x in xnodes(grid)])
+            self.depth_field = np.array(vals, dtype=np.float64)
+        elif isinstance(depth, HorizontalField):
+            if depth.data.shape != (grid.Nx, grid.Ny):
+                raise _lib.RippleArgumentError(-1, f"model `depth` Field must have horizontal size "
+                                                   f"{(grid.Nx, grid.Ny)}")
+            self.depth_field = depth.data.copy()
+        elif depth is not None and not isinstance(depth, InfiniteDepth) and np.ndim(depth) != 0:
+            raise _lib.RippleArgumentError(-1, "model `depth` must be InfiniteDepth(), a positive number, a function, "
+                                               "or a Field on the horizontal wave grid; raw arrays are not accepted")
+        if self.depth_field is not None:
+            if not (self.depth_field > 0).all():
+                raise _lib.RippleArgumentError(-1, "model `depth` values must be positive")
+            self.depth = HorizontalField(self.depth_field)
+        else:
+            self.depth = InfiniteDepth() if depth is None else depth
+            if not isinstance(self.depth, InfiniteDepth) and not float(self.depth) > 0:
+                raise ValueError("depth must be positive")
         self.horizontal_advection = horizontal_advection
         self.spectral_advection = spectral_advection
         self.sources = sources
@@ -152,6 +187,9 @@ class SpectralWaveModel:
 
         has_coupling = velocities is not None
         self.pseudomomentum = isinstance(velocities, PseudomomentumVelocities)
+        if self.pseudomomentum and self.depth_field is not None:
+            raise NotImplementedError("the pseudomomentum coupling takes a constant depth (the overlap tables of a "
+                                      "depth per column, pseudomomentum.jl:72-87, are not built)")
         if has_coupling and not isinstance(spectral_grid, PolarWaveVectorGrid):
             raise ValueError("CWCM current coupling requires a PolarWaveVectorGrid")
         # Models/tendencies.jl:18-109.
@@ -176,13 +214,19 @@ class SpectralWaveModel:
         lib, ctx = self._ctx.lib, self._ctx.ptr
         sg = spectral_grid
         depth_kind = _lib.RPL_DEPTH_INFINITE if isinstance(self.depth, InfiniteDepth) else _lib.RPL_DEPTH_CONSTANT
+        # a depth field replaces the constant depth's tables (rpl_set_depth_field below)
+        geometry_depth = float(self.depth_field.mean()) if self.depth_field is not None else self.depth
         arrays = [np.ascontiguousarray(a, dtype=np.float64) for a in
                   (sg.kappa, sg.kappa_faces, sg.phi, sg.phi_faces)]
         _lib.check(lib.rpl_set_geometry(ctx, float(grid.x_faces[1] - grid.x_faces[0]),
                                         float(grid.y_faces[1] - grid.y_faces[0]),
                                         *[_lib.dptr(a) for a in arrays], depth_kind,
-                                        0.0 if depth_kind == _lib.RPL_DEPTH_INFINITE else float(self.depth),
+                                        0.0 if depth_kind == _lib.RPL_DEPTH_INFINITE else float(geometry_depth),
                                         float(gravity)))
+        if self.depth_field is not None:
+            # transport.jl:102-139: the group velocity of every column at its depth
+            df = np.ascontiguousarray(self.depth_field.ravel(order="F"))
+            _lib.check(lib.rpl_set_depth_field(ctx, _lib.dptr(df)))
         self.velocities = None
         self.q_grid = None
         if self.pseudomomentum:
@@ -219,7 +263,10 @@ class SpectralWaveModel:
             fields.append(np.asfortranarray(a))
         # Coupling/lagrangian_velocities.jl:113-115: Q depth is the column depth
         # for InfiniteDepth, else the model depth.
-        q_depth = g.Lz if isinstance(self.depth, InfiniteDepth) else float(self.depth)
+        # (a depth field is each column's Q depth: the library takes it from
+        # rpl_set_depth_field and ignores this value)
+        q_depth = g.Lz if isinstance(self.depth, InfiniteDepth) else (
+            float(self.depth_field.mean()) if self.depth_field is not None else float(self.depth))
         zf = np.ascontiguousarray(g.z_faces, dtype=np.float64)
         uf = np.ascontiguousarray(fields[0].ravel(order="F"))
         vf = np.ascontiguousarray(fields[1].ravel(order="F"))
@@ -230,6 +277,8 @@ class SpectralWaveModel:
     def _q_depth(self, q_grid):
         # lagrangian_velocities.jl:113-115: the Q grid's column depth for
         # InfiniteDepth, else the model depth.
+        if self.depth_field is not None:
+            raise NotImplementedError("the pseudomomentum coupling and fields take a constant depth")
         return q_grid.Lz if isinstance(self.depth, InfiniteDepth) else float(self.depth)
 
     def _set_pseudomomentum(self, pv):

@@ -1,0 +1,93 @@
+"""Pins the oracle's spatially varying depth (a depth per column: the
+reference's is_spatially_varying_depth, transport.jl:102-139, q_transform.jl:
+124-125) (CPU only): every column's group velocity is the constant-depth
+table at that column's depth and tends to the deep- and shallow-water
+limits; every column's Q weights and Doppler contraction are those of a
+constant depth; the transport reads each cell's own velocity (a field that
+equals one value everywhere reproduces the constant table's tendency cell by
+cell)."""
+import math
+
+import numpy as np
+import pytest
+
+from ripple_amd import workloads
+
+
+def _depth(nx, ny, lo=0.5, hi=60.0, seed=1):
+    rng = np.random.default_rng(seed)
+    return rng.uniform(lo, hi, (nx, ny))
+
+
+def test_group_velocity_field_per_column(oracle):
+    wl = workloads.vortex_refraction(nsteps=1)
+    sg = wl.spectral_grid
+    d = _depth(5, 4)
+    d[0, 0], d[1, 0] = 1e4, 1e-4  # deep and very shallow columns
+    cgx, cgy = oracle.group_velocity_field(sg.kappa, sg.phi, d)
+    assert cgx.shape == (5, 4, len(sg.kappa), len(sg.phi))
+    for i, j in [(2, 3), (4, 1), (0, 0), (1, 0)]:
+        t = oracle.geometry(sg.kappa, sg.kappa_faces, sg.phi, sg.phi_faces, True, d[i, j])
+        assert np.array_equal(cgx[i, j], t["cgx"]) and np.array_equal(cgy[i, j], t["cgy"])
+    k = np.asarray(sg.kappa)[:, None]
+    phi = np.asarray(sg.phi)[None, :]
+    deep = 0.5 * np.sqrt(9.81 / k)
+    np.testing.assert_allclose(cgx[0, 0], deep * np.cos(phi), rtol=1e-12, atol=1e-15)
+    np.testing.assert_allclose(cgy[0, 0], deep * np.sin(phi), rtol=1e-12, atol=1e-15)
+    shallow = math.sqrt(9.81 * 1e-4)
+    np.testing.assert_allclose(np.hypot(cgx[1, 0], cgy[1, 0]), shallow, rtol=1e-6)
+
+
+def test_q_weights_and_doppler_per_column(oracle):
+    rng = np.random.default_rng(3)
+    nx, ny, nz = 4, 3, 6
+    zf = np.linspace(-8.0, 0.0, nz + 1)
+    kappa = np.array([0.05, 0.3, 1.1])
+    d = _depth(nx, ny, 2.0, 8.0)
+    Wc = oracle.q_weights_columns(zf, kappa, d)
+    for i in range(nx):
+        for j in range(ny):
+            assert np.array_equal(Wc[i, j], oracle.q_weights(zf, kappa, d[i, j]))
+    u, v = rng.standard_normal((nx, ny, nz)), rng.standard_normal((nx, ny, nz))
+    Ux, Uy = oracle.doppler_columns(u, v, Wc)
+    np.testing.assert_allclose(Ux, np.einsum("ijk,ijkm->ijm", u, Wc), rtol=1e-14, atol=1e-15)
+    np.testing.assert_allclose(Uy, np.einsum("ijk,ijkm->ijm", v, Wc), rtol=1e-14, atol=1e-15)
+    same = np.full((nx, ny), 5.0)
+    W5 = oracle.q_weights(zf, kappa, 5.0)
+    a, b = oracle.doppler_columns(u, v, oracle.q_weights_columns(zf, kappa, same))
+    ref = oracle.doppler(u, v, W5)
+    assert np.array_equal(a, ref[0]) and np.array_equal(b, ref[1])
+
+
+@pytest.mark.parametrize("mode", ["refraction", "intrinsic"])
+def test_transport_reads_each_cells_velocity(oracle, mode):
+    """The reference uses the cell-centre velocity on both faces
+    (wave_current_refraction.jl:97-98, intrinsic_transport.jl:22-23), so the
+    tendency of a cell depends on its own c_g only: with a depth field, the
+    tendency at a cell equals the tendency of a constant depth equal to that
+    cell's depth, at that cell."""
+    wl = workloads.vortex_refraction(nsteps=1)
+    nx, ny = wl.N0.shape[:2]
+    d = _depth(nx, ny, 0.3, 3.0, seed=7)
+    transport = oracle.MODE_REFRACTION if mode == "refraction" else oracle.MODE_INTRINSIC
+    kw = dict(transport=transport, timestepper="RK3")
+    # a spatially varying action, so that transport does something
+    x = np.arange(nx)[:, None, None, None]
+    y = np.arange(ny)[None, :, None, None]
+    N0 = np.asfortranarray(wl.N0 * (1.2 + np.sin(2 * np.pi * x / nx) * np.cos(4 * np.pi * y / ny)))
+    wl = workloads.Workload(wl.name, wl.grid, wl.spectral_grid, wl.velocities if mode == "refraction" else None,
+                            N0, wl.dt, 1, dict(wl.model_kwargs))
+    G = oracle.OracleModel(wl, depth=d, **kw).tendency()
+    if mode == "refraction":
+        # the Doppler velocity also varies with the column's Q depth; compare
+        # against the constant-depth model whose Doppler field is overridden
+        # by the field model's
+        fm = oracle.OracleModel(wl, depth=d, **kw)
+        U = fm.doppler_velocity()
+    for (i, j) in [(5, 7), (33, 2), (60, 60)]:
+        kw2 = dict(kw)
+        if mode == "refraction":
+            kw2["doppler_override"] = U
+        Gc = oracle.OracleModel(wl, depth=float(d[i, j]), **kw2).tendency()
+        assert np.array_equal(G[i, j], Gc[i, j])
+    assert not np.array_equal(G, oracle.OracleModel(wl, depth=float(d.mean()), **kw).tendency())
