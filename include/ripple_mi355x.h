@@ -104,8 +104,11 @@ int rpl_set_geometry(rpl_ctx *ctx, double dx, double dy,
  * q_projection_depth, lagrangian_velocities.jl:192-193).  Call after
  * rpl_set_geometry and before rpl_set_current (a current set before is
  * dropped); NULL returns to rpl_set_geometry's depth.  Non-positive values:
- * RPL_ERR_ARG (lagrangian_velocities.jl:123-124).  The pseudomomentum
- * coupling and ensembles take a constant depth (RPL_ERR_UNSUPPORTED). */
+ * RPL_ERR_ARG (lagrangian_velocities.jl:123-124).  A pseudomomentum coupling
+ * set afterwards builds one pair of overlap tables per column
+ * (pseudomomentum.jl:72-87), and rpl_pseudomomentum /
+ * rpl_pseudomomentum_tendency with q_depth = 0 take each column's depth.
+ * Ensembles take a constant depth (RPL_ERR_UNSUPPORTED). */
 int rpl_set_depth_field(rpl_ctx *ctx, const double *depth);
 
 /* Prescribed Lagrangian-mean current (Coupling/current_coupling.jl:73-91,

@@ -188,25 +188,42 @@ def pm_overlap(z_faces, kappa, depth):
     return O.reshape((nk, nk), order="F"), dO.reshape((nk, nk), order="F")
 
 
+def pm_overlap_columns(z_faces, kappa, depth):
+    """The overlap tables of every column at its depth (an (Nx, Ny) array),
+    each (Nx, Ny, Nk, Nk) [i, j, target, source]."""
+    d = np.asarray(depth, dtype=np.float64)
+    nx, ny = d.shape
+    zf, kc, df = _c(z_faces), _c(kappa), _c(d)
+    nk = len(kc)
+    O, dO = np.zeros(nx * ny * nk * nk), np.zeros(nx * ny * nk * nk)
+    lib().orc_pm_overlap_columns(_I(nx * ny), _I(len(zf) - 1), _p(zf), _I(nk), _p(kc), _p(df), _p(O), _p(dO))
+    shape = (nx, ny, nk, nk)
+    return O.reshape(shape, order="F"), dO.reshape(shape, order="F")
+
+
 def pm_doppler(N, O, dO, kx, ky, dtype=np.float64):
-    """Ux, Uy, dUx/dkappa, dUy/dkappa (each (Nx, Ny, Nk)) of the pseudomomentum coupling."""
+    """Ux, Uy, dUx/dkappa, dUy/dkappa (each (Nx, Ny, Nk)) of the pseudomomentum
+    coupling; O, dO one (Nk, Nk) table or one per column (Nx, Ny, Nk, Nk)."""
     nx, ny, nk, npf = N.shape
     Nf, Of, dOf, kxf, kyf = (_c(a, dtype) for a in (N, O, dO, kx, ky))
     out = [np.zeros(nx * ny * nk, dtype=dtype) for _ in range(4)]
     sfx = "f64" if np.dtype(dtype) == np.float64 else "f32"
+    lib().orc_set_pm_columns(_I(int(np.ndim(O) == 4)))
     getattr(lib(), f"orc_pm_doppler_{sfx}")(_I(nx), _I(ny), _I(nk), _I(npf), _p(Nf), _p(Of), _p(dOf), _p(kxf),
                                             _p(kyf), *[_p(a) for a in out])
     return [a.reshape((nx, ny, nk), order="F") for a in out]
 
 
 def pm_cells(N, W, z_faces, kx, ky, cell_average=True, dtype=np.float64):
-    """Pseudomomentum cell averages (or integrals) px, py, each (Nx, Ny, Nz)."""
+    """Pseudomomentum cell averages (or integrals) px, py, each (Nx, Ny, Nz);
+    W one (Nz, Nk) table or one per column (Nx, Ny, Nz, Nk)."""
     nx, ny, nk, npf = N.shape
-    nz = W.shape[0]
+    nz = W.shape[2] if np.ndim(W) == 4 else W.shape[0]
     Nf, Wf, kxf, kyf = (_c(a, dtype) for a in (N, W, kx, ky))
     zf = _c(z_faces)
     px, py = np.zeros(nx * ny * nz, dtype=dtype), np.zeros(nx * ny * nz, dtype=dtype)
     sfx = "f64" if np.dtype(dtype) == np.float64 else "f32"
+    lib().orc_set_pm_columns(_I(int(np.ndim(W) == 4)))
     getattr(lib(), f"orc_pm_cells_{sfx}")(_I(nx), _I(ny), _I(nk), _I(npf), _I(nz), _p(Nf), _p(Wf), _p(zf), _p(kxf),
                                           _p(kyf), _I(int(cell_average)), _p(px), _p(py))
     return px.reshape((nx, ny, nz), order="F"), py.reshape((nx, ny, nz), order="F")
@@ -303,11 +320,15 @@ class OracleModel:
         # Self-coupled pseudomomentum current: pseudomomentum=(z_faces, q_depth)
         # (Coupling/current_coupling.jl:93-121); U follows every stage input.
         self.pm = None
+        self._pm_cols = False
         if pseudomomentum is not None:
-            if field:
-                raise NotImplementedError("the oracle's pseudomomentum coupling takes a constant depth")
             zf, q_depth = pseudomomentum
-            O, _ = pm_overlap(zf, sg.kappa, q_depth)
+            if field:
+                # pseudomomentum.jl:72-87: one overlap table per column
+                O, _ = pm_overlap_columns(zf, sg.kappa, depth)
+                self._pm_cols = True
+            else:
+                O, _ = pm_overlap(zf, sg.kappa, q_depth)
             self.pm = [_c(O, t), _c(tabs["mu_x"], t), _c(tabs["mu_y"], t)]
             self.mode = MODE_PSEUDOMOMENTUM
             self.Ux, self.Uy = np.zeros(nx * ny * nk, dtype=t), np.zeros(nx * ny * nk, dtype=t)
@@ -343,6 +364,7 @@ class OracleModel:
     def _step(self, dt, nsteps=1, hook=None, user=None):
         L = lib()
         L.orc_set_cg_cells(_I(int(self._cg_cells)))
+        L.orc_set_pm_columns(_I(int(self._pm_cols)))
         nx, ny, nk, npf = self.size
         sfx = "f64" if self.dtype == np.float64 else "f32"
         fn = getattr(L, f"orc_run_{sfx}")
@@ -358,6 +380,7 @@ class OracleModel:
     def tendency(self):
         L = lib()
         L.orc_set_cg_cells(_I(int(self._cg_cells)))
+        L.orc_set_pm_columns(_I(int(self._pm_cols)))
         nx, ny, nk, npf = self.size
         sfx = "f64" if self.dtype == np.float64 else "f32"
         G = np.zeros_like(self.N)

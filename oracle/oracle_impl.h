@@ -244,11 +244,13 @@ void SFX(orc_pm_doppler)(int nx, int ny, int nk, int nphi, const REAL *N, const 
                 for (int s = 0; s < nk; ++s) {
                     REAL a = N[IDX4(i, j, s, n)];
                     size_t st = (size_t)s + (size_t)nk * n;
-                    REAL q = O[(size_t)t + (size_t)nk * s];
+                    size_t ot = g_pm_cols ? IDX3(i, j, 0) + (size_t)nx * ny * ((size_t)t + (size_t)nk * s)
+                                          : (size_t)t + (size_t)nk * s;
+                    REAL q = O[ot];
                     ax += q * a * kx[st];
                     ay += q * a * ky[st];
                     if (dO) {
-                        REAL dq = dO[(size_t)t + (size_t)nk * s];
+                        REAL dq = dO[ot];
                         dax += dq * a * kx[st];
                         day += dq * a * ky[st];
                     }
@@ -278,7 +280,8 @@ void SFX(orc_pm_cells)(int nx, int ny, int nk, int nphi, int nz, const REAL *N, 
             REAL ax = 0, ay = 0;
             for (int n = 0; n < nphi; ++n)
                 for (int m = 0; m < nk; ++m) {
-                    REAL q = W[(size_t)k + (size_t)nz * m];
+                    REAL q = g_pm_cols ? W[IDX3(i, j, 0) + (size_t)nx * ny * ((size_t)k + (size_t)nz * m)]
+                                       : W[(size_t)k + (size_t)nz * m];
                     REAL a = N[IDX4(i, j, m, n)];
                     size_t mn = (size_t)m + (size_t)nk * n;
                     ax += q * a * kx[mn];

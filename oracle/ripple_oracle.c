@@ -45,6 +45,17 @@ int orc_set_cg_cells(int on)
     return 0;
 }
 
+/* Pseudomomentum overlap tables and Q weights per column (a spatially
+ * varying depth: pseudomomentum.jl:72-87 overlap[i, j, t, s], :356/:380
+ * q_depth_at(depth, i, j)) when set: O[ij + nx ny (t + nk s)] and
+ * W[ij + nx ny (k + nz m)]; one table for every column otherwise. */
+static int g_pm_cols = 0;
+int orc_set_pm_columns(int on)
+{
+    g_pm_cols = on != 0;
+    return 0;
+}
+
 #include "oracle_impl.h"
 #undef REAL
 #undef REAL_EPS
@@ -313,6 +324,30 @@ void orc_pm_overlap(int nz, const double *zf, int nk, const double *kappa, doubl
             O[(size_t)t + (size_t)nk * s] = total;
             dO[(size_t)t + (size_t)nk * s] = dtotal;
         }
+}
+
+/* pseudomomentum.jl:72-87: the overlap tables of every column at its own
+ * depth (pseudomomentum_overlap, :103-122), O[ij + ncol (t + nk s)]. */
+void orc_pm_overlap_columns(int ncol, int nz, const double *zf, int nk, const double *kappa, const double *depth,
+                            double *O, double *dO)
+{
+#pragma omp parallel for schedule(static)
+    for (long ij = 0; ij < (long)ncol; ++ij)
+        for (int s = 0; s < nk; ++s)
+            for (int t = 0; t < nk; ++t) {
+                double total = 0, dtotal = 0;
+                for (int k = 0; k < nz; ++k) {
+                    double z1 = zf[k], z2 = zf[k + 1];
+                    double dz = fabs(z2 - z1);
+                    double sw = orc_q_cell_integral(kappa[s], z1, z2, depth[ij]);
+                    double tw = orc_q_cell_integral(kappa[t], z1, z2, depth[ij]);
+                    double td = orc_q_cell_integral_dkappa(kappa[t], z1, z2, depth[ij]);
+                    total += tw * sw / dz;
+                    dtotal += td * sw / dz;
+                }
+                O[(size_t)ij + (size_t)ncol * ((size_t)t + (size_t)nk * s)] = total;
+                dO[(size_t)ij + (size_t)ncol * ((size_t)t + (size_t)nk * s)] = dtotal;
+            }
 }
 
 /* doppler_velocity.jl:45-62: U[i,j,m] = sum_k uL[i,j,k] * W[k,m], sequential

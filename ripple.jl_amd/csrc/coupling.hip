@@ -197,7 +197,7 @@ __global__ __launch_bounds__(256) void pm_ordered_kernel(size_t plane, int nk, i
                                                          const T *__restrict__ N, const T *__restrict__ A,
                                                          const T *__restrict__ B, const T *__restrict__ kx,
                                                          const T *__restrict__ ky, const T *__restrict__ scale, T *X,
-                                                         T *Y, T *dX, T *dY)
+                                                         T *Y, T *dX, T *dY, bool per_column)
 {
     const size_t ij = (size_t)blockIdx.x * 256 + threadIdx.x;
     const int o0 = blockIdx.y * PM_CHUNK;
@@ -216,11 +216,15 @@ __global__ __launch_bounds__(256) void pm_ordered_kernel(size_t plane, int nk, i
             for (int c = 0; c < PM_CHUNK; ++c) {
                 const int o = o0 + c;
                 if (o < nout) {
-                    const T q = A[o + (size_t)nout * s];
+                    // one matrix, or the column's own (a depth per column,
+                    // pseudomomentum.jl:72-87): A[ij + plane (o + nout s)]
+                    const size_t os = (size_t)o + (size_t)nout * s;
+                    const size_t ai = per_column ? ij + plane * os : os;
+                    const T q = A[ai];
                     ax[c] += q * a * kxv;
                     ay[c] += q * a * kyv;
                     if (B) {
-                        const T dq = B[o + (size_t)nout * s];
+                        const T dq = B[ai];
                         bx[c] += dq * a * kxv;
                         by[c] += dq * a * kyv;
                     }
@@ -433,14 +437,14 @@ template hipError_t launch_gradients<float>(int, int, int, const float *, const 
 template <typename T>
 hipError_t launch_pm_contract(bool ordered, int nx, int ny, int nk, int nphi, int nout, const T *N, const T *A,
                               const T *B, const T *kx, const T *ky, const T *scale, T *X, T *Y, T *dX, T *dY,
-                              hipStream_t s)
+                              hipStream_t s, bool per_column)
 {
     const size_t plane = (size_t)nx * ny;
     const size_t lds = pm_factored_lds(nk, nout, B != nullptr, sizeof(T));
-    if (ordered || lds > PM_MOMENT_LDS_MAX) {
+    if (ordered || per_column || lds > PM_MOMENT_LDS_MAX) {
         dim3 grid((unsigned)((plane + 255) / 256), (unsigned)((nout + PM_CHUNK - 1) / PM_CHUNK));
         hipLaunchKernelGGL(pm_ordered_kernel<T>, grid, dim3(256), 0, s, plane, nk, nphi, nout, N, A, B, kx, ky,
-                           scale, X, Y, dX, dY);
+                           scale, X, Y, dX, dY, per_column);
     } else {
         hipLaunchKernelGGL(pm_moment_kernel<T>, dim3((unsigned)((plane + 63) / 64)), dim3(256), lds, s, plane, nk,
                            nphi, nout, N, A, B, kx, ky, scale, X, Y, dX, dY);
@@ -450,10 +454,87 @@ hipError_t launch_pm_contract(bool ordered, int nx, int ny, int nk, int nphi, in
 
 template hipError_t launch_pm_contract<float>(bool, int, int, int, int, int, const float *, const float *,
                                               const float *, const float *, const float *, const float *, float *,
-                                              float *, float *, float *, hipStream_t);
+                                              float *, float *, float *, hipStream_t, bool);
 template hipError_t launch_pm_contract<double>(bool, int, int, int, int, int, const double *, const double *,
                                                const double *, const double *, const double *, const double *,
-                                               double *, double *, double *, double *, hipStream_t);
+                                               double *, double *, double *, double *, hipStream_t, bool);
+
+// pseudomomentum.jl:72-122 with a depth per column: the overlap tables of
+// column ij at its own depth, O[ij + plane (t + nk s)] = sum over z cells of
+// W_t W_s / |dz| and dO with dW_t/dkappa, ascending in z as the reference's
+// pseudomomentum_overlap; the weights in FP64 (q_cdf_dev).
+template <typename T>
+__global__ __launch_bounds__(256) void pm_overlap_columns_kernel(size_t plane, int nz, int nk,
+                                                                 const double *__restrict__ zf,
+                                                                 const double *__restrict__ kc,
+                                                                 const double *__restrict__ depth, T *O, T *dO)
+{
+    const size_t ij = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const int ts = blockIdx.y, t = ts % nk, src = ts / nk;
+    if (ij >= plane)
+        return;
+    const double d = depth[ij];
+    double total = 0, dtotal = 0;
+    for (int k = 0; k < nz; ++k) {
+        const double z1 = zf[k], z2 = zf[k + 1];
+        const double hi = z1 < z2 ? z2 : z1, lo = z1 < z2 ? z1 : z2, dz = fabs(z2 - z1);
+        const double sw = q_cdf_dev(kc[src], hi, d) - q_cdf_dev(kc[src], lo, d);
+        const double tw = q_cdf_dev(kc[t], hi, d) - q_cdf_dev(kc[t], lo, d);
+        const double td = q_cdf_dkappa_dev(kc[t], hi, d) - q_cdf_dkappa_dev(kc[t], lo, d);
+        total += tw * sw / dz;
+        dtotal += td * sw / dz;
+    }
+    O[ij + plane * (size_t)ts] = (T)total;
+    dO[ij + plane * (size_t)ts] = (T)dtotal;
+}
+
+// The Q weights of every column at its own depth, W[ij + plane (k + nz m)]
+// (q_depth_at in the pseudomomentum cells, pseudomomentum.jl:375-400).
+template <typename T>
+__global__ __launch_bounds__(256) void q_weights_columns_kernel(size_t plane, int nz, int nk,
+                                                                const double *__restrict__ zf,
+                                                                const double *__restrict__ kc,
+                                                                const double *__restrict__ depth, T *W)
+{
+    const size_t ij = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const int km = blockIdx.y, k = km % nz, m = km / nz;
+    if (ij >= plane)
+        return;
+    const double z1 = zf[k], z2 = zf[k + 1];
+    const double hi = z1 < z2 ? z2 : z1, lo = z1 < z2 ? z1 : z2;
+    W[ij + plane * (size_t)km] = (T)(q_cdf_dev(kc[m], hi, depth[ij]) - q_cdf_dev(kc[m], lo, depth[ij]));
+}
+
+template <typename T>
+hipError_t launch_pm_overlap_columns(size_t plane, int nz, int nk, const double *zf, const double *kc,
+                                     const double *depth, T *O, T *dO, hipStream_t s)
+{
+    if ((long long)nk * nk > 65535)
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL(pm_overlap_columns_kernel<T>, dim3((unsigned)((plane + 255) / 256), (unsigned)(nk * nk)),
+                       dim3(256), 0, s, plane, nz, nk, zf, kc, depth, O, dO);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_q_weights_columns(size_t plane, int nz, int nk, const double *zf, const double *kc,
+                                    const double *depth, T *W, hipStream_t s)
+{
+    if ((long long)nz * nk > 65535)
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL(q_weights_columns_kernel<T>, dim3((unsigned)((plane + 255) / 256), (unsigned)(nz * nk)),
+                       dim3(256), 0, s, plane, nz, nk, zf, kc, depth, W);
+    return hipGetLastError();
+}
+
+template hipError_t launch_pm_overlap_columns<float>(size_t, int, int, const double *, const double *,
+                                                     const double *, float *, float *, hipStream_t);
+template hipError_t launch_pm_overlap_columns<double>(size_t, int, int, const double *, const double *,
+                                                      const double *, double *, double *, hipStream_t);
+template hipError_t launch_q_weights_columns<float>(size_t, int, int, const double *, const double *,
+                                                    const double *, float *, hipStream_t);
+template hipError_t launch_q_weights_columns<double>(size_t, int, int, const double *, const double *,
+                                                     const double *, double *, hipStream_t);
 
 template <typename T>
 hipError_t launch_action_tendency(size_t n, const T *Nnew, const T *Nold, T dt, T *out, hipStream_t s)

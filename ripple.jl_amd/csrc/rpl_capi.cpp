@@ -88,6 +88,8 @@ struct rpl_ctx {
     // speed per (column, kappa bin) and the bins' direction tables (kx, ky, k
     // for the reference-order kernel; kx / k, ky / k for the phi walk)
     bool has_depth_field = false;
+    bool pm_percol = false;      // the pseudomomentum overlap tables are per column
+    size_t pm_O_elems = 0;       // the allocated size of pm_O and pm_dO
     double gravity = 9.81;  // rpl_set_geometry's
     std::vector<double> h_depth;
     void *cgm = nullptr, *cg_vec = nullptr, *cg_dir = nullptr;
@@ -632,7 +634,7 @@ int launch_coupling(rpl_ctx *c, const void *Nk)
                                             (const double *)c->pm_O,
                                             (const double *)c->pm_dO, (const double *)c->pm_kx,
                                             (const double *)c->pm_ky, nullptr, (double *)c->Ux, (double *)c->Uy,
-                                            (double *)c->dUx, (double *)c->dUy, c->stream);
+                                            (double *)c->dUx, (double *)c->dUy, c->stream, c->pm_percol);
         if (e == hipSuccess && !use_fast(c))
             e = rpl::launch_gradients<double>(nx, ny, nk, (const double *)c->Ux, (const double *)c->Uy, 1.0 / c->dx,
                                               1.0 / c->dy, (double *)c->Uxx, (double *)c->Uxy, (double *)c->Uyx,
@@ -640,7 +642,7 @@ int launch_coupling(rpl_ctx *c, const void *Nk)
     } else {
         e = rpl::launch_pm_contract<float>(ordered, nx, ny, nk, np, nk, (const float *)Nk, (const float *)c->pm_O, (const float *)c->pm_dO, (const float *)c->pm_kx,
                                            (const float *)c->pm_ky, nullptr, (float *)c->Ux, (float *)c->Uy,
-                                           (float *)c->dUx, (float *)c->dUy, c->stream);
+                                           (float *)c->dUx, (float *)c->dUy, c->stream, c->pm_percol);
         if (e == hipSuccess && !use_fast(c))
             e = rpl::launch_gradients<float>(nx, ny, nk, (const float *)c->Ux, (const float *)c->Uy,
                                              1.0f / (float)c->dx, 1.0f / (float)c->dy, (float *)c->Uxx,
@@ -1668,12 +1670,9 @@ int rpl_set_pseudomomentum_coupling(rpl_ctx *c, int32_t nz, const double *z_face
         return fail(RPL_ERR_STATE, "rpl_set_geometry has not been called");
     if (c->cfg.transport != RPL_TRANSPORT_REFRACTION)
         return fail(RPL_ERR_STATE, "the pseudomomentum coupling drives the refraction transport (RPL_TRANSPORT_REFRACTION)");
-    if (c->has_depth_field)
-        return fail(RPL_ERR_UNSUPPORTED, "the pseudomomentum coupling takes a constant depth (the overlap tables "
-                                         "of a depth per column are not built)");
     if (nz < 1)
         return fail(RPL_ERR_ARG, "nz must be >= 1");
-    if (!(q_depth > 0))
+    if (!c->has_depth_field && !(q_depth > 0))
         return fail(RPL_ERR_ARG, "depth must be positive");  // q_kernel.jl:19
     for (int k = 0; k < nz; ++k)
         if (!(z_faces[k] != z_faces[k + 1]))
@@ -1686,8 +1685,9 @@ int rpl_set_pseudomomentum_coupling(rpl_ctx *c, int32_t nz, const double *z_face
             k = (float)k;
     // pseudomomentum.jl:89-122: overlap O[t,s] = sum_k W_t W_s / |dz| and its
     // derivative in the target kappa, target fastest.
-    std::vector<double> O((size_t)nk * nk), dO((size_t)nk * nk);
-    for (int src = 0; src < nk; ++src)
+    const size_t o_elems = (c->has_depth_field ? c->plane : 1) * (size_t)nk * nk;
+    std::vector<double> O(c->has_depth_field ? 0 : (size_t)nk * nk), dO(O.size());
+    for (int src = 0; src < nk && !c->has_depth_field; ++src)
         for (int t = 0; t < nk; ++t) {
             double total = 0, dtotal = 0;
             for (int k = 0; k < nz; ++k) {
@@ -1705,15 +1705,55 @@ int rpl_set_pseudomomentum_coupling(rpl_ctx *c, int32_t nz, const double *z_face
     int rc;
     if ((rc = ensure_dU(c)))
         return rc;
+    if (c->pm_O_elems < o_elems) {  // per-column tables need more room
+        free_ptr(c->pm_O);
+        free_ptr(c->pm_dO);
+        c->pm_O = c->pm_dO = nullptr;
+        c->pm_O_elems = 0;
+    }
     for (void **b : {&c->pm_O, &c->pm_dO})
-        if (!*b && (rc = alloc(b, (size_t)nk * nk * c->esize)))
+        if (!*b && (rc = alloc(b, o_elems * c->esize)))
             return rc;
+    c->pm_O_elems = std::max(c->pm_O_elems, o_elems);
     for (void **b : {&c->pm_kx, &c->pm_ky})
         if (!*b && (rc = alloc(b, tab * c->esize)))
             return rc;
-    if ((rc = upload_real(c, O.data(), c->pm_O, O.size())) || (rc = upload_real(c, dO.data(), c->pm_dO, dO.size())) ||
-        (rc = upload_real(c, c->h_mux.data(), c->pm_kx, tab)) || (rc = upload_real(c, c->h_muy.data(), c->pm_ky, tab)))
+    c->pm = false;  // until the tables are on the device
+    if ((rc = upload_real(c, c->h_mux.data(), c->pm_kx, tab)) || (rc = upload_real(c, c->h_muy.data(), c->pm_ky, tab)))
         return rc;
+    if (c->has_depth_field) {
+        // pseudomomentum.jl:72-87: the tables of every column at its depth
+        void *dzf = nullptr, *dkc = nullptr, *ddep = nullptr;
+        if ((rc = alloc(&dzf, (nz + 1) * sizeof(double))) || (rc = alloc(&dkc, nk * sizeof(double))) ||
+            (rc = alloc(&ddep, c->plane * sizeof(double)))) {
+            free_ptr(dzf);
+            free_ptr(dkc);
+            return rc;
+        }
+        hipError_t e = hipMemcpy(dzf, z_faces, (nz + 1) * sizeof(double), hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(dkc, kc.data(), nk * sizeof(double), hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(ddep, c->h_depth.data(), c->plane * sizeof(double), hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = is_f64(c) ? rpl::launch_pm_overlap_columns<double>(c->plane, nz, nk, (const double *)dzf,
+                                                                   (const double *)dkc, (const double *)ddep,
+                                                                   (double *)c->pm_O, (double *)c->pm_dO, c->stream)
+                          : rpl::launch_pm_overlap_columns<float>(c->plane, nz, nk, (const double *)dzf,
+                                                                  (const double *)dkc, (const double *)ddep,
+                                                                  (float *)c->pm_O, (float *)c->pm_dO, c->stream);
+        if (e == hipSuccess)
+            e = hipStreamSynchronize(c->stream);
+        free_ptr(dzf);
+        free_ptr(dkc);
+        free_ptr(ddep);
+        if (e != hipSuccess)
+            return fail(RPL_ERR_HIP, std::string("pseudomomentum overlap tables: ") + hipGetErrorString(e));
+    } else if ((rc = upload_real(c, O.data(), c->pm_O, O.size())) ||
+               (rc = upload_real(c, dO.data(), c->pm_dO, dO.size()))) {
+        return rc;
+    }
+    c->pm_percol = c->has_depth_field;
     // A new coupling holds zero velocities until its first update
     // (current_coupling.jl:104-109).
     for (void *b : {c->Ux, c->Uy, c->dUx, c->dUy})
@@ -1863,7 +1903,9 @@ static int pseudomomentum_cells(rpl_ctx *c, const void *Nsrc, int32_t nz, const 
         return fail(RPL_ERR_STATE, "rpl_set_geometry has not been called");
     if (nz < 1)
         return fail(RPL_ERR_ARG, "nz must be >= 1");
-    if (!(q_depth > 0))
+    // q_depth = 0 with a depth field: every column at its own depth
+    const bool percol = q_depth == 0 && c->has_depth_field;
+    if (!percol && !(q_depth > 0))
         return fail(RPL_ERR_ARG, "depth must be positive");
     for (int k = 0; k < nz; ++k)
         if (!(z_faces[k] != z_faces[k + 1]))
@@ -1875,8 +1917,8 @@ static int pseudomomentum_cells(rpl_ctx *c, const void *Nsrc, int32_t nz, const 
         for (double &k : kc)
             k = (float)k;
     // pseudomomentum.jl:375-400: W[k,m] (k fastest) and 1/|dz_k|.
-    std::vector<double> W((size_t)nz * nk), sc(nz);
-    for (int m = 0; m < nk; ++m)
+    std::vector<double> W(percol ? 0 : (size_t)nz * nk), sc(nz);
+    for (int m = 0; m < nk && !percol; ++m)
         for (int k = 0; k < nz; ++k)
             W[k + (size_t)nz * m] = q_cell(kc[m], z_faces[k], z_faces[k + 1], q_depth);
     for (int k = 0; k < nz; ++k)
@@ -1884,13 +1926,45 @@ static int pseudomomentum_cells(rpl_ctx *c, const void *Nsrc, int32_t nz, const 
     const size_t tab = (size_t)nk * np, out = c->plane * nz;
     void *dW = nullptr, *dS = nullptr, *dkx = nullptr, *dky = nullptr, *dpx = nullptr, *dpy = nullptr;
     int rc = RPL_OK;
-    if ((rc = alloc(&dW, W.size() * c->esize)) || (rc = alloc(&dS, sc.size() * c->esize)) ||
+    if ((rc = alloc(&dW, (percol ? c->plane * nz * nk : W.size()) * c->esize)) ||
+        (rc = alloc(&dS, sc.size() * c->esize)) ||
         (rc = alloc(&dkx, tab * c->esize)) || (rc = alloc(&dky, tab * c->esize)) ||
         (rc = alloc(&dpx, out * c->esize)) || (rc = alloc(&dpy, out * c->esize)))
         goto done;
-    if ((rc = upload_real(c, W.data(), dW, W.size())) || (rc = upload_real(c, sc.data(), dS, sc.size())) ||
+    if ((!percol && (rc = upload_real(c, W.data(), dW, W.size()))) || (rc = upload_real(c, sc.data(), dS, sc.size())) ||
         (rc = upload_real(c, c->h_mux.data(), dkx, tab)) || (rc = upload_real(c, c->h_muy.data(), dky, tab)))
         goto done;
+    if (percol) {
+        // q_depth_at(depth, i, j) (pseudomomentum.jl:375-400): each column's weights
+        void *dzf = nullptr, *dkc = nullptr, *ddep = nullptr;
+        hipError_t e = hipMalloc(&dzf, (nz + 1) * sizeof(double));
+        if (e == hipSuccess)
+            e = hipMalloc(&dkc, nk * sizeof(double));
+        if (e == hipSuccess)
+            e = hipMalloc(&ddep, c->plane * sizeof(double));
+        if (e == hipSuccess)
+            e = hipMemcpy(dzf, z_faces, (nz + 1) * sizeof(double), hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(dkc, kc.data(), nk * sizeof(double), hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(ddep, c->h_depth.data(), c->plane * sizeof(double), hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = is_f64(c) ? rpl::launch_q_weights_columns<double>(c->plane, nz, nk, (const double *)dzf,
+                                                                  (const double *)dkc, (const double *)ddep,
+                                                                  (double *)dW, c->stream)
+                          : rpl::launch_q_weights_columns<float>(c->plane, nz, nk, (const double *)dzf,
+                                                                 (const double *)dkc, (const double *)ddep,
+                                                                 (float *)dW, c->stream);
+        if (e == hipSuccess)
+            e = hipStreamSynchronize(c->stream);
+        (void)hipFree(dzf);
+        (void)hipFree(dkc);
+        (void)hipFree(ddep);
+        if (e != hipSuccess) {
+            rc = fail(RPL_ERR_HIP, std::string("pseudomomentum weights: ") + hipGetErrorString(e));
+            goto done;
+        }
+    }
     {
         hipError_t e;
         const bool ordered = pm_ordered(c);
@@ -1899,13 +1973,13 @@ static int pseudomomentum_cells(rpl_ctx *c, const void *Nsrc, int32_t nz, const 
                                                 (const double *)dW,
                                                 nullptr, (const double *)dkx, (const double *)dky,
                                                 cell_average ? (const double *)dS : nullptr, (double *)dpx,
-                                                (double *)dpy, nullptr, nullptr, c->stream);
+                                                (double *)dpy, nullptr, nullptr, c->stream, percol);
         else
             e = rpl::launch_pm_contract<float>(ordered, nx, ny, nk, np, nz, (const float *)Nsrc,
                                                (const float *)dW,
                                                nullptr, (const float *)dkx, (const float *)dky,
                                                cell_average ? (const float *)dS : nullptr, (float *)dpx,
-                                               (float *)dpy, nullptr, nullptr, c->stream);
+                                               (float *)dpy, nullptr, nullptr, c->stream, percol);
         if (e == hipSuccess)
             e = hipStreamSynchronize(c->stream);
         if (e != hipSuccess) {

@@ -170,10 +170,21 @@ inline size_t pm_factored_lds(int nk, int nout, bool with_b, size_t esize)
 {
     return (2 * (size_t)nk * 64 + (size_t)nk * nout * (with_b ? 2 : 1)) * esize;
 }
+// per_column: A and B hold one matrix per column, A[ij + nx ny (o + nout s)]
+// (a depth per column); the ordered kernel then serves every shape.
 template <typename T>
 hipError_t launch_pm_contract(bool ordered, int nx, int ny, int nk, int nphi, int nout, const T *N, const T *A,
                               const T *B, const T *kx, const T *ky, const T *scale, T *X, T *Y, T *dX, T *dY,
-                              hipStream_t s);
+                              hipStream_t s, bool per_column = false);
+// The pseudomomentum overlap tables and Q weights of every column at its own
+// depth (FP64 weights, cast to T): O, dO[ij + plane (t + nk s)],
+// W[ij + plane (k + nz m)]; zf, kc and depth on the device.
+template <typename T>
+hipError_t launch_pm_overlap_columns(size_t plane, int nz, int nk, const double *zf, const double *kc,
+                                     const double *depth, T *O, T *dO, hipStream_t s);
+template <typename T>
+hipError_t launch_q_weights_columns(size_t plane, int nz, int nk, const double *zf, const double *kc,
+                                    const double *depth, T *W, hipStream_t s);
 
 // out = (Nnew - Nold) / dt elementwise (pseudomomentum.jl:412).
 template <typename T>

@@ -68,7 +68,8 @@ def automatic_q_grid(grid, spectral_grid, depth, Nz=None, surface_spacing=None, 
         raise ValueError("automatic Q-grid selection requires a PolarWaveVectorGrid")
     if depth is None or isinstance(depth, InfiniteDepth):
         raise ValueError("automatic Q-grid selection requires finite model `depth`")
-    H = float(depth)
+    # lagrangian_velocities.jl:199: the deepest column (max_depth)
+    H = float(np.max(depth.data)) if isinstance(depth, HorizontalField) else float(depth)
     dz_surface = 1.0 / (20 * float(np.max(spectral_grid.kappa))) if surface_spacing is None else float(surface_spacing)
     if not dz_surface > 0:
         raise ValueError("automatic Q-grid `surface_spacing` must be positive")
@@ -187,9 +188,6 @@ class SpectralWaveModel:
 
         has_coupling = velocities is not None
         self.pseudomomentum = isinstance(velocities, PseudomomentumVelocities)
-        if self.pseudomomentum and self.depth_field is not None:
-            raise NotImplementedError("the pseudomomentum coupling takes a constant depth (the overlap tables of a "
-                                      "depth per column, pseudomomentum.jl:72-87, are not built)")
         if has_coupling and not isinstance(spectral_grid, PolarWaveVectorGrid):
             raise ValueError("CWCM current coupling requires a PolarWaveVectorGrid")
         # Models/tendencies.jl:18-109.
@@ -278,7 +276,7 @@ class SpectralWaveModel:
         # lagrangian_velocities.jl:113-115: the Q grid's column depth for
         # InfiniteDepth, else the model depth.
         if self.depth_field is not None:
-            raise NotImplementedError("the pseudomomentum coupling and fields take a constant depth")
+            return 0.0  # the library takes each column's depth from rpl_set_depth_field
         return q_grid.Lz if isinstance(self.depth, InfiniteDepth) else float(self.depth)
 
     def _set_pseudomomentum(self, pv):

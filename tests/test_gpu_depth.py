@@ -141,3 +141,45 @@ def test_depth_errors():
     b = build(wl, np.float32, "auto", None, "refraction")
     with pytest.raises(ra.RippleError):
         ra.Ensemble([a, b])
+
+
+def shelf(x, y):
+    """4 m at x = 0 rising to 12 m at x = 80 (the packet's box), with a bump."""
+    return 4.0 + 8.0 * (x / 80.0) + 1.0 * np.cos(2 * np.pi * y / 80.0) ** 2
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_pseudomomentum_with_depth_field(oracle, dtype):
+    """pseudomomentum.jl:72-87 with a depth per column: the overlap tables of
+    every column at its depth, the per-stage U and dU/dkappa, two RK3 steps
+    and the cell averages (q_depth_at, pseudomomentum.jl:375-400) against
+    the oracle's per-column tables (device FP64 exp: rounding-level)."""
+    say(f"pseudomomentum with a depth field {np.dtype(dtype).name}")
+    wl = workloads.pseudomomentum_packet(float_type=dtype, nx=32, ny=32, nz=8, nkappa=6, nphi=16)
+    g, sg = wl.grid, wl.spectral_grid
+    X, Y = np.meshgrid(ra.xnodes(g), ra.ynodes(g), indexing="ij")
+    d = shelf(X, Y)
+    m = ra.SpectralWaveModel(g, sg, depth=shelf, **wl.model_kwargs)
+    ra.set_(m, N=wl.N0)
+    ra.update_coupling_(m)
+    O, dO = oracle.pm_overlap_columns(g.z_faces, sg.kappa, d)
+    t = oracle.geometry(sg.kappa, sg.kappa_faces, sg.phi, sg.phi_faces)
+    want = oracle.pm_doppler(wl.N0, O, dO, t["mu_x"], t["mu_y"])
+    got = list(m.doppler_velocity()) + list(m.doppler_velocity_kappa_derivative())
+    tol_u = 1e-13 if dtype == np.float64 else 2e-6
+    for name, a, b in zip(("Ux", "Uy", "dUx", "dUy"), got, want):
+        say(f"{name} rel-L2 {rel_l2(a, b):.3e}")
+        assert rel_l2(a, b) < tol_u, name
+    # the depth matters: one table at the mean depth gives another velocity
+    Om, dOm = oracle.pm_overlap(g.z_faces, sg.kappa, float(d.mean()))
+    assert rel_l2(got[0], oracle.pm_doppler(wl.N0, Om, dOm, t["mu_x"], t["mu_y"])[0]) > 1e-4
+    P = ra.compute_pseudomomentum_cell_averages(m)
+    Pw = oracle.pm_cells(wl.N0, oracle.q_weights_columns(g.z_faces, sg.kappa, d), g.z_faces, t["mu_x"], t["mu_y"])
+    say(f"cell averages rel-L2 {rel_l2(P[0], Pw[0]):.3e}")
+    assert rel_l2(P[0], Pw[0]) < tol_u and rel_l2(P[1], Pw[1]) < tol_u
+    om = oracle.OracleModel(wl, dtype=dtype, depth=d, pseudomomentum=(g.z_faces, g.Lz))
+    ra.time_step_(m, wl.dt, 2)
+    om.step(wl.dt, 2)
+    err = rel_l2(m.action, om.action)
+    say(f"2 RK3 steps rel-L2 {err:.3e}")
+    assert err < (1e-12 if dtype == np.float64 else 5e-5)

@@ -91,3 +91,30 @@ def test_transport_reads_each_cells_velocity(oracle, mode):
         Gc = oracle.OracleModel(wl, depth=float(d[i, j]), **kw2).tendency()
         assert np.array_equal(G[i, j], Gc[i, j])
     assert not np.array_equal(G, oracle.OracleModel(wl, depth=float(d.mean()), **kw).tendency())
+
+
+def test_pseudomomentum_tables_per_column(oracle):
+    """pseudomomentum.jl:72-87 with a depth per column: each column's overlap
+    tables are the constant-depth tables (pinned in test_oracle_
+    pseudomomentum.py) at its depth, and the contraction and the cell
+    averages of every column use that column's tables."""
+    rng = np.random.default_rng(5)
+    nx, ny, nk, nphi, nz = 3, 4, 5, 6, 7
+    zf = np.linspace(-4.0, 0.0, nz + 1)
+    kappa = np.geomspace(0.1, 2.0, nk)
+    d = _depth(nx, ny, 1.0, 4.0, seed=2)
+    O, dO = oracle.pm_overlap_columns(zf, kappa, d)
+    N = np.asfortranarray(rng.uniform(0.0, 1.0, (nx, ny, nk, nphi)))
+    kx, ky = rng.standard_normal((nk, nphi)), rng.standard_normal((nk, nphi))
+    U = oracle.pm_doppler(N, O, dO, kx, ky)
+    Wc = oracle.q_weights_columns(zf, kappa, d)
+    P = oracle.pm_cells(N, Wc, zf, kx, ky)
+    for i in range(nx):
+        for j in range(ny):
+            Oc, dOc = oracle.pm_overlap(zf, kappa, d[i, j])
+            assert np.array_equal(O[i, j], Oc) and np.array_equal(dO[i, j], dOc)
+            Uc = oracle.pm_doppler(N[i:i + 1, j:j + 1], Oc, dOc, kx, ky)
+            for a, b in zip(U, Uc):
+                assert np.array_equal(a[i, j], b[0, 0])
+            Pc = oracle.pm_cells(N[i:i + 1, j:j + 1], oracle.q_weights(zf, kappa, d[i, j]), zf, kx, ky)
+            assert np.array_equal(P[0][i, j], Pc[0][0, 0]) and np.array_equal(P[1][i, j], Pc[1][0, 0])
