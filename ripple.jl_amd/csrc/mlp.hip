@@ -29,12 +29,35 @@ __device__ __forceinline__ bf16 f2bf(float f)
     return (bf16)(u >> 16);
 }
 
-__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+// Standard normal CDF Phi(z) = erfc(-z / sqrt 2) / 2, with erfc(x), x >= 0,
+// from the Chebyshev fit of Press et al., Numerical Recipes in C (2nd ed.,
+// section 6.2, erfcc: fractional error below 1.2e-7): erfc(x) = t exp(-x^2 +
+// P(t)), t = 1 / (1 + x / 2).  Evaluated with the hardware reciprocal and
+// exp2 it stays within 2.5e-5 relative in FP32 over the whole line, also in
+// the tail z << 0 where 1 + erf(z / sqrt 2) cancels; it is cheaper than
+// erff, which the GEMM epilogues evaluate once per activation.
+__device__ __forceinline__ float phi_cdf(float z)
+{
+    const float x = fabsf(z) * 0.70710678118654752f;
+    const float t = __builtin_amdgcn_rcpf(fmaf(0.5f, x, 1.0f));
+    float p = 0.17087277f;
+    p = fmaf(p, t, -0.82215223f);
+    p = fmaf(p, t, 1.48851587f);
+    p = fmaf(p, t, -1.13520398f);
+    p = fmaf(p, t, 0.27886807f);
+    p = fmaf(p, t, -0.18628806f);
+    p = fmaf(p, t, 0.09678418f);
+    p = fmaf(p, t, 0.37409196f);
+    p = fmaf(p, t, 1.00002368f);
+    p = fmaf(p, t, -1.26551223f);
+    const float h = 0.5f * t * __builtin_amdgcn_exp2f(fmaf(-x, x, p) * 1.4426950408889634f);  // erfc(x) / 2
+    return z >= 0.0f ? 1.0f - h : h;
+}
+__device__ __forceinline__ float gelu(float z) { return z * phi_cdf(z); }
 __device__ __forceinline__ float gelu_grad(float z)
 {
-    const float cdf = 0.5f * (1.0f + erff(z * 0.70710678118654752f));
-    const float pdf = 0.39894228040143268f * expf(-0.5f * z * z);
-    return cdf + z * pdf;
+    const float pdf = 0.39894228040143268f * __builtin_amdgcn_exp2f(-0.72134752044448170f * z * z);
+    return phi_cdf(z) + z * pdf;
 }
 
 // C[m, n] = sum_k A(m, k) B(k, n), A(m, k) = A[m*sam + k*sak], B(k, n) = B[k*sbk + n*sbn],
@@ -338,6 +361,178 @@ __global__ __launch_bounds__(512) void gemm_wide_kernel(GemmArgs g)
             }
 }
 
+// The same 128 x 128 block tile and eight 64 x 32 wavefront blocks for
+// operands that are both contiguous along k (the inference GEMMs and the
+// training forward), with the K steps moved from global memory straight into
+// LDS by global_load_lds_dwordx4: no staging registers and no LDS store
+// instructions, so the next step's loads run under the current step's MFMAs
+// with one barrier per step.  A load instruction fills 1 KB of LDS, lane l at
+// byte 16 l, i.e. eight 128-byte rows of one 64-deep K step; each lane picks
+// the global 16-byte chunk that belongs at its LDS position.  Within a row
+// chunk c is stored at position c ^ ((row >> 1) & 7): the rows of a
+// fragment read, all at one chunk, then spread over the banks.  Measured
+// with rocprofv3 (SQ_LDS_BANK_CONFLICT / SQ_LDS_IDX_ACTIVE): no conflicts in
+// the K loop, against 0.5 with c ^ (row & 7) ^ ((row >> 3) & 1) and 0.7
+// unswizzled (1.7x slower).  Measured at the surrogate's shapes (65,536
+// columns, 512 outputs; rate layer 1152 x 65,536) against the
+// register-staged kernel above: K = 1216 150 -> 108 us, K = 512 72 -> 61 us,
+// rate layer 191 -> 180 us.  Not kept: all of a step's fragment reads
+// before its MFMAs (no change; with a scheduling barrier between them 33 %
+// slower), raised wave priority around the MFMAs (no change), a third
+// activation buffer (80 KB per workgroup: the compiler then waits for every
+// load in flight before the fragment reads, or spills when unrolled).
+// Needs M and N multiples of 128, K of 64, 16-byte aligned operands and row
+// strides that are multiples of 8 elements.
+typedef __attribute__((address_space(1))) const void *global_ptr;
+typedef __attribute__((address_space(3))) void *lds_ptr;
+
+__device__ __forceinline__ int chunk_pos(int c, int row) { return c ^ ((row >> 1) & 7); }
+
+// 128 rows x 64 k of G (row stride s) into the 16 KB step buffer S: two
+// load instructions per wavefront.
+__device__ __forceinline__ void dma_step(char *S, const bf16 *__restrict__ G, size_t s, int wave, int lane)
+{
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const int r0 = (wave * 2 + i) * 8;
+        const int row = r0 + (lane >> 3);
+        const bf16 *src = G + (size_t)row * s + chunk_pos(lane & 7, row) * 8;
+        __builtin_amdgcn_global_load_lds((global_ptr)src, (lds_ptr)(S + r0 * 128), 16, 0, 0);
+    }
+}
+
+template <int EPI>
+__global__ __launch_bounds__(512) void gemm_dma_kernel(GemmArgs g)
+{
+    // Two step buffers per operand as separate arrays, so that the compiler
+    // sees that the loads into one do not feed the reads of the other.
+    __shared__ __attribute__((aligned(1024))) char As0[WM * 128];
+    __shared__ __attribute__((aligned(1024))) char As1[WM * 128];
+    __shared__ __attribute__((aligned(1024))) char Bs0[WN * 128];
+    __shared__ __attribute__((aligned(1024))) char Bs1[WN * 128];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    int m0, n0;
+    wide_tile(blockIdx.x, g.M / WM, g.N / WN, m0, n0);
+    const int wm = (wave & 1) * 64, wn = (wave >> 1) * 32;
+    const bf16 *Ab = g.A + (size_t)m0 * g.sam;
+    const bf16 *Bb = g.B + (size_t)n0 * g.sbn;
+    f32x4 acc[4][2];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+            acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+    // Fragment of a 16-row tile at k = 32 h + 8 (l / 16): row l % 16, chunk 4 h + l / 16.
+    const int r16 = lane & 15;
+    const int off[2] = {r16 * 128 + chunk_pos(lane >> 4, r16) * 16, r16 * 128 + chunk_pos(4 + (lane >> 4), r16) * 16};
+    auto multiply = [&](const char *As, const char *Bs) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            bf16x8 af[4], bfr[2];
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+                af[a] = *reinterpret_cast<const bf16x8 *>(As + (wm + a * 16) * 128 + off[h]);
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+                bfr[b] = *reinterpret_cast<const bf16x8 *>(Bs + (wn + b * 16) * 128 + off[h]);
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b)
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a], bfr[b], acc[a][b], 0, 0, 0);
+        }
+    };
+    const int nk = g.K / WK;
+    dma_step(As0, Ab, g.sam, wave, lane);
+    dma_step(Bs0, Bb, g.sbn, wave, lane);
+    // Two K steps per trip, one per buffer pair.  Before each step every
+    // wavefront waits for its own loads (vmcnt 0) and the barrier makes all
+    // of them visible; the loads issued after the barrier refill the pair
+    // that every wavefront finished reading before it.
+    for (int kt = 0; kt < nk; kt += 2) {
+        __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0)
+        __syncthreads();
+        if (kt + 1 < nk) {
+            dma_step(As1, Ab + (size_t)(kt + 1) * WK, g.sam, wave, lane);
+            dma_step(Bs1, Bb + (size_t)(kt + 1) * WK, g.sbn, wave, lane);
+        }
+        multiply(As0, Bs0);
+        if (kt + 1 >= nk)
+            break;
+        __builtin_amdgcn_s_waitcnt(0x0f70);
+        __syncthreads();
+        if (kt + 2 < nk) {
+            dma_step(As0, Ab + (size_t)(kt + 2) * WK, g.sam, wave, lane);
+            dma_step(Bs0, Bb + (size_t)(kt + 2) * WK, g.sbn, wave, lane);
+        }
+        multiply(As1, Bs1);
+    }
+    if constexpr (EPI == EPI_BIAS_GELU_INFER) {
+        if (g.scn == 1 && (g.scm & 7) == 0 && ((size_t)g.Cb & 15) == 0) {
+            // Each wavefront stages its 64 x 32 bf16 results (4 KB; four
+            // wavefronts in As0, four in Bs0, free after the barrier) with
+            // the row's four 16-byte chunks rotated by row, and writes them
+            // as 16-byte row chunks.
+            __syncthreads();
+            float bn[2];
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+                bn[b] = g.bias[n0 + wn + b * 16 + r16];
+            char *R = (wave < 4 ? As0 : Bs0) + (wave & 3) * 4096;
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int row = a * 16 + 4 * (lane >> 4) + r, col = b * 16 + r16;
+                        *reinterpret_cast<bf16 *>(R + row * 64 + (((col >> 3) ^ (row & 3)) * 16) + (col & 7) * 2) =
+                            f2bf(gelu(acc[a][b][r] + bn[b]));
+                    }
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int c = lane + 64 * k, row = c >> 2, j = c & 3;
+                const u32x4 v = *reinterpret_cast<const u32x4 *>(R + row * 64 + ((j ^ (row & 3)) * 16));
+                *reinterpret_cast<u32x4 *>(g.Cb + (size_t)(m0 + wm + row) * g.scm + (n0 + wn + j * 8)) = v;
+            }
+            return;
+        }
+    }
+    if constexpr (EPI == EPI_BIAS_ROW_F32) {
+        float bm[4][4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                bm[a][r] = g.bias[m0 + wm + a * 16 + 4 * (lane >> 4) + r];
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int m = m0 + wm + a * 16 + 4 * (lane >> 4) + r;
+                    const int n = n0 + wn + b * 16 + r16;
+                    const float v = acc[a][b][r] + bm[a][r];
+                    g.Cf[(size_t)m * g.scm + (size_t)n * g.scn] = v;
+                    if (g.nonfinite && !(fabsf(v) < 3.0e38f))
+                        g.nonfinite[n] = 2;
+                }
+        return;
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int m = m0 + wm + a * 16 + 4 * (lane >> 4) + r;
+                const int n = n0 + wn + b * 16 + r16;
+                epilogue<EPI>(g, m, n, acc[a][b][r]);
+            }
+}
+
 __global__ __launch_bounds__(256) void column_sum_kernel(const bf16 *__restrict__ D, int M, int N, float *out)
 {
     // out[n] = sum over m of D[m*N + n] (bias gradient).  Thread per n, rows in
@@ -591,7 +786,12 @@ __global__ __launch_bounds__(64) void audit_kernel(const T *__restrict__ N, size
 template <bool A_KC, bool B_KC, int EPI>
 static void gemm_launch(const GemmArgs &g, hipStream_t s)
 {
-    if (g.M % WM == 0 && g.N % WN == 0 && g.K % WK == 0)
+    const bool wide = g.M % WM == 0 && g.N % WN == 0 && g.K % WK == 0;
+    const bool dma_ok = A_KC && B_KC && ((size_t)g.A & 15) == 0 && ((size_t)g.B & 15) == 0 && (g.sam & 7) == 0 &&
+                        (g.sbn & 7) == 0;
+    if (wide && dma_ok)
+        hipLaunchKernelGGL((gemm_dma_kernel<EPI>), dim3((g.M / WM) * (g.N / WN)), dim3(512), 0, s, g);
+    else if (wide)
         hipLaunchKernelGGL((gemm_wide_kernel<A_KC, B_KC, EPI>), dim3((g.M / WM) * (g.N / WN)), dim3(512), 0, s, g);
     else
         hipLaunchKernelGGL((gemm_kernel<A_KC, B_KC, EPI>), dim3(g.M / BM, g.N / BN), dim3(256), 0, s, g);

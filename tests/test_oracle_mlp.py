@@ -111,3 +111,33 @@ def test_emulated_passes_stay_close_to_exact():
     a = np.concatenate([g[0].ravel() for g in g0])
     b = np.concatenate([g[0].ravel() for g in g1])
     assert np.linalg.norm(a - b) / np.linalg.norm(a) < 5e-2
+
+
+def test_device_normal_cdf_coefficients_reproduce_erfc():
+    """The GEMM epilogues evaluate GELU as z * Phi(z) with erfc from the
+    Numerical Recipes Chebyshev fit (erfcc); the coefficients are read from
+    the kernel source and checked against math.erfc: fractional error below
+    1.2e-7 (the published bound) on [0, 9], and Phi(z) + Phi(-z) = 1."""
+    import os
+    import re
+    src = open(os.path.join(os.path.dirname(__file__), "..", "ripple.jl_amd", "csrc", "mlp.hip")).read()
+    body = src[src.index("float phi_cdf(float z)"):src.index("float gelu(float z)")]
+    first = float(re.search(r"float p = ([-0-9.e]+)f;", body).group(1))
+    rest = [float(c) for c in re.findall(r"p = fmaf\(p, t, ([-0-9.e]+)f\);", body)]
+    assert len(rest) == 9
+    coef = [first] + rest
+
+    def erfc(x):
+        t = 1.0 / (1.0 + 0.5 * x)
+        p = coef[0]
+        for c in coef[1:]:
+            p = p * t + c
+        return t * math.exp(-x * x + p)
+
+    xs = np.linspace(0.0, 9.0, 9001)
+    worst = max(abs(erfc(x) - math.erfc(x)) / math.erfc(x) for x in xs)
+    assert worst < 1.2e-7, worst
+    for z in (0.3, 1.7, 4.2):
+        phi = lambda v: 1.0 - 0.5 * erfc(v / math.sqrt(2)) if v >= 0 else 0.5 * erfc(-v / math.sqrt(2))
+        assert abs(phi(z) + phi(-z) - 1.0) < 1e-15
+        assert abs(z * phi(z) - mo.gelu(np.array(z))) < 1.2e-7 * abs(z)
