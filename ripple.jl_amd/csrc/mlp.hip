@@ -109,6 +109,11 @@ __device__ __forceinline__ void epilogue(const GemmArgs &g, int m, int n, float 
         g.Cf[o] = v;
         if (g.nonfinite && !(fabsf(v) < 3.0e38f))
             g.nonfinite[n] = 2;
+    } else if (EPI == EPI_BIAS_ROW_F16) {
+        const _Float16 h = (_Float16)(c + g.bias[m]);
+        g.Ch[o] = h;
+        if (g.nonfinite && !(fabsf((float)h) <= 65504.0f))
+            g.nonfinite[n] = 2;
     } else {  // EPI_GELU_BACKWARD: dZ = dA * gelu'(Z)
         g.Cb[o] = f2bf(c * gelu_grad(g.Zin[o]));
     }
@@ -521,6 +526,46 @@ __global__ __launch_bounds__(512) void gemm_dma_kernel(GemmArgs g)
                 }
         return;
     }
+    if constexpr (EPI == EPI_BIAS_ROW_F16) {
+        if (g.scn == 1 && (g.scm & 7) == 0 && ((size_t)g.Ch & 15) == 0) {
+            // The rate planes in FP16: each wavefront stages its 64 x 32
+            // results (4 KB, as the GELU epilogue) and writes 64-byte row
+            // segments in 16-byte chunks.  A value that FP16 cannot hold
+            // flags its column.
+            __syncthreads();
+            char *R = (wave < 4 ? As0 : Bs0) + (wave & 3) * 4096;
+            bool bad[2] = {false, false};
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = a * 16 + 4 * (lane >> 4) + r;
+                    const float bm = g.bias[m0 + wm + row];
+#pragma unroll
+                    for (int b = 0; b < 2; ++b) {
+                        const int col = b * 16 + r16;
+                        const _Float16 h = (_Float16)(acc[a][b][r] + bm);
+                        bad[b] = bad[b] || !(fabsf((float)h) <= 65504.0f);
+                        *reinterpret_cast<_Float16 *>(R + row * 64 + (((col >> 3) ^ (row & 3)) * 16) + (col & 7) * 2) =
+                            h;
+                    }
+                }
+            if (g.nonfinite) {
+#pragma unroll
+                for (int b = 0; b < 2; ++b)
+                    if (bad[b])
+                        g.nonfinite[n0 + wn + b * 16 + r16] = 2;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int c = lane + 64 * k, row = c >> 2, j = c & 3;
+                const u32x4 v = *reinterpret_cast<const u32x4 *>(R + row * 64 + ((j ^ (row & 3)) * 16));
+                *reinterpret_cast<u32x4 *>(g.Ch + (size_t)(m0 + wm + row) * g.scm + (n0 + wn + j * 8)) = v;
+            }
+            return;
+        }
+    }
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
@@ -813,7 +858,7 @@ __global__ __launch_bounds__(256) void envelope_kernel(size_t plane, const T *__
 // (bit 2) so that the stage takes the plain source there.
 template <typename T>
 __global__ __launch_bounds__(64) void audit_kernel(const T *__restrict__ N, size_t plane, int nbins,
-                                                   const float *__restrict__ rate, size_t ld,
+                                                   const _Float16 *__restrict__ rate, size_t ld,
                                                    const float *__restrict__ out_scale,
                                                    unsigned char *use_plain, int stride, SourceParams<T> sp,
                                                    float bound, unsigned *counts, float *err_max)
@@ -825,7 +870,7 @@ __global__ __launch_bounds__(64) void audit_kernel(const T *__restrict__ N, size
     for (int b = threadIdx.x; b < nbins; b += 64) {
         const T a = N[col + plane * b];
         const float sp_plain = (float)source_term<T>(sp, (unsigned)col, b, a);
-        const float sp_net = rate[col + ld * b] * out_scale[b] * (float)a;
+        const float sp_net = (float)rate[col + ld * b] * out_scale[b] * (float)a;
         num += (sp_net - sp_plain) * (sp_net - sp_plain);
         den += sp_plain * sp_plain;
     }
@@ -877,6 +922,9 @@ static hipError_t gemm_dispatch(const GemmArgs &g, int epi, hipStream_t s)
         break;
     case EPI_BIAS_ROW_F32:
         gemm_launch<A_KC, B_KC, EPI_BIAS_ROW_F32>(g, s);
+        break;
+    case EPI_BIAS_ROW_F16:
+        gemm_launch<A_KC, B_KC, EPI_BIAS_ROW_F16>(g, s);
         break;
     default:
         gemm_launch<A_KC, B_KC, EPI_GELU_BACKWARD>(g, s);
@@ -976,7 +1024,7 @@ hipError_t envelope(size_t plane, const T *U10, const unsigned char *flags, cons
 }
 
 template <typename T>
-hipError_t audit(const T *N, size_t plane, int nbins, const float *rate, size_t ld, const float *out_scale,
+hipError_t audit(const T *N, size_t plane, int nbins, const _Float16 *rate, size_t ld, const float *out_scale,
                  unsigned char *use_plain, int stride, const SourceParams<T> &sp, float bound, unsigned *counts,
                  float *err_max, hipStream_t s)
 {
@@ -997,10 +1045,10 @@ template hipError_t envelope<float>(size_t, const float *, const unsigned char *
                                     unsigned *, hipStream_t);
 template hipError_t envelope<double>(size_t, const double *, const unsigned char *, const Envelope &,
                                      unsigned char *, unsigned *, hipStream_t);
-template hipError_t audit<float>(const float *, size_t, int, const float *, size_t, const float *,
+template hipError_t audit<float>(const float *, size_t, int, const _Float16 *, size_t, const float *,
                                  unsigned char *, int, const SourceParams<float> &, float, unsigned *, float *,
                                  hipStream_t);
-template hipError_t audit<double>(const double *, size_t, int, const float *, size_t, const float *,
+template hipError_t audit<double>(const double *, size_t, int, const _Float16 *, size_t, const float *,
                                   unsigned char *, int, const SourceParams<double> &, float, unsigned *, float *,
                                   hipStream_t);
 

@@ -207,7 +207,7 @@ namespace rpl {
 namespace mlp {
 
 hipError_t forward_rows(rpl_mlp *n, const bf16 *X, int rows, bf16 *act_a, bf16 *act_b, float *out, size_t ld,
-                        unsigned char *nonfinite, hipStream_t s)
+                        unsigned char *nonfinite, hipStream_t s, _Float16 *out_h)
 {
     const int L = nlayers(n);
     const bf16 *in = X;
@@ -234,11 +234,12 @@ hipError_t forward_rows(rpl_mlp *n, const bf16 *X, int rows, bf16 *act_a, bf16 *
     g.sbk = 1;
     g.sbn = (size_t)g.K;
     g.Cf = out;
+    g.Ch = out_h;
     g.scm = ld;
     g.scn = 1;
     g.bias = n->b[L - 1];
     g.nonfinite = nonfinite;
-    return gemm(g, EPI_BIAS_ROW_F32, s);
+    return gemm(g, out ? EPI_BIAS_ROW_F32 : EPI_BIAS_ROW_F16, s);
 }
 
 }  // namespace mlp

@@ -16,7 +16,8 @@ enum {
     EPI_BIAS_F32 = 2,        // C = product + bias[n] (FP32)
     EPI_GELU_BACKWARD = 3,   // dZ = product * gelu'(Zin) (bf16)
     EPI_BIAS_GELU_INFER = 4, // A = gelu(product + bias[n]) (bf16)
-    EPI_BIAS_ROW_F32 = 5     // C = product + bias[m] (FP32)
+    EPI_BIAS_ROW_F32 = 5,    // C = product + bias[m] (FP32)
+    EPI_BIAS_ROW_F16 = 6     // C = product + bias[m] (FP16, to Ch)
 };
 
 // C[m, n] = sum_k A(m, k) B(k, n); strides in elements.  M, N multiples of 64,
@@ -32,7 +33,8 @@ struct GemmArgs {
     size_t scm, scn;   // output strides
     const float *bias; // per n (per m for EPI_BIAS_ROW_F32)
     const float *Zin;  // pre-activation read by the GELU backward epilogue (strides scm, scn)
-    unsigned char *nonfinite;  // EPI_BIAS_ROW_F32: set to 2 at column n where an output is not finite (or null)
+    unsigned char *nonfinite;  // EPI_BIAS_ROW_F32/F16: set to 2 at column n where an output is not finite (or null)
+    _Float16 *Ch;              // FP16 output (EPI_BIAS_ROW_F16)
 };
 
 struct FeatureScales {
@@ -76,7 +78,7 @@ template <typename T>
 hipError_t envelope(size_t plane, const T *U10, const unsigned char *flags, const Envelope &env,
                     unsigned char *use_plain, unsigned *counts, hipStream_t s);
 template <typename T>
-hipError_t audit(const T *N, size_t plane, int nbins, const float *rate, size_t ld, const float *out_scale,
+hipError_t audit(const T *N, size_t plane, int nbins, const _Float16 *rate, size_t ld, const float *out_scale,
                  unsigned char *use_plain, int stride, const SourceParams<T> &sp, float bound, unsigned *counts,
                  float *err_max, hipStream_t s);
 
@@ -108,8 +110,9 @@ namespace rpl {
 namespace mlp {
 // Inference of `rows` (multiple of 64) feature rows X [rows x in_pad]: hidden
 // layers with the bf16 inference epilogue into act_a / act_b, output layer
-// computed transposed (W X^T) so that out[row + ld * o] is stored along rows.
+// computed transposed (W X^T) so that out[row + ld * o] is stored along rows:
+// in FP32 to out, or (out null) in FP16 to out_h.
 hipError_t forward_rows(rpl_mlp *net, const bf16 *X, int rows, bf16 *act_a, bf16 *act_b, float *out, size_t ld,
-                        unsigned char *nonfinite, hipStream_t s);
+                        unsigned char *nonfinite, hipStream_t s, _Float16 *out_h = nullptr);
 }  // namespace mlp
 }  // namespace rpl

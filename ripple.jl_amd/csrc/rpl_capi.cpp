@@ -130,7 +130,8 @@ struct rpl_ctx {
     rpl_surrogate scfg{};
     bool s_active = false;  // the stage takes the network's source where allowed
     rpl::mlp::bf16 *sX = nullptr, *sA = nullptr, *sB = nullptr;
-    float *rate = nullptr, *out_scale = nullptr, *s_err = nullptr, *bin_hi = nullptr, *s_amax = nullptr;
+    _Float16 *rate = nullptr;  // the network's rates, [out_pad][s_ld], FP16
+    float *out_scale = nullptr, *s_err = nullptr, *bin_hi = nullptr, *s_amax = nullptr;
     unsigned char *s_flags = nullptr;  // per column: envelope flags of the current stage
     unsigned char *use_plain = nullptr;
     unsigned *s_counts = nullptr;
@@ -429,8 +430,8 @@ int surrogate_rates(rpl_ctx *c, const T *Nk)
                                   (const T *)c->theta_u, fs, c->sX, c->bin_hi, g.action_hi, c->s_amax, c->s_flags,
                                   c->stream);
         if (e == hipSuccess)
-            e = rpl::mlp::forward_rows(c->net, c->sX, (int)rows, c->sA, c->sB, c->rate + col0, c->s_ld,
-                                       c->s_flags + col0, c->stream);
+            e = rpl::mlp::forward_rows(c->net, c->sX, (int)rows, c->sA, c->sB, nullptr, c->s_ld, c->s_flags + col0,
+                                       c->stream, c->rate + col0);
     }
     if (e == hipSuccess) {
         rpl::mlp::Envelope env{g.wind_lo, g.wind_hi, g.action_hi, g.audit_stride};
@@ -758,7 +759,8 @@ void free_surrogate(rpl_ctx *c)
                     (void *)c->s_amax, (void *)c->s_flags})
         free_ptr(p);
     c->sX = c->sA = c->sB = nullptr;
-    c->rate = c->out_scale = c->s_err = c->bin_hi = c->s_amax = nullptr;
+    c->rate = nullptr;
+    c->out_scale = c->s_err = c->bin_hi = c->s_amax = nullptr;
     c->s_flags = nullptr;
     c->use_plain = nullptr;
     c->s_counts = nullptr;
@@ -2987,7 +2989,7 @@ int rpl_attach_surrogate(rpl_ctx *c, rpl_mlp *net, const rpl_surrogate *cfg, con
     int rc;
     if ((rc = alloc((void **)&c->sX, c->s_chunk * in_pad * 2)) || (rc = alloc((void **)&c->sA, c->s_chunk * hid * 2)) ||
         (rc = alloc((void **)&c->sB, c->s_chunk * hid * 2)) ||
-        (rc = alloc((void **)&c->rate, c->s_ld * out_pad * sizeof(float))) ||
+        (rc = alloc((void **)&c->rate, c->s_ld * out_pad * sizeof(_Float16))) ||
         (rc = alloc((void **)&c->out_scale, out_pad * sizeof(float))) || (rc = alloc((void **)&c->s_err, 16)) ||
         (rc = alloc((void **)&c->use_plain, c->s_ld)) || (rc = alloc((void **)&c->s_counts, 16)) ||
         (rc = alloc((void **)&c->s_amax, c->s_ld * sizeof(float))) || (rc = alloc((void **)&c->s_flags, c->s_ld))) {

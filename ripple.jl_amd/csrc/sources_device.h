@@ -48,9 +48,13 @@ struct SourceParams {
     // bit 0 (outside the training envelope) nor bit 2 (failed audit), the
     // network term is rate[col + rate_ld * t] * out_scale[t] * N; it is the
     // whole source (surrogate_terms 0) or the wind input only, the
-    // dissipation staying the plain bundle's (surrogate_terms 1).
+    // dissipation staying the plain bundle's (surrogate_terms 1).  The rates
+    // are stored in FP16 (relative rounding 2^-11, well inside the network's
+    // own error); one that overflows FP16 flags its column non-finite, which
+    // sends the column to the plain bundle.
     int surrogate, surrogate_terms;
-    const float *rate, *out_scale;
+    const _Float16 *rate;
+    const float *out_scale;
     size_t rate_ld;
     const unsigned char *use_plain;
 };
@@ -236,7 +240,7 @@ __device__ __forceinline__ T source_term_cb(const SourceParams<T> &s, const Sour
     const bool net = c.net;
     T learned = T(0);
     if (net) {
-        learned = (T)(gld<AS1>(s.rate, col + s.rate_ld * (size_t)t) * b.out_scale) * a;
+        learned = (T)((float)gld<AS1>(s.rate, col + s.rate_ld * (size_t)t) * b.out_scale) * a;
         if (s.surrogate_terms == 0)
             return learned;
     }

@@ -368,9 +368,13 @@ def test_stage_source_is_the_network_and_targets_are_the_bundle():
     S_surr = (G_surr - G_none).reshape(Nx * Ny, nb, order="F")
     S_pred = Y * cfg.rate_unit * sig[None, :] * cols
     e_s = rel_l2(S_surr, S_pred)
-    say(f"targets vs bundle source {e_t:.2e}; stage source vs network {e_s:.2e}")
+    # the stage reads the rates in FP16 (relative rounding <= 2^-11)
+    S_pred16 = Y.astype(np.float16).astype(np.float64) * cfg.rate_unit * sig[None, :] * cols
+    e_s16 = rel_l2(S_surr, S_pred16)
+    say(f"targets vs bundle source {e_t:.2e}; stage source vs network {e_s:.2e}, vs its FP16 rates {e_s16:.2e}")
     assert e_t < 1e-4
     assert e_s < 1e-3
+    assert e_s16 < 1e-4
     net.close()
 
 
@@ -412,7 +416,8 @@ def test_wind_input_surrogate_keeps_the_plain_dissipation():
     Y = net.forward(X)
     sg.attach_surrogate(m_plain, net, part)
     S = (m_plain.tendency().astype(np.float64) - G_none).reshape(Nx * Ny, nb, order="F")
-    e_s = rel_l2(S, Y * 1e-4 * sigma[None, :] * cols + D)
+    # the stage reads the network's rates in FP16
+    e_s = rel_l2(S, Y.astype(np.float16).astype(np.float64) * 1e-4 * sigma[None, :] * cols + D)
     say(f"terms=1: (full - input) targets vs dissipation {e_t:.2e}; stage source vs network + dissipation {e_s:.2e}")
     assert e_t < 1e-5 and e_s < 1e-6
     net.close()
