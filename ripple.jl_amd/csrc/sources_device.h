@@ -317,9 +317,10 @@ __device__ __forceinline__ T source_term_cb(const SourceParams<T> &s, const Sour
 
 // Source kinds a stage kernel is compiled for: none, the common bundle of
 // the wind input (any drag, with or without the stress cap) and
-// local-saturation or no dissipation, without a network (SOURCE_JP_LS), or
-// every configuration through source_term_cb (SOURCE_GENERIC).
-enum { SOURCE_NONE = 0, SOURCE_JP_LS = 1, SOURCE_GENERIC = 2 };
+// local-saturation or no dissipation, without a network (SOURCE_JP_LS), the
+// same bundle with a network attached (SOURCE_NET_LS, FP32), or every
+// configuration through source_term_cb (SOURCE_GENERIC).
+enum { SOURCE_NONE = 0, SOURCE_JP_LS = 1, SOURCE_GENERIC = 2, SOURCE_NET_LS = 3 };
 
 // Whether a source configuration takes the SOURCE_JP_LS path.
 template <typename T>
@@ -329,18 +330,30 @@ __host__ __device__ inline bool source_is_jp_ls(const SourceParams<T> &s)
            (s.dissipation == 0 || s.dissipation == 2);
 }
 
+// Whether a source configuration takes the SOURCE_NET_LS path: the
+// SOURCE_JP_LS bundle (the network's plain fallback) with a network.
+template <typename T>
+__host__ __device__ inline bool source_is_net_ls(const SourceParams<T> &s)
+{
+    return s.on && s.surrogate && (s.wind_input == 0 || s.wind_input == 1) &&
+           (s.dissipation == 0 || s.dissipation == 2);
+}
+
 // The per-column inputs of SOURCE_JP_LS: u* (0 where U10 <= 0, which the
 // bundle skips), cos and sin of the wind angle, log z1 and the stress factor
 // (1 without the cap).  byte_off is the column's byte offset in an (nx, ny)
 // FP32 plane.
 struct JpColumn {
     float us, ctu, stu, lz1, fac;
+    bool net;  // SOURCE_NET_LS: the network supplies this column's source
 };
 
-template <bool AS1 = false>
+template <bool AS1 = false, bool NET = false>
 __device__ __forceinline__ JpColumn jp_column(const SourceParams<float> &s, unsigned byte_off)
 {
-    JpColumn c{0.0f, 0.0f, 0.0f, 0.0f, 1.0f};
+    JpColumn c{0.0f, 0.0f, 0.0f, 0.0f, 1.0f, false};
+    if constexpr (NET)
+        c.net = !(gld<AS1>(s.use_plain, byte_off >> 2) & 5);
     if (!s.wind_input)
         return c;
     typedef const __attribute__((address_space(1))) char gchar;
@@ -394,6 +407,41 @@ __device__ __forceinline__ float source_jp_ls(const SourceParams<float> &s, cons
         }
     }
     return positive - damping * a;
+}
+
+// source_term_cb for a SOURCE_NET_LS configuration, in its operation order
+// (the same bits): on a network column the learned rate (read from the FP16
+// rate plane by the caller) times out_scale and the action, plus, with
+// surrogate_terms 1, the plain local-saturation dissipation; elsewhere the
+// SOURCE_JP_LS bundle.  raw_out is the wind input's raw rate, 0 on network
+// columns.
+__device__ __forceinline__ float source_net_ls(const SourceParams<float> &s, const JpColumn &c,
+                                               const SourceBin<float> &b, float net_rate, float a, float &raw_out)
+{
+#pragma clang fp contract(off)
+    if (!c.net)
+        return source_jp_ls(s, c, b, a, raw_out);
+    raw_out = 0.0f;
+    const float learned = (net_rate * b.out_scale) * a;
+    if (s.surrogate_terms == 0)
+        return learned;
+    float positive = 0.0f, damping = 0.0f;
+    const float k = b.k;
+    if (s.dissipation == 2 && a > 0.0f && k > 0.0f) {
+        const float B = a * b.sk3;
+        float over = B * s.ls_inv_br - 1.0f;
+        over = over > 0.0f ? over : 0.0f;
+        if (over > 0.0f) {
+            over = over < 1e6f ? over : 1e6f;
+            const float op = s.ls_p == 2.0f ? over * over : powf(over, s.ls_p);
+            const float rate = -s.ls_cds * b.sq * op;
+            if (rate >= 0.0f)
+                damping += rate;
+            else
+                positive += -rate * a;
+        }
+    }
+    return (positive - damping * a) + learned;
 }
 
 // Source tendency of one cell: positive - damping * a.  col is the column

@@ -508,9 +508,12 @@ __device__ __forceinline__ void phi_tile(const T *__restrict__ Nk, const T *__re
     // the column's source inputs, loaded once for every bin of the tile
     SourceColumn<T> scol{};
     JpColumn jcol{};
-    static_assert(SK != SOURCE_JP_LS || std::is_same<T, float>::value, "the specialised source path is FP32");
+    static_assert((SK != SOURCE_JP_LS && SK != SOURCE_NET_LS) || std::is_same<T, float>::value,
+                  "the specialised source paths are FP32");
     if constexpr (SK == SOURCE_JP_LS)
         jcol = jp_column<AS1>(p.src, bo);
+    else if constexpr (SK == SOURCE_NET_LS)
+        jcol = jp_column<AS1, true>(p.src, bo);
     else if constexpr (SK == SOURCE_GENERIC)
         scol = source_column<T, AS1>(p.src, off);
     // What bin n reads: its x and y stencils, for refraction its kappa
@@ -522,6 +525,7 @@ __device__ __forceinline__ void phi_tile(const T *__restrict__ Nk, const T *__re
     // arithmetic costs 20 more registers and runs slower.
     struct BinLoads {
         T fx[7], fy[7], fk[7], wnext, x0, n0, add;
+        float nrate;  // SOURCE_NET_LS: the network's rate at the cell
         SourceBin<T> sb;
     };
     const unsigned cell_off = bo;
@@ -590,6 +594,12 @@ __device__ __forceinline__ void phi_tile(const T *__restrict__ Nk, const T *__re
         b.add = p.src_add ? ldo<AS1>(p.src_add + plane * bin, bo) : T(0);
         if (SRC)
             b.sb = source_bin<T, AS1>(p.src, (int)bin);
+        if constexpr (SK == SOURCE_NET_LS) {
+            // the FP16 rate plane at this column: byte offset bo / 2
+            typedef const __attribute__((address_space(1))) char gchar;
+            b.nrate = (float)*(const __attribute__((address_space(1))) _Float16 *)(
+                (gchar *)(p.src.rate + p.src.rate_ld * bin) + (bo >> 1));
+        }
     };
     // phi window f(n - 3 .. n + 3) at this cell, periodic in phi (refraction)
     T w[7] = {};
@@ -667,6 +677,8 @@ __device__ __forceinline__ void phi_tile(const T *__restrict__ Nk, const T *__re
         T raw = T(0);
         if constexpr (SK == SOURCE_JP_LS)
             L += source_jp_ls(p.src, jcol, cur.sb, x0, raw);
+        else if constexpr (SK == SOURCE_NET_LS)
+            L += source_net_ls(p.src, jcol, cur.sb, cur.nrate, x0, raw);
         else if constexpr (SK == SOURCE_GENERIC)
             L += source_term_cb<T, AS1>(p.src, scol, cur.sb, off, m + nk * n, x0, &raw);
         if (p.src_add)
@@ -988,13 +1000,16 @@ hipError_t launch_stage_fast(const StageParams<float> &p, int transport, int sta
                         : launch_phi<float, RPL_TRANSPORT_INTRINSIC, SOURCE_NONE, true>(p, stage, Nk, N0, out, s);
     }
     if (p.src.on) {
-        const bool jp = source_is_jp_ls(p.src);  // the specialised bundle path (same bits as the generic one)
+        // the specialised bundle paths (same bits as the generic one)
+        const bool jp = source_is_jp_ls(p.src), net = source_is_net_ls(p.src);
         if (transport == RPL_TRANSPORT_REFRACTION)
-            return jp ? launch_phi<float, RPL_TRANSPORT_REFRACTION, SOURCE_JP_LS>(p, stage, Nk, N0, out, s)
-                      : launch_phi<float, RPL_TRANSPORT_REFRACTION, SOURCE_GENERIC>(p, stage, Nk, N0, out, s);
+            return jp    ? launch_phi<float, RPL_TRANSPORT_REFRACTION, SOURCE_JP_LS>(p, stage, Nk, N0, out, s)
+                   : net ? launch_phi<float, RPL_TRANSPORT_REFRACTION, SOURCE_NET_LS>(p, stage, Nk, N0, out, s)
+                         : launch_phi<float, RPL_TRANSPORT_REFRACTION, SOURCE_GENERIC>(p, stage, Nk, N0, out, s);
         if (transport == RPL_TRANSPORT_INTRINSIC)
-            return jp ? launch_phi<float, RPL_TRANSPORT_INTRINSIC, SOURCE_JP_LS>(p, stage, Nk, N0, out, s)
-                      : launch_phi<float, RPL_TRANSPORT_INTRINSIC, SOURCE_GENERIC>(p, stage, Nk, N0, out, s);
+            return jp    ? launch_phi<float, RPL_TRANSPORT_INTRINSIC, SOURCE_JP_LS>(p, stage, Nk, N0, out, s)
+                   : net ? launch_phi<float, RPL_TRANSPORT_INTRINSIC, SOURCE_NET_LS>(p, stage, Nk, N0, out, s)
+                         : launch_phi<float, RPL_TRANSPORT_INTRINSIC, SOURCE_GENERIC>(p, stage, Nk, N0, out, s);
         return launch_t<RPL_TRANSPORT_NONE, true>(p, stage, Nk, N0, out, s);
     }
     if (transport == RPL_TRANSPORT_REFRACTION)
