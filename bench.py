@@ -285,11 +285,31 @@ def dia_line(steps, warmup):
     plain = S.PrecomputedSources(wind_input=src.wind_input, dissipation=src.dissipation)
     m2 = build_model(wl, np.float32, sources=plain)
     t_plain, _, _ = time_model(m2, wl.dt, steps, warmup)
+    # What a network of the hurricane network's width costs per stage on this
+    # grid (1443 -> 512 x 3 -> 1440, untrained weights: cost only), against
+    # the DIA launch.  The network's rate form r * sigma * N cannot carry the
+    # DIA's transfer into empty bins, so it stands in for the bundle's wind
+    # input and dissipation only; the timing is what replacing the DIA by an
+    # inference of this size would cost.
+    from ripple_amd import surrogate as sg
+    nb = wl.N0.shape[2] * wl.N0.shape[3]
+    net = sg.SourceNetwork(nb + 3, nb, hidden=512, hidden_layers=3, batch_capacity=64, seed=1)
+    sg.attach_surrogate(m2, net, sg.SurrogateConfig(terms=0))
+    ra.time_step_(m2, wl.dt, 1)
+    m2.synchronize()
+    phase_ms(m2, "rpl_get_surrogate_timing")
+    t_net, _, _ = time_model(m2, wl.dt, steps, 0)
+    t_inf, _ = phase_ms(m2, "rpl_get_surrogate_timing")
+    sg.attach_surrogate(m2, None, None)
+    net.close()
     cells = wl.cell_bins
     return {"workload": wl.name, "value": cells / t_step, "ms_per_step": t_step * 1e3, "stage_kernel_ms": l_stage,
             "dia_source_ms": l_dia, "dia_launches_timed": n.value,
             "without_dia_ms_per_step": t_plain * 1e3,
-            "dia_donor_pairs_per_s": 2 * cells / (l_dia * 1e-3) if l_dia > 0 else None}
+            "dia_donor_pairs_per_s": 2 * cells / (l_dia * 1e-3) if l_dia > 0 else None,
+            "network_inference_ms_per_stage": t_inf,
+            "without_dia_with_network_ms_per_step": t_net * 1e3,
+            "network": f"{nb + 3} -> 512 x 3 -> {nb}, untrained (cost only)"}
 
 
 def ensemble_line(count, steps, with_surrogate=True, **member_kw):
