@@ -112,6 +112,7 @@ struct rpl_ctx {
     // wave-supported stress sums of a stage's output, the per-bin weights,
     // and the buffer those sums belong to (null: none pending)
     void *tau_x = nullptr, *tau_y = nullptr, *tau_c = nullptr, *tau_s = nullptr;
+    void *bin_tab = nullptr;  // FP32: the per-bin tables packed for the specialised source paths
     const void *tau_for = nullptr;
     void *k_cell = nullptr, *theta_cell = nullptr, *sigma_cell = nullptr, *C_cell = nullptr;  // per bin
     void *t_phi = nullptr, *t_w = nullptr, *t_sigma_k = nullptr, *t_cos = nullptr, *t_sin = nullptr;
@@ -213,6 +214,8 @@ rpl::StageParams<T> stage_params(const rpl_ctx *c, double dt)
     p.cgx = (const T *)c->cgx;
     p.cgy = (const T *)c->cgy;
     p.cosp = (const T *)c->cosp;
+    if constexpr (std::is_same<T, float>::value)
+        p.bt = (const float *)c->bin_tab;
     p.sinp = (const T *)c->sinp;
     if (c->has_depth_field) {
         const size_t tab = (size_t)c->cfg.nkappa * c->cfg.nphi;
@@ -1253,6 +1256,8 @@ int rpl_create(rpl_ctx **out, const rpl_config *cfg)
         (rc = alloc(&c->cgy, tab * c->esize)) || (rc = alloc(&c->cosp, k.nphi * c->esize)) ||
         (rc = alloc(&c->sinp, k.nphi * c->esize)))
         goto error;
+    if (!is_f64(c) && (rc = alloc(&c->bin_tab, tab * rpl::BT_FIELDS * sizeof(float))))
+        goto error;
     if ((rc = alloc((void **)&c->w, tab * 8)) || (rc = alloc((void **)&c->trace, tab * 8)) ||
         (rc = alloc((void **)&c->mux, tab * 8)) || (rc = alloc((void **)&c->muy, tab * 8)) ||
         (rc = alloc((void **)&c->energy, tab * 8)))
@@ -1290,7 +1295,7 @@ int rpl_destroy(rpl_ctx *c)
                     (void *)c->yn, c->U10, c->theta_u, c->ustar, c->factor, c->cm0, c->kbar, c->sbar, c->k_cell,
                     c->theta_cell, c->sigma_cell, c->C_cell, c->t_phi, c->t_w, c->t_sigma_k, c->t_cos, c->t_sin,
                     c->t_kpow, c->t_spow, c->t_wc, c->f_cth, c->f_sth, c->f_logk, c->f_invc, c->f_sk3, c->f_sq,
-                    c->tau_x, c->tau_y, c->tau_c, c->tau_s,
+                    c->tau_x, c->tau_y, c->tau_c, c->tau_s, c->bin_tab,
                     c->f_logkap, c->f_invck, c->f_ctu, c->f_stu, c->f_lz1, (void *)c->sX, (void *)c->sA, (void *)c->sB, (void *)c->rate,
                     (void *)c->out_scale, (void *)c->s_err, (void *)c->use_plain, (void *)c->s_counts,
                     (void *)c->bin_hi, (void *)c->s_amax, (void *)c->s_flags, c->dUx, c->dUy, c->pm_O,

@@ -41,7 +41,16 @@ struct StageParams {
     // tau_c[t] (and tau_s) into tau_x / tau_y, (nx, ny, nk) planes.
     T *tau_x, *tau_y;
     const T *tau_c, *tau_s;           // (k / sigma) w rho_w g (cos theta, sin theta) per bin
+    // FP32 per-model phi-walk kernels without the generic source path: the
+    // per-bin values above packed 16 floats per bin (BT_* below), filled
+    // from the separate tables before every stage launch, so that a bin's
+    // values are one scalar load and one base address (null: not used).
+    const float *bt;
 };
+
+// Fields of a packed bin-table entry.
+enum { BT_CGX = 0, BT_CGY, BT_K, BT_CTH, BT_STH, BT_LOGK, BT_INVC, BT_SIGMA, BT_SK3, BT_SQ, BT_TAUC, BT_TAUS,
+       BT_OUTSCALE, BT_COSP, BT_SINP, BT_FIELDS = 16 };
 
 // One ensemble member's stage launch in the work-list kernel: its stage
 // parameters, buffers and work-list numbering (64 x TY tiles, nby =

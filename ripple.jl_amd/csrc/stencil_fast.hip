@@ -527,7 +527,12 @@ __device__ __forceinline__ void phi_tile(const T *__restrict__ Nk, const T *__re
         T fx[7], fy[7], fk[7], wnext, x0, n0, add;
         float nrate;  // SOURCE_NET_LS: the network's rate at the cell
         SourceBin<T> sb;
+        T bux, buy, btc, bts, bcs, bsn;  // packed table: group velocity, stress weights, cos and sin of phi
     };
+    // the FP32 per-model specialised source paths read one packed entry per bin
+    constexpr bool PACKED = std::is_same<T, float>::value && !AS1 && !CGF &&
+                            (SK == SOURCE_JP_LS || SK == SOURCE_NET_LS || SK == SOURCE_NONE);
+    typedef float f32x16 __attribute__((ext_vector_type(16)));
     const unsigned cell_off = bo;
     auto load_bin = [&](int n, BinLoads &b, auto interior) {
         // Every load below is a scalar slab base plus this 32-bit lane offset.
@@ -592,8 +597,26 @@ __device__ __forceinline__ void phi_tile(const T *__restrict__ Nk, const T *__re
         const size_t bin = (size_t)m + (size_t)nk * n;
         b.n0 = (STAGE == STAGE_2 || STAGE == STAGE_3) ? ldo<AS1>(N0 + plane * bin, bo) : T(0);
         b.add = p.src_add ? ldo<AS1>(p.src_add + plane * bin, bo) : T(0);
-        if (SRC)
+        if constexpr (PACKED) {
+            const f32x16 e = sld(reinterpret_cast<const f32x16 *>(p.bt), bin);
+            b.bux = e[BT_CGX];
+            b.buy = e[BT_CGY];
+            b.sb.k = e[BT_K];
+            b.sb.cth = e[BT_CTH];
+            b.sb.sth = e[BT_STH];
+            b.sb.logk = e[BT_LOGK];
+            b.sb.invc = e[BT_INVC];
+            b.sb.sigma = e[BT_SIGMA];
+            b.sb.sk3 = e[BT_SK3];
+            b.sb.sq = e[BT_SQ];
+            b.sb.out_scale = e[BT_OUTSCALE];
+            b.btc = e[BT_TAUC];
+            b.bts = e[BT_TAUS];
+            b.bcs = e[BT_COSP];
+            b.bsn = e[BT_SINP];
+        } else if (SRC) {
             b.sb = source_bin<T, AS1>(p.src, (int)bin);
+        }
         if constexpr (SK == SOURCE_NET_LS) {
             // the FP16 rate plane at this column: byte offset bo / 2
             typedef const __attribute__((address_space(1))) char gchar;
@@ -624,6 +647,9 @@ __device__ __forceinline__ void phi_tile(const T *__restrict__ Nk, const T *__re
             // the speed is nonnegative, so the upwind side is still the bin's
             ux = cgm * sld(p.cdx, (size_t)(m + nk * n));
             uy = cgm * sld(p.cdy, (size_t)(m + nk * n));
+        } else if constexpr (PACKED) {
+            ux = cur.bux;
+            uy = cur.buy;
         } else if constexpr (REF) {
             ux = gld<AS1>(p.cgx, m + nk * n);
             uy = gld<AS1>(p.cgy, m + nk * n);
@@ -633,7 +659,14 @@ __device__ __forceinline__ void phi_tile(const T *__restrict__ Nk, const T *__re
         }
         T ck = T(0), cp = T(0);
         if constexpr (REF) {
-            const T cs = gld<AS1>(p.cosp, n), sn = gld<AS1>(p.sinp, n);
+            T cs, sn;
+            if constexpr (PACKED) {
+                cs = cur.bcs;
+                sn = cur.bsn;
+            } else {
+                cs = gld<AS1>(p.cosp, n);
+                sn = gld<AS1>(p.sinp, n);
+            }
             const T cc = cs * cs, ss = sn * sn, csn = cs * sn;
             ck = -kap * (cc * Uxx + csn * (Uyx + Uxy) + ss * Uyy);
             cp = csn * (Uxx - Uyy) + ss * Uyx - cc * Uxy;
@@ -701,8 +734,13 @@ __device__ __forceinline__ void phi_tile(const T *__restrict__ Nk, const T *__re
             // (both source paths give raw the same bits)
             if (p.tau_x) {
                 const T mf = raw * r;
-                twx = fm(mf, sld(p.tau_c, bin), twx);
-                twy = fm(mf, sld(p.tau_s, bin), twy);
+                if constexpr (PACKED) {
+                    twx = fm(mf, cur.btc, twx);
+                    twy = fm(mf, cur.bts, twy);
+                } else {
+                    twx = fm(mf, sld(p.tau_c, bin), twx);
+                    twy = fm(mf, sld(p.tau_s, bin), twy);
+                }
             }
         }
         if constexpr (REF) {
@@ -758,6 +796,37 @@ void stage_phi_kernel(const T *__restrict__ Nk, const T *__restrict__ N0, T *out
     else
         t = tile_of_spatial(blockIdx.x, nbx * nby * p.nk, nbx, nby, p.nk);
     phi_tile<T, TRANSPORT, STAGE, SK, 4, false, CGF>(Nk, N0, out, p, t.ib, t.jb, t.m);
+}
+
+// The per-bin values of the FP32 specialised source paths packed 16 floats
+// per bin (BT_* fields; absent tables give 0).
+__global__ __launch_bounds__(256) void pack_bins_kernel(float *bt, int nbins, int nk, const float *cgx,
+                                                        const float *cgy, const float *cosp, const float *sinp,
+                                                        SourceParams<float> s, const float *tau_c,
+                                                        const float *tau_s)
+{
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= nbins)
+        return;
+    auto at = [&](const float *a) { return a ? a[t] : 0.0f; };
+    float *e = bt + (size_t)BT_FIELDS * t;
+    e[BT_CGX] = at(cgx);
+    e[BT_CGY] = at(cgy);
+    e[BT_K] = s.on ? at(s.k_cell) : 0.0f;
+    const bool wi = s.on && s.wind_input, ls = s.on && s.dissipation == 2;
+    e[BT_CTH] = wi ? at(s.cth_cell) : 0.0f;
+    e[BT_STH] = wi ? at(s.sth_cell) : 0.0f;
+    e[BT_LOGK] = wi ? at(s.logk_cell) : 0.0f;
+    e[BT_INVC] = wi ? at(s.invc_cell) : 0.0f;
+    e[BT_SIGMA] = wi ? at(s.sigma_cell) : 0.0f;
+    e[BT_SK3] = ls ? at(s.sk3_cell) : 0.0f;
+    e[BT_SQ] = ls ? at(s.sq_cell) : 0.0f;
+    e[BT_TAUC] = at(tau_c);
+    e[BT_TAUS] = at(tau_s);
+    e[BT_OUTSCALE] = s.on && s.surrogate ? at(s.out_scale) : 0.0f;
+    e[BT_COSP] = cosp ? cosp[t / nk] : 0.0f;
+    e[BT_SINP] = sinp ? sinp[t / nk] : 0.0f;
+    e[BT_FIELDS - 1] = 0.0f;
 }
 
 template <typename T, int TRANSPORT, int SK, bool CGF = false>
@@ -988,6 +1057,19 @@ hipError_t launch_ensemble_stage(const EnsembleMember *mem, int members, long lo
     return hipErrorInvalidValue;
 }
 
+// Fills the packed bin table of a phi-walk launch (refraction or intrinsic
+// transport; the kernels that read it are the per-model ones without the
+// generic source path).
+static void pack_bins(const StageParams<float> &p, int transport, hipStream_t s)
+{
+    if (!p.bt || transport == RPL_TRANSPORT_NONE)
+        return;
+    const int nbins = p.nk * p.nphi;
+    hipLaunchKernelGGL(pack_bins_kernel, dim3((unsigned)((nbins + 255) / 256)), dim3(256), 0, s,
+                       const_cast<float *>(p.bt), nbins, p.nk, p.cgx, p.cgy, p.cosp, p.sinp, p.src, p.tau_c,
+                       p.tau_s);
+}
+
 hipError_t launch_stage_fast(const StageParams<float> &p, int transport, int stage, const float *Nk, const float *N0,
                              float *out, hipStream_t s)
 {
@@ -1002,6 +1084,8 @@ hipError_t launch_stage_fast(const StageParams<float> &p, int transport, int sta
     if (p.src.on) {
         // the specialised bundle paths (same bits as the generic one)
         const bool jp = source_is_jp_ls(p.src), net = source_is_net_ls(p.src);
+        if (jp || net)
+            pack_bins(p, transport, s);
         if (transport == RPL_TRANSPORT_REFRACTION)
             return jp    ? launch_phi<float, RPL_TRANSPORT_REFRACTION, SOURCE_JP_LS>(p, stage, Nk, N0, out, s)
                    : net ? launch_phi<float, RPL_TRANSPORT_REFRACTION, SOURCE_NET_LS>(p, stage, Nk, N0, out, s)
@@ -1012,6 +1096,7 @@ hipError_t launch_stage_fast(const StageParams<float> &p, int transport, int sta
                          : launch_phi<float, RPL_TRANSPORT_INTRINSIC, SOURCE_GENERIC>(p, stage, Nk, N0, out, s);
         return launch_t<RPL_TRANSPORT_NONE, true>(p, stage, Nk, N0, out, s);
     }
+    pack_bins(p, transport, s);
     if (transport == RPL_TRANSPORT_REFRACTION)
         return launch_phi<float, RPL_TRANSPORT_REFRACTION, SOURCE_NONE>(p, stage, Nk, N0, out, s);
     if (transport == RPL_TRANSPORT_INTRINSIC)
