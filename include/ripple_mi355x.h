@@ -474,7 +474,8 @@ int rpl_ensemble_get_launch(rpl_ensemble *ens, int32_t *tile_rows, int32_t *bloc
 int rpl_ensemble_set_split(rpl_ensemble *ens, int64_t min_tiles);
 int rpl_ensemble_get_split(rpl_ensemble *ens, int64_t *min_tiles, int32_t *own_members);
 /* Times stage 1 (which leaves the members' state unchanged) for the splits
- * {none, 16384, 4096, 1024} and, for the work list, tile_rows in {2, 4, 8}
+ * {none, 16384, 4096, 1024, 1 (every member on its own chain)} and, for the
+ * work list, tile_rows in {2, 4, 8}
  * and grids of {2, 4, 8, 16} workgroups per CU or one workgroup per tile,
  * reps launches each; keeps the fastest and returns its shape (blocks 0: one
  * workgroup per tile; the split through rpl_ensemble_get_split). */

@@ -3335,7 +3335,8 @@ int rpl_ensemble_autotune(rpl_ensemble *e, double dt, int32_t reps, int32_t *til
     double best = 1e300;
     int best_ty = e->ty, best_blocks = e->blocks;
     long long best_split = e->split;
-    const long long splits[4] = {-1, 16384, ENSEMBLE_OWN_TILES, 1024};  // -1: every member in the work list
+    // -1: every member in the work list; 1: every member on its own chain
+    const long long splits[5] = {-1, 16384, ENSEMBLE_OWN_TILES, 1024, 1};
     const int tys[3] = {2, 4, 8};
     const int mult[5] = {2, 4, 8, 16, 0};  // workgroups per CU; 0: one workgroup per tile
     rc = RPL_OK;
