@@ -380,8 +380,9 @@ int rpl_mlp_train_device(rpl_mlp *net, int64_t rows, const float *X, const float
 /* Surrogate use in the time step: on every stage the network maps each
  * column's features to per-bin rates r (source = r * rate_unit * sigma_bin * N).
  * Columns outside the training envelope (U10 outside [wind_lo, wind_hi], an
- * action above action_hi or above the bin's bin_action_hi) or with a
- * non-finite output use the plain bundle,
+ * action above action_hi or above the bin's bin_action_hi) or with an
+ * output that is not finite in FP16 (the stage reads the rates in FP16) use
+ * the plain bundle,
  * which must be installed with rpl_set_sources (with a wind input).  Every
  * audit_stride-th column (0: none) is audited: its network source is compared
  * with the plain bundle's, and where the relative L2 error exceeds
