@@ -317,15 +317,23 @@ def test_fallbacks_reproduce_the_plain_bundle(float_type):
     ref = _model(wl, float_type)
     ra.time_step_(ref, wl.dt, 3)
     net = sg.SourceNetwork(nb + 3, nb, hidden=64, hidden_layers=2, batch_capacity=256, seed=7)
+    # a network whose every rate exceeds the FP16 range (the stage reads the
+    # rates in FP16): each column is flagged non-finite and takes the plain bundle
+    big = sg.SourceNetwork(nb + 3, nb, hidden=64, hidden_layers=2, batch_capacity=256, seed=7)
+    flat = big.params()
+    out_pad = big.layers(flat)[-1][1].size
+    flat[-out_pad:] = 1.0e5
+    big.set_params(flat)
     for name, cfg in (("envelope", sg.SurrogateConfig(wind_lo=1e6, wind_hi=2e6)),
-                      ("audit", sg.SurrogateConfig(audit_stride=1, audit_bound=0.0))):
+                      ("audit", sg.SurrogateConfig(audit_stride=1, audit_bound=0.0)),
+                      ("fp16 overflow", sg.SurrogateConfig())):
         m = _model(wl, float_type)
-        sg.attach_surrogate(m, net, cfg)
+        sg.attach_surrogate(m, big if name == "fp16 overflow" else net, cfg)
         ra.time_step_(m, wl.dt, 3)
         st = sg.surrogate_stats(m)
         say(f"{name}: {st}")
         assert st["columns"] == 9 * plane
-        if name == "envelope":
+        if name in ("envelope", "fp16 overflow"):
             assert st["plain"] == 9 * plane and st["audited"] == 0
         else:
             assert st["audited"] == 9 * plane and st["plain"] == 0
@@ -336,6 +344,7 @@ def test_fallbacks_reproduce_the_plain_bundle(float_type):
             assert rel_l2(m.action, ref.action) <= 1e-6, rel_l2(m.action, ref.action)
         sg.attach_surrogate(m, None, None)
     net.close()
+    big.close()
 
 
 def test_stage_source_is_the_network_and_targets_are_the_bundle():
