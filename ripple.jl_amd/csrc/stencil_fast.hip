@@ -651,8 +651,13 @@ __device__ __forceinline__ void phi_tile(const T *__restrict__ Nk, const T *__re
             ux = cur.bux;
             uy = cur.buy;
         } else if constexpr (REF) {
-            ux = gld<AS1>(p.cgx, m + nk * n);
-            uy = gld<AS1>(p.cgy, m + nk * n);
+            if constexpr (!AS1) {
+                ux = sld(p.cgx, (size_t)(m + nk * n));
+                uy = sld(p.cgy, (size_t)(m + nk * n));
+            } else {
+                ux = gld<AS1>(p.cgx, m + nk * n);
+                uy = gld<AS1>(p.cgy, m + nk * n);
+            }
         } else {
             ux = sld(p.cgx, (size_t)(m + nk * n));  // the bin's group velocity: wave-uniform
             uy = sld(p.cgy, (size_t)(m + nk * n));
@@ -664,8 +669,13 @@ __device__ __forceinline__ void phi_tile(const T *__restrict__ Nk, const T *__re
                 cs = cur.bcs;
                 sn = cur.bsn;
             } else {
-                cs = gld<AS1>(p.cosp, n);
-                sn = gld<AS1>(p.sinp, n);
+                if constexpr (!AS1) {
+                    cs = sld(p.cosp, (size_t)n);
+                    sn = sld(p.sinp, (size_t)n);
+                } else {
+                    cs = gld<AS1>(p.cosp, n);
+                    sn = gld<AS1>(p.sinp, n);
+                }
             }
             const T cc = cs * cs, ss = sn * sn, csn = cs * sn;
             ck = -kap * (cc * Uxx + csn * (Uyx + Uxy) + ss * Uyy);
