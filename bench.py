@@ -15,6 +15,7 @@ the other configurations' lines, the FP32 error against the FP64 path, and
 the CPU oracle timed on a bounded sample of the same workload on this host.
 """
 import argparse
+import dataclasses
 import json
 import math
 import os
@@ -138,6 +139,15 @@ def storm_plain_vs_surrogate(steps, warmup, nx=1024, ny=1024):
     t_surr, l_surr, _ = time_model(m, wl.dt, steps, 0)
     t_inf, _ = phase_ms(m, "rpl_get_surrogate_timing")
     st = sg.surrogate_stats(m)
+    # the same network with its rates inferred once per step and held
+    # through the step's stages (an approximation; its accuracy is in
+    # storm_surrogate_accuracy)
+    sg.attach_surrogate(m, net, dataclasses.replace(cfg, refresh=1), bin_action_hi=hi)
+    ra.time_step_(m, wl.dt, 1)
+    m.synchronize()
+    phase_ms(m, "rpl_get_surrogate_timing")
+    t_once, _, _ = time_model(m, wl.dt, steps, 0)
+    t_inf_once, _ = phase_ms(m, "rpl_get_surrogate_timing")
     flops = MLP_FLOP_PER_COLUMN * nx * ny
     cells = nx * ny * nk * nphi
     sg.attach_surrogate(m, None, None)
@@ -153,6 +163,9 @@ def storm_plain_vs_surrogate(steps, warmup, nx=1024, ny=1024):
                           "mlp_frac_of_bf16_dense_peak": flops / (t_inf * 1e-3) / 2.5e15,
                           "fallback_columns": st["plain"], "columns": st["columns"],
                           "network_terms": "every source term" if cfg.terms == 0 else "wind input (plain dissipation)"},
+            "surrogate_rates_once_per_step": {"value": cells / t_once, "ms_per_step": t_once * 1e3,
+                                              "inference_ms_per_step": t_inf_once,
+                                              "step_speedup_over_plain": t_plain / t_once},
             "step_speedup_surrogate_over_plain": t_plain / t_surr}
 
 
@@ -172,11 +185,11 @@ def storm_surrogate_accuracy(nx=1024, ny=1024, nsteps=None, network_file=None):
     net, cfg, hi, meta = sg.load_network(network_file or storm_surrogate.NETWORK_FILE, batch_capacity=64)
     out, fields = {}, {}
 
-    def run(label, float_type, surrogate=False, sources=True):
+    def run(label, float_type, surrogate=False, sources=True, refresh=0):
         kw = {} if sources else {"sources": None}
         m = build_model(wl, float_type, **kw)
         if surrogate:
-            sg.attach_surrogate(m, net, cfg, bin_action_hi=hi)
+            sg.attach_surrogate(m, net, dataclasses.replace(cfg, refresh=refresh), bin_action_hi=hi)
         t0 = time.perf_counter()
         ra.time_step_(m, wl.dt, nsteps)
         m.synchronize()
@@ -192,10 +205,11 @@ def storm_surrogate_accuracy(nx=1024, ny=1024, nsteps=None, network_file=None):
 
     run("plain_f32", np.float32)
     run("surrogate_f32", np.float32, surrogate=True)
+    run("surrogate_f32_rates_once_per_step", np.float32, surrogate=True, refresh=1)
     run("plain_f64", np.float64)
     run("no_sources_f32", np.float32, sources=False)
     net.close()
-    for label in ("surrogate_f32", "plain_f32", "no_sources_f32"):
+    for label in ("surrogate_f32", "surrogate_f32_rates_once_per_step", "plain_f32", "no_sources_f32"):
         hs, m0 = fields[label]
         for ref in ("plain_f32", "plain_f64"):
             if ref == label:

@@ -398,6 +398,12 @@ typedef struct rpl_surrogate {
     /* 0: the network supplies the whole source; 1: the wind input only (with
      * its stress cap), the dissipation staying the plain bundle's. */
     int32_t terms;
+    /* 0: the rates are inferred for every RK3 stage input (default); 1: once
+     * per step, for the step's first stage, and held through its later
+     * stages (the source stays rate * N of each stage input; columns on the
+     * plain bundle are still evaluated per stage).  An approximation that
+     * trades accuracy for a third of the inference. */
+    int32_t refresh;
 } rpl_surrogate;
 /* bin_action_hi (nkappa*nphi values, may be NULL): the per-bin largest
  * action of the training set; a column with any bin above it is outside the

@@ -130,6 +130,7 @@ struct rpl_ctx {
     rpl_mlp *net = nullptr;
     rpl_surrogate scfg{};
     bool s_active = false;  // the stage takes the network's source where allowed
+    bool s_due = true;      // refresh 1: the next source preparation infers (set by each step's wind)
     rpl::mlp::bf16 *sX = nullptr, *sA = nullptr, *sB = nullptr;
     _Float16 *rate = nullptr;  // the network's rates, [out_pad][s_ld], FP16
     float *out_scale = nullptr, *s_err = nullptr, *bin_hi = nullptr, *s_amax = nullptr;
@@ -324,6 +325,7 @@ rpl::StateTables<T> state_tables(const rpl_ctx *c)
 // Wind of the step at the clock time (constant over the three stages).
 int launch_wind(rpl_ctx *c)
 {
+    c->s_due = true;  // a new step (or a tendency evaluation) begins
     if (!c->has_sources || !c->src.wind_input)
         return RPL_OK;
     hipError_t e;
@@ -539,11 +541,16 @@ int launch_sources(rpl_ctx *c, const void *Nk)
         return rc0;
     if (!c->s_active)
         return launch_state(c, Nk);
-    int rc = record_begin(c, 1);
-    if (rc == RPL_OK)
-        rc = is_f64(c) ? surrogate_rates<double>(c, (const double *)Nk) : surrogate_rates<float>(c, (const float *)Nk);
-    if (rc == RPL_OK)
-        rc = record_end(c);
+    int rc = RPL_OK;
+    if (c->scfg.refresh == 0 || c->s_due) {
+        rc = record_begin(c, 1);
+        if (rc == RPL_OK)
+            rc = is_f64(c) ? surrogate_rates<double>(c, (const double *)Nk)
+                           : surrogate_rates<float>(c, (const float *)Nk);
+        if (rc == RPL_OK)
+            rc = record_end(c);
+        c->s_due = false;
+    }
     if (rc || (rc = launch_state(c, Nk)))
         return rc;
     if (c->scfg.audit_stride > 0)
@@ -1065,8 +1072,8 @@ int ensemble_list_stages(rpl_ensemble *e, int first, int last)
     for (int st = first; st <= last; ++st) {
         if (e->src) {
             for (rpl_ctx *c : e->list) {
-                if (!c->s_active)
-                    continue;
+                if (!c->s_active || (c->scfg.refresh == 1 && st != rpl::STAGE_1))
+                    continue;  // refresh 1: the first stage's rates serve the step
                 const void *Nk, *N0;
                 void *out;
                 stage_buffers(c, st, &Nk, &N0, &out);
@@ -2987,6 +2994,8 @@ int rpl_attach_surrogate(rpl_ctx *c, rpl_mlp *net, const rpl_surrogate *cfg, con
         return fail(RPL_ERR_ARG, "need rate_unit > 0, floor > 0, audit_stride >= 0");
     if (cfg->terms != 0 && cfg->terms != 1)
         return fail(RPL_ERR_ARG, "terms must be 0 (all sources) or 1 (wind input)");
+    if (cfg->refresh != 0 && cfg->refresh != 1)
+        return fail(RPL_ERR_ARG, "refresh must be 0 (every stage) or 1 (once per step)");
     free_surrogate(c);
     const int in_pad = net->dims[0], out_pad = net->dims.back(), hid = net->hidden;
     c->s_ld = (c->plane + 63) / 64 * 64;
@@ -3018,6 +3027,7 @@ int rpl_attach_surrogate(rpl_ctx *c, rpl_mlp *net, const rpl_surrogate *cfg, con
     c->net = net;
     c->scfg = *cfg;
     c->s_active = true;
+    c->s_due = true;
     c->s_columns = 0;
     return RPL_OK;
 }

@@ -67,7 +67,8 @@ class rpl_sources(ctypes.Structure):
 class rpl_surrogate(ctypes.Structure):
     _fields_ = [(n, ctypes.c_float) for n in ("floor", "shift", "scale", "wind_scale", "depth_feature", "rate_unit",
                                                "wind_lo", "wind_hi", "action_hi")] + \
-               [("audit_stride", ctypes.c_int32), ("audit_bound", ctypes.c_float), ("terms", ctypes.c_int32)]
+               [("audit_stride", ctypes.c_int32), ("audit_bound", ctypes.c_float), ("terms", ctypes.c_int32),
+                ("refresh", ctypes.c_int32)]
 
 
 _P = ctypes.c_void_p

@@ -119,6 +119,7 @@ class SurrogateConfig:
     audit_stride: int = 0         # every audit_stride-th column is audited (0: none)
     audit_bound: float = 0.1      # relative L2 source error above which an audit column falls back
     terms: int = 0                # 0: the network supplies every source term; 1: the wind input only
+    refresh: int = 0              # 0: rates inferred every RK3 stage; 1: once per step, held through its stages
 
     def to_struct(self):
         s = _lib.rpl_surrogate()

@@ -517,3 +517,36 @@ def test_surrogate_error_paths():
         sg.SourceNetwork(10, 10, hidden=100)
     net.close()
     bad.close()
+
+
+@pytest.mark.parametrize("float_type", [np.float32, np.float64])
+def test_rates_once_per_step(float_type):
+    """refresh = 1: the rates are inferred for each step's first stage only
+    (the column counter grows by one plane per step, not three) and held
+    through its later stages; the run stays close to per-stage rates but is
+    not the same; a tendency evaluation still infers on its own input, and
+    an invalid refresh is refused."""
+    say(f"surrogate rates once per step {np.dtype(float_type).name}")
+    wl = _storm(float_type)
+    nb = wl.spectral_grid.kappa.size * wl.spectral_grid.phi.size
+    plane = wl.grid.Nx * wl.grid.Ny
+    net = sg.SourceNetwork(nb + 3, nb, hidden=64, hidden_layers=2, batch_capacity=256, seed=11)
+    every, once = _model(wl, float_type), _model(wl, float_type)
+    sg.attach_surrogate(every, net, sg.SurrogateConfig(rate_unit=1e-3))
+    sg.attach_surrogate(once, net, sg.SurrogateConfig(rate_unit=1e-3, refresh=1))
+    ra.time_step_(every, wl.dt, 3)
+    ra.time_step_(once, wl.dt, 3)
+    se, so = sg.surrogate_stats(every), sg.surrogate_stats(once)
+    assert se["columns"] == 9 * plane and so["columns"] == 3 * plane
+    e = rel_l2(once.action, every.action)
+    say(f"once-per-step vs every-stage rates after 3 steps: rel-L2 {e:.2e}")
+    assert 0 < e < 1e-2
+    n_before = sg.surrogate_stats(once)["columns"]
+    once.tendency()
+    assert sg.surrogate_stats(once)["columns"] == n_before + plane
+    plain = _model(wl, float_type)
+    with pytest.raises(ra.RippleArgumentError):
+        sg.attach_surrogate(plain, net, sg.SurrogateConfig(refresh=2))
+    for m in (every, once):
+        sg.attach_surrogate(m, None, None)
+    net.close()
