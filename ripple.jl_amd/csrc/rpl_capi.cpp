@@ -3264,6 +3264,10 @@ int rpl_ensemble_step(rpl_ensemble *e, double dt, int32_t nsteps)
     if (e->use_graph && (e->exec == nullptr || e->graph_dt != dt || e->graph_ty != e->ty ||
                          e->graph_blocks != e->blocks || e->graph_split != e->split || e->graph_gen != gen)) {
         drop_graph(e);
+        // the captured stages stand for a whole step: members whose network
+        // infers once per step record that inference at their first stage
+        for (rpl_ctx *c : e->members)
+            c->s_due = true;
         hipGraph_t g = nullptr;
         RPL_HIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
         rc = ensemble_stages(e, dt);

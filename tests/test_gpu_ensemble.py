@@ -160,14 +160,14 @@ def test_ensemble_error_paths():
     ens.close()
 
 
-def _attach_networks(models, bins, seed=0, audit_stride=3):
+def _attach_networks(models, bins, seed=0, audit_stride=3, refresh=0):
     """Per-member 1155-style networks (bins + 3 -> 512 x 3 -> bins, GELU),
     random weights, with a wind envelope that sends some members to the plain
     bundle and an audit on every audit_stride-th column."""
     from ripple_amd import surrogate as sg
     nets = []
     cfg = sg.SurrogateConfig(rate_unit=1e-7, wind_lo=0.0, wind_hi=15.0, audit_stride=audit_stride,
-                             audit_bound=0.5)
+                             audit_bound=0.5, refresh=refresh)
     for k, (m, nb) in enumerate(zip(models, bins)):
         net = sg.SourceNetwork(nb + 3, nb, hidden=512, hidden_layers=3, batch_capacity=64, seed=seed + k)
         sg.attach_surrogate(m, net, cfg)
@@ -175,19 +175,21 @@ def _attach_networks(models, bins, seed=0, audit_stride=3):
     return nets
 
 
-def test_ensemble_with_surrogate_equals_members_stepped_alone():
+@pytest.mark.parametrize("refresh", [0, 1])
+def test_ensemble_with_surrogate_equals_members_stepped_alone(refresh):
     """Config 5's defining mode: every member takes its sources from its own
     512 x 3 network (plain bundle outside the wind envelope and on failed
     audits).  The ensemble equals the members stepped alone with the same
-    networks, bitwise, for two launch shapes, graph on and off."""
-    say("ensemble with surrogate vs single")
+    networks, bitwise, for two launch shapes, graph on and off, with the
+    rates inferred every stage or once per step."""
+    say(f"ensemble with surrogate vs single, refresh={refresh}")
     wls = members(5, True, seed_base=700)
     bins = [w.N0.shape[2] * w.N0.shape[3] for w in wls]
     dt, nsteps = 0.02, 3
     ref = []
     for k, w in enumerate(wls):
         m = build(w)
-        nets = _attach_networks([m], [bins[k]], seed=k)
+        nets = _attach_networks([m], [bins[k]], seed=k, refresh=refresh)
         ra.time_step_(m, dt, nsteps)
         ref.append(m.action)
         del nets
@@ -195,7 +197,7 @@ def test_ensemble_with_surrogate_equals_members_stepped_alone():
         ms = [build(w) for w in wls]
         nets = []
         for k, m in enumerate(ms):
-            nets += _attach_networks([m], [bins[k]], seed=k)
+            nets += _attach_networks([m], [bins[k]], seed=k, refresh=refresh)
         ens = ra.Ensemble(ms)
         ens.set_launch(ty, blocks)
         ens.set_split(split)
